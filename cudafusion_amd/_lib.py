@@ -1,0 +1,164 @@
+"""ctypes binding of ``libnrf.so`` (the C ABI declared in ``include/nrf.h``).
+
+The library is the only compute path: if it is missing, or if it cannot be
+loaded, every call raises.  ``torch`` is imported first so that the HIP
+runtime torch ships is the one the library binds to (both use the soname
+``libamdhip64.so.7``); device buffers handed to the library are torch
+tensors, passed by address.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnrf.so")
+
+# ------------------------------------------------------------ record dtypes
+# Byte layouts of the reference's POD structs (include/types.h:86-206).
+VOXEL = np.dtype([("tsdf", "<f4"), ("weight", "<f4"), ("color", "u1", 3), ("pad", "u1")])
+NODE = np.dtype([("pos", "<f4", 3), ("normal", "<f4", 3), ("radius", "<f4"), ("neighbors", "<i4", 8),
+                 ("neighbor_w", "<f4", 8), ("num_neighbors", "<i4")])
+DUALQUAT = np.dtype([("real", "<f4", 4), ("dual", "<f4", 4)])
+CORR = np.dtype([("src", "<f4", 3), ("dst", "<f4", 3), ("normal", "<f4", 3), ("node_ids", "<i4", 4),
+                 ("node_ws", "<f4", 4), ("weight", "<f4"), ("valid", "u1"), ("pad", "u1", 3)])
+assert VOXEL.itemsize == 12 and NODE.itemsize == 96 and DUALQUAT.itemsize == 32 and CORR.itemsize == 76
+
+
+class VolumeParams(C.Structure):
+    _fields_ = [("dims", C.c_int32 * 3), ("voxel_size", C.c_float), ("truncation", C.c_float),
+                ("origin", C.c_float * 3), ("decay_alpha", C.c_float), ("max_weight", C.c_float)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float),
+                ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("gn_iterations", C.c_int32), ("pcg_iterations", C.c_int32), ("pcg_tolerance", C.c_float),
+                ("lambda_smooth", C.c_float), ("lambda_rot", C.c_float), ("lambda_damping", C.c_float),
+                ("debug", C.c_int32), ("relative_tolerance", C.c_int32), ("precond_fp64", C.c_int32)]
+
+
+class GNParams(C.Structure):
+    _fields_ = [("gn_iterations", C.c_int32), ("update_scale", C.c_float), ("max_update_rot", C.c_float),
+                ("max_update_trans", C.c_float), ("validate", C.c_int32), ("max_dx_mean", C.c_float)]
+
+
+class PcgResult(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("total_iterations", C.c_int32), ("flags", C.c_int32),
+                ("gn_done", C.c_int32), ("final_residual", C.c_double), ("rhs_norm", C.c_double)]
+
+
+class PipelineConfig(C.Structure):
+    _fields_ = [("volume", VolumeParams), ("camera", Camera), ("solver", SolverParams), ("gn", GNParams),
+                ("max_nodes", C.c_int32), ("max_corrs", C.c_int32), ("max_triangles", C.c_int32),
+                ("min_opt_count", C.c_int32), ("use_graph", C.c_int32)]
+
+
+P = C.c_void_p
+I = C.c_int
+F = C.c_float
+
+# name: (restype, argtypes)
+_SIGS = {
+    "nrf_status_string": (C.c_char_p, [I]),
+    "nrf_last_error": (C.c_char_p, []),
+    "nrf_build_info": (C.c_char_p, []),
+    "nrf_tsdf_integrate": (I, [C.POINTER(VolumeParams), P, P, C.POINTER(Camera), P, P, P, I, P, P, P, P, I, P, P]),
+    "nrf_tsdf_reset": (I, [C.POINTER(VolumeParams), P, P]),
+    "nrf_mc_workspace_size": (C.c_size_t, [C.POINTER(VolumeParams)]),
+    "nrf_mc_extract": (I, [C.POINTER(VolumeParams), P, P, C.c_size_t, P, P, P, I, P, P]),
+    "nrf_mc_cases": (P, [P]),
+    "nrf_voxel_knn": (I, [C.POINTER(VolumeParams), P, P, I, P, P, P]),
+    "nrf_warp_apply_twist": (I, [P, P, I, F, F, F, P, P, P]),
+    "nrf_warp_centered_table": (I, [P, P, I, P, P]),
+    "nrf_warp_points": (I, [P, P, P, I, P, P, I, P, P]),
+    "nrf_corr_pack": (I, [C.POINTER(VolumeParams), P, P, P, P, P, P, P, I, P, P]),
+    "nrf_solver_create": (P, [C.POINTER(SolverParams), I]),
+    "nrf_solver_destroy": (None, [P]),
+    "nrf_solver_set_params": (I, [P, C.POINTER(SolverParams)]),
+    "nrf_solver_set_pattern": (I, [P, P, I]),
+    "nrf_solver_solve": (I, [P, P, I, P, P, I, P, C.POINTER(PcgResult), P]),
+    "nrf_solver_solve_gn": (I, [P, P, P, I, P, P, P, I, C.POINTER(GNParams), P, C.POINTER(PcgResult), P]),
+    "nrf_solver_system": (I, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(I), C.POINTER(P),
+                              C.POINTER(P)]),
+    "nrf_solver_device_result": (P, [P]),
+    "nrf_pipeline_create": (P, [C.POINTER(PipelineConfig)]),
+    "nrf_pipeline_destroy": (None, [P]),
+    "nrf_pipeline_voxels": (P, [P]),
+    "nrf_pipeline_transforms": (P, [P]),
+    "nrf_pipeline_nodes": (P, [P]),
+    "nrf_pipeline_knn": (P, [P]),
+    "nrf_pipeline_knn_w": (P, [P]),
+    "nrf_pipeline_mesh": (P, [P, C.POINTER(P)]),
+    "nrf_pipeline_solver": (P, [P]),
+    "nrf_pipeline_integrate_rigid": (I, [P, P, P, P]),
+    "nrf_pipeline_set_nodes": (I, [P, P, I]),
+    "nrf_pipeline_step": (I, [P, P, P, P, P, I, P]),
+    "nrf_pipeline_stage_ms": (I, [P, C.POINTER(F)]),
+}
+
+_lib = None
+
+
+class NrfError(RuntimeError):
+    pass
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load (once) and return the library; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise NrfError(f"{path} is missing: build it with __graft_entry__.build() (hipcc, gfx950)")
+    import torch  # noqa: F401  (binds the process to torch's HIP runtime first)
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    return sorted(_SIGS)
+
+
+def check(status: int, what: str = "") -> None:
+    if status != 0:
+        lib = load()
+        msg = lib.nrf_status_string(status).decode()
+        last = lib.nrf_last_error().decode()
+        raise NrfError(f"{what}: {msg}" + (f" ({last})" if last else ""))
+
+
+def volume_params(dims, voxel_size, truncation, origin, decay_alpha=1.0, max_weight=50.0) -> VolumeParams:
+    vp = VolumeParams()
+    vp.dims[:] = [int(d) for d in dims]
+    vp.voxel_size = voxel_size
+    vp.truncation = truncation
+    vp.origin[:] = [float(o) for o in origin]
+    vp.decay_alpha = decay_alpha
+    vp.max_weight = max_weight
+    return vp
+
+
+def camera(fx, fy, cx, cy, width, height) -> Camera:
+    return Camera(fx, fy, cx, cy, int(width), int(height))
+
+
+def solver_params(gn_iterations=3, pcg_iterations=10, pcg_tolerance=1e-4, lambda_smooth=1.0, lambda_rot=1.0,
+                  lambda_damping=1e-2, debug=0, relative_tolerance=0, precond_fp64=0) -> SolverParams:
+    return SolverParams(gn_iterations, pcg_iterations, pcg_tolerance, lambda_smooth, lambda_rot, lambda_damping,
+                        debug, relative_tolerance, precond_fp64)
+
+
+def gn_params(gn_iterations=3, update_scale=1.0, max_update_rot=0.12, max_update_trans=0.04, validate=1,
+              max_dx_mean=0.03) -> GNParams:
+    return GNParams(gn_iterations, update_scale, max_update_rot, max_update_trans, validate, max_dx_mean)

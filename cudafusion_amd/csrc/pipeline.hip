@@ -1,0 +1,209 @@
+// pipeline.hip - the per-frame non-rigid step as one stream-ordered call.
+//
+// Mirrors the caller contract of DynamicFusionPipeline::process_frame
+// (include/fusion_app.h:253-669) with the association held as an input:
+// solve (GN loop with validation and clamped twist updates) -> warped
+// integration -> marching cubes (the extract_surface_device form).  Nothing
+// synchronises the host inside a step, so a step can be captured into a
+// hipGraph and replayed.
+#include <cstring>
+#include <vector>
+
+#include "nrf_internal.h"
+
+using namespace nrf;
+
+struct nrf_pipeline {
+  nrf_pipeline_config cfg;
+  nrf_voxel *vox = nullptr;
+  int32_t *knn = nullptr;
+  float *knn_w = nullptr;
+  nrf_node *nodes = nullptr;
+  nrf_dualquat *T = nullptr;
+  nrf_dualquat *centered = nullptr;
+  int N = 0;
+  float *delta_x = nullptr;
+  void *mc_ws = nullptr;
+  size_t mc_bytes = 0;
+  float *verts = nullptr;
+  int32_t *tris = nullptr;
+  int32_t *counts = nullptr;
+  int32_t *updated = nullptr;
+  nrf_solver *solver = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // graph replay state
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  const void *g_key[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int g_n = -1;
+  float g_pose[16];
+  void *g_stream = nullptr;
+};
+
+static void pipeline_free(nrf_pipeline *p) {
+  void *ptrs[] = {p->vox, p->knn, p->knn_w, p->nodes, p->T, p->centered, p->delta_x, p->mc_ws,
+                  p->verts, p->tris, p->counts, p->updated};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  for (auto &e : p->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->solver) nrf_solver_destroy(p->solver);
+  delete p;
+}
+
+extern "C" nrf_pipeline *nrf_pipeline_create(const nrf_pipeline_config *cfg) {
+  if (!cfg || !volume_ok(&cfg->volume) || cfg->max_nodes <= 0 || cfg->max_corrs < 0 || cfg->max_triangles < 0)
+    return nullptr;
+  nrf_pipeline *p = new nrf_pipeline();
+  p->cfg = *cfg;
+  const size_t V = (size_t)volume_count(&cfg->volume);
+  bool ok = true;
+  ok = ok && hipMalloc(&p->vox, V * sizeof(nrf_voxel)) == hipSuccess;
+  ok = ok && hipMalloc(&p->knn, V * 4 * sizeof(int32_t)) == hipSuccess;
+  ok = ok && hipMalloc(&p->knn_w, V * 4 * sizeof(float)) == hipSuccess;
+  ok = ok && hipMalloc(&p->nodes, (size_t)cfg->max_nodes * sizeof(nrf_node)) == hipSuccess;
+  ok = ok && hipMalloc(&p->T, (size_t)cfg->max_nodes * sizeof(nrf_dualquat)) == hipSuccess;
+  ok = ok && hipMalloc(&p->centered, (size_t)cfg->max_nodes * sizeof(nrf_dualquat)) == hipSuccess;
+  ok = ok && hipMalloc(&p->delta_x, (size_t)cfg->max_nodes * 6 * sizeof(float)) == hipSuccess;
+  p->mc_bytes = nrf_mc_workspace_size(&cfg->volume);
+  ok = ok && hipMalloc(&p->mc_ws, p->mc_bytes) == hipSuccess;
+  const size_t T = (size_t)std::max(1, cfg->max_triangles);
+  ok = ok && hipMalloc(&p->verts, T * 9 * sizeof(float)) == hipSuccess;
+  ok = ok && hipMalloc(&p->tris, T * 3 * sizeof(int32_t)) == hipSuccess;
+  ok = ok && hipMalloc(&p->counts, 4 * sizeof(int32_t)) == hipSuccess;
+  ok = ok && hipMalloc(&p->updated, 4 * sizeof(int32_t)) == hipSuccess;
+  for (auto &e : p->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+  if (ok) {
+    p->solver = nrf_solver_create(&cfg->solver, cfg->max_nodes);
+    ok = p->solver != nullptr;
+  }
+  if (ok) ok = nrf_tsdf_reset(&cfg->volume, p->vox, nullptr) == NRF_OK;
+  if (ok) ok = hipMemset(p->knn, 0xff, V * 4 * sizeof(int32_t)) == hipSuccess;
+  if (ok) ok = hipMemset(p->knn_w, 0, V * 4 * sizeof(float)) == hipSuccess;
+  if (ok) ok = hipMemset(p->counts, 0, 4 * sizeof(int32_t)) == hipSuccess;
+  if (ok) ok = hipDeviceSynchronize() == hipSuccess;
+  if (!ok) {
+    pipeline_free(p);
+    return nullptr;
+  }
+  return p;
+}
+
+extern "C" void nrf_pipeline_destroy(nrf_pipeline *p) {
+  if (p) pipeline_free(p);
+}
+
+extern "C" nrf_voxel *nrf_pipeline_voxels(nrf_pipeline *p) { return p ? p->vox : nullptr; }
+extern "C" nrf_dualquat *nrf_pipeline_transforms(nrf_pipeline *p) { return p ? p->T : nullptr; }
+extern "C" nrf_node *nrf_pipeline_nodes(nrf_pipeline *p) { return p ? p->nodes : nullptr; }
+extern "C" int32_t *nrf_pipeline_knn(nrf_pipeline *p) { return p ? p->knn : nullptr; }
+extern "C" float *nrf_pipeline_knn_w(nrf_pipeline *p) { return p ? p->knn_w : nullptr; }
+extern "C" float *nrf_pipeline_mesh(nrf_pipeline *p, int32_t **counts) {
+  if (!p) return nullptr;
+  if (counts) *counts = p->counts;
+  return p->verts;
+}
+extern "C" nrf_solver *nrf_pipeline_solver(nrf_pipeline *p) { return p ? p->solver : nullptr; }
+
+extern "C" int nrf_pipeline_integrate_rigid(nrf_pipeline *p, const float *depth, const float camera_pose[16],
+                                            void *stream) {
+  if (!p || !depth || !camera_pose) return NRF_ERR_INVALID_ARGUMENT;
+  return nrf_tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, nullptr, nullptr, 0,
+                            nullptr, nullptr, nullptr, nullptr, 0, nullptr, stream);
+}
+
+extern "C" int nrf_pipeline_set_nodes(nrf_pipeline *p, const nrf_node *host_nodes, int num_nodes) {
+  if (!p || num_nodes < 0 || num_nodes > p->cfg.max_nodes || (num_nodes > 0 && !host_nodes))
+    return NRF_ERR_INVALID_ARGUMENT;
+  p->N = num_nodes;
+  if (num_nodes == 0) return NRF_OK;
+  NRF_HIP_CHECK(hipMemcpy(p->nodes, host_nodes, sizeof(nrf_node) * num_nodes, hipMemcpyHostToDevice));
+  std::vector<nrf_dualquat> id(num_nodes);
+  for (auto &q : id) {
+    q.real[0] = q.real[1] = q.real[2] = 0.f;
+    q.real[3] = 1.f;
+    q.dual[0] = q.dual[1] = q.dual[2] = q.dual[3] = 0.f;
+  }
+  NRF_HIP_CHECK(hipMemcpy(p->T, id.data(), sizeof(nrf_dualquat) * num_nodes, hipMemcpyHostToDevice));
+  int rc = nrf_warp_centered_table(p->nodes, p->T, num_nodes, p->centered, nullptr);
+  if (rc) return rc;
+  rc = nrf_solver_set_pattern(p->solver, host_nodes, num_nodes);
+  if (rc) return rc;
+  rc = nrf_voxel_knn(&p->cfg.volume, p->vox, p->nodes, num_nodes, p->knn, p->knn_w, nullptr);
+  if (rc) return rc;
+  NRF_HIP_CHECK(hipDeviceSynchronize());
+  if (p->exec) {
+    (void)hipGraphExecDestroy(p->exec);
+    p->exec = nullptr;
+  }
+  return NRF_OK;
+}
+
+static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float camera_pose[16],
+                            const nrf_corr *corrs, const float *canonical, int num_corrs, void *stream) {
+  hipStream_t st = as_stream(stream);
+  int rc;
+  NRF_HIP_CHECK(hipEventRecord(p->ev[0], st));
+  rc = nrf_solver_solve_gn(p->solver, corrs, canonical, num_corrs, p->nodes, p->T, p->centered, p->N, &p->cfg.gn,
+                           p->delta_x, nullptr, stream);
+  if (rc) return rc;
+  NRF_HIP_CHECK(hipEventRecord(p->ev[1], st));
+  rc = nrf_tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, p->nodes, p->T, p->N,
+                          p->centered, p->knn, p->knn_w, nullptr, p->cfg.min_opt_count, nullptr, stream);
+  if (rc) return rc;
+  NRF_HIP_CHECK(hipEventRecord(p->ev[2], st));
+  rc = nrf_mc_extract(&p->cfg.volume, p->vox, p->mc_ws, p->mc_bytes, p->verts, nullptr, p->tris,
+                      p->cfg.max_triangles, p->counts, stream);
+  if (rc) return rc;
+  NRF_HIP_CHECK(hipEventRecord(p->ev[3], st));
+  return NRF_OK;
+}
+
+extern "C" int nrf_pipeline_step(nrf_pipeline *p, const float *depth, const float camera_pose[16],
+                                 const nrf_corr *corrs, const float *canonical, int num_corrs, void *stream) {
+  if (!p || !depth || !camera_pose || num_corrs < 0 || num_corrs > p->cfg.max_corrs || (num_corrs > 0 && !corrs))
+    return NRF_ERR_INVALID_ARGUMENT;
+  if (p->N <= 0) return NRF_ERR_NOT_READY;
+  if (!p->cfg.use_graph) return pipeline_enqueue(p, depth, camera_pose, corrs, canonical, num_corrs, stream);
+  const void *key[5] = {depth, corrs, canonical, stream, nullptr};
+  bool same = p->exec && p->g_n == num_corrs && memcmp(p->g_pose, camera_pose, sizeof(p->g_pose)) == 0;
+  for (int k = 0; k < 4 && same; ++k) same = key[k] == p->g_key[k];
+  if (!same) {
+    if (p->exec) {
+      (void)hipGraphExecDestroy(p->exec);
+      p->exec = nullptr;
+    }
+    if (p->graph) {
+      (void)hipGraphDestroy(p->graph);
+      p->graph = nullptr;
+    }
+    hipStream_t st = as_stream(stream);
+    NRF_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = pipeline_enqueue(p, depth, camera_pose, corrs, canonical, num_corrs, stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) {
+      set_last_error("hipStreamEndCapture", e);
+      return NRF_ERR_HIP;
+    }
+    p->graph = g;
+    NRF_HIP_CHECK(hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
+    for (int k = 0; k < 4; ++k) p->g_key[k] = key[k];
+    p->g_n = num_corrs;
+    memcpy(p->g_pose, camera_pose, sizeof(p->g_pose));
+  }
+  NRF_HIP_CHECK(hipGraphLaunch(p->exec, as_stream(stream)));
+  return NRF_OK;
+}
+
+extern "C" int nrf_pipeline_stage_ms(nrf_pipeline *p, float out_ms[3]) {
+  if (!p || !out_ms) return NRF_ERR_INVALID_ARGUMENT;
+  for (int k = 0; k < 3; ++k) NRF_HIP_CHECK(hipEventElapsedTime(&out_ms[k], p->ev[k], p->ev[k + 1]));
+  return NRF_OK;
+}

@@ -1,0 +1,1124 @@
+// solver.hip - Gauss-Newton warp-field solve for gfx950.
+//
+// Reference: GaussNewtonSolver (include/solver.h, kernels/solver_kernels.cu).
+// The normal equations are the reference's: point-to-plane data term with
+// Huber weights (delta 0.2) over DQ-skinned correspondences, ARAP smoothness
+// over graph edges i->j with j > i (Huber 0.05, strain weight), Levenberg
+// damping of the diagonal, block-Jacobi PCG.  The implementation differs:
+//
+//  * Assembly is atomic-free and deterministic.  Per frame, the
+//    (correspondence, neighbour-slot) pairs are bucketed by node and each
+//    bucket sorted by correspondence index ("memberships").  One wavefront
+//    then owns one block row i: it stages 64 memberships at a time in LDS
+//    (residual, Huber weight, the four 6-vector Jacobian rows, the row
+//    positions of the four neighbours) and 36 lanes accumulate the 6x6
+//    entries of every block of the row in LDS (one lane per entry, so there
+//    are no conflicts and the summation order is fixed), six lanes the RHS.
+//    The ARAP edge contributions of the row follow, then damping, then the
+//    block-Jacobi Cholesky inverse of the diagonal block.  One kernel per GN
+//    iteration replaces the reference's data / smooth / damping /
+//    preconditioner kernels and their float atomics.
+//  * PCG is one persistent kernel: each workgroup owns a contiguous range of
+//    block rows and keeps their matrix blocks in LDS for all iterations;
+//    dot products are FP64, reduced per workgroup and combined in a fixed
+//    order after a grid barrier, so every workgroup holds bit-identical
+//    alpha / beta and takes the same branch.  The update p = z + beta p is
+//    recomputed on the fly by the readers of p (double-buffered), which
+//    leaves two grid barriers per iteration.  The barrier wait is bounded:
+//    on timeout the kernel raises a flag and every workgroup exits.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "nrf_internal.h"
+#include "nrf_math.h"
+
+namespace nrf {
+namespace {
+
+constexpr int kPcgThreads = 256;
+constexpr unsigned kBarrierSpinLimit = 1u << 22;
+
+// ---------------------------------------------------------------- helpers
+__global__ void fill_i32(int32_t *p, int n, int32_t v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Membership pass 1: count valid (corr, slot) pairs per node.  A slot is
+// valid under the reference's test (kernels/solver_kernels.cu:94-96).
+__global__ void mem_count_kernel(const nrf_corr *__restrict__ corrs, int n, int N, int32_t *__restrict__ cnt) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const nrf_corr &cr = corrs[c];
+  if (!cr.valid) return;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int id = cr.node_ids[k];
+    if (id >= 0 && id < N && cr.node_ws[k] > 1e-8f) atomicAdd(&cnt[id], 1);
+  }
+}
+
+__global__ __launch_bounds__(1024) void exclusive_scan_kernel(const int32_t *__restrict__ cnt,
+                                                              int32_t *__restrict__ off,
+                                                              int32_t *__restrict__ cur, int n) {
+  __shared__ int32_t part[1024];
+  const int t = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int b = t * per, e = min(n, b + per);
+  int s = 0;
+  for (int i = b; i < e; ++i) s += cnt[i];
+  part[t] = s;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    int v = (t >= d) ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+  for (int i = b; i < e; ++i) {
+    off[i] = run;
+    if (cur) cur[i] = run;
+    run += cnt[i];
+  }
+  if (t == 1023) off[n] = part[1023];
+}
+
+__global__ void mem_scatter_kernel(const nrf_corr *__restrict__ corrs, int n, int N, int32_t *__restrict__ cur,
+                                   int32_t *__restrict__ keys) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const nrf_corr &cr = corrs[c];
+  if (!cr.valid) return;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int id = cr.node_ids[k];
+    if (id >= 0 && id < N && cr.node_ws[k] > 1e-8f) {
+      int pos = atomicAdd(&cur[id], 1);
+      keys[pos] = c * 4 + k;
+    }
+  }
+}
+
+// Orders each node's bucket by key (= corr * 4 + slot): keys are unique, so
+// an element's rank is the count of smaller keys in its bucket.
+constexpr int kSortLds = 2048;
+__global__ __launch_bounds__(64) void mem_sort_kernel(const int32_t *__restrict__ off,
+                                                      const int32_t *__restrict__ keys_in,
+                                                      int32_t *__restrict__ keys_out) {
+  __shared__ int32_t sk[kSortLds];
+  const int i = blockIdx.x;
+  const int b = off[i], e = off[i + 1], n = e - b;
+  if (n <= 0) return;
+  const bool in_lds = n <= kSortLds;
+  if (in_lds) {
+    for (int j = threadIdx.x; j < n; j += 64) sk[j] = keys_in[b + j];
+    __syncthreads();
+  }
+  for (int j = threadIdx.x; j < n; j += 64) {
+    const int kj = in_lds ? sk[j] : keys_in[b + j];
+    int rank = 0;
+    for (int m = 0; m < n; ++m) rank += ((in_lds ? sk[m] : keys_in[b + m]) < kj);
+    keys_out[b + rank] = kj;
+  }
+}
+
+// ------------------------------------------------------------ assembly
+struct AsmArgs {
+  const nrf_corr *corrs;
+  const float *canonical;  // NULL: use stored src
+  const nrf_node *nodes;
+  const DQ *T;
+  const DQ *centered;
+  int N;
+  const int32_t *row_ptr, *col_idx, *diag_rel;
+  const int32_t *m_off, *m_keys;
+  const int32_t *es_off, *es_entry, *es_offrel;
+  const int32_t *e_i, *e_j;
+  const float *e_w;
+  float lambda_smooth, lambda_damping;
+  int precond_fp64;
+  int Rmax;
+  float *values, *b, *Minv;
+  const int32_t *stop;
+};
+
+// staging layout per entry (floats): fw, r, kslot, pos[4] (as float bits), J[4][6]
+constexpr int kStageData = 3 + 4 + 24;
+// staging per ARAP edge-side: lam, side, offrel, r[3], Js[6][3], Jo[6][3]
+constexpr int kStageArap = 6 + 18 + 18;
+constexpr int kStage = kStageArap > kStageData ? kStageArap : kStageData;
+
+__device__ inline void arap_jac(float3 q, float J[6][3]) {
+  // compute_arap_jacobian (kernels/solver_kernels.cu:140-165)
+  J[0][0] = 0.0f; J[0][1] = q.z; J[0][2] = -q.y;
+  J[1][0] = -q.z; J[1][1] = 0.0f; J[1][2] = q.x;
+  J[2][0] = q.y; J[2][1] = -q.x; J[2][2] = 0.0f;
+  J[3][0] = 1.0f; J[3][1] = 0.0f; J[3][2] = 0.0f;
+  J[4][0] = 0.0f; J[4][1] = 1.0f; J[4][2] = 0.0f;
+  J[5][0] = 0.0f; J[5][1] = 0.0f; J[5][2] = 1.0f;
+}
+
+template <typename R>
+__device__ inline bool chol_inv6(const R *A, R *Ainv_col, int col, bool &ok_out) {
+  // invert6x6_cholesky (include/se3_math.cuh:152-202), one column per call.
+  R L[6][6];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) L[i][j] = 0;
+  for (int i = 0; i < 6; ++i) {
+    for (int j = 0; j <= i; ++j) {
+      R s = A[i * 6 + j];
+      for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k];
+      if (i == j) {
+        if (s <= 0) {
+          ok_out = false;
+          return false;
+        }
+        L[i][j] = sqrt(s);
+      } else {
+        L[i][j] = s / L[j][j];
+      }
+    }
+  }
+  R y[6], x[6];
+  for (int i = 0; i < 6; ++i) {
+    R s = (i == col) ? (R)1 : (R)0;
+    for (int k = 0; k < i; ++k) s -= L[i][k] * y[k];
+    y[i] = s / L[i][i];
+  }
+  for (int i = 5; i >= 0; --i) {
+    R s = y[i];
+    for (int k = i + 1; k < 6; ++k) s -= L[k][i] * x[k];
+    x[i] = s / L[i][i];
+  }
+  for (int i = 0; i < 6; ++i) Ainv_col[i] = x[i];
+  ok_out = true;
+  return true;
+}
+
+__global__ __launch_bounds__(64) void assemble_rows_kernel(AsmArgs a) {
+  extern __shared__ float lds[];
+  const int i = blockIdx.x;
+  if (a.stop && *a.stop) return;
+  const int lane = threadIdx.x;
+  const int rb = a.row_ptr[i], rlen = a.row_ptr[i + 1] - rb;
+  float *acc = lds;                                      // [Rmax][36]
+  int32_t *cols = reinterpret_cast<int32_t *>(acc + a.Rmax * 36);  // [Rmax]
+  float *stage = reinterpret_cast<float *>(cols + a.Rmax);         // [64][kStage]
+  for (int j = lane; j < rlen * 36; j += 64) acc[j] = 0.f;
+  for (int j = lane; j < rlen; j += 64) cols[j] = a.col_idx[rb + j];
+  __syncthreads();
+  const int ab = lane;            // lanes 0..35: entry (a, b) = (ab / 6, ab % 6)
+  const int ea = ab / 6, eb = ab % 6;
+  const int ra = lane - 36;       // lanes 36..41: RHS row ra
+  float bacc = 0.f;
+  const int diag = a.diag_rel[i];
+
+  // ---- data term (kernels/solver_kernels.cu:52-138)
+  const int mb = a.m_off[i], me = a.m_off[i + 1];
+  for (int base = mb; base < me; base += 64) {
+    const int e = base + lane;
+    float *st = stage + lane * kStage;
+    float fw = 0.f;
+    if (e < me) {
+      const int key = a.m_keys[e];
+      const int c = key >> 2, kslot = key & 3;
+      const nrf_corr &cr = a.corrs[c];
+      const float3 dst = f3(cr.dst), nrm = f3(cr.normal);
+      const int ids[4] = {cr.node_ids[0], cr.node_ids[1], cr.node_ids[2], cr.node_ids[3]};
+      const float ws[4] = {cr.node_ws[0], cr.node_ws[1], cr.node_ws[2], cr.node_ws[3]};
+      float3 src;
+      if (a.canonical) {
+        src = f3(a.canonical[3 * c], a.canonical[3 * c + 1], a.canonical[3 * c + 2]);
+        DQ bl;
+        if (dq_blend4(a.centered, ids, ws, bl)) src = dq_transform(bl, src);
+      } else {
+        src = f3(cr.src);
+      }
+      const float r = dot(nrm, sub(src, dst));
+      fw = fmaxf(cr.weight, 0.0f) * huber_w(r, 0.2f);
+      st[0] = fw;
+      st[1] = r;
+      st[2] = __int_as_float(kslot);
+#pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        int pos = -1;
+        const int id = ids[l];
+        if (id >= 0 && id < a.N && ws[l] > 1e-8f) {
+          const float3 xp = f3(a.nodes[id].pos);
+          const float3 lp = sub(src, xp);
+          const float3 cp = cross(lp, nrm);
+          const float w = ws[l];
+          float *J = st + 7 + 6 * l;
+          J[0] = w * cp.x; J[1] = w * cp.y; J[2] = w * cp.z;
+          J[3] = w * nrm.x; J[4] = w * nrm.y; J[5] = w * nrm.z;
+          for (int q = 0; q < rlen; ++q)  // rows are short; linear scan
+            if (cols[q] == id) { pos = q; break; }
+          if (pos < 0) pos = -2;  // valid neighbour, block absent from the pattern
+        }
+        st[3 + l] = __int_as_float(pos);
+      }
+    }
+    st[0] = fw;  // inactive lanes hold 0; NaN stays active as in the reference
+    __syncthreads();
+    const int cnt = min(64, me - base);
+    for (int q = 0; q < cnt; ++q) {
+      const float *sq = stage + q * kStage;
+      const float f = sq[0];
+      if (f <= 1e-8f) continue;  // kernels/solver_kernels.cu:84
+      const int ks = __float_as_int(sq[2]);
+      const float *Jk = sq + 7 + 6 * ks;
+      if (lane < 36) {
+        const float fa = f * Jk[ea];
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+          const int pos = __float_as_int(sq[3 + l]);
+          if (pos >= 0) acc[pos * 36 + ab] += fa * sq[7 + 6 * l + eb];
+        }
+      } else if (ra >= 0 && ra < 6) {
+        bacc += -f * Jk[ra] * sq[1];
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- ARAP term (kernels/solver_kernels.cu:202-352)
+  const int sb = a.es_off[i], se = a.es_off[i + 1];
+  for (int base = sb; base < se; base += 64) {
+    const int e = base + lane;
+    float *st = stage + lane * kStage;
+    float lam = 0.f;
+    if (e < se) {
+      const int ent = a.es_entry[e];
+      const int edge = ent >> 1, side = ent & 1;
+      const int ni = a.e_i[edge], nj = a.e_j[edge];
+      const float nw = fmaxf(a.e_w[edge], 0.05f);
+      const float3 xi = f3(a.nodes[ni].pos), xj = f3(a.nodes[nj].pos);
+      const DQ Ti = a.T[ni], Tj = a.T[nj];
+      const float3 Ti_xj = dq_transform_centered(Ti, xj, xi);
+      const float3 Ti_xi = dq_transform_centered(Ti, xi, xi);
+      const float3 Tj_xj = dq_transform_centered(Tj, xj, xj);
+      const float3 rest = sub(xi, xj);
+      const float3 curr = sub(Ti_xi, Tj_xj);
+      const float dr = sqrtf(rest.x * rest.x + rest.y * rest.y + rest.z * rest.z);
+      const float dc = sqrtf(curr.x * curr.x + curr.y * curr.y + curr.z * curr.z);
+      const float strain = fabsf(dc - dr) / fmaxf(dr, 1e-6f);
+      const float ratio = strain / 0.25f;
+      const float sw = expf(-(ratio * ratio));
+      const float3 r = sub(Ti_xj, Tj_xj);
+      const float rn = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
+      const float hw = huber_w(rn, 0.05f);
+      lam = a.lambda_smooth * nw * hw * sw;
+      if (!(lam <= 1e-8f)) {  // kernels/solver_kernels.cu:271
+        float Ji[6][3], Jj[6][3];
+        arap_jac(sub(Ti_xj, xi), Ji);
+        arap_jac(sub(Tj_xj, xj), Jj);
+        st[1] = __int_as_float(side);
+        st[2] = __int_as_float(a.es_offrel[e]);
+        st[3] = r.x; st[4] = r.y; st[5] = r.z;
+        for (int p = 0; p < 6; ++p)
+          for (int c = 0; c < 3; ++c) {
+            st[6 + p * 3 + c] = side ? Jj[p][c] : Ji[p][c];
+            st[24 + p * 3 + c] = side ? Ji[p][c] : Jj[p][c];
+          }
+      }
+    }
+    st[0] = (lam <= 1e-8f) ? 0.f : lam;
+    __syncthreads();
+    const int cnt = min(64, se - base);
+    for (int q = 0; q < cnt; ++q) {
+      const float *sq = stage + q * kStage;
+      const float lm = sq[0];
+      if (lm <= 1e-8f) continue;
+      const int side = __float_as_int(sq[1]);
+      const int offrel = __float_as_int(sq[2]);
+      const float *Js = sq + 6, *Jo = sq + 24;
+      if (lane < 36) {
+        float vss = 0, vso = 0;
+        for (int c = 0; c < 3; ++c) vss += Js[ea * 3 + c] * Js[eb * 3 + c];
+        for (int c = 0; c < 3; ++c) vso += Js[ea * 3 + c] * Jo[eb * 3 + c];
+        acc[diag * 36 + ab] += lm * vss;
+        acc[offrel * 36 + ab] += -lm * vso;
+      } else if (ra >= 0 && ra < 6) {
+        float v = 0;
+        for (int c = 0; c < 3; ++c) v += Js[ra * 3 + c] * sq[3 + c];
+        bacc += side ? lm * v : -lm * v;
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- damping (kernels/solver_kernels.cu:167-194)
+  if (lane < 36 && ea == eb) {
+    const float cur = acc[diag * 36 + ab];
+    const float fl = (ea < 3) ? 1e-3f : 1e-2f;
+    acc[diag * 36 + ab] = cur + a.lambda_damping * fmaxf(fabsf(cur), fl);
+  }
+  __syncthreads();
+  for (int j = lane; j < rlen * 36; j += 64) a.values[(size_t)rb * 36 + j] = acc[j];
+  if (ra >= 0 && ra < 6) a.b[i * 6 + ra] = bacc;
+
+  // ---- block-Jacobi inverse (kernels/solver_kernels.cu:358-399)
+  if (lane < 6) {
+    const float *D = acc + diag * 36;
+    float outc[6];
+    bool ok = true;
+    if (a.precond_fp64) {
+      double d[36], x[6];
+      for (int k = 0; k < 36; ++k) d[k] = D[k];
+      for (int k = 0; k < 6; ++k) d[k * 7] += (double)1e-4f;
+      chol_inv6<double>(d, x, lane, ok);
+      for (int k = 0; k < 6; ++k) outc[k] = (float)x[k];
+      if (!ok)
+        for (int k = 0; k < 6; ++k) outc[k] = (k == lane) ? (float)(1.0 / (d[k * 7] + (double)1e-6f)) : 0.f;
+    } else {
+      float d[36];
+      for (int k = 0; k < 36; ++k) d[k] = D[k];
+      for (int k = 0; k < 6; ++k) d[k * 7] += 1e-4f;
+      chol_inv6<float>(d, outc, lane, ok);
+      if (!ok)
+        for (int k = 0; k < 6; ++k) outc[k] = (k == lane) ? 1.f / (d[k * 7] + 1e-6f) : 0.f;
+    }
+    for (int k = 0; k < 6; ++k) a.Minv[(size_t)i * 36 + k * 6 + lane] = outc[k];
+  }
+}
+
+// ----------------------------------------------------------------- PCG
+struct Barrier {
+  unsigned count;
+  unsigned gen;
+  unsigned abort;
+  unsigned pad;
+};
+
+__device__ inline bool grid_sync(Barrier *bar, unsigned G, unsigned &gen) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  __shared__ unsigned s_abort;
+  if (threadIdx.x == 0) {
+    unsigned my = gen;
+    unsigned arrived = __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned ab = 0;
+    if (arrived == G - 1) {
+      __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&bar->gen, my + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      unsigned spins = 0;
+      while (__hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == my) {
+        if (__hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        if (++spins > kBarrierSpinLimit) {
+          __hip_atomic_store(&bar->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    ab = __hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    gen = my + 1;
+    s_abort = ab;
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return s_abort != 0;
+}
+
+__device__ inline double block_sum(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  double s = 0;
+  for (int k = 0; k < kPcgThreads / 64; ++k) s += red[k];
+  return s;
+}
+
+// Sum of G partials, identical in every workgroup (fixed order).
+__device__ inline double grid_sum(const double *part, int G, double *red) {
+  double v = 0;
+  for (int g = threadIdx.x; g < G; g += kPcgThreads) v += __hip_atomic_load(&part[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return block_sum(v, red);
+}
+
+struct PcgArgs {
+  const int32_t *row_ptr, *col_idx;
+  const float *values, *b, *Minv;
+  float *x, *r, *z, *P0, *P1, *Ap;
+  double *part;  // [4][G]
+  Barrier *bar;
+  const int32_t *rows;  // [G+1] node partition
+  int N, maxit, G, lds_a;
+  float tol;
+  int rel;
+  nrf_pcg_result *res;
+  const int32_t *stop;
+};
+
+__global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
+  extern __shared__ float lds[];
+  __shared__ double red[kPcgThreads / 64];
+  if (a.stop && *a.stop) return;
+  const int g = blockIdx.x, G = a.G;
+  const int n0 = a.rows[g], n1 = a.rows[g + 1];
+  const int s0 = 6 * n0, s1 = 6 * n1;  // owned scalar rows
+  const int blk0 = a.row_ptr[n0], blk1 = a.row_ptr[n1];
+  float *Al = lds;
+  int32_t *Cl = reinterpret_cast<int32_t *>(lds + (a.lds_a ? (size_t)(blk1 - blk0) * 36 : 0));
+  if (a.lds_a) {
+    const float4 *src = reinterpret_cast<const float4 *>(a.values + (size_t)blk0 * 36);
+    float4 *dst = reinterpret_cast<float4 *>(Al);
+    for (int j = threadIdx.x; j < (blk1 - blk0) * 9; j += kPcgThreads) dst[j] = src[j];
+    for (int j = threadIdx.x; j < blk1 - blk0; j += kPcgThreads) Cl[j] = a.col_idx[blk0 + j];
+  }
+  double *p_rz = a.part, *p_pap = a.part + G, *p_rr = a.part + 2 * G, *p_bb = a.part + 3 * G;
+  unsigned gen = 0;
+  if (threadIdx.x == 0) gen = __hip_atomic_load(&a.bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+  // init: x = 0, r = b, z = M^-1 r  (kernels/solver_kernels.cu:620-632)
+  double lrz = 0, lbb = 0;
+  for (int s = s0 + threadIdx.x; s < s1; s += kPcgThreads) {
+    const int i = s / 6, row = s % 6;
+    a.x[s] = 0.f;
+    const float bv = a.b[s];
+    a.r[s] = bv;
+    const float *Mi = a.Minv + (size_t)i * 36;
+    float acc = 0;
+    for (int c = 0; c < 6; ++c) acc += Mi[row * 6 + c] * a.b[i * 6 + c];
+    a.z[s] = acc;
+    lrz += (double)bv * (double)acc;
+    lbb += (double)bv * (double)bv;
+  }
+  lrz = block_sum(lrz, red);
+  lbb = block_sum(lbb, red);
+  if (threadIdx.x == 0) {
+    p_rz[g] = lrz;
+    p_bb[g] = lbb;
+  }
+  bool aborted = grid_sync(a.bar, G, gen);
+  double rz = grid_sum(p_rz, G, red);
+  const double bb = grid_sum(p_bb, G, red);
+  double rr = bb;  // r = b
+  const double tol = a.rel ? (double)a.tol * sqrt(bb) : (double)a.tol;
+  int iters = 0, flags = 0;
+  float betaf = 0.f;
+  for (int it = 0; it < a.maxit && !aborted; ++it) {
+    float *Pold = (it & 1) ? a.P1 : a.P0;
+    float *Pnew = (it & 1) ? a.P0 : a.P1;
+    // Phase A: p = z + beta p (on the fly), Ap = A p, partial p.Ap
+    double lpap = 0;
+    for (int s = s0 + threadIdx.x; s < s1; s += kPcgThreads) {
+      const int i = s / 6, row = s % 6;
+      const int rbeg = a.row_ptr[i], rend = a.row_ptr[i + 1];
+      float y = 0.f;
+      for (int bk = rbeg; bk < rend; ++bk) {
+        const int j = a.lds_a ? Cl[bk - blk0] : a.col_idx[bk];
+        const float *A = a.lds_a ? (Al + (size_t)(bk - blk0) * 36) : (a.values + (size_t)bk * 36);
+        const float *zj = a.z + j * 6;
+        if (it == 0) {
+          for (int c = 0; c < 6; ++c) y += A[row * 6 + c] * zj[c];
+        } else {
+          const float *pj = Pold + j * 6;
+          for (int c = 0; c < 6; ++c) y += A[row * 6 + c] * (zj[c] + betaf * pj[c]);
+        }
+      }
+      const float pv = (it == 0) ? a.z[s] : (a.z[s] + betaf * Pold[s]);
+      Pnew[s] = pv;
+      a.Ap[s] = y;
+      lpap += (double)pv * (double)y;
+    }
+    lpap = block_sum(lpap, red);
+    if (threadIdx.x == 0) p_pap[g] = lpap;
+    aborted = grid_sync(a.bar, G, gen);
+    if (aborted) break;
+    const double pAp = grid_sum(p_pap, G, red);
+    if (fabs(pAp) < 1e-12) {
+      flags |= 1;
+      break;
+    }
+    const double alpha = rz / pAp;
+    const float alf = (float)alpha;
+    // Phase B: x += alpha p, r -= alpha Ap, z = M^-1 r, partial r.r and r.z
+    for (int s = s0 + threadIdx.x; s < s1; s += kPcgThreads) {
+      a.x[s] += alf * Pnew[s];
+      a.r[s] += -alf * a.Ap[s];
+    }
+    __syncthreads();
+    double lrr = 0, lrz2 = 0;
+    for (int s = s0 + threadIdx.x; s < s1; s += kPcgThreads) {
+      const int i = s / 6, row = s % 6;
+      const float *Mi = a.Minv + (size_t)i * 36;
+      const float *ri = a.r + i * 6;
+      float acc = 0;
+      for (int c = 0; c < 6; ++c) acc += Mi[row * 6 + c] * ri[c];
+      a.z[s] = acc;
+      const float rv = a.r[s];
+      lrr += (double)rv * (double)rv;
+      lrz2 += (double)rv * (double)acc;
+    }
+    lrr = block_sum(lrr, red);
+    lrz2 = block_sum(lrz2, red);
+    if (threadIdx.x == 0) {
+      p_rr[g] = lrr;
+      p_rz[g] = lrz2;
+    }
+    aborted = grid_sync(a.bar, G, gen);
+    if (aborted) break;
+    rr = grid_sum(p_rr, G, red);
+    const double rz_new = grid_sum(p_rz, G, red);
+    iters = it + 1;
+    if (sqrt(rr) < tol) {
+      flags |= 2;
+      break;
+    }
+    betaf = (float)(rz_new / rz);
+    rz = rz_new;
+  }
+  if (aborted) flags |= 8;
+  if (g == 0 && threadIdx.x == 0) {
+    a.res->iterations = iters;
+    a.res->total_iterations += iters;
+    a.res->flags |= flags;
+    a.res->final_residual = sqrt(rr);
+    a.res->rhs_norm = sqrt(bb);
+  }
+}
+
+__global__ void pcg_prep_kernel(Barrier *bar) {
+  if (threadIdx.x == 0) {
+    bar->count = 0;
+    bar->abort = 0;
+  }
+}
+
+__global__ void gn_begin_kernel(int32_t *stop, nrf_pcg_result *res) {
+  if (threadIdx.x == 0) {
+    *stop = 0;
+    res->iterations = 0;
+    res->total_iterations = 0;
+    res->flags = 0;
+    res->gn_done = 0;
+    res->final_residual = 0;
+    res->rhs_norm = 0;
+  }
+}
+
+// Step validation (include/fusion_app.h:409-491) and application with the
+// per-node clamp + scale of upload_scaled_delta_x (fusion_app.h:1353-1375)
+// followed by apply_twist_increment(..., 1.0) (src/warp_field.cu:1121-1141).
+struct GnArgs {
+  const float *dx;
+  DQ *T;
+  DQ *centered;
+  const nrf_node *nodes;
+  int N;
+  float scale, max_rot, max_trans, max_dx_mean;
+  int validate;
+  int32_t *stop;
+  nrf_pcg_result *res;
+};
+
+__global__ __launch_bounds__(1024) void gn_apply_kernel(GnArgs a) {
+  if (*a.stop) return;
+  __shared__ double s_sum[4][1024 / 64];
+  __shared__ float s_max[4][1024 / 64];
+  __shared__ int s_fin[1024 / 64];
+  __shared__ int s_ok;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (a.validate) {
+    double rs = 0, ts = 0, ers = 0, ets = 0;
+    float rm = 0, tm = 0, erm = 0, etm = 0;
+    int fin = 1;
+    for (int i = t; i < a.N; i += 1024) {
+      const float *d = a.dx + i * 6;
+      const float rn = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      const float tn = sqrtf(d[3] * d[3] + d[4] * d[4] + d[5] * d[5]);
+      fin &= (isfinite(rn) && isfinite(tn)) ? 1 : 0;
+      rs += rn; ts += tn;
+      rm = fmaxf(rm, rn); tm = fmaxf(tm, tn);
+      float er = a.scale * rn, et = a.scale * tn;
+      if (a.max_rot > 0.0f) er = fminf(er, a.max_rot);
+      if (a.max_trans > 0.0f) et = fminf(et, a.max_trans);
+      ers += er; ets += et;
+      erm = fmaxf(erm, er); etm = fmaxf(etm, et);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      rs += __shfl_xor(rs, off, 64); ts += __shfl_xor(ts, off, 64);
+      ers += __shfl_xor(ers, off, 64); ets += __shfl_xor(ets, off, 64);
+      rm = fmaxf(rm, __shfl_xor(rm, off, 64)); tm = fmaxf(tm, __shfl_xor(tm, off, 64));
+      erm = fmaxf(erm, __shfl_xor(erm, off, 64)); etm = fmaxf(etm, __shfl_xor(etm, off, 64));
+      fin &= __shfl_xor(fin, off, 64);
+    }
+    if (lane == 0) {
+      s_sum[0][w] = rs; s_sum[1][w] = ts; s_sum[2][w] = ers; s_sum[3][w] = ets;
+      s_max[0][w] = rm; s_max[1][w] = tm; s_max[2][w] = erm; s_max[3][w] = etm;
+      s_fin[w] = fin;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double S[4] = {0, 0, 0, 0};
+      float M[4] = {0, 0, 0, 0};
+      int F = 1;
+      for (int k = 0; k < 1024 / 64; ++k) {
+        for (int q = 0; q < 4; ++q) {
+          S[q] += s_sum[q][k];
+          M[q] = fmaxf(M[q], s_max[q][k]);
+        }
+        F &= s_fin[k];
+      }
+      const float n = (float)max(1, a.N);
+      const float eff_rot_mean = (float)S[2] / n, eff_trans_mean = (float)S[3] / n;
+      const float reject_rot = a.max_rot > 0.0f ? 5.0f * a.max_rot : INFINITY;
+      const float reject_trans = a.max_trans > 0.0f ? 5.0f * a.max_trans : INFINITY;
+      const bool bad_spike = (a.scale * M[0] > reject_rot) || (a.scale * M[1] > reject_trans);
+      const bool mean_ok = (a.max_dx_mean <= 0.0f) ||
+                           (eff_rot_mean <= a.max_dx_mean && eff_trans_mean <= a.max_dx_mean);
+      const bool ok = F && isfinite(eff_rot_mean) && isfinite(eff_trans_mean) && isfinite(M[2]) &&
+                      isfinite(M[3]) && (M[2] < reject_rot) && (M[3] < reject_trans) && mean_ok && !bad_spike;
+      s_ok = ok ? 1 : 0;
+      if (!ok) {
+        *a.stop = 1;
+        a.res->flags |= 4;
+      }
+    }
+    __syncthreads();
+    if (!s_ok) return;
+  }
+  for (int i = t; i < a.N; i += 1024) {
+    const float *d = a.dx + i * 6;
+    float tw[6];
+    for (int k = 0; k < 6; ++k) tw[k] = d[k];
+    const float rn = sqrtf(tw[0] * tw[0] + tw[1] * tw[1] + tw[2] * tw[2]);
+    const float tn = sqrtf(tw[3] * tw[3] + tw[4] * tw[4] + tw[5] * tw[5]);
+    float rsc = 1.0f, tsc = 1.0f;
+    if (a.max_rot > 0.0f && rn > a.max_rot) rsc = a.max_rot / rn;
+    if (a.max_trans > 0.0f && tn > a.max_trans) tsc = a.max_trans / tn;
+    for (int k = 0; k < 3; ++k) tw[k] *= rsc * a.scale;
+    for (int k = 3; k < 6; ++k) tw[k] *= tsc * a.scale;
+    // apply_twist_kernel with update_scale = 1
+    for (int k = 0; k < 6; ++k) tw[k] = 1.0f * tw[k];
+    const float rn2 = sqrtf(tw[0] * tw[0] + tw[1] * tw[1] + tw[2] * tw[2]);
+    if (a.max_rot > 0.0f && rn2 > a.max_rot) {
+      const float f = a.max_rot / rn2;
+      tw[0] *= f; tw[1] *= f; tw[2] *= f;
+    }
+    const float tn2 = sqrtf(tw[3] * tw[3] + tw[4] * tw[4] + tw[5] * tw[5]);
+    if (a.max_trans > 0.0f && tn2 > a.max_trans) {
+      const float f = a.max_trans / tn2;
+      tw[3] *= f; tw[4] *= f; tw[5] *= f;
+    }
+    const DQ dT = mat_to_dq(exp_se3(tw));
+    const DQ Tn = dq_mul(dT, a.T[i]);
+    a.T[i] = Tn;
+    if (a.centered) a.centered[i] = dq_centered(Tn, f3(a.nodes[i].pos));
+  }
+  if (t == 0) a.res->gn_done += 1;
+}
+
+}  // namespace
+
+// ------------------------------------------------------- host-side solver
+struct Pattern {
+  int N = -1, nb = 0, E = 0, Rmax = 0;
+  std::vector<int32_t> row_ptr, col_idx, diag_rel, e_i, e_j, es_off, es_entry, es_offrel;
+  std::vector<float> e_w;
+};
+
+// build_sparsity_pattern (kernels/solver_kernels.cu:498-541) plus the ARAP
+// edge lists of each block row, in the order a sequential pass over nodes
+// and their neighbour slots visits them (solver_kernels.cu:225-233).
+static void build_pattern(const nrf_node *h, int N, Pattern &P) {
+  std::vector<std::vector<int32_t>> rc(N);
+  for (int i = 0; i < N; ++i) rc[i].push_back(i);
+  auto nnb = [](const nrf_node &n) { return std::max(0, std::min((int)n.num_neighbors, (int)NRF_K_GRAPH)); };
+  for (int i = 0; i < N; ++i)
+    for (int k = 0; k < nnb(h[i]); ++k) {
+      int j = h[i].neighbors[k];
+      if (j < 0 || j >= N) continue;
+      rc[i].push_back(j);
+      rc[j].push_back(i);
+    }
+  P.N = N;
+  P.row_ptr.assign(N + 1, 0);
+  P.col_idx.clear();
+  P.diag_rel.assign(N, 0);
+  P.Rmax = 0;
+  for (int i = 0; i < N; ++i) {
+    auto &c = rc[i];
+    std::sort(c.begin(), c.end());
+    c.erase(std::unique(c.begin(), c.end()), c.end());
+    P.row_ptr[i] = (int32_t)P.col_idx.size();
+    for (size_t q = 0; q < c.size(); ++q) {
+      if (c[q] == i) P.diag_rel[i] = (int32_t)q;
+      P.col_idx.push_back(c[q]);
+    }
+    P.Rmax = std::max(P.Rmax, (int)c.size());
+  }
+  P.row_ptr[N] = (int32_t)P.col_idx.size();
+  P.nb = (int)P.col_idx.size();
+  auto rel = [&](int row, int col) {
+    auto b = P.col_idx.begin() + P.row_ptr[row], e = P.col_idx.begin() + P.row_ptr[row + 1];
+    return (int32_t)(std::lower_bound(b, e, col) - b);
+  };
+  P.e_i.clear(); P.e_j.clear(); P.e_w.clear();
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> es(N);
+  for (int i = 0; i < N; ++i)
+    for (int k = 0; k < nnb(h[i]); ++k) {
+      int j = h[i].neighbors[k];
+      if (j < 0 || j >= N || j <= i) continue;
+      int e = (int)P.e_i.size();
+      P.e_i.push_back(i);
+      P.e_j.push_back(j);
+      P.e_w.push_back(h[i].neighbor_w[k]);
+      es[i].push_back({e * 2 + 0, rel(i, j)});
+      es[j].push_back({e * 2 + 1, rel(j, i)});
+    }
+  P.E = (int)P.e_i.size();
+  P.es_off.assign(N + 1, 0);
+  P.es_entry.clear();
+  P.es_offrel.clear();
+  for (int i = 0; i < N; ++i) {
+    P.es_off[i] = (int32_t)P.es_entry.size();
+    for (auto &pr : es[i]) {
+      P.es_entry.push_back(pr.first);
+      P.es_offrel.push_back(pr.second);
+    }
+  }
+  P.es_off[N] = (int32_t)P.es_entry.size();
+}
+
+template <typename T>
+static int dev_alloc(T *&p, size_t n, size_t &cap) {
+  if (n <= cap && p) return NRF_OK;
+  if (p) hipFree(p);
+  p = nullptr;
+  cap = 0;
+  size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  NRF_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&p), bytes));
+  cap = n;
+  return NRF_OK;
+}
+
+template <typename T>
+static int upload(T *&p, size_t &cap, const std::vector<T> &v) {
+  int st = dev_alloc(p, v.size(), cap);
+  if (st) return st;
+  if (!v.empty()) NRF_HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return NRF_OK;
+}
+
+}  // namespace nrf
+
+using namespace nrf;
+
+struct nrf_solver {
+  nrf_solver_params params;
+  int max_nodes = 0;
+  Pattern P;
+  // pattern buffers
+  int32_t *row_ptr = nullptr, *col_idx = nullptr, *diag_rel = nullptr, *e_i = nullptr, *e_j = nullptr,
+          *es_off = nullptr, *es_entry = nullptr, *es_offrel = nullptr, *rows = nullptr;
+  float *e_w = nullptr;
+  size_t c_row_ptr = 0, c_col_idx = 0, c_diag_rel = 0, c_e_i = 0, c_e_j = 0, c_es_off = 0, c_es_entry = 0,
+         c_es_offrel = 0, c_rows = 0, c_e_w = 0;
+  // system
+  float *values = nullptr, *b = nullptr, *Minv = nullptr;
+  size_t c_values = 0, c_b = 0, c_Minv = 0;
+  // pcg
+  float *r = nullptr, *z = nullptr, *P0 = nullptr, *P1 = nullptr, *Ap = nullptr;
+  size_t c_r = 0, c_z = 0, c_P0 = 0, c_P1 = 0, c_Ap = 0;
+  double *part = nullptr;
+  size_t c_part = 0;
+  Barrier *bar = nullptr;
+  size_t c_bar = 0;
+  int G = 1, lds_a = 0;
+  size_t pcg_lds = 0;
+  // memberships
+  int32_t *m_cnt = nullptr, *m_off = nullptr, *m_cur = nullptr, *m_keys = nullptr, *m_keys2 = nullptr;
+  size_t c_m_cnt = 0, c_m_off = 0, c_m_cur = 0, c_m_keys = 0, c_m_keys2 = 0;
+  // GN state
+  int32_t *stop = nullptr;
+  size_t c_stop = 0;
+  nrf_pcg_result *d_res = nullptr;
+  size_t c_res = 0;
+  int cus = 0, max_lds = 0, coop = 0;
+  int forced_G = 0;
+};
+
+namespace nrf {
+
+static int solver_set_pattern_host(nrf_solver *s, const nrf_node *h, int N) {
+  build_pattern(h, N, s->P);
+  const Pattern &P = s->P;
+  int st;
+  if ((st = upload(s->row_ptr, s->c_row_ptr, P.row_ptr))) return st;
+  if ((st = upload(s->col_idx, s->c_col_idx, P.col_idx))) return st;
+  if ((st = upload(s->diag_rel, s->c_diag_rel, P.diag_rel))) return st;
+  if ((st = upload(s->e_i, s->c_e_i, P.e_i))) return st;
+  if ((st = upload(s->e_j, s->c_e_j, P.e_j))) return st;
+  if ((st = upload(s->e_w, s->c_e_w, P.e_w))) return st;
+  if ((st = upload(s->es_off, s->c_es_off, P.es_off))) return st;
+  if ((st = upload(s->es_entry, s->c_es_entry, P.es_entry))) return st;
+  if ((st = upload(s->es_offrel, s->c_es_offrel, P.es_offrel))) return st;
+  if ((st = dev_alloc(s->values, (size_t)std::max(P.nb, 1) * 36, s->c_values))) return st;
+  // PCG partition: contiguous node ranges with about nb / G blocks each.
+  int G = s->forced_G;
+  if (G <= 0) G = std::max(1, std::min(s->cus, (N + 31) / 32));
+  G = std::max(1, std::min(G, std::max(1, N)));
+  std::vector<int32_t> rows(G + 1, N);
+  rows[0] = 0;
+  {
+    int g = 1;
+    for (int i = 0; i < N && g < G; ++i) {
+      const long target = (long)P.nb * g / G;
+      if (P.row_ptr[i + 1] >= target) rows[g++] = i + 1;
+    }
+    for (; g < G; ++g) rows[g] = N;
+  }
+  size_t maxblk = 0;
+  for (int g = 0; g < G; ++g) maxblk = std::max(maxblk, (size_t)(P.row_ptr[rows[g + 1]] - P.row_ptr[rows[g]]));
+  s->G = G;
+  const size_t need = maxblk * 36 * sizeof(float) + maxblk * sizeof(int32_t);
+  const size_t budget = (size_t)std::max(0, s->max_lds - 2048);
+  s->lds_a = need <= budget ? 1 : 0;
+  s->pcg_lds = s->lds_a ? align_up(need, 16) : 16;
+  if ((st = upload(s->rows, s->c_rows, rows))) return st;
+  if ((st = dev_alloc(s->part, (size_t)4 * G, s->c_part))) return st;
+  return NRF_OK;
+}
+
+static int solver_build_memberships(nrf_solver *s, const nrf_corr *corrs, int n, int N, hipStream_t st) {
+  int rc;
+  if ((rc = dev_alloc(s->m_cnt, (size_t)N, s->c_m_cnt))) return rc;
+  if ((rc = dev_alloc(s->m_off, (size_t)N + 1, s->c_m_off))) return rc;
+  if ((rc = dev_alloc(s->m_cur, (size_t)N, s->c_m_cur))) return rc;
+  if ((rc = dev_alloc(s->m_keys, (size_t)n * 4, s->c_m_keys))) return rc;
+  if ((rc = dev_alloc(s->m_keys2, (size_t)n * 4, s->c_m_keys2))) return rc;
+  fill_i32<<<(N + 255) / 256, 256, 0, st>>>(s->m_cnt, N, 0);
+  NRF_LAUNCH_CHECK();
+  if (n > 0) {
+    mem_count_kernel<<<(n + 255) / 256, 256, 0, st>>>(corrs, n, N, s->m_cnt);
+    NRF_LAUNCH_CHECK();
+  }
+  exclusive_scan_kernel<<<1, 1024, 0, st>>>(s->m_cnt, s->m_off, s->m_cur, N);
+  NRF_LAUNCH_CHECK();
+  if (n > 0) {
+    mem_scatter_kernel<<<(n + 255) / 256, 256, 0, st>>>(corrs, n, N, s->m_cur, s->m_keys);
+    NRF_LAUNCH_CHECK();
+  }
+  mem_sort_kernel<<<N, 64, 0, st>>>(s->m_off, s->m_keys, s->m_keys2);
+  NRF_LAUNCH_CHECK();
+  return NRF_OK;
+}
+
+static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, const float *canonical, const nrf_node *nodes,
+                           const DQ *T, const DQ *centered, int N, hipStream_t st) {
+  AsmArgs a;
+  a.corrs = corrs;
+  a.canonical = canonical;
+  a.nodes = nodes;
+  a.T = T;
+  a.centered = centered;
+  a.N = N;
+  a.row_ptr = s->row_ptr;
+  a.col_idx = s->col_idx;
+  a.diag_rel = s->diag_rel;
+  a.m_off = s->m_off;
+  a.m_keys = s->m_keys2;
+  a.es_off = s->es_off;
+  a.es_entry = s->es_entry;
+  a.es_offrel = s->es_offrel;
+  a.e_i = s->e_i;
+  a.e_j = s->e_j;
+  a.e_w = s->e_w;
+  a.lambda_smooth = s->params.lambda_smooth;
+  a.lambda_damping = s->params.lambda_damping;
+  a.precond_fp64 = s->params.precond_fp64;
+  a.Rmax = s->P.Rmax;
+  a.values = s->values;
+  a.b = s->b;
+  a.Minv = s->Minv;
+  a.stop = s->stop;
+  const size_t lds = (size_t)s->P.Rmax * 36 * 4 + (size_t)s->P.Rmax * 4 + 64 * kStage * 4;
+  if (lds > 64 * 1024) return NRF_ERR_TOO_LARGE;
+  assemble_rows_kernel<<<N, 64, lds, st>>>(a);
+  NRF_LAUNCH_CHECK();
+  return NRF_OK;
+}
+
+static int solver_pcg(nrf_solver *s, float *x, int N, hipStream_t st) {
+  pcg_prep_kernel<<<1, 64, 0, st>>>(s->bar);
+  NRF_LAUNCH_CHECK();
+  PcgArgs a;
+  a.row_ptr = s->row_ptr;
+  a.col_idx = s->col_idx;
+  a.values = s->values;
+  a.b = s->b;
+  a.Minv = s->Minv;
+  a.x = x;
+  a.r = s->r;
+  a.z = s->z;
+  a.P0 = s->P0;
+  a.P1 = s->P1;
+  a.Ap = s->Ap;
+  a.part = s->part;
+  a.bar = s->bar;
+  a.rows = s->rows;
+  a.N = N;
+  a.maxit = s->params.pcg_iterations;
+  a.G = s->G;
+  a.lds_a = s->lds_a;
+  a.tol = s->params.pcg_tolerance;
+  a.rel = s->params.relative_tolerance;
+  a.res = s->d_res;
+  a.stop = s->stop;
+  void *args[] = {&a};
+  if (s->coop) {
+    NRF_HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(pcg_kernel), dim3(s->G),
+                                             dim3(kPcgThreads), args, (unsigned)s->pcg_lds, st));
+  } else {
+    pcg_kernel<<<s->G, kPcgThreads, s->pcg_lds, st>>>(a);
+    NRF_LAUNCH_CHECK();
+  }
+  return NRF_OK;
+}
+
+static int solver_fetch_result(nrf_solver *s, nrf_pcg_result *out, hipStream_t st) {
+  if (!out) return NRF_OK;
+  NRF_HIP_CHECK(hipMemcpyAsync(out, s->d_res, sizeof(nrf_pcg_result), hipMemcpyDeviceToHost, st));
+  NRF_HIP_CHECK(hipStreamSynchronize(st));
+  if (out->flags & 8) return NRF_ERR_BARRIER_TIMEOUT;
+  return NRF_OK;
+}
+
+static int solver_ensure_pattern(nrf_solver *s, const nrf_node *d_nodes, int N, hipStream_t st) {
+  if (N == s->P.N) return NRF_OK;
+  // Q4 of the reference: rebuilt only when the node count changes
+  // (kernels/solver_kernels.cu:554-561).
+  std::vector<nrf_node> h(N);
+  NRF_HIP_CHECK(hipMemcpyAsync(h.data(), d_nodes, sizeof(nrf_node) * N, hipMemcpyDeviceToHost, st));
+  NRF_HIP_CHECK(hipStreamSynchronize(st));
+  return solver_set_pattern_host(s, h.data(), N);
+}
+
+}  // namespace nrf
+
+extern "C" nrf_solver *nrf_solver_create(const nrf_solver_params *params, int max_nodes) {
+  if (!params || max_nodes <= 0) return nullptr;
+  nrf_solver *s = new nrf_solver();
+  s->params = *params;
+  s->max_nodes = max_nodes;
+  s->cus = device_cu_count();
+  s->max_lds = device_max_lds_per_block();
+  s->coop = device_cooperative() ? 1 : 0;
+  const char *fg = getenv("NRF_PCG_WORKGROUPS");
+  if (fg) s->forced_G = atoi(fg);
+  const size_t n6 = (size_t)max_nodes * 6;
+  bool ok = dev_alloc(s->b, n6, s->c_b) == 0 && dev_alloc(s->Minv, (size_t)max_nodes * 36, s->c_Minv) == 0 &&
+            dev_alloc(s->r, n6, s->c_r) == 0 && dev_alloc(s->z, n6, s->c_z) == 0 &&
+            dev_alloc(s->P0, n6, s->c_P0) == 0 && dev_alloc(s->P1, n6, s->c_P1) == 0 &&
+            dev_alloc(s->Ap, n6, s->c_Ap) == 0 && dev_alloc(s->bar, 1, s->c_bar) == 0 &&
+            dev_alloc(s->stop, 1, s->c_stop) == 0 && dev_alloc(s->d_res, 1, s->c_res) == 0;
+  if (ok) ok = hipMemset(s->bar, 0, sizeof(Barrier)) == hipSuccess &&
+               hipMemset(s->stop, 0, sizeof(int32_t)) == hipSuccess &&
+               hipMemset(s->d_res, 0, sizeof(nrf_pcg_result)) == hipSuccess;
+  if (!ok) {
+    nrf_solver_destroy(s);
+    return nullptr;
+  }
+  return s;
+}
+
+extern "C" void nrf_solver_destroy(nrf_solver *s) {
+  if (!s) return;
+  void *ptrs[] = {s->row_ptr, s->col_idx, s->diag_rel, s->e_i, s->e_j, s->es_off, s->es_entry, s->es_offrel,
+                  s->rows, s->e_w, s->values, s->b, s->Minv, s->r, s->z, s->P0, s->P1, s->Ap, s->part, s->bar,
+                  s->m_cnt, s->m_off, s->m_cur, s->m_keys, s->m_keys2, s->stop, s->d_res};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  delete s;
+}
+
+extern "C" int nrf_solver_set_params(nrf_solver *s, const nrf_solver_params *params) {
+  if (!s || !params) return NRF_ERR_INVALID_ARGUMENT;
+  s->params = *params;
+  return NRF_OK;
+}
+
+extern "C" int nrf_solver_set_pattern(nrf_solver *s, const nrf_node *host_nodes, int num_nodes) {
+  if (!s || num_nodes < 0 || num_nodes > s->max_nodes || (num_nodes > 0 && !host_nodes))
+    return NRF_ERR_INVALID_ARGUMENT;
+  return solver_set_pattern_host(s, host_nodes, num_nodes);
+}
+
+extern "C" int nrf_solver_solve(nrf_solver *s, const nrf_corr *corrs, int num_corrs, const nrf_node *nodes,
+                                const nrf_dualquat *transforms, int num_nodes, float *delta_x,
+                                nrf_pcg_result *result, void *stream) {
+  if (!s || num_corrs < 0 || num_nodes < 0 || num_nodes > s->max_nodes) return NRF_ERR_INVALID_ARGUMENT;
+  if (num_nodes == 0) return NRF_OK;
+  if (!nodes || !transforms || !delta_x || (num_corrs > 0 && !corrs)) return NRF_ERR_INVALID_ARGUMENT;
+  hipStream_t st = as_stream(stream);
+  int rc;
+  if ((rc = solver_ensure_pattern(s, nodes, num_nodes, st))) return rc;
+  if ((rc = solver_build_memberships(s, corrs, num_corrs, num_nodes, st))) return rc;
+  gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res);
+  NRF_LAUNCH_CHECK();
+  if ((rc = solver_assemble(s, corrs, nullptr, nodes, reinterpret_cast<const DQ *>(transforms), nullptr,
+                            num_nodes, st)))
+    return rc;
+  if ((rc = solver_pcg(s, delta_x, num_nodes, st))) return rc;
+  return solver_fetch_result(s, result, st);
+}
+
+extern "C" int nrf_solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical, int num_corrs,
+                                   const nrf_node *nodes, nrf_dualquat *transforms, nrf_dualquat *centered,
+                                   int num_nodes, const nrf_gn_params *gn, float *delta_x,
+                                   nrf_pcg_result *result, void *stream) {
+  if (!s || !gn || num_corrs < 0 || num_nodes < 0 || num_nodes > s->max_nodes) return NRF_ERR_INVALID_ARGUMENT;
+  if (num_nodes == 0) return NRF_OK;
+  if (!nodes || !transforms || !centered || !delta_x || (num_corrs > 0 && !corrs)) return NRF_ERR_INVALID_ARGUMENT;
+  if (s->P.N != num_nodes) return NRF_ERR_NOT_READY;
+  hipStream_t st = as_stream(stream);
+  int rc;
+  if ((rc = solver_build_memberships(s, corrs, num_corrs, num_nodes, st))) return rc;
+  gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res);
+  NRF_LAUNCH_CHECK();
+  GnArgs g;
+  g.dx = delta_x;
+  g.T = reinterpret_cast<DQ *>(transforms);
+  g.centered = reinterpret_cast<DQ *>(centered);
+  g.nodes = nodes;
+  g.N = num_nodes;
+  g.scale = gn->update_scale;
+  g.max_rot = gn->max_update_rot;
+  g.max_trans = gn->max_update_trans;
+  g.max_dx_mean = gn->max_dx_mean;
+  g.validate = gn->validate;
+  g.stop = s->stop;
+  g.res = s->d_res;
+  for (int it = 0; it < std::max(1, gn->gn_iterations); ++it) {
+    if ((rc = solver_assemble(s, corrs, canonical, nodes, reinterpret_cast<const DQ *>(transforms),
+                              reinterpret_cast<const DQ *>(centered), num_nodes, st)))
+      return rc;
+    if ((rc = solver_pcg(s, delta_x, num_nodes, st))) return rc;
+    gn_apply_kernel<<<1, 1024, 0, st>>>(g);
+    NRF_LAUNCH_CHECK();
+  }
+  return solver_fetch_result(s, result, st);
+}
+
+extern "C" int nrf_solver_system(const nrf_solver *s, const int32_t **row_ptr, const int32_t **col_idx,
+                                 const float **values, int *num_blocks, const float **rhs, const float **precond) {
+  if (!s) return NRF_ERR_INVALID_ARGUMENT;
+  if (s->P.N < 0) return NRF_ERR_NOT_READY;
+  if (row_ptr) *row_ptr = s->row_ptr;
+  if (col_idx) *col_idx = s->col_idx;
+  if (values) *values = s->values;
+  if (num_blocks) *num_blocks = s->P.nb;
+  if (rhs) *rhs = s->b;
+  if (precond) *precond = s->Minv;
+  return NRF_OK;
+}
+
+extern "C" const nrf_pcg_result *nrf_solver_device_result(const nrf_solver *s) { return s ? s->d_res : nullptr; }

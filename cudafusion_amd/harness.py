@@ -1,0 +1,187 @@
+"""Workload driver: synthetic scene -> device inputs -> the non-rigid step.
+
+Used by bench.py, smoke() and the GPU tests.  Frame 0 is integrated
+rigidly, the node graph is sampled on the canonical surface and the voxel
+kNN snapshot taken (the reference's initialize_node_graph,
+include/fusion_app.h:994-1003); every later frame is one call of
+nrf_pipeline_step on inputs that are already resident in device memory.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import scene as S
+
+
+def _t(a: np.ndarray, dev) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def identity_pose() -> np.ndarray:
+    return np.eye(4, dtype=np.float32).reshape(16)
+
+
+def dq_to_rt(q: np.ndarray):
+    """Rotation matrix and translation of a unit dual quaternion (x, y, z, w)."""
+    q = np.asarray(q, np.float64)
+    r, d = q[:4] / np.linalg.norm(q[:4]), q[4:] / np.linalg.norm(q[:4])
+    x, y, z, w = r
+    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                  [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                  [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+    # t = 2 d r*
+    cx, cy, cz, cw = -x, -y, -z, w
+    dx, dy, dz, dw = d
+    tx = dw * cx + dx * cw + dy * cz - dz * cy
+    ty = dw * cy - dx * cz + dy * cw + dz * cx
+    tz = dw * cz + dx * cy - dy * cx + dz * cw
+    return R, 2 * np.array([tx, ty, tz])
+
+
+def transform_errors(Ta, Tb):
+    """Max rotation angle (rad) and translation (m) differences between two
+    transform sets (DUALQUAT records or (N, 8) arrays)."""
+    A = np.ascontiguousarray(Ta).view(np.float32).reshape(-1, 8) if np.asarray(Ta).dtype == L.DUALQUAT \
+        else np.asarray(Ta, np.float64).reshape(-1, 8)
+    B = np.ascontiguousarray(Tb).view(np.float32).reshape(-1, 8) if np.asarray(Tb).dtype == L.DUALQUAT \
+        else np.asarray(Tb, np.float64).reshape(-1, 8)
+    er, et = 0.0, 0.0
+    for a, b in zip(A, B):
+        Ra, ta = dq_to_rt(a)
+        Rb, tb = dq_to_rt(b)
+        c = np.clip((np.trace(Ra.T @ Rb) - 1) / 2, -1, 1)
+        er = max(er, float(np.arccos(c)))
+        et = max(et, float(np.linalg.norm(ta - tb)))
+    return er, et
+
+
+class Harness:
+    def __init__(self, kind: str = "body", frames: int = 40, use_graph: bool = False, seed: int = 0,
+                 pcg_tolerance: float = 0.0, relative_tolerance: bool = False, precond_fp64: bool = False):
+        self.kind = kind
+        self.scene = S.make_scene(kind, frames, seed)
+        self.w = self.scene.w
+        w = self.w
+        self.frames = frames
+        self.vp = L.volume_params(w.dims, w.voxel_size, w.truncation, w.origin, w.decay_alpha, w.max_weight)
+        self.cam = L.camera(w.fx, w.fy, w.cx, w.cy, w.width, w.height)
+        self.sp = L.solver_params(w.gn_iterations, w.pcg_iterations, pcg_tolerance, w.lambda_smooth, 1.0,
+                                  w.lambda_damping, 0, int(relative_tolerance), int(precond_fp64))
+        self.gn = L.gn_params(w.gn_iterations, w.update_scale, w.max_update_rot, w.max_update_trans, 1,
+                              w.max_dx_mean)
+        self.use_graph = use_graph
+        self.pose = identity_pose()
+        self.pipe = None
+
+    # ------------------------------------------------------------ setup
+    def setup(self, max_triangles: int | None = None) -> None:
+        lib = L.load()
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.dev = dev
+        w = self.w
+        npx = w.width * w.height
+        cfg = L.PipelineConfig()
+        cfg.volume = self.vp
+        cfg.camera = self.cam
+        cfg.solver = self.sp
+        cfg.gn = self.gn
+        cfg.max_nodes = 8192
+        cfg.max_corrs = npx
+        cfg.max_triangles = max_triangles or (2_000_000 if w.dims[0] >= 256 else 400_000)
+        cfg.min_opt_count = w.min_opt_count
+        cfg.use_graph = int(self.use_graph)
+        self.cfg = cfg
+        self.pipe = lib.nrf_pipeline_create(C.byref(cfg))
+        if not self.pipe:
+            raise L.NrfError("nrf_pipeline_create failed: " + lib.nrf_last_error().decode())
+        self.lib = lib
+        self.frames_host = [self.scene.render(f) for f in range(self.frames + 1)]
+        self.depth = [_t(fr.depth.reshape(-1), dev) for fr in self.frames_host]
+        stream = torch.cuda.current_stream().cuda_stream
+        L.check(lib.nrf_pipeline_integrate_rigid(self.pipe, self.depth[0].data_ptr(), self.pose.ctypes.data,
+                                                 stream), "integrate_rigid")
+        torch.cuda.synchronize()
+        self.nodes = S.scene_nodes(self.scene)
+        self.N = len(self.nodes)
+        L.check(lib.nrf_pipeline_set_nodes(self.pipe, self.nodes.ctypes.data, self.N), "set_nodes")
+        # association inputs for frames 1..F, resident on the device
+        self.canonical = [None]
+        self.corrs = [None]
+        knn = lib.nrf_pipeline_knn(self.pipe)
+        knn_w = lib.nrf_pipeline_knn_w(self.pipe)
+        for f in range(1, self.frames + 1):
+            fr = self.frames_host[f]
+            can = _t(fr.canonical.reshape(-1), dev)
+            dst = _t(fr.dst.reshape(-1), dev)
+            nrm = _t(fr.normal.reshape(-1), dev)
+            val = _t(fr.valid.reshape(-1), dev)
+            out = torch.empty(npx * 76, dtype=torch.uint8, device=dev)
+            L.check(lib.nrf_corr_pack(C.byref(self.vp), knn, knn_w, can.data_ptr(), dst.data_ptr(),
+                                      nrm.data_ptr(), None, val.data_ptr(), npx, out.data_ptr(), stream),
+                    "corr_pack")
+            self.canonical.append(can)
+            self.corrs.append(out)
+        torch.cuda.synchronize()
+
+    # ------------------------------------------------------------- step
+    def step(self, f: int) -> None:
+        stream = torch.cuda.current_stream().cuda_stream
+        L.check(self.lib.nrf_pipeline_step(self.pipe, self.depth[f].data_ptr(), self.pose.ctypes.data,
+                                           self.corrs[f].data_ptr(), self.canonical[f].data_ptr(),
+                                           self.w.width * self.w.height, stream), "pipeline_step")
+
+    def stage_ms(self):
+        out = (C.c_float * 3)()
+        L.check(self.lib.nrf_pipeline_stage_ms(self.pipe, out), "stage_ms")
+        return list(out)
+
+    # ---------------------------------------------------------- readback
+    def _d2h(self, ptr: int, count: int, dtype) -> np.ndarray:
+        from .fusion import copy_device_to_numpy
+        return copy_device_to_numpy(ptr, count, dtype)
+
+    def download_transforms(self) -> np.ndarray:
+        return self._d2h(self.lib.nrf_pipeline_transforms(self.pipe), self.N, L.DUALQUAT)
+
+    def download_voxels(self) -> np.ndarray:
+        V = int(np.prod(self.w.dims))
+        return self._d2h(self.lib.nrf_pipeline_voxels(self.pipe), V, L.VOXEL)
+
+    def download_knn(self):
+        V = int(np.prod(self.w.dims))
+        return (self._d2h(self.lib.nrf_pipeline_knn(self.pipe), 4 * V, np.int32),
+                self._d2h(self.lib.nrf_pipeline_knn_w(self.pipe), 4 * V, np.float32))
+
+    def triangle_count(self) -> int:
+        cnt = C.c_void_p()
+        self.lib.nrf_pipeline_mesh(self.pipe, C.byref(cnt))
+        return int(self._d2h(cnt.value, 2, np.int32)[1])
+
+    def mesh(self):
+        cnt = C.c_void_p()
+        v = self.lib.nrf_pipeline_mesh(self.pipe, C.byref(cnt))
+        T = int(self._d2h(cnt.value, 2, np.int32)[1])
+        T = min(T, self.cfg.max_triangles)
+        return self._d2h(v, T * 9, np.float32).reshape(-1, 3)
+
+    def frame_corrs_host(self, f: int):
+        npx = self.w.width * self.w.height
+        corrs = self.corrs[f].cpu().numpy().view(L.CORR)
+        can = self.canonical[f].cpu().numpy().astype(np.float32)
+        assert len(corrs) == npx
+        return corrs, can
+
+    def close(self) -> None:
+        if self.pipe:
+            self.lib.nrf_pipeline_destroy(self.pipe)
+            self.pipe = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,302 @@
+/*
+ * nrf.h - C ABI of the MI355X (gfx950) non-rigid fusion step.
+ *
+ * Plain C: POD descriptors, raw device pointers, sizes and a HIP stream
+ * passed as `void*` (a hipStream_t; NULL = the legacy default stream).
+ * Every entry point returns an nrf_status; all device work is enqueued on
+ * the given stream and nothing synchronises unless the comment says so.
+ * No entry point allocates memory the caller can see, except the handle
+ * constructors (nrf_solver_create, nrf_pipeline_create), which own their
+ * scratch space.  Functions that need scratch take a caller workspace whose
+ * size is returned by the matching *_workspace_size query.
+ *
+ * The structs below reproduce the byte layout of the reference's POD types
+ * (include/types.h:26-206) so that buffers written by either side can be
+ * exchanged without repacking.
+ */
+#ifndef NRF_H
+#define NRF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+typedef enum nrf_status {
+  NRF_OK = 0,
+  NRF_ERR_INVALID_ARGUMENT = 1, /* null pointer, negative size, bad dims   */
+  NRF_ERR_TOO_LARGE = 2,        /* exceeds a documented capacity          */
+  NRF_ERR_HIP = 3,              /* a HIP runtime call failed              */
+  NRF_ERR_WORKSPACE = 4,        /* caller workspace smaller than required */
+  NRF_ERR_NOT_READY = 5,        /* pattern / graph not set                */
+  NRF_ERR_BARRIER_TIMEOUT = 6,  /* a persistent kernel's bounded wait ran out */
+  NRF_ERR_NO_KERNEL = 7         /* the HIP code object has no gfx950 kernels */
+} nrf_status;
+
+const char *nrf_status_string(int status);
+/* Last HIP error text seen by the library on this thread ("" if none). */
+const char *nrf_last_error(void);
+/* Build string: target, compile flags that affect numerics. */
+const char *nrf_build_info(void);
+
+/* ------------------------------------------------------------- POD types */
+enum { NRF_K_NEIGHBORS = 4, NRF_K_GRAPH = 8, NRF_BLOCK_DIM = 6, NRF_BLOCK_SIZE = 36 };
+
+/* = TSDFVoxel (include/types.h:168-176), 12 bytes. Initial state tsdf=1, w=0. */
+typedef struct nrf_voxel {
+  float tsdf;
+  float weight;
+  uint8_t color[3];
+  uint8_t pad;
+} nrf_voxel;
+
+/* = DeformNode (include/types.h:185-192), 96 bytes. */
+typedef struct nrf_node {
+  float pos[3];
+  float normal[3];
+  float radius;
+  int32_t neighbors[NRF_K_GRAPH];
+  float neighbor_w[NRF_K_GRAPH];
+  int32_t num_neighbors;
+} nrf_node;
+
+/* = DualQuat (include/types.h:86-94), 32 bytes, quaternions (x, y, z, w). */
+typedef struct nrf_dualquat {
+  float real[4];
+  float dual[4];
+} nrf_dualquat;
+
+/* = Correspondence (include/types.h:198-206), 76 bytes. */
+typedef struct nrf_corr {
+  float src[3];
+  float dst[3];
+  float normal[3];
+  int32_t node_ids[NRF_K_NEIGHBORS];
+  float node_ws[NRF_K_NEIGHBORS];
+  float weight;
+  uint8_t valid;
+  uint8_t pad[3];
+} nrf_corr;
+
+/* = CameraIntrinsics (include/types.h:148-162), 24 bytes. */
+typedef struct nrf_camera {
+  float fx, fy, cx, cy;
+  int32_t width, height;
+} nrf_camera;
+
+/* = TSDFVolume::Params (include/tsdf_volume.h:12-20). */
+typedef struct nrf_volume_params {
+  int32_t dims[3];
+  float voxel_size;
+  float truncation;
+  float origin[3];
+  float decay_alpha;
+  float max_weight;
+} nrf_volume_params;
+
+/* = GaussNewtonSolver::Params (include/solver.h:13-22) + build options. */
+typedef struct nrf_solver_params {
+  int32_t gn_iterations;  /* 3   */
+  int32_t pcg_iterations; /* 10  */
+  float pcg_tolerance;    /* 1e-4, absolute ||r||_2 unless relative_tolerance */
+  float lambda_smooth;    /* 1   */
+  float lambda_rot;       /* 1, unused (as in the reference) */
+  float lambda_damping;   /* 1e-2 */
+  int32_t debug;          /* 0   */
+  /* options with no reference counterpart (0 = reference behaviour) */
+  int32_t relative_tolerance; /* 1: stop when ||r|| < tol * ||b||          */
+  int32_t precond_fp64;       /* 1: 6x6 Cholesky inverses computed in FP64 */
+} nrf_solver_params;
+
+/* Caller-side GN loop settings (include/fusion_app.h:72-119 names). */
+typedef struct nrf_gn_params {
+  int32_t gn_iterations;
+  float update_scale;      /* multiplies delta_x before the clamps */
+  float max_update_rot;    /* per-node ||omega|| clamp, <=0 disables */
+  float max_update_trans;  /* per-node ||v|| clamp, <=0 disables */
+  int32_t validate;        /* 1: reject non-finite / too-large steps (fusion_app.h:409-491) */
+  float max_dx_mean;       /* mean-step limit used by the validation */
+} nrf_gn_params;
+
+/* PCG outcome of the last solve (no reference counterpart: the reference
+   discards the final residual, kernels/solver_kernels.cu:569). */
+typedef struct nrf_pcg_result {
+  int32_t iterations;      /* PCG iterations of the last GN iteration */
+  int32_t total_iterations;/* summed over GN iterations */
+  int32_t flags;           /* bit0 pAp breakdown, bit1 converged, bit2 step rejected, bit3 barrier timeout */
+  int32_t gn_done;         /* GN iterations whose step was applied */
+  double final_residual;   /* ||r||_2 of the PCG recurrence */
+  double rhs_norm;         /* ||b||_2 */
+} nrf_pcg_result;
+
+/* ----------------------------------------------------- TSDF integration */
+/* Replaces TSDFVolume::integrate (src/tsdf_volume.cu:23-63) and
+   tsdf_integrate_kernel (kernels/tsdf_kernels.cu:85-174).  camera_pose is
+   the 4x4 row-major camera-to-world pose; it is inverted on the host exactly
+   as the reference does.  Warping is on iff nodes != NULL and num_nodes > 0.
+   centered (optional, device, num_nodes entries) is the per-node table of
+   dq_centered(transforms[k], nodes[k].pos) from nrf_warp_centered_table; when
+   NULL the kernel recomputes it per voxel.  updated_count (optional, device
+   int) receives the number of voxels written (added to its current value). */
+int nrf_tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels,
+                       const float *depth, const nrf_camera *cam,
+                       const float camera_pose[16], const nrf_node *nodes,
+                       const nrf_dualquat *transforms, int num_nodes,
+                       const nrf_dualquat *centered, const int32_t *voxel_knn,
+                       const float *voxel_knn_w, const int32_t *voxel_opt_counts,
+                       int min_opt_count, int32_t *updated_count, void *stream);
+
+/* Fills voxels with the reference's initial state (tsdf=1, w=0, colour 128). */
+int nrf_tsdf_reset(const nrf_volume_params *vp, nrf_voxel *voxels, void *stream);
+
+/* -------------------------------------------------------- marching cubes */
+/* Replaces TSDFVolume::extract_surface / extract_surface_device
+   (src/marching_cubes.cu:455-541, kernels K2-K4 :349-451).  Output is the
+   reference's triangle soup (3 vertices per triangle, per-face normals,
+   triangles[t] = {3t, 3t+1, 3t+2}) in increasing cube-index order, so it is
+   deterministic.  counts (device, 2 ints) receives {vertices, triangles}
+   (the full count even if it exceeds max_triangles; writes are clamped).
+   normals may be NULL (extract_surface_device form). */
+size_t nrf_mc_workspace_size(const nrf_volume_params *vp);
+int nrf_mc_extract(const nrf_volume_params *vp, const nrf_voxel *voxels,
+                   void *workspace, size_t workspace_bytes, float *vertices,
+                   float *normals, int32_t *triangles, int max_triangles,
+                   int32_t *counts, void *stream);
+/* Optional: case index per cube (device, dims product bytes; 0 for cubes the
+   reference skips), from the workspace of the last nrf_mc_extract call. */
+const uint8_t *nrf_mc_cases(const void *workspace);
+
+/* ------------------------------------------------------------ warp field */
+/* Replaces WarpField::compute_voxel_knn (src/warp_field.cu:957-984) and
+   compute_voxel_knn_kernel (:299-405): brute-force 4-NN over all nodes for
+   voxels with weight > 0, ids -1 / weights 0 elsewhere. */
+int nrf_voxel_knn(const nrf_volume_params *vp, const nrf_voxel *voxels,
+                  const nrf_node *nodes, int num_nodes, int32_t *voxel_knn,
+                  float *voxel_knn_w, void *stream);
+
+/* Replaces WarpField::apply_twist_increment (src/warp_field.cu:1121-1141) and
+   apply_twist_kernel (:250-291): T_i <- dq(exp(clamp(s * dx_i))) * T_i.
+   centered (optional) is refreshed with dq_centered(T_i, nodes[i].pos). */
+int nrf_warp_apply_twist(const float *delta_x, nrf_dualquat *transforms,
+                         int num_nodes, float max_rot, float max_trans,
+                         float update_scale, const nrf_node *nodes,
+                         nrf_dualquat *centered, void *stream);
+
+/* Per-node table dq_centered(transforms[k], nodes[k].pos)
+   (include/se3_math.cuh:405-415), the operand of every DQ blend. */
+int nrf_warp_centered_table(const nrf_node *nodes, const nrf_dualquat *transforms,
+                            int num_nodes, nrf_dualquat *centered, void *stream);
+
+/* Warps points through their 4-NN: out[i] = warp_point(points[i], ...)
+   (include/se3_math.cuh:450-510). */
+int nrf_warp_points(const float *points, const int32_t *knn_ids,
+                    const float *knn_ws, int count, const nrf_node *nodes,
+                    const nrf_dualquat *transforms, int num_nodes, float *out,
+                    void *stream);
+
+/* Association input: packs one correspondence per point (src = canonical
+   point, 4-NN copied from the voxel containing it, weight 1 or weights[i])
+   as compute_pixel_knn_kernel (kernels/tsdf_kernels.cu:573-604) and
+   find_correspondences_kernel (:386-563) fill Correspondence.  A point is
+   valid iff valid[i] != 0, it lies in the volume and its voxel has nodes. */
+int nrf_corr_pack(const nrf_volume_params *vp, const int32_t *voxel_knn,
+                  const float *voxel_knn_w, const float *canonical,
+                  const float *dst, const float *normals, const float *weights,
+                  const uint8_t *valid, int count, nrf_corr *out, void *stream);
+
+/* ---------------------------------------------------------------- solver */
+typedef struct nrf_solver nrf_solver;
+
+/* GaussNewtonSolver(const Params&, int max_nodes) (kernels/solver_kernels.cu:486-496). */
+nrf_solver *nrf_solver_create(const nrf_solver_params *params, int max_nodes);
+void nrf_solver_destroy(nrf_solver *s);
+int nrf_solver_set_params(nrf_solver *s, const nrf_solver_params *params);
+
+/* build_sparsity_pattern (kernels/solver_kernels.cu:498-541) from HOST nodes.
+   Synchronous (uploads the pattern). */
+int nrf_solver_set_pattern(nrf_solver *s, const nrf_node *host_nodes, int num_nodes);
+
+/* GaussNewtonSolver::solve (kernels/solver_kernels.cu:543-570): one
+   linearisation with the stored corrs[i].src, delta_x (6N floats, device) is
+   zeroed and overwritten.  Rebuilds the pattern (D2H of nodes, sync) only when
+   num_nodes differs from the previous call.  If result != NULL the call
+   synchronises the stream and fills it. */
+int nrf_solver_solve(nrf_solver *s, const nrf_corr *corrs, int num_corrs,
+                     const nrf_node *nodes, const nrf_dualquat *transforms,
+                     int num_nodes, float *delta_x, nrf_pcg_result *result,
+                     void *stream);
+
+/* The caller-side GN loop of include/fusion_app.h:256-646 with the
+   association held fixed: for gn in [0, gn_iterations):
+     src_i = warp_point(canonical_i) through corrs[i].node_ids/node_ws,
+     assemble, precondition, PCG, validate, T <- exp(clamp(scale*dx)) * T.
+   canonical (device, 3*num_corrs floats) replaces the stored src; if NULL
+   the stored src is used and not re-warped.  transforms are updated in
+   place.  centered (device, num_nodes) must hold the table of the incoming
+   transforms and is kept current.  The pattern must have been set. */
+int nrf_solver_solve_gn(nrf_solver *s, const nrf_corr *corrs,
+                        const float *canonical, int num_corrs,
+                        const nrf_node *nodes, nrf_dualquat *transforms,
+                        nrf_dualquat *centered, int num_nodes,
+                        const nrf_gn_params *gn, float *delta_x,
+                        nrf_pcg_result *result, void *stream);
+
+/* system_matrix() / rhs() (include/solver.h:38-39): device pointers owned by
+   the solver, valid until the next solve or pattern change. */
+int nrf_solver_system(const nrf_solver *s, const int32_t **row_ptr,
+                      const int32_t **col_idx, const float **values,
+                      int *num_blocks, const float **rhs, const float **precond);
+/* Device pointer to the last nrf_pcg_result written by the solver (for
+   stream-ordered readers that do not want a host sync). */
+const nrf_pcg_result *nrf_solver_device_result(const nrf_solver *s);
+
+/* ------------------------------------------------------------- pipeline */
+/* The per-frame non-rigid step (include/fusion_app.h:253-669 with the
+   association as a caller input): GN solve, DQ-warped integration, marching
+   cubes.  The handle owns the volume, node table, kNN field and mesh. */
+typedef struct nrf_pipeline nrf_pipeline;
+
+typedef struct nrf_pipeline_config {
+  nrf_volume_params volume;
+  nrf_camera camera;
+  nrf_solver_params solver;
+  nrf_gn_params gn;
+  int32_t max_nodes;
+  int32_t max_corrs;
+  int32_t max_triangles;
+  int32_t min_opt_count;   /* integrate_min_optimized_count */
+  int32_t use_graph;       /* 1: replay the step as a captured hipGraph */
+} nrf_pipeline_config;
+
+nrf_pipeline *nrf_pipeline_create(const nrf_pipeline_config *cfg);
+void nrf_pipeline_destroy(nrf_pipeline *p);
+/* Device pointers owned by the pipeline. */
+nrf_voxel *nrf_pipeline_voxels(nrf_pipeline *p);
+nrf_dualquat *nrf_pipeline_transforms(nrf_pipeline *p);
+nrf_node *nrf_pipeline_nodes(nrf_pipeline *p);
+int32_t *nrf_pipeline_knn(nrf_pipeline *p);
+float *nrf_pipeline_knn_w(nrf_pipeline *p);
+float *nrf_pipeline_mesh(nrf_pipeline *p, int32_t **counts);
+nrf_solver *nrf_pipeline_solver(nrf_pipeline *p);
+/* Unwarped integration of a frame (frame 0 of fusion_app.h:233). */
+int nrf_pipeline_integrate_rigid(nrf_pipeline *p, const float *depth,
+                                 const float camera_pose[16], void *stream);
+/* Uploads host nodes (identity transforms), builds the pattern and the voxel
+   kNN snapshot (fusion_app.h:994-1003 minus node sampling).  Synchronous. */
+int nrf_pipeline_set_nodes(nrf_pipeline *p, const nrf_node *host_nodes, int num_nodes);
+/* One non-rigid frame: solve_gn on (corrs, canonical), warped integrate of
+   depth, marching cubes.  Inputs are device pointers. */
+int nrf_pipeline_step(nrf_pipeline *p, const float *depth,
+                      const float camera_pose[16], const nrf_corr *corrs,
+                      const float *canonical, int num_corrs, void *stream);
+/* Per-stage device times of the last step in ms (hipEvent based; valid after
+   the stream has been synchronised): [solve, integrate, mc]. */
+int nrf_pipeline_stage_ms(nrf_pipeline *p, float out_ms[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NRF_H */

@@ -1,0 +1,233 @@
+"""ctypes wrapper of the CPU oracle (oracle/oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+TABLE = os.path.join(HERE, "mc_cases.txt")
+
+P = C.c_void_p
+I = C.c_int
+L = C.c_long
+F = C.c_float
+D = C.c_double
+
+_SIGS = {
+    "orc_threads": (I, [I]),
+    "orc_load_mc_table": (I, [C.c_char_p]),
+    "orc_exp_se3_f64": (None, [P, P]),
+    "orc_exp_se3_f32": (None, [P, P]),
+    "orc_mat_to_dq_f64": (None, [P, P]),
+    "orc_dq_to_mat_f64": (None, [P, P]),
+    "orc_dq_mul_f64": (None, [P, P, P]),
+    "orc_dq_centered_f64": (None, [P, P, P]),
+    "orc_dq_transform_f64": (None, [P, P, P]),
+    "orc_twist_update_f64": (None, [P, P, D, D, D, P]),
+    "orc_twist_update_f32": (None, [P, P, F, F, F, P]),
+    "orc_warp_points_f64": (None, [P, P, P, I, P, P, I, P]),
+    "orc_warp_points_f32": (None, [P, P, P, I, P, P, I, P]),
+    "orc_integrate_f32": (L, [P, P, P, P, P, P, P, I, P, P, P, I]),
+    "orc_integrate_f64": (L, [P, P, P, P, P, P, P, P, I, P, P, P, I]),
+    "orc_mc_f32": (L, [P, P, P, P, P, L]),
+    "orc_mc_f64": (L, [P, P, P, P, P, P, L]),
+    "orc_knn_f32": (None, [P, P, P, I, P, P]),
+    "orc_knn_f64": (None, [P, P, P, I, P, P]),
+    "orc_pattern": (I, [P, I, P, P, I]),
+    "orc_assemble_f32": (I, [P, I, P, P, P, I, F, F, I, P, P, P]),
+    "orc_assemble_f64": (I, [P, I, P, P, P, I, D, D, P, P, P]),
+    "orc_pcg_f32": (D, [P, P, P, P, P, I, I, F, I, P, P, P, P]),
+    "orc_pcg_f64": (D, [P, P, P, P, P, I, I, D, I, P, P, P, P]),
+    "orc_spmv_f64": (None, [P, P, P, P, P, I]),
+    "orc_solve_gn_f32": (I, [P, P, I, P, I, P, P, P, P, P, P, P]),
+    "orc_solve_gn_f64": (I, [P, P, I, P, I, P, P, P, P, P, P, P]),
+}
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    lib = C.CDLL(LIB)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.orc_load_mc_table(TABLE.encode()) != 0:
+        raise RuntimeError("oracle: cannot read " + TABLE)
+    _lib = lib
+    return lib
+
+
+def ptr(a):
+    """Address of a numpy array or ctypes object.  The caller must keep the
+    object referenced until the C call returns (no temporaries)."""
+    if a is None:
+        return None
+    if isinstance(a, (C.Structure, C.Array)):
+        return C.addressof(a)
+    return a.ctypes.data
+
+
+def threads(n: int = 0) -> int:
+    return load().orc_threads(n)
+
+
+def identity_dq64(n: int) -> np.ndarray:
+    t = np.zeros((n, 8))
+    t[:, 3] = 1.0
+    return t
+
+
+def dq_records_to64(T) -> np.ndarray:
+    a = np.ascontiguousarray(T).view(np.float32).reshape(-1, 8)
+    return a.astype(np.float64)
+
+
+# ------------------------------------------------------------ components
+
+def integrate_f32(vp, vox, depth, cam, pose, nodes=None, T=None, knn=None, knn_w=None, opt=None, min_opt=0):
+    lib = load()
+    n = 0 if nodes is None else len(nodes)
+    return lib.orc_integrate_f32(C.byref(vp), ptr(vox), ptr(depth), C.byref(cam), ptr(pose), ptr(nodes), ptr(T), n,
+                                 ptr(knn), ptr(knn_w), ptr(opt), min_opt)
+
+
+def integrate_f64(vp, tsdf, weight, depth, cam, pose, nodes=None, T64=None, knn=None, knn_w=None, opt=None,
+                  min_opt=0):
+    lib = load()
+    n = 0 if nodes is None else len(nodes)
+    return lib.orc_integrate_f64(C.byref(vp), ptr(tsdf), ptr(weight), ptr(depth), C.byref(cam), ptr(pose),
+                                 ptr(nodes), ptr(T64), n, ptr(knn), ptr(knn_w), ptr(opt), min_opt)
+
+
+def mc_f32(vp, vox, normals=True):
+    lib = load()
+    V = int(np.prod(vp.dims[:]))
+    cases = np.zeros(V, np.uint8)
+    T = lib.orc_mc_f32(C.byref(vp), ptr(vox), ptr(cases), None, None, 0)
+    verts = np.zeros(max(T, 1) * 9, np.float32)
+    nrm = np.zeros(max(T, 1) * 9, np.float32) if normals else None
+    lib.orc_mc_f32(C.byref(vp), ptr(vox), None, ptr(verts), ptr(nrm), T)
+    return cases, verts[: T * 9].reshape(-1, 3), (nrm[: T * 9].reshape(-1, 3) if normals else None)
+
+
+def mc_f64(vp, tsdf, weight, normals=True):
+    lib = load()
+    V = int(np.prod(vp.dims[:]))
+    cases = np.zeros(V, np.uint8)
+    T = lib.orc_mc_f64(C.byref(vp), ptr(tsdf), ptr(weight), ptr(cases), None, None, 0)
+    verts = np.zeros(max(T, 1) * 9, np.float64)
+    nrm = np.zeros(max(T, 1) * 9, np.float64) if normals else None
+    lib.orc_mc_f64(C.byref(vp), ptr(tsdf), ptr(weight), None, ptr(verts), ptr(nrm), T)
+    return cases, verts[: T * 9].reshape(-1, 3), (nrm[: T * 9].reshape(-1, 3) if normals else None)
+
+
+def knn_f32(vp, vox, nodes):
+    V = int(np.prod(vp.dims[:]))
+    knn = np.zeros(V * 4, np.int32)
+    w = np.zeros(V * 4, np.float32)
+    load().orc_knn_f32(C.byref(vp), ptr(vox), ptr(nodes), len(nodes), ptr(knn), ptr(w))
+    return knn, w
+
+
+def pattern(nodes):
+    lib = load()
+    N = len(nodes)
+    rp = np.zeros(N + 1, np.int32)
+    nb = lib.orc_pattern(ptr(nodes), N, ptr(rp), None, 0)
+    ci = np.zeros(max(nb, 1), np.int32)
+    lib.orc_pattern(ptr(nodes), N, ptr(rp), ptr(ci), nb)
+    return rp, ci[:nb]
+
+
+def assemble_f32(corrs, nodes, T, lam_s, lam_d, canonical=None, precond_fp64=0):
+    N = len(nodes)
+    rp, ci = pattern(nodes)
+    nb = len(ci)
+    A = np.zeros(nb * 36, np.float32)
+    b = np.zeros(N * 6, np.float32)
+    M = np.zeros(N * 36, np.float32)
+    load().orc_assemble_f32(ptr(corrs), len(corrs), ptr(canonical), ptr(nodes), ptr(T), N, lam_s, lam_d,
+                            precond_fp64, ptr(A), ptr(b), ptr(M))
+    return rp, ci, A.reshape(nb, 6, 6), b, M.reshape(N, 6, 6)
+
+
+def assemble_f64(corrs, nodes, T64, lam_s, lam_d, canonical=None):
+    N = len(nodes)
+    rp, ci = pattern(nodes)
+    nb = len(ci)
+    A = np.zeros(nb * 36)
+    b = np.zeros(N * 6)
+    M = np.zeros(N * 36)
+    load().orc_assemble_f64(ptr(corrs), len(corrs), ptr(canonical), ptr(nodes), ptr(T64), N, lam_s, lam_d, ptr(A),
+                            ptr(b), ptr(M))
+    return rp, ci, A.reshape(nb, 6, 6), b, M.reshape(N, 6, 6)
+
+
+def pcg_f64(rp, ci, A, b, M, maxit, tol, rel=0, trace=False):
+    N = len(rp) - 1
+    x = np.zeros(6 * N)
+    it = C.c_int()
+    fl = C.c_int()
+    tr = np.zeros(max(maxit, 1)) if trace else None
+    A = np.ascontiguousarray(A, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    M = np.ascontiguousarray(M, np.float64)
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    r = load().orc_pcg_f64(ptr(rp), ptr(ci), ptr(A), ptr(b), ptr(M), N, maxit, tol, rel, ptr(x), C.byref(it),
+                           C.byref(fl), ptr(tr))
+    return x, it.value, fl.value, r, (tr[: it.value] if trace else None)
+
+
+def pcg_f32(rp, ci, A, b, M, maxit, tol, rel=0):
+    N = len(rp) - 1
+    x = np.zeros(6 * N, np.float32)
+    it = C.c_int()
+    fl = C.c_int()
+    A = np.ascontiguousarray(A, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    M = np.ascontiguousarray(M, np.float32)
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    r = load().orc_pcg_f32(ptr(rp), ptr(ci), ptr(A), ptr(b), ptr(M), N, maxit, tol, rel, ptr(x), C.byref(it),
+                           C.byref(fl), None)
+    return x, it.value, fl.value, r
+
+
+def solve_gn_f32(corrs, canonical, nodes, T, gn, sp):
+    T = np.array(T, copy=True)
+    N = len(nodes)
+    dx = np.zeros(6 * N, np.float32)
+    tot, done, last = C.c_int(), C.c_int(), C.c_int()
+    load().orc_solve_gn_f32(ptr(corrs), ptr(canonical), len(corrs), ptr(nodes), N, ptr(T), C.byref(gn),
+                            C.byref(sp), ptr(dx), C.byref(tot), C.byref(done), C.byref(last))
+    return T, dx, tot.value, done.value, last.value
+
+
+def solve_gn_f64(corrs, canonical, nodes, T64, gn, sp):
+    T64 = np.array(T64, dtype=np.float64, copy=True)
+    N = len(nodes)
+    dx = np.zeros(6 * N)
+    tot, done, last = C.c_int(), C.c_int(), C.c_int()
+    load().orc_solve_gn_f64(ptr(corrs), ptr(canonical), len(corrs), ptr(nodes), N, ptr(T64), C.byref(gn),
+                            C.byref(sp), ptr(dx), C.byref(tot), C.byref(done), C.byref(last))
+    return T64, dx, tot.value, done.value, last.value
