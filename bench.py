@@ -69,8 +69,12 @@ def main() -> int:
                           "n_gpus": args.gpus}))
         return 2
 
+    import ctypes as C
+
+    import numpy as np
     import torch
 
+    from cudafusion_amd import _lib as L
     from cudafusion_amd.harness import Harness
 
     t0 = time.perf_counter()
@@ -80,23 +84,59 @@ def main() -> int:
         f"{int(h.frames_host[1].valid.sum())} valid pixels in frame 1")
     nf = args.frames
     seq = [1 + (i % nf) for i in range(args.warmup + args.steps)]
+    lib = h.lib
+    if not args.graph:
+        L.check(lib.nrf_pipeline_profile(h.pipe, max(64, args.steps)), "profile")
     for f in seq[: args.warmup]:
         h.step(f)
     torch.cuda.synchronize()
-    stage = [0.0, 0.0, 0.0]
+    upd0 = lib.nrf_pipeline_updated_voxels(h.pipe)
     t0 = time.perf_counter()
     for f in seq[args.warmup:]:
         h.step(f)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    upd = (lib.nrf_pipeline_updated_voxels(h.pipe) - upd0) / args.steps
     ms = h.stage_ms()
     fps = args.steps / elapsed
+    res = h.solver_result()
     log(f"[bench] {args.steps} steps in {elapsed * 1e3:.1f} ms -> {fps:.1f} frames/s; last step stages "
-        f"solve {ms[0]:.3f} ms, integrate {ms[1]:.3f} ms, mc {ms[2]:.3f} ms; tris {h.triangle_count()}")
-    # integration roofline: knn (32 B/voxel) + voxel read/write of updated voxels + depth + node table
-    V = 256 ** 3
-    integ_bytes = V * 32 + 4 * 640 * 480 + 32 * h.N
-    achieved = integ_bytes / (ms[1] * 1e-3) / 1e9
+        f"solve {ms[0]:.3f} ms, integrate {ms[1]:.3f} ms, mc {ms[2]:.3f} ms; tris {h.triangle_count()}; "
+        f"last step GN applied {res.gn_done}, PCG iterations {res.total_iterations}")
+    kern = None
+    if not args.graph:
+        import numpy as np
+        rows = []
+        for k in range(args.steps):
+            out = (C.c_float * 6)()
+            L.check(lib.nrf_pipeline_profile_read(h.pipe, k, out), "profile_read")
+            rows.append(list(out))
+        kern = np.mean(np.array(rows), axis=0)
+        log("[bench] per-step kernel ms: membership %.4f assembly %.4f pcg %.4f update %.4f integrate %.4f "
+            "mc %.4f" % tuple(kern))
+    w = h.w
+    V = int(np.prod(w.dims))
+    G = w.gn_iterations
+    nb = h.num_blocks()
+    # algorithmic bytes per launch (counted once, minimum traffic of the algorithm)
+    algo = {
+        # kNN ids + weights (32 B/voxel), read + write of updated voxels, depth, centred node table
+        "integrate": V * 32 + upd * 24 + 4 * w.width * w.height + 32 * h.N,
+        # per iteration: BSR values + column ids, row pointers, p gather + Ap, block-Jacobi
+        # blocks, r/z/x/p vector traffic of the updates and the three dot products
+        "pcg": w.pcg_iterations * (nb * (36 * 4 + 4) + 4 * (h.N + 1) + 2 * 24 * h.N + h.N * 36 * 4
+                                   + 2 * 24 * h.N + 3 * 2 * 24 * h.N + 3 * 3 * 24 * h.N),
+        # one read of the 12-byte voxels + 36 B of vertices and 12 B of indices per triangle
+        "mc": V * 12 + h.triangle_count() * 48,
+    }
+    if kern is not None:
+        per_launch = {"integrate": kern[4], "pcg": kern[2] / G, "mc": kern[5]}
+        per_step = {"integrate": kern[4], "pcg": kern[2], "mc": kern[5]}
+    else:
+        per_launch = {"integrate": ms[1], "pcg": float("nan"), "mc": ms[2]}
+        per_step = dict(per_launch)
+    dom = max(per_step, key=lambda k: per_step[k])
+    achieved = algo[dom] / (per_launch[dom] * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -120,8 +160,14 @@ def main() -> int:
         "config": {"workload": "capsule-body 256^3 @4mm, 640x480, %d nodes, 5 GN x 30 PCG" % h.N,
                    "graph": bool(args.graph)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "integrate"},
-        "stages_ms": {"solve": ms[0], "integrate": ms[1], "mc": ms[2]},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                     "launch_ms": round(per_launch[dom], 5), "algorithmic_bytes": int(algo[dom])},
+        "kernel_ms_per_step": None if kern is None else dict(zip(
+            ["membership", "assembly", "pcg", "update", "integrate", "mc"], [round(float(v), 5) for v in kern])),
+        "roofline_other": {k: round(algo[k] / (per_launch[k] * 1e-3) / 1e9, 1) for k in algo
+                           if per_launch[k] == per_launch[k]},
+        "work": {"gn_applied_last_step": res.gn_done, "pcg_iterations_last_step": res.total_iterations,
+                 "updated_voxels_per_step": int(upd), "nodes": h.N, "bsr_blocks": nb},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -75,6 +75,13 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp,
           "-Wl,-rpath,/opt/rocm/lib", "-Wl,--no-undefined"])
     os.replace(tmp, LIB)
+    # C++ program that exercises the drop-in class API (run by a GPU test)
+    test_src = os.path.join(ROOT, "tests", "cpp", "test_dropin.cpp")
+    if os.path.exists(test_src):
+        exe = os.path.join(ROOT, "tests", "cpp", "test_dropin")
+        _run([hipcc, "-O2", "-std=c++17", "-I", INCLUDE, test_src, "-o", exe + ".tmp", "-L", PKG, "-lnrf",
+              "-Wl,-rpath,$ORIGIN/../../cudafusion_amd", "-Wl,-rpath,/opt/rocm/lib"])
+        os.replace(exe + ".tmp", exe)
     return LIB
 
 

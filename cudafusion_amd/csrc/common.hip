@@ -47,10 +47,9 @@ const DevInfo &info() {
 
 int device_cu_count() { return info().cus > 0 ? info().cus : 1; }
 int device_max_lds_per_block() {
-  // Dynamic LDS above 64 KiB needs an opt-in attribute per kernel; the
-  // persistent PCG kernel stays within 64 KiB per workgroup.
-  int l = info().lds;
-  return l < 65536 ? l : 65536;
+  // gfx950 has 160 KiB of LDS per CU; dynamic LDS above 64 KiB needs the
+  // per-kernel opt-in attribute (set by the callers that use it).
+  return info().lds > 0 ? info().lds : 65536;
 }
 bool device_cooperative() { return info().coop != 0; }
 

@@ -37,6 +37,13 @@ struct CamXf {
 CamXf invert_pose(const float pose[16]);
 
 int device_cu_count();
+
+// Optional event sink of nrf_solver_solve_gn (used by the pipeline's
+// per-kernel profiling): when set, events are recorded on the stream after
+// the membership build, then per GN iteration after the assembly, just
+// before the PCG kernel, after the PCG kernel and after the update, i.e.
+// 1 + 4 * gn_iterations events.
+void solver_set_event_sink(nrf_solver *s, hipEvent_t *events, int count);
 int device_max_lds_per_block();
 bool device_cooperative();
 

@@ -34,6 +34,10 @@ struct nrf_pipeline {
   // graph replay state
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
+  // per-kernel profiling ring (eager mode)
+  std::vector<hipEvent_t> prof;
+  int prof_ring = 0, prof_per = 0, prof_G = 0;
+  long prof_steps = 0;
   const void *g_key[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int g_n = -1;
   float g_pose[16];
@@ -46,6 +50,8 @@ static void pipeline_free(nrf_pipeline *p) {
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (auto &e : p->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto &e : p->prof)
     if (e) (void)hipEventDestroy(e);
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
@@ -83,6 +89,7 @@ extern "C" nrf_pipeline *nrf_pipeline_create(const nrf_pipeline_config *cfg) {
   if (ok) ok = hipMemset(p->knn, 0xff, V * 4 * sizeof(int32_t)) == hipSuccess;
   if (ok) ok = hipMemset(p->knn_w, 0, V * 4 * sizeof(float)) == hipSuccess;
   if (ok) ok = hipMemset(p->counts, 0, 4 * sizeof(int32_t)) == hipSuccess;
+  if (ok) ok = hipMemset(p->updated, 0, 4 * sizeof(int32_t)) == hipSuccess;
   if (ok) ok = hipDeviceSynchronize() == hipSuccess;
   if (!ok) {
     pipeline_free(p);
@@ -142,22 +149,35 @@ extern "C" int nrf_pipeline_set_nodes(nrf_pipeline *p, const nrf_node *host_node
 }
 
 static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float camera_pose[16],
-                            const nrf_corr *corrs, const float *canonical, int num_corrs, void *stream) {
+                            const nrf_corr *corrs, const float *canonical, int num_corrs, void *stream,
+                            bool profile) {
   hipStream_t st = as_stream(stream);
   int rc;
+  hipEvent_t *pe = nullptr;
+  if (profile) {
+    pe = &p->prof[(size_t)(p->prof_steps % p->prof_ring) * p->prof_per];
+    NRF_HIP_CHECK(hipEventRecord(pe[0], st));
+    solver_set_event_sink(p->solver, pe + 1, p->prof_per - 3);
+  }
   NRF_HIP_CHECK(hipEventRecord(p->ev[0], st));
   rc = nrf_solver_solve_gn(p->solver, corrs, canonical, num_corrs, p->nodes, p->T, p->centered, p->N, &p->cfg.gn,
                            p->delta_x, nullptr, stream);
+  if (profile) solver_set_event_sink(p->solver, nullptr, 0);
   if (rc) return rc;
   NRF_HIP_CHECK(hipEventRecord(p->ev[1], st));
   rc = nrf_tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, p->nodes, p->T, p->N,
-                          p->centered, p->knn, p->knn_w, nullptr, p->cfg.min_opt_count, nullptr, stream);
+                          p->centered, p->knn, p->knn_w, nullptr, p->cfg.min_opt_count, p->updated, stream);
   if (rc) return rc;
   NRF_HIP_CHECK(hipEventRecord(p->ev[2], st));
+  if (profile) NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 2], st));
   rc = nrf_mc_extract(&p->cfg.volume, p->vox, p->mc_ws, p->mc_bytes, p->verts, nullptr, p->tris,
                       p->cfg.max_triangles, p->counts, stream);
   if (rc) return rc;
   NRF_HIP_CHECK(hipEventRecord(p->ev[3], st));
+  if (profile) {
+    NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 1], st));
+    ++p->prof_steps;
+  }
   return NRF_OK;
 }
 
@@ -166,7 +186,8 @@ extern "C" int nrf_pipeline_step(nrf_pipeline *p, const float *depth, const floa
   if (!p || !depth || !camera_pose || num_corrs < 0 || num_corrs > p->cfg.max_corrs || (num_corrs > 0 && !corrs))
     return NRF_ERR_INVALID_ARGUMENT;
   if (p->N <= 0) return NRF_ERR_NOT_READY;
-  if (!p->cfg.use_graph) return pipeline_enqueue(p, depth, camera_pose, corrs, canonical, num_corrs, stream);
+  if (!p->cfg.use_graph || p->prof_ring > 0)
+    return pipeline_enqueue(p, depth, camera_pose, corrs, canonical, num_corrs, stream, p->prof_ring > 0);
   const void *key[5] = {depth, corrs, canonical, stream, nullptr};
   bool same = p->exec && p->g_n == num_corrs && memcmp(p->g_pose, camera_pose, sizeof(p->g_pose)) == 0;
   for (int k = 0; k < 4 && same; ++k) same = key[k] == p->g_key[k];
@@ -181,7 +202,7 @@ extern "C" int nrf_pipeline_step(nrf_pipeline *p, const float *depth, const floa
     }
     hipStream_t st = as_stream(stream);
     NRF_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = pipeline_enqueue(p, depth, camera_pose, corrs, canonical, num_corrs, stream);
+    int rc = pipeline_enqueue(p, depth, camera_pose, corrs, canonical, num_corrs, stream, false);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(st, &g);
     if (rc) {
@@ -206,4 +227,56 @@ extern "C" int nrf_pipeline_stage_ms(nrf_pipeline *p, float out_ms[3]) {
   if (!p || !out_ms) return NRF_ERR_INVALID_ARGUMENT;
   for (int k = 0; k < 3; ++k) NRF_HIP_CHECK(hipEventElapsedTime(&out_ms[k], p->ev[k], p->ev[k + 1]));
   return NRF_OK;
+}
+
+extern "C" int nrf_pipeline_profile(nrf_pipeline *p, int ring_steps) {
+  if (!p || ring_steps < 0) return NRF_ERR_INVALID_ARGUMENT;
+  for (auto &e : p->prof)
+    if (e) (void)hipEventDestroy(e);
+  p->prof.clear();
+  p->prof_ring = 0;
+  p->prof_steps = 0;
+  if (ring_steps == 0) return NRF_OK;
+  const int G = std::max(1, p->cfg.gn.gn_iterations);
+  p->prof_G = G;
+  p->prof_per = 1 + (1 + 4 * G) + 2;
+  p->prof.assign((size_t)ring_steps * p->prof_per, nullptr);
+  for (auto &e : p->prof) NRF_HIP_CHECK(hipEventCreate(&e));
+  p->prof_ring = ring_steps;
+  return NRF_OK;
+}
+
+extern "C" int nrf_pipeline_profile_read(nrf_pipeline *p, int step_back, float out_ms[6]) {
+  if (!p || !out_ms || p->prof_ring == 0 || step_back < 0 || step_back >= p->prof_ring ||
+      step_back >= p->prof_steps)
+    return NRF_ERR_INVALID_ARGUMENT;
+  const long step = p->prof_steps - 1 - step_back;
+  hipEvent_t *e = &p->prof[(size_t)(step % p->prof_ring) * p->prof_per];
+  auto ms = [&](int a, int b, float &acc) -> int {
+    float v = 0;
+    NRF_HIP_CHECK(hipEventElapsedTime(&v, e[a], e[b]));
+    acc += v;
+    return NRF_OK;
+  };
+  for (int k = 0; k < 6; ++k) out_ms[k] = 0;
+  int rc;
+  if ((rc = ms(0, 1, out_ms[0]))) return rc;  // membership build
+  int prev = 1;
+  for (int g = 0; g < p->prof_G; ++g) {
+    const int a = 2 + 4 * g;  // after assembly, before PCG, after PCG, after update
+    if ((rc = ms(prev, a, out_ms[1]))) return rc;
+    if ((rc = ms(a + 1, a + 2, out_ms[2]))) return rc;
+    if ((rc = ms(a + 2, a + 3, out_ms[3]))) return rc;
+    prev = a + 3;
+  }
+  if ((rc = ms(prev, p->prof_per - 2, out_ms[4]))) return rc;
+  if ((rc = ms(p->prof_per - 2, p->prof_per - 1, out_ms[5]))) return rc;
+  return NRF_OK;
+}
+
+extern "C" long long nrf_pipeline_updated_voxels(nrf_pipeline *p) {
+  if (!p) return -1;
+  int32_t v = 0;
+  if (hipMemcpy(&v, p->updated, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  return v;
 }

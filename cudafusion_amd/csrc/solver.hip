@@ -386,43 +386,24 @@ __global__ __launch_bounds__(64) void assemble_rows_kernel(AsmArgs a) {
 }
 
 // ----------------------------------------------------------------- PCG
-struct Barrier {
-  unsigned count;
-  unsigned gen;
+// One 64-byte record per workgroup: its partial sums of the current and the
+// previous synchronisation point and the generation it has reached.  A
+// synchronisation point is an all-reduce: every workgroup publishes its
+// partials and generation, then lanes 0..G-1 each wait for one record and
+// the partials are summed in a fixed tree, so all workgroups hold the same
+// bits.  Values are double-buffered by the parity of the point: a workgroup
+// can be at most one point ahead of the slowest one.
+struct alignas(64) SyncRec {
+  double v[2][2];
+  unsigned long long gen;
+  unsigned long long pad[3];
+};
+struct PcgCtl {
+  unsigned long long epoch;  // advanced once per launch by pcg_prep_kernel
   unsigned abort;
   unsigned pad;
 };
-
-__device__ inline bool grid_sync(Barrier *bar, unsigned G, unsigned &gen) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __syncthreads();
-  __shared__ unsigned s_abort;
-  if (threadIdx.x == 0) {
-    unsigned my = gen;
-    unsigned arrived = __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned ab = 0;
-    if (arrived == G - 1) {
-      __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&bar->gen, my + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      unsigned spins = 0;
-      while (__hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == my) {
-        if (__hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-        if (++spins > kBarrierSpinLimit) {
-          __hip_atomic_store(&bar->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
-      }
-    }
-    ab = __hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    gen = my + 1;
-    s_abort = ab;
-  }
-  __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return s_abort != 0;
-}
+constexpr int kPcgMaxNodesPerWg = 42;  // 6 x 42 = 252 scalar rows <= 256 lanes
 
 __device__ inline double block_sum(double v, double *red) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -435,20 +416,14 @@ __device__ inline double block_sum(double v, double *red) {
   return s;
 }
 
-// Sum of G partials, identical in every workgroup (fixed order).
-__device__ inline double grid_sum(const double *part, int G, double *red) {
-  double v = 0;
-  for (int g = threadIdx.x; g < G; g += kPcgThreads) v += __hip_atomic_load(&part[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return block_sum(v, red);
-}
-
 struct PcgArgs {
-  const int32_t *row_ptr, *col_idx;
+  const int32_t *row_ptr, *lcol;
   const float *values, *b, *Minv;
-  float *x, *r, *z, *P0, *P1, *Ap;
-  double *part;  // [4][G]
-  Barrier *bar;
-  const int32_t *rows;  // [G+1] node partition
+  float *x;
+  float *zpub, *ppub0, *ppub1;
+  SyncRec *recs;
+  PcgCtl *ctl;
+  const int32_t *rows, *halo_off, *halo;
   int N, maxit, G, lds_a;
   float tol;
   int rel;
@@ -456,127 +431,166 @@ struct PcgArgs {
   const int32_t *stop;
 };
 
+// All-reduce of two doubles across the G workgroups; also the point after
+// which every vector element published before it is visible.  Returns true
+// when the bounded wait ran out (then every workgroup leaves the solve).
+__device__ inline bool allreduce2(const PcgArgs &a, unsigned long long target, int slot, double &v0, double &v1,
+                                  double *red0, double *red1, int *s_abort) {
+  v0 = block_sum(v0, red0);
+  v1 = block_sum(v1, red1);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    SyncRec *me = a.recs + blockIdx.x;
+    me->v[slot][0] = v0;
+    me->v[slot][1] = v1;
+    __hip_atomic_store(&me->gen, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (threadIdx.x < a.G) {
+    const SyncRec *o = a.recs + threadIdx.x;
+    unsigned spins = 0;
+    while (__hip_atomic_load(&o->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(&a.ctl->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      if (++spins > kBarrierSpinLimit) {
+        __hip_atomic_store(&a.ctl->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (threadIdx.x == 0) *s_abort = __hip_atomic_load(&a.ctl->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  double g0 = 0, g1 = 0;
+  if (threadIdx.x < a.G) {
+    g0 = a.recs[threadIdx.x].v[slot][0];
+    g1 = a.recs[threadIdx.x].v[slot][1];
+  }
+  v0 = block_sum(g0, red0);  // block_sum's barriers also publish *s_abort
+  v1 = block_sum(g1, red1);
+  return *s_abort != 0;
+}
+
+// Block-Jacobi PCG (kernels/solver_kernels.cu:616-687, Saad Alg. 9.1, x0 = 0)
+// as one persistent kernel.  Workgroup g owns nodes [rows[g], rows[g+1]),
+// at most 42, so lane t holds scalar row 6 rows[g] + t and keeps x, r, z, p
+// and Ap of that row in registers for the whole solve.  The blocks of the
+// owned rows (and their columns, renumbered into the workgroup's halo
+// list) are copied to LDS once.  Per iteration:
+//   A: gather p_j = z_j + beta p_j of the halo nodes from the published
+//      vectors into LDS, Ap from LDS, all-reduce p.Ap;
+//   B: x += alpha p, r -= alpha Ap, z = M^-1 r, publish z, all-reduce
+//      (r.r, r.z).
+// Every workgroup evaluates the same expressions on the same published
+// bits, so the owner's p and its readers' copies are identical.
 __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
   extern __shared__ float lds[];
-  __shared__ double red[kPcgThreads / 64];
+  __shared__ double red0[kPcgThreads / 64], red1[kPcgThreads / 64];
+  __shared__ int s_abort;
+  __shared__ unsigned long long s_base;
   if (a.stop && *a.stop) return;
-  const int g = blockIdx.x, G = a.G;
+  const int g = blockIdx.x, t = threadIdx.x;
   const int n0 = a.rows[g], n1 = a.rows[g + 1];
-  const int s0 = 6 * n0, s1 = 6 * n1;  // owned scalar rows
-  const int blk0 = a.row_ptr[n0], blk1 = a.row_ptr[n1];
+  const int S = 6 * (n1 - n0);
+  const int blk0 = a.row_ptr[n0], nblk = a.row_ptr[n1] - blk0;
+  const int h0 = a.halo_off[g], nh = a.halo_off[g + 1] - h0;
   float *Al = lds;
-  int32_t *Cl = reinterpret_cast<int32_t *>(lds + (a.lds_a ? (size_t)(blk1 - blk0) * 36 : 0));
+  int32_t *Ll = reinterpret_cast<int32_t *>(Al + (a.lds_a ? (size_t)nblk * 36 : 0));
+  float *ph = reinterpret_cast<float *>(Ll + (a.lds_a ? nblk : 0));
+  float *rl = ph + (size_t)nh * 6;
   if (a.lds_a) {
     const float4 *src = reinterpret_cast<const float4 *>(a.values + (size_t)blk0 * 36);
     float4 *dst = reinterpret_cast<float4 *>(Al);
-    for (int j = threadIdx.x; j < (blk1 - blk0) * 9; j += kPcgThreads) dst[j] = src[j];
-    for (int j = threadIdx.x; j < blk1 - blk0; j += kPcgThreads) Cl[j] = a.col_idx[blk0 + j];
+    for (int j = t; j < nblk * 9; j += kPcgThreads) dst[j] = src[j];
+    for (int j = t; j < nblk; j += kPcgThreads) Ll[j] = a.lcol[blk0 + j];
   }
-  double *p_rz = a.part, *p_pap = a.part + G, *p_rr = a.part + 2 * G, *p_bb = a.part + 3 * G;
-  unsigned gen = 0;
-  if (threadIdx.x == 0) gen = __hip_atomic_load(&a.bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-
-  // init: x = 0, r = b, z = M^-1 r  (kernels/solver_kernels.cu:620-632)
-  double lrz = 0, lbb = 0;
-  for (int s = s0 + threadIdx.x; s < s1; s += kPcgThreads) {
-    const int i = s / 6, row = s % 6;
-    a.x[s] = 0.f;
-    const float bv = a.b[s];
-    a.r[s] = bv;
-    const float *Mi = a.Minv + (size_t)i * 36;
-    float acc = 0;
-    for (int c = 0; c < 6; ++c) acc += Mi[row * 6 + c] * a.b[i * 6 + c];
-    a.z[s] = acc;
-    lrz += (double)bv * (double)acc;
-    lbb += (double)bv * (double)bv;
+  if (t == 0) {
+    s_base = __hip_atomic_load(&a.ctl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) << 16;
+    s_abort = 0;
   }
-  lrz = block_sum(lrz, red);
-  lbb = block_sum(lbb, red);
-  if (threadIdx.x == 0) {
-    p_rz[g] = lrz;
-    p_bb[g] = lbb;
+  const bool own = t < S;
+  const int li = t / 6, row = t % 6, i = n0 + li, s = 6 * n0 + t;
+  float M[6] = {0, 0, 0, 0, 0, 0};
+  float bv = 0.f;
+  int rb = 0, re = 0;
+  if (own) {
+    for (int c = 0; c < 6; ++c) M[c] = a.Minv[(size_t)i * 36 + row * 6 + c];
+    bv = a.b[s];
+    rb = a.row_ptr[i];
+    re = a.row_ptr[i + 1];
   }
-  bool aborted = grid_sync(a.bar, G, gen);
-  double rz = grid_sum(p_rz, G, red);
-  const double bb = grid_sum(p_bb, G, red);
-  double rr = bb;  // r = b
+  // init: x = 0, r = b, z = M^-1 r, p = z (kernels/solver_kernels.cu:620-632)
+  float x = 0.f, r = bv, z = 0.f, p = 0.f;
+  rl[t] = r;
+  __syncthreads();
+  const unsigned long long base = s_base;
+  if (own) {
+    for (int c = 0; c < 6; ++c) z += M[c] * rl[li * 6 + c];
+    a.zpub[s] = z;
+  }
+  double rz = own ? (double)r * (double)z : 0.0, bb = own ? (double)bv * (double)bv : 0.0;
+  unsigned long long pt = 1;
+  bool aborted = allreduce2(a, base + pt, (int)(pt & 1), rz, bb, red0, red1, &s_abort);
+  double rr = bb;
   const double tol = a.rel ? (double)a.tol * sqrt(bb) : (double)a.tol;
   int iters = 0, flags = 0;
   float betaf = 0.f;
   for (int it = 0; it < a.maxit && !aborted; ++it) {
-    float *Pold = (it & 1) ? a.P1 : a.P0;
-    float *Pnew = (it & 1) ? a.P0 : a.P1;
-    // Phase A: p = z + beta p (on the fly), Ap = A p, partial p.Ap
-    double lpap = 0;
-    for (int s = s0 + threadIdx.x; s < s1; s += kPcgThreads) {
-      const int i = s / 6, row = s % 6;
-      const int rbeg = a.row_ptr[i], rend = a.row_ptr[i + 1];
-      float y = 0.f;
-      for (int bk = rbeg; bk < rend; ++bk) {
-        const int j = a.lds_a ? Cl[bk - blk0] : a.col_idx[bk];
-        const float *A = a.lds_a ? (Al + (size_t)(bk - blk0) * 36) : (a.values + (size_t)bk * 36);
-        const float *zj = a.z + j * 6;
-        if (it == 0) {
-          for (int c = 0; c < 6; ++c) y += A[row * 6 + c] * zj[c];
-        } else {
-          const float *pj = Pold + j * 6;
-          for (int c = 0; c < 6; ++c) y += A[row * 6 + c] * (zj[c] + betaf * pj[c]);
-        }
-      }
-      const float pv = (it == 0) ? a.z[s] : (a.z[s] + betaf * Pold[s]);
-      Pnew[s] = pv;
-      a.Ap[s] = y;
-      lpap += (double)pv * (double)y;
+    const float *Pold = (it & 1) ? a.ppub1 : a.ppub0;
+    float *Pnew = (it & 1) ? a.ppub0 : a.ppub1;
+    // A: halo p into LDS, Ap, p.Ap
+    for (int e = t; e < nh * 6; e += kPcgThreads) {
+      const int idx = 6 * a.halo[h0 + e / 6] + e % 6;
+      const float zv = a.zpub[idx];
+      ph[e] = (it == 0) ? zv : zv + betaf * Pold[idx];
     }
-    lpap = block_sum(lpap, red);
-    if (threadIdx.x == 0) p_pap[g] = lpap;
-    aborted = grid_sync(a.bar, G, gen);
+    p = (it == 0) ? z : z + betaf * p;
+    if (own) Pnew[s] = p;
+    __syncthreads();
+    float y = 0.f;
+    if (own) {
+      for (int bk = rb; bk < re; ++bk) {
+        const int lc = a.lds_a ? Ll[bk - blk0] : a.lcol[bk];
+        const float *A = a.lds_a ? (Al + (size_t)(bk - blk0) * 36) : (a.values + (size_t)bk * 36);
+        const float *pj = ph + lc * 6;
+        for (int c = 0; c < 6; ++c) y += A[row * 6 + c] * pj[c];
+      }
+    }
+    double pAp = own ? (double)p * (double)y : 0.0, unused = 0.0;
+    ++pt;
+    aborted = allreduce2(a, base + pt, (int)(pt & 1), pAp, unused, red0, red1, &s_abort);
     if (aborted) break;
-    const double pAp = grid_sum(p_pap, G, red);
     if (fabs(pAp) < 1e-12) {
       flags |= 1;
       break;
     }
-    const double alpha = rz / pAp;
-    const float alf = (float)alpha;
-    // Phase B: x += alpha p, r -= alpha Ap, z = M^-1 r, partial r.r and r.z
-    for (int s = s0 + threadIdx.x; s < s1; s += kPcgThreads) {
-      a.x[s] += alf * Pnew[s];
-      a.r[s] += -alf * a.Ap[s];
-    }
+    const float alf = (float)(rz / pAp);
+    // B: x += alpha p, r -= alpha Ap, z = M^-1 r, (r.r, r.z)
+    x += alf * p;
+    r += -alf * y;
+    rl[t] = r;
     __syncthreads();
-    double lrr = 0, lrz2 = 0;
-    for (int s = s0 + threadIdx.x; s < s1; s += kPcgThreads) {
-      const int i = s / 6, row = s % 6;
-      const float *Mi = a.Minv + (size_t)i * 36;
-      const float *ri = a.r + i * 6;
-      float acc = 0;
-      for (int c = 0; c < 6; ++c) acc += Mi[row * 6 + c] * ri[c];
-      a.z[s] = acc;
-      const float rv = a.r[s];
-      lrr += (double)rv * (double)rv;
-      lrz2 += (double)rv * (double)acc;
+    z = 0.f;
+    if (own) {
+      for (int c = 0; c < 6; ++c) z += M[c] * rl[li * 6 + c];
+      a.zpub[s] = z;
     }
-    lrr = block_sum(lrr, red);
-    lrz2 = block_sum(lrz2, red);
-    if (threadIdx.x == 0) {
-      p_rr[g] = lrr;
-      p_rz[g] = lrz2;
-    }
-    aborted = grid_sync(a.bar, G, gen);
+    double rr_l = own ? (double)r * (double)r : 0.0, rz_l = own ? (double)r * (double)z : 0.0;
+    ++pt;
+    aborted = allreduce2(a, base + pt, (int)(pt & 1), rr_l, rz_l, red0, red1, &s_abort);
     if (aborted) break;
-    rr = grid_sum(p_rr, G, red);
-    const double rz_new = grid_sum(p_rz, G, red);
+    rr = rr_l;
     iters = it + 1;
     if (sqrt(rr) < tol) {
       flags |= 2;
       break;
     }
-    betaf = (float)(rz_new / rz);
-    rz = rz_new;
+    betaf = (float)(rz_l / rz);
+    rz = rz_l;
   }
   if (aborted) flags |= 8;
-  if (g == 0 && threadIdx.x == 0) {
+  if (own) a.x[s] = x;
+  if (g == 0 && t == 0) {
     a.res->iterations = iters;
     a.res->total_iterations += iters;
     a.res->flags |= flags;
@@ -585,10 +599,10 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
   }
 }
 
-__global__ void pcg_prep_kernel(Barrier *bar) {
+__global__ void pcg_prep_kernel(PcgCtl *ctl) {
   if (threadIdx.x == 0) {
-    bar->count = 0;
-    bar->abort = 0;
+    ctl->epoch += 1;
+    ctl->abort = 0;
   }
 }
 
@@ -825,15 +839,17 @@ struct nrf_solver {
   // system
   float *values = nullptr, *b = nullptr, *Minv = nullptr;
   size_t c_values = 0, c_b = 0, c_Minv = 0;
-  // pcg
-  float *r = nullptr, *z = nullptr, *P0 = nullptr, *P1 = nullptr, *Ap = nullptr;
-  size_t c_r = 0, c_z = 0, c_P0 = 0, c_P1 = 0, c_Ap = 0;
-  double *part = nullptr;
-  size_t c_part = 0;
-  Barrier *bar = nullptr;
-  size_t c_bar = 0;
+  // pcg: published z / p vectors, per-workgroup sync records, halo lists
+  float *zpub = nullptr, *P0 = nullptr, *P1 = nullptr;
+  size_t c_zpub = 0, c_P0 = 0, c_P1 = 0;
+  SyncRec *recs = nullptr;
+  size_t c_recs = 0;
+  PcgCtl *ctl = nullptr;
+  size_t c_ctl = 0;
+  int32_t *lcol = nullptr, *halo_off = nullptr, *halo = nullptr;
+  size_t c_lcol = 0, c_halo_off = 0, c_halo = 0;
   int G = 1, lds_a = 0;
-  size_t pcg_lds = 0;
+  size_t pcg_lds = 0, lds_attr = 65536;
   // memberships
   int32_t *m_cnt = nullptr, *m_off = nullptr, *m_cur = nullptr, *m_keys = nullptr, *m_keys2 = nullptr;
   size_t c_m_cnt = 0, c_m_off = 0, c_m_cur = 0, c_m_keys = 0, c_m_keys2 = 0;
@@ -844,9 +860,22 @@ struct nrf_solver {
   size_t c_res = 0;
   int cus = 0, max_lds = 0, coop = 0;
   int forced_G = 0;
+  // optional profiling events (see solver_set_event_sink)
+  hipEvent_t *sink = nullptr;
+  int sink_n = 0, sink_i = 0;
 };
 
 namespace nrf {
+
+void solver_set_event_sink(nrf_solver *s, hipEvent_t *events, int count) {
+  s->sink = events;
+  s->sink_n = events ? count : 0;
+  s->sink_i = 0;
+}
+
+static void sink_record(nrf_solver *s, hipStream_t st) {
+  if (s->sink && s->sink_i < s->sink_n) (void)hipEventRecord(s->sink[s->sink_i++], st);
+}
 
 static int solver_set_pattern_host(nrf_solver *s, const nrf_node *h, int N) {
   build_pattern(h, N, s->P);
@@ -862,29 +891,50 @@ static int solver_set_pattern_host(nrf_solver *s, const nrf_node *h, int N) {
   if ((st = upload(s->es_entry, s->c_es_entry, P.es_entry))) return st;
   if ((st = upload(s->es_offrel, s->c_es_offrel, P.es_offrel))) return st;
   if ((st = dev_alloc(s->values, (size_t)std::max(P.nb, 1) * 36, s->c_values))) return st;
-  // PCG partition: contiguous node ranges with about nb / G blocks each.
+  // PCG partition: contiguous ranges of at most 42 nodes (one scalar row
+  // per lane), each with its halo (the distinct columns of its rows) and the
+  // blocks' columns renumbered into that halo.
   int G = s->forced_G;
-  if (G <= 0) G = std::max(1, std::min(s->cus, (N + 31) / 32));
-  G = std::max(1, std::min(G, std::max(1, N)));
-  std::vector<int32_t> rows(G + 1, N);
-  rows[0] = 0;
-  {
-    int g = 1;
-    for (int i = 0; i < N && g < G; ++i) {
-      const long target = (long)P.nb * g / G;
-      if (P.row_ptr[i + 1] >= target) rows[g++] = i + 1;
-    }
-    for (; g < G; ++g) rows[g] = N;
+  const int min_G = std::max(1, (N + kPcgMaxNodesPerWg - 1) / kPcgMaxNodesPerWg);
+  if (G < min_G) G = std::max(min_G, std::min(s->cus, (N + 39) / 40));
+  if (G > kPcgThreads) return NRF_ERR_TOO_LARGE;  // lanes 0..G-1 wait on the records
+  std::vector<int32_t> rows(G + 1);
+  for (int g = 0; g <= G; ++g) rows[g] = (int32_t)((long)N * g / G);
+  std::vector<int32_t> lcol(P.nb), hoff(G + 1, 0), halo;
+  size_t need_max = 0;
+  for (int g = 0; g < G; ++g) {
+    const int b0 = P.row_ptr[rows[g]], b1 = P.row_ptr[rows[g + 1]];
+    std::vector<int32_t> h(P.col_idx.begin() + b0, P.col_idx.begin() + b1);
+    std::sort(h.begin(), h.end());
+    h.erase(std::unique(h.begin(), h.end()), h.end());
+    for (int q = b0; q < b1; ++q)
+      lcol[q] = (int32_t)(std::lower_bound(h.begin(), h.end(), P.col_idx[q]) - h.begin());
+    hoff[g] = (int32_t)halo.size();
+    halo.insert(halo.end(), h.begin(), h.end());
+    const size_t nblk = (size_t)(b1 - b0);
+    need_max = std::max(need_max, nblk * (36 * 4 + 4) + h.size() * 24 + kPcgThreads * 4);
   }
-  size_t maxblk = 0;
-  for (int g = 0; g < G; ++g) maxblk = std::max(maxblk, (size_t)(P.row_ptr[rows[g + 1]] - P.row_ptr[rows[g]]));
+  hoff[G] = (int32_t)halo.size();
+  size_t halo_max = 0;
+  for (int g = 0; g < G; ++g) halo_max = std::max(halo_max, (size_t)(hoff[g + 1] - hoff[g]));
   s->G = G;
-  const size_t need = maxblk * 36 * sizeof(float) + maxblk * sizeof(int32_t);
-  const size_t budget = (size_t)std::max(0, s->max_lds - 2048);
-  s->lds_a = need <= budget ? 1 : 0;
-  s->pcg_lds = s->lds_a ? align_up(need, 16) : 16;
+  const size_t budget = (size_t)std::max(0, s->max_lds - 1024);
+  s->lds_a = need_max <= budget ? 1 : 0;
+  s->pcg_lds = align_up(s->lds_a ? need_max : halo_max * 24 + kPcgThreads * 4, 16);
   if ((st = upload(s->rows, s->c_rows, rows))) return st;
-  if ((st = dev_alloc(s->part, (size_t)4 * G, s->c_part))) return st;
+  if ((st = upload(s->lcol, s->c_lcol, lcol))) return st;
+  if ((st = upload(s->halo_off, s->c_halo_off, hoff))) return st;
+  if ((st = upload(s->halo, s->c_halo, halo))) return st;
+  if (s->pcg_lds > 65536 && s->pcg_lds > s->lds_attr) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(pcg_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)s->pcg_lds) == hipSuccess) {
+      s->lds_attr = s->pcg_lds;
+    } else {
+      (void)hipGetLastError();
+      s->lds_a = 0;  // keep the matrix in global memory
+      s->pcg_lds = align_up(halo_max * 24 + kPcgThreads * 4, 16);
+    }
+  }
   return NRF_OK;
 }
 
@@ -948,23 +998,24 @@ static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, const float *ca
 }
 
 static int solver_pcg(nrf_solver *s, float *x, int N, hipStream_t st) {
-  pcg_prep_kernel<<<1, 64, 0, st>>>(s->bar);
+  pcg_prep_kernel<<<1, 64, 0, st>>>(s->ctl);
   NRF_LAUNCH_CHECK();
+  sink_record(s, st);
   PcgArgs a;
   a.row_ptr = s->row_ptr;
-  a.col_idx = s->col_idx;
+  a.lcol = s->lcol;
   a.values = s->values;
   a.b = s->b;
   a.Minv = s->Minv;
   a.x = x;
-  a.r = s->r;
-  a.z = s->z;
-  a.P0 = s->P0;
-  a.P1 = s->P1;
-  a.Ap = s->Ap;
-  a.part = s->part;
-  a.bar = s->bar;
+  a.zpub = s->zpub;
+  a.ppub0 = s->P0;
+  a.ppub1 = s->P1;
+  a.recs = s->recs;
+  a.ctl = s->ctl;
   a.rows = s->rows;
+  a.halo_off = s->halo_off;
+  a.halo = s->halo;
   a.N = N;
   a.maxit = s->params.pcg_iterations;
   a.G = s->G;
@@ -1016,11 +1067,12 @@ extern "C" nrf_solver *nrf_solver_create(const nrf_solver_params *params, int ma
   if (fg) s->forced_G = atoi(fg);
   const size_t n6 = (size_t)max_nodes * 6;
   bool ok = dev_alloc(s->b, n6, s->c_b) == 0 && dev_alloc(s->Minv, (size_t)max_nodes * 36, s->c_Minv) == 0 &&
-            dev_alloc(s->r, n6, s->c_r) == 0 && dev_alloc(s->z, n6, s->c_z) == 0 &&
-            dev_alloc(s->P0, n6, s->c_P0) == 0 && dev_alloc(s->P1, n6, s->c_P1) == 0 &&
-            dev_alloc(s->Ap, n6, s->c_Ap) == 0 && dev_alloc(s->bar, 1, s->c_bar) == 0 &&
-            dev_alloc(s->stop, 1, s->c_stop) == 0 && dev_alloc(s->d_res, 1, s->c_res) == 0;
-  if (ok) ok = hipMemset(s->bar, 0, sizeof(Barrier)) == hipSuccess &&
+            dev_alloc(s->zpub, n6, s->c_zpub) == 0 && dev_alloc(s->P0, n6, s->c_P0) == 0 &&
+            dev_alloc(s->P1, n6, s->c_P1) == 0 && dev_alloc(s->recs, (size_t)kPcgThreads, s->c_recs) == 0 &&
+            dev_alloc(s->ctl, 1, s->c_ctl) == 0 && dev_alloc(s->stop, 1, s->c_stop) == 0 &&
+            dev_alloc(s->d_res, 1, s->c_res) == 0;
+  if (ok) ok = hipMemset(s->recs, 0, sizeof(SyncRec) * kPcgThreads) == hipSuccess &&
+               hipMemset(s->ctl, 0, sizeof(PcgCtl)) == hipSuccess &&
                hipMemset(s->stop, 0, sizeof(int32_t)) == hipSuccess &&
                hipMemset(s->d_res, 0, sizeof(nrf_pcg_result)) == hipSuccess;
   if (!ok) {
@@ -1033,7 +1085,8 @@ extern "C" nrf_solver *nrf_solver_create(const nrf_solver_params *params, int ma
 extern "C" void nrf_solver_destroy(nrf_solver *s) {
   if (!s) return;
   void *ptrs[] = {s->row_ptr, s->col_idx, s->diag_rel, s->e_i, s->e_j, s->es_off, s->es_entry, s->es_offrel,
-                  s->rows, s->e_w, s->values, s->b, s->Minv, s->r, s->z, s->P0, s->P1, s->Ap, s->part, s->bar,
+                  s->rows, s->e_w, s->values, s->b, s->Minv, s->zpub, s->P0, s->P1, s->recs, s->ctl, s->lcol,
+                  s->halo_off, s->halo,
                   s->m_cnt, s->m_off, s->m_cur, s->m_keys, s->m_keys2, s->stop, s->d_res};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -1084,6 +1137,7 @@ extern "C" int nrf_solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const f
   if ((rc = solver_build_memberships(s, corrs, num_corrs, num_nodes, st))) return rc;
   gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res);
   NRF_LAUNCH_CHECK();
+  sink_record(s, st);
   GnArgs g;
   g.dx = delta_x;
   g.T = reinterpret_cast<DQ *>(transforms);
@@ -1101,9 +1155,12 @@ extern "C" int nrf_solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const f
     if ((rc = solver_assemble(s, corrs, canonical, nodes, reinterpret_cast<const DQ *>(transforms),
                               reinterpret_cast<const DQ *>(centered), num_nodes, st)))
       return rc;
+    sink_record(s, st);
     if ((rc = solver_pcg(s, delta_x, num_nodes, st))) return rc;
+    sink_record(s, st);
     gn_apply_kernel<<<1, 1024, 0, st>>>(g);
     NRF_LAUNCH_CHECK();
+    sink_record(s, st);
   }
   return solver_fetch_result(s, result, st);
 }

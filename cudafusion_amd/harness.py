@@ -134,6 +134,19 @@ class Harness:
                                            self.corrs[f].data_ptr(), self.canonical[f].data_ptr(),
                                            self.w.width * self.w.height, stream), "pipeline_step")
 
+    def solver_result(self) -> L.PcgResult:
+        """The device-side PCG / GN record of the last step (syncs)."""
+        solver = self.lib.nrf_pipeline_solver(self.pipe)
+        ptr = self.lib.nrf_solver_device_result(solver)
+        raw = self._d2h(ptr, C.sizeof(L.PcgResult), np.uint8)
+        return L.PcgResult.from_buffer_copy(raw.tobytes())
+
+    def num_blocks(self) -> int:
+        solver = self.lib.nrf_pipeline_solver(self.pipe)
+        nb = C.c_int()
+        L.check(self.lib.nrf_solver_system(solver, None, None, None, C.byref(nb), None, None), "system")
+        return nb.value
+
     def stage_ms(self):
         out = (C.c_float * 3)()
         L.check(self.lib.nrf_pipeline_stage_ms(self.pipe, out), "stage_ms")

@@ -295,6 +295,16 @@ int nrf_pipeline_step(nrf_pipeline *p, const float *depth,
 /* Per-stage device times of the last step in ms (hipEvent based; valid after
    the stream has been synchronised): [solve, integrate, mc]. */
 int nrf_pipeline_stage_ms(nrf_pipeline *p, float out_ms[3]);
+/* Per-kernel profiling: with ring_steps > 0, every step records hipEvents
+   around each kernel group on the step's stream (eager launches, no graph)
+   into a ring of ring_steps steps; 0 disables.  nrf_pipeline_profile_read
+   returns, for the step `step_back` steps before the last one, the device
+   time in ms of [membership build, assembly, PCG, twist update, integration,
+   marching cubes], assembly / PCG / update summed over the GN iterations. */
+int nrf_pipeline_profile(nrf_pipeline *p, int ring_steps);
+int nrf_pipeline_profile_read(nrf_pipeline *p, int step_back, float out_ms[6]);
+/* Voxels written by the warped integrations of all steps so far (syncs). */
+long long nrf_pipeline_updated_voxels(nrf_pipeline *p);
 
 #ifdef __cplusplus
 }
