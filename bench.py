@@ -105,7 +105,6 @@ def main() -> int:
         f"last step GN applied {res.gn_done}, PCG iterations {res.total_iterations}")
     kern = None
     if not args.graph:
-        import numpy as np
         rows = []
         for k in range(args.steps):
             out = (C.c_float * 6)()

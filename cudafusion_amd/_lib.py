@@ -73,10 +73,16 @@ _SIGS = {
     "nrf_mc_workspace_size": (C.c_size_t, [C.POINTER(VolumeParams)]),
     "nrf_mc_extract": (I, [C.POINTER(VolumeParams), P, P, C.c_size_t, P, P, P, I, P, P]),
     "nrf_mc_cases": (P, [P]),
+    "nrf_tsdf16_reset": (I, [C.POINTER(VolumeParams), P, P, P]),
+    "nrf_tsdf16_from_voxels": (I, [C.POINTER(VolumeParams), P, P, P, P]),
+    "nrf_knn_compact": (I, [C.POINTER(VolumeParams), P, P, P, P, P]),
+    "nrf_tsdf16_integrate": (I, [C.POINTER(VolumeParams), P, P, P, C.POINTER(Camera), P, P, I, P, P, P, P]),
+    "nrf_mc16_extract": (I, [C.POINTER(VolumeParams), P, P, P, C.c_size_t, P, P, I, P, P]),
     "nrf_voxel_knn": (I, [C.POINTER(VolumeParams), P, P, I, P, P, P]),
     "nrf_warp_apply_twist": (I, [P, P, I, F, F, F, P, P, P]),
     "nrf_warp_centered_table": (I, [P, P, I, P, P]),
     "nrf_warp_points": (I, [P, P, P, I, P, P, I, P, P]),
+    "nrf_warp_normals": (I, [P, P, P, I, P, I, P, P]),
     "nrf_corr_pack": (I, [C.POINTER(VolumeParams), P, P, P, P, P, P, P, I, P, P]),
     "nrf_solver_create": (P, [C.POINTER(SolverParams), I]),
     "nrf_solver_destroy": (None, [P]),
@@ -162,6 +168,7 @@ def solver_params(gn_iterations=3, pcg_iterations=10, pcg_tolerance=1e-4, lambda
                         debug, relative_tolerance, precond_fp64)
 
 
-def gn_params(gn_iterations=3, update_scale=1.0, max_update_rot=0.12, max_update_trans=0.04, validate=1,
+def gn_params(gn_iterations=3, update_scale=0.25, max_update_rot=0.05, max_update_trans=0.03, validate=1,
               max_dx_mean=0.03) -> GNParams:
+    """Defaults are the reference pipeline's (include/fusion_app.h:72-119)."""
     return GNParams(gn_iterations, update_scale, max_update_rot, max_update_trans, validate, max_dx_mean)

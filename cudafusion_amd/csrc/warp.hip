@@ -149,6 +149,39 @@ __global__ void warp_points_kernel(const float *__restrict__ pts, const int4 *__
   out[3 * i + 2] = p.z;
 }
 
+// warp_normal (include/se3_math.cuh:512-558): blends the real parts of the
+// *uncentred* node transforms (running-sign fix on the real part only),
+// normalises, rotates n and renormalises it.  There is no weight-sum test:
+// with no valid neighbour the blend normalises to the identity.
+__global__ void warp_normals_kernel(const float *__restrict__ nin, const int4 *__restrict__ ids4,
+                                    const float4 *__restrict__ ws4, int count, const DQ *__restrict__ T, int N,
+                                    float *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int4 i4 = ids4[i];
+  const float4 w4 = ws4[i];
+  const int ids[4] = {i4.x, i4.y, i4.z, i4.w};
+  const float ws[4] = {w4.x, w4.y, w4.z, w4.w};
+  DQ b;
+  b.r = make_float4(0.f, 0.f, 0.f, 0.f);
+  b.d = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int k = 0; k < 4; ++k) {
+    const int id = ids[k];
+    const float w = ws[k];
+    if (id < 0 || id >= N || w < 1e-8f) continue;
+    float4 q = T[id].r;
+    if (qdot(q, b.r) < 0) q = make_float4(-q.x, -q.y, -q.z, -q.w);
+    b.r.x += w * q.x; b.r.y += w * q.y; b.r.z += w * q.z; b.r.w += w * q.w;
+  }
+  b = dq_normalize(b);
+  const float4 r = qmul(qmul(b.r, make_float4(nin[3 * i], nin[3 * i + 1], nin[3 * i + 2], 0.f)), qconj(b.r));
+  const float n = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
+  float3 o = f3(r.x, r.y, r.z);
+  if (n > 1e-9f) o = f3(r.x / n, r.y / n, r.z / n);
+  out[3 * i] = o.x;
+  out[3 * i + 1] = o.y;
+  out[3 * i + 2] = o.z;
+}
 
 // Association input packing: a correspondence per point with src = the
 // canonical point and the 4-NN of the voxel containing it, as
@@ -257,6 +290,18 @@ extern "C" int nrf_corr_pack(const nrf_volume_params *vp, const int32_t *voxel_k
       canonical, dst, normals, weights, valid, count, vp->dims[0], vp->dims[1], vp->dims[2],
       make_float3(vp->origin[0], vp->origin[1], vp->origin[2]), vp->voxel_size,
       reinterpret_cast<const int4 *>(voxel_knn), reinterpret_cast<const float4 *>(voxel_knn_w), out);
+  NRF_LAUNCH_CHECK();
+  return NRF_OK;
+}
+
+extern "C" int nrf_warp_normals(const float *normals, const int32_t *knn_ids, const float *knn_ws, int count,
+                                const nrf_dualquat *transforms, int num_nodes, float *out, void *stream) {
+  if (count < 0 || num_nodes < 0) return NRF_ERR_INVALID_ARGUMENT;
+  if (count == 0) return NRF_OK;
+  if (!normals || !knn_ids || !knn_ws || !out || (num_nodes > 0 && !transforms)) return NRF_ERR_INVALID_ARGUMENT;
+  warp_normals_kernel<<<(count + 255) / 256, 256, 0, as_stream(stream)>>>(
+      normals, reinterpret_cast<const int4 *>(knn_ids), reinterpret_cast<const float4 *>(knn_ws), count,
+      reinterpret_cast<const DQ *>(transforms), num_nodes, out);
   NRF_LAUNCH_CHECK();
   return NRF_OK;
 }

@@ -58,9 +58,10 @@ class Workload:
     lambda_damping: float = 1e-2
     decay_alpha: float = 1.0
     max_weight: float = 50.0
-    update_scale: float = 1.0
-    max_update_rot: float = 0.12
-    max_update_trans: float = 0.04
+    # caller step policy: the reference pipeline's defaults (include/fusion_app.h:72-119)
+    update_scale: float = 0.25
+    max_update_rot: float = 0.05
+    max_update_trans: float = 0.03
     max_dx_mean: float = 0.03
     min_opt_count: int = 0
     seed: int = 0

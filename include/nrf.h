@@ -169,6 +169,28 @@ int nrf_mc_extract(const nrf_volume_params *vp, const nrf_voxel *voxels,
    reference skips), from the workspace of the last nrf_mc_extract call. */
 const uint8_t *nrf_mc_cases(const void *workspace);
 
+/* ---------------------------------------------- compact (precision) form */
+/* The TSDF stored as FP16 value + uint8 weight (3 B/voxel) and the voxel kNN
+   as uint16 ids (0xffff = none) + FP16 weights (16 B/voxel).  Integration and
+   marching cubes run the FP32 arithmetic of nrf_tsdf_integrate /
+   nrf_mc_extract on the decoded state and round once when storing (FP16
+   round-to-nearest-even; weight rounded to the nearest integer, exact when
+   decay_alpha = 1).  tsdf16 holds IEEE binary16 bit patterns. */
+int nrf_tsdf16_reset(const nrf_volume_params *vp, uint16_t *tsdf16, uint8_t *weight8, void *stream);
+int nrf_tsdf16_from_voxels(const nrf_volume_params *vp, const nrf_voxel *voxels, uint16_t *tsdf16,
+                           uint8_t *weight8, void *stream);
+int nrf_knn_compact(const nrf_volume_params *vp, const int32_t *voxel_knn, const float *voxel_knn_w,
+                    uint16_t *knn16, uint16_t *knn_w16, void *stream);
+/* Warped when num_nodes > 0 (centered = nrf_warp_centered_table output). */
+int nrf_tsdf16_integrate(const nrf_volume_params *vp, uint16_t *tsdf16, uint8_t *weight8, const float *depth,
+                         const nrf_camera *cam, const float camera_pose[16], const nrf_dualquat *centered,
+                         int num_nodes, const uint16_t *knn16, const uint16_t *knn_w16,
+                         int32_t *updated_count, void *stream);
+/* Same workspace and output conventions as nrf_mc_extract (no normals). */
+int nrf_mc16_extract(const nrf_volume_params *vp, const uint16_t *tsdf16, const uint8_t *weight8,
+                     void *workspace, size_t workspace_bytes, float *vertices, int32_t *triangles,
+                     int max_triangles, int32_t *counts, void *stream);
+
 /* ------------------------------------------------------------ warp field */
 /* Replaces WarpField::compute_voxel_knn (src/warp_field.cu:957-984) and
    compute_voxel_knn_kernel (:299-405): brute-force 4-NN over all nodes for
@@ -196,6 +218,14 @@ int nrf_warp_points(const float *points, const int32_t *knn_ids,
                     const float *knn_ws, int count, const nrf_node *nodes,
                     const nrf_dualquat *transforms, int num_nodes, float *out,
                     void *stream);
+
+/* Rotates normals through their 4-NN as warp_normal does
+   (include/se3_math.cuh:512-558): blend of the uncentred transforms' real
+   parts, no weight-sum test. */
+int nrf_warp_normals(const float *normals, const int32_t *knn_ids,
+                     const float *knn_ws, int count,
+                     const nrf_dualquat *transforms, int num_nodes, float *out,
+                     void *stream);
 
 /* Association input: packs one correspondence per point (src = canonical
    point, 4-NN copied from the voxel containing it, weight 1 or weights[i])

@@ -209,6 +209,25 @@ V3<R> warp_point(V3<R> p, const DQ<R> *centered, int N, const int32_t ids[4], co
   return dq_xf(dq_normalize(b), p);
 }
 
+// warp_normal: se3_math.cuh:512-558 (uncentred real parts, no weight-sum test)
+template <typename R>
+V3<R> warp_normal(V3<R> n, const DQ<R> *T, int N, const int32_t ids[4], const float ws[4]) {
+  DQ<R> b{{0, 0, 0, 0}, {0, 0, 0, 0}};
+  for (int k = 0; k < 4; ++k) {
+    int id = ids[k];
+    if (id < 0 || id >= N || ws[k] < 1e-8f) continue;
+    R w = (R)ws[k];
+    Q4<R> q = T[id].r;
+    if (qdot(q, b.r) < 0) q = {-q.x, -q.y, -q.z, -q.w};
+    b.r.x += w * q.x; b.r.y += w * q.y; b.r.z += w * q.z; b.r.w += w * q.w;
+  }
+  b = dq_normalize(b);
+  Q4<R> r = qmul(qmul(b.r, Q4<R>{n.x, n.y, n.z, 0}), qconj(b.r));
+  R len = std::sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+  if (len > (R)1e-9f) return {r.x / len, r.y / len, r.z / len};
+  return {r.x, r.y, r.z};
+}
+
 template <typename R>
 std::vector<DQ<R>> centered_table(const nrf_node *nodes, const DQ<R> *T, int N) {
   std::vector<DQ<R>> c(N);
@@ -856,6 +875,64 @@ void orc_warp_points_f32(const float *pts, const int32_t *ids, const float *ws, 
     V3<float> p = warp_point(v3f<float>(pts + 3 * i), cen.data(), N, ids + 4 * i, ws + 4 * i);
     out[3 * i] = p.x; out[3 * i + 1] = p.y; out[3 * i + 2] = p.z;
   }
+}
+
+void orc_warp_normals_f32(const float *nin, const int32_t *ids, const float *ws, int count,
+                          const nrf_dualquat *Tq, int N, float *out) {
+  auto T = load_T<float>(Tq, N);
+  for (int i = 0; i < count; ++i) {
+    V3<float> r = warp_normal(v3f<float>(nin + 3 * i), T.data(), N, ids + 4 * i, ws + 4 * i);
+    out[3 * i] = r.x; out[3 * i + 1] = r.y; out[3 * i + 2] = r.z;
+  }
+}
+void orc_warp_normals_f64(const float *nin, const int32_t *ids, const float *ws, int count, const double *T8,
+                          int N, double *out) {
+  std::vector<DQ<double>> T(N);
+  for (int i = 0; i < N; ++i) T[i] = load_dq8(T8 + 8 * i);
+  for (int i = 0; i < count; ++i) {
+    V3<double> r = warp_normal(v3f<double>(nin + 3 * i), T.data(), N, ids + 4 * i, ws + 4 * i);
+    out[3 * i] = r.x; out[3 * i + 1] = r.y; out[3 * i + 2] = r.z;
+  }
+}
+
+// Distance of each voxel's FP64 projection from the rounding boundaries of
+// the integration test (for parity checks of FP32 kernels): px = distance of
+// (u, v) from the nearest half-integer (where round-half-even flips), sdf =
+// |sdf + truncation| / truncation at the sampled pixel; +inf where the voxel
+// is not projected.
+void orc_integrate_margin_f64(const nrf_volume_params *vp, const float *depth, const nrf_camera *cam,
+                              const float pose[16], const nrf_node *nodes, const double *T8, int N,
+                              const int32_t *knn, const float *knn_w, double *m_px, double *m_sdf) {
+  const int X = vp->dims[0], Y = vp->dims[1], Z = vp->dims[2];
+  const bool warp = nodes != nullptr && N > 0;
+  float Tc[3][4];
+  invert_pose(pose, Tc);
+  std::vector<DQ<double>> T(N), cen;
+  for (int i = 0; i < N; ++i) T[i] = load_dq8(T8 + 8 * i);
+  if (warp) cen = centered_table(nodes, T.data(), N);
+  const double s = vp->voxel_size, tr = vp->truncation;
+#pragma omp parallel for schedule(static)
+  for (int z = 0; z < Z; ++z)
+    for (int y = 0; y < Y; ++y)
+      for (int x = 0; x < X; ++x) {
+        const long idx = ((long)z * Y + y) * X + x;
+        m_px[idx] = m_sdf[idx] = INFINITY;
+        V3<double> p = {(double)vp->origin[0] + (x + 0.5) * s, (double)vp->origin[1] + (y + 0.5) * s,
+                        (double)vp->origin[2] + (z + 0.5) * s};
+        if (warp) p = warp_point(p, cen.data(), N, knn + idx * 4, knn_w + idx * 4);
+        double cx = Tc[0][0] * p.x + Tc[0][1] * p.y + Tc[0][2] * p.z + Tc[0][3];
+        double cy = Tc[1][0] * p.x + Tc[1][1] * p.y + Tc[1][2] * p.z + Tc[1][3];
+        double cz = Tc[2][0] * p.x + Tc[2][1] * p.y + Tc[2][2] * p.z + Tc[2][3];
+        if (cz <= 0) continue;
+        double fu = cam->fx * cx / cz + cam->cx, fv = cam->fy * cy / cz + cam->cy;
+        if (!(fu > -1e9 && fu < 1e9 && fv > -1e9 && fv < 1e9)) continue;
+        m_px[idx] = std::min(std::fabs(fu - std::floor(fu) - 0.5), std::fabs(fv - std::floor(fv) - 0.5));
+        int u = (int)std::nearbyint(fu), v = (int)std::nearbyint(fv);
+        if (u < 0 || u >= cam->width || v < 0 || v >= cam->height) continue;
+        double d = depth[(long)v * cam->width + u];
+        if (d <= 0) continue;
+        m_sdf[idx] = std::fabs(d - cz + tr) / tr;
+      }
 }
 
 long orc_integrate_f32(const nrf_volume_params *vp, nrf_voxel *vox, const float *depth, const nrf_camera *cam,

@@ -35,6 +35,9 @@ _SIGS = {
     "orc_twist_update_f32": (None, [P, P, F, F, F, P]),
     "orc_warp_points_f64": (None, [P, P, P, I, P, P, I, P]),
     "orc_warp_points_f32": (None, [P, P, P, I, P, P, I, P]),
+    "orc_warp_normals_f32": (None, [P, P, P, I, P, I, P]),
+    "orc_warp_normals_f64": (None, [P, P, P, I, P, I, P]),
+    "orc_integrate_margin_f64": (None, [P, P, P, P, P, P, I, P, P, P, P]),
     "orc_integrate_f32": (L, [P, P, P, P, P, P, P, I, P, P, P, I]),
     "orc_integrate_f64": (L, [P, P, P, P, P, P, P, P, I, P, P, P, I]),
     "orc_mc_f32": (L, [P, P, P, P, P, L]),
@@ -116,6 +119,34 @@ def integrate_f64(vp, tsdf, weight, depth, cam, pose, nodes=None, T64=None, knn=
     n = 0 if nodes is None else len(nodes)
     return lib.orc_integrate_f64(C.byref(vp), ptr(tsdf), ptr(weight), ptr(depth), C.byref(cam), ptr(pose),
                                  ptr(nodes), ptr(T64), n, ptr(knn), ptr(knn_w), ptr(opt), min_opt)
+
+
+def integrate_margin_f64(vp, depth, cam, pose, nodes=None, T64=None, knn=None, knn_w=None):
+    """Per-voxel distance of the FP64 projection from the pixel-rounding
+    boundary (pixels) and of the sdf from the truncation cut (units of the
+    truncation)."""
+    V = int(np.prod(vp.dims[:]))
+    m_px = np.empty(V)
+    m_sdf = np.empty(V)
+    n = 0 if nodes is None else len(nodes)
+    T64 = None if T64 is None else np.ascontiguousarray(T64, np.float64)
+    load().orc_integrate_margin_f64(C.byref(vp), ptr(depth), C.byref(cam), ptr(pose), ptr(nodes), ptr(T64), n,
+                                    ptr(knn), ptr(knn_w), ptr(m_px), ptr(m_sdf))
+    return m_px, m_sdf
+
+
+def tsdf_fp64_errors(gpu_vox, t64, w64, margins, px_eps=2e-3, sdf_eps=1e-4):
+    """Compares an FP32 device TSDF with the FP64 oracle on the voxels that
+    are not within a rounding margin of a projection or truncation boundary.
+    Returns (max |tsdf error| on unambiguous voxels, fraction of ambiguous
+    voxels, count of unambiguous weight mismatches)."""
+    m_px, m_sdf = margins
+    ambiguous = (m_px < px_eps) | (m_sdf < sdf_eps)
+    ok = ~ambiguous
+    wmis = int(((gpu_vox["weight"] != w64.astype(np.float32)) & ok).sum())
+    both = ok & (gpu_vox["weight"] == w64.astype(np.float32))
+    err = float(np.abs(gpu_vox["tsdf"][both] - t64[both]).max()) if both.any() else 0.0
+    return err, float(ambiguous.mean()), wmis
 
 
 def mc_f32(vp, vox, normals=True):

@@ -100,7 +100,7 @@ static void tsdf_integration() {
     obs += x.weight > 0;
     near += x.weight > 0 && std::fabs(x.tsdf) < 1.f;
   }
-  CHECK(obs > 100000 && near > 10000, "tsdf_integration: %ld observed voxels, %ld inside the truncation band", obs, near);
+  CHECK(obs > 50000 && near > 5000, "tsdf_integration: %ld observed voxels, %ld inside the truncation band", obs, near);
 }
 
 static std::vector<DeformNode> ring_nodes(int n) {

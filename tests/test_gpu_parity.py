@@ -83,11 +83,12 @@ def test_integrate_rigid(sphere, orc):
     assert n_upd > 10000
     np.testing.assert_array_equal(gpu["weight"], ref["weight"])
     np.testing.assert_allclose(gpu["tsdf"], ref["tsdf"], atol=2e-7)
-    # FP64: voxels whose projection or sdf test sits on an FP32 rounding
-    # boundary may be taken by one precision and not the other
-    agree = gpu["weight"] == w64.astype(np.float32)
-    assert (~agree).sum() <= 1e-5 * len(agree)
-    assert np.abs(gpu["tsdf"][agree] - t64[agree]).max() < 1e-3
+    # FP64: a voxel whose projection sits on a pixel-rounding boundary (or
+    # whose sdf sits on the truncation cut) may legitimately sample another
+    # pixel in FP32; such voxels are identified from the FP64 projection
+    err, amb, wmis = orc.tsdf_fp64_errors(gpu, t64, w64, orc.integrate_margin_f64(vol.vp, depth, cam, pose))
+    print(f"[integrate] FP64 max err {err:.2e}, ambiguous {amb:.2e}, weight mismatches {wmis}", flush=True)
+    assert wmis == 0 and amb < 1e-3 and err < 1e-3
 
 
 def test_voxel_knn_and_warped_integration(sphere, orc):
@@ -132,9 +133,10 @@ def test_voxel_knn_and_warped_integration(sphere, orc):
     orc.integrate_f64(vol.vp, t64, w64, d1, cam, pose, nodes, T64, rk, rw)
     changed = g1["weight"] != vox["weight"]
     assert changed.sum() > 10000
-    agree = g1["weight"] == w64.astype(np.float32)
-    assert agree.mean() > 0.9999  # projections on a pixel boundary may round differently
-    assert np.abs(g1["tsdf"][agree] - t64[agree]).max() < 1e-3
+    err, amb, wmis = orc.tsdf_fp64_errors(g1, t64, w64,
+                                          orc.integrate_margin_f64(vol.vp, d1, cam, pose, nodes, T64, rk, rw))
+    print(f"[warped] FP64 max err {err:.2e}, ambiguous {amb:.2e}, weight mismatches {wmis}", flush=True)
+    assert wmis == 0 and amb < 1e-3 and err < 1e-3
 
 
 def test_marching_cubes(sphere, orc):
@@ -223,8 +225,9 @@ def test_twist_and_warp_points(sphere, orc):
                                 wf.device_transforms().data_ptr(), N, out.data_ptr(), None), "warp_points")
     torch.cuda.synchronize()
     refp = np.zeros(1500, np.float32)
+    gT = np.ascontiguousarray(gv)  # keep the array alive across the call
     orc.load().orc_warp_points_f32(pts.ctypes.data, ids.ctypes.data, ws.ctypes.data, 500, nodes.ctypes.data,
-                                   gv.copy().ctypes.data, N, refp.ctypes.data)
+                                   gT.ctypes.data, N, refp.ctypes.data)
     np.testing.assert_allclose(out.cpu().numpy(), refp, atol=2e-6)
     np.testing.assert_array_equal(out.cpu().numpy()[:21], pts[:7].reshape(-1))
 
@@ -316,7 +319,8 @@ def test_solver_system_and_pcg(sphere, orc, solver_inputs):
 
 
 def test_solve_gn_transforms(sphere, orc, solver_inputs):
-    """3 GN x 20 PCG with re-warped src: node transforms vs the FP64 oracle."""
+    """3 GN x 20 PCG with re-warped src and the reference caller's step
+    policy (fusion_app.h:72-119 defaults): node transforms vs the FP64 oracle."""
     import torch
 
     from cudafusion_amd import _lib as L
@@ -330,7 +334,7 @@ def test_solve_gn_transforms(sphere, orc, solver_inputs):
     p = GaussNewtonSolver.Params(pcg_iterations=20, pcg_tolerance=0.0, lambda_smooth=w.lambda_smooth)
     s = GaussNewtonSolver(p, 8192)
     s.set_pattern(nodes)
-    gn = L.gn_params(3, 1.0, 0.12, 0.04, 1, 0.03)
+    gn = L.gn_params(3, 0.25, 0.05, 0.03, 1, 0.03)
     cen = wf.centered_table()
     dx = torch.zeros(6 * N, dtype=torch.float32, device="cuda")
     res = s.solve_gn(solver_inputs["dcorr"], solver_inputs["dcan"], len(solver_inputs["hcorr"]), wf.device_nodes(),
@@ -362,3 +366,27 @@ def test_pipeline_step_sphere(gpu, orc):
     ms = h.stage_ms()
     assert all(m > 0 for m in ms)
     h.close()
+
+
+def test_warp_normals(sphere, orc):
+    import torch
+
+    from cudafusion_amd import _lib as L
+    nodes = sphere["nodes"]
+    N = len(nodes)
+    T, _ = _perturbed_T(orc, nodes, 13, rot=0.2, trans=0.01)
+    rng = np.random.default_rng(14)
+    nrm = rng.normal(0, 1, (400, 3)).astype(np.float32)
+    ids = rng.integers(0, N, (400, 4)).astype(np.int32)
+    ws = rng.uniform(0, 1, (400, 4)).astype(np.float32)
+    ids[:5] = -1
+    out = torch.empty(1200, dtype=torch.float32, device="cuda")
+    dn, di, dw, dT = _dev(nrm.reshape(-1)), _dev(ids.reshape(-1)), _dev(ws.reshape(-1)), _dev(T.view(np.uint8))
+    L.check(L.load().nrf_warp_normals(dn.data_ptr(), di.data_ptr(), dw.data_ptr(), 400, dT.data_ptr(), N,
+                                      out.data_ptr(), None), "warp_normals")
+    torch.cuda.synchronize()
+    ref = np.zeros(1200, np.float32)
+    Tc = np.ascontiguousarray(T)
+    orc.load().orc_warp_normals_f32(nrm.ctypes.data, ids.ctypes.data, ws.ctypes.data, 400, Tc.ctypes.data, N,
+                                    ref.ctypes.data)
+    np.testing.assert_allclose(out.cpu().numpy(), ref, atol=2e-6)

@@ -187,3 +187,35 @@ def test_warp_antipodal_invariance(orc, golden):
     Tn[2] *= -1  # q and -q are the same rigid transform
     b = warp64(orc, pts, ids, ws, nodes, Tn)
     np.testing.assert_allclose(a, b, atol=1e-12)
+
+
+def warpn64(orc, nrm, ids, ws, T8):
+    out = np.zeros(nrm.shape, np.float64)
+    n = np.ascontiguousarray(nrm, np.float32)
+    i = np.ascontiguousarray(ids, np.int32)
+    w = np.ascontiguousarray(ws, np.float32)
+    t = np.ascontiguousarray(T8, np.float64)
+    orc.load().orc_warp_normals_f64(orc.ptr(n), orc.ptr(i), orc.ptr(w), len(nrm), orc.ptr(t), len(T8), orc.ptr(out))
+    return out
+
+
+def test_warp_normal_rotates_by_uncentred_blend(orc, golden):
+    """warp_normal (se3_math.cuh:512-558): with every neighbour carrying the
+    same rotation R the normal becomes R n (translations and centres play no
+    part); with no valid neighbour it is only normalised."""
+    rng = np.random.default_rng(6)
+    m = golden["expm"][7][:3]
+    T = np.tile(mat2dq(orc, m), (4, 1))
+    nrm = rng.normal(0, 1, (12, 3))
+    ids = np.tile([0, 1, 2, 3], (12, 1))
+    ws = rng.uniform(0.1, 1, (12, 4))
+    out = warpn64(orc, nrm, ids, ws, T)
+    exp = nrm.astype(np.float32).astype(np.float64) @ m[:, :3].T
+    exp /= np.linalg.norm(exp, axis=1, keepdims=True)
+    np.testing.assert_allclose(out, exp, atol=1e-6)
+    T2 = T.copy()
+    T2[1] *= -1  # antipodal copy of the same rotation
+    np.testing.assert_allclose(warpn64(orc, nrm, ids, ws, T2), exp, atol=1e-6)
+    none = warpn64(orc, nrm, np.full((12, 4), -1), ws, T)
+    n32 = nrm.astype(np.float32).astype(np.float64)
+    np.testing.assert_allclose(none, n32 / np.linalg.norm(n32, axis=1, keepdims=True), atol=1e-7)
