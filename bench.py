@@ -6,6 +6,15 @@ dots).  One step = GN solve (assembly, block-Jacobi PCG, validated twist
 update) + DQ-warped integration + marching cubes on one frame whose depth
 and association inputs are already resident in HBM.
 
+The solver runs with the ARAP rotation Jacobian in the sign convention of
+the data term (solver option arap_consistent_sign).  With the reference's
+negated rows (kernels/solver_kernels.cu:145-153) the rotations of this
+sequence drift every GN iteration, the reference's step validation
+(include/fusion_app.h:409-491) rejects every step from frame 3 on, and the
+remaining GN iterations of each frame do no work.  The two conventions run
+the same kernels with the same traffic; --reference-sign selects the
+reference's.
+
 Prints one JSON line (value = frames/s over the timed steps).
 """
 from __future__ import annotations
@@ -63,6 +72,11 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="before the timed steps, run one elementwise copy of MIB MiB (x * 1 -> y): a kernel of "
+                         "known HBM traffic that calibrates hardware byte counters")
+    ap.add_argument("--reference-sign", action="store_true",
+                    help="ARAP rotation Jacobian with the reference's sign (steps get rejected)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs the step on one GPU; multi-GPU runs are not part of it",
@@ -78,7 +92,7 @@ def main() -> int:
     from cudafusion_amd.harness import Harness
 
     t0 = time.perf_counter()
-    h = Harness("body", frames=args.frames, use_graph=args.graph)
+    h = Harness("body", frames=args.frames, use_graph=args.graph, arap_consistent_sign=not args.reference_sign)
     h.setup()
     log(f"[bench] setup {time.perf_counter() - t0:.1f} s, {h.N} nodes, "
         f"{int(h.frames_host[1].valid.sum())} valid pixels in frame 1")
@@ -90,19 +104,33 @@ def main() -> int:
     for f in seq[: args.warmup]:
         h.step(f)
     torch.cuda.synchronize()
+    if args.copy_probe > 0:
+        n = args.copy_probe * (1 << 20) // 4
+        xs = torch.ones(n, dtype=torch.float32, device="cuda")
+        ys = torch.empty_like(xs)
+        torch.cuda.synchronize()
+        torch.mul(xs, 1.0, out=ys)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: {n * 4} bytes read, {n * 4} bytes written")
+        del xs, ys
     upd0 = lib.nrf_pipeline_updated_voxels(h.pipe)
+    tot0 = h.solver_totals()
     t0 = time.perf_counter()
     for f in seq[args.warmup:]:
         h.step(f)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     upd = (lib.nrf_pipeline_updated_voxels(h.pipe) - upd0) / args.steps
+    tot1 = h.solver_totals()
+    gn_per_step = (tot1.gn_done - tot0.gn_done) / args.steps
+    it_per_step = (tot1.pcg_iterations - tot0.pcg_iterations) / args.steps
+    rejected = tot1.rejected - tot0.rejected
     ms = h.stage_ms()
     fps = args.steps / elapsed
-    res = h.solver_result()
     log(f"[bench] {args.steps} steps in {elapsed * 1e3:.1f} ms -> {fps:.1f} frames/s; last step stages "
         f"solve {ms[0]:.3f} ms, integrate {ms[1]:.3f} ms, mc {ms[2]:.3f} ms; tris {h.triangle_count()}; "
-        f"last step GN applied {res.gn_done}, PCG iterations {res.total_iterations}")
+        f"GN steps applied per frame {gn_per_step:.2f}, PCG iterations per frame {it_per_step:.1f}, "
+        f"frames ending on a rejected step {rejected}")
     kern = None
     if not args.graph:
         rows = []
@@ -128,6 +156,10 @@ def main() -> int:
         # one read of the 12-byte voxels + 36 B of vertices and 12 B of indices per triangle
         "mc": V * 12 + h.triangle_count() * 48,
     }
+    # PCG: algorithmic bytes per iteration times the iterations one launch runs
+    # on average (launches after a rejected step run none)
+    pcg_launch_iters = it_per_step / G
+    algo["pcg"] = algo["pcg"] / w.pcg_iterations * pcg_launch_iters
     if kern is not None:
         per_launch = {"integrate": kern[4], "pcg": kern[2] / G, "mc": kern[5]}
         per_step = {"integrate": kern[4], "pcg": kern[2], "mc": kern[5]}
@@ -157,7 +189,8 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "capsule-body 256^3 @4mm, 640x480, %d nodes, 5 GN x 30 PCG" % h.N,
-                   "graph": bool(args.graph)},
+                   "graph": bool(args.graph),
+                   "arap_rotation_sign": "reference" if args.reference_sign else "consistent (opt-in)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
                      "launch_ms": round(per_launch[dom], 5), "algorithmic_bytes": int(algo[dom])},
@@ -165,8 +198,9 @@ def main() -> int:
             ["membership", "assembly", "pcg", "update", "integrate", "mc"], [round(float(v), 5) for v in kern])),
         "roofline_other": {k: round(algo[k] / (per_launch[k] * 1e-3) / 1e9, 1) for k in algo
                            if per_launch[k] == per_launch[k]},
-        "work": {"gn_applied_last_step": res.gn_done, "pcg_iterations_last_step": res.total_iterations,
-                 "updated_voxels_per_step": int(upd), "nodes": h.N, "bsr_blocks": nb},
+        "work": {"gn_applied_per_step": round(gn_per_step, 3), "pcg_iterations_per_step": round(it_per_step, 2),
+                 "rejected_steps": int(rejected), "updated_voxels_per_step": int(upd), "nodes": h.N,
+                 "bsr_blocks": nb},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

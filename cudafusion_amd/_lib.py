@@ -40,7 +40,8 @@ class Camera(C.Structure):
 class SolverParams(C.Structure):
     _fields_ = [("gn_iterations", C.c_int32), ("pcg_iterations", C.c_int32), ("pcg_tolerance", C.c_float),
                 ("lambda_smooth", C.c_float), ("lambda_rot", C.c_float), ("lambda_damping", C.c_float),
-                ("debug", C.c_int32), ("relative_tolerance", C.c_int32), ("precond_fp64", C.c_int32)]
+                ("debug", C.c_int32), ("relative_tolerance", C.c_int32), ("precond_fp64", C.c_int32),
+                ("arap_consistent_sign", C.c_int32)]
 
 
 class GNParams(C.Structure):
@@ -51,6 +52,11 @@ class GNParams(C.Structure):
 class PcgResult(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("total_iterations", C.c_int32), ("flags", C.c_int32),
                 ("gn_done", C.c_int32), ("final_residual", C.c_double), ("rhs_norm", C.c_double)]
+
+
+class SolverTotals(C.Structure):
+    _fields_ = [("solves", C.c_int64), ("gn_done", C.c_int64), ("pcg_iterations", C.c_int64),
+                ("rejected", C.c_int64)]
 
 
 class PipelineConfig(C.Structure):
@@ -93,6 +99,7 @@ _SIGS = {
     "nrf_solver_system": (I, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(I), C.POINTER(P),
                               C.POINTER(P)]),
     "nrf_solver_device_result": (P, [P]),
+    "nrf_solver_get_totals": (I, [P, C.POINTER(SolverTotals), P]),
     "nrf_pipeline_create": (P, [C.POINTER(PipelineConfig)]),
     "nrf_pipeline_destroy": (None, [P]),
     "nrf_pipeline_voxels": (P, [P]),
@@ -163,9 +170,10 @@ def camera(fx, fy, cx, cy, width, height) -> Camera:
 
 
 def solver_params(gn_iterations=3, pcg_iterations=10, pcg_tolerance=1e-4, lambda_smooth=1.0, lambda_rot=1.0,
-                  lambda_damping=1e-2, debug=0, relative_tolerance=0, precond_fp64=0) -> SolverParams:
+                  lambda_damping=1e-2, debug=0, relative_tolerance=0, precond_fp64=0,
+                  arap_consistent_sign=0) -> SolverParams:
     return SolverParams(gn_iterations, pcg_iterations, pcg_tolerance, lambda_smooth, lambda_rot, lambda_damping,
-                        debug, relative_tolerance, precond_fp64)
+                        debug, relative_tolerance, precond_fp64, arap_consistent_sign)
 
 
 def gn_params(gn_iterations=3, update_scale=0.25, max_update_rot=0.05, max_update_trans=0.03, validate=1,

@@ -139,6 +139,7 @@ struct AsmArgs {
   const int32_t *e_i, *e_j;
   const float *e_w;
   float lambda_smooth, lambda_damping;
+  float arap_sign;  // +1 reference rotation rows, -1 consistent sign
   int precond_fp64;
   int Rmax;
   float *values, *b, *Minv;
@@ -151,11 +152,14 @@ constexpr int kStageData = 3 + 4 + 24;
 constexpr int kStageArap = 6 + 18 + 18;
 constexpr int kStage = kStageArap > kStageData ? kStageArap : kStageData;
 
-__device__ inline void arap_jac(float3 q, float J[6][3]) {
-  // compute_arap_jacobian (kernels/solver_kernels.cu:140-165)
-  J[0][0] = 0.0f; J[0][1] = q.z; J[0][2] = -q.y;
-  J[1][0] = -q.z; J[1][1] = 0.0f; J[1][2] = q.x;
-  J[2][0] = q.y; J[2][1] = -q.x; J[2][2] = 0.0f;
+__device__ inline void arap_jac(float3 q, float sgn, float J[6][3]) {
+  // compute_arap_jacobian (kernels/solver_kernels.cu:140-165).  sgn = 1 gives
+  // the reference's rotation rows; sgn = -1 negates them (exact in FP32), which
+  // makes them d(omega x q)/d(omega) like the data term's.
+  const float3 s = make_float3(sgn * q.x, sgn * q.y, sgn * q.z);
+  J[0][0] = 0.0f; J[0][1] = s.z; J[0][2] = -s.y;
+  J[1][0] = -s.z; J[1][1] = 0.0f; J[1][2] = s.x;
+  J[2][0] = s.y; J[2][1] = -s.x; J[2][2] = 0.0f;
   J[3][0] = 1.0f; J[3][1] = 0.0f; J[3][2] = 0.0f;
   J[4][0] = 0.0f; J[4][1] = 1.0f; J[4][2] = 0.0f;
   J[5][0] = 0.0f; J[5][1] = 0.0f; J[5][2] = 1.0f;
@@ -313,8 +317,8 @@ __global__ __launch_bounds__(64) void assemble_rows_kernel(AsmArgs a) {
       lam = a.lambda_smooth * nw * hw * sw;
       if (!(lam <= 1e-8f)) {  // kernels/solver_kernels.cu:271
         float Ji[6][3], Jj[6][3];
-        arap_jac(sub(Ti_xj, xi), Ji);
-        arap_jac(sub(Tj_xj, xj), Jj);
+        arap_jac(sub(Ti_xj, xi), a.arap_sign, Ji);
+        arap_jac(sub(Tj_xj, xj), a.arap_sign, Jj);
         st[1] = __int_as_float(side);
         st[2] = __int_as_float(a.es_offrel[e]);
         st[3] = r.x; st[4] = r.y; st[5] = r.z;
@@ -606,8 +610,13 @@ __global__ void pcg_prep_kernel(PcgCtl *ctl) {
   }
 }
 
-__global__ void gn_begin_kernel(int32_t *stop, nrf_pcg_result *res) {
+__global__ void gn_begin_kernel(int32_t *stop, nrf_pcg_result *res, nrf_solver_totals *tot) {
   if (threadIdx.x == 0) {
+    // fold the previous solve into the running totals before clearing it
+    tot->solves += 1;
+    tot->gn_done += res->gn_done;
+    tot->pcg_iterations += res->total_iterations;
+    tot->rejected += (res->flags & 4) ? 1 : 0;
     *stop = 0;
     res->iterations = 0;
     res->total_iterations = 0;
@@ -858,6 +867,8 @@ struct nrf_solver {
   size_t c_stop = 0;
   nrf_pcg_result *d_res = nullptr;
   size_t c_res = 0;
+  nrf_solver_totals *d_tot = nullptr;
+  size_t c_tot = 0;
   int cus = 0, max_lds = 0, coop = 0;
   int forced_G = 0;
   // optional profiling events (see solver_set_event_sink)
@@ -983,6 +994,7 @@ static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, const float *ca
   a.e_j = s->e_j;
   a.e_w = s->e_w;
   a.lambda_smooth = s->params.lambda_smooth;
+  a.arap_sign = s->params.arap_consistent_sign ? -1.0f : 1.0f;
   a.lambda_damping = s->params.lambda_damping;
   a.precond_fp64 = s->params.precond_fp64;
   a.Rmax = s->P.Rmax;
@@ -1063,6 +1075,12 @@ extern "C" nrf_solver *nrf_solver_create(const nrf_solver_params *params, int ma
   s->cus = device_cu_count();
   s->max_lds = device_max_lds_per_block();
   s->coop = device_cooperative() ? 1 : 0;
+  // NRF_PCG_COOP=0 launches the persistent PCG kernel with an ordinary launch
+  // (G <= CU count workgroups are co-resident on an otherwise idle device, and
+  // the grid barrier's spin is bounded either way); profilers that cannot
+  // follow cooperative launches need it.
+  const char *fc = getenv("NRF_PCG_COOP");
+  if (fc && fc[0] == '0') s->coop = 0;
   const char *fg = getenv("NRF_PCG_WORKGROUPS");
   if (fg) s->forced_G = atoi(fg);
   const size_t n6 = (size_t)max_nodes * 6;
@@ -1070,11 +1088,12 @@ extern "C" nrf_solver *nrf_solver_create(const nrf_solver_params *params, int ma
             dev_alloc(s->zpub, n6, s->c_zpub) == 0 && dev_alloc(s->P0, n6, s->c_P0) == 0 &&
             dev_alloc(s->P1, n6, s->c_P1) == 0 && dev_alloc(s->recs, (size_t)kPcgThreads, s->c_recs) == 0 &&
             dev_alloc(s->ctl, 1, s->c_ctl) == 0 && dev_alloc(s->stop, 1, s->c_stop) == 0 &&
-            dev_alloc(s->d_res, 1, s->c_res) == 0;
+            dev_alloc(s->d_res, 1, s->c_res) == 0 && dev_alloc(s->d_tot, 1, s->c_tot) == 0;
   if (ok) ok = hipMemset(s->recs, 0, sizeof(SyncRec) * kPcgThreads) == hipSuccess &&
                hipMemset(s->ctl, 0, sizeof(PcgCtl)) == hipSuccess &&
                hipMemset(s->stop, 0, sizeof(int32_t)) == hipSuccess &&
-               hipMemset(s->d_res, 0, sizeof(nrf_pcg_result)) == hipSuccess;
+               hipMemset(s->d_res, 0, sizeof(nrf_pcg_result)) == hipSuccess &&
+               hipMemset(s->d_tot, 0, sizeof(nrf_solver_totals)) == hipSuccess;
   if (!ok) {
     nrf_solver_destroy(s);
     return nullptr;
@@ -1087,7 +1106,7 @@ extern "C" void nrf_solver_destroy(nrf_solver *s) {
   void *ptrs[] = {s->row_ptr, s->col_idx, s->diag_rel, s->e_i, s->e_j, s->es_off, s->es_entry, s->es_offrel,
                   s->rows, s->e_w, s->values, s->b, s->Minv, s->zpub, s->P0, s->P1, s->recs, s->ctl, s->lcol,
                   s->halo_off, s->halo,
-                  s->m_cnt, s->m_off, s->m_cur, s->m_keys, s->m_keys2, s->stop, s->d_res};
+                  s->m_cnt, s->m_off, s->m_cur, s->m_keys, s->m_keys2, s->stop, s->d_res, s->d_tot};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
@@ -1115,7 +1134,7 @@ extern "C" int nrf_solver_solve(nrf_solver *s, const nrf_corr *corrs, int num_co
   int rc;
   if ((rc = solver_ensure_pattern(s, nodes, num_nodes, st))) return rc;
   if ((rc = solver_build_memberships(s, corrs, num_corrs, num_nodes, st))) return rc;
-  gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res);
+  gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res, s->d_tot);
   NRF_LAUNCH_CHECK();
   if ((rc = solver_assemble(s, corrs, nullptr, nodes, reinterpret_cast<const DQ *>(transforms), nullptr,
                             num_nodes, st)))
@@ -1135,7 +1154,7 @@ extern "C" int nrf_solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const f
   hipStream_t st = as_stream(stream);
   int rc;
   if ((rc = solver_build_memberships(s, corrs, num_corrs, num_nodes, st))) return rc;
-  gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res);
+  gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res, s->d_tot);
   NRF_LAUNCH_CHECK();
   sink_record(s, st);
   GnArgs g;
@@ -1179,3 +1198,17 @@ extern "C" int nrf_solver_system(const nrf_solver *s, const int32_t **row_ptr, c
 }
 
 extern "C" const nrf_pcg_result *nrf_solver_device_result(const nrf_solver *s) { return s ? s->d_res : nullptr; }
+
+extern "C" int nrf_solver_get_totals(nrf_solver *s, nrf_solver_totals *out, void *stream) {
+  if (!s || !out) return NRF_ERR_INVALID_ARGUMENT;
+  hipStream_t st = as_stream(stream);
+  nrf_pcg_result last;
+  NRF_HIP_CHECK(hipMemcpyAsync(out, s->d_tot, sizeof(nrf_solver_totals), hipMemcpyDeviceToHost, st));
+  NRF_HIP_CHECK(hipMemcpyAsync(&last, s->d_res, sizeof(nrf_pcg_result), hipMemcpyDeviceToHost, st));
+  NRF_HIP_CHECK(hipStreamSynchronize(st));
+  // the last solve has not been folded in yet
+  out->gn_done += last.gn_done;
+  out->pcg_iterations += last.total_iterations;
+  out->rejected += (last.flags & 4) ? 1 : 0;
+  return NRF_OK;
+}

@@ -234,11 +234,13 @@ class GaussNewtonSolver:
         debug: bool = False
         relative_tolerance: bool = False
         precond_fp64: bool = False
+        arap_consistent_sign: bool = False  # opt-in: ARAP rotation rows with the data term's sign
 
         def to_c(self) -> L.SolverParams:
             return L.solver_params(self.gn_iterations, self.pcg_iterations, self.pcg_tolerance, self.lambda_smooth,
                                    self.lambda_rot, self.lambda_damping, int(self.debug),
-                                   int(self.relative_tolerance), int(self.precond_fp64))
+                                   int(self.relative_tolerance), int(self.precond_fp64),
+                                   int(self.arap_consistent_sign))
 
     def __init__(self, p: "GaussNewtonSolver.Params", max_nodes: int):
         self.lib = L.load()

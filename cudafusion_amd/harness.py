@@ -61,7 +61,8 @@ def transform_errors(Ta, Tb):
 
 class Harness:
     def __init__(self, kind: str = "body", frames: int = 40, use_graph: bool = False, seed: int = 0,
-                 pcg_tolerance: float = 0.0, relative_tolerance: bool = False, precond_fp64: bool = False):
+                 pcg_tolerance: float = 0.0, relative_tolerance: bool = False, precond_fp64: bool = False,
+                 arap_consistent_sign: bool = False):
         self.kind = kind
         self.scene = S.make_scene(kind, frames, seed)
         self.w = self.scene.w
@@ -70,7 +71,8 @@ class Harness:
         self.vp = L.volume_params(w.dims, w.voxel_size, w.truncation, w.origin, w.decay_alpha, w.max_weight)
         self.cam = L.camera(w.fx, w.fy, w.cx, w.cy, w.width, w.height)
         self.sp = L.solver_params(w.gn_iterations, w.pcg_iterations, pcg_tolerance, w.lambda_smooth, 1.0,
-                                  w.lambda_damping, 0, int(relative_tolerance), int(precond_fp64))
+                                  w.lambda_damping, 0, int(relative_tolerance), int(precond_fp64),
+                                  int(arap_consistent_sign))
         self.gn = L.gn_params(w.gn_iterations, w.update_scale, w.max_update_rot, w.max_update_trans, 1,
                               w.max_dx_mean)
         self.use_graph = use_graph
@@ -140,6 +142,14 @@ class Harness:
         ptr = self.lib.nrf_solver_device_result(solver)
         raw = self._d2h(ptr, C.sizeof(L.PcgResult), np.uint8)
         return L.PcgResult.from_buffer_copy(raw.tobytes())
+
+    def solver_totals(self) -> L.SolverTotals:
+        """Running GN / PCG totals over every step so far (syncs)."""
+        out = L.SolverTotals()
+        solver = self.lib.nrf_pipeline_solver(self.pipe)
+        L.check(self.lib.nrf_solver_get_totals(solver, C.byref(out), torch.cuda.current_stream().cuda_stream),
+                "solver_totals")
+        return out
 
     def num_blocks(self) -> int:
         solver = self.lib.nrf_pipeline_solver(self.pipe)

@@ -109,6 +109,10 @@ typedef struct nrf_solver_params {
   /* options with no reference counterpart (0 = reference behaviour) */
   int32_t relative_tolerance; /* 1: stop when ||r|| < tol * ||b||          */
   int32_t precond_fp64;       /* 1: 6x6 Cholesky inverses computed in FP64 */
+  int32_t arap_consistent_sign; /* 1: ARAP rotation Jacobian rows with the sign of
+                                   d(omega x q)/d(omega), as the data term and the
+                                   exp map use; 0 keeps the reference's negated rows
+                                   (kernels/solver_kernels.cu:145-153) */
 } nrf_solver_params;
 
 /* Caller-side GN loop settings (include/fusion_app.h:72-119 names). */
@@ -282,6 +286,18 @@ int nrf_solver_system(const nrf_solver *s, const int32_t **row_ptr,
 /* Device pointer to the last nrf_pcg_result written by the solver (for
    stream-ordered readers that do not want a host sync). */
 const nrf_pcg_result *nrf_solver_device_result(const nrf_solver *s);
+
+/* Running totals over every solve since creation (no reference counterpart:
+   the reference prints per-iteration diagnostics instead).  Accumulated on
+   the device by the next solve's first kernel, so reading them costs no
+   per-step synchronisation; this call synchronises the stream. */
+typedef struct nrf_solver_totals {
+  int64_t solves;          /* nrf_solver_solve + nrf_solver_solve_gn calls */
+  int64_t gn_done;         /* GN steps applied */
+  int64_t pcg_iterations;  /* PCG iterations run */
+  int64_t rejected;        /* solves that ended on a rejected step */
+} nrf_solver_totals;
+int nrf_solver_get_totals(nrf_solver *s, nrf_solver_totals *out, void *stream);
 
 /* ------------------------------------------------------------- pipeline */
 /* The per-frame non-rigid step (include/fusion_app.h:253-669 with the

@@ -488,7 +488,8 @@ template <typename R> R huber(R r, R delta) {
 // corrections in slot order, then edges in (node, slot) order.
 template <typename R>
 void assemble(const nrf_corr *corrs, int n, const float *canonical, const nrf_node *nodes, const DQ<R> *T, int N,
-              R lam_s, R lam_d, const int32_t *rp, const int32_t *ci, R *A, R *b, R *Minv, int precond_fp64) {
+              R lam_s, R lam_d, const int32_t *rp, const int32_t *ci, R *A, R *b, R *Minv, int precond_fp64,
+              int consistent_sign) {
   const int nb = rp[N];
   std::fill(A, A + (long)nb * 36, (R)0);
   std::fill(b, b + (long)N * 6, (R)0);
@@ -547,7 +548,10 @@ void assemble(const nrf_corr *corrs, int n, const float *canonical, const nrf_no
       R rn = std::sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
       R lam = lam_s * nw * huber(rn, (R)0.05f) * sw;
       if (lam <= (R)1e-8f) continue;
-      auto jac = [](V3<R> q, R J[6][3]) {  // compute_arap_jacobian :140-165
+      // compute_arap_jacobian :140-165; consistent_sign negates the rotation
+      // rows so that they equal d(omega x q)/d(omega)
+      auto jac = [consistent_sign](V3<R> q, R J[6][3]) {
+        if (consistent_sign) q = V3<R>{-q.x, -q.y, -q.z};
         R t[6][3] = {{0, q.z, -q.y}, {-q.z, 0, q.x}, {q.y, -q.x, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
         std::memcpy(J, t, sizeof(t));
       };
@@ -777,7 +781,7 @@ int solve_gn(const nrf_corr *corrs, const float *canonical, int n, const nrf_nod
   int tot = 0, done = 0, li = 0;
   for (int it = 0; it < std::max(1, g.gn_iterations); ++it) {
     assemble(corrs, n, canonical, nodes, T, N, (R)sp.lambda_smooth, (R)sp.lambda_damping, rp.data(), ci.data(),
-             A.data(), b.data(), M.data(), sp.precond_fp64);
+             A.data(), b.data(), M.data(), sp.precond_fp64, sp.arap_consistent_sign);
     int iters = 0, flags = 0;
     pcg(rp.data(), ci.data(), A.data(), b.data(), M.data(), N, sp.pcg_iterations, (R)sp.pcg_tolerance,
         sp.relative_tolerance, x.data(), &iters, &flags, nullptr);
@@ -897,9 +901,9 @@ void orc_warp_normals_f64(const float *nin, const int32_t *ids, const float *ws,
 
 // Distance of each voxel's FP64 projection from the rounding boundaries of
 // the integration test (for parity checks of FP32 kernels): px = distance of
-// (u, v) from the nearest half-integer (where round-half-even flips), sdf =
-// |sdf + truncation| / truncation at the sampled pixel; +inf where the voxel
-// is not projected.
+// (u, v) from the nearest half-integer (where round-half-even flips), for
+// voxels projecting into the image or its 1-pixel border; sdf =
+// |sdf + truncation| / truncation at the sampled pixel; +inf elsewhere.
 void orc_integrate_margin_f64(const nrf_volume_params *vp, const float *depth, const nrf_camera *cam,
                               const float pose[16], const nrf_node *nodes, const double *T8, int N,
                               const int32_t *knn, const float *knn_w, double *m_px, double *m_sdf) {
@@ -925,7 +929,7 @@ void orc_integrate_margin_f64(const nrf_volume_params *vp, const float *depth, c
         double cz = Tc[2][0] * p.x + Tc[2][1] * p.y + Tc[2][2] * p.z + Tc[2][3];
         if (cz <= 0) continue;
         double fu = cam->fx * cx / cz + cam->cx, fv = cam->fy * cy / cz + cam->cy;
-        if (!(fu > -1e9 && fu < 1e9 && fv > -1e9 && fv < 1e9)) continue;
+        if (!(fu > -1.5 && fu < cam->width + 0.5 && fv > -1.5 && fv < cam->height + 0.5)) continue;
         m_px[idx] = std::min(std::fabs(fu - std::floor(fu) - 0.5), std::fabs(fv - std::floor(fv) - 0.5));
         int u = (int)std::nearbyint(fu), v = (int)std::nearbyint(fv);
         if (u < 0 || u >= cam->width || v < 0 || v >= cam->height) continue;
@@ -993,20 +997,23 @@ int orc_pattern(const nrf_node *nodes, int N, int32_t *row_ptr, int32_t *col_idx
 
 int orc_assemble_f32(const nrf_corr *corrs, int n, const float *canonical, const nrf_node *nodes,
                      const nrf_dualquat *Tq, int N, float lam_s, float lam_d, int precond_fp64, float *A, float *b,
-                     float *M) {
+                     float *M, int consistent_sign) {
   std::vector<int32_t> rp, ci;
   pattern(nodes, N, rp, ci);
   auto T = load_T<float>(Tq, N);
-  assemble(corrs, n, canonical, nodes, T.data(), N, lam_s, lam_d, rp.data(), ci.data(), A, b, M, precond_fp64);
+  assemble(corrs, n, canonical, nodes, T.data(), N, lam_s, lam_d, rp.data(), ci.data(), A, b, M, precond_fp64,
+           consistent_sign);
   return rp[N];
 }
 int orc_assemble_f64(const nrf_corr *corrs, int n, const float *canonical, const nrf_node *nodes,
-                     const double *T8, int N, double lam_s, double lam_d, double *A, double *b, double *M) {
+                     const double *T8, int N, double lam_s, double lam_d, double *A, double *b, double *M,
+                     int consistent_sign) {
   std::vector<int32_t> rp, ci;
   pattern(nodes, N, rp, ci);
   std::vector<DQ<double>> T(N);
   for (int i = 0; i < N; ++i) T[i] = load_dq8(T8 + 8 * i);
-  assemble(corrs, n, canonical, nodes, T.data(), N, lam_s, lam_d, rp.data(), ci.data(), A, b, M, 1);
+  assemble(corrs, n, canonical, nodes, T.data(), N, lam_s, lam_d, rp.data(), ci.data(), A, b, M, 1,
+           consistent_sign);
   return rp[N];
 }
 

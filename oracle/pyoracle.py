@@ -45,8 +45,8 @@ _SIGS = {
     "orc_knn_f32": (None, [P, P, P, I, P, P]),
     "orc_knn_f64": (None, [P, P, P, I, P, P]),
     "orc_pattern": (I, [P, I, P, P, I]),
-    "orc_assemble_f32": (I, [P, I, P, P, P, I, F, F, I, P, P, P]),
-    "orc_assemble_f64": (I, [P, I, P, P, P, I, D, D, P, P, P]),
+    "orc_assemble_f32": (I, [P, I, P, P, P, I, F, F, I, P, P, P, I]),
+    "orc_assemble_f64": (I, [P, I, P, P, P, I, D, D, P, P, P, I]),
     "orc_pcg_f32": (D, [P, P, P, P, P, I, I, F, I, P, P, P, P]),
     "orc_pcg_f64": (D, [P, P, P, P, P, I, I, D, I, P, P, P, P]),
     "orc_spmv_f64": (None, [P, P, P, P, P, I]),
@@ -139,14 +139,19 @@ def tsdf_fp64_errors(gpu_vox, t64, w64, margins, px_eps=2e-3, sdf_eps=1e-4):
     """Compares an FP32 device TSDF with the FP64 oracle on the voxels that
     are not within a rounding margin of a projection or truncation boundary.
     Returns (max |tsdf error| on unambiguous voxels, fraction of ambiguous
-    voxels, count of unambiguous weight mismatches)."""
+    voxels among those projecting into the image, count of unambiguous
+    weight mismatches).  The voxel grid puts a few percent of projections
+    exactly on half-pixels (rational coincidences such as 262.5 * 0.075 /
+    0.625 = 31.5), where FP32 and FP64 may round to different pixels."""
     m_px, m_sdf = margins
     ambiguous = (m_px < px_eps) | (m_sdf < sdf_eps)
+    in_image = np.isfinite(m_px)
     ok = ~ambiguous
     wmis = int(((gpu_vox["weight"] != w64.astype(np.float32)) & ok).sum())
     both = ok & (gpu_vox["weight"] == w64.astype(np.float32))
     err = float(np.abs(gpu_vox["tsdf"][both] - t64[both]).max()) if both.any() else 0.0
-    return err, float(ambiguous.mean()), wmis
+    amb = float((ambiguous & in_image).sum() / max(1, int(in_image.sum())))
+    return err, amb, wmis
 
 
 def mc_f32(vp, vox, normals=True):
@@ -189,7 +194,7 @@ def pattern(nodes):
     return rp, ci[:nb]
 
 
-def assemble_f32(corrs, nodes, T, lam_s, lam_d, canonical=None, precond_fp64=0):
+def assemble_f32(corrs, nodes, T, lam_s, lam_d, canonical=None, precond_fp64=0, consistent_sign=0):
     N = len(nodes)
     rp, ci = pattern(nodes)
     nb = len(ci)
@@ -197,11 +202,11 @@ def assemble_f32(corrs, nodes, T, lam_s, lam_d, canonical=None, precond_fp64=0):
     b = np.zeros(N * 6, np.float32)
     M = np.zeros(N * 36, np.float32)
     load().orc_assemble_f32(ptr(corrs), len(corrs), ptr(canonical), ptr(nodes), ptr(T), N, lam_s, lam_d,
-                            precond_fp64, ptr(A), ptr(b), ptr(M))
+                            precond_fp64, ptr(A), ptr(b), ptr(M), consistent_sign)
     return rp, ci, A.reshape(nb, 6, 6), b, M.reshape(N, 6, 6)
 
 
-def assemble_f64(corrs, nodes, T64, lam_s, lam_d, canonical=None):
+def assemble_f64(corrs, nodes, T64, lam_s, lam_d, canonical=None, consistent_sign=0):
     N = len(nodes)
     rp, ci = pattern(nodes)
     nb = len(ci)
@@ -209,7 +214,7 @@ def assemble_f64(corrs, nodes, T64, lam_s, lam_d, canonical=None):
     b = np.zeros(N * 6)
     M = np.zeros(N * 36)
     load().orc_assemble_f64(ptr(corrs), len(corrs), ptr(canonical), ptr(nodes), ptr(T64), N, lam_s, lam_d, ptr(A),
-                            ptr(b), ptr(M))
+                            ptr(b), ptr(M), consistent_sign)
     return rp, ci, A.reshape(nb, 6, 6), b, M.reshape(N, 6, 6)
 
 

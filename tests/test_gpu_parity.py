@@ -88,7 +88,7 @@ def test_integrate_rigid(sphere, orc):
     # pixel in FP32; such voxels are identified from the FP64 projection
     err, amb, wmis = orc.tsdf_fp64_errors(gpu, t64, w64, orc.integrate_margin_f64(vol.vp, depth, cam, pose))
     print(f"[integrate] FP64 max err {err:.2e}, ambiguous {amb:.2e}, weight mismatches {wmis}", flush=True)
-    assert wmis == 0 and amb < 1e-3 and err < 1e-3
+    assert wmis == 0 and amb < 0.05 and err < 1e-3
 
 
 def test_voxel_knn_and_warped_integration(sphere, orc):
@@ -136,7 +136,7 @@ def test_voxel_knn_and_warped_integration(sphere, orc):
     err, amb, wmis = orc.tsdf_fp64_errors(g1, t64, w64,
                                           orc.integrate_margin_f64(vol.vp, d1, cam, pose, nodes, T64, rk, rw))
     print(f"[warped] FP64 max err {err:.2e}, ambiguous {amb:.2e}, weight mismatches {wmis}", flush=True)
-    assert wmis == 0 and amb < 1e-3 and err < 1e-3
+    assert wmis == 0 and amb < 0.05 and err < 1e-3
 
 
 def test_marching_cubes(sphere, orc):
@@ -269,8 +269,10 @@ def solver_inputs(sphere, orc):
     return dict(wf=wf, dcorr=dcorr, dcan=dcan, hcorr=hcorr, hcan=hcan)
 
 
-def test_solver_system_and_pcg(sphere, orc, solver_inputs):
-    """solve(): one linearisation with the stored src (GaussNewtonSolver::solve)."""
+@pytest.mark.parametrize("consistent", [0, 1])
+def test_solver_system_and_pcg(sphere, orc, solver_inputs, consistent):
+    """solve(): one linearisation with the stored src (GaussNewtonSolver::solve),
+    in the reference's ARAP sign convention and in the opt-in consistent one."""
     import torch
 
     from cudafusion_amd import _lib as L
@@ -292,13 +294,15 @@ def test_solver_system_and_pcg(sphere, orc, solver_inputs):
                                    Tc.ctypes.data, N, src.ctypes.data)
     corrs["src"] = src.reshape(-1, 3)
     dcorr = torch.from_numpy(corrs.view(np.uint8).copy()).cuda()
-    p = GaussNewtonSolver.Params(pcg_iterations=20, pcg_tolerance=0.0, lambda_smooth=w.lambda_smooth)
+    p = GaussNewtonSolver.Params(pcg_iterations=20, pcg_tolerance=0.0, lambda_smooth=w.lambda_smooth,
+                                 arap_consistent_sign=bool(consistent))
     s = GaussNewtonSolver(p, 8192)
     dx = torch.zeros(6 * N, dtype=torch.float32, device="cuda")
     res = s.solve(dcorr, len(corrs), wf.device_nodes(), wf.device_transforms(), N, dx)
     assert res.iterations == 20 and not (res.flags & 8)
     rp_p, ci_p, va_p, nb, rhs_p, pc_p = s.system_matrix()
-    rp, ci, A, b, M = orc.assemble_f32(corrs, nodes, T, w.lambda_smooth, w.lambda_damping)
+    rp, ci, A, b, M = orc.assemble_f32(corrs, nodes, T, w.lambda_smooth, w.lambda_damping,
+                                       consistent_sign=consistent)
     assert nb == len(ci)
     np.testing.assert_array_equal(copy_device_to_numpy(rp_p, N + 1, np.int32), rp)
     np.testing.assert_array_equal(copy_device_to_numpy(ci_p, nb, np.int32), ci)
@@ -308,7 +312,8 @@ def test_solver_system_and_pcg(sphere, orc, solver_inputs):
     scale = np.abs(A).max()
     np.testing.assert_allclose(gA, A, atol=2e-6 * scale)
     np.testing.assert_allclose(gb, b, atol=2e-6 * np.abs(b).max())
-    _, _, A64, b64, M64 = orc.assemble_f64(corrs, nodes, T64, w.lambda_smooth, w.lambda_damping)
+    _, _, A64, b64, M64 = orc.assemble_f64(corrs, nodes, T64, w.lambda_smooth, w.lambda_damping,
+                                           consistent_sign=consistent)
     np.testing.assert_allclose(gA, A64, atol=1e-5 * scale)
     np.testing.assert_allclose(gM, M64, rtol=1e-3, atol=1e-4 * np.abs(M64).max())
     # PCG: 20 iterations from x0 = 0 on the same system, FP64 oracle
@@ -318,9 +323,11 @@ def test_solver_system_and_pcg(sphere, orc, solver_inputs):
     assert abs(res.final_residual - r64) <= 1e-2 * r64 + 1e-9
 
 
-def test_solve_gn_transforms(sphere, orc, solver_inputs):
+@pytest.mark.parametrize("consistent", [0, 1])
+def test_solve_gn_transforms(sphere, orc, solver_inputs, consistent):
     """3 GN x 20 PCG with re-warped src and the reference caller's step
-    policy (fusion_app.h:72-119 defaults): node transforms vs the FP64 oracle."""
+    policy (fusion_app.h:72-119 defaults): node transforms vs the FP64 oracle,
+    in both ARAP sign conventions."""
     import torch
 
     from cudafusion_amd import _lib as L
@@ -331,7 +338,8 @@ def test_solve_gn_transforms(sphere, orc, solver_inputs):
     N = len(nodes)
     wf = solver_inputs["wf"]
     wf.reset_transforms()
-    p = GaussNewtonSolver.Params(pcg_iterations=20, pcg_tolerance=0.0, lambda_smooth=w.lambda_smooth)
+    p = GaussNewtonSolver.Params(pcg_iterations=20, pcg_tolerance=0.0, lambda_smooth=w.lambda_smooth,
+                                 arap_consistent_sign=bool(consistent))
     s = GaussNewtonSolver(p, 8192)
     s.set_pattern(nodes)
     gn = L.gn_params(3, 0.25, 0.05, 0.03, 1, 0.03)
@@ -340,7 +348,7 @@ def test_solve_gn_transforms(sphere, orc, solver_inputs):
     res = s.solve_gn(solver_inputs["dcorr"], solver_inputs["dcan"], len(solver_inputs["hcorr"]), wf.device_nodes(),
                      wf.device_transforms(), cen, N, gn, dx)
     Tg = wf.download_transforms()
-    sp = L.solver_params(3, 20, 0.0, w.lambda_smooth, 1.0, w.lambda_damping)
+    sp = L.solver_params(3, 20, 0.0, w.lambda_smooth, 1.0, w.lambda_damping, arap_consistent_sign=consistent)
     T64, _, tot, done, _ = orc.solve_gn_f64(solver_inputs["hcorr"], solver_inputs["hcan"], nodes,
                                             orc.identity_dq64(N), gn, sp)
     print(f"[gn] gpu gn_done={res.gn_done} iters={res.total_iterations} oracle gn_done={done} iters={tot}", flush=True)

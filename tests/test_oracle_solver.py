@@ -98,8 +98,9 @@ def _huber(r, d):
     return 1.0 if abs(r) <= d else d / abs(r)
 
 
-def _brute_force(orc, nodes, corrs, T, lam_s, lam_d):
-    """Dense normal equations from finite-difference Jacobians."""
+def _brute_force(orc, nodes, corrs, T, lam_s, lam_d, consistent_sign=0):
+    """Dense normal equations from finite-difference Jacobians.  Unless
+    consistent_sign, the ARAP rotation columns are negated (Q1)."""
     N = len(nodes)
     P = nodes["pos"].astype(np.float64)
     rp, ci = orc.pattern(nodes)
@@ -175,7 +176,7 @@ def _brute_force(orc, nodes, corrs, T, lam_s, lam_d):
                         else:
                             rr.append(Tij - _xf_c(orc, Tn, xj, xj))
                     col = (rr[0] - rr[1]) / (2 * eps)
-                    if a < 3:
+                    if a < 3 and not consistent_sign:
                         col = -col  # Q1: the reference's rotation Jacobian has the opposite sign
                     Jfull[:, 6 * node + a] = col
             A += lam * Jfull.T @ Jfull
@@ -209,8 +210,9 @@ def test_pattern_symmetrised_sorted(orc):
         assert list(ci[rp[i]:rp[i + 1]]) == sorted(exp[i])
 
 
+@pytest.mark.parametrize("consistent", [0, 1])
 @pytest.mark.parametrize("moved", [False, True])
-def test_assembly_equals_finite_difference_normal_equations(orc, moved):
+def test_assembly_equals_finite_difference_normal_equations(orc, moved, consistent):
     nodes = _graph()
     corrs = _corrs(nodes)
     N = len(nodes)
@@ -220,9 +222,9 @@ def test_assembly_equals_finite_difference_normal_equations(orc, moved):
     if moved:
         for i in range(N):
             T[i] = _twist(orc, np.concatenate([rng.normal(0, 0.03, 3), rng.normal(0, 0.004, 3)]), T[i])
-    rp, ci, A, b, M = orc.assemble_f64(corrs, nodes, T, 10.0, 1e-2)
+    rp, ci, A, b, M = orc.assemble_f64(corrs, nodes, T, 10.0, 1e-2, consistent_sign=consistent)
     Ad = _bsr_to_dense(rp, ci, A, N)
-    Ab, bb = _brute_force(orc, nodes, corrs, T, 10.0, 1e-2)
+    Ab, bb = _brute_force(orc, nodes, corrs, T, 10.0, 1e-2, consistent)
     scale = np.abs(Ab).max()
     np.testing.assert_allclose(Ad, Ab, atol=1e-7 * scale)
     np.testing.assert_allclose(b, bb, atol=1e-7 * max(np.abs(bb).max(), 1e-12))
@@ -230,8 +232,14 @@ def test_assembly_equals_finite_difference_normal_equations(orc, moved):
         assert np.abs(bb).max() > 1e-6  # the ARAP residuals contribute to b
     # FP32 restatement agrees with FP64 to single precision
     T32 = T.astype(np.float32).view(L.DUALQUAT).reshape(-1)
-    _, _, A32, b32, _ = orc.assemble_f32(corrs, nodes, T32, 10.0, 1e-2)
+    _, _, A32, b32, _ = orc.assemble_f32(corrs, nodes, T32, 10.0, 1e-2, consistent_sign=consistent)
     np.testing.assert_allclose(A32, A, atol=1e-5 * scale)
+    if moved:
+        # the two modes differ exactly in the sign of the ARAP rotation rows:
+        # rot-rot and trans-trans blocks agree, b differs in the rotation part
+        _, _, A_o, b_o, _ = orc.assemble_f64(corrs, nodes, T, 10.0, 1e-2, consistent_sign=1 - consistent)
+        np.testing.assert_allclose(A_o[:, 3:, 3:], A[:, 3:, 3:], atol=1e-12 * scale)
+        assert np.abs(b_o.reshape(-1, 6)[:, :3] - b.reshape(-1, 6)[:, :3]).max() > 1e-9
 
 
 def test_block_jacobi_inverse_and_fallback(orc):
