@@ -390,6 +390,35 @@ __global__ __launch_bounds__(64) void assemble_rows_kernel(AsmArgs a) {
 }
 
 // ----------------------------------------------------------------- PCG
+// Cross-workgroup traffic of the persistent PCG never relies on the L2: each
+// XCD has its own L2, and an agent-scope fence costs a write-back
+// (buffer_wbl2) plus an invalidation (buffer_inv) of a whole L2 per
+// workgroup per synchronisation point.  Instead every value another
+// workgroup reads (published z / p, all-reduce partials, generations) is
+// written and read with agent-scope relaxed atomic accesses, which gfx950
+// performs coherently at the device level (sc1), and a workgroup orders its
+// data before its generation by waiting for its own stores to complete
+// (s_waitcnt vmcnt(0)) before the barrier that precedes the generation store.
+__device__ inline void st_agent(float *p, float v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_agent(double *p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline float ld_agent(const float *p) {
+  return __hip_atomic_load(const_cast<float *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double ld_agent(const double *p) {
+  return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Waits until every store this wave issued has completed at its scope; the
+// compiler barriers keep the surrounding accesses on their side of the wait.
+__device__ inline void wait_own_stores() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // One 64-byte record per workgroup: its partial sums of the current and the
 // previous synchronisation point and the generation it has reached.  A
 // synchronisation point is an all-reduce: every workgroup publishes its
@@ -440,35 +469,38 @@ struct PcgArgs {
 // when the bounded wait ran out (then every workgroup leaves the solve).
 __device__ inline bool allreduce2(const PcgArgs &a, unsigned long long target, int slot, double &v0, double &v1,
                                   double *red0, double *red1, int *s_abort) {
-  v0 = block_sum(v0, red0);
+  wait_own_stores();  // this wave's published vector elements are device-visible
+  v0 = block_sum(v0, red0);  // (its barriers: every wave has waited)
   v1 = block_sum(v1, red1);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __syncthreads();
   if (threadIdx.x == 0) {
     SyncRec *me = a.recs + blockIdx.x;
-    me->v[slot][0] = v0;
-    me->v[slot][1] = v1;
-    __hip_atomic_store(&me->gen, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    st_agent(&me->v[slot][0], v0);
+    st_agent(&me->v[slot][1], v1);
+    wait_own_stores();
+    __hip_atomic_store(&me->gen, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (threadIdx.x < a.G) {
     const SyncRec *o = a.recs + threadIdx.x;
     unsigned spins = 0;
     while (__hip_atomic_load(&o->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (__hip_atomic_load(&a.ctl->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-      if (++spins > kBarrierSpinLimit) {
-        __hip_atomic_store(&a.ctl->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
+      if ((++spins & 255u) == 0) {
+        if (__hip_atomic_load(&a.ctl->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        if (spins > kBarrierSpinLimit) {
+          __hip_atomic_store(&a.ctl->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
       }
       __builtin_amdgcn_s_sleep(1);
     }
   }
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
   if (threadIdx.x == 0) *s_abort = __hip_atomic_load(&a.ctl->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   double g0 = 0, g1 = 0;
   if (threadIdx.x < a.G) {
-    g0 = a.recs[threadIdx.x].v[slot][0];
-    g1 = a.recs[threadIdx.x].v[slot][1];
+    g0 = ld_agent(&a.recs[threadIdx.x].v[slot][0]);
+    g1 = ld_agent(&a.recs[threadIdx.x].v[slot][1]);
   }
   v0 = block_sum(g0, red0);  // block_sum's barriers also publish *s_abort
   v1 = block_sum(g1, red1);
@@ -530,7 +562,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
   const unsigned long long base = s_base;
   if (own) {
     for (int c = 0; c < 6; ++c) z += M[c] * rl[li * 6 + c];
-    a.zpub[s] = z;
+    st_agent(a.zpub + s, z);
   }
   double rz = own ? (double)r * (double)z : 0.0, bb = own ? (double)bv * (double)bv : 0.0;
   unsigned long long pt = 1;
@@ -545,11 +577,11 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
     // A: halo p into LDS, Ap, p.Ap
     for (int e = t; e < nh * 6; e += kPcgThreads) {
       const int idx = 6 * a.halo[h0 + e / 6] + e % 6;
-      const float zv = a.zpub[idx];
-      ph[e] = (it == 0) ? zv : zv + betaf * Pold[idx];
+      const float zv = ld_agent(a.zpub + idx);
+      ph[e] = (it == 0) ? zv : zv + betaf * ld_agent(Pold + idx);
     }
     p = (it == 0) ? z : z + betaf * p;
-    if (own) Pnew[s] = p;
+    if (own) st_agent(Pnew + s, p);
     __syncthreads();
     float y = 0.f;
     if (own) {
@@ -577,7 +609,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
     z = 0.f;
     if (own) {
       for (int c = 0; c < 6; ++c) z += M[c] * rl[li * 6 + c];
-      a.zpub[s] = z;
+      st_agent(a.zpub + s, z);
     }
     double rr_l = own ? (double)r * (double)r : 0.0, rz_l = own ? (double)r * (double)z : 0.0;
     ++pt;
