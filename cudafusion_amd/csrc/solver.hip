@@ -37,7 +37,7 @@
 namespace nrf {
 namespace {
 
-constexpr int kPcgThreads = 256;
+constexpr int kPcgThreads = 1024;
 constexpr unsigned kBarrierSpinLimit = 1u << 22;
 
 // ---------------------------------------------------------------- helpers
@@ -390,70 +390,62 @@ __global__ __launch_bounds__(64) void assemble_rows_kernel(AsmArgs a) {
 }
 
 // ----------------------------------------------------------------- PCG
-// Cross-workgroup traffic of the persistent PCG never relies on the L2: each
-// XCD has its own L2, and an agent-scope fence costs a write-back
-// (buffer_wbl2) plus an invalidation (buffer_inv) of a whole L2 per
-// workgroup per synchronisation point.  Instead every value another
-// workgroup reads (published z / p, all-reduce partials, generations) is
-// written and read with agent-scope relaxed atomic accesses, which gfx950
-// performs coherently at the device level (sc1), and a workgroup orders its
-// data before its generation by waiting for its own stores to complete
-// (s_waitcnt vmcnt(0)) before the barrier that precedes the generation store.
-__device__ inline void st_agent(float *p, float v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void st_agent(double *p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline float ld_agent(const float *p) {
-  return __hip_atomic_load(const_cast<float *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline double ld_agent(const double *p) {
-  return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// Waits until every store this wave issued has completed at its scope; the
-// compiler barriers keep the surrounding accesses on their side of the wait.
-__device__ inline void wait_own_stores() {
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __builtin_amdgcn_s_waitcnt(0);
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-}
+// Cross-workgroup traffic of the persistent PCG uses a flag-in-data
+// protocol and never relies on the L2 (each XCD has its own, and an
+// agent-scope fence costs a write-back and an invalidation of a whole L2):
+// every value another workgroup reads travels in a 64-bit word
+// (tag << 32 | 32-bit payload) written and read with agent-scope relaxed
+// atomic accesses, which gfx950 performs coherently at the device level.  A
+// 64-bit access is single-copy atomic, so a reader that sees the expected tag
+// also sees the payload written with it: no store ordering, no fences and no
+// separate generation counters are needed, and a reader waits only for the
+// words it uses.  The tag of a word is the synchronisation point it belongs
+// to, (launch epoch << 16) + point, compared for equality.
+typedef unsigned long long ll_word;
 
-// One 64-byte record per workgroup: its partial sums of the current and the
-// previous synchronisation point and the generation it has reached.  A
-// synchronisation point is an all-reduce: every workgroup publishes its
-// partials and generation, then lanes 0..G-1 each wait for one record and
-// the partials are summed in a fixed tree, so all workgroups hold the same
-// bits.  Values are double-buffered by the parity of the point: a workgroup
-// can be at most one point ahead of the slowest one.
 struct alignas(64) SyncRec {
-  double v[2][2];
-  unsigned long long gen;
-  unsigned long long pad[3];
+  ll_word w[2][4];  // per parity: hi / lo halves of the two partial sums
 };
 struct PcgCtl {
   unsigned long long epoch;  // advanced once per launch by pcg_prep_kernel
   unsigned abort;
   unsigned pad;
 };
-constexpr int kPcgMaxNodesPerWg = 42;  // 6 x 42 = 252 scalar rows <= 256 lanes
+constexpr int kPcgSub = 4;                        // lanes per scalar row
+constexpr int kPcgRows = kPcgThreads / kPcgSub;   // scalar rows per workgroup
+constexpr int kPcgMaxNodesPerWg = kPcgRows / 6;   // 42 nodes = 252 rows
+constexpr int kPcgMaxWg = 256;                    // records (one per workgroup)
+constexpr int kPcgMaxPoints = 65535;              // synchronisation points per launch (16 tag bits)
+constexpr int kPcgWaves = kPcgThreads / 64;
 
-__device__ inline double block_sum(double v, double *red) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+// Sums a pair of doubles over the workgroup in a fixed order (xor tree in
+// each wave, then the waves in index order); every lane gets the result.
+__device__ inline void block_sum2(double &v0, double &v1, double *red) {
+  for (int off = 32; off > 0; off >>= 1) {
+    v0 += __shfl_xor(v0, off, 64);
+    v1 += __shfl_xor(v1, off, 64);
+  }
   const int w = threadIdx.x >> 6;
   __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[w] = v;
+  if ((threadIdx.x & 63) == 0) {
+    red[2 * w] = v0;
+    red[2 * w + 1] = v1;
+  }
   __syncthreads();
-  double s = 0;
-  for (int k = 0; k < kPcgThreads / 64; ++k) s += red[k];
-  return s;
+  double s0 = 0, s1 = 0;
+  for (int k = 0; k < kPcgWaves; ++k) {
+    s0 += red[2 * k];
+    s1 += red[2 * k + 1];
+  }
+  v0 = s0;
+  v1 = s1;
 }
 
 struct PcgArgs {
   const int32_t *row_ptr, *lcol;
   const float *values, *b, *Minv;
   float *x;
-  float *zpub, *ppub0, *ppub1;
+  ll_word *zpub, *ppub0, *ppub1;
   SyncRec *recs;
   PcgCtl *ctl;
   const int32_t *rows, *halo_off, *halo;
@@ -464,66 +456,84 @@ struct PcgArgs {
   const int32_t *stop;
 };
 
-// All-reduce of two doubles across the G workgroups; also the point after
-// which every vector element published before it is visible.  Returns true
-// when the bounded wait ran out (then every workgroup leaves the solve).
-__device__ inline bool allreduce2(const PcgArgs &a, unsigned long long target, int slot, double &v0, double &v1,
-                                  double *red0, double *red1, int *s_abort) {
-  wait_own_stores();  // this wave's published vector elements are device-visible
-  v0 = block_sum(v0, red0);  // (its barriers: every wave has waited)
-  v1 = block_sum(v1, red1);
-  if (threadIdx.x == 0) {
-    SyncRec *me = a.recs + blockIdx.x;
-    st_agent(&me->v[slot][0], v0);
-    st_agent(&me->v[slot][1], v1);
-    wait_own_stores();
-    __hip_atomic_store(&me->gen, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  if (threadIdx.x < a.G) {
-    const SyncRec *o = a.recs + threadIdx.x;
-    unsigned spins = 0;
-    while (__hip_atomic_load(&o->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if ((++spins & 255u) == 0) {
-        if (__hip_atomic_load(&a.ctl->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-        if (spins > kBarrierSpinLimit) {
-          __hip_atomic_store(&a.ctl->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-      }
-      __builtin_amdgcn_s_sleep(1);
+__device__ inline void ll_put(ll_word *p, unsigned tag, unsigned payload) {
+  __hip_atomic_store(p, ((ll_word)tag << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline ll_word ll_load(const ll_word *p) {
+  return __hip_atomic_load(const_cast<ll_word *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline bool ll_has(ll_word w, unsigned tag) { return (unsigned)(w >> 32) == tag; }
+
+// Bounded wait of the polling loops: every 256 rounds the launch-wide abort
+// flag is checked, and raised when the spin limit ran out.  Returns true when
+// the caller has to give up (and then sets *bad).
+__device__ inline bool ll_spin(unsigned &spins, PcgCtl *ctl, int *bad) {
+  if ((++spins & 255u) == 0) {
+    if (__hip_atomic_load(&ctl->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || spins > kBarrierSpinLimit) {
+      __hip_atomic_store(&ctl->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      *bad = 1;
+      return true;
     }
   }
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __syncthreads();
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  if (threadIdx.x == 0) *s_abort = __hip_atomic_load(&a.ctl->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  double g0 = 0, g1 = 0;
-  if (threadIdx.x < a.G) {
-    g0 = ld_agent(&a.recs[threadIdx.x].v[slot][0]);
-    g1 = ld_agent(&a.recs[threadIdx.x].v[slot][1]);
+  __builtin_amdgcn_s_sleep(1);
+  return false;
+}
+
+// All-reduce of two doubles across the G workgroups: every workgroup
+// publishes its partials under `tag`, lanes 0..G-1 each read one workgroup's
+// record (its four words in one round trip) and the block sums them in lane
+// order, so every workgroup holds the same bits.  Records are double-buffered
+// by the parity of the point: a workgroup is at most one point ahead of the
+// slowest one.  Returns true when the solve has to be abandoned.
+__device__ inline bool allreduce2(const PcgArgs &a, unsigned tag, int slot, double &v0, double &v1, double *red,
+                                  int *s_bad) {
+  block_sum2(v0, v1, red);
+  const int t = threadIdx.x;
+  if (t < 4) {
+    const ll_word bits = (ll_word)__double_as_longlong(t < 2 ? v0 : v1);
+    ll_put(&a.recs[blockIdx.x].w[slot][t], tag, (t & 1) ? (unsigned)bits : (unsigned)(bits >> 32));
   }
-  v0 = block_sum(g0, red0);  // block_sum's barriers also publish *s_abort
-  v1 = block_sum(g1, red1);
-  return *s_abort != 0;
+  double g0 = 0, g1 = 0;
+  if (t < a.G) {
+    const ll_word *w = a.recs[t].w[slot];
+    ll_word w0, w1, w2, w3;
+    unsigned spins = 0;
+    for (;;) {
+      w0 = ll_load(w + 0);
+      w1 = ll_load(w + 1);
+      w2 = ll_load(w + 2);
+      w3 = ll_load(w + 3);
+      if (ll_has(w0, tag) && ll_has(w1, tag) && ll_has(w2, tag) && ll_has(w3, tag)) break;
+      if (ll_spin(spins, a.ctl, s_bad)) break;
+    }
+    g0 = __longlong_as_double((long long)((w0 << 32) | (w1 & 0xffffffffull)));
+    g1 = __longlong_as_double((long long)((w2 << 32) | (w3 & 0xffffffffull)));
+  }
+  block_sum2(g0, g1, red);  // its barriers also publish *s_bad
+  v0 = g0;
+  v1 = g1;
+  return *s_bad != 0;
 }
 
 // Block-Jacobi PCG (kernels/solver_kernels.cu:616-687, Saad Alg. 9.1, x0 = 0)
 // as one persistent kernel.  Workgroup g owns nodes [rows[g], rows[g+1]),
-// at most 42, so lane t holds scalar row 6 rows[g] + t and keeps x, r, z, p
-// and Ap of that row in registers for the whole solve.  The blocks of the
-// owned rows (and their columns, renumbered into the workgroup's halo
-// list) are copied to LDS once.  Per iteration:
-//   A: gather p_j = z_j + beta p_j of the halo nodes from the published
-//      vectors into LDS, Ap from LDS, all-reduce p.Ap;
-//   B: x += alpha p, r -= alpha Ap, z = M^-1 r, publish z, all-reduce
+// at most 42, i.e. at most 252 scalar rows; four adjacent lanes share scalar
+// row q = t / 4, hold identical copies of its x, r, z, p, M^-1 row in
+// registers for the whole solve and split the row's blocks among themselves
+// in the product.  The blocks of the owned rows (and their columns,
+// renumbered into the workgroup's halo list) are copied to LDS once.  Per
+// iteration:
+//   A: p = z + beta p; the halo's p_j = z_j + beta p_j: own nodes from LDS,
+//      the others from the published words; Ap from LDS; all-reduce p.Ap;
+//   B: x += alpha p, r -= alpha Ap, z = M^-1 r, publish z; all-reduce
 //      (r.r, r.z).
 // Every workgroup evaluates the same expressions on the same published
 // bits, so the owner's p and its readers' copies are identical.
 __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
   extern __shared__ float lds[];
-  __shared__ double red0[kPcgThreads / 64], red1[kPcgThreads / 64];
-  __shared__ int s_abort;
-  __shared__ unsigned long long s_base;
+  __shared__ double red[2 * kPcgWaves];
+  __shared__ int s_bad;
+  __shared__ unsigned s_base;
   if (a.stop && *a.stop) return;
   const int g = blockIdx.x, t = threadIdx.x;
   const int n0 = a.rows[g], n1 = a.rows[g + 1];
@@ -533,7 +543,8 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
   float *Al = lds;
   int32_t *Ll = reinterpret_cast<int32_t *>(Al + (a.lds_a ? (size_t)nblk * 36 : 0));
   float *ph = reinterpret_cast<float *>(Ll + (a.lds_a ? nblk : 0));
-  float *rl = ph + (size_t)nh * 6;
+  float *rl = ph + (size_t)nh * 6;  // own rows' r, then own rows' p (kPcgRows each)
+  float *pl = rl + kPcgRows;
   if (a.lds_a) {
     const float4 *src = reinterpret_cast<const float4 *>(a.values + (size_t)blk0 * 36);
     float4 *dst = reinterpret_cast<float4 *>(Al);
@@ -541,11 +552,12 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
     for (int j = t; j < nblk; j += kPcgThreads) Ll[j] = a.lcol[blk0 + j];
   }
   if (t == 0) {
-    s_base = __hip_atomic_load(&a.ctl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) << 16;
-    s_abort = 0;
+    s_base = (unsigned)(__hip_atomic_load(&a.ctl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) << 16);
+    s_bad = 0;
   }
-  const bool own = t < S;
-  const int li = t / 6, row = t % 6, i = n0 + li, s = 6 * n0 + t;
+  const int q = t / kPcgSub, sub = t % kPcgSub;
+  const bool own = q < S, lead = own && sub == 0;
+  const int li = q / 6, row = q % 6, i = n0 + li, s = 6 * n0 + q;
   float M[6] = {0, 0, 0, 0, 0, 0};
   float bv = 0.f;
   int rb = 0, re = 0;
@@ -557,44 +569,66 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
   }
   // init: x = 0, r = b, z = M^-1 r, p = z (kernels/solver_kernels.cu:620-632)
   float x = 0.f, r = bv, z = 0.f, p = 0.f;
-  rl[t] = r;
+  if (sub == 0) rl[q] = r;
   __syncthreads();
-  const unsigned long long base = s_base;
+  const unsigned base = s_base;
+  unsigned pt = 1;
   if (own) {
     for (int c = 0; c < 6; ++c) z += M[c] * rl[li * 6 + c];
-    st_agent(a.zpub + s, z);
+    if (lead) ll_put(a.zpub + s, base + pt, __float_as_uint(z));
   }
-  double rz = own ? (double)r * (double)z : 0.0, bb = own ? (double)bv * (double)bv : 0.0;
-  unsigned long long pt = 1;
-  bool aborted = allreduce2(a, base + pt, (int)(pt & 1), rz, bb, red0, red1, &s_abort);
+  unsigned z_tag = base + pt, pold_tag = 0;
+  double rz = lead ? (double)r * (double)z : 0.0, bb = lead ? (double)bv * (double)bv : 0.0;
+  bool aborted = allreduce2(a, base + pt, (int)(pt & 1), rz, bb, red, &s_bad);
   double rr = bb;
   const double tol = a.rel ? (double)a.tol * sqrt(bb) : (double)a.tol;
   int iters = 0, flags = 0;
   float betaf = 0.f;
   for (int it = 0; it < a.maxit && !aborted; ++it) {
-    const float *Pold = (it & 1) ? a.ppub1 : a.ppub0;
-    float *Pnew = (it & 1) ? a.ppub0 : a.ppub1;
-    // A: halo p into LDS, Ap, p.Ap
-    for (int e = t; e < nh * 6; e += kPcgThreads) {
-      const int idx = 6 * a.halo[h0 + e / 6] + e % 6;
-      const float zv = ld_agent(a.zpub + idx);
-      ph[e] = (it == 0) ? zv : zv + betaf * ld_agent(Pold + idx);
-    }
+    const ll_word *Pold = (it & 1) ? a.ppub1 : a.ppub0;
+    ll_word *Pnew = (it & 1) ? a.ppub0 : a.ppub1;
+    ++pt;  // the p.Ap point
+    // A: own p (registers -> LDS), the halo's p, Ap, p.Ap
     p = (it == 0) ? z : z + betaf * p;
-    if (own) st_agent(Pnew + s, p);
+    if (lead) {
+      ll_put(Pnew + s, base + pt, __float_as_uint(p));
+      pl[q] = p;
+    }
+    __syncthreads();
+    for (int e = t; e < nh * 6; e += kPcgThreads) {
+      const int nd = a.halo[h0 + e / 6], c = e % 6;
+      if (nd >= n0 && nd < n1) {
+        ph[e] = pl[(nd - n0) * 6 + c];
+      } else {
+        const int idx = 6 * nd + c;
+        ll_word wz, wp = 0;
+        unsigned spins = 0;
+        for (;;) {  // both words in one round trip
+          wz = ll_load(a.zpub + idx);
+          if (it > 0) wp = ll_load(Pold + idx);
+          if (ll_has(wz, z_tag) && (it == 0 || ll_has(wp, pold_tag))) break;
+          if (ll_spin(spins, a.ctl, &s_bad)) break;
+        }
+        const float zv = __uint_as_float((unsigned)wz);
+        ph[e] = (it == 0) ? zv : zv + betaf * __uint_as_float((unsigned)wp);
+      }
+    }
     __syncthreads();
     float y = 0.f;
     if (own) {
-      for (int bk = rb; bk < re; ++bk) {
+      for (int bk = rb + sub; bk < re; bk += kPcgSub) {
         const int lc = a.lds_a ? Ll[bk - blk0] : a.lcol[bk];
         const float *A = a.lds_a ? (Al + (size_t)(bk - blk0) * 36) : (a.values + (size_t)bk * 36);
         const float *pj = ph + lc * 6;
         for (int c = 0; c < 6; ++c) y += A[row * 6 + c] * pj[c];
       }
     }
-    double pAp = own ? (double)p * (double)y : 0.0, unused = 0.0;
-    ++pt;
-    aborted = allreduce2(a, base + pt, (int)(pt & 1), pAp, unused, red0, red1, &s_abort);
+    // the row's four partial products, combined identically in all four lanes
+    y += __shfl_xor(y, 1, 64);
+    y += __shfl_xor(y, 2, 64);
+    pold_tag = base + pt;
+    double pAp = lead ? (double)p * (double)y : 0.0, unused = 0.0;
+    aborted = allreduce2(a, base + pt, (int)(pt & 1), pAp, unused, red, &s_bad);
     if (aborted) break;
     if (fabs(pAp) < 1e-12) {
       flags |= 1;
@@ -602,18 +636,19 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
     }
     const float alf = (float)(rz / pAp);
     // B: x += alpha p, r -= alpha Ap, z = M^-1 r, (r.r, r.z)
+    ++pt;  // the (r.r, r.z) point
     x += alf * p;
     r += -alf * y;
-    rl[t] = r;
+    if (sub == 0) rl[q] = r;
     __syncthreads();
     z = 0.f;
     if (own) {
       for (int c = 0; c < 6; ++c) z += M[c] * rl[li * 6 + c];
-      st_agent(a.zpub + s, z);
+      if (lead) ll_put(a.zpub + s, base + pt, __float_as_uint(z));
     }
-    double rr_l = own ? (double)r * (double)r : 0.0, rz_l = own ? (double)r * (double)z : 0.0;
-    ++pt;
-    aborted = allreduce2(a, base + pt, (int)(pt & 1), rr_l, rz_l, red0, red1, &s_abort);
+    z_tag = base + pt;
+    double rr_l = lead ? (double)r * (double)r : 0.0, rz_l = lead ? (double)r * (double)z : 0.0;
+    aborted = allreduce2(a, base + pt, (int)(pt & 1), rr_l, rz_l, red, &s_bad);
     if (aborted) break;
     rr = rr_l;
     iters = it + 1;
@@ -625,7 +660,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
     rz = rz_l;
   }
   if (aborted) flags |= 8;
-  if (own) a.x[s] = x;
+  if (lead) a.x[s] = x;
   if (g == 0 && t == 0) {
     a.res->iterations = iters;
     a.res->total_iterations += iters;
@@ -881,7 +916,7 @@ struct nrf_solver {
   float *values = nullptr, *b = nullptr, *Minv = nullptr;
   size_t c_values = 0, c_b = 0, c_Minv = 0;
   // pcg: published z / p vectors, per-workgroup sync records, halo lists
-  float *zpub = nullptr, *P0 = nullptr, *P1 = nullptr;
+  ll_word *zpub = nullptr, *P0 = nullptr, *P1 = nullptr;
   size_t c_zpub = 0, c_P0 = 0, c_P1 = 0;
   SyncRec *recs = nullptr;
   size_t c_recs = 0;
@@ -940,7 +975,7 @@ static int solver_set_pattern_host(nrf_solver *s, const nrf_node *h, int N) {
   int G = s->forced_G;
   const int min_G = std::max(1, (N + kPcgMaxNodesPerWg - 1) / kPcgMaxNodesPerWg);
   if (G < min_G) G = std::max(min_G, std::min(s->cus, (N + 39) / 40));
-  if (G > kPcgThreads) return NRF_ERR_TOO_LARGE;  // lanes 0..G-1 wait on the records
+  if (G > kPcgMaxWg) return NRF_ERR_TOO_LARGE;  // one record per workgroup
   std::vector<int32_t> rows(G + 1);
   for (int g = 0; g <= G; ++g) rows[g] = (int32_t)((long)N * g / G);
   std::vector<int32_t> lcol(P.nb), hoff(G + 1, 0), halo;
@@ -955,7 +990,7 @@ static int solver_set_pattern_host(nrf_solver *s, const nrf_node *h, int N) {
     hoff[g] = (int32_t)halo.size();
     halo.insert(halo.end(), h.begin(), h.end());
     const size_t nblk = (size_t)(b1 - b0);
-    need_max = std::max(need_max, nblk * (36 * 4 + 4) + h.size() * 24 + kPcgThreads * 4);
+    need_max = std::max(need_max, nblk * (36 * 4 + 4) + h.size() * 24 + kPcgRows * 8);
   }
   hoff[G] = (int32_t)halo.size();
   size_t halo_max = 0;
@@ -963,7 +998,7 @@ static int solver_set_pattern_host(nrf_solver *s, const nrf_node *h, int N) {
   s->G = G;
   const size_t budget = (size_t)std::max(0, s->max_lds - 1024);
   s->lds_a = need_max <= budget ? 1 : 0;
-  s->pcg_lds = align_up(s->lds_a ? need_max : halo_max * 24 + kPcgThreads * 4, 16);
+  s->pcg_lds = align_up(s->lds_a ? need_max : halo_max * 24 + kPcgRows * 8, 16);
   if ((st = upload(s->rows, s->c_rows, rows))) return st;
   if ((st = upload(s->lcol, s->c_lcol, lcol))) return st;
   if ((st = upload(s->halo_off, s->c_halo_off, hoff))) return st;
@@ -975,7 +1010,7 @@ static int solver_set_pattern_host(nrf_solver *s, const nrf_node *h, int N) {
     } else {
       (void)hipGetLastError();
       s->lds_a = 0;  // keep the matrix in global memory
-      s->pcg_lds = align_up(halo_max * 24 + kPcgThreads * 4, 16);
+      s->pcg_lds = align_up(halo_max * 24 + kPcgRows * 8, 16);
     }
   }
   return NRF_OK;
@@ -1099,8 +1134,14 @@ static int solver_ensure_pattern(nrf_solver *s, const nrf_node *d_nodes, int N, 
 
 }  // namespace nrf
 
+// A PCG launch has 1 + 2 * pcg_iterations synchronisation points, which must
+// fit the 16 point bits of a published word's tag.
+static bool params_ok(const nrf_solver_params *p) {
+  return p->pcg_iterations >= 0 && 1 + 2 * (long)p->pcg_iterations <= kPcgMaxPoints;
+}
+
 extern "C" nrf_solver *nrf_solver_create(const nrf_solver_params *params, int max_nodes) {
-  if (!params || max_nodes <= 0) return nullptr;
+  if (!params || max_nodes <= 0 || !params_ok(params)) return nullptr;
   nrf_solver *s = new nrf_solver();
   s->params = *params;
   s->max_nodes = max_nodes;
@@ -1118,10 +1159,14 @@ extern "C" nrf_solver *nrf_solver_create(const nrf_solver_params *params, int ma
   const size_t n6 = (size_t)max_nodes * 6;
   bool ok = dev_alloc(s->b, n6, s->c_b) == 0 && dev_alloc(s->Minv, (size_t)max_nodes * 36, s->c_Minv) == 0 &&
             dev_alloc(s->zpub, n6, s->c_zpub) == 0 && dev_alloc(s->P0, n6, s->c_P0) == 0 &&
-            dev_alloc(s->P1, n6, s->c_P1) == 0 && dev_alloc(s->recs, (size_t)kPcgThreads, s->c_recs) == 0 &&
+            dev_alloc(s->P1, n6, s->c_P1) == 0 && dev_alloc(s->recs, (size_t)kPcgMaxWg, s->c_recs) == 0 &&
             dev_alloc(s->ctl, 1, s->c_ctl) == 0 && dev_alloc(s->stop, 1, s->c_stop) == 0 &&
             dev_alloc(s->d_res, 1, s->c_res) == 0 && dev_alloc(s->d_tot, 1, s->c_tot) == 0;
-  if (ok) ok = hipMemset(s->recs, 0, sizeof(SyncRec) * kPcgThreads) == hipSuccess &&
+  // published words start with tag 0, which no synchronisation point uses
+  if (ok) ok = hipMemset(s->zpub, 0, sizeof(ll_word) * n6) == hipSuccess &&
+               hipMemset(s->P0, 0, sizeof(ll_word) * n6) == hipSuccess &&
+               hipMemset(s->P1, 0, sizeof(ll_word) * n6) == hipSuccess &&
+               hipMemset(s->recs, 0, sizeof(SyncRec) * kPcgMaxWg) == hipSuccess &&
                hipMemset(s->ctl, 0, sizeof(PcgCtl)) == hipSuccess &&
                hipMemset(s->stop, 0, sizeof(int32_t)) == hipSuccess &&
                hipMemset(s->d_res, 0, sizeof(nrf_pcg_result)) == hipSuccess &&
@@ -1145,7 +1190,7 @@ extern "C" void nrf_solver_destroy(nrf_solver *s) {
 }
 
 extern "C" int nrf_solver_set_params(nrf_solver *s, const nrf_solver_params *params) {
-  if (!s || !params) return NRF_ERR_INVALID_ARGUMENT;
+  if (!s || !params || !params_ok(params)) return NRF_ERR_INVALID_ARGUMENT;
   s->params = *params;
   return NRF_OK;
 }

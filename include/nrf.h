@@ -244,7 +244,9 @@ int nrf_corr_pack(const nrf_volume_params *vp, const int32_t *voxel_knn,
 /* ---------------------------------------------------------------- solver */
 typedef struct nrf_solver nrf_solver;
 
-/* GaussNewtonSolver(const Params&, int max_nodes) (kernels/solver_kernels.cu:486-496). */
+/* GaussNewtonSolver(const Params&, int max_nodes) (kernels/solver_kernels.cu:486-496).
+   Returns NULL (and set_params NRF_ERR_INVALID_ARGUMENT) for pcg_iterations
+   outside [0, 32767]. */
 nrf_solver *nrf_solver_create(const nrf_solver_params *params, int max_nodes);
 void nrf_solver_destroy(nrf_solver *s);
 int nrf_solver_set_params(nrf_solver *s, const nrf_solver_params *params);
