@@ -27,6 +27,7 @@
 //    leaves two grid barriers per iteration.  The barrier wait is bounded:
 //    on timeout the kernel raises a flag and every workgroup exits.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -46,18 +47,46 @@ __global__ void fill_i32(int32_t *p, int n, int32_t v) {
   if (i < n) p[i] = v;
 }
 
-// Membership pass 1: count valid (corr, slot) pairs per node.  A slot is
-// valid under the reference's test (kernels/solver_kernels.cu:94-96).
-__global__ void mem_count_kernel(const nrf_corr *__restrict__ corrs, int n, int N, int32_t *__restrict__ cnt) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  const nrf_corr &cr = corrs[c];
-  if (!cr.valid) return;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    int id = cr.node_ids[k];
-    if (id >= 0 && id < N && cr.node_ws[k] > 1e-8f) atomicAdd(&cnt[id], 1);
+// Memberships (node -> its correspondence slots) are built once per frame:
+// count per node, exclusive scan, scatter, then each node's bucket is
+// ordered by key (corr * 4 + slot), which fixes the order in which the
+// assembly sums a row's contributions.
+//
+// Appends one entry per lane whose id >= 0 to bucket `id`: the lanes of the
+// wave that share an id take consecutive positions from one atomic issued by
+// a leader lane, so a node seen by many adjacent pixels costs one atomic per
+// wave instead of one per pixel.  Returns the lane's position (-1 if none).
+// Every lane of the wave must call it.
+__device__ inline int wave_claim(int32_t *ctr, int id) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long active = __ballot(id >= 0);
+  int pos = -1;
+  while (active) {
+    const int leader = __ffsll((unsigned long long)active) - 1;
+    const int v = __shfl(id, leader, 64);
+    const unsigned long long grp = __ballot(id == v) & active;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&ctr[v], __popcll(grp));
+    base = __shfl(base, leader, 64);
+    if ((grp >> lane) & 1ull) pos = base + __popcll(grp & ((1ull << lane) - 1ull));
+    active &= ~grp;
   }
+  return pos;
+}
+
+// The node of slot k of a valid corr, or -1 when the slot takes no part
+// (kernels/solver_kernels.cu:94-96: id in range and weight > 1e-8).
+__device__ inline int member_node(const nrf_corr &cr, int k, int N) {
+  const int id = cr.node_ids[k];
+  return (id >= 0 && id < N && cr.node_ws[k] > 1e-8f) ? id : -1;
+}
+
+__global__ __launch_bounds__(256) void mem_count_kernel(const nrf_corr *__restrict__ corrs, int n, int N,
+                                                        int32_t *__restrict__ cnt) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = c < n && corrs[c].valid;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) wave_claim(cnt, ok ? member_node(corrs[c], k, N) : -1);
 }
 
 __global__ __launch_bounds__(1024) void exclusive_scan_kernel(const int32_t *__restrict__ cnt,
@@ -86,43 +115,58 @@ __global__ __launch_bounds__(1024) void exclusive_scan_kernel(const int32_t *__r
   if (t == 1023) off[n] = part[1023];
 }
 
-__global__ void mem_scatter_kernel(const nrf_corr *__restrict__ corrs, int n, int N, int32_t *__restrict__ cur,
-                                   int32_t *__restrict__ keys) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  const nrf_corr &cr = corrs[c];
-  if (!cr.valid) return;
+__global__ __launch_bounds__(256) void mem_scatter_kernel(const nrf_corr *__restrict__ corrs, int n, int N,
+                                                          int32_t *__restrict__ cur, int32_t *__restrict__ keys) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = c < n && corrs[c].valid;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    int id = cr.node_ids[k];
-    if (id >= 0 && id < N && cr.node_ws[k] > 1e-8f) {
-      int pos = atomicAdd(&cur[id], 1);
-      keys[pos] = c * 4 + k;
-    }
+    const int pos = wave_claim(cur, ok ? member_node(corrs[c], k, N) : -1);
+    if (pos >= 0) keys[pos] = c * 4 + k;
   }
 }
 
-// Orders each node's bucket by key (= corr * 4 + slot): keys are unique, so
-// an element's rank is the count of smaller keys in its bucket.
+// Orders each node's bucket by key.  Buckets of up to kSortLds keys are
+// bitonic-sorted in LDS (keys are unique, padding is INT_MAX); longer ones
+// fall back to ranking every key against the whole bucket.
 constexpr int kSortLds = 2048;
-__global__ __launch_bounds__(64) void mem_sort_kernel(const int32_t *__restrict__ off,
-                                                      const int32_t *__restrict__ keys_in,
-                                                      int32_t *__restrict__ keys_out) {
+constexpr int kSortThreads = 256;
+__global__ __launch_bounds__(kSortThreads) void mem_sort_kernel(const int32_t *__restrict__ off,
+                                                                const int32_t *__restrict__ keys_in,
+                                                                int32_t *__restrict__ keys_out) {
   __shared__ int32_t sk[kSortLds];
-  const int i = blockIdx.x;
-  const int b = off[i], e = off[i + 1], n = e - b;
-  if (n <= 0) return;
-  const bool in_lds = n <= kSortLds;
-  if (in_lds) {
-    for (int j = threadIdx.x; j < n; j += 64) sk[j] = keys_in[b + j];
-    __syncthreads();
+  const int i = blockIdx.x, t = threadIdx.x;
+  const int b = off[i], n = off[i + 1] - b;
+  if (n <= 0) return;  // uniform in the workgroup
+  if (n > kSortLds) {
+    for (int j = t; j < n; j += kSortThreads) {
+      const int kj = keys_in[b + j];
+      int rank = 0;
+      for (int m = 0; m < n; ++m) rank += (keys_in[b + m] < kj);
+      keys_out[b + rank] = kj;
+    }
+    return;
   }
-  for (int j = threadIdx.x; j < n; j += 64) {
-    const int kj = in_lds ? sk[j] : keys_in[b + j];
-    int rank = 0;
-    for (int m = 0; m < n; ++m) rank += ((in_lds ? sk[m] : keys_in[b + m]) < kj);
-    keys_out[b + rank] = kj;
-  }
+  int P = 1;
+  while (P < n) P <<= 1;
+  for (int j = t; j < P; j += kSortThreads) sk[j] = j < n ? keys_in[b + j] : INT_MAX;
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int x = t; x < P; x += kSortThreads) {
+        const int y = x ^ j;
+        if (y > x) {
+          const bool up = (x & k) == 0;
+          const int u = sk[x], v = sk[y];
+          if ((u > v) == up) {
+            sk[x] = v;
+            sk[y] = u;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (int j = t; j < n; j += kSortThreads) keys_out[b + j] = sk[j];
 }
 
 // ------------------------------------------------------------ assembly
@@ -202,93 +246,61 @@ __device__ inline bool chol_inv6(const R *A, R *Ainv_col, int col, bool &ok_out)
   return true;
 }
 
-__global__ __launch_bounds__(64) void assemble_rows_kernel(AsmArgs a) {
-  extern __shared__ float lds[];
-  const int i = blockIdx.x;
-  if (a.stop && *a.stop) return;
-  const int lane = threadIdx.x;
-  const int rb = a.row_ptr[i], rlen = a.row_ptr[i + 1] - rb;
-  float *acc = lds;                                      // [Rmax][36]
-  int32_t *cols = reinterpret_cast<int32_t *>(acc + a.Rmax * 36);  // [Rmax]
-  float *stage = reinterpret_cast<float *>(cols + a.Rmax);         // [64][kStage]
-  for (int j = lane; j < rlen * 36; j += 64) acc[j] = 0.f;
-  for (int j = lane; j < rlen; j += 64) cols[j] = a.col_idx[rb + j];
-  __syncthreads();
-  const int ab = lane;            // lanes 0..35: entry (a, b) = (ab / 6, ab % 6)
-  const int ea = ab / 6, eb = ab % 6;
-  const int ra = lane - 36;       // lanes 36..41: RHS row ra
-  float bacc = 0.f;
-  const int diag = a.diag_rel[i];
-
-  // ---- data term (kernels/solver_kernels.cu:52-138)
-  const int mb = a.m_off[i], me = a.m_off[i + 1];
-  for (int base = mb; base < me; base += 64) {
-    const int e = base + lane;
-    float *st = stage + lane * kStage;
-    float fw = 0.f;
-    if (e < me) {
-      const int key = a.m_keys[e];
-      const int c = key >> 2, kslot = key & 3;
-      const nrf_corr &cr = a.corrs[c];
-      const float3 dst = f3(cr.dst), nrm = f3(cr.normal);
-      const int ids[4] = {cr.node_ids[0], cr.node_ids[1], cr.node_ids[2], cr.node_ids[3]};
-      const float ws[4] = {cr.node_ws[0], cr.node_ws[1], cr.node_ws[2], cr.node_ws[3]};
-      float3 src;
-      if (a.canonical) {
-        src = f3(a.canonical[3 * c], a.canonical[3 * c + 1], a.canonical[3 * c + 2]);
-        DQ bl;
-        if (dq_blend4(a.centered, ids, ws, bl)) src = dq_transform(bl, src);
-      } else {
-        src = f3(cr.src);
-      }
-      const float r = dot(nrm, sub(src, dst));
-      fw = fmaxf(cr.weight, 0.0f) * huber_w(r, 0.2f);
-      st[0] = fw;
-      st[1] = r;
-      st[2] = __int_as_float(kslot);
-#pragma unroll
-      for (int l = 0; l < 4; ++l) {
-        int pos = -1;
-        const int id = ids[l];
-        if (id >= 0 && id < a.N && ws[l] > 1e-8f) {
-          const float3 xp = f3(a.nodes[id].pos);
-          const float3 lp = sub(src, xp);
-          const float3 cp = cross(lp, nrm);
-          const float w = ws[l];
-          float *J = st + 7 + 6 * l;
-          J[0] = w * cp.x; J[1] = w * cp.y; J[2] = w * cp.z;
-          J[3] = w * nrm.x; J[4] = w * nrm.y; J[5] = w * nrm.z;
-          for (int q = 0; q < rlen; ++q)  // rows are short; linear scan
-            if (cols[q] == id) { pos = q; break; }
-          if (pos < 0) pos = -2;  // valid neighbour, block absent from the pattern
-        }
-        st[3 + l] = __int_as_float(pos);
-      }
-    }
-    st[0] = fw;  // inactive lanes hold 0; NaN stays active as in the reference
-    __syncthreads();
-    const int cnt = min(64, me - base);
-    for (int q = 0; q < cnt; ++q) {
-      const float *sq = stage + q * kStage;
-      const float f = sq[0];
-      if (f <= 1e-8f) continue;  // kernels/solver_kernels.cu:84
-      const int ks = __float_as_int(sq[2]);
-      const float *Jk = sq + 7 + 6 * ks;
-      if (lane < 36) {
-        const float fa = f * Jk[ea];
-#pragma unroll
-        for (int l = 0; l < 4; ++l) {
-          const int pos = __float_as_int(sq[3 + l]);
-          if (pos >= 0) acc[pos * 36 + ab] += fa * sq[7 + 6 * l + eb];
-        }
-      } else if (ra >= 0 && ra < 6) {
-        bacc += -f * Jk[ra] * sq[1];
-      }
-    }
-    __syncthreads();
+// Staged residual and Jacobians of one membership (corr c, slot kslot) of
+// row i: the DQ-warped source point (re-warped from the canonical point when
+// one is given), the Huber-weighted residual and the four skinning-slot
+// Jacobians w_l [ (src - x_l) x n, n ] (kernels/solver_kernels.cu:52-138,
+// include/se3_math.cuh:121-145).  pos[l] is the slot's block position in the
+// row, -1 for an invalid slot, -2 for a valid neighbour whose block is not in
+// the pattern (dropped, Q2).  Returns fw (0 for an inactive lane).
+__device__ inline float stage_membership(const AsmArgs &a, int key, const int32_t *cols, int rlen, float &r,
+                                         int &kslot, float J[4][6], int pos[4]) {
+  const int c = key >> 2;
+  kslot = key & 3;
+  const nrf_corr &cr = a.corrs[c];
+  const float3 dst = f3(cr.dst), nrm = f3(cr.normal);
+  const int ids[4] = {cr.node_ids[0], cr.node_ids[1], cr.node_ids[2], cr.node_ids[3]};
+  const float ws[4] = {cr.node_ws[0], cr.node_ws[1], cr.node_ws[2], cr.node_ws[3]};
+  float3 src;
+  if (a.canonical) {
+    src = f3(a.canonical[3 * c], a.canonical[3 * c + 1], a.canonical[3 * c + 2]);
+    DQ bl;
+    if (dq_blend4(a.centered, ids, ws, bl)) src = dq_transform(bl, src);
+  } else {
+    src = f3(cr.src);
   }
+  r = dot(nrm, sub(src, dst));
+  const float fw = fmaxf(cr.weight, 0.0f) * huber_w(r, 0.2f);
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    pos[l] = -1;
+    for (int q = 0; q < 6; ++q) J[l][q] = 0.f;
+    const int id = ids[l];
+    if (id >= 0 && id < a.N && ws[l] > 1e-8f) {
+      const float3 xp = f3(a.nodes[id].pos);
+      const float3 lp = sub(src, xp);
+      const float3 cp = cross(lp, nrm);
+      const float w = ws[l];
+      J[l][0] = w * cp.x; J[l][1] = w * cp.y; J[l][2] = w * cp.z;
+      J[l][3] = w * nrm.x; J[l][4] = w * nrm.y; J[l][5] = w * nrm.z;
+      int p = -2;
+      for (int q = 0; q < rlen; ++q)  // rows are short; linear scan
+        if (cols[q] == id) { p = q; break; }
+      pos[l] = p;
+    }
+  }
+  return fw;
+}
 
-  // ---- ARAP term (kernels/solver_kernels.cu:202-352)
+// ARAP edge-sides of row i (kernels/solver_kernels.cu:202-352), damping
+// (:167-194), write-out of the row's blocks and RHS, then the block-Jacobi
+// inverse of the diagonal block (:358-399, include/se3_math.cuh:152-202).
+// acc holds the row's blocks with the data term already in, bacc (lanes
+// 36..41) the data term's RHS.
+__device__ inline void row_tail(const AsmArgs &a, int i, int rb, int rlen, float *acc, float *stage, float bacc) {
+  const int lane = threadIdx.x;
+  const int ab = lane, ea = ab / 6, eb = ab % 6, ra = lane - 36;
+  const int diag = a.diag_rel[i];
   const int sb = a.es_off[i], se = a.es_off[i + 1];
   for (int base = sb; base < se; base += 64) {
     const int e = base + lane;
@@ -387,6 +399,170 @@ __global__ __launch_bounds__(64) void assemble_rows_kernel(AsmArgs a) {
     }
     for (int k = 0; k < 6; ++k) a.Minv[(size_t)i * 36 + k * 6 + lane] = outc[k];
   }
+}
+
+// One wavefront per block row i: the data term with every membership's
+// contribution accumulated in LDS by 36 lanes (one per block entry) in
+// membership order, then row_tail.  Used for rows wider than the MFMA form
+// takes.
+__global__ __launch_bounds__(64) void assemble_rows_kernel(AsmArgs a) {
+  extern __shared__ float lds[];
+  const int i = blockIdx.x;
+  if (a.stop && *a.stop) return;
+  const int lane = threadIdx.x;
+  const int rb = a.row_ptr[i], rlen = a.row_ptr[i + 1] - rb;
+  float *acc = lds;                                      // [Rmax][36]
+  int32_t *cols = reinterpret_cast<int32_t *>(acc + a.Rmax * 36);  // [Rmax]
+  float *stage = reinterpret_cast<float *>(cols + a.Rmax);         // [64][kStage]
+  for (int j = lane; j < rlen * 36; j += 64) acc[j] = 0.f;
+  for (int j = lane; j < rlen; j += 64) cols[j] = a.col_idx[rb + j];
+  __syncthreads();
+  const int ab = lane;  // lanes 0..35: entry (a, b) = (ab / 6, ab % 6)
+  const int ea = ab / 6, eb = ab % 6;
+  const int ra = lane - 36;  // lanes 36..41: RHS row ra
+  float bacc = 0.f;
+  const int mb = a.m_off[i], me = a.m_off[i + 1];
+  for (int base = mb; base < me; base += 64) {
+    const int e = base + lane;
+    float *st = stage + lane * kStage;
+    float fw = 0.f;
+    if (e < me) {
+      float r, J[4][6];
+      int kslot, pos[4];
+      fw = stage_membership(a, a.m_keys[e], cols, rlen, r, kslot, J, pos);
+      st[1] = r;
+      st[2] = __int_as_float(kslot);
+      for (int l = 0; l < 4; ++l) {
+        st[3 + l] = __int_as_float(pos[l]);
+        for (int q = 0; q < 6; ++q) st[7 + 6 * l + q] = J[l][q];
+      }
+    }
+    st[0] = fw;  // inactive lanes hold 0; NaN stays active as in the reference
+    __syncthreads();
+    const int cnt = min(64, me - base);
+    for (int q = 0; q < cnt; ++q) {
+      const float *sq = stage + q * kStage;
+      const float f = sq[0];
+      if (f <= 1e-8f) continue;  // kernels/solver_kernels.cu:84
+      const int ks = __float_as_int(sq[2]);
+      const float *Jk = sq + 7 + 6 * ks;
+      if (lane < 36) {
+        const float fa = f * Jk[ea];
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+          const int pos = __float_as_int(sq[3 + l]);
+          if (pos >= 0) acc[pos * 36 + ab] += fa * sq[7 + 6 * l + eb];
+        }
+      } else if (ra >= 0 && ra < 6) {
+        bacc += -f * Jk[ra] * sq[1];
+      }
+    }
+    __syncthreads();
+  }
+  row_tail(a, i, rb, rlen, acc, stage, bacc);
+}
+
+// MFMA form of the data term (NS = column slabs of 16 covering the 6 R
+// block columns plus one RHS column).  For a batch of 64 memberships e of
+// row i, with a_e = f_e J_k(e) (the row node's Jacobian times the Huber
+// weight) and the dense row b_e = [J_l(e) in the columns of block pos_l(e),
+// r_e in column 6 R], the batch adds
+//   C (6 x (6 R + 1)) += sum_e a_e b_e^T
+// as v_mfma_f32_16x16x4f32 products (rows 6..15 of A are zero).  Then the
+// blocks are C[:, 0:6R] and the RHS is -C[:, 6R].  Same products as the
+// scalar form, FP32, summed in the matrix unit's order.
+template <int NS>
+__global__ __launch_bounds__(64) void assemble_rows_mfma_kernel(AsmArgs a) {
+  extern __shared__ float lds[];
+  const int i = blockIdx.x;
+  if (a.stop && *a.stop) return;
+  const int lane = threadIdx.x;
+  const int rb = a.row_ptr[i], rlen = a.row_ptr[i + 1] - rb;
+  float *acc = lds;                                      // [Rmax][36]
+  int32_t *cols = reinterpret_cast<int32_t *>(acc + a.Rmax * 36);  // [Rmax]
+  float *stage = reinterpret_cast<float *>(cols + a.Rmax);         // [64][kStage]
+  __shared__ float s_rhs[6];
+  for (int j = lane; j < rlen; j += 64) cols[j] = a.col_idx[rb + j];
+  __syncthreads();
+  // staging layout per membership: a[6], r, packed pos (4 bytes), J[4][6]
+  constexpr int kA = 0, kR = 6, kP = 7, kJ = 8;
+  typedef float floatx4 __attribute__((ext_vector_type(4)));
+  floatx4 C[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) C[s] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int ncol = 6 * rlen + 1;  // block columns + RHS column
+  const int am = lane % 16;       // A row / B, D column within a slab
+  const int kq = lane / 16;       // k within a 4-step
+  const int mb = a.m_off[i], me = a.m_off[i + 1];
+  for (int base = mb; base < me; base += 64) {
+    const int e = base + lane;
+    float *st = stage + lane * kStage;
+    float fw = 0.f, r = 0.f, J[4][6];
+    int kslot = 0, pos[4] = {-1, -1, -1, -1};
+    if (e < me) fw = stage_membership(a, a.m_keys[e], cols, rlen, r, kslot, J, pos);
+    // f <= 1e-8 skips the membership (kernels/solver_kernels.cu:84); NaN stays
+    const bool act = e < me && !(fw <= 1e-8f);
+    for (int q = 0; q < 6; ++q) st[kA + q] = act ? fw * J[kslot][q] : 0.f;
+    st[kR] = act ? r : 0.f;
+    unsigned packed = 0xffffffffu;
+    if (act)
+      for (int l = 0; l < 4; ++l)
+        if (pos[l] >= 0) packed = (packed & ~(0xffu << (8 * l))) | ((unsigned)pos[l] << (8 * l));
+    st[kP] = __uint_as_float(packed);
+    for (int l = 0; l < 4; ++l)
+      for (int q = 0; q < 6; ++q) st[kJ + 6 * l + q] = act ? J[l][q] : 0.f;
+    __syncthreads();
+    // k-steps outer, slabs inner: the slabs' products are independent, and
+    // A and the packed positions are read once per k-step.  B[e][j] is one
+    // LDS read at the offset of the slot whose block is column j's (or of r
+    // for the RHS column), 0 when no slot of e maps there.
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const float *sq = stage + (4 * t + kq) * kStage;
+      const float av = am < 6 ? sq[kA + am] : 0.f;
+      const unsigned pk = __float_as_uint(sq[kP]);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        if (16 * s >= ncol) break;  // wave-uniform
+        const int j = 16 * s + am;  // this lane's B / D column
+        const unsigned jb = (unsigned)(j / 6);
+        int off = -1;
+        if (j < ncol - 1) {
+#pragma unroll
+          for (int l = 0; l < 4; ++l)
+            if (((pk >> (8 * l)) & 0xffu) == jb) off = kJ + 6 * l + j % 6;
+        } else if (j == ncol - 1) {
+          off = kR;
+        }
+        const float bv = off >= 0 ? sq[off] : 0.f;
+        C[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, C[s], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  // D[m][j] sits in lane j % 16 + 16 (m / 4), element m % 4: rows 0..5 are in
+  // lanes 0..31.
+  for (int j = lane; j < rlen * 36; j += 64) acc[j] = 0.f;
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int j = 16 * s + am;
+    if (lane < 32 && j < ncol) {
+      for (int v = 0; v < 4; ++v) {
+        const int m = 4 * kq + v;
+        if (m < 6) {
+          if (j < ncol - 1)
+            acc[(j / 6) * 36 + m * 6 + j % 6] = C[s][v];
+          else
+            s_rhs[m] = -C[s][v];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int ra = lane - 36;
+  const float bacc = (ra >= 0 && ra < 6) ? s_rhs[ra] : 0.f;
+  row_tail(a, i, rb, rlen, acc, stage, bacc);
 }
 
 // ----------------------------------------------------------------- PCG
@@ -938,6 +1114,7 @@ struct nrf_solver {
   size_t c_tot = 0;
   int cus = 0, max_lds = 0, coop = 0;
   int forced_G = 0;
+  int asm_mfma = 1;  // data term on the matrix cores (rows of up to 21 blocks)
   // optional profiling events (see solver_set_event_sink)
   hipEvent_t *sink = nullptr;
   int sink_n = 0, sink_i = 0;
@@ -1035,7 +1212,7 @@ static int solver_build_memberships(nrf_solver *s, const nrf_corr *corrs, int n,
     mem_scatter_kernel<<<(n + 255) / 256, 256, 0, st>>>(corrs, n, N, s->m_cur, s->m_keys);
     NRF_LAUNCH_CHECK();
   }
-  mem_sort_kernel<<<N, 64, 0, st>>>(s->m_off, s->m_keys, s->m_keys2);
+  mem_sort_kernel<<<N, kSortThreads, 0, st>>>(s->m_off, s->m_keys, s->m_keys2);
   NRF_LAUNCH_CHECK();
   return NRF_OK;
 }
@@ -1071,7 +1248,18 @@ static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, const float *ca
   a.stop = s->stop;
   const size_t lds = (size_t)s->P.Rmax * 36 * 4 + (size_t)s->P.Rmax * 4 + 64 * kStage * 4;
   if (lds > 64 * 1024) return NRF_ERR_TOO_LARGE;
-  assemble_rows_kernel<<<N, 64, lds, st>>>(a);
+  const int ns = (6 * s->P.Rmax + 1 + 15) / 16;
+  switch (s->asm_mfma ? ns : 0) {
+    case 1: assemble_rows_mfma_kernel<1><<<N, 64, lds, st>>>(a); break;
+    case 2: assemble_rows_mfma_kernel<2><<<N, 64, lds, st>>>(a); break;
+    case 3: assemble_rows_mfma_kernel<3><<<N, 64, lds, st>>>(a); break;
+    case 4: assemble_rows_mfma_kernel<4><<<N, 64, lds, st>>>(a); break;
+    case 5: assemble_rows_mfma_kernel<5><<<N, 64, lds, st>>>(a); break;
+    case 6: assemble_rows_mfma_kernel<6><<<N, 64, lds, st>>>(a); break;
+    case 7: assemble_rows_mfma_kernel<7><<<N, 64, lds, st>>>(a); break;
+    case 8: assemble_rows_mfma_kernel<8><<<N, 64, lds, st>>>(a); break;
+    default: assemble_rows_kernel<<<N, 64, lds, st>>>(a); break;
+  }
   NRF_LAUNCH_CHECK();
   return NRF_OK;
 }
@@ -1154,6 +1342,9 @@ extern "C" nrf_solver *nrf_solver_create(const nrf_solver_params *params, int ma
   // follow cooperative launches need it.
   const char *fc = getenv("NRF_PCG_COOP");
   if (fc && fc[0] == '0') s->coop = 0;
+  // NRF_ASM_MFMA=0 assembles the data term with the scalar LDS form
+  const char *fm = getenv("NRF_ASM_MFMA");
+  if (fm && fm[0] == '0') s->asm_mfma = 0;
   const char *fg = getenv("NRF_PCG_WORKGROUPS");
   if (fg) s->forced_G = atoi(fg);
   const size_t n6 = (size_t)max_nodes * 6;

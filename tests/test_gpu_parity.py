@@ -269,10 +269,12 @@ def solver_inputs(sphere, orc):
     return dict(wf=wf, dcorr=dcorr, dcan=dcan, hcorr=hcorr, hcan=hcan)
 
 
+@pytest.mark.parametrize("form", ["mfma", "scalar"])
 @pytest.mark.parametrize("consistent", [0, 1])
-def test_solver_system_and_pcg(sphere, orc, solver_inputs, consistent):
+def test_solver_system_and_pcg(sphere, orc, solver_inputs, consistent, form, monkeypatch):
     """solve(): one linearisation with the stored src (GaussNewtonSolver::solve),
-    in the reference's ARAP sign convention and in the opt-in consistent one."""
+    in the reference's ARAP sign convention and in the opt-in consistent one,
+    with the data term on the matrix cores and in the scalar LDS form."""
     import torch
 
     from cudafusion_amd import _lib as L
@@ -296,6 +298,7 @@ def test_solver_system_and_pcg(sphere, orc, solver_inputs, consistent):
     dcorr = torch.from_numpy(corrs.view(np.uint8).copy()).cuda()
     p = GaussNewtonSolver.Params(pcg_iterations=20, pcg_tolerance=0.0, lambda_smooth=w.lambda_smooth,
                                  arap_consistent_sign=bool(consistent))
+    monkeypatch.setenv("NRF_ASM_MFMA", "1" if form == "mfma" else "0")  # read when the solver is created
     s = GaussNewtonSolver(p, 8192)
     dx = torch.zeros(6 * N, dtype=torch.float32, device="cuda")
     res = s.solve(dcorr, len(corrs), wf.device_nodes(), wf.device_transforms(), N, dx)
