@@ -79,6 +79,8 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="before the timed steps, run one elementwise copy of MIB MiB (x * 1 -> y): a kernel of "
                          "known HBM traffic that calibrates hardware byte counters")
+    ap.add_argument("--textbook-pcg", action="store_true",
+                    help="A p recomputed each iteration instead of A z + beta A p overlapping the all-reduce")
     ap.add_argument("--reference-policy", action="store_true",
                     help="reference ARAP rotation sign and whole-step rejection (frames lose GN work)")
     args = ap.parse_args()
@@ -97,7 +99,8 @@ def main() -> int:
 
     t0 = time.perf_counter()
     ref = args.reference_policy
-    h = Harness("body", frames=args.frames, use_graph=args.graph, arap_consistent_sign=not ref, validate=ref)
+    h = Harness("body", frames=args.frames, use_graph=args.graph, arap_consistent_sign=not ref, validate=ref,
+                pcg_pipelined=not args.textbook_pcg)
     h.setup()
     log(f"[bench] setup {time.perf_counter() - t0:.1f} s, {h.N} nodes, "
         f"{int(h.frames_host[1].valid.sum())} valid pixels in frame 1")
@@ -196,7 +199,8 @@ def main() -> int:
         "config": {"workload": "capsule-body 256^3 @4mm, 640x480, %d nodes, 5 GN x 30 PCG" % h.N,
                    "graph": bool(args.graph),
                    "solver_policy": ("reference (ARAP sign, whole-step rejection)" if ref else
-                                     "consistent ARAP sign, per-node clamp, no whole-step rejection")},
+                                     "consistent ARAP sign, per-node clamp, no whole-step rejection"),
+                   "pcg": "textbook" if args.textbook_pcg else "pipelined"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
                      "launch_ms": round(per_launch[dom], 5), "algorithmic_bytes": int(algo[dom])},

@@ -41,7 +41,7 @@ class SolverParams(C.Structure):
     _fields_ = [("gn_iterations", C.c_int32), ("pcg_iterations", C.c_int32), ("pcg_tolerance", C.c_float),
                 ("lambda_smooth", C.c_float), ("lambda_rot", C.c_float), ("lambda_damping", C.c_float),
                 ("debug", C.c_int32), ("relative_tolerance", C.c_int32), ("precond_fp64", C.c_int32),
-                ("arap_consistent_sign", C.c_int32)]
+                ("pcg_pipelined", C.c_int32), ("arap_consistent_sign", C.c_int32)]
 
 
 class GNParams(C.Structure):
@@ -171,9 +171,9 @@ def camera(fx, fy, cx, cy, width, height) -> Camera:
 
 def solver_params(gn_iterations=3, pcg_iterations=10, pcg_tolerance=1e-4, lambda_smooth=1.0, lambda_rot=1.0,
                   lambda_damping=1e-2, debug=0, relative_tolerance=0, precond_fp64=0,
-                  arap_consistent_sign=0) -> SolverParams:
+                  arap_consistent_sign=0, pcg_pipelined=0) -> SolverParams:
     return SolverParams(gn_iterations, pcg_iterations, pcg_tolerance, lambda_smooth, lambda_rot, lambda_damping,
-                        debug, relative_tolerance, precond_fp64, arap_consistent_sign)
+                        debug, relative_tolerance, precond_fp64, pcg_pipelined, arap_consistent_sign)
 
 
 def gn_params(gn_iterations=3, update_scale=0.25, max_update_rot=0.05, max_update_trans=0.03, validate=1,

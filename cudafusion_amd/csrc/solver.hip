@@ -594,27 +594,72 @@ constexpr int kPcgMaxWg = 256;                    // records (one per workgroup)
 constexpr int kPcgMaxPoints = 65535;              // synchronisation points per launch (16 tag bits)
 constexpr int kPcgWaves = kPcgThreads / 64;
 
-// Sums a pair of doubles over the workgroup in a fixed order (xor tree in
-// each wave, then the waves in index order); every lane gets the result.
-__device__ inline void block_sum2(double &v0, double &v1, double *red) {
-  for (int off = 32; off > 0; off >>= 1) {
-    v0 += __shfl_xor(v0, off, 64);
-    v1 += __shfl_xor(v1, off, 64);
-  }
+// Wave-level sum of a double in a fixed order (pairs, quads, half rows,
+// rows through DPP lane moves, then rows 0+1 and 2+3 and the two halves via
+// the row broadcasts); the total is in lane 63.  Every lane of the wave must
+// take part.
+template <int CTRL, int ROWMASK>
+__device__ inline double dpp_add(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWMASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, 0xF, false);
+  return v + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ inline double wave_sum_last(double v) {
+  v = dpp_add<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+  v = dpp_add<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+  v = dpp_add<0x141, 0xF>(v);  // row_half_mirror
+  v = dpp_add<0x140, 0xF>(v);  // row_mirror
+  v = dpp_add<0x142, 0xA>(v);  // row_bcast:15 into rows 1, 3
+  v = dpp_add<0x143, 0xC>(v);  // row_bcast:31 into rows 2, 3
+  return v;
+}
+
+// Workgroup sums of a pair of doubles.  red holds 2 * kPcgWaves (first
+// stage) + 8 (second stage) doubles; the two stages use separate parts so
+// that one barrier each suffices between consecutive reductions.
+//
+// First stage (this workgroup's partials): every wave reduces its lanes, the
+// wave sums are added in wave order by lanes 0..3, which are the lanes that
+// publish them.
+__device__ inline void wg_sum2_lanes03(double &v0, double &v1, double *red) {
+  const double s0 = wave_sum_last(v0), s1 = wave_sum_last(v1);
   const int w = threadIdx.x >> 6;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) {
-    red[2 * w] = v0;
-    red[2 * w + 1] = v1;
+  if ((threadIdx.x & 63) == 63) {
+    red[2 * w] = s0;
+    red[2 * w + 1] = s1;
   }
   __syncthreads();
-  double s0 = 0, s1 = 0;
-  for (int k = 0; k < kPcgWaves; ++k) {
-    s0 += red[2 * k];
-    s1 += red[2 * k + 1];
+  if (threadIdx.x < 4) {
+    double t0 = 0, t1 = 0;
+    for (int k = 0; k < kPcgWaves; ++k) {
+      t0 += red[2 * k];
+      t1 += red[2 * k + 1];
+    }
+    v0 = t0;
+    v1 = t1;
   }
-  v0 = s0;
-  v1 = s1;
+}
+// Second stage (the G records held by lanes 0..G-1): the waves that hold
+// records reduce them, every lane adds those wave sums in order.
+__device__ inline void wg_sum2_records(double &v0, double &v1, int G, double *red) {
+  double *r2 = red + 2 * kPcgWaves;
+  const int w = threadIdx.x >> 6, nw = (G + 63) >> 6;
+  if (w < nw) {
+    const double s0 = wave_sum_last(v0), s1 = wave_sum_last(v1);
+    if ((threadIdx.x & 63) == 63) {
+      r2[2 * w] = s0;
+      r2[2 * w + 1] = s1;
+    }
+  }
+  __syncthreads();
+  double t0 = 0, t1 = 0;
+  for (int k = 0; k < nw; ++k) {
+    t0 += r2[2 * k];
+    t1 += r2[2 * k + 1];
+  }
+  v0 = t0;
+  v1 = t1;
 }
 
 struct PcgArgs {
@@ -655,20 +700,25 @@ __device__ inline bool ll_spin(unsigned &spins, PcgCtl *ctl, int *bad) {
   return false;
 }
 
-// All-reduce of two doubles across the G workgroups: every workgroup
-// publishes its partials under `tag`, lanes 0..G-1 each read one workgroup's
-// record (its four words in one round trip) and the block sums them in lane
-// order, so every workgroup holds the same bits.  Records are double-buffered
-// by the parity of the point: a workgroup is at most one point ahead of the
-// slowest one.  Returns true when the solve has to be abandoned.
-__device__ inline bool allreduce2(const PcgArgs &a, unsigned tag, int slot, double &v0, double &v1, double *red,
-                                  int *s_bad) {
-  block_sum2(v0, v1, red);
+// All-reduce of two doubles across the G workgroups, in two halves so that
+// work can be placed between them.  ar2_publish: this workgroup's sums go
+// out under `tag` (lanes 0..3 write the hi / lo words).  ar2_collect: lanes
+// 0..G-1 each read one workgroup's record (its four words in one round trip
+// per poll) and the records are summed in lane order, so every workgroup
+// holds the same bits.  Records are double-buffered by the parity of the
+// point: a workgroup is at most one point ahead of the slowest one.
+// ar2_collect returns true when the solve has to be abandoned.
+__device__ inline void ar2_publish(const PcgArgs &a, unsigned tag, int slot, double v0, double v1, double *red) {
+  wg_sum2_lanes03(v0, v1, red);
   const int t = threadIdx.x;
   if (t < 4) {
     const ll_word bits = (ll_word)__double_as_longlong(t < 2 ? v0 : v1);
     ll_put(&a.recs[blockIdx.x].w[slot][t], tag, (t & 1) ? (unsigned)bits : (unsigned)(bits >> 32));
   }
+}
+__device__ inline bool ar2_collect(const PcgArgs &a, unsigned tag, int slot, double &v0, double &v1,
+                                   double *red, int *s_bad) {
+  const int t = threadIdx.x;
   double g0 = 0, g1 = 0;
   if (t < a.G) {
     const ll_word *w = a.recs[t].w[slot];
@@ -685,10 +735,15 @@ __device__ inline bool allreduce2(const PcgArgs &a, unsigned tag, int slot, doub
     g0 = __longlong_as_double((long long)((w0 << 32) | (w1 & 0xffffffffull)));
     g1 = __longlong_as_double((long long)((w2 << 32) | (w3 & 0xffffffffull)));
   }
-  block_sum2(g0, g1, red);  // its barriers also publish *s_bad
+  wg_sum2_records(g0, g1, a.G, red);  // its barrier also publishes *s_bad
   v0 = g0;
   v1 = g1;
   return *s_bad != 0;
+}
+__device__ inline bool allreduce2(const PcgArgs &a, unsigned tag, int slot, double &v0, double &v1, double *red,
+                                  int *s_bad) {
+  ar2_publish(a, tag, slot, v0, v1, red);
+  return ar2_collect(a, tag, slot, v0, v1, red, s_bad);
 }
 
 // Block-Jacobi PCG (kernels/solver_kernels.cu:616-687, Saad Alg. 9.1, x0 = 0)
@@ -707,7 +762,7 @@ __device__ inline bool allreduce2(const PcgArgs &a, unsigned tag, int slot, doub
 // bits, so the owner's p and its readers' copies are identical.
 __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
   extern __shared__ float lds[];
-  __shared__ double red[2 * kPcgWaves];
+  __shared__ double red[2 * kPcgWaves + 8];
   __shared__ int s_bad;
   __shared__ unsigned s_base;
   if (a.stop && *a.stop) return;
@@ -834,6 +889,162 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
     }
     betaf = (float)(rz_l / rz);
     rz = rz_l;
+  }
+  if (aborted) flags |= 8;
+  if (lead) a.x[s] = x;
+  if (g == 0 && t == 0) {
+    a.res->iterations = iters;
+    a.res->total_iterations += iters;
+    a.res->flags |= flags;
+    a.res->final_residual = sqrt(rr);
+    a.res->rhs_norm = sqrt(bb);
+  }
+}
+
+// Block-Jacobi PCG with the product overlapping the second all-reduce: the
+// recurrences of kernels/solver_kernels.cu:616-687 (Saad Alg. 9.1, x0 = 0)
+// with A p_{k+1} = A z_{k+1} + beta A p_k, so that the only operand another
+// workgroup's rows contribute is z, published before the (r.r, r.z)
+// all-reduce.  Per iteration:
+//   A: all-reduce p.Ap; alpha = r.z / p.Ap (breakdown |p.Ap| < 1e-12);
+//   B: x += alpha p, r -= alpha Ap, z = M^-1 r, publish z and the (r.r, r.z)
+//      partials; while they travel, gather the halo's z and form A z; then
+//      collect, stop on ||r|| < tol, beta = r.z / (r.z)_prev,
+//      p = z + beta p, Ap = A z + beta Ap.
+// Same iterates as pcg_kernel in exact arithmetic; in FP32 the one extra
+// recurrence (for Ap) keeps the iterates within a few ulps of the textbook
+// ones.  Layout, publication and the lane-per-row split are pcg_kernel's.
+__global__ __launch_bounds__(kPcgThreads) void pcg_pipe_kernel(PcgArgs a) {
+  extern __shared__ float lds[];
+  __shared__ double red[2 * kPcgWaves + 8];
+  __shared__ int s_bad;
+  __shared__ unsigned s_base;
+  if (a.stop && *a.stop) return;
+  const int g = blockIdx.x, t = threadIdx.x;
+  const int n0 = a.rows[g], n1 = a.rows[g + 1];
+  const int S = 6 * (n1 - n0);
+  const int blk0 = a.row_ptr[n0], nblk = a.row_ptr[n1] - blk0;
+  const int h0 = a.halo_off[g], nh = a.halo_off[g + 1] - h0;
+  float *Al = lds;
+  int32_t *Ll = reinterpret_cast<int32_t *>(Al + (a.lds_a ? (size_t)nblk * 36 : 0));
+  float *ph = reinterpret_cast<float *>(Ll + (a.lds_a ? nblk : 0));
+  float *rl = ph + (size_t)nh * 6;  // own rows' r, then own rows' z (kPcgRows each)
+  float *zl = rl + kPcgRows;
+  if (a.lds_a) {
+    const float4 *src = reinterpret_cast<const float4 *>(a.values + (size_t)blk0 * 36);
+    float4 *dst = reinterpret_cast<float4 *>(Al);
+    for (int j = t; j < nblk * 9; j += kPcgThreads) dst[j] = src[j];
+    for (int j = t; j < nblk; j += kPcgThreads) Ll[j] = a.lcol[blk0 + j];
+  }
+  if (t == 0) {
+    s_base = (unsigned)(__hip_atomic_load(&a.ctl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) << 16);
+    s_bad = 0;
+  }
+  const int q = t / kPcgSub, sub = t % kPcgSub;
+  const bool own = q < S, lead = own && sub == 0;
+  const int li = q / 6, row = q % 6, i = n0 + li, s = 6 * n0 + q;
+  float M[6] = {0, 0, 0, 0, 0, 0};
+  float bv = 0.f;
+  int rb = 0, re = 0;
+  if (own) {
+    for (int c = 0; c < 6; ++c) M[c] = a.Minv[(size_t)i * 36 + row * 6 + c];
+    bv = a.b[s];
+    rb = a.row_ptr[i];
+    re = a.row_ptr[i + 1];
+  }
+  // z = M^-1 r for this lane's row, published under `tag`; z of the owned rows
+  // is left in zl for the product
+  auto precond_publish = [&](float r, unsigned tag) -> float {
+    if (sub == 0) rl[q] = r;
+    __syncthreads();
+    float z = 0.f;
+    if (own) {
+      for (int c = 0; c < 6; ++c) z += M[c] * rl[li * 6 + c];
+      if (lead) {
+        ll_put(a.zpub + s, tag, __float_as_uint(z));
+        zl[q] = z;
+      }
+    }
+    return z;
+  };
+  // (A z) for this lane's row: owned rows' z from zl, the halo's from the
+  // words published under `tag`
+  auto product = [&](unsigned tag) -> float {
+    __syncthreads();
+    for (int e = t; e < nh * 6; e += kPcgThreads) {
+      const int nd = a.halo[h0 + e / 6], c = e % 6;
+      if (nd >= n0 && nd < n1) {
+        ph[e] = zl[(nd - n0) * 6 + c];
+      } else {
+        const ll_word *src = a.zpub + 6 * nd + c;
+        ll_word wv;
+        unsigned spins = 0;
+        for (;;) {
+          wv = ll_load(src);
+          if (ll_has(wv, tag)) break;
+          if (ll_spin(spins, a.ctl, &s_bad)) break;
+        }
+        ph[e] = __uint_as_float((unsigned)wv);
+      }
+    }
+    __syncthreads();
+    float y = 0.f;
+    if (own) {
+      for (int bk = rb + sub; bk < re; bk += kPcgSub) {
+        const int lc = a.lds_a ? Ll[bk - blk0] : a.lcol[bk];
+        const float *A = a.lds_a ? (Al + (size_t)(bk - blk0) * 36) : (a.values + (size_t)bk * 36);
+        const float *pj = ph + lc * 6;
+        for (int c = 0; c < 6; ++c) y += A[row * 6 + c] * pj[c];
+      }
+    }
+    y += __shfl_xor(y, 1, 64);
+    y += __shfl_xor(y, 2, 64);
+    return y;
+  };
+
+  __syncthreads();
+  const unsigned base = s_base;
+  // init: x = 0, r = b, z = M^-1 r, p = z, Ap = A z (kernels/solver_kernels.cu:620-632)
+  unsigned pt = 1;
+  float x = 0.f, r = bv;
+  float z = precond_publish(r, base + pt);
+  double rz = lead ? (double)r * (double)z : 0.0, bb = lead ? (double)bv * (double)bv : 0.0;
+  ar2_publish(a, base + pt, (int)(pt & 1), rz, bb, red);
+  float p = z, y = product(base + pt);
+  bool aborted = ar2_collect(a, base + pt, (int)(pt & 1), rz, bb, red, &s_bad);
+  double rr = bb;
+  const double tol = a.rel ? (double)a.tol * sqrt(bb) : (double)a.tol;
+  int iters = 0, flags = 0;
+  for (int it = 0; it < a.maxit && !aborted; ++it) {
+    ++pt;  // the p.Ap point
+    double pAp = lead ? (double)p * (double)y : 0.0, unused = 0.0;
+    ar2_publish(a, base + pt, (int)(pt & 1), pAp, unused, red);
+    aborted = ar2_collect(a, base + pt, (int)(pt & 1), pAp, unused, red, &s_bad);
+    if (aborted) break;
+    if (fabs(pAp) < 1e-12) {
+      flags |= 1;
+      break;
+    }
+    const float alf = (float)(rz / pAp);
+    ++pt;  // the (r.r, r.z) point
+    x += alf * p;
+    r += -alf * y;
+    z = precond_publish(r, base + pt);
+    double rr_l = lead ? (double)r * (double)r : 0.0, rz_l = lead ? (double)r * (double)z : 0.0;
+    ar2_publish(a, base + pt, (int)(pt & 1), rr_l, rz_l, red);
+    const float az = product(base + pt);
+    aborted = ar2_collect(a, base + pt, (int)(pt & 1), rr_l, rz_l, red, &s_bad);
+    if (aborted) break;
+    rr = rr_l;
+    iters = it + 1;
+    if (sqrt(rr) < tol) {
+      flags |= 2;
+      break;
+    }
+    const float betaf = (float)(rz_l / rz);
+    rz = rz_l;
+    p = z + betaf * p;
+    y = az + betaf * y;
   }
   if (aborted) flags |= 8;
   if (lead) a.x[s] = x;
@@ -1182,7 +1393,9 @@ static int solver_set_pattern_host(nrf_solver *s, const nrf_node *h, int N) {
   if ((st = upload(s->halo, s->c_halo, halo))) return st;
   if (s->pcg_lds > 65536 && s->pcg_lds > s->lds_attr) {
     if (hipFuncSetAttribute(reinterpret_cast<const void *>(pcg_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)s->pcg_lds) == hipSuccess) {
+                            (int)s->pcg_lds) == hipSuccess &&
+        hipFuncSetAttribute(reinterpret_cast<const void *>(pcg_pipe_kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->pcg_lds) == hipSuccess) {
       s->lds_attr = s->pcg_lds;
     } else {
       (void)hipGetLastError();
@@ -1292,11 +1505,15 @@ static int solver_pcg(nrf_solver *s, float *x, int N, hipStream_t st) {
   a.res = s->d_res;
   a.stop = s->stop;
   void *args[] = {&a};
+  const bool pipe = s->params.pcg_pipelined != 0;
   if (s->coop) {
-    NRF_HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(pcg_kernel), dim3(s->G),
-                                             dim3(kPcgThreads), args, (unsigned)s->pcg_lds, st));
+    const void *fn = pipe ? reinterpret_cast<const void *>(pcg_pipe_kernel) : reinterpret_cast<const void *>(pcg_kernel);
+    NRF_HIP_CHECK(hipLaunchCooperativeKernel(fn, dim3(s->G), dim3(kPcgThreads), args, (unsigned)s->pcg_lds, st));
   } else {
-    pcg_kernel<<<s->G, kPcgThreads, s->pcg_lds, st>>>(a);
+    if (pipe)
+      pcg_pipe_kernel<<<s->G, kPcgThreads, s->pcg_lds, st>>>(a);
+    else
+      pcg_kernel<<<s->G, kPcgThreads, s->pcg_lds, st>>>(a);
     NRF_LAUNCH_CHECK();
   }
   return NRF_OK;

@@ -235,12 +235,13 @@ class GaussNewtonSolver:
         relative_tolerance: bool = False
         precond_fp64: bool = False
         arap_consistent_sign: bool = False  # opt-in: ARAP rotation rows with the data term's sign
+        pcg_pipelined: bool = False  # opt-in: A p by recurrence, product overlapping an all-reduce
 
         def to_c(self) -> L.SolverParams:
             return L.solver_params(self.gn_iterations, self.pcg_iterations, self.pcg_tolerance, self.lambda_smooth,
                                    self.lambda_rot, self.lambda_damping, int(self.debug),
                                    int(self.relative_tolerance), int(self.precond_fp64),
-                                   int(self.arap_consistent_sign))
+                                   int(self.arap_consistent_sign), int(self.pcg_pipelined))
 
     def __init__(self, p: "GaussNewtonSolver.Params", max_nodes: int):
         self.lib = L.load()
@@ -257,6 +258,11 @@ class GaussNewtonSolver:
         if h:
             self.lib.nrf_solver_destroy(h)
             self.handle = None
+
+    def set_params(self, p: "GaussNewtonSolver.Params") -> None:
+        self.params = p
+        self._cp = p.to_c()
+        L.check(self.lib.nrf_solver_set_params(self.handle, C.byref(self._cp)), "nrf_solver_set_params")
 
     def set_pattern(self, host_nodes: np.ndarray) -> None:
         arr = np.ascontiguousarray(host_nodes, dtype=L.NODE)

@@ -62,7 +62,7 @@ def transform_errors(Ta, Tb):
 class Harness:
     def __init__(self, kind: str = "body", frames: int = 40, use_graph: bool = False, seed: int = 0,
                  pcg_tolerance: float = 0.0, relative_tolerance: bool = False, precond_fp64: bool = False,
-                 arap_consistent_sign: bool = False, validate: bool = True):
+                 arap_consistent_sign: bool = False, validate: bool = True, pcg_pipelined: bool = False):
         self.kind = kind
         self.scene = S.make_scene(kind, frames, seed)
         self.w = self.scene.w
@@ -72,7 +72,7 @@ class Harness:
         self.cam = L.camera(w.fx, w.fy, w.cx, w.cy, w.width, w.height)
         self.sp = L.solver_params(w.gn_iterations, w.pcg_iterations, pcg_tolerance, w.lambda_smooth, 1.0,
                                   w.lambda_damping, 0, int(relative_tolerance), int(precond_fp64),
-                                  int(arap_consistent_sign))
+                                  int(arap_consistent_sign), int(pcg_pipelined))
         self.gn = L.gn_params(w.gn_iterations, w.update_scale, w.max_update_rot, w.max_update_trans,
                               int(validate), w.max_dx_mean)
         self.use_graph = use_graph

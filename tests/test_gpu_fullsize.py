@@ -89,14 +89,14 @@ def test_marching_cubes_full_size(body, orc):
 def test_tracking_sequence_consistent_policy(gpu, orc):
     """Eight frames of the sequence through the pipeline with the benchmark's
     solver policy (consistent ARAP sign, per-node clamp, no whole-step
-    rejection).  Per frame: the node transforms equal the FP64 oracle's solve
+    rejection, pipelined PCG).  Per frame: the node transforms equal the FP64 oracle's solve
     from the same starting transforms (1e-4 rad / 1e-5 m), every GN step is
     applied, and the point-to-plane RMS of the correspondences after the solve
     is at the depth-noise level (sigma = 0.0012 z^2 = 1.7 mm at 1.2 m)."""
     import torch
 
     from cudafusion_amd.harness import Harness, transform_errors
-    h = Harness("body", frames=9, arap_consistent_sign=True, validate=False)
+    h = Harness("body", frames=9, arap_consistent_sign=True, validate=False, pcg_pipelined=True)
     h.setup()
     N = h.N
 

@@ -269,12 +269,14 @@ def solver_inputs(sphere, orc):
     return dict(wf=wf, dcorr=dcorr, dcan=dcan, hcorr=hcorr, hcan=hcan)
 
 
+@pytest.mark.parametrize("pcg", ["textbook", "pipelined"])
 @pytest.mark.parametrize("form", ["mfma", "scalar"])
 @pytest.mark.parametrize("consistent", [0, 1])
-def test_solver_system_and_pcg(sphere, orc, solver_inputs, consistent, form, monkeypatch):
+def test_solver_system_and_pcg(sphere, orc, solver_inputs, consistent, form, pcg, monkeypatch):
     """solve(): one linearisation with the stored src (GaussNewtonSolver::solve),
     in the reference's ARAP sign convention and in the opt-in consistent one,
-    with the data term on the matrix cores and in the scalar LDS form."""
+    with the data term on the matrix cores and in the scalar LDS form, and
+    with the textbook and the pipelined PCG recurrences."""
     import torch
 
     from cudafusion_amd import _lib as L
@@ -297,7 +299,7 @@ def test_solver_system_and_pcg(sphere, orc, solver_inputs, consistent, form, mon
     corrs["src"] = src.reshape(-1, 3)
     dcorr = torch.from_numpy(corrs.view(np.uint8).copy()).cuda()
     p = GaussNewtonSolver.Params(pcg_iterations=20, pcg_tolerance=0.0, lambda_smooth=w.lambda_smooth,
-                                 arap_consistent_sign=bool(consistent))
+                                 arap_consistent_sign=bool(consistent), pcg_pipelined=pcg == "pipelined")
     monkeypatch.setenv("NRF_ASM_MFMA", "1" if form == "mfma" else "0")  # read when the solver is created
     s = GaussNewtonSolver(p, 8192)
     dx = torch.zeros(6 * N, dtype=torch.float32, device="cuda")
@@ -324,6 +326,13 @@ def test_solver_system_and_pcg(sphere, orc, solver_inputs, consistent, form, mon
     gx = dx.cpu().numpy()
     assert np.linalg.norm(gx - x64) <= 1e-3 * np.linalg.norm(x64)
     assert abs(res.final_residual - r64) <= 1e-2 * r64 + 1e-9
+    # stopping rule: relative tolerance 1e-3 ends at the oracle's iteration (+-1 at the threshold)
+    p.pcg_iterations, p.pcg_tolerance, p.relative_tolerance = 200, 1e-3, True
+    s.set_params(p)
+    res2 = s.solve(dcorr, len(corrs), wf.device_nodes(), wf.device_transforms(), N, dx)
+    _, it64, fl64, r64b, _ = orc.pcg_f64(rp, ci, A64, b64, M64, 200, 1e-3, rel=1)
+    print(f"[pcg {pcg}] rel 1e-3: gpu {res2.iterations} it (flags {res2.flags}), oracle {it64} it", flush=True)
+    assert (res2.flags & 2) and (fl64 & 2) and abs(res2.iterations - it64) <= 1
 
 
 @pytest.mark.parametrize("consistent", [0, 1])
