@@ -188,6 +188,7 @@ struct AsmArgs {
   int Rmax;
   float *values, *b, *Minv;
   const int32_t *stop;
+  const float *crec;  // per-corr records (corr_prep_kernel), MFMA form only
 };
 
 // staging layout per entry (floats): fw, r, kslot, pos[4] (as float bits), J[4][6]
@@ -462,6 +463,54 @@ __global__ __launch_bounds__(64) void assemble_rows_kernel(AsmArgs a) {
   row_tail(a, i, rb, rlen, acc, stage, bacc);
 }
 
+// Per-correspondence record of one GN iteration, computed once per valid
+// corr instead of once per membership (each corr belongs to up to four
+// rows): [fw, r, id0..id3 (int bits, -1 = slot takes no part), J[4][6],
+// pad] = 32 floats (kRec).  Same expressions as stage_membership.
+constexpr int kRec = 32;
+__global__ __launch_bounds__(256) void corr_prep_kernel(AsmArgs a, int n, float *__restrict__ rec) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const nrf_corr &cr = a.corrs[c];
+  if (!cr.valid) return;
+  const float3 dst = f3(cr.dst), nrm = f3(cr.normal);
+  const int ids[4] = {cr.node_ids[0], cr.node_ids[1], cr.node_ids[2], cr.node_ids[3]};
+  const float ws[4] = {cr.node_ws[0], cr.node_ws[1], cr.node_ws[2], cr.node_ws[3]};
+  float3 src;
+  if (a.canonical) {
+    src = f3(a.canonical[3 * c], a.canonical[3 * c + 1], a.canonical[3 * c + 2]);
+    DQ bl;
+    if (dq_blend4(a.centered, ids, ws, bl)) src = dq_transform(bl, src);
+  } else {
+    src = f3(cr.src);
+  }
+  const float r = dot(nrm, sub(src, dst));
+  const float fw = fmaxf(cr.weight, 0.0f) * huber_w(r, 0.2f);
+  float o[kRec];
+  o[0] = fw;
+  o[1] = r;
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    const int id = ids[l];
+    const bool ok = id >= 0 && id < a.N && ws[l] > 1e-8f;
+    o[2 + l] = __int_as_float(ok ? id : -1);
+    float J[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (ok) {
+      const float3 xp = f3(a.nodes[id].pos);
+      const float3 lp = sub(src, xp);
+      const float3 cp = cross(lp, nrm);
+      const float w = ws[l];
+      J[0] = w * cp.x; J[1] = w * cp.y; J[2] = w * cp.z;
+      J[3] = w * nrm.x; J[4] = w * nrm.y; J[5] = w * nrm.z;
+    }
+    for (int q = 0; q < 6; ++q) o[6 + 6 * l + q] = J[q];
+  }
+  o[30] = o[31] = 0.f;
+  float4 *dst4 = reinterpret_cast<float4 *>(rec + (size_t)c * kRec);
+#pragma unroll
+  for (int k = 0; k < kRec / 4; ++k) dst4[k] = make_float4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+}
+
 // MFMA form of the data term (NS = column slabs of 16 covering the 6 R
 // block columns plus one RHS column).  For a batch of 64 memberships e of
 // row i, with a_e = f_e J_k(e) (the row node's Jacobian times the Huber
@@ -484,8 +533,8 @@ __global__ __launch_bounds__(64) void assemble_rows_mfma_kernel(AsmArgs a) {
   __shared__ float s_rhs[6];
   for (int j = lane; j < rlen; j += 64) cols[j] = a.col_idx[rb + j];
   __syncthreads();
-  // staging layout per membership: a[6], r, packed pos (4 bytes), J[4][6]
-  constexpr int kA = 0, kR = 6, kP = 7, kJ = 8;
+  // staging layout per membership: a[6], r, packed pos (4 bytes), J[4][6], 0
+  constexpr int kA = 0, kR = 6, kP = 7, kJ = 8, kZ = 32;
   typedef float floatx4 __attribute__((ext_vector_type(4)));
   floatx4 C[NS];
 #pragma unroll
@@ -493,49 +542,74 @@ __global__ __launch_bounds__(64) void assemble_rows_mfma_kernel(AsmArgs a) {
   const int ncol = 6 * rlen + 1;  // block columns + RHS column
   const int am = lane % 16;       // A row / B, D column within a slab
   const int kq = lane / 16;       // k within a 4-step
+  // this lane's B / D column of every slab: block, entry within the block,
+  // and whether it is the RHS column (no block matches a column >= 6 R)
+  unsigned col_blk[NS];
+  int col_c[NS];
+  bool col_rhs[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int j = 16 * s + am;
+    col_blk[s] = (unsigned)(j / 6);
+    col_c[s] = j % 6;
+    col_rhs[s] = j == ncol - 1;
+  }
   const int mb = a.m_off[i], me = a.m_off[i + 1];
   for (int base = mb; base < me; base += 64) {
     const int e = base + lane;
     float *st = stage + lane * kStage;
-    float fw = 0.f, r = 0.f, J[4][6];
-    int kslot = 0, pos[4] = {-1, -1, -1, -1};
-    if (e < me) fw = stage_membership(a, a.m_keys[e], cols, rlen, r, kslot, J, pos);
+    float R[kRec];
+    int kslot = 0;
+    if (e < me) {
+      const int key = a.m_keys[e];
+      kslot = key & 3;
+      const float4 *src4 = reinterpret_cast<const float4 *>(a.crec + (size_t)(key >> 2) * kRec);
+#pragma unroll
+      for (int k = 0; k < kRec / 4; ++k) {
+        const float4 v = src4[k];
+        R[4 * k] = v.x; R[4 * k + 1] = v.y; R[4 * k + 2] = v.z; R[4 * k + 3] = v.w;
+      }
+    } else {
+      for (int k = 0; k < kRec; ++k) R[k] = 0.f;
+    }
+    const float fw = R[0];
     // f <= 1e-8 skips the membership (kernels/solver_kernels.cu:84); NaN stays
     const bool act = e < me && !(fw <= 1e-8f);
-    for (int q = 0; q < 6; ++q) st[kA + q] = act ? fw * J[kslot][q] : 0.f;
-    st[kR] = act ? r : 0.f;
+    float Jk[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q)  // the row node's slot, selected without dynamic indexing
+      Jk[q] = kslot == 0 ? R[6 + q] : kslot == 1 ? R[12 + q] : kslot == 2 ? R[18 + q] : R[24 + q];
+    for (int q = 0; q < 6; ++q) st[kA + q] = act ? fw * Jk[q] : 0.f;
+    st[kR] = act ? R[1] : 0.f;
     unsigned packed = 0xffffffffu;
-    if (act)
-      for (int l = 0; l < 4; ++l)
-        if (pos[l] >= 0) packed = (packed & ~(0xffu << (8 * l))) | ((unsigned)pos[l] << (8 * l));
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const int id = __float_as_int(R[2 + l]);
+      int p = -1;
+      if (act && id >= 0)
+        for (int q = 0; q < rlen; ++q)  // rows are short; linear scan (absent: dropped, Q2)
+          if (cols[q] == id) { p = q; break; }
+      if (p >= 0) packed = (packed & ~(0xffu << (8 * l))) | ((unsigned)p << (8 * l));
+    }
     st[kP] = __uint_as_float(packed);
-    for (int l = 0; l < 4; ++l)
-      for (int q = 0; q < 6; ++q) st[kJ + 6 * l + q] = act ? J[l][q] : 0.f;
+    for (int q = 0; q < 24; ++q) st[kJ + q] = act ? R[6 + q] : 0.f;
+    st[kZ] = 0.f;
     __syncthreads();
     // k-steps outer, slabs inner: the slabs' products are independent, and
     // A and the packed positions are read once per k-step.  B[e][j] is one
-    // LDS read at the offset of the slot whose block is column j's (or of r
-    // for the RHS column), 0 when no slot of e maps there.
+    // LDS read at the offset of the slot whose block is column j's (r for the
+    // RHS column, the staged zero otherwise): straight-line code, no branches.
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
       const float *sq = stage + (4 * t + kq) * kStage;
-      const float av = am < 6 ? sq[kA + am] : 0.f;
+      const float av = sq[am < 6 ? kA + am : kZ];
       const unsigned pk = __float_as_uint(sq[kP]);
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
-        if (16 * s >= ncol) break;  // wave-uniform
-        const int j = 16 * s + am;  // this lane's B / D column
-        const unsigned jb = (unsigned)(j / 6);
-        int off = -1;
-        if (j < ncol - 1) {
+        int off = col_rhs[s] ? kR : kZ;
 #pragma unroll
-          for (int l = 0; l < 4; ++l)
-            if (((pk >> (8 * l)) & 0xffu) == jb) off = kJ + 6 * l + j % 6;
-        } else if (j == ncol - 1) {
-          off = kR;
-        }
-        const float bv = off >= 0 ? sq[off] : 0.f;
-        C[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, C[s], 0, 0, 0);
+        for (int l = 0; l < 4; ++l) off = (((pk >> (8 * l)) & 0xffu) == col_blk[s]) ? kJ + 6 * l + col_c[s] : off;
+        C[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, sq[off], C[s], 0, 0, 0);
       }
     }
     __syncthreads();
@@ -1319,6 +1393,8 @@ struct nrf_solver {
   // GN state
   int32_t *stop = nullptr;
   size_t c_stop = 0;
+  float *crec = nullptr;  // per-corr records of the MFMA assembly
+  size_t c_crec = 0;
   nrf_pcg_result *d_res = nullptr;
   size_t c_res = 0;
   nrf_solver_totals *d_tot = nullptr;
@@ -1430,8 +1506,8 @@ static int solver_build_memberships(nrf_solver *s, const nrf_corr *corrs, int n,
   return NRF_OK;
 }
 
-static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, const float *canonical, const nrf_node *nodes,
-                           const DQ *T, const DQ *centered, int N, hipStream_t st) {
+static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, int n, const float *canonical,
+                           const nrf_node *nodes, const DQ *T, const DQ *centered, int N, hipStream_t st) {
   AsmArgs a;
   a.corrs = corrs;
   a.canonical = canonical;
@@ -1462,7 +1538,16 @@ static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, const float *ca
   const size_t lds = (size_t)s->P.Rmax * 36 * 4 + (size_t)s->P.Rmax * 4 + 64 * kStage * 4;
   if (lds > 64 * 1024) return NRF_ERR_TOO_LARGE;
   const int ns = (6 * s->P.Rmax + 1 + 15) / 16;
-  switch (s->asm_mfma ? ns : 0) {
+  const bool mfma = s->asm_mfma && ns <= 8;
+  a.crec = nullptr;
+  if (mfma && n > 0) {
+    int rc;
+    if ((rc = dev_alloc(s->crec, (size_t)n * kRec, s->c_crec))) return rc;
+    a.crec = s->crec;
+    corr_prep_kernel<<<(n + 255) / 256, 256, 0, st>>>(a, n, s->crec);
+    NRF_LAUNCH_CHECK();
+  }
+  switch (mfma ? ns : 0) {
     case 1: assemble_rows_mfma_kernel<1><<<N, 64, lds, st>>>(a); break;
     case 2: assemble_rows_mfma_kernel<2><<<N, 64, lds, st>>>(a); break;
     case 3: assemble_rows_mfma_kernel<3><<<N, 64, lds, st>>>(a); break;
@@ -1591,7 +1676,8 @@ extern "C" void nrf_solver_destroy(nrf_solver *s) {
   void *ptrs[] = {s->row_ptr, s->col_idx, s->diag_rel, s->e_i, s->e_j, s->es_off, s->es_entry, s->es_offrel,
                   s->rows, s->e_w, s->values, s->b, s->Minv, s->zpub, s->P0, s->P1, s->recs, s->ctl, s->lcol,
                   s->halo_off, s->halo,
-                  s->m_cnt, s->m_off, s->m_cur, s->m_keys, s->m_keys2, s->stop, s->d_res, s->d_tot};
+                  s->m_cnt, s->m_off, s->m_cur, s->m_keys, s->m_keys2, s->stop, s->d_res, s->d_tot,
+                  s->crec};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
@@ -1621,7 +1707,7 @@ extern "C" int nrf_solver_solve(nrf_solver *s, const nrf_corr *corrs, int num_co
   if ((rc = solver_build_memberships(s, corrs, num_corrs, num_nodes, st))) return rc;
   gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res, s->d_tot);
   NRF_LAUNCH_CHECK();
-  if ((rc = solver_assemble(s, corrs, nullptr, nodes, reinterpret_cast<const DQ *>(transforms), nullptr,
+  if ((rc = solver_assemble(s, corrs, num_corrs, nullptr, nodes, reinterpret_cast<const DQ *>(transforms), nullptr,
                             num_nodes, st)))
     return rc;
   if ((rc = solver_pcg(s, delta_x, num_nodes, st))) return rc;
@@ -1656,7 +1742,7 @@ extern "C" int nrf_solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const f
   g.stop = s->stop;
   g.res = s->d_res;
   for (int it = 0; it < std::max(1, gn->gn_iterations); ++it) {
-    if ((rc = solver_assemble(s, corrs, canonical, nodes, reinterpret_cast<const DQ *>(transforms),
+    if ((rc = solver_assemble(s, corrs, num_corrs, canonical, nodes, reinterpret_cast<const DQ *>(transforms),
                               reinterpret_cast<const DQ *>(centered), num_nodes, st)))
       return rc;
     sink_record(s, st);
