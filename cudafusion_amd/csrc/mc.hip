@@ -44,40 +44,79 @@ __global__ void fill_zero_kernel(int32_t *p, int n) {
   if (i < n) p[i] = 0;
 }
 
-__global__ __launch_bounds__(kMcBlock) void classify_kernel(const nrf_voxel *__restrict__ vox, int X, int Y,
-                                                            int Z, uint8_t *__restrict__ cases,
-                                                            int32_t *__restrict__ rowcnt) {
-  const int x = blockIdx.x * kMcBlock + threadIdx.x;
-  const int y = blockIdx.y, z = blockIdx.z;
-  const int idx = z * X * Y + y * X + x;
-  int cube = 0;
-  if (x < X - 1 && y < Y - 1 && z < Z - 1) {
-    cube = 0;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int vi = idx + kMcCorner[c][0] + kMcCorner[c][1] * X + kMcCorner[c][2] * X * Y;
-      const float w = vox[vi].weight;
-      const float t = vox[vi].tsdf;
-      if (w <= 0.f) cube = -1;
-      if (cube >= 0 && t > 0.f) cube |= (1 << c);
+// Case index of every cube (marching_cubes.cu:323-355 semantics: a corner
+// with weight <= 0 voids the cube, bit c is set when tsdf_c > 0 (Q9), cases 0
+// and 255 emit nothing) and the triangle count of every x-row of cubes.
+// One wave per (63-cube x-tile, y, chunk of kMcZChunk z-slices): lane l reads
+// voxel column x = 63 tile + l of rows y and y + 1 while walking z, keeps the
+// previous slice in registers and takes the x + 1 corners from lane l + 1,
+// so each voxel is read about twice instead of eight times.
+constexpr int kMcZChunk = 32;
+__global__ __launch_bounds__(64) void classify_kernel(const nrf_voxel *__restrict__ vox, int X, int Y, int Z,
+                                                      uint8_t *__restrict__ cases, int32_t *__restrict__ rowcnt) {
+  const int lane = threadIdx.x;
+  const int x = blockIdx.x * 63 + lane, y = blockIdx.y;
+  const int z0 = blockIdx.z * kMcZChunk, z1 = min(Z, z0 + kMcZChunk);
+  auto ld = [&](int yy, int zz, float &t, float &w) {
+    t = 0.f;
+    w = 0.f;
+    if (x < X && yy < Y && zz < Z) {
+      const nrf_voxel &v = vox[((size_t)zz * Y + yy) * X + x];
+      t = v.tsdf;
+      w = v.weight;
     }
-    if (cube <= 0 || cube >= 255) cube = 0;
+  };
+  float t00, w00, t10, w10;  // (x, y, z), (x, y + 1, z)
+  ld(y, z0, t00, w00);
+  ld(y + 1, z0, t10, w10);
+  for (int z = z0; z < z1; ++z) {
+    float t01, w01, t11, w11;  // (x, y, z + 1), (x, y + 1, z + 1)
+    ld(y, z + 1, t01, w01);
+    ld(y + 1, z + 1, t11, w11);
+    // corners in kMcCorner order: (0,0,0) (1,0,0) (1,1,0) (0,1,0) (0,0,1) (1,0,1) (1,1,1) (0,1,1)
+    const float ct[8] = {t00, __shfl_down(t00, 1, 64), __shfl_down(t10, 1, 64), t10,
+                         t01, __shfl_down(t01, 1, 64), __shfl_down(t11, 1, 64), t11};
+    const float cw[8] = {w00, __shfl_down(w00, 1, 64), __shfl_down(w10, 1, 64), w10,
+                         w01, __shfl_down(w01, 1, 64), __shfl_down(w11, 1, 64), w11};
+    int cube = 0;
+    if (lane < 63 && x < X - 1 && y < Y - 1 && z < Z - 1) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        if (cw[c] <= 0.f) cube = -1;
+        if (cube >= 0 && ct[c] > 0.f) cube |= (1 << c);
+      }
+      if (cube <= 0 || cube >= 255) cube = 0;
+    }
+    if (lane < 63 && x < X) cases[((size_t)z * Y + y) * X + x] = (uint8_t)cube;
+    int nt = kMcNTri[cube];
+    for (int off = 32; off > 0; off >>= 1) nt += __shfl_xor(nt, off, 64);
+    if (lane == 0 && nt) atomicAdd(&rowcnt[z * Y + y], nt);
+    t00 = t01; w00 = w01;
+    t10 = t11; w10 = w11;
   }
-  if (x < X) cases[idx] = (uint8_t)cube;
-  int nt = kMcNTri[cube];
-  for (int off = 32; off > 0; off >>= 1) nt += __shfl_xor(nt, off, 64);
-  if ((threadIdx.x & 63) == 0 && nt) atomicAdd(&rowcnt[z * Y + y], nt);
 }
 
-// Exclusive scan of n row counts by one workgroup of 1024 lanes.
+// Exclusive scan of n row counts by one workgroup of 1024 lanes.  Lane t
+// owns the contiguous chunk [t * per, (t + 1) * per) with per a multiple of
+// 4, read and written as 16-byte vectors (the chunk's loads are issued
+// together); the chunk sums are scanned in LDS.
 __global__ __launch_bounds__(1024) void scan_rows_kernel(const int32_t *__restrict__ cnt, int32_t *__restrict__ off,
                                                          int n, int32_t *__restrict__ counts) {
   __shared__ int32_t part[1024];
   const int t = threadIdx.x;
-  const int per = (n + 1023) / 1024;
-  const int b = t * per, e = min(n, b + per);
+  const int per = (((n + 1023) / 1024) + 3) & ~3;
+  const int b = min(n, t * per), e = min(n, b + per);
+  const int nv = (e - b) >> 2;  // whole vectors; the tail (< 4) only in the last non-empty chunk
+  const int4 *c4 = reinterpret_cast<const int4 *>(cnt + b);
   int s = 0;
-  for (int i = b; i < e; ++i) s += cnt[i];
+  for (int k0 = 0; k0 < nv; k0 += 16) {
+    int4 v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = (k0 + k < nv) ? c4[k0 + k] : make_int4(0, 0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s += v[k].x + v[k].y + v[k].z + v[k].w;
+  }
+  for (int i = b + 4 * nv; i < e; ++i) s += cnt[i];
   part[t] = s;
   __syncthreads();
   for (int d = 1; d < 1024; d <<= 1) {  // Hillis-Steele inclusive scan
@@ -87,7 +126,24 @@ __global__ __launch_bounds__(1024) void scan_rows_kernel(const int32_t *__restri
     __syncthreads();
   }
   int run = part[t] - s;
-  for (int i = b; i < e; ++i) {
+  int4 *o4 = reinterpret_cast<int4 *>(off + b);
+  for (int k0 = 0; k0 < nv; k0 += 16) {
+    int4 v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = (k0 + k < nv) ? c4[k0 + k] : make_int4(0, 0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k0 + k < nv) {
+        int4 w;
+        w.x = run; run += v[k].x;
+        w.y = run; run += v[k].y;
+        w.z = run; run += v[k].z;
+        w.w = run; run += v[k].w;
+        o4[k0 + k] = w;
+      }
+    }
+  }
+  for (int i = b + 4 * nv; i < e; ++i) {
     off[i] = run;
     run += cnt[i];
   }
@@ -211,7 +267,8 @@ extern "C" int nrf_mc_extract(const nrf_volume_params *vp, const nrf_voxel *voxe
   hipStream_t st = as_stream(stream);
   fill_zero_kernel<<<(R + 255) / 256, 256, 0, st>>>(rowcnt, R);
   NRF_LAUNCH_CHECK();
-  classify_kernel<<<dim3((X + kMcBlock - 1) / kMcBlock, Y, Z), kMcBlock, 0, st>>>(voxels, X, Y, Z, cases, rowcnt);
+  classify_kernel<<<dim3((X + 62) / 63, Y, (Z + kMcZChunk - 1) / kMcZChunk), 64, 0, st>>>(voxels, X, Y, Z, cases,
+                                                                                    rowcnt);
   NRF_LAUNCH_CHECK();
   scan_rows_kernel<<<1, 1024, 0, st>>>(rowcnt, rowoff, R, counts);
   NRF_LAUNCH_CHECK();

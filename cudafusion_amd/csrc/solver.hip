@@ -1155,9 +1155,9 @@ __global__ void gn_begin_kernel(int32_t *stop, nrf_pcg_result *res, nrf_solver_t
   }
 }
 
-// Step validation (include/fusion_app.h:409-491) and application with the
-// per-node clamp + scale of upload_scaled_delta_x (fusion_app.h:1353-1375)
-// followed by apply_twist_increment(..., 1.0) (src/warp_field.cu:1121-1141).
+// Step validation (include/fusion_app.h:409-491) by one workgroup: a
+// rejected step raises the stop flag, which turns the update and every
+// later kernel of the solve into no-ops.
 struct GnArgs {
   const float *dx;
   DQ *T;
@@ -1170,14 +1170,13 @@ struct GnArgs {
   nrf_pcg_result *res;
 };
 
-__global__ __launch_bounds__(1024) void gn_apply_kernel(GnArgs a) {
+__global__ __launch_bounds__(1024) void gn_validate_kernel(GnArgs a) {
   if (*a.stop) return;
   __shared__ double s_sum[4][1024 / 64];
   __shared__ float s_max[4][1024 / 64];
   __shared__ int s_fin[1024 / 64];
-  __shared__ int s_ok;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  if (a.validate) {
+  {
     double rs = 0, ts = 0, ers = 0, ets = 0;
     float rm = 0, tm = 0, erm = 0, etm = 0;
     int fin = 1;
@@ -1227,16 +1226,22 @@ __global__ __launch_bounds__(1024) void gn_apply_kernel(GnArgs a) {
                            (eff_rot_mean <= a.max_dx_mean && eff_trans_mean <= a.max_dx_mean);
       const bool ok = F && isfinite(eff_rot_mean) && isfinite(eff_trans_mean) && isfinite(M[2]) &&
                       isfinite(M[3]) && (M[2] < reject_rot) && (M[3] < reject_trans) && mean_ok && !bad_spike;
-      s_ok = ok ? 1 : 0;
       if (!ok) {
         *a.stop = 1;
         a.res->flags |= 4;
       }
     }
-    __syncthreads();
-    if (!s_ok) return;
   }
-  for (int i = t; i < a.N; i += 1024) {
+}
+
+// Application of the validated step, one node per lane: per-node clamp and
+// scale (upload_scaled_delta_x, include/fusion_app.h:1353-1375), then
+// apply_twist_increment(..., 1.0) (src/warp_field.cu:1121-1141) and the
+// centred-table refresh.
+__global__ __launch_bounds__(256) void gn_update_kernel(GnArgs a) {
+  if (*a.stop) return;
+  const int t = threadIdx.x;
+  for (int i = blockIdx.x * 256 + t; i < a.N; i += gridDim.x * 256) {
     const float *d = a.dx + i * 6;
     float tw[6];
     for (int k = 0; k < 6; ++k) tw[k] = d[k];
@@ -1264,7 +1269,7 @@ __global__ __launch_bounds__(1024) void gn_apply_kernel(GnArgs a) {
     a.T[i] = Tn;
     if (a.centered) a.centered[i] = dq_centered(Tn, f3(a.nodes[i].pos));
   }
-  if (t == 0) a.res->gn_done += 1;
+  if (blockIdx.x == 0 && t == 0) a.res->gn_done += 1;
 }
 
 }  // namespace
@@ -1748,7 +1753,11 @@ extern "C" int nrf_solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const f
     sink_record(s, st);
     if ((rc = solver_pcg(s, delta_x, num_nodes, st))) return rc;
     sink_record(s, st);
-    gn_apply_kernel<<<1, 1024, 0, st>>>(g);
+    if (g.validate) {
+      gn_validate_kernel<<<1, 1024, 0, st>>>(g);
+      NRF_LAUNCH_CHECK();
+    }
+    gn_update_kernel<<<(num_nodes + 255) / 256, 256, 0, st>>>(g);
     NRF_LAUNCH_CHECK();
     sink_record(s, st);
   }
