@@ -40,6 +40,31 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic():
+    """HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (profiles/hbm_traffic.json, calibrated on a copy of known size), keyed by
+    kernel group; None when the file is missing or was measured on other
+    kernel sources than the ones built here."""
+    import hashlib
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        meas = json.load(f)
+    csrc = os.path.join(ROOT, "cudafusion_amd", "csrc")
+    h = hashlib.sha256()
+    for fn in sorted(os.listdir(csrc)):
+        with open(os.path.join(csrc, fn), "rb") as f:
+            h.update(f.read())
+    if h.hexdigest() != meas.get("source_sha256"):
+        return None
+    per = meas["bytes_per_dispatch"]
+    out = dict(per)
+    if "mc_classify" in per and "mc_emit" in per:
+        out["mc"] = per["mc_classify"] + per["mc_emit"]
+    return out
+
+
 def cpu_baseline(h, frames: int, threads: int):
     """The oracle (C++/OpenMP restatement, FP32) on `frames` frames of the
     same workload: GN solve + warped integration + marching cubes."""
@@ -176,6 +201,7 @@ def main() -> int:
         per_step = dict(per_launch)
     dom = max(per_step, key=lambda k: per_step[k])
     achieved = algo[dom] / (per_launch[dom] * 1e-3) / 1e9
+    traffic = measured_traffic()
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -202,7 +228,8 @@ def main() -> int:
                                      "consistent ARAP sign, per-node clamp, no whole-step rejection"),
                    "pcg": "textbook" if args.textbook_pcg else "pipelined"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None if traffic is None or dom not in traffic else traffic[dom], "kernel": dom,
                      "launch_ms": round(per_launch[dom], 5), "algorithmic_bytes": int(algo[dom])},
         "kernel_ms_per_step": None if kern is None else dict(zip(
             ["membership", "assembly", "pcg", "update", "integrate", "mc"], [round(float(v), 5) for v in kern])),
