@@ -106,6 +106,10 @@ def main() -> int:
                          "known HBM traffic that calibrates hardware byte counters")
     ap.add_argument("--textbook-pcg", action="store_true",
                     help="A p recomputed each iteration instead of A z + beta A p overlapping the all-reduce")
+    ap.add_argument("--precision", choices=["bf16", "fp16"], default=None,
+                    help="precision form: FP16 TSDF + u8 weight, compact kNN, JtJ blocks in BF16 / FP16 on the "
+                         "matrix cores, FP64 Cholesky inverses, one FP64-residual refinement step per solve; "
+                         "reports errors against an FP64 run of one frame")
     ap.add_argument("--reference-policy", action="store_true",
                     help="reference ARAP rotation sign and whole-step rejection (frames lose GN work)")
     args = ap.parse_args()
@@ -124,8 +128,10 @@ def main() -> int:
 
     t0 = time.perf_counter()
     ref = args.reference_policy
+    prec = {"bf16": 1, "fp16": 2}.get(args.precision, 0)
     h = Harness("body", frames=args.frames, use_graph=args.graph, arap_consistent_sign=not ref, validate=ref,
-                pcg_pipelined=not args.textbook_pcg)
+                pcg_pipelined=not args.textbook_pcg, jtj_precision=prec, refine_iterations=10 if prec else 0,
+                precond_fp64=bool(prec), storage16=bool(prec))
     h.setup()
     log(f"[bench] setup {time.perf_counter() - t0:.1f} s, {h.N} nodes, "
         f"{int(h.frames_host[1].valid.sum())} valid pixels in frame 1")
@@ -240,9 +246,55 @@ def main() -> int:
                  "bsr_blocks": nb},
         "cpu_baseline": cpu,
     }
+    if prec:
+        out["config"]["workload"] += ", precision form (%s JtJ, FP16 TSDF + u8 weight, FP64 Cholesky and " \
+                                     "refinement)" % args.precision
+        out["dtype"] = args.precision + "/f16/f32"
+        out["errors_vs_fp64"] = precision_errors(h, seq[-1] % nf + 1)
     print(json.dumps(out))
     h.close()
     return 0
+
+
+def precision_errors(h, f):
+    """One more (untimed) frame f, and the same frame from the same state
+    run by the FP64 oracle: node transforms (max rotation / translation
+    difference), TSDF (max difference on voxels away from a rounding
+    boundary of the FP64 projection, in truncation units) and vertices
+    (largest distance from a device vertex to the nearest FP64 vertex, and
+    back, in voxels)."""
+    import numpy as np
+    import torch
+    from scipy.spatial import cKDTree
+
+    from cudafusion_amd.harness import transform_errors
+    from oracle import pyoracle as po
+    po.build()
+    T0 = h.download_transforms().copy()
+    vox0 = h.download_voxels().copy()
+    knn, knn_w = h.download_knn()
+    h.step(f)
+    torch.cuda.synchronize()
+    T1 = h.download_transforms().copy()
+    vox1 = h.download_voxels()
+    mesh = h.mesh().copy()
+    corrs, can = h.frame_corrs_host(f)
+    T64, _, _, _, _ = po.solve_gn_f64(corrs, can, h.nodes, T0.view(np.float32).reshape(-1, 8).astype(np.float64),
+                                      h.gn, h.sp)
+    er, et = transform_errors(T1, T64)
+    t64, w64 = vox0["tsdf"].astype(np.float64), vox0["weight"].astype(np.float64)
+    depth = h.frames_host[f].depth.reshape(-1)
+    po.integrate_f64(h.vp, t64, w64, depth, h.cam, h.pose, h.nodes, T64, knn, knn_w)
+    margins = po.integrate_margin_f64(h.vp, depth, h.cam, h.pose, h.nodes, T64, knn, knn_w)
+    tsdf_err, amb, wmis = po.tsdf_fp64_errors(vox1, t64, w64, margins, px_eps=2e-2, sdf_eps=2e-3)
+    _, v64, _ = po.mc_f64(h.vp, t64, w64, normals=False)
+    s = h.w.voxel_size
+    d1, _ = cKDTree(v64).query(mesh.reshape(-1, 3).astype(np.float64))
+    d2, _ = cKDTree(mesh.reshape(-1, 3).astype(np.float64)).query(v64)
+    return {"frame": int(f), "transform_rot_rad": float(er), "transform_trans_m": float(et),
+            "tsdf_max": float(tsdf_err), "tsdf_weight_mismatches": int(wmis), "tsdf_ambiguous_frac": float(amb),
+            "vertices_max_vox": float(max(d1.max(), d2.max()) / s), "vertices_p99_vox": float(np.percentile(d1, 99) / s),
+            "triangles": [int(len(mesh) // 3), int(len(v64) // 3)]}
 
 
 if __name__ == "__main__":

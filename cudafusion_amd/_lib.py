@@ -41,7 +41,8 @@ class SolverParams(C.Structure):
     _fields_ = [("gn_iterations", C.c_int32), ("pcg_iterations", C.c_int32), ("pcg_tolerance", C.c_float),
                 ("lambda_smooth", C.c_float), ("lambda_rot", C.c_float), ("lambda_damping", C.c_float),
                 ("debug", C.c_int32), ("relative_tolerance", C.c_int32), ("precond_fp64", C.c_int32),
-                ("pcg_pipelined", C.c_int32), ("arap_consistent_sign", C.c_int32)]
+                ("pcg_pipelined", C.c_int32), ("arap_consistent_sign", C.c_int32), ("jtj_precision", C.c_int32),
+                ("refine_iterations", C.c_int32)]
 
 
 class GNParams(C.Structure):
@@ -62,7 +63,7 @@ class SolverTotals(C.Structure):
 class PipelineConfig(C.Structure):
     _fields_ = [("volume", VolumeParams), ("camera", Camera), ("solver", SolverParams), ("gn", GNParams),
                 ("max_nodes", C.c_int32), ("max_corrs", C.c_int32), ("max_triangles", C.c_int32),
-                ("min_opt_count", C.c_int32), ("use_graph", C.c_int32)]
+                ("min_opt_count", C.c_int32), ("use_graph", C.c_int32), ("storage16", C.c_int32)]
 
 
 P = C.c_void_p
@@ -100,6 +101,7 @@ _SIGS = {
                               C.POINTER(P)]),
     "nrf_solver_device_result": (P, [P]),
     "nrf_solver_get_totals": (I, [P, C.POINTER(SolverTotals), P]),
+    "nrf_solver_true_residual": (I, [P, P, C.POINTER(C.c_double), P]),
     "nrf_pipeline_create": (P, [C.POINTER(PipelineConfig)]),
     "nrf_pipeline_destroy": (None, [P]),
     "nrf_pipeline_voxels": (P, [P]),
@@ -109,6 +111,7 @@ _SIGS = {
     "nrf_pipeline_knn_w": (P, [P]),
     "nrf_pipeline_mesh": (P, [P, C.POINTER(P)]),
     "nrf_pipeline_solver": (P, [P]),
+    "nrf_pipeline_volume16": (I, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
     "nrf_pipeline_integrate_rigid": (I, [P, P, P, P]),
     "nrf_pipeline_set_nodes": (I, [P, P, I]),
     "nrf_pipeline_step": (I, [P, P, P, P, P, I, P]),
@@ -171,9 +174,10 @@ def camera(fx, fy, cx, cy, width, height) -> Camera:
 
 def solver_params(gn_iterations=3, pcg_iterations=10, pcg_tolerance=1e-4, lambda_smooth=1.0, lambda_rot=1.0,
                   lambda_damping=1e-2, debug=0, relative_tolerance=0, precond_fp64=0,
-                  arap_consistent_sign=0, pcg_pipelined=0) -> SolverParams:
+                  arap_consistent_sign=0, pcg_pipelined=0, jtj_precision=0, refine_iterations=0) -> SolverParams:
     return SolverParams(gn_iterations, pcg_iterations, pcg_tolerance, lambda_smooth, lambda_rot, lambda_damping,
-                        debug, relative_tolerance, precond_fp64, pcg_pipelined, arap_consistent_sign)
+                        debug, relative_tolerance, precond_fp64, pcg_pipelined, arap_consistent_sign, jtj_precision,
+                        refine_iterations)
 
 
 def gn_params(gn_iterations=3, update_scale=0.25, max_update_rot=0.05, max_update_trans=0.03, validate=1,

@@ -9,12 +9,14 @@
 // Instead of the reference's two passes with global atomic output counters
 // (nondeterministic order, a host round trip between them) the extraction is
 // three stream-ordered kernels with no host synchronisation:
-//   1. classify: one lane per cube, writes the case byte and adds the row's
+//   1. classify (mc_front.h): one workgroup per 64 x 4 x 4 brick of cubes,
+//      voxels staged once in LDS; writes the case byte and adds the row's
 //      triangle count (x-rows of cubes, integer atomics, so deterministic);
 //   2. scan: exclusive prefix sum of the row counts (one workgroup);
 //   3. emit: one workgroup per non-empty row; a block scan of per-cube
 //      triangle counts places every triangle at its rank in cube order.
 // Output order is therefore increasing cube index, reproducible run to run.
+#include "mc_front.h"
 #include "mc_tables.h"
 #include "nrf_internal.h"
 #include "nrf_math.h"
@@ -39,122 +41,16 @@ McLayout mc_layout(const nrf_volume_params *vp) {
   return L;
 }
 
+struct VoxelReaderF32 {
+  const nrf_voxel *v;
+  __device__ float2 operator()(size_t i) const { return make_float2(v[i].tsdf, v[i].weight); }
+};
+
 __global__ void fill_zero_kernel(int32_t *p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
 
-// Case index of every cube (marching_cubes.cu:323-355 semantics: a corner
-// with weight <= 0 voids the cube, bit c is set when tsdf_c > 0 (Q9), cases 0
-// and 255 emit nothing) and the triangle count of every x-row of cubes.
-// One wave per (63-cube x-tile, y, chunk of kMcZChunk z-slices): lane l reads
-// voxel column x = 63 tile + l of rows y and y + 1 while walking z, keeps the
-// previous slice in registers and takes the x + 1 corners from lane l + 1,
-// so each voxel is read about twice instead of eight times.
-constexpr int kMcZChunk = 32;
-__global__ __launch_bounds__(64) void classify_kernel(const nrf_voxel *__restrict__ vox, int X, int Y, int Z,
-                                                      uint8_t *__restrict__ cases, int32_t *__restrict__ rowcnt) {
-  const int lane = threadIdx.x;
-  const int x = blockIdx.x * 63 + lane, y = blockIdx.y;
-  const int z0 = blockIdx.z * kMcZChunk, z1 = min(Z, z0 + kMcZChunk);
-  auto ld = [&](int yy, int zz, float &t, float &w) {
-    t = 0.f;
-    w = 0.f;
-    if (x < X && yy < Y && zz < Z) {
-      const nrf_voxel &v = vox[((size_t)zz * Y + yy) * X + x];
-      t = v.tsdf;
-      w = v.weight;
-    }
-  };
-  float t00, w00, t10, w10;  // (x, y, z), (x, y + 1, z)
-  ld(y, z0, t00, w00);
-  ld(y + 1, z0, t10, w10);
-  for (int z = z0; z < z1; ++z) {
-    float t01, w01, t11, w11;  // (x, y, z + 1), (x, y + 1, z + 1)
-    ld(y, z + 1, t01, w01);
-    ld(y + 1, z + 1, t11, w11);
-    // corners in kMcCorner order: (0,0,0) (1,0,0) (1,1,0) (0,1,0) (0,0,1) (1,0,1) (1,1,1) (0,1,1)
-    const float ct[8] = {t00, __shfl_down(t00, 1, 64), __shfl_down(t10, 1, 64), t10,
-                         t01, __shfl_down(t01, 1, 64), __shfl_down(t11, 1, 64), t11};
-    const float cw[8] = {w00, __shfl_down(w00, 1, 64), __shfl_down(w10, 1, 64), w10,
-                         w01, __shfl_down(w01, 1, 64), __shfl_down(w11, 1, 64), w11};
-    int cube = 0;
-    if (lane < 63 && x < X - 1 && y < Y - 1 && z < Z - 1) {
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        if (cw[c] <= 0.f) cube = -1;
-        if (cube >= 0 && ct[c] > 0.f) cube |= (1 << c);
-      }
-      if (cube <= 0 || cube >= 255) cube = 0;
-    }
-    if (lane < 63 && x < X) cases[((size_t)z * Y + y) * X + x] = (uint8_t)cube;
-    int nt = kMcNTri[cube];
-    for (int off = 32; off > 0; off >>= 1) nt += __shfl_xor(nt, off, 64);
-    if (lane == 0 && nt) atomicAdd(&rowcnt[z * Y + y], nt);
-    t00 = t01; w00 = w01;
-    t10 = t11; w10 = w11;
-  }
-}
-
-// Exclusive scan of n row counts by one workgroup of 1024 lanes.  Lane t
-// owns the contiguous chunk [t * per, (t + 1) * per) with per a multiple of
-// 4, read and written as 16-byte vectors (the chunk's loads are issued
-// together); the chunk sums are scanned in LDS.
-__global__ __launch_bounds__(1024) void scan_rows_kernel(const int32_t *__restrict__ cnt, int32_t *__restrict__ off,
-                                                         int n, int32_t *__restrict__ counts) {
-  __shared__ int32_t part[1024];
-  const int t = threadIdx.x;
-  const int per = (((n + 1023) / 1024) + 3) & ~3;
-  const int b = min(n, t * per), e = min(n, b + per);
-  const int nv = (e - b) >> 2;  // whole vectors; the tail (< 4) only in the last non-empty chunk
-  const int4 *c4 = reinterpret_cast<const int4 *>(cnt + b);
-  int s = 0;
-  for (int k0 = 0; k0 < nv; k0 += 16) {
-    int4 v[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) v[k] = (k0 + k < nv) ? c4[k0 + k] : make_int4(0, 0, 0, 0);
-#pragma unroll
-    for (int k = 0; k < 16; ++k) s += v[k].x + v[k].y + v[k].z + v[k].w;
-  }
-  for (int i = b + 4 * nv; i < e; ++i) s += cnt[i];
-  part[t] = s;
-  __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {  // Hillis-Steele inclusive scan
-    int v = (t >= d) ? part[t - d] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  int run = part[t] - s;
-  int4 *o4 = reinterpret_cast<int4 *>(off + b);
-  for (int k0 = 0; k0 < nv; k0 += 16) {
-    int4 v[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) v[k] = (k0 + k < nv) ? c4[k0 + k] : make_int4(0, 0, 0, 0);
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      if (k0 + k < nv) {
-        int4 w;
-        w.x = run; run += v[k].x;
-        w.y = run; run += v[k].y;
-        w.z = run; run += v[k].z;
-        w.w = run; run += v[k].w;
-        o4[k0 + k] = w;
-      }
-    }
-  }
-  for (int i = b + 4 * nv; i < e; ++i) {
-    off[i] = run;
-    run += cnt[i];
-  }
-  if (t == 1023) {
-    off[n] = part[1023];
-    if (counts) {
-      counts[0] = 3 * part[1023];
-      counts[1] = part[1023];
-    }
-  }
-}
 
 __device__ inline float3 edge_point(int e, const float *val, int x, int y, int z, float3 org, float s) {
   const int c0 = kMcEdge[e][0], c1 = kMcEdge[e][1];
@@ -267,8 +163,8 @@ extern "C" int nrf_mc_extract(const nrf_volume_params *vp, const nrf_voxel *voxe
   hipStream_t st = as_stream(stream);
   fill_zero_kernel<<<(R + 255) / 256, 256, 0, st>>>(rowcnt, R);
   NRF_LAUNCH_CHECK();
-  classify_kernel<<<dim3((X + 62) / 63, Y, (Z + kMcZChunk - 1) / kMcZChunk), 64, 0, st>>>(voxels, X, Y, Z, cases,
-                                                                                    rowcnt);
+  classify_brick_kernel<<<classify_brick_grid(X, Y, Z), kMcBrickThreads, 0, st>>>(VoxelReaderF32{voxels}, X, Y, Z,
+                                                                                   cases, rowcnt);
   NRF_LAUNCH_CHECK();
   scan_rows_kernel<<<1, 1024, 0, st>>>(rowcnt, rowoff, R, counts);
   NRF_LAUNCH_CHECK();

@@ -29,6 +29,9 @@ struct nrf_pipeline {
   int32_t *tris = nullptr;
   int32_t *counts = nullptr;
   int32_t *updated = nullptr;
+  // precision form (storage16)
+  uint16_t *t16 = nullptr, *k16 = nullptr, *kw16 = nullptr;
+  uint8_t *w8 = nullptr;
   nrf_solver *solver = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // graph replay state
@@ -46,7 +49,7 @@ struct nrf_pipeline {
 
 static void pipeline_free(nrf_pipeline *p) {
   void *ptrs[] = {p->vox, p->knn, p->knn_w, p->nodes, p->T, p->centered, p->delta_x, p->mc_ws,
-                  p->verts, p->tris, p->counts, p->updated};
+                  p->verts, p->tris, p->counts, p->updated, p->t16, p->w8, p->k16, p->kw16};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (auto &e : p->ev)
@@ -80,6 +83,13 @@ extern "C" nrf_pipeline *nrf_pipeline_create(const nrf_pipeline_config *cfg) {
   ok = ok && hipMalloc(&p->tris, T * 3 * sizeof(int32_t)) == hipSuccess;
   ok = ok && hipMalloc(&p->counts, 4 * sizeof(int32_t)) == hipSuccess;
   ok = ok && hipMalloc(&p->updated, 4 * sizeof(int32_t)) == hipSuccess;
+  if (cfg->storage16) {
+    ok = ok && hipMalloc(&p->t16, V * sizeof(uint16_t)) == hipSuccess;
+    ok = ok && hipMalloc(&p->w8, V * sizeof(uint8_t)) == hipSuccess;
+    ok = ok && hipMalloc(&p->k16, V * 4 * sizeof(uint16_t)) == hipSuccess;
+    ok = ok && hipMalloc(&p->kw16, V * 4 * sizeof(uint16_t)) == hipSuccess;
+    if (ok) ok = nrf_tsdf16_reset(&cfg->volume, p->t16, p->w8, nullptr) == NRF_OK;
+  }
   for (auto &e : p->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
   if (ok) {
     p->solver = nrf_solver_create(&cfg->solver, cfg->max_nodes);
@@ -113,6 +123,15 @@ extern "C" float *nrf_pipeline_mesh(nrf_pipeline *p, int32_t **counts) {
   return p->verts;
 }
 extern "C" nrf_solver *nrf_pipeline_solver(nrf_pipeline *p) { return p ? p->solver : nullptr; }
+extern "C" int nrf_pipeline_volume16(nrf_pipeline *p, uint16_t **tsdf16, uint8_t **weight8, uint16_t **knn16,
+                                     uint16_t **knn_w16) {
+  if (!p) return NRF_ERR_INVALID_ARGUMENT;
+  if (tsdf16) *tsdf16 = p->t16;
+  if (weight8) *weight8 = p->w8;
+  if (knn16) *knn16 = p->k16;
+  if (knn_w16) *knn_w16 = p->kw16;
+  return NRF_OK;
+}
 
 extern "C" int nrf_pipeline_integrate_rigid(nrf_pipeline *p, const float *depth, const float camera_pose[16],
                                             void *stream) {
@@ -140,6 +159,12 @@ extern "C" int nrf_pipeline_set_nodes(nrf_pipeline *p, const nrf_node *host_node
   if (rc) return rc;
   rc = nrf_voxel_knn(&p->cfg.volume, p->vox, p->nodes, num_nodes, p->knn, p->knn_w, nullptr);
   if (rc) return rc;
+  if (p->cfg.storage16) {  // the FP32 state so far, rounded once into the compact form
+    rc = nrf_tsdf16_from_voxels(&p->cfg.volume, p->vox, p->t16, p->w8, nullptr);
+    if (rc) return rc;
+    rc = nrf_knn_compact(&p->cfg.volume, p->knn, p->knn_w, p->k16, p->kw16, nullptr);
+    if (rc) return rc;
+  }
   NRF_HIP_CHECK(hipDeviceSynchronize());
   if (p->exec) {
     (void)hipGraphExecDestroy(p->exec);
@@ -165,13 +190,21 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
   if (profile) solver_set_event_sink(p->solver, nullptr, 0);
   if (rc) return rc;
   NRF_HIP_CHECK(hipEventRecord(p->ev[1], st));
-  rc = nrf_tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, p->nodes, p->T, p->N,
-                          p->centered, p->knn, p->knn_w, nullptr, p->cfg.min_opt_count, p->updated, stream);
+  if (p->cfg.storage16)
+    rc = nrf_tsdf16_integrate(&p->cfg.volume, p->t16, p->w8, depth, &p->cfg.camera, camera_pose, p->centered, p->N,
+                              p->k16, p->kw16, p->updated, stream);
+  else
+    rc = nrf_tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, p->nodes, p->T, p->N,
+                            p->centered, p->knn, p->knn_w, nullptr, p->cfg.min_opt_count, p->updated, stream);
   if (rc) return rc;
   NRF_HIP_CHECK(hipEventRecord(p->ev[2], st));
   if (profile) NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 2], st));
-  rc = nrf_mc_extract(&p->cfg.volume, p->vox, p->mc_ws, p->mc_bytes, p->verts, nullptr, p->tris,
-                      p->cfg.max_triangles, p->counts, stream);
+  if (p->cfg.storage16)
+    rc = nrf_mc16_extract(&p->cfg.volume, p->t16, p->w8, p->mc_ws, p->mc_bytes, p->verts, p->tris,
+                          p->cfg.max_triangles, p->counts, stream);
+  else
+    rc = nrf_mc_extract(&p->cfg.volume, p->vox, p->mc_ws, p->mc_bytes, p->verts, nullptr, p->tris,
+                        p->cfg.max_triangles, p->counts, stream);
   if (rc) return rc;
   NRF_HIP_CHECK(hipEventRecord(p->ev[3], st));
   if (profile) {

@@ -9,6 +9,7 @@
 // decay_alpha = 1 configurations since every update adds 1).
 #include <hip/hip_fp16.h>
 
+#include "mc_front.h"
 #include "mc_tables.h"
 #include "nrf_internal.h"
 #include "nrf_math.h"
@@ -144,58 +145,13 @@ __global__ void zero_i32(int32_t *p, int n) {
   if (i < n) p[i] = 0;
 }
 
-__global__ __launch_bounds__(kMcBlock) void classify16_kernel(const __half *__restrict__ t16,
-                                                              const uint8_t *__restrict__ w8, int X, int Y, int Z,
-                                                              uint8_t *__restrict__ cases,
-                                                              int32_t *__restrict__ rowcnt) {
-  const int x = blockIdx.x * kMcBlock + threadIdx.x;
-  const int y = blockIdx.y, z = blockIdx.z;
-  const int idx = z * X * Y + y * X + x;
-  int cube = 0;
-  if (x < X - 1 && y < Y - 1 && z < Z - 1) {
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int vi = idx + kMcCorner[c][0] + kMcCorner[c][1] * X + kMcCorner[c][2] * X * Y;
-      if (w8[vi] == 0) cube = -1;
-      if (cube >= 0 && __half2float(t16[vi]) > 0.f) cube |= (1 << c);
-    }
-    if (cube <= 0 || cube >= 255) cube = 0;
-  }
-  if (x < X) cases[idx] = (uint8_t)cube;
-  int nt = kMcNTri[cube];
-  for (int off = 32; off > 0; off >>= 1) nt += __shfl_xor(nt, off, 64);
-  if ((threadIdx.x & 63) == 0 && nt) atomicAdd(&rowcnt[z * Y + y], nt);
-}
+struct VoxelReaderF16 {
+  const __half *t;
+  const uint8_t *w;
+  __device__ float2 operator()(size_t i) const { return make_float2(__half2float(t[i]), (float)w[i]); }
+};
 
-__global__ __launch_bounds__(1024) void scan_rows16_kernel(const int32_t *__restrict__ cnt, int32_t *__restrict__ off,
-                                                           int n, int32_t *__restrict__ counts) {
-  __shared__ int32_t part[1024];
-  const int t = threadIdx.x;
-  const int per = (n + 1023) / 1024;
-  const int b = t * per, e = min(n, b + per);
-  int s = 0;
-  for (int i = b; i < e; ++i) s += cnt[i];
-  part[t] = s;
-  __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
-    int v = (t >= d) ? part[t - d] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  int run = part[t] - s;
-  for (int i = b; i < e; ++i) {
-    off[i] = run;
-    run += cnt[i];
-  }
-  if (t == 1023) {
-    off[n] = part[1023];
-    if (counts) {
-      counts[0] = 3 * part[1023];
-      counts[1] = part[1023];
-    }
-  }
-}
+
 
 __device__ inline float3 edge_point16(int e, const float *val, int x, int y, int z, float3 org, float s) {
   const int c0 = kMcEdge[e][0], c1 = kMcEdge[e][1];
@@ -348,10 +304,10 @@ extern "C" int nrf_mc16_extract(const nrf_volume_params *vp, const uint16_t *tsd
   const __half *t16 = reinterpret_cast<const __half *>(tsdf16);
   zero_i32<<<(unsigned)((R + 255) / 256), 256, 0, st>>>(rowcnt, (int)R);
   NRF_LAUNCH_CHECK();
-  classify16_kernel<<<dim3((X + kMcBlock - 1) / kMcBlock, Y, Z), kMcBlock, 0, st>>>(t16, weight8, X, Y, Z, cases,
-                                                                                    rowcnt);
+  classify_brick_kernel<<<classify_brick_grid(X, Y, Z), kMcBrickThreads, 0, st>>>(VoxelReaderF16{t16, weight8}, X,
+                                                                                   Y, Z, cases, rowcnt);
   NRF_LAUNCH_CHECK();
-  scan_rows16_kernel<<<1, 1024, 0, st>>>(rowcnt, rowoff, (int)R, counts);
+  scan_rows_kernel<<<1, 1024, 0, st>>>(rowcnt, rowoff, (int)R, counts);
   NRF_LAUNCH_CHECK();
   if (X >= 2 && Y >= 2 && Z >= 2 && max_triangles > 0) {
     emit16_kernel<<<dim3(Y, Z), kMcBlock, 0, st>>>(t16, X, Y, Z,

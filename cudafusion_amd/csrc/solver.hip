@@ -468,6 +468,7 @@ __global__ __launch_bounds__(64) void assemble_rows_kernel(AsmArgs a) {
 // rows): [fw, r, id0..id3 (int bits, -1 = slot takes no part), J[4][6],
 // pad] = 32 floats (kRec).  Same expressions as stage_membership.
 constexpr int kRec = 32;
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void corr_prep_kernel(AsmArgs a, int n, float *__restrict__ rec) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
@@ -520,7 +521,35 @@ __global__ __launch_bounds__(256) void corr_prep_kernel(AsmArgs a, int n, float 
 // as v_mfma_f32_16x16x4f32 products (rows 6..15 of A are zero).  Then the
 // blocks are C[:, 0:6R] and the RHS is -C[:, 6R].  Same products as the
 // scalar form, FP32, summed in the matrix unit's order.
-template <int NS>
+// Reduced-precision operands of the JtJ products: bf16 (round to nearest
+// even on the upper 16 bits of the FP32 pattern, NaN kept quiet) or fp16
+// (the hardware conversion, round to nearest even).
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+template <int PREC> struct LpType;
+template <> struct LpType<1> { typedef short elem; typedef bf16x4 vec; };
+template <> struct LpType<2> { typedef _Float16 elem; typedef f16x4 vec; };
+template <> struct LpType<0> { typedef short elem; typedef bf16x4 vec; };  // unused
+template <int PREC>
+__device__ inline typename LpType<PREC>::elem to_lp(float f) {
+  if constexpr (PREC == 2) {
+    return (_Float16)f;
+  } else {
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (short)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+  }
+}
+template <int PREC>
+__device__ inline floatx4_t mfma_lp(typename LpType<PREC>::vec a, typename LpType<PREC>::vec b, floatx4_t c) {
+  if constexpr (PREC == 2)
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+template <int NS, int PREC>
 __global__ __launch_bounds__(64) void assemble_rows_mfma_kernel(AsmArgs a) {
   extern __shared__ float lds[];
   const int i = blockIdx.x;
@@ -535,10 +564,12 @@ __global__ __launch_bounds__(64) void assemble_rows_mfma_kernel(AsmArgs a) {
   __syncthreads();
   // staging layout per membership: a[6], r, packed pos (4 bytes), J[4][6], 0
   constexpr int kA = 0, kR = 6, kP = 7, kJ = 8, kZ = 32;
-  typedef float floatx4 __attribute__((ext_vector_type(4)));
+  typedef floatx4_t floatx4;
+  typedef typename LpType<PREC>::vec lp4;
   floatx4 C[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) C[s] = floatx4{0.f, 0.f, 0.f, 0.f};
+  floatx4 Crhs = {0.f, 0.f, 0.f, 0.f};  // FP32 RHS column of the reduced-precision forms
   const int ncol = 6 * rlen + 1;  // block columns + RHS column
   const int am = lane % 16;       // A row / B, D column within a slab
   const int kq = lane / 16;       // k within a 4-step
@@ -595,21 +626,60 @@ __global__ __launch_bounds__(64) void assemble_rows_mfma_kernel(AsmArgs a) {
     for (int q = 0; q < 24; ++q) st[kJ + q] = act ? R[6 + q] : 0.f;
     st[kZ] = 0.f;
     __syncthreads();
-    // k-steps outer, slabs inner: the slabs' products are independent, and
-    // A and the packed positions are read once per k-step.  B[e][j] is one
-    // LDS read at the offset of the slot whose block is column j's (r for the
-    // RHS column, the staged zero otherwise): straight-line code, no branches.
+    if constexpr (PREC == 0) {
+      // k-steps outer, slabs inner: the slabs' products are independent, and
+      // A and the packed positions are read once per k-step.  B[e][j] is one
+      // LDS read at the offset of the slot whose block is column j's (r for
+      // the RHS column, the staged zero otherwise): straight-line code.
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      const float *sq = stage + (4 * t + kq) * kStage;
-      const float av = sq[am < 6 ? kA + am : kZ];
-      const unsigned pk = __float_as_uint(sq[kP]);
+      for (int t = 0; t < 16; ++t) {
+        const float *sq = stage + (4 * t + kq) * kStage;
+        const float av = sq[am < 6 ? kA + am : kZ];
+        const unsigned pk = __float_as_uint(sq[kP]);
 #pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        int off = col_rhs[s] ? kR : kZ;
+        for (int s = 0; s < NS; ++s) {
+          int off = col_rhs[s] ? kR : kZ;
 #pragma unroll
-        for (int l = 0; l < 4; ++l) off = (((pk >> (8 * l)) & 0xffu) == col_blk[s]) ? kJ + 6 * l + col_c[s] : off;
-        C[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, sq[off], C[s], 0, 0, 0);
+          for (int l = 0; l < 4; ++l)
+            off = (((pk >> (8 * l)) & 0xffu) == col_blk[s]) ? kJ + 6 * l + col_c[s] : off;
+          C[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, sq[off], C[s], 0, 0, 0);
+        }
+      }
+    } else {
+      // JtJ blocks in BF16 (PREC 1) or FP16 (PREC 2) with FP32 accumulation:
+      // 16 memberships per product, lane (am, kq) supplying memberships
+      // 16 t + 4 kq + q, q = 0..3.  The RHS column stays FP32 (below), so
+      // the reduced precision touches the Hessian approximation only.
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const float *sqq[4];
+        unsigned pk[4];
+        lp4 av;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          sqq[q] = stage + (16 * t + 4 * kq + q) * kStage;
+          pk[q] = __float_as_uint(sqq[q][kP]);
+          av[q] = to_lp<PREC>(sqq[q][am < 6 ? kA + am : kZ]);
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+          lp4 bv;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            int off = kZ;
+#pragma unroll
+            for (int l = 0; l < 4; ++l)
+              off = (((pk[q] >> (8 * l)) & 0xffu) == col_blk[s]) ? kJ + 6 * l + col_c[s] : off;
+            bv[q] = to_lp<PREC>(sqq[q][off]);
+          }
+          C[s] = mfma_lp<PREC>(av, bv, C[s]);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const float *sq = stage + (4 * t + kq) * kStage;
+        Crhs = __builtin_amdgcn_mfma_f32_16x16x4f32(sq[am < 6 ? kA + am : kZ], am == 0 ? sq[kR] : 0.f, Crhs, 0, 0,
+                                                    0);
       }
     }
     __syncthreads();
@@ -627,12 +697,15 @@ __global__ __launch_bounds__(64) void assemble_rows_mfma_kernel(AsmArgs a) {
         if (m < 6) {
           if (j < ncol - 1)
             acc[(j / 6) * 36 + m * 6 + j % 6] = C[s][v];
-          else
+          else if (PREC == 0)
             s_rhs[m] = -C[s][v];
         }
       }
     }
   }
+  if (PREC != 0 && lane < 32 && am == 0)  // column 0 of the FP32 RHS product
+    for (int v = 0; v < 4; ++v)
+      if (4 * kq + v < 6) s_rhs[4 * kq + v] = -Crhs[v];
   __syncthreads();
   const int ra = lane - 36;
   const float bacc = (ra >= 0 && ra < 6) ? s_rhs[ra] : 0.f;
@@ -1131,6 +1204,68 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_pipe_kernel(PcgArgs a) {
   }
 }
 
+// r = b - A x for every scalar row, products and sums in FP64 (the matrix
+// and x are FP32); optionally stores r (rounded to FP32) and per-workgroup
+// partial sums of |r|^2 and |b|^2 for the true relative residual.
+constexpr int kResThreads = 256;
+__global__ __launch_bounds__(kResThreads) void residual_fp64_kernel(const int32_t *__restrict__ row_ptr,
+                                                                    const int32_t *__restrict__ col_idx,
+                                                                    const float *__restrict__ values,
+                                                                    const float *__restrict__ b,
+                                                                    const float *__restrict__ x, int N,
+                                                                    float *__restrict__ r_out,
+                                                                    double *__restrict__ partial) {
+  __shared__ double red[2 * kResThreads / 64];
+  const int q = blockIdx.x * kResThreads + threadIdx.x;
+  double rr = 0.0, bb = 0.0;
+  if (q < 6 * N) {
+    const int i = q / 6, row = q % 6;
+    double acc = 0.0;
+    for (int bk = row_ptr[i]; bk < row_ptr[i + 1]; ++bk) {
+      const float *A = values + (size_t)bk * 36 + row * 6;
+      const float *xj = x + 6 * col_idx[bk];
+      for (int c = 0; c < 6; ++c) acc += (double)A[c] * (double)xj[c];
+    }
+    const double r = (double)b[q] - acc;
+    if (r_out) r_out[q] = (float)r;
+    rr = r * r;
+    bb = (double)b[q] * (double)b[q];
+  }
+  rr = wave_sum_last(rr);
+  bb = wave_sum_last(bb);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 63) {
+    red[2 * w] = rr;
+    red[2 * w + 1] = bb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && partial) {
+    double s0 = 0, s1 = 0;
+    for (int k = 0; k < kResThreads / 64; ++k) {
+      s0 += red[2 * k];
+      s1 += red[2 * k + 1];
+    }
+    partial[2 * blockIdx.x] = s0;
+    partial[2 * blockIdx.x + 1] = s1;
+  }
+}
+// Sums the per-workgroup partials in index order: out = {|b - Ax|, |b|}.
+__global__ void residual_norms_kernel(const double *__restrict__ partial, int nblocks, double *__restrict__ out) {
+  if (threadIdx.x == 0) {
+    double s0 = 0, s1 = 0;
+    for (int k = 0; k < nblocks; ++k) {
+      s0 += partial[2 * k];
+      s1 += partial[2 * k + 1];
+    }
+    out[0] = sqrt(s0);
+    out[1] = sqrt(s1);
+  }
+}
+__global__ void axpy_kernel(float *__restrict__ x, const float *__restrict__ d, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) x[q] += d[q];
+}
+
 __global__ void pcg_prep_kernel(PcgCtl *ctl) {
   if (threadIdx.x == 0) {
     ctl->epoch += 1;
@@ -1400,6 +1535,10 @@ struct nrf_solver {
   size_t c_stop = 0;
   float *crec = nullptr;  // per-corr records of the MFMA assembly
   size_t c_crec = 0;
+  float *rres = nullptr, *dref = nullptr;  // refinement: residual and correction
+  size_t c_rres = 0, c_dref = 0;
+  double *rpart = nullptr, *rnorm = nullptr;  // true residual partials and norms
+  size_t c_rpart = 0, c_rnorm = 0;
   nrf_pcg_result *d_res = nullptr;
   size_t c_res = 0;
   nrf_solver_totals *d_tot = nullptr;
@@ -1552,30 +1691,62 @@ static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, int n, const fl
     corr_prep_kernel<<<(n + 255) / 256, 256, 0, st>>>(a, n, s->crec);
     NRF_LAUNCH_CHECK();
   }
+  const int prec = s->params.jtj_precision;
+  if (!mfma && prec != 0) return NRF_ERR_INVALID_ARGUMENT;  // reduced precision exists in the MFMA form only
+#define NRF_ASM_CASE(NSV)                                                                       \
+  case NSV:                                                                                     \
+    if (prec == 1)                                                                              \
+      assemble_rows_mfma_kernel<NSV, 1><<<N, 64, lds, st>>>(a);                                 \
+    else if (prec == 2)                                                                         \
+      assemble_rows_mfma_kernel<NSV, 2><<<N, 64, lds, st>>>(a);                                 \
+    else                                                                                        \
+      assemble_rows_mfma_kernel<NSV, 0><<<N, 64, lds, st>>>(a);                                 \
+    break;
   switch (mfma ? ns : 0) {
-    case 1: assemble_rows_mfma_kernel<1><<<N, 64, lds, st>>>(a); break;
-    case 2: assemble_rows_mfma_kernel<2><<<N, 64, lds, st>>>(a); break;
-    case 3: assemble_rows_mfma_kernel<3><<<N, 64, lds, st>>>(a); break;
-    case 4: assemble_rows_mfma_kernel<4><<<N, 64, lds, st>>>(a); break;
-    case 5: assemble_rows_mfma_kernel<5><<<N, 64, lds, st>>>(a); break;
-    case 6: assemble_rows_mfma_kernel<6><<<N, 64, lds, st>>>(a); break;
-    case 7: assemble_rows_mfma_kernel<7><<<N, 64, lds, st>>>(a); break;
-    case 8: assemble_rows_mfma_kernel<8><<<N, 64, lds, st>>>(a); break;
+    NRF_ASM_CASE(1)
+    NRF_ASM_CASE(2)
+    NRF_ASM_CASE(3)
+    NRF_ASM_CASE(4)
+    NRF_ASM_CASE(5)
+    NRF_ASM_CASE(6)
+    NRF_ASM_CASE(7)
+    NRF_ASM_CASE(8)
     default: assemble_rows_kernel<<<N, 64, lds, st>>>(a); break;
   }
+#undef NRF_ASM_CASE
   NRF_LAUNCH_CHECK();
   return NRF_OK;
 }
 
+static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxit, bool record, hipStream_t st);
+
 static int solver_pcg(nrf_solver *s, float *x, int N, hipStream_t st) {
+  int rc = pcg_launch(s, s->b, x, N, s->params.pcg_iterations, true, st);
+  if (rc || s->params.refine_iterations <= 0) return rc;
+  // one iterative-refinement step: r = b - A x accumulated in FP64, A d = r
+  // by refine_iterations PCG iterations, x += d
+  const int n6 = 6 * N, nblk = (n6 + kResThreads - 1) / kResThreads;
+  if ((rc = dev_alloc(s->rres, (size_t)n6, s->c_rres))) return rc;
+  if ((rc = dev_alloc(s->dref, (size_t)n6, s->c_dref))) return rc;
+  residual_fp64_kernel<<<nblk, kResThreads, 0, st>>>(s->row_ptr, s->col_idx, s->values, s->b, x, N, s->rres,
+                                                     nullptr);
+  NRF_LAUNCH_CHECK();
+  if ((rc = pcg_launch(s, s->rres, s->dref, N, s->params.refine_iterations, false, st))) return rc;
+  axpy_kernel<<<nblk, kResThreads, 0, st>>>(x, s->dref, n6);
+  NRF_LAUNCH_CHECK();
+  return NRF_OK;
+}
+
+// `record`: the profiling sink's event that opens the PCG group (one per solve).
+static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxit, bool record, hipStream_t st) {
   pcg_prep_kernel<<<1, 64, 0, st>>>(s->ctl);
   NRF_LAUNCH_CHECK();
-  sink_record(s, st);
+  if (record) sink_record(s, st);
   PcgArgs a;
   a.row_ptr = s->row_ptr;
   a.lcol = s->lcol;
   a.values = s->values;
-  a.b = s->b;
+  a.b = rhs;
   a.Minv = s->Minv;
   a.x = x;
   a.zpub = s->zpub;
@@ -1587,7 +1758,7 @@ static int solver_pcg(nrf_solver *s, float *x, int N, hipStream_t st) {
   a.halo_off = s->halo_off;
   a.halo = s->halo;
   a.N = N;
-  a.maxit = s->params.pcg_iterations;
+  a.maxit = maxit;
   a.G = s->G;
   a.lds_a = s->lds_a;
   a.tol = s->params.pcg_tolerance;
@@ -1632,7 +1803,8 @@ static int solver_ensure_pattern(nrf_solver *s, const nrf_node *d_nodes, int N, 
 // A PCG launch has 1 + 2 * pcg_iterations synchronisation points, which must
 // fit the 16 point bits of a published word's tag.
 static bool params_ok(const nrf_solver_params *p) {
-  return p->pcg_iterations >= 0 && 1 + 2 * (long)p->pcg_iterations <= kPcgMaxPoints;
+  return p->pcg_iterations >= 0 && 1 + 2 * (long)p->pcg_iterations <= kPcgMaxPoints && p->jtj_precision >= 0 &&
+         p->jtj_precision <= 2;
 }
 
 extern "C" nrf_solver *nrf_solver_create(const nrf_solver_params *params, int max_nodes) {
@@ -1682,7 +1854,7 @@ extern "C" void nrf_solver_destroy(nrf_solver *s) {
                   s->rows, s->e_w, s->values, s->b, s->Minv, s->zpub, s->P0, s->P1, s->recs, s->ctl, s->lcol,
                   s->halo_off, s->halo,
                   s->m_cnt, s->m_off, s->m_cur, s->m_keys, s->m_keys2, s->stop, s->d_res, s->d_tot,
-                  s->crec};
+                  s->crec, s->rres, s->dref, s->rpart, s->rnorm};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
@@ -1778,6 +1950,25 @@ extern "C" int nrf_solver_system(const nrf_solver *s, const int32_t **row_ptr, c
 }
 
 extern "C" const nrf_pcg_result *nrf_solver_device_result(const nrf_solver *s) { return s ? s->d_res : nullptr; }
+
+extern "C" int nrf_solver_true_residual(nrf_solver *s, const float *x, double *out, void *stream) {
+  if (!s || !x || !out) return NRF_ERR_INVALID_ARGUMENT;
+  const int N = s->P.N;
+  if (N <= 0) return NRF_ERR_NOT_READY;
+  hipStream_t st = as_stream(stream);
+  const int n6 = 6 * N, nblk = (n6 + kResThreads - 1) / kResThreads;
+  int rc;
+  if ((rc = dev_alloc(s->rpart, (size_t)nblk * 2, s->c_rpart))) return rc;
+  if ((rc = dev_alloc(s->rnorm, 2, s->c_rnorm))) return rc;
+  residual_fp64_kernel<<<nblk, kResThreads, 0, st>>>(s->row_ptr, s->col_idx, s->values, s->b, x, N, nullptr,
+                                                     s->rpart);
+  NRF_LAUNCH_CHECK();
+  residual_norms_kernel<<<1, 64, 0, st>>>(s->rpart, nblk, s->rnorm);
+  NRF_LAUNCH_CHECK();
+  NRF_HIP_CHECK(hipMemcpyAsync(out, s->rnorm, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  NRF_HIP_CHECK(hipStreamSynchronize(st));
+  return NRF_OK;
+}
 
 extern "C" int nrf_solver_get_totals(nrf_solver *s, nrf_solver_totals *out, void *stream) {
   if (!s || !out) return NRF_ERR_INVALID_ARGUMENT;

@@ -236,12 +236,15 @@ class GaussNewtonSolver:
         precond_fp64: bool = False
         arap_consistent_sign: bool = False  # opt-in: ARAP rotation rows with the data term's sign
         pcg_pipelined: bool = False  # opt-in: A p by recurrence, product overlapping an all-reduce
+        jtj_precision: int = 0  # 0 FP32, 1 BF16, 2 FP16 JtJ blocks on the matrix cores (RHS stays FP32)
+        refine_iterations: int = 0  # > 0: one FP64-residual iterative-refinement step per solve
 
         def to_c(self) -> L.SolverParams:
             return L.solver_params(self.gn_iterations, self.pcg_iterations, self.pcg_tolerance, self.lambda_smooth,
                                    self.lambda_rot, self.lambda_damping, int(self.debug),
                                    int(self.relative_tolerance), int(self.precond_fp64),
-                                   int(self.arap_consistent_sign), int(self.pcg_pipelined))
+                                   int(self.arap_consistent_sign), int(self.pcg_pipelined),
+                                   int(self.jtj_precision), int(self.refine_iterations))
 
     def __init__(self, p: "GaussNewtonSolver.Params", max_nodes: int):
         self.lib = L.load()
@@ -258,6 +261,12 @@ class GaussNewtonSolver:
         if h:
             self.lib.nrf_solver_destroy(h)
             self.handle = None
+
+    def true_residual(self, x) -> tuple:
+        """(||b - A x||, ||b||) of the last assembled system, in FP64."""
+        out = (C.c_double * 2)()
+        L.check(self.lib.nrf_solver_true_residual(self.handle, _ptr(x), out, None), "nrf_solver_true_residual")
+        return out[0], out[1]
 
     def set_params(self, p: "GaussNewtonSolver.Params") -> None:
         self.params = p

@@ -62,7 +62,8 @@ def transform_errors(Ta, Tb):
 class Harness:
     def __init__(self, kind: str = "body", frames: int = 40, use_graph: bool = False, seed: int = 0,
                  pcg_tolerance: float = 0.0, relative_tolerance: bool = False, precond_fp64: bool = False,
-                 arap_consistent_sign: bool = False, validate: bool = True, pcg_pipelined: bool = False):
+                 arap_consistent_sign: bool = False, validate: bool = True, pcg_pipelined: bool = False,
+                 jtj_precision: int = 0, refine_iterations: int = 0, storage16: bool = False):
         self.kind = kind
         self.scene = S.make_scene(kind, frames, seed)
         self.w = self.scene.w
@@ -72,10 +73,12 @@ class Harness:
         self.cam = L.camera(w.fx, w.fy, w.cx, w.cy, w.width, w.height)
         self.sp = L.solver_params(w.gn_iterations, w.pcg_iterations, pcg_tolerance, w.lambda_smooth, 1.0,
                                   w.lambda_damping, 0, int(relative_tolerance), int(precond_fp64),
-                                  int(arap_consistent_sign), int(pcg_pipelined))
+                                  int(arap_consistent_sign), int(pcg_pipelined), int(jtj_precision),
+                                  int(refine_iterations))
         self.gn = L.gn_params(w.gn_iterations, w.update_scale, w.max_update_rot, w.max_update_trans,
                               int(validate), w.max_dx_mean)
         self.use_graph = use_graph
+        self.storage16 = storage16
         self.pose = identity_pose()
         self.pipe = None
 
@@ -96,6 +99,7 @@ class Harness:
         cfg.max_triangles = max_triangles or (2_000_000 if w.dims[0] >= 256 else 400_000)
         cfg.min_opt_count = w.min_opt_count
         cfg.use_graph = int(self.use_graph)
+        cfg.storage16 = int(self.storage16)
         self.cfg = cfg
         self.pipe = lib.nrf_pipeline_create(C.byref(cfg))
         if not self.pipe:
@@ -171,8 +175,17 @@ class Harness:
         return self._d2h(self.lib.nrf_pipeline_transforms(self.pipe), self.N, L.DUALQUAT)
 
     def download_voxels(self) -> np.ndarray:
+        """The volume as nrf_voxel records (the compact volume decoded in the
+        precision form)."""
         V = int(np.prod(self.w.dims))
-        return self._d2h(self.lib.nrf_pipeline_voxels(self.pipe), V, L.VOXEL)
+        if not self.storage16:
+            return self._d2h(self.lib.nrf_pipeline_voxels(self.pipe), V, L.VOXEL)
+        t16, w8 = C.c_void_p(), C.c_void_p()
+        L.check(self.lib.nrf_pipeline_volume16(self.pipe, C.byref(t16), C.byref(w8), None, None), "volume16")
+        out = np.zeros(V, L.VOXEL)
+        out["tsdf"] = self._d2h(t16.value, V, np.float16).astype(np.float32)
+        out["weight"] = self._d2h(w8.value, V, np.uint8).astype(np.float32)
+        return out
 
     def download_knn(self):
         V = int(np.prod(self.w.dims))

@@ -116,6 +116,12 @@ typedef struct nrf_solver_params {
                                    d(omega x q)/d(omega), as the data term and the
                                    exp map use; 0 keeps the reference's negated rows
                                    (kernels/solver_kernels.cu:145-153) */
+  int32_t jtj_precision;       /* data-term JtJ blocks on the matrix cores in FP32 (0),
+                                  BF16 (1) or FP16 (2), FP32 accumulation; the RHS
+                                  stays FP32 (precision form) */
+  int32_t refine_iterations;   /* > 0: after each solve, one iterative-refinement
+                                  step: r = b - A x accumulated in FP64, A d = r by
+                                  this many PCG iterations, x += d */
 } nrf_solver_params;
 
 /* Caller-side GN loop settings (include/fusion_app.h:72-119 names). */
@@ -249,7 +255,7 @@ typedef struct nrf_solver nrf_solver;
 
 /* GaussNewtonSolver(const Params&, int max_nodes) (kernels/solver_kernels.cu:486-496).
    Returns NULL (and set_params NRF_ERR_INVALID_ARGUMENT) for pcg_iterations
-   outside [0, 32767]. */
+   outside [0, 32767] or jtj_precision outside [0, 2]. */
 nrf_solver *nrf_solver_create(const nrf_solver_params *params, int max_nodes);
 void nrf_solver_destroy(nrf_solver *s);
 int nrf_solver_set_params(nrf_solver *s, const nrf_solver_params *params);
@@ -292,6 +298,12 @@ int nrf_solver_system(const nrf_solver *s, const int32_t **row_ptr,
    stream-ordered readers that do not want a host sync). */
 const nrf_pcg_result *nrf_solver_device_result(const nrf_solver *s);
 
+/* True residual of the last assembled system for a device vector x:
+   out[0] = ||b - A x||_2, out[1] = ||b||_2, products and sums in FP64
+   (no reference counterpart: the reference discards the residual,
+   kernels/solver_kernels.cu:569).  Synchronises the stream. */
+int nrf_solver_true_residual(nrf_solver *s, const float *x, double out[2], void *stream);
+
 /* Running totals over every solve since creation (no reference counterpart:
    the reference prints per-iteration diagnostics instead).  Accumulated on
    the device by the next solve's first kernel, so reading them costs no
@@ -320,6 +332,10 @@ typedef struct nrf_pipeline_config {
   int32_t max_triangles;
   int32_t min_opt_count;   /* integrate_min_optimized_count */
   int32_t use_graph;       /* 1: replay the step as a captured hipGraph */
+  int32_t storage16;       /* 1: precision form of the per-frame step: FP16 TSDF + u8
+                              weight and u16 / FP16 voxel kNN (nrf_tsdf16_*, nrf_mc16_*);
+                              frame 0 and the kNN snapshot use the FP32 volume, which
+                              is converted once by nrf_pipeline_set_nodes */
 } nrf_pipeline_config;
 
 nrf_pipeline *nrf_pipeline_create(const nrf_pipeline_config *cfg);
@@ -332,6 +348,9 @@ int32_t *nrf_pipeline_knn(nrf_pipeline *p);
 float *nrf_pipeline_knn_w(nrf_pipeline *p);
 float *nrf_pipeline_mesh(nrf_pipeline *p, int32_t **counts);
 nrf_solver *nrf_pipeline_solver(nrf_pipeline *p);
+/* Compact volume and kNN of a storage16 pipeline (NULLs otherwise). */
+int nrf_pipeline_volume16(nrf_pipeline *p, uint16_t **tsdf16, uint8_t **weight8, uint16_t **knn16,
+                          uint16_t **knn_w16);
 /* Unwarped integration of a frame (frame 0 of fusion_app.h:233). */
 int nrf_pipeline_integrate_rigid(nrf_pipeline *p, const float *depth,
                                  const float camera_pose[16], void *stream);

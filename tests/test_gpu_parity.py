@@ -410,3 +410,93 @@ def test_warp_normals(sphere, orc):
     orc.load().orc_warp_normals_f32(nrm.ctypes.data, ids.ctypes.data, ws.ctypes.data, 400, Tc.ctypes.data, N,
                                     ref.ctypes.data)
     np.testing.assert_allclose(out.cpu().numpy(), ref, atol=2e-6)
+
+
+def _sphere_system(sphere, orc, solver_inputs, **params):
+    """Assembles the sphere system of frame 2 at perturbed transforms with the
+    given solver options; returns (solver, dcorr, corrs, T, T64, dx, res)."""
+    import torch
+
+    from cudafusion_amd.fusion import GaussNewtonSolver
+    w = sphere["w"]
+    nodes = sphere["nodes"]
+    N = len(nodes)
+    wf = solver_inputs["wf"]
+    T, T64 = _perturbed_T(orc, nodes, 11, rot=0.01, trans=0.003)
+    wf.upload_transforms(T)
+    corrs = solver_inputs["hcorr"].copy()
+    src = np.zeros_like(solver_inputs["hcan"])
+    ids = np.ascontiguousarray(corrs["node_ids"])
+    ws = np.ascontiguousarray(corrs["node_ws"])
+    can = np.ascontiguousarray(solver_inputs["hcan"])
+    Tc = np.ascontiguousarray(T)
+    orc.load().orc_warp_points_f32(can.ctypes.data, ids.ctypes.data, ws.ctypes.data, len(corrs), nodes.ctypes.data,
+                                   Tc.ctypes.data, N, src.ctypes.data)
+    corrs["src"] = src.reshape(-1, 3)
+    dcorr = torch.from_numpy(corrs.view(np.uint8).copy()).cuda()
+    p = GaussNewtonSolver.Params(pcg_iterations=20, pcg_tolerance=0.0, lambda_smooth=w.lambda_smooth, **params)
+    s = GaussNewtonSolver(p, 8192)
+    dx = torch.zeros(6 * N, dtype=torch.float32, device="cuda")
+    res = s.solve(dcorr, len(corrs), wf.device_nodes(), wf.device_transforms(), N, dx)
+    return s, dcorr, corrs, T, T64, dx, res
+
+
+@pytest.mark.parametrize("prec", [1, 2], ids=["bf16", "fp16"])
+def test_reduced_precision_jtj(sphere, orc, solver_inputs, prec):
+    """Precision form: the data term's JtJ blocks from BF16 / FP16 matrix-core
+    products (FP32 accumulation) stay within the type's rounding of the FP64
+    oracle; the RHS (FP32 column) keeps the FP32 bar."""
+    from cudafusion_amd.fusion import copy_device_to_numpy
+    w = sphere["w"]
+    nodes = sphere["nodes"]
+    N = len(nodes)
+    s, dcorr, corrs, T, T64, dx, res = _sphere_system(sphere, orc, solver_inputs, jtj_precision=prec)
+    assert not (res.flags & 8)
+    rp_p, ci_p, va_p, nb, rhs_p, pc_p = s.system_matrix()
+    rp, ci, A, b, M = orc.assemble_f32(corrs, nodes, T, w.lambda_smooth, w.lambda_damping)
+    _, _, A64, b64, _ = orc.assemble_f64(corrs, nodes, T64, w.lambda_smooth, w.lambda_damping)
+    gA = copy_device_to_numpy(va_p, nb * 36, np.float32).reshape(nb, 6, 6)
+    gb = copy_device_to_numpy(rhs_p, 6 * N, np.float32)
+    scale = np.abs(A64).max()
+    err = np.abs(gA - A64).max() / scale
+    tol = {1: 1e-2, 2: 2e-3}[prec]
+    print(f"[jtj prec {prec}] blocks max err {err:.2e} of the largest entry (bar {tol:.0e})", flush=True)
+    assert 0 < err < tol
+    np.testing.assert_allclose(gb, b, atol=2e-6 * np.abs(b).max())
+
+
+def test_true_residual_and_refinement(sphere, orc, solver_inputs):
+    """nrf_solver_true_residual equals ||b - A x|| computed in FP64 from the
+    assembled FP32 system; one refinement step (FP64 residual, 10 PCG
+    iterations on the correction) lowers the true residual of 10 iterations
+    to that of about 20."""
+    from cudafusion_amd.fusion import copy_device_to_numpy
+    N = len(sphere["nodes"])
+    s, dcorr, corrs, T, T64, dx, res = _sphere_system(sphere, orc, solver_inputs)
+    rp_p, ci_p, va_p, nb, rhs_p, pc_p = s.system_matrix()
+    rp = copy_device_to_numpy(rp_p, N + 1, np.int32)
+    ci = copy_device_to_numpy(ci_p, nb, np.int32)
+    A = copy_device_to_numpy(va_p, nb * 36, np.float32).reshape(nb, 6, 6).astype(np.float64)
+    b = copy_device_to_numpy(rhs_p, 6 * N, np.float32).astype(np.float64)
+    x = dx.cpu().numpy().astype(np.float64)
+    Ax = np.zeros(6 * N)
+    for i in range(N):
+        for q in range(rp[i], rp[i + 1]):
+            Ax[6 * i:6 * i + 6] += A[q] @ x[6 * ci[q]:6 * ci[q] + 6]
+    r_np, b_np = np.linalg.norm(b - Ax), np.linalg.norm(b)
+    r_gpu, b_gpu = s.true_residual(dx)
+    assert abs(r_gpu - r_np) <= 1e-9 * b_np and abs(b_gpu - b_np) <= 1e-12 * b_np
+    rel20 = r_gpu / b_gpu
+    p = s.params
+    p.pcg_iterations = 10
+    s.set_params(p)
+    wf = solver_inputs["wf"]
+    s.solve(dcorr, len(corrs), wf.device_nodes(), wf.device_transforms(), N, dx)
+    rel10 = s.true_residual(dx)[0] / b_gpu
+    p.refine_iterations = 10
+    s.set_params(p)
+    s.solve(dcorr, len(corrs), wf.device_nodes(), wf.device_transforms(), N, dx)
+    rel_ref = s.true_residual(dx)[0] / b_gpu
+    print(f"[refine] true relative residual: 10 it {rel10:.2e}, 10 + refine 10 {rel_ref:.2e}, 20 it {rel20:.2e}",
+          flush=True)
+    assert rel_ref < 0.5 * rel10 and rel_ref < 3 * rel20

@@ -129,3 +129,41 @@ def test_compact_integration_and_mc(gpu, orc):
     np.testing.assert_array_equal(work[:V].cpu().numpy(), rc)
     assert T_ * 3 == len(rv) and T_ > 1000
     np.testing.assert_allclose(verts[: T_ * 9].cpu().numpy().reshape(-1, 3), rv, atol=1e-6)
+
+
+@pytest.mark.parametrize("prec", [1, 2], ids=["bf16", "fp16"])
+def test_pipeline_precision_form_tracks_fp32(gpu, orc, prec):
+    """The per-frame step in the precision form (FP16 TSDF + u8 weight,
+    compact kNN, BF16 / FP16 JtJ blocks, FP64 Cholesky inverses, one
+    FP64-residual refinement step per solve) against the FP32 step on the
+    same sphere frames.  The RHS stays FP32, so the GN fixed point is the
+    FP32 one; the transforms differ by what an unconverged 20-iteration solve
+    of a Hessian rounded to the type changes (bars: 8 resp. 11 significant
+    bits), the TSDF by the FP16 rounding, the mesh by a few triangles."""
+    import torch
+
+    from cudafusion_amd.harness import Harness, transform_errors
+    runs = {}
+    # same solver settings (FP64 Cholesky, one refinement step) in both runs,
+    # so the difference is the storage and the JtJ precision
+    for name, kw in (("fp32", {}), ("precision", dict(storage16=True, jtj_precision=prec))):
+        h = Harness("sphere", frames=3, arap_consistent_sign=True, validate=False, pcg_pipelined=True,
+                    refine_iterations=5, precond_fp64=True, **kw)
+        h.setup()
+        for f in (1, 2):
+            h.step(f)
+        torch.cuda.synchronize()
+        runs[name] = (h.download_transforms().copy(), h.download_voxels().copy(), h.triangle_count(),
+                      h.solver_result())
+        h.close()
+    (T32, v32, n32, r32), (T16, v16, n16, r16) = runs["fp32"], runs["precision"]
+    er, et = transform_errors(T16, T32.view(np.float32).reshape(-1, 8).astype(np.float64))
+    both = (v32["weight"] > 0) & (v16["weight"] > 0)
+    dt = np.abs(v16["tsdf"][both] - v32["tsdf"][both])
+    print(f"[precision form] vs FP32 step: rot {er:.2e} rad, trans {et:.2e} m, TSDF p99.9 {np.percentile(dt, 99.9):.2e} "
+          f"max {dt.max():.2e}, triangles {n16} vs {n32}, gn_done {r16.gn_done}", flush=True)
+    assert r16.gn_done == r32.gn_done == 3 and not (r16.flags & 8)
+    assert er < {1: 2e-2, 2: 5e-3}[prec] and et < {1: 5e-4, 2: 2e-4}[prec]
+    np.testing.assert_array_equal(v16["weight"] > 0, v32["weight"] > 0)
+    assert np.percentile(dt, 99.9) < 2e-3
+    assert abs(n16 - n32) <= 0.02 * n32
