@@ -246,6 +246,7 @@ def main() -> int:
                  "bsr_blocks": nb},
         "cpu_baseline": cpu,
     }
+    out["pcg_to_1e-6"] = pcg_to_tolerance(h, 1e-6)
     if prec:
         out["config"]["workload"] += ", precision form (%s JtJ, FP16 TSDF + u8 weight, FP64 Cholesky and " \
                                      "refinement)" % args.precision
@@ -254,6 +255,41 @@ def main() -> int:
     print(json.dumps(out))
     h.close()
     return 0
+
+
+def pcg_to_tolerance(h, rel_tol, reps=20):
+    """Second half of the metric: the PCG alone on the last frame's last
+    assembled system, run to ||r|| < rel_tol ||b|| (the recurrence's residual,
+    at most 5000 iterations) from x0 = 0; ms per solve from HIP events over
+    `reps` solves, and the true relative residual ||b - A x|| / ||b|| of the
+    result recomputed in FP64.  Preconditioner: block-Jacobi (the learned
+    preconditioner is not built)."""
+    import ctypes as C
+
+    import torch
+
+    from cudafusion_amd import _lib as L
+    lib = h.lib
+    solver = lib.nrf_pipeline_solver(h.pipe)
+    sp = L.solver_params(**{f: getattr(h.sp, f) for f, _ in L.SolverParams._fields_})
+    sp.pcg_iterations, sp.pcg_tolerance, sp.relative_tolerance, sp.refine_iterations = 5000, rel_tol, 1, 0
+    L.check(lib.nrf_solver_set_params(solver, C.byref(sp)), "set_params")
+    x = torch.zeros(6 * h.N, dtype=torch.float32, device="cuda")
+    res = L.PcgResult()
+    stream = torch.cuda.current_stream()
+    L.check(lib.nrf_solver_resolve(solver, x.data_ptr(), C.byref(res), stream.cuda_stream), "resolve")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        L.check(lib.nrf_solver_resolve(solver, x.data_ptr(), None, stream.cuda_stream), "resolve")
+    e1.record(stream)
+    torch.cuda.synchronize()
+    out = (C.c_double * 2)()
+    L.check(lib.nrf_solver_true_residual(solver, x.data_ptr(), out, stream.cuda_stream), "true_residual")
+    L.check(lib.nrf_solver_set_params(solver, C.byref(h.sp)), "set_params")
+    return {"preconditioner": "block-Jacobi", "ms": round(e0.elapsed_time(e1) / reps, 4),
+            "iterations": int(res.iterations), "true_rel_residual": float(out[0] / out[1]),
+            "learned": None}
 
 
 def precision_errors(h, f):

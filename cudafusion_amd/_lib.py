@@ -102,6 +102,7 @@ _SIGS = {
     "nrf_solver_device_result": (P, [P]),
     "nrf_solver_get_totals": (I, [P, C.POINTER(SolverTotals), P]),
     "nrf_solver_true_residual": (I, [P, P, C.POINTER(C.c_double), P]),
+    "nrf_solver_resolve": (I, [P, P, C.POINTER(PcgResult), P]),
     "nrf_pipeline_create": (P, [C.POINTER(PipelineConfig)]),
     "nrf_pipeline_destroy": (None, [P]),
     "nrf_pipeline_voxels": (P, [P]),

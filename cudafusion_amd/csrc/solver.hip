@@ -1951,6 +1951,18 @@ extern "C" int nrf_solver_system(const nrf_solver *s, const int32_t **row_ptr, c
 
 extern "C" const nrf_pcg_result *nrf_solver_device_result(const nrf_solver *s) { return s ? s->d_res : nullptr; }
 
+extern "C" int nrf_solver_resolve(nrf_solver *s, float *delta_x, nrf_pcg_result *result, void *stream) {
+  if (!s || !delta_x) return NRF_ERR_INVALID_ARGUMENT;
+  const int N = s->P.N;
+  if (N <= 0) return NRF_ERR_NOT_READY;
+  hipStream_t st = as_stream(stream);
+  gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res, s->d_tot);
+  NRF_LAUNCH_CHECK();
+  int rc = solver_pcg(s, delta_x, N, st);
+  if (rc) return rc;
+  return solver_fetch_result(s, result, st);
+}
+
 extern "C" int nrf_solver_true_residual(nrf_solver *s, const float *x, double *out, void *stream) {
   if (!s || !x || !out) return NRF_ERR_INVALID_ARGUMENT;
   const int N = s->P.N;

@@ -298,6 +298,12 @@ int nrf_solver_system(const nrf_solver *s, const int32_t **row_ptr,
    stream-ordered readers that do not want a host sync). */
 const nrf_pcg_result *nrf_solver_device_result(const nrf_solver *s);
 
+/* PCG alone on the last assembled system (A, b, block-Jacobi inverse), with
+   the solver's current parameters: for timing the solve and its stopping
+   rules in isolation (no reference counterpart).  Stream-ordered; fills
+   result (and synchronises) when result != NULL. */
+int nrf_solver_resolve(nrf_solver *s, float *delta_x, nrf_pcg_result *result, void *stream);
+
 /* True residual of the last assembled system for a device vector x:
    out[0] = ||b - A x||_2, out[1] = ||b||_2, products and sums in FP64
    (no reference counterpart: the reference discards the residual,
