@@ -52,12 +52,27 @@ class GNParams(C.Structure):
 
 class PcgResult(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("total_iterations", C.c_int32), ("flags", C.c_int32),
-                ("gn_done", C.c_int32), ("final_residual", C.c_double), ("rhs_norm", C.c_double)]
+                ("gn_done", C.c_int32), ("final_residual", C.c_double), ("rhs_norm", C.c_double),
+                ("converged", C.c_int32), ("breakdowns", C.c_int32), ("runs", C.c_int32), ("pad", C.c_int32)]
 
 
 class SolverTotals(C.Structure):
     _fields_ = [("solves", C.c_int64), ("gn_done", C.c_int64), ("pcg_iterations", C.c_int64),
-                ("rejected", C.c_int64)]
+                ("rejected", C.c_int64), ("pcg_runs", C.c_int64), ("pcg_converged", C.c_int64),
+                ("pcg_breakdowns", C.c_int64)]
+
+
+MAX_BATCH = 32
+
+
+class BatchFrame(C.Structure):
+    _fields_ = [("depth", C.c_void_p), ("camera_pose", C.c_float * 16), ("corrs", C.c_void_p),
+                ("canonical", C.c_void_p), ("num_corrs", C.c_int32), ("pad", C.c_int32)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("lanes_per_row", C.c_int32), ("wg_nodes", C.c_int32), ("z_chunk", C.c_int32),
+                ("launch_groups", C.c_int32), ("pcg_ms", C.c_float), ("integrate_ms", C.c_float)]
 
 
 class PipelineConfig(C.Structure):
@@ -103,6 +118,7 @@ _SIGS = {
     "nrf_solver_get_totals": (I, [P, C.POINTER(SolverTotals), P]),
     "nrf_solver_true_residual": (I, [P, P, C.POINTER(C.c_double), P]),
     "nrf_solver_resolve": (I, [P, P, C.POINTER(PcgResult), P]),
+    "nrf_solver_set_pcg_layout": (I, [P, I, I]),
     "nrf_pipeline_create": (P, [C.POINTER(PipelineConfig)]),
     "nrf_pipeline_destroy": (None, [P]),
     "nrf_pipeline_voxels": (P, [P]),
@@ -120,6 +136,13 @@ _SIGS = {
     "nrf_pipeline_profile": (I, [P, I]),
     "nrf_pipeline_profile_read": (I, [P, I, C.POINTER(F)]),
     "nrf_pipeline_updated_voxels": (C.c_longlong, [P]),
+    "nrf_batch_create": (P, [C.POINTER(P), I, C.POINTER(BatchTuning), I]),
+    "nrf_batch_destroy": (None, [P]),
+    "nrf_batch_set_tuning": (I, [P, C.POINTER(BatchTuning)]),
+    "nrf_batch_get_tuning": (I, [P, C.POINTER(BatchTuning)]),
+    "nrf_batch_autotune": (I, [P, C.POINTER(BatchFrame), I, C.POINTER(BatchTuning), P]),
+    "nrf_batch_step": (I, [P, C.POINTER(BatchFrame), P]),
+    "nrf_batch_stage_ms": (I, [P, C.POINTER(F)]),
 }
 
 _lib = None

@@ -12,6 +12,8 @@
 // and every lane has ZC independent voxels in flight.  The per-voxel DQ
 // blend reads a per-node table of centred transforms (N x 32 B, resident in
 // L2) instead of recomputing dq_centered four times per voxel.
+#include <vector>
+
 #include "nrf_internal.h"
 #include "nrf_math.h"
 
@@ -49,17 +51,17 @@ __device__ inline DQ fetch_centered(const IntegrateArgs &a, int id, bool table) 
   return dq_centered(a.transforms[id], f3(n.pos));
 }
 
-template <bool WARP, bool TABLE>
-__global__ __launch_bounds__(256) void integrate_kernel(IntegrateArgs a) {
-  const int x = blockIdx.x * kTileX + (threadIdx.x & (kTileX - 1));
-  const int y = blockIdx.y * kTileY + (threadIdx.x / kTileX);
-  const int z0 = blockIdx.z * kZChunk;
+template <bool WARP, bool TABLE, int ZC>
+__device__ void integrate_tile(const IntegrateArgs &a, int bx, int by, int bz) {
+  const int x = bx * kTileX + (threadIdx.x & (kTileX - 1));
+  const int y = by * kTileY + (threadIdx.x / kTileX);
+  const int z0 = bz * ZC;
   int n_upd = 0;
   if (x < a.X && y < a.Y) {
     const float px_w = a.ox + (x + 0.5f) * a.s;
     const float py_w = a.oy + (y + 0.5f) * a.s;
 #pragma unroll
-    for (int dz = 0; dz < kZChunk; ++dz) {
+    for (int dz = 0; dz < ZC; ++dz) {
       const int z = z0 + dz;
       if (z >= a.Z) break;
       const int idx = z * a.X * a.Y + y * a.X + x;
@@ -121,6 +123,37 @@ __global__ __launch_bounds__(256) void integrate_kernel(IntegrateArgs a) {
     if ((threadIdx.x & 63) == 0 && n_upd) atomicAdd(a.updated, n_upd);
   }
 }
+template <bool WARP, bool TABLE>
+__global__ __launch_bounds__(256) void integrate_kernel(IntegrateArgs a) {
+  integrate_tile<WARP, TABLE, kZChunk>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// All instances of a batch in one launch.  Instance k owns the tiles
+// [tile_base[k], tile_base[k + 1]) of the 1-D grid, numbered x fastest, then
+// y, then z as in the single-volume grid; depth and camera transform are the
+// frame's (kernel arguments), everything else the instance's record.
+struct IntBatchDev {
+  IntegrateArgs a;
+  int tiles_x, tiles_y;
+};
+template <int ZC>
+__global__ __launch_bounds__(256) void integrate_batch_kernel(const IntBatchDev *__restrict__ inst,
+                                                              const int32_t *__restrict__ tile_base, int count,
+                                                              BatchDepth fr) {
+  const int t = blockIdx.x;
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tile_base[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  const IntBatchDev &d = inst[lo];
+  IntegrateArgs a = d.a;
+  a.depth = fr.depth[lo];
+  a.T = fr.xf[lo];
+  const int l = t - tile_base[lo];
+  const int bx = l % d.tiles_x, r = l / d.tiles_x;
+  integrate_tile<true, true, ZC>(a, bx, r % d.tiles_y, r / d.tiles_y);
+}
 
 __global__ __launch_bounds__(256) void reset_kernel(nrf_voxel *v, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -138,6 +171,92 @@ __global__ __launch_bounds__(256) void reset_kernel(nrf_voxel *v, int64_t n) {
 }  // namespace nrf
 
 using namespace nrf;
+
+struct nrf::IntegrateBatch {
+  int count = 0, z_chunk = 4, tiles = 0;
+  IntBatchDev *d_inst = nullptr;
+  int32_t *tile_base = nullptr;
+};
+
+namespace nrf {
+
+void integrate_batch_destroy(IntegrateBatch *b) {
+  if (!b) return;
+  if (b->d_inst) (void)hipFree(b->d_inst);
+  if (b->tile_base) (void)hipFree(b->tile_base);
+  delete b;
+}
+
+int integrate_batch_create(const IntegrateBatchInstance *inst, int count, int z_chunk, IntegrateBatch **out) {
+  if (!inst || !out || count < 1 || count > kMaxBatch) return NRF_ERR_INVALID_ARGUMENT;
+  if (z_chunk != 1 && z_chunk != 2 && z_chunk != 4 && z_chunk != 8) return NRF_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  std::vector<IntBatchDev> h(count);
+  std::vector<int32_t> base(count + 1, 0);
+  long tiles = 0;
+  for (int k = 0; k < count; ++k) {
+    const IntegrateBatchInstance &q = inst[k];
+    if (!volume_ok(&q.vp) || !q.vox || q.cam.width <= 0 || q.cam.height <= 0 || q.N <= 0 || !q.nodes || !q.T ||
+        !q.centered || !q.knn || !q.knn_w)
+      return NRF_ERR_INVALID_ARGUMENT;
+    IntegrateArgs &a = h[k].a;
+    a.vox = q.vox;
+    a.X = q.vp.dims[0]; a.Y = q.vp.dims[1]; a.Z = q.vp.dims[2];
+    a.ox = q.vp.origin[0]; a.oy = q.vp.origin[1]; a.oz = q.vp.origin[2];
+    a.s = q.vp.voxel_size;
+    a.trunc = q.vp.truncation;
+    a.depth = nullptr;
+    a.W = q.cam.width; a.H = q.cam.height;
+    a.fx = q.cam.fx; a.fy = q.cam.fy; a.cx = q.cam.cx; a.cy = q.cam.cy;
+    a.nodes = q.nodes;
+    a.transforms = reinterpret_cast<const DQ *>(q.T);
+    a.centered = reinterpret_cast<const DQ *>(q.centered);
+    a.N = q.N;
+    a.knn = reinterpret_cast<const int4 *>(q.knn);
+    a.knn_w = reinterpret_cast<const float4 *>(q.knn_w);
+    a.opt = nullptr;
+    a.min_opt = 0;
+    a.decay = q.vp.decay_alpha;
+    a.wmax = q.vp.max_weight;
+    a.updated = q.updated;
+    h[k].tiles_x = (a.X + kTileX - 1) / kTileX;
+    h[k].tiles_y = (a.Y + kTileY - 1) / kTileY;
+    base[k] = (int32_t)tiles;
+    tiles += (long)h[k].tiles_x * h[k].tiles_y * ((a.Z + z_chunk - 1) / z_chunk);
+    if (tiles > INT32_MAX) return NRF_ERR_TOO_LARGE;
+  }
+  base[count] = (int32_t)tiles;
+  IntegrateBatch *b = new IntegrateBatch();
+  b->count = count;
+  b->z_chunk = z_chunk;
+  b->tiles = (int)tiles;
+  bool ok = hipMalloc(&b->d_inst, sizeof(IntBatchDev) * count) == hipSuccess &&
+            hipMalloc(&b->tile_base, sizeof(int32_t) * (count + 1)) == hipSuccess &&
+            hipMemcpy(b->d_inst, h.data(), sizeof(IntBatchDev) * count, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(b->tile_base, base.data(), sizeof(int32_t) * (count + 1), hipMemcpyHostToDevice) ==
+                hipSuccess;
+  if (!ok) {
+    set_last_error("integrate_batch_create", hipGetLastError());
+    integrate_batch_destroy(b);
+    return NRF_ERR_HIP;
+  }
+  *out = b;
+  return NRF_OK;
+}
+
+int integrate_batch_run(IntegrateBatch *b, const BatchDepth &fr, hipStream_t st) {
+  if (!b) return NRF_ERR_INVALID_ARGUMENT;
+  switch (b->z_chunk) {
+    case 1: integrate_batch_kernel<1><<<b->tiles, 256, 0, st>>>(b->d_inst, b->tile_base, b->count, fr); break;
+    case 2: integrate_batch_kernel<2><<<b->tiles, 256, 0, st>>>(b->d_inst, b->tile_base, b->count, fr); break;
+    case 8: integrate_batch_kernel<8><<<b->tiles, 256, 0, st>>>(b->d_inst, b->tile_base, b->count, fr); break;
+    default: integrate_batch_kernel<4><<<b->tiles, 256, 0, st>>>(b->d_inst, b->tile_base, b->count, fr); break;
+  }
+  NRF_LAUNCH_CHECK();
+  return NRF_OK;
+}
+
+}  // namespace nrf
 
 extern "C" int nrf_tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels,
                                   const float *depth, const nrf_camera *cam,

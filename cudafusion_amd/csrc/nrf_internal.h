@@ -47,6 +47,74 @@ void solver_set_event_sink(nrf_solver *s, hipEvent_t *events, int count);
 int device_max_lds_per_block();
 bool device_cooperative();
 
+// ---------------------------------------------------------------- batches
+// Independent instances (one pipeline each) stepped by common launches.
+// Per-frame inputs travel as kernel arguments, so a captured batch step
+// depends on nothing but them and the instances' device state.
+constexpr int kMaxBatch = 32;
+struct BatchCorrs {
+  const nrf_corr *corrs[kMaxBatch];
+  const float *canonical[kMaxBatch];
+  int32_t n[kMaxBatch];
+  int32_t max_n;
+};
+struct BatchDepth {
+  const float *depth[kMaxBatch];
+  CamXf xf[kMaxBatch];
+};
+
+struct SolverBatch;
+struct SolverBatchInstance {
+  nrf_solver *solver;
+  const nrf_node *nodes;  // device
+  nrf_dualquat *T, *centered;
+  int N;
+  nrf_gn_params gn;
+  float *delta_x;
+  int max_corrs;
+};
+// Sets every solver's PCG layout (lanes per scalar row, target nodes per
+// workgroup), groups the instances into persistent launches whose
+// workgroups fit on the device together, and uploads the device records.
+int solver_batch_create(const SolverBatchInstance *inst, int count, int lanes_per_row, int wg_nodes,
+                        SolverBatch **out);
+void solver_batch_destroy(SolverBatch *b);
+// The GN loop of every instance (memberships, gn_iterations x (records,
+// assembly, PCG, validation, update)), all stream-ordered.
+int solver_batch_solve_gn(SolverBatch *b, const BatchCorrs &fr, hipStream_t st);
+// Pieces used by the tuner: memberships + assembly at the current transforms
+// (results and totals are saved first), PCG alone, and the restore.
+int solver_batch_prepare(SolverBatch *b, const BatchCorrs &fr, hipStream_t st);
+int solver_batch_pcg(SolverBatch *b, hipStream_t st);
+int solver_batch_restore(SolverBatch *b, hipStream_t st);
+int solver_batch_launch_groups(const SolverBatch *b);
+int solver_batch_workgroups(const SolverBatch *b);
+int solver_rmax(const nrf_solver *s);
+// Sizes the per-frame buffers of the solver (correspondence records,
+// memberships, refinement vectors) for frames of up to max_corrs records at
+// the current node count, so that a step (possibly under capture) allocates
+// nothing.
+int solver_reserve(nrf_solver *s, int max_corrs);
+// Changes whenever a solver's partition is rebuilt or any solver buffer is
+// reallocated: captured steps compare it before a replay.
+unsigned solver_state_generation(const nrf_solver *s);
+
+struct IntegrateBatch;
+struct IntegrateBatchInstance {
+  nrf_volume_params vp;
+  nrf_camera cam;
+  nrf_voxel *vox;
+  const nrf_node *nodes;
+  const nrf_dualquat *T, *centered;
+  int N;
+  const int32_t *knn;
+  const float *knn_w;
+  int32_t *updated;
+};
+int integrate_batch_create(const IntegrateBatchInstance *inst, int count, int z_chunk, IntegrateBatch **out);
+void integrate_batch_destroy(IntegrateBatch *b);
+int integrate_batch_run(IntegrateBatch *b, const BatchDepth &fr, hipStream_t st);
+
 }  // namespace nrf
 
 #define NRF_HIP_CHECK(call)                        \

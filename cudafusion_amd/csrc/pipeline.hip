@@ -6,7 +6,11 @@
 // integration -> marching cubes (the extract_surface_device form).  Nothing
 // synchronises the host inside a step, so a step can be captured into a
 // hipGraph and replayed.
+#include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "nrf_internal.h"
@@ -45,6 +49,9 @@ struct nrf_pipeline {
   int g_n = -1;
   float g_pose[16];
   void *g_stream = nullptr;
+  unsigned g_gen = 0;
+  hipStream_t cap = nullptr;  // capture stream (the caller's may be the null stream)
+  bool graph_step = false;    // the last step was a graph replay (stage events: whole step only)
 };
 
 static void pipeline_free(nrf_pipeline *p) {
@@ -58,6 +65,7 @@ static void pipeline_free(nrf_pipeline *p) {
     if (e) (void)hipEventDestroy(e);
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->cap) (void)hipStreamDestroy(p->cap);
   if (p->solver) nrf_solver_destroy(p->solver);
   delete p;
 }
@@ -91,6 +99,7 @@ extern "C" nrf_pipeline *nrf_pipeline_create(const nrf_pipeline_config *cfg) {
     if (ok) ok = nrf_tsdf16_reset(&cfg->volume, p->t16, p->w8, nullptr) == NRF_OK;
   }
   for (auto &e : p->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+  ok = ok && hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking) == hipSuccess;
   if (ok) {
     p->solver = nrf_solver_create(&cfg->solver, cfg->max_nodes);
     ok = p->solver != nullptr;
@@ -157,6 +166,8 @@ extern "C" int nrf_pipeline_set_nodes(nrf_pipeline *p, const nrf_node *host_node
   if (rc) return rc;
   rc = nrf_solver_set_pattern(p->solver, host_nodes, num_nodes);
   if (rc) return rc;
+  rc = solver_reserve(p->solver, p->cfg.max_corrs);
+  if (rc) return rc;
   rc = nrf_voxel_knn(&p->cfg.volume, p->vox, p->nodes, num_nodes, p->knn, p->knn_w, nullptr);
   if (rc) return rc;
   if (p->cfg.storage16) {  // the FP32 state so far, rounded once into the compact form
@@ -173,9 +184,10 @@ extern "C" int nrf_pipeline_set_nodes(nrf_pipeline *p, const nrf_node *host_node
   return NRF_OK;
 }
 
+// `events`: record the stage events (not inside a capture).
 static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float camera_pose[16],
                             const nrf_corr *corrs, const float *canonical, int num_corrs, void *stream,
-                            bool profile) {
+                            bool profile, bool events = true) {
   hipStream_t st = as_stream(stream);
   int rc;
   hipEvent_t *pe = nullptr;
@@ -184,12 +196,12 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
     NRF_HIP_CHECK(hipEventRecord(pe[0], st));
     solver_set_event_sink(p->solver, pe + 1, p->prof_per - 3);
   }
-  NRF_HIP_CHECK(hipEventRecord(p->ev[0], st));
+  if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[0], st));
   rc = nrf_solver_solve_gn(p->solver, corrs, canonical, num_corrs, p->nodes, p->T, p->centered, p->N, &p->cfg.gn,
                            p->delta_x, nullptr, stream);
   if (profile) solver_set_event_sink(p->solver, nullptr, 0);
   if (rc) return rc;
-  NRF_HIP_CHECK(hipEventRecord(p->ev[1], st));
+  if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[1], st));
   if (p->cfg.storage16)
     rc = nrf_tsdf16_integrate(&p->cfg.volume, p->t16, p->w8, depth, &p->cfg.camera, camera_pose, p->centered, p->N,
                               p->k16, p->kw16, p->updated, stream);
@@ -197,7 +209,7 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
     rc = nrf_tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, p->nodes, p->T, p->N,
                             p->centered, p->knn, p->knn_w, nullptr, p->cfg.min_opt_count, p->updated, stream);
   if (rc) return rc;
-  NRF_HIP_CHECK(hipEventRecord(p->ev[2], st));
+  if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[2], st));
   if (profile) NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 2], st));
   if (p->cfg.storage16)
     rc = nrf_mc16_extract(&p->cfg.volume, p->t16, p->w8, p->mc_ws, p->mc_bytes, p->verts, p->tris,
@@ -206,7 +218,7 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
     rc = nrf_mc_extract(&p->cfg.volume, p->vox, p->mc_ws, p->mc_bytes, p->verts, nullptr, p->tris,
                         p->cfg.max_triangles, p->counts, stream);
   if (rc) return rc;
-  NRF_HIP_CHECK(hipEventRecord(p->ev[3], st));
+  if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[3], st));
   if (profile) {
     NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 1], st));
     ++p->prof_steps;
@@ -219,10 +231,13 @@ extern "C" int nrf_pipeline_step(nrf_pipeline *p, const float *depth, const floa
   if (!p || !depth || !camera_pose || num_corrs < 0 || num_corrs > p->cfg.max_corrs || (num_corrs > 0 && !corrs))
     return NRF_ERR_INVALID_ARGUMENT;
   if (p->N <= 0) return NRF_ERR_NOT_READY;
-  if (!p->cfg.use_graph || p->prof_ring > 0)
+  if (!p->cfg.use_graph || p->prof_ring > 0) {
+    p->graph_step = false;
     return pipeline_enqueue(p, depth, camera_pose, corrs, canonical, num_corrs, stream, p->prof_ring > 0);
+  }
   const void *key[5] = {depth, corrs, canonical, stream, nullptr};
-  bool same = p->exec && p->g_n == num_corrs && memcmp(p->g_pose, camera_pose, sizeof(p->g_pose)) == 0;
+  bool same = p->exec && p->g_n == num_corrs && memcmp(p->g_pose, camera_pose, sizeof(p->g_pose)) == 0 &&
+              p->g_gen == solver_state_generation(p->solver);
   for (int k = 0; k < 4 && same; ++k) same = key[k] == p->g_key[k];
   if (!same) {
     if (p->exec) {
@@ -233,9 +248,9 @@ extern "C" int nrf_pipeline_step(nrf_pipeline *p, const float *depth, const floa
       (void)hipGraphDestroy(p->graph);
       p->graph = nullptr;
     }
-    hipStream_t st = as_stream(stream);
+    hipStream_t st = p->cap;
     NRF_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = pipeline_enqueue(p, depth, camera_pose, corrs, canonical, num_corrs, stream, false);
+    int rc = pipeline_enqueue(p, depth, camera_pose, corrs, canonical, num_corrs, st, false, false);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(st, &g);
     if (rc) {
@@ -251,13 +266,23 @@ extern "C" int nrf_pipeline_step(nrf_pipeline *p, const float *depth, const floa
     for (int k = 0; k < 4; ++k) p->g_key[k] = key[k];
     p->g_n = num_corrs;
     memcpy(p->g_pose, camera_pose, sizeof(p->g_pose));
+    p->g_gen = solver_state_generation(p->solver);
   }
-  NRF_HIP_CHECK(hipGraphLaunch(p->exec, as_stream(stream)));
+  hipStream_t st = as_stream(stream);
+  NRF_HIP_CHECK(hipEventRecord(p->ev[0], st));
+  NRF_HIP_CHECK(hipGraphLaunch(p->exec, st));
+  NRF_HIP_CHECK(hipEventRecord(p->ev[3], st));
+  p->graph_step = true;
   return NRF_OK;
 }
 
 extern "C" int nrf_pipeline_stage_ms(nrf_pipeline *p, float out_ms[3]) {
   if (!p || !out_ms) return NRF_ERR_INVALID_ARGUMENT;
+  if (p->graph_step) {  // a replayed graph: the whole step, no stage split
+    NRF_HIP_CHECK(hipEventElapsedTime(&out_ms[0], p->ev[0], p->ev[3]));
+    out_ms[1] = out_ms[2] = -1.f;
+    return NRF_OK;
+  }
   for (int k = 0; k < 3; ++k) NRF_HIP_CHECK(hipEventElapsedTime(&out_ms[k], p->ev[k], p->ev[k + 1]));
   return NRF_OK;
 }
@@ -312,4 +337,379 @@ extern "C" long long nrf_pipeline_updated_voxels(nrf_pipeline *p) {
   int32_t v = 0;
   if (hipMemcpy(&v, p->updated, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess) return -1;
   return v;
+}
+
+// ------------------------------------------------------------------ batch
+
+static_assert(NRF_MAX_BATCH == kMaxBatch, "batch size limits must agree");
+
+struct nrf_batch {
+  std::vector<nrf_pipeline *> p;
+  nrf_batch_tuning tune{};
+  SolverBatch *sb = nullptr;
+  IntegrateBatch *ib = nullptr;
+  int use_graph = 0;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  std::vector<nrf_batch_frame> g_frames;
+  void *g_stream = nullptr;
+  std::vector<unsigned> g_gen;
+  std::vector<unsigned> sb_gen;  // member solver generations the device records were built at
+  hipStream_t cap = nullptr;     // capture stream
+  bool graph_step = false;
+};
+
+static std::vector<unsigned> batch_generations(const nrf_batch *b) {
+  std::vector<unsigned> g;
+  for (const nrf_pipeline *p : b->p) g.push_back(solver_state_generation(p->solver));
+  return g;
+}
+
+static void batch_drop_graph(nrf_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+static void batch_free(nrf_batch *b) {
+  batch_drop_graph(b);
+  solver_batch_destroy(b->sb);
+  integrate_batch_destroy(b->ib);
+  for (auto &e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->cap) (void)hipStreamDestroy(b->cap);
+  delete b;
+}
+
+static int batch_build_integrate(const std::vector<nrf_pipeline *> &p, int z_chunk, nrf_voxel *const *vox,
+                                 int32_t *updated, IntegrateBatch **out) {
+  std::vector<IntegrateBatchInstance> ii(p.size());
+  for (size_t k = 0; k < p.size(); ++k) {
+    IntegrateBatchInstance &q = ii[k];
+    q.vp = p[k]->cfg.volume;
+    q.cam = p[k]->cfg.camera;
+    q.vox = vox ? vox[k] : p[k]->vox;
+    q.nodes = p[k]->nodes;
+    q.T = p[k]->T;
+    q.centered = p[k]->centered;
+    q.N = p[k]->N;
+    q.knn = p[k]->knn;
+    q.knn_w = p[k]->knn_w;
+    q.updated = updated ? updated : p[k]->updated;
+  }
+  return integrate_batch_create(ii.data(), (int)p.size(), z_chunk, out);
+}
+
+static int batch_build_solver(const std::vector<nrf_pipeline *> &p, int lanes, int wg_nodes, SolverBatch **out) {
+  std::vector<SolverBatchInstance> si(p.size());
+  for (size_t k = 0; k < p.size(); ++k) {
+    SolverBatchInstance &q = si[k];
+    q.solver = p[k]->solver;
+    q.nodes = p[k]->nodes;
+    q.T = p[k]->T;
+    q.centered = p[k]->centered;
+    q.N = p[k]->N;
+    q.gn = p[k]->cfg.gn;
+    q.delta_x = p[k]->delta_x;
+    q.max_corrs = p[k]->cfg.max_corrs;
+  }
+  return solver_batch_create(si.data(), (int)p.size(), lanes, wg_nodes, out);
+}
+
+static int batch_apply_tuning(nrf_batch *b, const nrf_batch_tuning &t) {
+  // the rebuild rewrites the members' partition tables: no step may be in flight
+  NRF_HIP_CHECK(hipDeviceSynchronize());
+  batch_drop_graph(b);
+  solver_batch_destroy(b->sb);
+  b->sb = nullptr;
+  integrate_batch_destroy(b->ib);
+  b->ib = nullptr;
+  int rc = batch_build_solver(b->p, t.lanes_per_row, t.wg_nodes, &b->sb);
+  if (!rc) rc = batch_build_integrate(b->p, t.z_chunk, nullptr, nullptr, &b->ib);
+  if (rc) return rc;
+  b->tune.lanes_per_row = t.lanes_per_row;
+  b->tune.wg_nodes = t.wg_nodes;
+  b->tune.z_chunk = t.z_chunk;
+  b->tune.launch_groups = solver_batch_launch_groups(b->sb);
+  b->sb_gen = batch_generations(b);
+  return NRF_OK;
+}
+
+extern "C" nrf_batch *nrf_batch_create(nrf_pipeline *const *pipelines, int count, const nrf_batch_tuning *tuning,
+                                       int use_graph) {
+  if (!pipelines || count < 1 || count > kMaxBatch) return nullptr;
+  for (int k = 0; k < count; ++k) {
+    const nrf_pipeline *q = pipelines[k];
+    if (!q || q->N <= 0 || q->cfg.storage16) return nullptr;
+    for (int j = 0; j < k; ++j)
+      if (pipelines[j] == q) return nullptr;
+  }
+  nrf_batch *b = new nrf_batch();
+  b->p.assign(pipelines, pipelines + count);
+  b->use_graph = use_graph ? 1 : 0;
+  nrf_batch_tuning t{};
+  t.lanes_per_row = 4;
+  t.wg_nodes = 40;
+  t.z_chunk = 4;
+  if (tuning) t = *tuning;
+  bool ok = true;
+  for (auto &e : b->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+  ok = ok && hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) == hipSuccess;
+  if (!ok || batch_apply_tuning(b, t) != NRF_OK) {
+    batch_free(b);
+    return nullptr;
+  }
+  return b;
+}
+
+extern "C" void nrf_batch_destroy(nrf_batch *b) {
+  if (b) batch_free(b);
+}
+
+extern "C" int nrf_batch_set_tuning(nrf_batch *b, const nrf_batch_tuning *tuning) {
+  if (!b || !tuning) return NRF_ERR_INVALID_ARGUMENT;
+  const nrf_batch_tuning old = b->tune;
+  int rc = batch_apply_tuning(b, *tuning);
+  if (rc && batch_apply_tuning(b, old) != NRF_OK) return NRF_ERR_HIP;
+  return rc;
+}
+
+extern "C" int nrf_batch_get_tuning(const nrf_batch *b, nrf_batch_tuning *tuning) {
+  if (!b || !tuning) return NRF_ERR_INVALID_ARGUMENT;
+  *tuning = b->tune;
+  return NRF_OK;
+}
+
+static int batch_frames(const nrf_batch *b, const nrf_batch_frame *frames, BatchCorrs &fc, BatchDepth &fd) {
+  memset(&fc, 0, sizeof(fc));
+  memset(&fd, 0, sizeof(fd));
+  for (size_t k = 0; k < b->p.size(); ++k) {
+    const nrf_batch_frame &f = frames[k];
+    const nrf_pipeline *q = b->p[k];
+    if (!f.depth || f.num_corrs < 0 || f.num_corrs > q->cfg.max_corrs || (f.num_corrs > 0 && !f.corrs))
+      return NRF_ERR_INVALID_ARGUMENT;
+    fc.corrs[k] = f.corrs;
+    fc.canonical[k] = f.canonical;
+    fc.n[k] = f.num_corrs;
+    fc.max_n = std::max(fc.max_n, f.num_corrs);
+    fd.depth[k] = f.depth;
+    fd.xf[k] = invert_pose(f.camera_pose);
+  }
+  return NRF_OK;
+}
+
+static int batch_enqueue(nrf_batch *b, const BatchCorrs &fc, const BatchDepth &fd, hipStream_t st, bool events) {
+  int rc;
+  if (events) NRF_HIP_CHECK(hipEventRecord(b->ev[0], st));
+  if ((rc = solver_batch_solve_gn(b->sb, fc, st))) return rc;
+  if (events) NRF_HIP_CHECK(hipEventRecord(b->ev[1], st));
+  if ((rc = integrate_batch_run(b->ib, fd, st))) return rc;
+  if (events) NRF_HIP_CHECK(hipEventRecord(b->ev[2], st));
+  for (nrf_pipeline *p : b->p) {
+    rc = nrf_mc_extract(&p->cfg.volume, p->vox, p->mc_ws, p->mc_bytes, p->verts, nullptr, p->tris,
+                        p->cfg.max_triangles, p->counts, st);
+    if (rc) return rc;
+  }
+  if (events) NRF_HIP_CHECK(hipEventRecord(b->ev[3], st));
+  return NRF_OK;
+}
+
+extern "C" int nrf_batch_step(nrf_batch *b, const nrf_batch_frame *frames, void *stream) {
+  if (!b || !frames) return NRF_ERR_INVALID_ARGUMENT;
+  BatchCorrs fc;
+  BatchDepth fd;
+  int rc = batch_frames(b, frames, fc, fd);
+  if (rc) return rc;
+  hipStream_t st = as_stream(stream);
+  // a member's partition or solver buffers changed since the records were built
+  if (b->sb_gen != batch_generations(b) && (rc = batch_apply_tuning(b, b->tune))) return rc;
+  if (!b->use_graph) {
+    b->graph_step = false;
+    return batch_enqueue(b, fc, fd, st, true);
+  }
+  const size_t n = b->p.size();
+  bool same = b->exec && b->g_stream == stream && b->g_frames.size() == n &&
+              memcmp(b->g_frames.data(), frames, n * sizeof(nrf_batch_frame)) == 0 &&
+              b->g_gen == batch_generations(b);
+  if (!same) {
+    batch_drop_graph(b);
+    NRF_HIP_CHECK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    rc = batch_enqueue(b, fc, fd, b->cap, false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->cap, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) {
+      set_last_error("hipStreamEndCapture", e);
+      return NRF_ERR_HIP;
+    }
+    b->graph = g;
+    NRF_HIP_CHECK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    b->g_frames.assign(frames, frames + n);
+    b->g_stream = stream;
+    b->g_gen = batch_generations(b);
+  }
+  NRF_HIP_CHECK(hipEventRecord(b->ev[0], st));
+  NRF_HIP_CHECK(hipGraphLaunch(b->exec, st));
+  NRF_HIP_CHECK(hipEventRecord(b->ev[3], st));
+  b->graph_step = true;
+  return NRF_OK;
+}
+
+extern "C" int nrf_batch_stage_ms(nrf_batch *b, float out_ms[3]) {
+  if (!b || !out_ms) return NRF_ERR_INVALID_ARGUMENT;
+  if (b->graph_step) {
+    NRF_HIP_CHECK(hipEventElapsedTime(&out_ms[0], b->ev[0], b->ev[3]));
+    out_ms[1] = out_ms[2] = -1.f;
+    return NRF_OK;
+  }
+  for (int k = 0; k < 3; ++k) NRF_HIP_CHECK(hipEventElapsedTime(&out_ms[k], b->ev[k], b->ev[k + 1]));
+  return NRF_OK;
+}
+
+// ---------------------------------------------------------------- tuner
+
+static std::mutex g_tune_mu;
+static std::map<std::string, nrf_batch_tuning> g_tune_cache;
+
+static std::string batch_shape_key(const nrf_batch *b) {
+  std::string key;
+  char buf[160];
+  for (const nrf_pipeline *p : b->p) {
+    const nrf_volume_params &v = p->cfg.volume;
+    snprintf(buf, sizeof(buf), "%d/%d/%d.%d.%d/%dx%d/%d;", p->N, solver_rmax(p->solver), v.dims[0], v.dims[1],
+             v.dims[2], p->cfg.camera.width, p->cfg.camera.height, p->cfg.gn.gn_iterations);
+    key += buf;
+  }
+  return key;
+}
+
+// Mean device time of `reps` runs of fn after one untimed run.
+template <typename F>
+static int time_ms(hipStream_t st, hipEvent_t e0, hipEvent_t e1, int reps, F fn, float &ms) {
+  int rc;
+  if ((rc = fn())) return rc;
+  NRF_HIP_CHECK(hipEventRecord(e0, st));
+  for (int r = 0; r < reps; ++r)
+    if ((rc = fn())) return rc;
+  NRF_HIP_CHECK(hipEventRecord(e1, st));
+  NRF_HIP_CHECK(hipEventSynchronize(e1));
+  NRF_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  ms /= (float)reps;
+  return NRF_OK;
+}
+
+static int batch_tune(nrf_batch *b, const BatchCorrs &fc, const BatchDepth &fd, hipStream_t st,
+                      nrf_batch_tuning &best) {
+  const int kReps = 3;
+  int rc = NRF_OK;
+  batch_drop_graph(b);
+  // systems of this frame at the current transforms (results and totals saved)
+  if ((rc = solver_batch_prepare(b->sb, fc, st))) return rc;
+  NRF_HIP_CHECK(hipStreamSynchronize(st));
+  SolverBatch *prep = b->sb;
+  b->sb = nullptr;
+  struct Cand {
+    int lanes, nodes;
+  };
+  const Cand cands[] = {{4, 24}, {4, 32}, {4, 40}, {2, 48}, {2, 64}, {2, 84}, {1, 96}, {1, 128}, {1, 170}};
+  best.pcg_ms = -1.f;
+  for (const Cand &c : cands) {
+    SolverBatch *sb = nullptr;
+    if (batch_build_solver(b->p, c.lanes, c.nodes, &sb) != NRF_OK) {
+      (void)hipGetLastError();
+      continue;  // does not fit this layout
+    }
+    float ms = 0.f;
+    rc = time_ms(st, b->ev[0], b->ev[1], kReps, [&]() { return solver_batch_pcg(sb, st); }, ms);
+    solver_batch_destroy(sb);
+    if (rc) break;
+    if (best.pcg_ms < 0.f || ms < best.pcg_ms) {
+      best.pcg_ms = ms;
+      best.lanes_per_row = c.lanes;
+      best.wg_nodes = c.nodes;
+    }
+  }
+  int rc2 = solver_batch_restore(prep, st);
+  if (!rc) rc = rc2;
+  rc2 = hipStreamSynchronize(st) == hipSuccess ? NRF_OK : NRF_ERR_HIP;
+  if (!rc) rc = rc2;
+  solver_batch_destroy(prep);
+  if (!rc && best.pcg_ms < 0.f) rc = NRF_ERR_TOO_LARGE;
+  if (rc) return rc;
+  // integration on scratch copies of the volumes
+  const size_t n = b->p.size();
+  std::vector<nrf_voxel *> copies(n, nullptr);
+  int32_t *upd = nullptr;
+  bool ok = hipMalloc(&upd, 4 * sizeof(int32_t)) == hipSuccess;
+  for (size_t k = 0; k < n && ok; ++k) {
+    const size_t bytes = (size_t)volume_count(&b->p[k]->cfg.volume) * sizeof(nrf_voxel);
+    ok = hipMalloc(&copies[k], bytes) == hipSuccess &&
+         hipMemcpyAsync(copies[k], b->p[k]->vox, bytes, hipMemcpyDeviceToDevice, st) == hipSuccess;
+  }
+  if (!ok) {
+    set_last_error("autotune scratch volumes", hipGetLastError());
+    rc = NRF_ERR_HIP;
+  }
+  best.integrate_ms = -1.f;
+  const int zcs[] = {1, 2, 4, 8};
+  for (int zc : zcs) {
+    if (rc) break;
+    IntegrateBatch *ib = nullptr;
+    if ((rc = batch_build_integrate(b->p, zc, copies.data(), upd, &ib))) break;
+    float ms = 0.f;
+    rc = time_ms(st, b->ev[0], b->ev[1], kReps, [&]() { return integrate_batch_run(ib, fd, st); }, ms);
+    integrate_batch_destroy(ib);
+    if (!rc && (best.integrate_ms < 0.f || ms < best.integrate_ms)) {
+      best.integrate_ms = ms;
+      best.z_chunk = zc;
+    }
+  }
+  (void)hipStreamSynchronize(st);
+  for (nrf_voxel *c : copies)
+    if (c) (void)hipFree(c);
+  if (upd) (void)hipFree(upd);
+  return rc;
+}
+
+extern "C" int nrf_batch_autotune(nrf_batch *b, const nrf_batch_frame *frames, int use_cache,
+                                  nrf_batch_tuning *out, void *stream) {
+  if (!b || !frames) return NRF_ERR_INVALID_ARGUMENT;
+  BatchCorrs fc;
+  BatchDepth fd;
+  int rc = batch_frames(b, frames, fc, fd);
+  if (rc) return rc;
+  const std::string key = batch_shape_key(b);
+  nrf_batch_tuning best = b->tune;
+  bool cached = false;
+  if (use_cache) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      best = it->second;
+      cached = true;
+    }
+  }
+  hipStream_t st = as_stream(stream);
+  if (!cached) {
+    rc = batch_tune(b, fc, fd, st, best);
+    if (rc) {
+      // leave the batch usable with its previous layout
+      if (batch_apply_tuning(b, b->tune) != NRF_OK) return NRF_ERR_HIP;
+      return rc;
+    }
+  }
+  if ((rc = batch_apply_tuning(b, best))) return rc;
+  b->tune.pcg_ms = best.pcg_ms;
+  b->tune.integrate_ms = best.integrate_ms;
+  if (!cached) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = b->tune;
+  }
+  if (out) *out = b->tune;
+  return NRF_OK;
 }

@@ -26,6 +26,7 @@
 //    recomputed on the fly by the readers of p (double-buffered), which
 //    leaves two grid barriers per iteration.  The barrier wait is bounded:
 //    on timeout the kernel raises a flag and every workgroup exits.
+#include <atomic>
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -81,17 +82,19 @@ __device__ inline int member_node(const nrf_corr &cr, int k, int N) {
   return (id >= 0 && id < N && cr.node_ws[k] > 1e-8f) ? id : -1;
 }
 
-__global__ __launch_bounds__(256) void mem_count_kernel(const nrf_corr *__restrict__ corrs, int n, int N,
-                                                        int32_t *__restrict__ cnt) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void mem_count_body(const nrf_corr *__restrict__ corrs, int n, int N, int32_t *__restrict__ cnt,
+                                      int c) {
   const bool ok = c < n && corrs[c].valid;
 #pragma unroll
   for (int k = 0; k < 4; ++k) wave_claim(cnt, ok ? member_node(corrs[c], k, N) : -1);
 }
+__global__ __launch_bounds__(256) void mem_count_kernel(const nrf_corr *__restrict__ corrs, int n, int N,
+                                                        int32_t *__restrict__ cnt) {
+  mem_count_body(corrs, n, N, cnt, blockIdx.x * blockDim.x + threadIdx.x);
+}
 
-__global__ __launch_bounds__(1024) void exclusive_scan_kernel(const int32_t *__restrict__ cnt,
-                                                              int32_t *__restrict__ off,
-                                                              int32_t *__restrict__ cur, int n) {
+__device__ inline void exclusive_scan_body(const int32_t *__restrict__ cnt, int32_t *__restrict__ off,
+                                           int32_t *__restrict__ cur, int n) {
   __shared__ int32_t part[1024];
   const int t = threadIdx.x;
   const int per = (n + 1023) / 1024;
@@ -114,10 +117,14 @@ __global__ __launch_bounds__(1024) void exclusive_scan_kernel(const int32_t *__r
   }
   if (t == 1023) off[n] = part[1023];
 }
+__global__ __launch_bounds__(1024) void exclusive_scan_kernel(const int32_t *__restrict__ cnt,
+                                                              int32_t *__restrict__ off,
+                                                              int32_t *__restrict__ cur, int n) {
+  exclusive_scan_body(cnt, off, cur, n);
+}
 
-__global__ __launch_bounds__(256) void mem_scatter_kernel(const nrf_corr *__restrict__ corrs, int n, int N,
-                                                          int32_t *__restrict__ cur, int32_t *__restrict__ keys) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void mem_scatter_body(const nrf_corr *__restrict__ corrs, int n, int N,
+                                        int32_t *__restrict__ cur, int32_t *__restrict__ keys, int c) {
   const bool ok = c < n && corrs[c].valid;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -125,17 +132,20 @@ __global__ __launch_bounds__(256) void mem_scatter_kernel(const nrf_corr *__rest
     if (pos >= 0) keys[pos] = c * 4 + k;
   }
 }
+__global__ __launch_bounds__(256) void mem_scatter_kernel(const nrf_corr *__restrict__ corrs, int n, int N,
+                                                          int32_t *__restrict__ cur, int32_t *__restrict__ keys) {
+  mem_scatter_body(corrs, n, N, cur, keys, blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // Orders each node's bucket by key.  Buckets of up to kSortLds keys are
 // bitonic-sorted in LDS (keys are unique, padding is INT_MAX); longer ones
 // fall back to ranking every key against the whole bucket.
 constexpr int kSortLds = 2048;
 constexpr int kSortThreads = 256;
-__global__ __launch_bounds__(kSortThreads) void mem_sort_kernel(const int32_t *__restrict__ off,
-                                                                const int32_t *__restrict__ keys_in,
-                                                                int32_t *__restrict__ keys_out) {
+__device__ void mem_sort_body(const int32_t *__restrict__ off, const int32_t *__restrict__ keys_in,
+                              int32_t *__restrict__ keys_out, int i) {
   __shared__ int32_t sk[kSortLds];
-  const int i = blockIdx.x, t = threadIdx.x;
+  const int t = threadIdx.x;
   const int b = off[i], n = off[i + 1] - b;
   if (n <= 0) return;  // uniform in the workgroup
   if (n > kSortLds) {
@@ -167,6 +177,11 @@ __global__ __launch_bounds__(kSortThreads) void mem_sort_kernel(const int32_t *_
       __syncthreads();
     }
   for (int j = t; j < n; j += kSortThreads) keys_out[b + j] = sk[j];
+}
+__global__ __launch_bounds__(kSortThreads) void mem_sort_kernel(const int32_t *__restrict__ off,
+                                                                const int32_t *__restrict__ keys_in,
+                                                                int32_t *__restrict__ keys_out) {
+  mem_sort_body(off, keys_in, keys_out, (int)blockIdx.x);
 }
 
 // ------------------------------------------------------------ assembly
@@ -469,9 +484,7 @@ __global__ __launch_bounds__(64) void assemble_rows_kernel(AsmArgs a) {
 // pad] = 32 floats (kRec).  Same expressions as stage_membership.
 constexpr int kRec = 32;
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
-__global__ __launch_bounds__(256) void corr_prep_kernel(AsmArgs a, int n, float *__restrict__ rec) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
+__device__ void corr_prep_one(const AsmArgs &a, int c, float *__restrict__ rec) {
   const nrf_corr &cr = a.corrs[c];
   if (!cr.valid) return;
   const float3 dst = f3(cr.dst), nrm = f3(cr.normal);
@@ -510,6 +523,10 @@ __global__ __launch_bounds__(256) void corr_prep_kernel(AsmArgs a, int n, float 
   float4 *dst4 = reinterpret_cast<float4 *>(rec + (size_t)c * kRec);
 #pragma unroll
   for (int k = 0; k < kRec / 4; ++k) dst4[k] = make_float4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+}
+__global__ __launch_bounds__(256) void corr_prep_kernel(AsmArgs a, int n, float *__restrict__ rec) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) corr_prep_one(a, c, rec);
 }
 
 // MFMA form of the data term (NS = column slabs of 16 covering the 6 R
@@ -550,16 +567,13 @@ __device__ inline floatx4_t mfma_lp(typename LpType<PREC>::vec a, typename LpTyp
 }
 
 template <int NS, int PREC>
-__global__ __launch_bounds__(64) void assemble_rows_mfma_kernel(AsmArgs a) {
-  extern __shared__ float lds[];
-  const int i = blockIdx.x;
+__device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_rhs) {
   if (a.stop && *a.stop) return;
   const int lane = threadIdx.x;
   const int rb = a.row_ptr[i], rlen = a.row_ptr[i + 1] - rb;
   float *acc = lds;                                      // [Rmax][36]
   int32_t *cols = reinterpret_cast<int32_t *>(acc + a.Rmax * 36);  // [Rmax]
   float *stage = reinterpret_cast<float *>(cols + a.Rmax);         // [64][kStage]
-  __shared__ float s_rhs[6];
   for (int j = lane; j < rlen; j += 64) cols[j] = a.col_idx[rb + j];
   __syncthreads();
   // staging layout per membership: a[6], r, packed pos (4 bytes), J[4][6], 0
@@ -711,6 +725,12 @@ __global__ __launch_bounds__(64) void assemble_rows_mfma_kernel(AsmArgs a) {
   const float bacc = (ra >= 0 && ra < 6) ? s_rhs[ra] : 0.f;
   row_tail(a, i, rb, rlen, acc, stage, bacc);
 }
+template <int NS, int PREC>
+__global__ __launch_bounds__(64) void assemble_rows_mfma_kernel(AsmArgs a) {
+  extern __shared__ float lds[];
+  __shared__ float s_rhs[6];
+  assemble_row_mfma<NS, PREC>(a, (int)blockIdx.x, lds, s_rhs);
+}
 
 // ----------------------------------------------------------------- PCG
 // Cross-workgroup traffic of the persistent PCG uses a flag-in-data
@@ -855,12 +875,13 @@ __device__ inline bool ll_spin(unsigned &spins, PcgCtl *ctl, int *bad) {
 // holds the same bits.  Records are double-buffered by the parity of the
 // point: a workgroup is at most one point ahead of the slowest one.
 // ar2_collect returns true when the solve has to be abandoned.
-__device__ inline void ar2_publish(const PcgArgs &a, unsigned tag, int slot, double v0, double v1, double *red) {
+__device__ inline void ar2_publish(const PcgArgs &a, int g, unsigned tag, int slot, double v0, double v1,
+                                   double *red) {
   wg_sum2_lanes03(v0, v1, red);
   const int t = threadIdx.x;
   if (t < 4) {
     const ll_word bits = (ll_word)__double_as_longlong(t < 2 ? v0 : v1);
-    ll_put(&a.recs[blockIdx.x].w[slot][t], tag, (t & 1) ? (unsigned)bits : (unsigned)(bits >> 32));
+    ll_put(&a.recs[g].w[slot][t], tag, (t & 1) ? (unsigned)bits : (unsigned)(bits >> 32));
   }
 }
 __device__ inline bool ar2_collect(const PcgArgs &a, unsigned tag, int slot, double &v0, double &v1,
@@ -889,7 +910,7 @@ __device__ inline bool ar2_collect(const PcgArgs &a, unsigned tag, int slot, dou
 }
 __device__ inline bool allreduce2(const PcgArgs &a, unsigned tag, int slot, double &v0, double &v1, double *red,
                                   int *s_bad) {
-  ar2_publish(a, tag, slot, v0, v1, red);
+  ar2_publish(a, (int)blockIdx.x, tag, slot, v0, v1, red);
   return ar2_collect(a, tag, slot, v0, v1, red, s_bad);
 }
 
@@ -1045,6 +1066,9 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
     a.res->flags |= flags;
     a.res->final_residual = sqrt(rr);
     a.res->rhs_norm = sqrt(bb);
+    a.res->converged += (flags & 2) ? 1 : 0;
+    a.res->breakdowns += (flags & 1) ? 1 : 0;
+    a.res->runs += 1;
   }
 }
 
@@ -1061,13 +1085,11 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
 // Same iterates as pcg_kernel in exact arithmetic; in FP32 the one extra
 // recurrence (for Ap) keeps the iterates within a few ulps of the textbook
 // ones.  Layout, publication and the lane-per-row split are pcg_kernel's.
-__global__ __launch_bounds__(kPcgThreads) void pcg_pipe_kernel(PcgArgs a) {
-  extern __shared__ float lds[];
-  __shared__ double red[2 * kPcgWaves + 8];
-  __shared__ int s_bad;
-  __shared__ unsigned s_base;
+template <int SUB>
+__device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, int &s_bad, unsigned &s_base) {
+  constexpr int ROWS = kPcgThreads / SUB;
   if (a.stop && *a.stop) return;
-  const int g = blockIdx.x, t = threadIdx.x;
+  const int t = threadIdx.x;
   const int n0 = a.rows[g], n1 = a.rows[g + 1];
   const int S = 6 * (n1 - n0);
   const int blk0 = a.row_ptr[n0], nblk = a.row_ptr[n1] - blk0;
@@ -1075,8 +1097,8 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_pipe_kernel(PcgArgs a) {
   float *Al = lds;
   int32_t *Ll = reinterpret_cast<int32_t *>(Al + (a.lds_a ? (size_t)nblk * 36 : 0));
   float *ph = reinterpret_cast<float *>(Ll + (a.lds_a ? nblk : 0));
-  float *rl = ph + (size_t)nh * 6;  // own rows' r, then own rows' z (kPcgRows each)
-  float *zl = rl + kPcgRows;
+  float *rl = ph + (size_t)nh * 6;  // own rows' r, then own rows' z (ROWS each)
+  float *zl = rl + ROWS;
   if (a.lds_a) {
     const float4 *src = reinterpret_cast<const float4 *>(a.values + (size_t)blk0 * 36);
     float4 *dst = reinterpret_cast<float4 *>(Al);
@@ -1087,7 +1109,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_pipe_kernel(PcgArgs a) {
     s_base = (unsigned)(__hip_atomic_load(&a.ctl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) << 16);
     s_bad = 0;
   }
-  const int q = t / kPcgSub, sub = t % kPcgSub;
+  const int q = t / SUB, sub = t % SUB;
   const bool own = q < S, lead = own && sub == 0;
   const int li = q / 6, row = q % 6, i = n0 + li, s = 6 * n0 + q;
   float M[6] = {0, 0, 0, 0, 0, 0};
@@ -1137,15 +1159,15 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_pipe_kernel(PcgArgs a) {
     __syncthreads();
     float y = 0.f;
     if (own) {
-      for (int bk = rb + sub; bk < re; bk += kPcgSub) {
+      for (int bk = rb + sub; bk < re; bk += SUB) {
         const int lc = a.lds_a ? Ll[bk - blk0] : a.lcol[bk];
         const float *A = a.lds_a ? (Al + (size_t)(bk - blk0) * 36) : (a.values + (size_t)bk * 36);
         const float *pj = ph + lc * 6;
         for (int c = 0; c < 6; ++c) y += A[row * 6 + c] * pj[c];
       }
     }
-    y += __shfl_xor(y, 1, 64);
-    y += __shfl_xor(y, 2, 64);
+    if constexpr (SUB >= 2) y += __shfl_xor(y, 1, 64);
+    if constexpr (SUB >= 4) y += __shfl_xor(y, 2, 64);
     return y;
   };
 
@@ -1156,7 +1178,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_pipe_kernel(PcgArgs a) {
   float x = 0.f, r = bv;
   float z = precond_publish(r, base + pt);
   double rz = lead ? (double)r * (double)z : 0.0, bb = lead ? (double)bv * (double)bv : 0.0;
-  ar2_publish(a, base + pt, (int)(pt & 1), rz, bb, red);
+  ar2_publish(a, g, base + pt, (int)(pt & 1), rz, bb, red);
   float p = z, y = product(base + pt);
   bool aborted = ar2_collect(a, base + pt, (int)(pt & 1), rz, bb, red, &s_bad);
   double rr = bb;
@@ -1165,7 +1187,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_pipe_kernel(PcgArgs a) {
   for (int it = 0; it < a.maxit && !aborted; ++it) {
     ++pt;  // the p.Ap point
     double pAp = lead ? (double)p * (double)y : 0.0, unused = 0.0;
-    ar2_publish(a, base + pt, (int)(pt & 1), pAp, unused, red);
+    ar2_publish(a, g, base + pt, (int)(pt & 1), pAp, unused, red);
     aborted = ar2_collect(a, base + pt, (int)(pt & 1), pAp, unused, red, &s_bad);
     if (aborted) break;
     if (fabs(pAp) < 1e-12) {
@@ -1178,7 +1200,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_pipe_kernel(PcgArgs a) {
     r += -alf * y;
     z = precond_publish(r, base + pt);
     double rr_l = lead ? (double)r * (double)r : 0.0, rz_l = lead ? (double)r * (double)z : 0.0;
-    ar2_publish(a, base + pt, (int)(pt & 1), rr_l, rz_l, red);
+    ar2_publish(a, g, base + pt, (int)(pt & 1), rr_l, rz_l, red);
     const float az = product(base + pt);
     aborted = ar2_collect(a, base + pt, (int)(pt & 1), rr_l, rz_l, red, &s_bad);
     if (aborted) break;
@@ -1201,8 +1223,21 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_pipe_kernel(PcgArgs a) {
     a.res->flags |= flags;
     a.res->final_residual = sqrt(rr);
     a.res->rhs_norm = sqrt(bb);
+    a.res->converged += (flags & 2) ? 1 : 0;
+    a.res->breakdowns += (flags & 1) ? 1 : 0;
+    a.res->runs += 1;
   }
 }
+
+template <int SUB>
+__global__ __launch_bounds__(kPcgThreads) void pcg_pipe_kernel(PcgArgs a) {
+  extern __shared__ float lds[];
+  __shared__ double red[2 * kPcgWaves + 8];
+  __shared__ int s_bad;
+  __shared__ unsigned s_base;
+  pcg_pipe_body<SUB>(a, (int)blockIdx.x, lds, red, s_bad, s_base);
+}
+
 
 // r = b - A x for every scalar row, products and sums in FP64 (the matrix
 // and x are FP32); optionally stores r (rounded to FP32) and per-workgroup
@@ -1273,14 +1308,20 @@ __global__ void pcg_prep_kernel(PcgCtl *ctl) {
   }
 }
 
-__global__ void gn_begin_kernel(int32_t *stop, nrf_pcg_result *res, nrf_solver_totals *tot) {
-  if (threadIdx.x == 0) {
+__device__ inline void gn_begin_body(int32_t *stop, nrf_pcg_result *res, nrf_solver_totals *tot) {
+  {
     // fold the previous solve into the running totals before clearing it
     tot->solves += 1;
     tot->gn_done += res->gn_done;
     tot->pcg_iterations += res->total_iterations;
     tot->rejected += (res->flags & 4) ? 1 : 0;
+    tot->pcg_runs += res->runs;
+    tot->pcg_converged += res->converged;
+    tot->pcg_breakdowns += res->breakdowns;
     *stop = 0;
+    res->runs = 0;
+    res->converged = 0;
+    res->breakdowns = 0;
     res->iterations = 0;
     res->total_iterations = 0;
     res->flags = 0;
@@ -1288,6 +1329,9 @@ __global__ void gn_begin_kernel(int32_t *stop, nrf_pcg_result *res, nrf_solver_t
     res->final_residual = 0;
     res->rhs_norm = 0;
   }
+}
+__global__ void gn_begin_kernel(int32_t *stop, nrf_pcg_result *res, nrf_solver_totals *tot) {
+  if (threadIdx.x == 0) gn_begin_body(stop, res, tot);
 }
 
 // Step validation (include/fusion_app.h:409-491) by one workgroup: a
@@ -1305,7 +1349,7 @@ struct GnArgs {
   nrf_pcg_result *res;
 };
 
-__global__ __launch_bounds__(1024) void gn_validate_kernel(GnArgs a) {
+__device__ void gn_validate_body(const GnArgs &a) {
   if (*a.stop) return;
   __shared__ double s_sum[4][1024 / 64];
   __shared__ float s_max[4][1024 / 64];
@@ -1368,12 +1412,13 @@ __global__ __launch_bounds__(1024) void gn_validate_kernel(GnArgs a) {
     }
   }
 }
+__global__ __launch_bounds__(1024) void gn_validate_kernel(GnArgs a) { gn_validate_body(a); }
 
 // Application of the validated step, one node per lane: per-node clamp and
 // scale (upload_scaled_delta_x, include/fusion_app.h:1353-1375), then
 // apply_twist_increment(..., 1.0) (src/warp_field.cu:1121-1141) and the
 // centred-table refresh.
-__global__ __launch_bounds__(256) void gn_update_kernel(GnArgs a) {
+__device__ void gn_update_body(const GnArgs &a) {
   if (*a.stop) return;
   const int t = threadIdx.x;
   for (int i = blockIdx.x * 256 + t; i < a.N; i += gridDim.x * 256) {
@@ -1405,6 +1450,103 @@ __global__ __launch_bounds__(256) void gn_update_kernel(GnArgs a) {
     if (a.centered) a.centered[i] = dq_centered(Tn, f3(a.nodes[i].pos));
   }
   if (blockIdx.x == 0 && t == 0) a.res->gn_done += 1;
+}
+__global__ __launch_bounds__(256) void gn_update_kernel(GnArgs a) { gn_update_body(a); }
+
+// ------------------------------------------------------------- batches
+// Several independent solves stepped by common launches (nrf_batch): every
+// launch covers all instances of the batch.  An instance's device state is
+// one InstDev record; per-frame inputs (BatchCorrs) travel as kernel
+// arguments.  Elementwise work runs on 2-D grids (blockIdx.y = instance,
+// blocks past the instance's size leave at once); per-row work (bucket
+// sort, assembly) and the persistent PCG take row / workgroup tables that
+// map a block to (instance, local index).
+struct InstDev {
+  AsmArgs as;
+  PcgArgs pcg;
+  GnArgs gn;
+  int32_t *m_cnt, *m_off, *m_cur, *m_keys;
+  nrf_solver_totals *tot;
+};
+
+__global__ __launch_bounds__(256) void mem_fill_batch_kernel(const InstDev *__restrict__ inst) {
+  const InstDev &d = inst[blockIdx.y];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < d.as.N) d.m_cnt[i] = 0;
+}
+__global__ __launch_bounds__(256) void mem_count_batch_kernel(const InstDev *__restrict__ inst, BatchCorrs fr) {
+  const int k = blockIdx.y, n = fr.n[k];
+  if ((int)blockIdx.x * 256 >= n) return;  // whole workgroup: wave_claim needs every lane
+  mem_count_body(fr.corrs[k], n, inst[k].as.N, inst[k].m_cnt, blockIdx.x * 256 + threadIdx.x);
+}
+__global__ __launch_bounds__(1024) void exclusive_scan_batch_kernel(const InstDev *__restrict__ inst) {
+  const InstDev &d = inst[blockIdx.x];
+  exclusive_scan_body(d.m_cnt, d.m_off, d.m_cur, d.as.N);
+}
+__global__ __launch_bounds__(256) void mem_scatter_batch_kernel(const InstDev *__restrict__ inst, BatchCorrs fr) {
+  const int k = blockIdx.y, n = fr.n[k];
+  if ((int)blockIdx.x * 256 >= n) return;
+  mem_scatter_body(fr.corrs[k], n, inst[k].as.N, inst[k].m_cur, inst[k].m_keys, blockIdx.x * 256 + threadIdx.x);
+}
+__global__ __launch_bounds__(kSortThreads) void mem_sort_batch_kernel(const InstDev *__restrict__ inst,
+                                                                      const int32_t *__restrict__ row_inst,
+                                                                      const int32_t *__restrict__ row_local) {
+  const InstDev &d = inst[row_inst[blockIdx.x]];
+  mem_sort_body(d.m_off, d.m_keys, const_cast<int32_t *>(d.as.m_keys), row_local[blockIdx.x]);
+}
+__global__ void gn_begin_batch_kernel(const InstDev *__restrict__ inst, int count) {
+  const int k = threadIdx.x;
+  if (k < count) gn_begin_body(const_cast<int32_t *>(inst[k].as.stop), inst[k].pcg.res, inst[k].tot);
+}
+__global__ __launch_bounds__(256) void corr_prep_batch_kernel(const InstDev *__restrict__ inst, BatchCorrs fr) {
+  const int k = blockIdx.y;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= fr.n[k]) return;
+  AsmArgs a = inst[k].as;
+  a.corrs = fr.corrs[k];
+  a.canonical = fr.canonical[k];
+  corr_prep_one(a, c, const_cast<float *>(a.crec));
+}
+template <int NS, int PREC>
+__global__ __launch_bounds__(64) void assemble_rows_batch_kernel(const InstDev *__restrict__ inst, BatchCorrs fr,
+                                                                 const int32_t *__restrict__ row_inst,
+                                                                 const int32_t *__restrict__ row_local) {
+  extern __shared__ float lds[];
+  __shared__ float s_rhs[6];
+  const int k = row_inst[blockIdx.x];
+  AsmArgs a = inst[k].as;
+  a.corrs = fr.corrs[k];
+  a.canonical = fr.canonical[k];
+  assemble_row_mfma<NS, PREC>(a, row_local[blockIdx.x], lds, s_rhs);
+}
+__global__ void pcg_prep_batch_kernel(const InstDev *__restrict__ inst, int count) {
+  const int k = threadIdx.x;
+  if (k < count) {
+    inst[k].pcg.ctl->epoch += 1;
+    inst[k].pcg.ctl->abort = 0;
+  }
+}
+// Several independent solves in one persistent launch: workgroup b works on
+// instance wg_inst[b] as its workgroup wg_local[b].  The instances share no
+// data; each synchronises its own workgroups through its own records, so
+// only the workgroups of one instance need to be co-resident.
+template <int SUB>
+__global__ __launch_bounds__(kPcgThreads) void pcg_pipe_batch_kernel(const InstDev *__restrict__ inst,
+                                                                     const int32_t *__restrict__ wg_inst,
+                                                                     const int32_t *__restrict__ wg_local) {
+  extern __shared__ float lds[];
+  __shared__ double red[2 * kPcgWaves + 8];
+  __shared__ int s_bad;
+  __shared__ unsigned s_base;
+  const PcgArgs a = inst[wg_inst[blockIdx.x]].pcg;
+  pcg_pipe_body<SUB>(a, wg_local[blockIdx.x], lds, red, s_bad, s_base);
+}
+__global__ __launch_bounds__(1024) void gn_validate_batch_kernel(const InstDev *__restrict__ inst) {
+  const GnArgs &g = inst[blockIdx.x].gn;
+  if (g.validate) gn_validate_body(g);
+}
+__global__ __launch_bounds__(256) void gn_update_batch_kernel(const InstDev *__restrict__ inst) {
+  gn_update_body(inst[blockIdx.y].gn);
 }
 
 }  // namespace
@@ -1479,9 +1621,14 @@ static void build_pattern(const nrf_node *h, int N, Pattern &P) {
   P.es_off[N] = (int32_t)P.es_entry.size();
 }
 
+// Counts reallocations of solver buffers: a captured step that holds the old
+// addresses must be captured again (solver_state_generation).
+static std::atomic<unsigned> g_realloc{0};
+
 template <typename T>
 static int dev_alloc(T *&p, size_t n, size_t &cap) {
   if (n <= cap && p) return NRF_OK;
+  g_realloc.fetch_add(1);
   if (p) hipFree(p);
   p = nullptr;
   cap = 0;
@@ -1545,7 +1692,10 @@ struct nrf_solver {
   size_t c_tot = 0;
   int cus = 0, max_lds = 0, coop = 0;
   int forced_G = 0;
+  int pcg_sub = 4;        // lanes per scalar row of the PCG (1, 2, 4)
+  int pcg_wg_nodes = 40;  // target nodes per PCG workgroup
   int asm_mfma = 1;  // data term on the matrix cores (rows of up to 21 blocks)
+  unsigned gen = 0;  // advanced by every partition rebuild
   // optional profiling events (see solver_set_event_sink)
   hipEvent_t *sink = nullptr;
   int sink_n = 0, sink_i = 0;
@@ -1563,6 +1713,8 @@ static void sink_record(nrf_solver *s, hipStream_t st) {
   if (s->sink && s->sink_i < s->sink_n) (void)hipEventRecord(s->sink[s->sink_i++], st);
 }
 
+static int solver_build_partition(nrf_solver *s);
+
 static int solver_set_pattern_host(nrf_solver *s, const nrf_node *h, int N) {
   build_pattern(h, N, s->P);
   const Pattern &P = s->P;
@@ -1577,12 +1729,23 @@ static int solver_set_pattern_host(nrf_solver *s, const nrf_node *h, int N) {
   if ((st = upload(s->es_entry, s->c_es_entry, P.es_entry))) return st;
   if ((st = upload(s->es_offrel, s->c_es_offrel, P.es_offrel))) return st;
   if ((st = dev_alloc(s->values, (size_t)std::max(P.nb, 1) * 36, s->c_values))) return st;
-  // PCG partition: contiguous ranges of at most 42 nodes (one scalar row
-  // per lane), each with its halo (the distinct columns of its rows) and the
-  // blocks' columns renumbered into that halo.
+  return solver_build_partition(s);
+}
+
+// PCG partition: contiguous node ranges of at most 1024 / (6 pcg_sub) nodes
+// (pcg_sub lanes per scalar row), each with its halo (the distinct columns of
+// its rows) and the blocks' columns renumbered into that halo.  The target
+// size (pcg_wg_nodes) trades workgroups for rows per workgroup.
+static int solver_build_partition(nrf_solver *s) {
+  ++s->gen;
+  const Pattern &P = s->P;
+  const int N = P.N;
+  int st;
+  const int max_nodes = kPcgThreads / s->pcg_sub / 6;
+  const int target = std::max(1, std::min(max_nodes, s->pcg_wg_nodes));
   int G = s->forced_G;
-  const int min_G = std::max(1, (N + kPcgMaxNodesPerWg - 1) / kPcgMaxNodesPerWg);
-  if (G < min_G) G = std::max(min_G, std::min(s->cus, (N + 39) / 40));
+  const int min_G = std::max(1, (N + max_nodes - 1) / max_nodes);
+  if (G < min_G) G = std::max(min_G, std::min(s->cus, (N + target - 1) / target));
   if (G > kPcgMaxWg) return NRF_ERR_TOO_LARGE;  // one record per workgroup
   std::vector<int32_t> rows(G + 1);
   for (int g = 0; g <= G; ++g) rows[g] = (int32_t)((long)N * g / G);
@@ -1598,7 +1761,7 @@ static int solver_set_pattern_host(nrf_solver *s, const nrf_node *h, int N) {
     hoff[g] = (int32_t)halo.size();
     halo.insert(halo.end(), h.begin(), h.end());
     const size_t nblk = (size_t)(b1 - b0);
-    need_max = std::max(need_max, nblk * (36 * 4 + 4) + h.size() * 24 + kPcgRows * 8);
+    need_max = std::max(need_max, nblk * (36 * 4 + 4) + h.size() * 24 + (size_t)kPcgThreads * 8);
   }
   hoff[G] = (int32_t)halo.size();
   size_t halo_max = 0;
@@ -1606,21 +1769,27 @@ static int solver_set_pattern_host(nrf_solver *s, const nrf_node *h, int N) {
   s->G = G;
   const size_t budget = (size_t)std::max(0, s->max_lds - 1024);
   s->lds_a = need_max <= budget ? 1 : 0;
-  s->pcg_lds = align_up(s->lds_a ? need_max : halo_max * 24 + kPcgRows * 8, 16);
+  s->pcg_lds = align_up(s->lds_a ? need_max : halo_max * 24 + (size_t)kPcgThreads * 8, 16);
   if ((st = upload(s->rows, s->c_rows, rows))) return st;
   if ((st = upload(s->lcol, s->c_lcol, lcol))) return st;
   if ((st = upload(s->halo_off, s->c_halo_off, hoff))) return st;
   if ((st = upload(s->halo, s->c_halo, halo))) return st;
   if (s->pcg_lds > 65536 && s->pcg_lds > s->lds_attr) {
-    if (hipFuncSetAttribute(reinterpret_cast<const void *>(pcg_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)s->pcg_lds) == hipSuccess &&
-        hipFuncSetAttribute(reinterpret_cast<const void *>(pcg_pipe_kernel),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->pcg_lds) == hipSuccess) {
+    const void *fns[] = {reinterpret_cast<const void *>(pcg_kernel), reinterpret_cast<const void *>(pcg_pipe_kernel<1>),
+                         reinterpret_cast<const void *>(pcg_pipe_kernel<2>),
+                         reinterpret_cast<const void *>(pcg_pipe_kernel<4>),
+                         reinterpret_cast<const void *>(pcg_pipe_batch_kernel<1>),
+                         reinterpret_cast<const void *>(pcg_pipe_batch_kernel<2>),
+                         reinterpret_cast<const void *>(pcg_pipe_batch_kernel<4>)};
+    bool ok = true;
+    for (const void *fn : fns)
+      ok = ok && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->pcg_lds) == hipSuccess;
+    if (ok) {
       s->lds_attr = s->pcg_lds;
     } else {
       (void)hipGetLastError();
       s->lds_a = 0;  // keep the matrix in global memory
-      s->pcg_lds = align_up(halo_max * 24 + kPcgRows * 8, 16);
+      s->pcg_lds = align_up(halo_max * 24 + (size_t)kPcgThreads * 8, 16);
     }
   }
   return NRF_OK;
@@ -1650,8 +1819,8 @@ static int solver_build_memberships(nrf_solver *s, const nrf_corr *corrs, int n,
   return NRF_OK;
 }
 
-static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, int n, const float *canonical,
-                           const nrf_node *nodes, const DQ *T, const DQ *centered, int N, hipStream_t st) {
+static AsmArgs solver_asm_args(const nrf_solver *s, const nrf_corr *corrs, const float *canonical,
+                               const nrf_node *nodes, const DQ *T, const DQ *centered, int N) {
   AsmArgs a;
   a.corrs = corrs;
   a.canonical = canonical;
@@ -1679,11 +1848,22 @@ static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, int n, const fl
   a.b = s->b;
   a.Minv = s->Minv;
   a.stop = s->stop;
-  const size_t lds = (size_t)s->P.Rmax * 36 * 4 + (size_t)s->P.Rmax * 4 + 64 * kStage * 4;
-  if (lds > 64 * 1024) return NRF_ERR_TOO_LARGE;
-  const int ns = (6 * s->P.Rmax + 1 + 15) / 16;
-  const bool mfma = s->asm_mfma && ns <= 8;
   a.crec = nullptr;
+  return a;
+}
+
+// LDS bytes of one assembly workgroup and the MFMA column strips of a row
+// of Rmax blocks (block columns + the RHS column, 16 per strip).
+static size_t asm_lds_bytes(int Rmax) { return (size_t)Rmax * 36 * 4 + (size_t)Rmax * 4 + 64 * kStage * 4; }
+static int asm_strips(int Rmax) { return (6 * Rmax + 1 + 15) / 16; }
+
+static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, int n, const float *canonical,
+                           const nrf_node *nodes, const DQ *T, const DQ *centered, int N, hipStream_t st) {
+  AsmArgs a = solver_asm_args(s, corrs, canonical, nodes, T, centered, N);
+  const size_t lds = asm_lds_bytes(s->P.Rmax);
+  if (lds > 64 * 1024) return NRF_ERR_TOO_LARGE;
+  const int ns = asm_strips(s->P.Rmax);
+  const bool mfma = s->asm_mfma && ns <= 8;
   if (mfma && n > 0) {
     int rc;
     if ((rc = dev_alloc(s->crec, (size_t)n * kRec, s->c_crec))) return rc;
@@ -1738,10 +1918,7 @@ static int solver_pcg(nrf_solver *s, float *x, int N, hipStream_t st) {
 }
 
 // `record`: the profiling sink's event that opens the PCG group (one per solve).
-static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxit, bool record, hipStream_t st) {
-  pcg_prep_kernel<<<1, 64, 0, st>>>(s->ctl);
-  NRF_LAUNCH_CHECK();
-  if (record) sink_record(s, st);
+static PcgArgs solver_pcg_args(const nrf_solver *s, const float *rhs, float *x, int N, int maxit) {
   PcgArgs a;
   a.row_ptr = s->row_ptr;
   a.lcol = s->lcol;
@@ -1765,17 +1942,37 @@ static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxi
   a.rel = s->params.relative_tolerance;
   a.res = s->d_res;
   a.stop = s->stop;
+  return a;
+}
+
+// A persistent kernel captured into a graph takes an ordinary launch: its
+// grid is sized to be co-resident on the device (at most the CU count, or
+// the occupancy bound for batches), and every wait in it is bounded.
+static bool stream_capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return cs != hipStreamCaptureStatusNone;
+}
+
+static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxit, bool record, hipStream_t st) {
+  pcg_prep_kernel<<<1, 64, 0, st>>>(s->ctl);
+  NRF_LAUNCH_CHECK();
+  if (record) sink_record(s, st);
+  PcgArgs a = solver_pcg_args(s, rhs, x, N, maxit);
   void *args[] = {&a};
   const bool pipe = s->params.pcg_pipelined != 0;
-  if (s->coop) {
-    const void *fn = pipe ? reinterpret_cast<const void *>(pcg_pipe_kernel) : reinterpret_cast<const void *>(pcg_kernel);
+  if (!pipe && s->pcg_sub != kPcgSub) return NRF_ERR_INVALID_ARGUMENT;  // textbook form: 4 lanes per row only
+  const void *fn = !pipe              ? reinterpret_cast<const void *>(pcg_kernel)
+                   : s->pcg_sub == 1 ? reinterpret_cast<const void *>(pcg_pipe_kernel<1>)
+                   : s->pcg_sub == 2 ? reinterpret_cast<const void *>(pcg_pipe_kernel<2>)
+                                     : reinterpret_cast<const void *>(pcg_pipe_kernel<4>);
+  if (s->coop && !stream_capturing(st)) {
     NRF_HIP_CHECK(hipLaunchCooperativeKernel(fn, dim3(s->G), dim3(kPcgThreads), args, (unsigned)s->pcg_lds, st));
   } else {
-    if (pipe)
-      pcg_pipe_kernel<<<s->G, kPcgThreads, s->pcg_lds, st>>>(a);
-    else
-      pcg_kernel<<<s->G, kPcgThreads, s->pcg_lds, st>>>(a);
-    NRF_LAUNCH_CHECK();
+    NRF_HIP_CHECK(hipLaunchKernel(fn, dim3(s->G), dim3(kPcgThreads), args, (unsigned)s->pcg_lds, st));
   }
   return NRF_OK;
 }
@@ -1951,6 +2148,15 @@ extern "C" int nrf_solver_system(const nrf_solver *s, const int32_t **row_ptr, c
 
 extern "C" const nrf_pcg_result *nrf_solver_device_result(const nrf_solver *s) { return s ? s->d_res : nullptr; }
 
+extern "C" int nrf_solver_set_pcg_layout(nrf_solver *s, int lanes_per_row, int wg_nodes) {
+  if (!s || (lanes_per_row != 1 && lanes_per_row != 2 && lanes_per_row != 4) || wg_nodes < 1)
+    return NRF_ERR_INVALID_ARGUMENT;
+  s->pcg_sub = lanes_per_row;
+  s->pcg_wg_nodes = wg_nodes;
+  if (s->P.N <= 0) return NRF_OK;
+  return solver_build_partition(s);
+}
+
 extern "C" int nrf_solver_resolve(nrf_solver *s, float *delta_x, nrf_pcg_result *result, void *stream) {
   if (!s || !delta_x) return NRF_ERR_INVALID_ARGUMENT;
   const int N = s->P.N;
@@ -1993,5 +2199,309 @@ extern "C" int nrf_solver_get_totals(nrf_solver *s, nrf_solver_totals *out, void
   out->gn_done += last.gn_done;
   out->pcg_iterations += last.total_iterations;
   out->rejected += (last.flags & 4) ? 1 : 0;
+  out->pcg_runs += last.runs;
+  out->pcg_converged += last.converged;
+  out->pcg_breakdowns += last.breakdowns;
   return NRF_OK;
 }
+
+// ------------------------------------------------------------------ batches
+
+struct nrf::SolverBatch {
+  std::vector<SolverBatchInstance> in;
+  int count = 0, sub = 4, prec = 0, ns = 1, gn_iters = 1, any_validate = 0, coop = 0;
+  int max_N = 0, rows = 0, wgs = 0;
+  size_t asm_lds = 0, pcg_lds = 0;
+  InstDev *d_inst = nullptr;
+  int32_t *row_inst = nullptr, *row_local = nullptr, *wg_inst = nullptr, *wg_local = nullptr;
+  std::vector<std::pair<int, int>> groups;  // [first, last) workgroup of each persistent launch
+  nrf_pcg_result *save_res = nullptr;
+  nrf_solver_totals *save_tot = nullptr;
+};
+
+namespace nrf {
+
+void solver_batch_destroy(SolverBatch *b) {
+  if (!b) return;
+  void *ptrs[] = {b->d_inst, b->row_inst, b->row_local, b->wg_inst, b->wg_local, b->save_res, b->save_tot};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  delete b;
+}
+
+int solver_rmax(const nrf_solver *s) { return s ? s->P.Rmax : 0; }
+
+int solver_reserve(nrf_solver *s, int max_corrs) {
+  if (!s || max_corrs < 0) return NRF_ERR_INVALID_ARGUMENT;
+  const size_t N = (size_t)std::max(1, s->P.N), nc = (size_t)std::max(1, max_corrs);
+  int rc;
+  if ((rc = dev_alloc(s->crec, nc * kRec, s->c_crec))) return rc;
+  if ((rc = dev_alloc(s->m_cnt, N, s->c_m_cnt))) return rc;
+  if ((rc = dev_alloc(s->m_off, N + 1, s->c_m_off))) return rc;
+  if ((rc = dev_alloc(s->m_cur, N, s->c_m_cur))) return rc;
+  if ((rc = dev_alloc(s->m_keys, nc * 4, s->c_m_keys))) return rc;
+  if ((rc = dev_alloc(s->m_keys2, nc * 4, s->c_m_keys2))) return rc;
+  if ((rc = dev_alloc(s->rres, 6 * N, s->c_rres))) return rc;
+  return dev_alloc(s->dref, 6 * N, s->c_dref);
+}
+
+unsigned solver_state_generation(const nrf_solver *s) { return (s ? s->gen : 0u) + g_realloc.load(); }
+
+template <typename T>
+static int upload_new(T *&p, const std::vector<T> &v) {
+  size_t cap = 0;
+  return upload(p, cap, v);
+}
+
+static const void *pcg_batch_fn(int sub) {
+  return sub == 1   ? reinterpret_cast<const void *>(pcg_pipe_batch_kernel<1>)
+         : sub == 2 ? reinterpret_cast<const void *>(pcg_pipe_batch_kernel<2>)
+                    : reinterpret_cast<const void *>(pcg_pipe_batch_kernel<4>);
+}
+
+int solver_batch_create(const SolverBatchInstance *inst, int count, int lanes_per_row, int wg_nodes,
+                        SolverBatch **out) {
+  if (!inst || !out || count < 1 || count > kMaxBatch) return NRF_ERR_INVALID_ARGUMENT;
+  if ((lanes_per_row != 1 && lanes_per_row != 2 && lanes_per_row != 4) || wg_nodes < 1)
+    return NRF_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  SolverBatch *b = new SolverBatch();
+  b->in.assign(inst, inst + count);
+  b->count = count;
+  b->sub = lanes_per_row;
+  int rc = NRF_OK;
+  std::vector<InstDev> h(count);
+  std::vector<int32_t> row_inst, row_local, wg_inst, wg_local;
+  for (int k = 0; k < count && !rc; ++k) {
+    const SolverBatchInstance &q = inst[k];
+    nrf_solver *s = q.solver;
+    if (!s || q.N <= 0 || !q.nodes || !q.T || !q.centered || !q.delta_x || q.max_corrs < 0) {
+      rc = NRF_ERR_INVALID_ARGUMENT;
+      break;
+    }
+    if (s->P.N != q.N) {
+      rc = NRF_ERR_NOT_READY;
+      break;
+    }
+    // one kernel form for all: the overlapped PCG, the MFMA assembly, shared
+    // precision and GN iteration count, no refinement step
+    const nrf_solver_params &sp = s->params;
+    const int ns = asm_strips(s->P.Rmax);
+    if (!sp.pcg_pipelined || sp.refine_iterations != 0 || !s->asm_mfma || ns > 8 ||
+        asm_lds_bytes(s->P.Rmax) > 64 * 1024 || (k > 0 && (sp.jtj_precision != b->prec ||
+                                                           std::max(1, q.gn.gn_iterations) != b->gn_iters))) {
+      rc = NRF_ERR_INVALID_ARGUMENT;
+      break;
+    }
+    if (k == 0) {
+      b->prec = sp.jtj_precision;
+      b->gn_iters = std::max(1, q.gn.gn_iterations);
+      b->coop = s->coop;
+    }
+    b->ns = std::max(b->ns, ns);
+    b->asm_lds = std::max(b->asm_lds, asm_lds_bytes(s->P.Rmax));
+    b->any_validate |= q.gn.validate ? 1 : 0;
+    b->max_N = std::max(b->max_N, q.N);
+    s->pcg_sub = lanes_per_row;
+    s->pcg_wg_nodes = wg_nodes;
+    if ((rc = solver_build_partition(s))) break;
+    b->pcg_lds = std::max(b->pcg_lds, s->pcg_lds);
+    // buffers sized for the largest frame, so a step allocates nothing
+    if ((rc = solver_reserve(s, q.max_corrs))) break;
+    InstDev &d = h[k];
+    d.as = solver_asm_args(s, nullptr, nullptr, q.nodes, reinterpret_cast<const DQ *>(q.T),
+                           reinterpret_cast<const DQ *>(q.centered), q.N);
+    d.as.crec = s->crec;
+    d.pcg = solver_pcg_args(s, s->b, q.delta_x, q.N, sp.pcg_iterations);
+    GnArgs &g = d.gn;
+    g.dx = q.delta_x;
+    g.T = reinterpret_cast<DQ *>(q.T);
+    g.centered = reinterpret_cast<DQ *>(q.centered);
+    g.nodes = q.nodes;
+    g.N = q.N;
+    g.scale = q.gn.update_scale;
+    g.max_rot = q.gn.max_update_rot;
+    g.max_trans = q.gn.max_update_trans;
+    g.max_dx_mean = q.gn.max_dx_mean;
+    g.validate = q.gn.validate;
+    g.stop = s->stop;
+    g.res = s->d_res;
+    d.m_cnt = s->m_cnt;
+    d.m_off = s->m_off;
+    d.m_cur = s->m_cur;
+    d.m_keys = s->m_keys;
+    d.tot = s->d_tot;
+    for (int i = 0; i < q.N; ++i) {
+      row_inst.push_back(k);
+      row_local.push_back(i);
+    }
+    for (int g2 = 0; g2 < s->G; ++g2) {
+      wg_inst.push_back(k);
+      wg_local.push_back(g2);
+    }
+  }
+  if (!rc && b->pcg_lds > 65536 &&
+      hipFuncSetAttribute(pcg_batch_fn(b->sub), hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->pcg_lds) !=
+          hipSuccess) {
+    (void)hipGetLastError();
+    rc = NRF_ERR_TOO_LARGE;
+  }
+  if (!rc) {
+    // launch groups: consecutive instances whose workgroups are co-resident
+    int per_cu = 0;
+    NRF_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pcg_batch_fn(b->sub), kPcgThreads,
+                                                               b->pcg_lds));
+    const int cap = per_cu * device_cu_count();
+    int first = 0, used = 0, wg = 0;
+    for (int k = 0; k < count && !rc; ++k) {
+      const int G = b->in[k].solver->G;
+      if (G > cap) {
+        rc = NRF_ERR_TOO_LARGE;
+        break;
+      }
+      if (used + G > cap) {
+        b->groups.push_back({first, wg});
+        first = wg;
+        used = 0;
+      }
+      used += G;
+      wg += G;
+    }
+    if (!rc) b->groups.push_back({first, wg});
+    b->wgs = wg;
+  }
+  b->rows = (int)row_inst.size();
+  if (!rc) rc = upload_new(b->d_inst, h);
+  if (!rc) rc = upload_new(b->row_inst, row_inst);
+  if (!rc) rc = upload_new(b->row_local, row_local);
+  if (!rc) rc = upload_new(b->wg_inst, wg_inst);
+  if (!rc) rc = upload_new(b->wg_local, wg_local);
+  if (!rc) {
+    size_t c1 = 0, c2 = 0;
+    if ((rc = dev_alloc(b->save_res, (size_t)count, c1)) == 0) rc = dev_alloc(b->save_tot, (size_t)count, c2);
+  }
+  if (rc) {
+    solver_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return NRF_OK;
+}
+
+int solver_batch_launch_groups(const SolverBatch *b) { return b ? (int)b->groups.size() : 0; }
+int solver_batch_workgroups(const SolverBatch *b) { return b ? b->wgs : 0; }
+
+static int batch_memberships(SolverBatch *b, const BatchCorrs &fr, hipStream_t st) {
+  const dim3 gN((b->max_N + 255) / 256, b->count), gC((std::max(1, fr.max_n) + 255) / 256, b->count);
+  mem_fill_batch_kernel<<<gN, 256, 0, st>>>(b->d_inst);
+  NRF_LAUNCH_CHECK();
+  if (fr.max_n > 0) {
+    mem_count_batch_kernel<<<gC, 256, 0, st>>>(b->d_inst, fr);
+    NRF_LAUNCH_CHECK();
+  }
+  exclusive_scan_batch_kernel<<<b->count, 1024, 0, st>>>(b->d_inst);
+  NRF_LAUNCH_CHECK();
+  if (fr.max_n > 0) {
+    mem_scatter_batch_kernel<<<gC, 256, 0, st>>>(b->d_inst, fr);
+    NRF_LAUNCH_CHECK();
+  }
+  mem_sort_batch_kernel<<<b->rows, kSortThreads, 0, st>>>(b->d_inst, b->row_inst, b->row_local);
+  NRF_LAUNCH_CHECK();
+  gn_begin_batch_kernel<<<1, 64, 0, st>>>(b->d_inst, b->count);
+  NRF_LAUNCH_CHECK();
+  return NRF_OK;
+}
+
+static int batch_assemble(SolverBatch *b, const BatchCorrs &fr, hipStream_t st) {
+  if (fr.max_n > 0) {
+    corr_prep_batch_kernel<<<dim3((fr.max_n + 255) / 256, b->count), 256, 0, st>>>(b->d_inst, fr);
+    NRF_LAUNCH_CHECK();
+  }
+  const size_t lds = b->asm_lds;
+#define NRF_ASMB_CASE(NSV)                                                                                    \
+  case NSV:                                                                                                   \
+    if (b->prec == 1)                                                                                         \
+      assemble_rows_batch_kernel<NSV, 1><<<b->rows, 64, lds, st>>>(b->d_inst, fr, b->row_inst, b->row_local); \
+    else if (b->prec == 2)                                                                                    \
+      assemble_rows_batch_kernel<NSV, 2><<<b->rows, 64, lds, st>>>(b->d_inst, fr, b->row_inst, b->row_local); \
+    else                                                                                                      \
+      assemble_rows_batch_kernel<NSV, 0><<<b->rows, 64, lds, st>>>(b->d_inst, fr, b->row_inst, b->row_local); \
+    break;
+  switch (b->ns) {
+    NRF_ASMB_CASE(1)
+    NRF_ASMB_CASE(2)
+    NRF_ASMB_CASE(3)
+    NRF_ASMB_CASE(4)
+    NRF_ASMB_CASE(5)
+    NRF_ASMB_CASE(6)
+    NRF_ASMB_CASE(7)
+    NRF_ASMB_CASE(8)
+    default: return NRF_ERR_INVALID_ARGUMENT;
+  }
+#undef NRF_ASMB_CASE
+  NRF_LAUNCH_CHECK();
+  return NRF_OK;
+}
+
+int solver_batch_pcg(SolverBatch *b, hipStream_t st) {
+  pcg_prep_batch_kernel<<<1, 64, 0, st>>>(b->d_inst, b->count);
+  NRF_LAUNCH_CHECK();
+  const void *fn = pcg_batch_fn(b->sub);
+  const bool coop = b->coop && !stream_capturing(st);
+  for (const auto &g : b->groups) {
+    const InstDev *inst = b->d_inst;
+    const int32_t *wi = b->wg_inst + g.first, *wl = b->wg_local + g.first;
+    void *args[] = {&inst, &wi, &wl};
+    const dim3 grid(g.second - g.first);
+    if (coop) {
+      NRF_HIP_CHECK(hipLaunchCooperativeKernel(fn, grid, dim3(kPcgThreads), args, (unsigned)b->pcg_lds, st));
+    } else {
+      NRF_HIP_CHECK(hipLaunchKernel(fn, grid, dim3(kPcgThreads), args, (unsigned)b->pcg_lds, st));
+    }
+  }
+  return NRF_OK;
+}
+
+int solver_batch_solve_gn(SolverBatch *b, const BatchCorrs &fr, hipStream_t st) {
+  if (!b) return NRF_ERR_INVALID_ARGUMENT;
+  int rc;
+  if ((rc = batch_memberships(b, fr, st))) return rc;
+  const dim3 gN((b->max_N + 255) / 256, b->count);
+  for (int it = 0; it < b->gn_iters; ++it) {
+    if ((rc = batch_assemble(b, fr, st))) return rc;
+    if ((rc = solver_batch_pcg(b, st))) return rc;
+    if (b->any_validate) {
+      gn_validate_batch_kernel<<<b->count, 1024, 0, st>>>(b->d_inst);
+      NRF_LAUNCH_CHECK();
+    }
+    gn_update_batch_kernel<<<gN, 256, 0, st>>>(b->d_inst);
+    NRF_LAUNCH_CHECK();
+  }
+  return NRF_OK;
+}
+
+int solver_batch_prepare(SolverBatch *b, const BatchCorrs &fr, hipStream_t st) {
+  if (!b) return NRF_ERR_INVALID_ARGUMENT;
+  for (int k = 0; k < b->count; ++k) {
+    const nrf_solver *s = b->in[k].solver;
+    NRF_HIP_CHECK(hipMemcpyAsync(b->save_res + k, s->d_res, sizeof(nrf_pcg_result), hipMemcpyDeviceToDevice, st));
+    NRF_HIP_CHECK(
+        hipMemcpyAsync(b->save_tot + k, s->d_tot, sizeof(nrf_solver_totals), hipMemcpyDeviceToDevice, st));
+  }
+  int rc;
+  if ((rc = batch_memberships(b, fr, st))) return rc;
+  return batch_assemble(b, fr, st);
+}
+
+int solver_batch_restore(SolverBatch *b, hipStream_t st) {
+  if (!b) return NRF_ERR_INVALID_ARGUMENT;
+  for (int k = 0; k < b->count; ++k) {
+    nrf_solver *s = b->in[k].solver;
+    NRF_HIP_CHECK(hipMemcpyAsync(s->d_res, b->save_res + k, sizeof(nrf_pcg_result), hipMemcpyDeviceToDevice, st));
+    NRF_HIP_CHECK(
+        hipMemcpyAsync(s->d_tot, b->save_tot + k, sizeof(nrf_solver_totals), hipMemcpyDeviceToDevice, st));
+  }
+  return NRF_OK;
+}
+
+}  // namespace nrf

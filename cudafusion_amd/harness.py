@@ -63,9 +63,10 @@ class Harness:
     def __init__(self, kind: str = "body", frames: int = 40, use_graph: bool = False, seed: int = 0,
                  pcg_tolerance: float = 0.0, relative_tolerance: bool = False, precond_fp64: bool = False,
                  arap_consistent_sign: bool = False, validate: bool = True, pcg_pipelined: bool = False,
-                 jtj_precision: int = 0, refine_iterations: int = 0, storage16: bool = False):
+                 jtj_precision: int = 0, refine_iterations: int = 0, storage16: bool = False,
+                 workload: S.Workload | None = None):
         self.kind = kind
-        self.scene = S.make_scene(kind, frames, seed)
+        self.scene = S.scene_for(workload) if workload is not None else S.make_scene(kind, frames, seed)
         self.w = self.scene.w
         w = self.w
         self.frames = frames
@@ -83,7 +84,11 @@ class Harness:
         self.pipe = None
 
     # ------------------------------------------------------------ setup
-    def setup(self, max_triangles: int | None = None) -> None:
+    def setup(self, max_triangles: int | None = None, host: "Harness | None" = None) -> None:
+        """Creates the pipeline, integrates frame 0, samples the nodes and
+        packs every frame's correspondences on the device.  ``host``: take
+        the rendered frames and the nodes from another harness of the same
+        workload instead of computing them again."""
         lib = L.load()
         dev = torch.device("cuda", torch.cuda.current_device())
         self.dev = dev
@@ -105,13 +110,14 @@ class Harness:
         if not self.pipe:
             raise L.NrfError("nrf_pipeline_create failed: " + lib.nrf_last_error().decode())
         self.lib = lib
-        self.frames_host = [self.scene.render(f) for f in range(self.frames + 1)]
+        self.frames_host = (host.frames_host if host is not None
+                            else [self.scene.render(f) for f in range(self.frames + 1)])
         self.depth = [_t(fr.depth.reshape(-1), dev) for fr in self.frames_host]
         stream = torch.cuda.current_stream().cuda_stream
         L.check(lib.nrf_pipeline_integrate_rigid(self.pipe, self.depth[0].data_ptr(), self.pose.ctypes.data,
                                                  stream), "integrate_rigid")
         torch.cuda.synchronize()
-        self.nodes = S.scene_nodes(self.scene)
+        self.nodes = host.nodes if host is not None else S.scene_nodes(self.scene)
         self.N = len(self.nodes)
         L.check(lib.nrf_pipeline_set_nodes(self.pipe, self.nodes.ctypes.data, self.N), "set_nodes")
         # association inputs for frames 1..F, resident on the device
@@ -215,6 +221,68 @@ class Harness:
         if self.pipe:
             self.lib.nrf_pipeline_destroy(self.pipe)
             self.pipe = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchHarness:
+    """Several harnesses (already set up) stepped together through the
+    nrf_batch_* entry points."""
+
+    def __init__(self, harnesses: list, use_graph: bool = False, tuning: L.BatchTuning | None = None):
+        self.h = list(harnesses)
+        self.lib = L.load()
+        arr = (C.c_void_p * len(self.h))(*[x.pipe for x in self.h])
+        self.batch = self.lib.nrf_batch_create(arr, len(self.h), C.byref(tuning) if tuning is not None else None,
+                                               int(use_graph))
+        if not self.batch:
+            raise L.NrfError("nrf_batch_create failed: " + self.lib.nrf_last_error().decode())
+        self._frames = {}
+
+    def frames(self, f: int):
+        """The BatchFrame array of frame f (kept alive, same address per f)."""
+        if f not in self._frames:
+            arr = (L.BatchFrame * len(self.h))()
+            for k, x in enumerate(self.h):
+                arr[k].depth = x.depth[f].data_ptr()
+                arr[k].camera_pose[:] = [float(v) for v in x.pose.reshape(-1)]
+                arr[k].corrs = x.corrs[f].data_ptr()
+                arr[k].canonical = x.canonical[f].data_ptr()
+                arr[k].num_corrs = x.w.width * x.w.height
+            self._frames[f] = arr
+        return self._frames[f]
+
+    def step(self, f: int) -> None:
+        L.check(self.lib.nrf_batch_step(self.batch, self.frames(f), torch.cuda.current_stream().cuda_stream),
+                "batch_step")
+
+    def autotune(self, f: int, use_cache: bool = True) -> L.BatchTuning:
+        out = L.BatchTuning()
+        L.check(self.lib.nrf_batch_autotune(self.batch, self.frames(f), int(use_cache), C.byref(out),
+                                            torch.cuda.current_stream().cuda_stream), "batch_autotune")
+        return out
+
+    def tuning(self) -> L.BatchTuning:
+        out = L.BatchTuning()
+        L.check(self.lib.nrf_batch_get_tuning(self.batch, C.byref(out)), "batch_get_tuning")
+        return out
+
+    def set_tuning(self, t: L.BatchTuning) -> None:
+        L.check(self.lib.nrf_batch_set_tuning(self.batch, C.byref(t)), "batch_set_tuning")
+
+    def stage_ms(self):
+        out = (C.c_float * 3)()
+        L.check(self.lib.nrf_batch_stage_ms(self.batch, out), "batch_stage_ms")
+        return list(out)
+
+    def close(self) -> None:
+        if self.batch:
+            self.lib.nrf_batch_destroy(self.batch)
+            self.batch = None
 
     def __del__(self):
         try:

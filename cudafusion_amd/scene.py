@@ -78,6 +78,43 @@ def body_workload(frames: int = 40, seed: int = 0) -> Workload:
                     480, 525.0, 525.0, 319.5, 239.5, frames, 0.020, 5, 30, 10.0, seed=seed)
 
 
+def body_instance_workload(nodes: int, dims: int, voxel_size: float, width: int, frames: int = 40,
+                           seed: int = 0, gn_iterations: int = 5, pcg_iterations: int = 30) -> Workload:
+    """A capsule-body sequence of another size: node spacing chosen for about
+    ``nodes`` nodes (2382 at 20 mm, count ~ spacing^-2), a dims^3 volume of
+    ``voxel_size`` centred on the body (truncation 5 voxels), and a 640x480 or
+    320x240 camera with the intrinsics of the full-scale workload scaled."""
+    ext = dims * voxel_size
+    sc = width / 640.0
+    spacing = 0.020 * math.sqrt(2382.0 / nodes)
+    return Workload(f"capsule-body-{dims}^3-{width}x{int(480 * sc)}-n{nodes}", (dims, dims, dims), voxel_size,
+                    5 * voxel_size, (-ext / 2, -ext / 2, 1.21 - ext / 2), width, int(480 * sc), 525.0 * sc,
+                    525.0 * sc, 319.5 * sc + 0.5 * (sc - 1.0), 239.5 * sc + 0.5 * (sc - 1.0), frames, spacing,
+                    gn_iterations, pcg_iterations, 10.0, seed=seed)
+
+
+# (dims, voxel size) pairs of the batched form: 128^3 to 320^3 at 4-8 mm,
+# each about 1.0-1.3 m across so that the body fits.
+BATCH_VOLUMES = ((128, 0.008), (160, 0.0065), (192, 0.0055), (224, 0.005), (256, 0.004), (288, 0.004),
+                 (320, 0.004))
+
+
+def batch_workloads(count: int = 16, frames: int = 40, seed: int = 0, nodes_lo: int = 500, nodes_hi: int = 8000,
+                    gn_iterations: int = 5, pcg_iterations: int = 30) -> list:
+    """Uneven capsule-body sequences for the batched form: node counts
+    log-uniform in [nodes_lo, nodes_hi], volumes from BATCH_VOLUMES, 640x480
+    or 320x240 depth; instance k's sequence (joint walk, noise) uses seed k."""
+    rng = np.random.default_rng(seed + 4242)
+    out = []
+    for k in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(nodes_lo), math.log(nodes_hi)))))
+        dims, vs = BATCH_VOLUMES[int(rng.integers(len(BATCH_VOLUMES)))]
+        width = 640 if rng.uniform() < 0.5 else 320
+        out.append(body_instance_workload(n, dims, vs, width, frames, seed=k, gn_iterations=gn_iterations,
+                                          pcg_iterations=pcg_iterations))
+    return out
+
+
 # ------------------------------------------------------------- primitives
 
 def pixel_rays(w: Workload) -> np.ndarray:
@@ -424,6 +461,11 @@ def sample_nodes(points: np.ndarray, normals: np.ndarray, spacing: float, radius
         nodes["neighbor_w"][i, :k] = np.exp(-(dist * dist) / (2 * r * r))
         nodes["num_neighbors"][i] = k
     return nodes
+
+
+def scene_for(w: Workload):
+    """The scene of a workload: capsule body for "capsule-body*", else sphere."""
+    return BodyScene(w) if w.name.startswith("capsule-body") else SphereScene(w)
 
 
 def make_scene(kind: str, frames: int | None = None, seed: int = 0):

@@ -143,6 +143,10 @@ typedef struct nrf_pcg_result {
   int32_t gn_done;         /* GN iterations whose step was applied */
   double final_residual;   /* ||r||_2 of the PCG recurrence */
   double rhs_norm;         /* ||b||_2 */
+  int32_t converged;       /* PCG runs of this solve that met the tolerance */
+  int32_t breakdowns;      /* PCG runs of this solve that stopped on |p.Ap| < 1e-12 */
+  int32_t runs;            /* PCG runs of this solve */
+  int32_t pad;
 } nrf_pcg_result;
 
 /* ----------------------------------------------------- TSDF integration */
@@ -298,6 +302,12 @@ int nrf_solver_system(const nrf_solver *s, const int32_t **row_ptr,
    stream-ordered readers that do not want a host sync). */
 const nrf_pcg_result *nrf_solver_device_result(const nrf_solver *s);
 
+/* Layout of the persistent PCG (no reference counterpart): lanes per scalar
+   row (1, 2 or 4) and the target number of nodes per workgroup (capped at
+   1024 / (6 lanes_per_row)); rebuilds the partition of the current pattern.
+   The textbook PCG form (pcg_pipelined = 0) needs 4 lanes per row. */
+int nrf_solver_set_pcg_layout(nrf_solver *s, int lanes_per_row, int wg_nodes);
+
 /* PCG alone on the last assembled system (A, b, block-Jacobi inverse), with
    the solver's current parameters: for timing the solve and its stopping
    rules in isolation (no reference counterpart).  Stream-ordered; fills
@@ -319,6 +329,9 @@ typedef struct nrf_solver_totals {
   int64_t gn_done;         /* GN steps applied */
   int64_t pcg_iterations;  /* PCG iterations run */
   int64_t rejected;        /* solves that ended on a rejected step */
+  int64_t pcg_runs;        /* PCG runs (one per GN iteration that reached the PCG) */
+  int64_t pcg_converged;   /* PCG runs that met the tolerance */
+  int64_t pcg_breakdowns;  /* PCG runs that stopped on |p.Ap| < 1e-12 */
 } nrf_solver_totals;
 int nrf_solver_get_totals(nrf_solver *s, nrf_solver_totals *out, void *stream);
 
@@ -369,7 +382,8 @@ int nrf_pipeline_step(nrf_pipeline *p, const float *depth,
                       const float camera_pose[16], const nrf_corr *corrs,
                       const float *canonical, int num_corrs, void *stream);
 /* Per-stage device times of the last step in ms (hipEvent based; valid after
-   the stream has been synchronised): [solve, integrate, mc]. */
+   the stream has been synchronised): [solve, integrate, mc].  After a graph
+   replay (use_graph) out_ms[0] is the whole step and the others are -1. */
 int nrf_pipeline_stage_ms(nrf_pipeline *p, float out_ms[3]);
 /* Per-kernel profiling: with ring_steps > 0, every step records hipEvents
    around each kernel group on the step's stream (eager launches, no graph)
@@ -381,6 +395,62 @@ int nrf_pipeline_profile(nrf_pipeline *p, int ring_steps);
 int nrf_pipeline_profile_read(nrf_pipeline *p, int step_back, float out_ms[6]);
 /* Voxels written by the warped integrations of all steps so far (syncs). */
 long long nrf_pipeline_updated_voxels(nrf_pipeline *p);
+
+/* ---------------------------------------------------------------- batch
+   Build-only form (no counterpart in the reference): up to NRF_MAX_BATCH
+   independent pipelines (their own volume, camera, nodes and solver
+   settings) stepped together, so that small instances share the device
+   instead of each leaving most of it idle.  Per frame, the membership
+   build, the correspondence records, the assembly, the PCG, the GN update
+   and the warped integration each run as one launch over all instances
+   (the persistent PCG as one launch per group of instances whose
+   workgroups fit on the device together); marching cubes runs per
+   instance.  A batch step of an instance gives the same bits as that
+   pipeline's own nrf_pipeline_step with the same PCG layout.
+   Requirements: every pipeline has its nodes set, the FP32 storage form,
+   the overlapped PCG (pcg_pipelined = 1), no refinement step, and all share
+   jtj_precision and gn_iterations.  Changing a member's nodes or PCG layout
+   afterwards invalidates the batch (destroy and create it again). */
+#define NRF_MAX_BATCH 32
+typedef struct nrf_batch nrf_batch;
+typedef struct nrf_batch_frame {
+  const float *depth;       /* device, that instance's camera size */
+  float camera_pose[16];    /* host values, row-major camera-to-world */
+  const nrf_corr *corrs;    /* device, num_corrs records */
+  const float *canonical;   /* device, 3 floats per corr, or NULL (stored src) */
+  int32_t num_corrs;        /* <= that pipeline's max_corrs */
+  int32_t pad;
+} nrf_batch_frame;
+typedef struct nrf_batch_tuning {
+  int32_t lanes_per_row;    /* PCG lanes per scalar row: 1, 2 or 4 */
+  int32_t wg_nodes;         /* PCG target nodes per workgroup */
+  int32_t z_chunk;          /* integration voxels per lane along z: 1, 2, 4, 8 */
+  int32_t launch_groups;    /* out: persistent PCG launches per iteration */
+  float pcg_ms;             /* out (nrf_batch_autotune): PCG stage time, all instances */
+  float integrate_ms;       /* out (nrf_batch_autotune): integration time, all instances */
+} nrf_batch_tuning;
+/* tuning NULL: 4 lanes per row, 40 nodes per workgroup, z_chunk 4.
+   use_graph: replay each step as a captured hipGraph (re-captured when the
+   frame pointers, counts or poses change). */
+nrf_batch *nrf_batch_create(nrf_pipeline *const *pipelines, int count, const nrf_batch_tuning *tuning,
+                            int use_graph);
+void nrf_batch_destroy(nrf_batch *b);
+int nrf_batch_set_tuning(nrf_batch *b, const nrf_batch_tuning *tuning);
+int nrf_batch_get_tuning(const nrf_batch *b, nrf_batch_tuning *tuning);
+/* Times every candidate PCG layout on the systems of `frames` at the
+   current transforms, and every integration z chunk on scratch copies of
+   the volumes, then applies the fastest.  Leaves the instances' state
+   (transforms, volumes, solver totals) as it was.  use_cache: reuse the
+   choice made earlier in this process for the same shapes (node counts,
+   row lengths, volume and image sizes).  Synchronous. */
+int nrf_batch_autotune(nrf_batch *b, const nrf_batch_frame *frames, int use_cache, nrf_batch_tuning *out,
+                       void *stream);
+/* One frame of every instance (frames[k] for pipeline k). */
+int nrf_batch_step(nrf_batch *b, const nrf_batch_frame *frames, void *stream);
+/* Device times of the last step in ms (after a synchronisation):
+   [solve, integrate, mc], all instances; after a graph replay out_ms[0] is
+   the whole step and the others are -1. */
+int nrf_batch_stage_ms(nrf_batch *b, float out_ms[3]);
 
 #ifdef __cplusplus
 }

@@ -44,3 +44,35 @@ def test_status_strings_without_device():
     vp = _lib.volume_params((0, 4, 4), 0.01, 0.03, (0, 0, 0))
     assert lib.nrf_tsdf_reset(C.byref(vp), C.c_void_p(16), None) == 1
     assert lib.nrf_mc_workspace_size(C.byref(vp)) == 0
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every ctypes record of the binding has the size and field offsets the
+    C compiler gives the header's struct."""
+    import shutil
+    import subprocess
+
+    from cudafusion_amd import _lib as L
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    pairs = {"nrf_volume_params": L.VolumeParams, "nrf_camera": L.Camera, "nrf_solver_params": L.SolverParams,
+             "nrf_gn_params": L.GNParams, "nrf_pcg_result": L.PcgResult, "nrf_solver_totals": L.SolverTotals,
+             "nrf_pipeline_config": L.PipelineConfig, "nrf_batch_frame": L.BatchFrame,
+             "nrf_batch_tuning": L.BatchTuning}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "nrf.h"', "int main(void) {"]
+    for cname, py in pairs.items():
+        lines.append(f'  printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                               text=True).stdout.splitlines())
+    for cname, py in pairs.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, f"{cname}.{f}"

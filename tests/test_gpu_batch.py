@@ -1,0 +1,161 @@
+"""Batched form: several independent pipelines of uneven size stepped by
+common launches (nrf_batch_*).  A batched step of an instance must give the
+same bits as that pipeline's own step with the same PCG layout: node
+transforms, solver record, TSDF volume and mesh, over several frames, with
+eager launches and with the step captured in a hipGraph; the autotuner must
+leave the instances' state untouched."""
+import ctypes as C
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _workloads():
+    from cudafusion_amd import scene as S
+    base = S.sphere_workload(frames=3)
+    return [
+        (replace(base, seed=0), dict(validate=True)),
+        (replace(base, dims=(96, 96, 96), voxel_size=0.008, truncation=0.024, node_spacing=0.020, seed=1),
+         dict(validate=False, arap_consistent_sign=True)),
+        (replace(base, dims=(112, 112, 112), voxel_size=0.007, truncation=0.021, origin=(-0.392, -0.392, 0.608),
+                 node_spacing=0.032, seed=2), dict(validate=False, arap_consistent_sign=True)),
+    ]
+
+
+def _sets():
+    """Two harness sets over the same workloads (the second reuses the
+    first's rendered frames and nodes)."""
+    from cudafusion_amd.harness import Harness
+    a, b = [], []
+    for w, kw in _workloads():
+        ha = Harness(workload=w, frames=3, pcg_pipelined=True, pcg_tolerance=1e-6, relative_tolerance=True, **kw)
+        ha.setup()
+        hb = Harness(workload=w, frames=3, pcg_pipelined=True, pcg_tolerance=1e-6, relative_tolerance=True, **kw)
+        hb.setup(host=ha)
+        a.append(ha)
+        b.append(hb)
+    return a, b
+
+
+def _state(h):
+    r = h.solver_result()
+    return (h.download_transforms().copy(), h.download_voxels().copy(), h.mesh().copy(),
+            (r.iterations, r.total_iterations, r.flags, r.gn_done))
+
+
+def _compare(a, b, tag):
+    for k, (ha, hb) in enumerate(zip(a, b)):
+        Ta, va, ma, ra = _state(ha)
+        Tb, vb, mb, rb = _state(hb)
+        print(f"[batch {tag}] instance {k}: N={ha.N} solver {ra} vs {rb}, triangles {len(ma) // 3} vs "
+              f"{len(mb) // 3}", flush=True)
+        assert ra == rb
+        assert Ta.tobytes() == Tb.tobytes()
+        assert va.tobytes() == vb.tobytes()
+        assert ma.tobytes() == mb.tobytes() and len(ma) > 300
+
+
+def _set_layout(hs, t):
+    from cudafusion_amd import _lib as L
+    for h in hs:
+        L.check(h.lib.nrf_solver_set_pcg_layout(h.lib.nrf_pipeline_solver(h.pipe), t.lanes_per_row, t.wg_nodes),
+                "set_pcg_layout")
+
+
+def test_batch_matches_unbatched_eager(gpu):
+    import torch
+
+    from cudafusion_amd.harness import BatchHarness
+    a, b = _sets()
+    bh = BatchHarness(b)
+    t = bh.tuning()
+    assert (t.lanes_per_row, t.wg_nodes, t.z_chunk, t.launch_groups) == (4, 40, 4, 1)
+    _set_layout(a, t)
+    for f in (1, 2, 3):
+        for h in a:
+            h.step(f)
+        bh.step(f)
+        torch.cuda.synchronize()
+        _compare(a, b, f"eager f{f}")
+    bh.close()
+    for h in a + b:
+        h.close()
+
+
+def test_batch_autotune_and_graph(gpu):
+    import torch
+
+    from cudafusion_amd.harness import BatchHarness
+    a, b = _sets()
+    bh = BatchHarness(b, use_graph=True)
+    before = [_state(h) for h in b]
+    tot_before = [(t.solves, t.gn_done, t.pcg_iterations) for t in (h.solver_totals() for h in b)]
+    t = bh.autotune(1, use_cache=False)
+    print(f"[batch tune] lanes {t.lanes_per_row} nodes/wg {t.wg_nodes} z_chunk {t.z_chunk} groups "
+          f"{t.launch_groups} pcg {t.pcg_ms:.3f} ms integrate {t.integrate_ms:.3f} ms", flush=True)
+    assert t.lanes_per_row in (1, 2, 4) and t.z_chunk in (1, 2, 4, 8) and t.pcg_ms > 0 and t.integrate_ms > 0
+    after = [_state(h) for h in b]
+    for x, y in zip(before, after):  # the tuner leaves transforms, volume, mesh and record as they were
+        for u, v in zip(x, y):
+            assert (u.tobytes() == v.tobytes()) if isinstance(u, np.ndarray) else u == v
+    assert tot_before == [(t2.solves, t2.gn_done, t2.pcg_iterations) for t2 in (h.solver_totals() for h in b)]
+    t2 = bh.autotune(1, use_cache=True)  # same shapes: the cached choice
+    assert (t2.lanes_per_row, t2.wg_nodes, t2.z_chunk) == (t.lanes_per_row, t.wg_nodes, t.z_chunk)
+    _set_layout(a, t)
+    for i, f in enumerate((1, 1, 2, 3)):  # the second step replays the first one's graph
+        for h in a:
+            h.step(f)
+        bh.step(f)
+        torch.cuda.synchronize()
+        _compare(a, b, f"graph step {i} f{f}")
+    bh.close()
+    for h in a + b:
+        h.close()
+
+
+def test_batch_rejects_mismatched_instances(gpu):
+    """The batch needs one kernel form for all: different GN iteration
+    counts (or the textbook PCG) are refused at creation."""
+    from cudafusion_amd import _lib as L
+    from cudafusion_amd import scene as S
+    from cudafusion_amd.harness import Harness
+    w = S.sphere_workload(frames=1)
+    h1 = Harness(workload=w, frames=1, pcg_pipelined=True)
+    h1.setup()
+    h2 = Harness(workload=replace(w, gn_iterations=2), frames=1, pcg_pipelined=True)
+    h2.setup(host=h1)
+    h3 = Harness(workload=w, frames=1, pcg_pipelined=False)
+    h3.setup(host=h1)
+    lib = L.load()
+    for pair in ((h1, h2), (h1, h3), (h1, h1)):
+        arr = (C.c_void_p * 2)(*[x.pipe for x in pair])
+        assert not lib.nrf_batch_create(arr, 2, None, 0)
+    for h in (h1, h2, h3):
+        h.close()
+
+
+def test_pipeline_graph_replay_matches_eager(gpu):
+    """One pipeline with use_graph: captured on its own stream, replayed on
+    the caller's; repeated frames replay the graph, new inputs re-capture it.
+    Every replay equals the eager step bit for bit."""
+    import torch
+
+    from cudafusion_amd import scene as S
+    from cudafusion_amd.harness import Harness
+    w = S.sphere_workload(frames=3)
+    a = Harness(workload=w, frames=3, pcg_pipelined=True)
+    a.setup()
+    b = Harness(workload=w, frames=3, pcg_pipelined=True, use_graph=True)
+    b.setup(host=a)
+    for i, f in enumerate((1, 1, 2, 2, 3)):
+        a.step(f)
+        b.step(f)
+        torch.cuda.synchronize()
+        _compare([a], [b], f"pipeline graph step {i} f{f}")
+        ms = b.stage_ms()
+        assert ms[0] > 0 and ms[1] == -1
+    a.close()
+    b.close()
