@@ -52,6 +52,13 @@ _SIGS = {
     "orc_spmv_f64": (None, [P, P, P, P, P, I]),
     "orc_solve_gn_f32": (I, [P, P, I, P, I, P, P, P, P, P, P, P]),
     "orc_solve_gn_f64": (I, [P, P, I, P, I, P, P, P, P, P, P, P]),
+    "orc_lp_param_count": (I, []),
+    "orc_lp_offsets": (I, [P, I]),
+    "orc_lp_features_f64": (None, [P, I, P, P, P, P]),
+    "orc_lp_forward_f64": (None, [P, I, P, I, P, P, P]),
+    "orc_lp_apply_f64": (None, [I, P, I, P, P, P, P, P, I, P]),
+    "orc_lp_loss_grad_f64": (D, [P, I, P, I, P, P, P, P, P, P, P, I, P]),
+    "orc_lp_pcg_f64": (I, [I, P, I, P, P, P, P, P, P, P, P, I, D, P, P]),
 }
 
 _lib = None
@@ -267,3 +274,90 @@ def solve_gn_f64(corrs, canonical, nodes, T64, gn, sp):
     load().orc_solve_gn_f64(ptr(corrs), ptr(canonical), len(corrs), ptr(nodes), N, ptr(T64), C.byref(gn),
                             C.byref(sp), ptr(dx), C.byref(tot), C.byref(done), C.byref(last))
     return T64, dx, tot.value, done.value, last.value
+
+
+# ------------------------------------------------- learned preconditioner
+LP_NAMES = ["W0", "b0", "Wm0", "bm0", "Wu0", "bu0", "Wm1", "bm1", "Wu1", "bu1", "Wm2", "bm2", "Wu2", "bu2", "Wo", "bo",
+            "We", "be"]
+
+
+def lp_offsets() -> dict:
+    """Parameter name -> (offset, size) of the flat parameter vector."""
+    out = np.zeros(32, np.int32)
+    n = load().orc_lp_offsets(ptr(out), 32)
+    offs = [int(v) for v in out[:n]]
+    return {name: (offs[k], offs[k + 1] - offs[k]) for k, name in enumerate(LP_NAMES)}
+
+
+def lp_param_count() -> int:
+    return int(load().orc_lp_param_count())
+
+
+def lp_diag_blocks(rp, ci, A):
+    """The diagonal 6x6 block of every block row of a BSR system."""
+    N = len(rp) - 1
+    out = np.zeros((N, 36))
+    for i in range(N):
+        for q in range(rp[i], rp[i + 1]):
+            if ci[q] == i:
+                out[i] = np.asarray(A[q], np.float64).reshape(36)
+    return out
+
+
+def lp_features_f64(nodes, Adiag, Minv):
+    N = len(nodes)
+    x = np.zeros((N, 28))
+    Cm = np.zeros((N, 36))
+    Adiag = np.ascontiguousarray(Adiag, np.float64).reshape(N, 36)
+    Minv = np.ascontiguousarray(Minv, np.float64).reshape(N, 36)
+    load().orc_lp_features_f64(ptr(nodes), N, ptr(Adiag), ptr(Minv), ptr(x), ptr(Cm))
+    return x, Cm
+
+
+def lp_forward_f64(params, variant, nodes, x):
+    N = len(nodes)
+    Td = np.zeros((N, 21))
+    Te = np.zeros((N, 8, 36))
+    params = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    load().orc_lp_forward_f64(ptr(params), variant, ptr(nodes), N, ptr(x), ptr(Td), ptr(Te))
+    return Td, Te
+
+
+def lp_loss_grad_f64(params, variant, nodes, x, Cm, Minv, rp, ci, A, V, grad=True):
+    N = len(nodes)
+    params = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    Cm = np.ascontiguousarray(Cm, np.float64)
+    Minv = np.ascontiguousarray(Minv, np.float64).reshape(N, 36)
+    A = np.ascontiguousarray(A, np.float64)
+    V = np.ascontiguousarray(V, np.float64)
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    g = np.zeros(len(params)) if grad else None
+    loss = load().orc_lp_loss_grad_f64(ptr(params), variant, ptr(nodes), N, ptr(x), ptr(Cm), ptr(Minv), ptr(rp),
+                                       ptr(ci), ptr(A), ptr(V), V.shape[0], ptr(g))
+    return loss, g
+
+
+def lp_apply_f64(variant, nodes, Cm, Minv, Td, Te, R):
+    N = len(nodes)
+    R = np.ascontiguousarray(R, np.float64).reshape(-1, 6 * N)
+    Z = np.zeros_like(R)
+    args = [np.ascontiguousarray(a, np.float64) for a in (Cm, Minv, Td, Te)]
+    load().orc_lp_apply_f64(variant, ptr(nodes), N, *[ptr(a) for a in args], ptr(R), R.shape[0], ptr(Z))
+    return Z
+
+
+def lp_pcg_f64(variant, nodes, rp, ci, A, b, Cm, Minv, Td, Te, maxit, tol):
+    """FP64 PCG with M^-1 = BJ^-1 + L L^T (relative tolerance); returns
+    (iterations, true relative residual, x)."""
+    N = len(nodes)
+    x = np.zeros(6 * N)
+    tr = C.c_double()
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (A, b, Cm, Minv, Td, Te)]
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    it = load().orc_lp_pcg_f64(variant, ptr(nodes), N, ptr(rp), ptr(ci), *[ptr(a) for a in arrs], maxit, tol,
+                               ptr(x), C.byref(tr))
+    return it, tr.value, x
