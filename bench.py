@@ -274,8 +274,9 @@ def pcg_to_tolerance(h, rel_tol, reps=20):
     assembled system, run to ||r|| < rel_tol ||b|| (the recurrence's residual,
     at most 5000 iterations) from x0 = 0; ms per solve from HIP events over
     `reps` solves, and the true relative residual ||b - A x|| / ||b|| of the
-    result recomputed in FP64.  Preconditioner: block-Jacobi (the learned
-    preconditioner is not built)."""
+    result recomputed in FP64.  Preconditioners: block-Jacobi, and the
+    learned block-diagonal one when its trained weights are present (the
+    network's forward time is reported apart from the PCG's)."""
     import ctypes as C
 
     import torch
@@ -298,10 +299,35 @@ def pcg_to_tolerance(h, rel_tol, reps=20):
     torch.cuda.synchronize()
     out = (C.c_double * 2)()
     L.check(lib.nrf_solver_true_residual(solver, x.data_ptr(), out, stream.cuda_stream), "true_residual")
+    result = {"preconditioner": "block-Jacobi", "ms": round(e0.elapsed_time(e1) / reps, 4),
+              "iterations": int(res.iterations), "true_rel_residual": float(out[0] / out[1]), "learned": None}
+    # learned block-diagonal preconditioner (trained weights of cudafusion_amd.train_lp): network
+    # forward + fold into the block-Jacobi inverses once, then the same PCG
+    from cudafusion_amd import learned as LP
+    path = os.path.join(ROOT, "cudafusion_amd", "lp_weights_v0.npz")
+    if os.path.exists(path):
+        lp = LP.LearnedPreconditioner(0, max_nodes=h.N, max_probes=1)
+        lp.load(path)
+        lp.set_graph(h.nodes)
+        d_nodes = lib.nrf_pipeline_nodes(h.pipe)
+        f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        f0.record(stream)
+        lp.forward(solver, d_nodes)
+        lp.fold(solver)
+        f1.record(stream)
+        L.check(lib.nrf_solver_resolve(solver, x.data_ptr(), C.byref(res), stream.cuda_stream), "resolve")
+        e0.record(stream)
+        for _ in range(reps):
+            L.check(lib.nrf_solver_resolve(solver, x.data_ptr(), None, stream.cuda_stream), "resolve")
+        e1.record(stream)
+        torch.cuda.synchronize()
+        L.check(lib.nrf_solver_true_residual(solver, x.data_ptr(), out, stream.cuda_stream), "true_residual")
+        result["learned"] = {"preconditioner": "BJ^-1 + L L^T, L block diagonal (3-layer MPNN, BF16 weights)",
+                             "ms": round(e0.elapsed_time(e1) / reps, 4), "network_ms": round(f0.elapsed_time(f1), 4),
+                             "iterations": int(res.iterations), "true_rel_residual": float(out[0] / out[1])}
+        lp.close()
     L.check(lib.nrf_solver_set_params(solver, C.byref(h.sp)), "set_params")
-    return {"preconditioner": "block-Jacobi", "ms": round(e0.elapsed_time(e1) / reps, 4),
-            "iterations": int(res.iterations), "true_rel_residual": float(out[0] / out[1]),
-            "learned": None}
+    return result
 
 
 def precision_errors(h, f):

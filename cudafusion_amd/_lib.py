@@ -143,6 +143,18 @@ _SIGS = {
     "nrf_batch_autotune": (I, [P, C.POINTER(BatchFrame), I, C.POINTER(BatchTuning), P]),
     "nrf_batch_step": (I, [P, C.POINTER(BatchFrame), P]),
     "nrf_batch_stage_ms": (I, [P, C.POINTER(F)]),
+    "nrf_lp_create": (P, [I, I, I]),
+    "nrf_lp_destroy": (None, [P]),
+    "nrf_lp_param_count": (I, []),
+    "nrf_lp_params": (P, [P]),
+    "nrf_lp_grads": (P, [P]),
+    "nrf_lp_reset_optimizer": (I, [P]),
+    "nrf_lp_set_graph": (I, [P, P, I]),
+    "nrf_lp_forward": (I, [P, P, P, I, P]),
+    "nrf_lp_loss_grad": (I, [P, P, P, P, I, C.POINTER(C.c_double), P]),
+    "nrf_lp_adam": (I, [P, F, I, P]),
+    "nrf_lp_fold": (I, [P, P, P]),
+    "nrf_lp_blocks": (I, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
 }
 
 _lib = None

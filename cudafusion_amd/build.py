@@ -20,7 +20,8 @@ INCLUDE = os.path.join(ROOT, "include")
 OBJDIR = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libnrf.so")
 
-HIP_SOURCES = ["common.hip", "integrate.hip", "mc.hip", "warp.hip", "solver.hip", "pipeline.hip", "precision.hip"]
+HIP_SOURCES = ["common.hip", "integrate.hip", "mc.hip", "warp.hip", "solver.hip", "pipeline.hip", "precision.hip",
+               "learned.hip"]
 CXX_SOURCES = ["dropin.cpp"]
 
 HIPCC_FLAGS = [

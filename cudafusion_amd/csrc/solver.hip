@@ -1029,7 +1029,9 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
     double pAp = lead ? (double)p * (double)y : 0.0, unused = 0.0;
     aborted = allreduce2(a, base + pt, (int)(pt & 1), pAp, unused, red, &s_bad);
     if (aborted) break;
-    if (fabs(pAp) < 1e-12) {
+    // the reference's absolute guard (|p.Ap| < 1e-12); with the relative
+    // stopping rule only a true breakdown (p.Ap not positive) stops the run
+    if (a.rel ? !(pAp > 0.0) : fabs(pAp) < 1e-12) {
       flags |= 1;
       break;
     }
@@ -1190,7 +1192,9 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
     ar2_publish(a, g, base + pt, (int)(pt & 1), pAp, unused, red);
     aborted = ar2_collect(a, base + pt, (int)(pt & 1), pAp, unused, red, &s_bad);
     if (aborted) break;
-    if (fabs(pAp) < 1e-12) {
+    // the reference's absolute guard (|p.Ap| < 1e-12); with the relative
+    // stopping rule only a true breakdown (p.Ap not positive) stops the run
+    if (a.rel ? !(pAp > 0.0) : fabs(pAp) < 1e-12) {
       flags |= 1;
       break;
     }
@@ -2011,13 +2015,16 @@ extern "C" nrf_solver *nrf_solver_create(const nrf_solver_params *params, int ma
   s->max_nodes = max_nodes;
   s->cus = device_cu_count();
   s->max_lds = device_max_lds_per_block();
-  s->coop = device_cooperative() ? 1 : 0;
-  // NRF_PCG_COOP=0 launches the persistent PCG kernel with an ordinary launch
-  // (G <= CU count workgroups are co-resident on an otherwise idle device, and
-  // the grid barrier's spin is bounded either way); profilers that cannot
-  // follow cooperative launches need it.
+  // The persistent PCG kernels take an ordinary launch: their grids are
+  // sized to be co-resident (at most the CU count at one 1024-lane workgroup
+  // per CU, or the occupancy bound for a batch group) and every wait in them
+  // is bounded, so a grid that could not become resident ends with the
+  // barrier-timeout flag instead of hanging.  NRF_PCG_COOP=1 uses the
+  // cooperative launch instead, which guarantees co-residency but takes the
+  // device's global wave-sync resource: another process holding it (one that
+  // made a cooperative launch before) makes the launch fail.
   const char *fc = getenv("NRF_PCG_COOP");
-  if (fc && fc[0] == '0') s->coop = 0;
+  s->coop = (fc && fc[0] == '1' && device_cooperative()) ? 1 : 0;
   // NRF_ASM_MFMA=0 assembles the data term with the scalar LDS form
   const char *fm = getenv("NRF_ASM_MFMA");
   if (fm && fm[0] == '0') s->asm_mfma = 0;

@@ -319,3 +319,17 @@ def copy_device_to_numpy(ptr: int, count: int, dtype) -> np.ndarray:
     if rc != 0:
         raise L.NrfError(f"hipMemcpy D2H failed ({rc})")
     return out
+
+
+def copy_numpy_to_device(ptr: int, arr: np.ndarray) -> None:
+    """Synchronous H2D copy of a contiguous array to a raw device address."""
+    arr = np.ascontiguousarray(arr)
+    if arr.nbytes == 0:
+        return
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipMemcpy.restype = C.c_int
+    torch.cuda.synchronize()
+    rc = hip.hipMemcpy(C.c_void_p(ptr), arr.ctypes.data, arr.nbytes, 1)  # hipMemcpyHostToDevice
+    if rc != 0:
+        raise L.NrfError(f"hipMemcpy H2D failed ({rc})")

@@ -107,7 +107,9 @@ typedef struct nrf_solver_params {
   float lambda_damping;   /* 1e-2 */
   int32_t debug;          /* 0   */
   /* options with no reference counterpart (0 = reference behaviour) */
-  int32_t relative_tolerance; /* 1: stop when ||r|| < tol * ||b||          */
+  int32_t relative_tolerance; /* 1: stop when ||r|| < tol * ||b||; the breakdown
+                                 stop is then p.Ap <= 0 instead of the
+                                 reference's |p.Ap| < 1e-12                   */
   int32_t precond_fp64;       /* 1: 6x6 Cholesky inverses computed in FP64 */
   int32_t pcg_pipelined;       /* 1: A p by the recurrence A z + beta A p, so that the
                                   product overlaps the (r.r, r.z) all-reduce; the same
@@ -144,7 +146,7 @@ typedef struct nrf_pcg_result {
   double final_residual;   /* ||r||_2 of the PCG recurrence */
   double rhs_norm;         /* ||b||_2 */
   int32_t converged;       /* PCG runs of this solve that met the tolerance */
-  int32_t breakdowns;      /* PCG runs of this solve that stopped on |p.Ap| < 1e-12 */
+  int32_t breakdowns;      /* PCG runs of this solve that stopped on the breakdown test */
   int32_t runs;            /* PCG runs of this solve */
   int32_t pad;
 } nrf_pcg_result;
@@ -331,7 +333,7 @@ typedef struct nrf_solver_totals {
   int64_t rejected;        /* solves that ended on a rejected step */
   int64_t pcg_runs;        /* PCG runs (one per GN iteration that reached the PCG) */
   int64_t pcg_converged;   /* PCG runs that met the tolerance */
-  int64_t pcg_breakdowns;  /* PCG runs that stopped on |p.Ap| < 1e-12 */
+  int64_t pcg_breakdowns;  /* PCG runs that stopped on the breakdown test */
 } nrf_solver_totals;
 int nrf_solver_get_totals(nrf_solver *s, nrf_solver_totals *out, void *stream);
 
@@ -451,6 +453,44 @@ int nrf_batch_step(nrf_batch *b, const nrf_batch_frame *frames, void *stream);
    [solve, integrate, mc], all instances; after a graph replay out_ms[0] is
    the whole step and the others are -1. */
 int nrf_batch_stage_ms(nrf_batch *b, float out_ms[3]);
+
+/* ------------------------------------------------ learned preconditioner
+   Build-only component (no counterpart in the reference).  A 3-layer
+   message-passing network over the deformation graph (hidden width 64,
+   BF16 weights, FP32 accumulation) gives the blocks of a block lower-
+   triangular L; the preconditioner is M^-1 = BJ^-1 + L L^T with
+   L_ii = C_i lower(Td_i), L_ij = C_i Te_ij (neighbours j < i) and
+   C_i C_i^T = BJ_i^-1.  variant 0: diagonal blocks only (folded into the
+   solver's block-Jacobi inverses by nrf_lp_fold, so the solver's PCG applies
+   it); variant 1: the graph pattern.  Parameters are a flat FP32 vector of
+   nrf_lp_param_count() floats (layout of oracle/learned.cpp), trained on the
+   device: nrf_lp_forward + nrf_lp_loss_grad (probe loss
+   sum_p |M^-1 A v_p - v_p|^2 / (P 6N) and its gradient) + nrf_lp_adam. */
+typedef struct nrf_lp nrf_lp;
+nrf_lp *nrf_lp_create(int variant, int max_nodes, int max_probes);
+void nrf_lp_destroy(nrf_lp *lp);
+int nrf_lp_param_count(void);
+float *nrf_lp_params(nrf_lp *lp);   /* device, FP32 master parameters */
+float *nrf_lp_grads(nrf_lp *lp);    /* device, gradient of the last nrf_lp_loss_grad */
+int nrf_lp_reset_optimizer(nrf_lp *lp);
+/* The graph (host nodes): reverse adjacency for the transposed products. */
+int nrf_lp_set_graph(nrf_lp *lp, const nrf_node *host_nodes, int num_nodes);
+/* Features, network and L blocks for the solver's assembled system (its
+   diagonal blocks and block-Jacobi inverses); nodes on the device. */
+int nrf_lp_forward(nrf_lp *lp, const nrf_solver *s, const nrf_node *nodes, int num_nodes, void *stream);
+/* Loss over num_probes device vectors [num_probes][6N] and its gradient
+   (into nrf_lp_grads), after nrf_lp_forward on the same solver; loss (host,
+   may be NULL) synchronises the stream. */
+int nrf_lp_loss_grad(nrf_lp *lp, const nrf_solver *s, const nrf_node *nodes, const float *probes,
+                     int num_probes, double *loss, void *stream);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the FP32 parameters. */
+int nrf_lp_adam(nrf_lp *lp, float lr, int step, void *stream);
+/* variant 0: the solver's block-Jacobi inverses become BJ_i^-1 + L_ii L_ii^T
+   (until its next assembly), after nrf_lp_forward on that solver. */
+int nrf_lp_fold(nrf_lp *lp, nrf_solver *s, void *stream);
+/* Device pointers of the last forward: C (N x 36), Td (N x 21), Te
+   (N x 8 x 36), features (N x 28). */
+int nrf_lp_blocks(const nrf_lp *lp, const float **C, const float **Td, const float **Te, const float **features);
 
 #ifdef __cplusplus
 }

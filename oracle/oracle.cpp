@@ -680,7 +680,7 @@ template <typename R> R dotv(const R *a, const R *b, int n) {
 }
 
 // pcg_solve: kernels/solver_kernels.cu:616-687 with x0 = 0.  tol is absolute
-// unless rel (then tol * ||b||).
+// unless rel (then tol * ||b||, and the breakdown test is p.Ap <= 0).
 template <typename R>
 R pcg(const int32_t *rp, const int32_t *ci, const R *A, const R *b, const R *M, int N, int maxit, R tol, int rel,
       R *x, int *iters, int *flags, std::vector<double> *trace) {
@@ -696,7 +696,8 @@ R pcg(const int32_t *rp, const int32_t *ci, const R *A, const R *b, const R *M, 
   for (int it = 0; it < maxit; ++it) {
     spmv(rp, ci, A, p.data(), Ap.data(), N);
     R pAp = dotv(p.data(), Ap.data(), n);
-    if (std::fabs(pAp) < (R)1e-12f) { fl |= 1; used = it; break; }
+    // reference guard |p.Ap| < 1e-12; with the relative rule only p.Ap <= 0
+    if (rel ? !(pAp > (R)0) : std::fabs(pAp) < (R)1e-12f) { fl |= 1; used = it; break; }
     R alpha = rz / pAp;
     for (int i = 0; i < n; ++i) x[i] += alpha * p[i];
     for (int i = 0; i < n; ++i) r[i] += -alpha * Ap[i];
