@@ -23,7 +23,7 @@ namespace {
 
 constexpr int kTileX = 64;
 constexpr int kTileY = 4;
-constexpr int kZChunk = 4;
+constexpr int kZChunk = 8;
 
 struct IntegrateArgs {
   nrf_voxel *vox;
@@ -69,7 +69,10 @@ __device__ void integrate_tile(const IntegrateArgs &a, int bx, int by, int bz) {
       float3 p = f3(px_w, py_w, a.oz + (z + 0.5f) * a.s);
       if (WARP) {
         const int4 id4 = a.knn[idx];
-        const float4 w4 = a.knn_w[idx];
+        // voxels unobserved at the snapshot (all ids -1, most of the volume)
+        // integrate unwarped: their weights are never read
+        float4 w4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if ((id4.x & id4.y & id4.z & id4.w) >= 0) w4 = a.knn_w[idx];  // sign bit clear: some id >= 0
         const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
         const float ws[4] = {w4.x, w4.y, w4.z, w4.w};
         DQ b;
