@@ -106,6 +106,9 @@ def main() -> int:
                          "known HBM traffic that calibrates hardware byte counters")
     ap.add_argument("--textbook-pcg", action="store_true",
                     help="A p recomputed each iteration instead of A z + beta A p overlapping the all-reduce")
+    ap.add_argument("--pcg-form", type=int, choices=[1, 2], default=2,
+                    help="1: A p = A z + beta A p with the product overlapping the (r.r, r.z) all-reduce; 2: also "
+                         "p.Ap from z.Az, one all-reduce per iteration (Chronopoulos-Gear)")
     ap.add_argument("--precision", choices=["bf16", "fp16"], default=None,
                     help="precision form: FP16 TSDF + u8 weight, compact kNN, JtJ blocks in BF16 / FP16 on the "
                          "matrix cores, FP64 Cholesky inverses, one FP64-residual refinement step per solve; "
@@ -140,7 +143,7 @@ def main() -> int:
     ref = args.reference_policy
     prec = {"bf16": 1, "fp16": 2}.get(args.precision, 0)
     h = Harness("body", frames=args.frames, use_graph=args.graph, arap_consistent_sign=not ref, validate=ref,
-                pcg_pipelined=not args.textbook_pcg, jtj_precision=prec, refine_iterations=10 if prec else 0,
+                pcg_pipelined=0 if args.textbook_pcg else args.pcg_form, jtj_precision=prec, refine_iterations=10 if prec else 0,
                 precond_fp64=bool(prec), storage16=bool(prec))
     h.setup()
     log(f"[bench] setup {time.perf_counter() - t0:.1f} s, {h.N} nodes, "
@@ -244,7 +247,8 @@ def main() -> int:
                    "graph": bool(args.graph),
                    "solver_policy": ("reference (ARAP sign, whole-step rejection)" if ref else
                                      "consistent ARAP sign, per-node clamp, no whole-step rejection"),
-                   "pcg": "textbook" if args.textbook_pcg else "pipelined"},
+                   "pcg": "textbook" if args.textbook_pcg else
+                          {1: "overlapped (A z + beta A p)", 2: "single reduction (Chronopoulos-Gear)"}[args.pcg_form]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None or dom not in traffic else traffic[dom], "kernel": dom,
@@ -389,7 +393,7 @@ def run_batched(args) -> int:
     t0 = time.perf_counter()
     ws = S.batch_workloads(count, frames=nf, seed=0, pcg_iterations=300)
     kw = dict(frames=nf, use_graph=args.graph, pcg_tolerance=1e-6, relative_tolerance=True,
-              arap_consistent_sign=True, validate=False, pcg_pipelined=True)
+              arap_consistent_sign=True, validate=False, pcg_pipelined=args.pcg_form)
     solo, batch = [], []
     for w in ws:
         a = Harness(workload=w, **kw)
@@ -487,7 +491,8 @@ def run_batched(args) -> int:
                                % (count, min(nodes), max(nodes), sum(nodes)),
                    "instances": [w.name for w in ws], "graph": bool(args.graph),
                    "solver_policy": "consistent ARAP sign, per-node clamp, no whole-step rejection",
-                   "pcg": "pipelined"},
+                   "pcg": {1: "overlapped (A z + beta A p)", 2: "single reduction (Chronopoulos-Gear)"}[
+                       args.pcg_form]},
         "tuning": {"lanes_per_row": tune.lanes_per_row, "wg_nodes": tune.wg_nodes, "z_chunk": tune.z_chunk,
                    "pcg_launch_groups": tune.launch_groups, "tune_s": round(tune_s, 2)},
         "batched": {"frames_per_s": round(fps_b, 2), "step_ms_p50": round(float(np.percentile(lat_b, 50)), 3),

@@ -747,7 +747,7 @@ __global__ __launch_bounds__(64) void assemble_rows_mfma_kernel(AsmArgs a) {
 typedef unsigned long long ll_word;
 
 struct alignas(64) SyncRec {
-  ll_word w[2][4];  // per parity: hi / lo halves of the two partial sums
+  ll_word w[2][6];  // per parity: hi / lo halves of up to three partial sums
 };
 struct PcgCtl {
   unsigned long long epoch;  // advanced once per launch by pcg_prep_kernel
@@ -760,6 +760,9 @@ constexpr int kPcgMaxNodesPerWg = kPcgRows / 6;   // 42 nodes = 252 rows
 constexpr int kPcgMaxWg = 256;                    // records (one per workgroup)
 constexpr int kPcgMaxPoints = 65535;              // synchronisation points per launch (16 tag bits)
 constexpr int kPcgWaves = kPcgThreads / 64;
+// workgroup reduction scratch: first stage 3 per wave, second stage 3 per
+// wave of record-holding lanes (at most kPcgMaxWg / 64 = 4 waves)
+constexpr int kRedSize = 3 * kPcgWaves + 3 * 4;
 
 // Wave-level sum of a double in a fixed order (pairs, quads, half rows,
 // rows through DPP lane moves, then rows 0+1 and 2+3 and the two halves via
@@ -914,6 +917,64 @@ __device__ inline bool allreduce2(const PcgArgs &a, unsigned tag, int slot, doub
   return ar2_collect(a, tag, slot, v0, v1, red, s_bad);
 }
 
+// The same all-reduce for three doubles (lanes 0..5 publish the hi / lo
+// words; the sums follow the fixed orders of the two-value form).
+__device__ inline void ar3_publish(const PcgArgs &a, int g, unsigned tag, int slot, double (&v)[3], double *red) {
+  double s[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) s[k] = wave_sum_last(v[k]);
+  const int t = threadIdx.x, w = t >> 6;
+  if ((t & 63) == 63)
+    for (int k = 0; k < 3; ++k) red[3 * w + k] = s[k];
+  __syncthreads();
+  if (t < 6) {
+    double acc = 0;
+    for (int kk = 0; kk < kPcgWaves; ++kk) acc += red[3 * kk + (t >> 1)];
+    const ll_word bits = (ll_word)__double_as_longlong(acc);
+    ll_put(&a.recs[g].w[slot][t], tag, (t & 1) ? (unsigned)bits : (unsigned)(bits >> 32));
+  }
+}
+__device__ inline bool ar3_collect(const PcgArgs &a, unsigned tag, int slot, double (&v)[3], double *red,
+                                   int *s_bad) {
+  const int t = threadIdx.x;
+  double g[3] = {0, 0, 0};
+  if (t < a.G) {
+    const ll_word *w = a.recs[t].w[slot];
+    ll_word x[6];
+    unsigned spins = 0;
+    for (;;) {
+      bool all = true;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        x[k] = ll_load(w + k);
+        all = all && ll_has(x[k], tag);
+      }
+      if (all) break;
+      if (ll_spin(spins, a.ctl, s_bad)) break;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      g[k] = __longlong_as_double((long long)((x[2 * k] << 32) | (x[2 * k + 1] & 0xffffffffull)));
+  }
+  double *r2 = red + 3 * kPcgWaves;
+  const int w = t >> 6, nw = (a.G + 63) >> 6;
+  if (w < nw) {
+    double sm[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sm[k] = wave_sum_last(g[k]);
+    if ((t & 63) == 63)
+      for (int k = 0; k < 3; ++k) r2[3 * w + k] = sm[k];
+  }
+  __syncthreads();  // also publishes *s_bad
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double acc = 0;
+    for (int kk = 0; kk < nw; ++kk) acc += r2[3 * kk + k];
+    v[k] = acc;
+  }
+  return *s_bad != 0;
+}
+
 // Block-Jacobi PCG (kernels/solver_kernels.cu:616-687, Saad Alg. 9.1, x0 = 0)
 // as one persistent kernel.  Workgroup g owns nodes [rows[g], rows[g+1]),
 // at most 42, i.e. at most 252 scalar rows; four adjacent lanes share scalar
@@ -930,7 +991,7 @@ __device__ inline bool allreduce2(const PcgArgs &a, unsigned tag, int slot, doub
 // bits, so the owner's p and its readers' copies are identical.
 __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
   extern __shared__ float lds[];
-  __shared__ double red[2 * kPcgWaves + 8];
+  __shared__ double red[kRedSize];
   __shared__ int s_bad;
   __shared__ unsigned s_base;
   if (a.stop && *a.stop) return;
@@ -1087,7 +1148,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
 // Same iterates as pcg_kernel in exact arithmetic; in FP32 the one extra
 // recurrence (for Ap) keeps the iterates within a few ulps of the textbook
 // ones.  Layout, publication and the lane-per-row split are pcg_kernel's.
-template <int SUB>
+template <int SUB, bool CG1>
 __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, int &s_bad, unsigned &s_base) {
   constexpr int ROWS = kPcgThreads / SUB;
   if (a.stop && *a.stop) return;
@@ -1175,6 +1236,72 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
 
   __syncthreads();
   const unsigned base = s_base;
+  if constexpr (CG1) {
+    // One all-reduce per iteration (Chronopoulos and Gear, 1989): with
+    // gamma = r.z and delta = z.Az reduced together with r.r, p.Ap follows
+    // from delta - beta gamma / alpha_old (equal to p.Ap in exact
+    // arithmetic, since z_{k+1}.r_k = 0 and p_k.Ap_k = gamma_k / alpha_k),
+    // so A z no longer waits for a p.Ap reduction.  A p is formed as
+    // A z + beta A p as in the overlapped form.
+    unsigned pt = 1;
+    float x = 0.f, r = bv;
+    float z = precond_publish(r, base + pt);
+    float az = product(base + pt);
+    double v[3] = {lead ? (double)r * (double)z : 0.0, lead ? (double)z * (double)az : 0.0,
+                   lead ? (double)bv * (double)bv : 0.0};
+    ar3_publish(a, g, base + pt, (int)(pt & 1), v, red);
+    bool aborted = ar3_collect(a, base + pt, (int)(pt & 1), v, red, &s_bad);
+    double gamma = v[0], pAp = v[1];
+    const double bb = v[2];
+    double rr = bb;
+    const double tol = a.rel ? (double)a.tol * sqrt(bb) : (double)a.tol;
+    int iters = 0, flags = 0;
+    float p = z, y = az;
+    for (int it = 0; it < a.maxit && !aborted; ++it) {
+      if (a.rel ? !(pAp > 0.0) : fabs(pAp) < 1e-12) {
+        flags |= 1;
+        break;
+      }
+      const double alpha = gamma / pAp;
+      const float alf = (float)alpha;
+      x += alf * p;
+      r += -alf * y;
+      ++pt;
+      z = precond_publish(r, base + pt);
+      az = product(base + pt);
+      v[0] = lead ? (double)r * (double)r : 0.0;
+      v[1] = lead ? (double)r * (double)z : 0.0;
+      v[2] = lead ? (double)z * (double)az : 0.0;
+      ar3_publish(a, g, base + pt, (int)(pt & 1), v, red);
+      aborted = ar3_collect(a, base + pt, (int)(pt & 1), v, red, &s_bad);
+      if (aborted) break;
+      rr = v[0];
+      iters = it + 1;
+      if (sqrt(rr) < tol) {
+        flags |= 2;
+        break;
+      }
+      const double beta = v[1] / gamma;
+      gamma = v[1];
+      pAp = v[2] - beta * gamma / alpha;
+      const float betaf = (float)beta;
+      p = z + betaf * p;
+      y = az + betaf * y;
+    }
+    if (aborted) flags |= 8;
+    if (lead) a.x[s] = x;
+    if (g == 0 && t == 0) {
+      a.res->iterations = iters;
+      a.res->total_iterations += iters;
+      a.res->flags |= flags;
+      a.res->final_residual = sqrt(rr);
+      a.res->rhs_norm = sqrt(bb);
+      a.res->converged += (flags & 2) ? 1 : 0;
+      a.res->breakdowns += (flags & 1) ? 1 : 0;
+      a.res->runs += 1;
+    }
+    return;
+  }
   // init: x = 0, r = b, z = M^-1 r, p = z, Ap = A z (kernels/solver_kernels.cu:620-632)
   unsigned pt = 1;
   float x = 0.f, r = bv;
@@ -1233,13 +1360,13 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
   }
 }
 
-template <int SUB>
+template <int SUB, bool CG1>
 __global__ __launch_bounds__(kPcgThreads) void pcg_pipe_kernel(PcgArgs a) {
   extern __shared__ float lds[];
-  __shared__ double red[2 * kPcgWaves + 8];
+  __shared__ double red[kRedSize];
   __shared__ int s_bad;
   __shared__ unsigned s_base;
-  pcg_pipe_body<SUB>(a, (int)blockIdx.x, lds, red, s_bad, s_base);
+  pcg_pipe_body<SUB, CG1>(a, (int)blockIdx.x, lds, red, s_bad, s_base);
 }
 
 
@@ -1534,16 +1661,16 @@ __global__ void pcg_prep_batch_kernel(const InstDev *__restrict__ inst, int coun
 // instance wg_inst[b] as its workgroup wg_local[b].  The instances share no
 // data; each synchronises its own workgroups through its own records, so
 // only the workgroups of one instance need to be co-resident.
-template <int SUB>
+template <int SUB, bool CG1>
 __global__ __launch_bounds__(kPcgThreads) void pcg_pipe_batch_kernel(const InstDev *__restrict__ inst,
                                                                      const int32_t *__restrict__ wg_inst,
                                                                      const int32_t *__restrict__ wg_local) {
   extern __shared__ float lds[];
-  __shared__ double red[2 * kPcgWaves + 8];
+  __shared__ double red[kRedSize];
   __shared__ int s_bad;
   __shared__ unsigned s_base;
   const PcgArgs a = inst[wg_inst[blockIdx.x]].pcg;
-  pcg_pipe_body<SUB>(a, wg_local[blockIdx.x], lds, red, s_bad, s_base);
+  pcg_pipe_body<SUB, CG1>(a, wg_local[blockIdx.x], lds, red, s_bad, s_base);
 }
 __global__ __launch_bounds__(1024) void gn_validate_batch_kernel(const InstDev *__restrict__ inst) {
   const GnArgs &g = inst[blockIdx.x].gn;
@@ -1779,12 +1906,19 @@ static int solver_build_partition(nrf_solver *s) {
   if ((st = upload(s->halo_off, s->c_halo_off, hoff))) return st;
   if ((st = upload(s->halo, s->c_halo, halo))) return st;
   if (s->pcg_lds > 65536 && s->pcg_lds > s->lds_attr) {
-    const void *fns[] = {reinterpret_cast<const void *>(pcg_kernel), reinterpret_cast<const void *>(pcg_pipe_kernel<1>),
-                         reinterpret_cast<const void *>(pcg_pipe_kernel<2>),
-                         reinterpret_cast<const void *>(pcg_pipe_kernel<4>),
-                         reinterpret_cast<const void *>(pcg_pipe_batch_kernel<1>),
-                         reinterpret_cast<const void *>(pcg_pipe_batch_kernel<2>),
-                         reinterpret_cast<const void *>(pcg_pipe_batch_kernel<4>)};
+    const void *fns[] = {reinterpret_cast<const void *>(pcg_kernel),
+                         reinterpret_cast<const void *>(pcg_pipe_kernel<1, false>),
+                         reinterpret_cast<const void *>(pcg_pipe_kernel<2, false>),
+                         reinterpret_cast<const void *>(pcg_pipe_kernel<4, false>),
+                         reinterpret_cast<const void *>(pcg_pipe_kernel<1, true>),
+                         reinterpret_cast<const void *>(pcg_pipe_kernel<2, true>),
+                         reinterpret_cast<const void *>(pcg_pipe_kernel<4, true>),
+                         reinterpret_cast<const void *>(pcg_pipe_batch_kernel<1, false>),
+                         reinterpret_cast<const void *>(pcg_pipe_batch_kernel<2, false>),
+                         reinterpret_cast<const void *>(pcg_pipe_batch_kernel<4, false>),
+                         reinterpret_cast<const void *>(pcg_pipe_batch_kernel<1, true>),
+                         reinterpret_cast<const void *>(pcg_pipe_batch_kernel<2, true>),
+                         reinterpret_cast<const void *>(pcg_pipe_batch_kernel<4, true>)};
     bool ok = true;
     for (const void *fn : fns)
       ok = ok && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->pcg_lds) == hipSuccess;
@@ -1969,10 +2103,14 @@ static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxi
   void *args[] = {&a};
   const bool pipe = s->params.pcg_pipelined != 0;
   if (!pipe && s->pcg_sub != kPcgSub) return NRF_ERR_INVALID_ARGUMENT;  // textbook form: 4 lanes per row only
-  const void *fn = !pipe              ? reinterpret_cast<const void *>(pcg_kernel)
-                   : s->pcg_sub == 1 ? reinterpret_cast<const void *>(pcg_pipe_kernel<1>)
-                   : s->pcg_sub == 2 ? reinterpret_cast<const void *>(pcg_pipe_kernel<2>)
-                                     : reinterpret_cast<const void *>(pcg_pipe_kernel<4>);
+  const bool cg1 = s->params.pcg_pipelined == 2;
+  const void *fn = !pipe                     ? reinterpret_cast<const void *>(pcg_kernel)
+                   : cg1 && s->pcg_sub == 1  ? reinterpret_cast<const void *>(pcg_pipe_kernel<1, true>)
+                   : cg1 && s->pcg_sub == 2  ? reinterpret_cast<const void *>(pcg_pipe_kernel<2, true>)
+                   : cg1                     ? reinterpret_cast<const void *>(pcg_pipe_kernel<4, true>)
+                   : s->pcg_sub == 1         ? reinterpret_cast<const void *>(pcg_pipe_kernel<1, false>)
+                   : s->pcg_sub == 2         ? reinterpret_cast<const void *>(pcg_pipe_kernel<2, false>)
+                                             : reinterpret_cast<const void *>(pcg_pipe_kernel<4, false>);
   if (s->coop && !stream_capturing(st)) {
     NRF_HIP_CHECK(hipLaunchCooperativeKernel(fn, dim3(s->G), dim3(kPcgThreads), args, (unsigned)s->pcg_lds, st));
   } else {
@@ -2005,7 +2143,7 @@ static int solver_ensure_pattern(nrf_solver *s, const nrf_node *d_nodes, int N, 
 // fit the 16 point bits of a published word's tag.
 static bool params_ok(const nrf_solver_params *p) {
   return p->pcg_iterations >= 0 && 1 + 2 * (long)p->pcg_iterations <= kPcgMaxPoints && p->jtj_precision >= 0 &&
-         p->jtj_precision <= 2;
+         p->jtj_precision <= 2 && p->pcg_pipelined >= 0 && p->pcg_pipelined <= 2;
 }
 
 extern "C" nrf_solver *nrf_solver_create(const nrf_solver_params *params, int max_nodes) {
@@ -2217,6 +2355,7 @@ extern "C" int nrf_solver_get_totals(nrf_solver *s, nrf_solver_totals *out, void
 struct nrf::SolverBatch {
   std::vector<SolverBatchInstance> in;
   int count = 0, sub = 4, prec = 0, ns = 1, gn_iters = 1, any_validate = 0, coop = 0;
+  bool cg1 = false;  // the single-reduction PCG form (pcg_pipelined = 2)
   int max_N = 0, rows = 0, wgs = 0;
   size_t asm_lds = 0, pcg_lds = 0;
   InstDev *d_inst = nullptr;
@@ -2260,10 +2399,14 @@ static int upload_new(T *&p, const std::vector<T> &v) {
   return upload(p, cap, v);
 }
 
-static const void *pcg_batch_fn(int sub) {
-  return sub == 1   ? reinterpret_cast<const void *>(pcg_pipe_batch_kernel<1>)
-         : sub == 2 ? reinterpret_cast<const void *>(pcg_pipe_batch_kernel<2>)
-                    : reinterpret_cast<const void *>(pcg_pipe_batch_kernel<4>);
+static const void *pcg_batch_fn(int sub, bool cg1) {
+  if (cg1)
+    return sub == 1   ? reinterpret_cast<const void *>(pcg_pipe_batch_kernel<1, true>)
+           : sub == 2 ? reinterpret_cast<const void *>(pcg_pipe_batch_kernel<2, true>)
+                      : reinterpret_cast<const void *>(pcg_pipe_batch_kernel<4, true>);
+  return sub == 1   ? reinterpret_cast<const void *>(pcg_pipe_batch_kernel<1, false>)
+         : sub == 2 ? reinterpret_cast<const void *>(pcg_pipe_batch_kernel<2, false>)
+                    : reinterpret_cast<const void *>(pcg_pipe_batch_kernel<4, false>);
 }
 
 int solver_batch_create(const SolverBatchInstance *inst, int count, int lanes_per_row, int wg_nodes,
@@ -2295,12 +2438,14 @@ int solver_batch_create(const SolverBatchInstance *inst, int count, int lanes_pe
     const nrf_solver_params &sp = s->params;
     const int ns = asm_strips(s->P.Rmax);
     if (!sp.pcg_pipelined || sp.refine_iterations != 0 || !s->asm_mfma || ns > 8 ||
-        asm_lds_bytes(s->P.Rmax) > 64 * 1024 || (k > 0 && (sp.jtj_precision != b->prec ||
-                                                           std::max(1, q.gn.gn_iterations) != b->gn_iters))) {
+        asm_lds_bytes(s->P.Rmax) > 64 * 1024 ||
+        (k > 0 && (sp.jtj_precision != b->prec || std::max(1, q.gn.gn_iterations) != b->gn_iters ||
+                   (sp.pcg_pipelined == 2) != b->cg1))) {
       rc = NRF_ERR_INVALID_ARGUMENT;
       break;
     }
     if (k == 0) {
+      b->cg1 = sp.pcg_pipelined == 2;
       b->prec = sp.jtj_precision;
       b->gn_iters = std::max(1, q.gn.gn_iterations);
       b->coop = s->coop;
@@ -2348,7 +2493,7 @@ int solver_batch_create(const SolverBatchInstance *inst, int count, int lanes_pe
     }
   }
   if (!rc && b->pcg_lds > 65536 &&
-      hipFuncSetAttribute(pcg_batch_fn(b->sub), hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->pcg_lds) !=
+      hipFuncSetAttribute(pcg_batch_fn(b->sub, b->cg1), hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->pcg_lds) !=
           hipSuccess) {
     (void)hipGetLastError();
     rc = NRF_ERR_TOO_LARGE;
@@ -2356,7 +2501,7 @@ int solver_batch_create(const SolverBatchInstance *inst, int count, int lanes_pe
   if (!rc) {
     // launch groups: consecutive instances whose workgroups are co-resident
     int per_cu = 0;
-    NRF_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pcg_batch_fn(b->sub), kPcgThreads,
+    NRF_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pcg_batch_fn(b->sub, b->cg1), kPcgThreads,
                                                                b->pcg_lds));
     const int cap = per_cu * device_cu_count();
     int first = 0, used = 0, wg = 0;
@@ -2453,7 +2598,7 @@ static int batch_assemble(SolverBatch *b, const BatchCorrs &fr, hipStream_t st) 
 int solver_batch_pcg(SolverBatch *b, hipStream_t st) {
   pcg_prep_batch_kernel<<<1, 64, 0, st>>>(b->d_inst, b->count);
   NRF_LAUNCH_CHECK();
-  const void *fn = pcg_batch_fn(b->sub);
+  const void *fn = pcg_batch_fn(b->sub, b->cg1);
   const bool coop = b->coop && !stream_capturing(st);
   for (const auto &g : b->groups) {
     const InstDev *inst = b->d_inst;

@@ -62,7 +62,7 @@ def transform_errors(Ta, Tb):
 class Harness:
     def __init__(self, kind: str = "body", frames: int = 40, use_graph: bool = False, seed: int = 0,
                  pcg_tolerance: float = 0.0, relative_tolerance: bool = False, precond_fp64: bool = False,
-                 arap_consistent_sign: bool = False, validate: bool = True, pcg_pipelined: bool = False,
+                 arap_consistent_sign: bool = False, validate: bool = True, pcg_pipelined: int = 0,
                  jtj_precision: int = 0, refine_iterations: int = 0, storage16: bool = False,
                  workload: S.Workload | None = None):
         self.kind = kind

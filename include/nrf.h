@@ -112,8 +112,10 @@ typedef struct nrf_solver_params {
                                  reference's |p.Ap| < 1e-12                   */
   int32_t precond_fp64;       /* 1: 6x6 Cholesky inverses computed in FP64 */
   int32_t pcg_pipelined;       /* 1: A p by the recurrence A z + beta A p, so that the
-                                  product overlaps the (r.r, r.z) all-reduce; the same
-                                  iterates in exact arithmetic */
+                                  product overlaps the (r.r, r.z) all-reduce; 2: in
+                                  addition p.Ap from z.Az (Chronopoulos-Gear), reduced
+                                  with r.r and r.z: one all-reduce per iteration.  The
+                                  same iterates in exact arithmetic */
   int32_t arap_consistent_sign; /* 1: ARAP rotation Jacobian rows with the sign of
                                    d(omega x q)/d(omega), as the data term and the
                                    exp map use; 0 keeps the reference's negated rows

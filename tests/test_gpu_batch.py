@@ -25,15 +25,18 @@ def _workloads():
     ]
 
 
-def _sets():
+def _sets(pcg_form=1):
     """Two harness sets over the same workloads (the second reuses the
-    first's rendered frames and nodes)."""
+    first's rendered frames and nodes); pcg_form 1 overlapped, 2 single
+    reduction."""
     from cudafusion_amd.harness import Harness
     a, b = [], []
     for w, kw in _workloads():
-        ha = Harness(workload=w, frames=3, pcg_pipelined=True, pcg_tolerance=1e-6, relative_tolerance=True, **kw)
+        ha = Harness(workload=w, frames=3, pcg_pipelined=pcg_form, pcg_tolerance=1e-6, relative_tolerance=True,
+                     **kw)
         ha.setup()
-        hb = Harness(workload=w, frames=3, pcg_pipelined=True, pcg_tolerance=1e-6, relative_tolerance=True, **kw)
+        hb = Harness(workload=w, frames=3, pcg_pipelined=pcg_form, pcg_tolerance=1e-6, relative_tolerance=True,
+                     **kw)
         hb.setup(host=ha)
         a.append(ha)
         b.append(hb)
@@ -86,10 +89,11 @@ def test_batch_matches_unbatched_eager(gpu):
 
 
 def test_batch_autotune_and_graph(gpu):
+    """Tuned layout, graph replays and the single-reduction PCG form."""
     import torch
 
     from cudafusion_amd.harness import BatchHarness
-    a, b = _sets()
+    a, b = _sets(pcg_form=2)
     bh = BatchHarness(b, use_graph=True)
     before = [_state(h) for h in b]
     tot_before = [(t.solves, t.gn_done, t.pcg_iterations) for t in (h.solver_totals() for h in b)]

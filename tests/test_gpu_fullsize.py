@@ -86,17 +86,19 @@ def test_marching_cubes_full_size(body, orc):
     np.testing.assert_allclose(mesh, verts, atol=1e-6)
 
 
-def test_tracking_sequence_consistent_policy(gpu, orc):
+@pytest.mark.parametrize("pcg_form", [1, 2], ids=["overlapped", "single_reduction"])
+def test_tracking_sequence_consistent_policy(gpu, orc, pcg_form):
     """Eight frames of the sequence through the pipeline with the benchmark's
     solver policy (consistent ARAP sign, per-node clamp, no whole-step
-    rejection, pipelined PCG).  Per frame: the node transforms equal the FP64 oracle's solve
-    from the same starting transforms (1e-4 rad / 1e-5 m), every GN step is
+    rejection), with the overlapped and the single-reduction PCG forms.
+    Per frame: the node transforms equal the FP64 oracle's solve from the
+    same starting transforms (1e-4 rad / 1e-5 m), every GN step is
     applied, and the point-to-plane RMS of the correspondences after the solve
     is at the depth-noise level (sigma = 0.0012 z^2 = 1.7 mm at 1.2 m)."""
     import torch
 
     from cudafusion_amd.harness import Harness, transform_errors
-    h = Harness("body", frames=9, arap_consistent_sign=True, validate=False, pcg_pipelined=True)
+    h = Harness("body", frames=9, arap_consistent_sign=True, validate=False, pcg_pipelined=pcg_form)
     h.setup()
     N = h.N
 
