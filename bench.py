@@ -65,6 +65,14 @@ def measured_traffic():
     return out
 
 
+def observed_voxels(h) -> int:
+    """Voxels whose kNN snapshot holds a node (the others integrate unwarped
+    and their kNN weights are not read)."""
+    import numpy as np
+    knn, _ = h.download_knn()
+    return int((knn.reshape(-1, 4).max(axis=1) >= 0).sum())
+
+
 def cpu_baseline(h, frames: int, threads: int):
     """The oracle (C++/OpenMP restatement, FP32) on `frames` frames of the
     same workload: GN solve + warped integration + marching cubes."""
@@ -198,9 +206,11 @@ def main() -> int:
     G = w.gn_iterations
     nb = h.num_blocks()
     # algorithmic bytes per launch (counted once, minimum traffic of the algorithm)
+    n_obs = observed_voxels(h)
     algo = {
-        # kNN ids + weights (32 B/voxel), read + write of updated voxels, depth, centred node table
-        "integrate": V * 32 + upd * 24 + 4 * w.width * w.height + 32 * h.N,
+        # kNN ids of every voxel (16 B), kNN weights of the voxels with a snapshot neighbour (16 B), read +
+        # write of updated voxels, depth, centred node table
+        "integrate": V * 16 + n_obs * 16 + upd * 24 + 4 * w.width * w.height + 32 * h.N,
         # per iteration: BSR values + column ids, row pointers, p gather + Ap, block-Jacobi
         # blocks, r/z/x/p vector traffic of the updates and the three dot products
         "pcg": w.pcg_iterations * (nb * (36 * 4 + 4) + 4 * (h.N + 1) + 2 * 24 * h.N + h.N * 36 * 4
@@ -258,7 +268,8 @@ def main() -> int:
         "roofline_other": {k: round(algo[k] / (per_launch[k] * 1e-3) / 1e9, 1) for k in algo
                            if per_launch[k] == per_launch[k]},
         "work": {"gn_applied_per_step": round(gn_per_step, 3), "pcg_iterations_per_step": round(it_per_step, 2),
-                 "rejected_steps": int(rejected), "updated_voxels_per_step": int(upd), "nodes": h.N,
+                 "rejected_steps": int(rejected), "updated_voxels_per_step": int(upd), "observed_voxels": n_obs,
+                 "nodes": h.N,
                  "bsr_blocks": nb},
         "cpu_baseline": cpu,
     }
@@ -470,7 +481,7 @@ def run_batched(args) -> int:
     algo = 0
     for h, u in zip(batch, upd):
         V = int(np.prod(h.w.dims))
-        algo += V * 32 + u * 24 + 4 * h.w.width * h.w.height + 32 * h.N
+        algo += V * 16 + observed_voxels(h) * 16 + u * 24 + 4 * h.w.width * h.w.height + 32 * h.N
     int_ms = stages[1] if stages[1] > 0 else tune.integrate_ms  # graph replays have no stage split
     ach = algo / (int_ms * 1e-3) / 1e9
     out = {
