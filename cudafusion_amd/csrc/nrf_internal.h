@@ -52,11 +52,16 @@ bool device_cooperative();
 // Per-frame inputs travel as kernel arguments, so a captured batch step
 // depends on nothing but them and the instances' device state.
 constexpr int kMaxBatch = 32;
+// memberships per chunk of the MFMA assembly (rows with more are split)
+constexpr int kAsmChunk = 256;
 struct BatchCorrs {
   const nrf_corr *corrs[kMaxBatch];
   const float *canonical[kMaxBatch];
   int32_t n[kMaxBatch];
   int32_t max_n;
+  // first assembly chunk slot of each instance in the batched launch:
+  // cbase[k + 1] - cbase[k] = N_k + ceil(4 n_k / kAsmChunk)
+  int32_t cbase[kMaxBatch + 1];
 };
 struct BatchDepth {
   const float *depth[kMaxBatch];

@@ -494,6 +494,7 @@ static int batch_frames(const nrf_batch *b, const nrf_batch_frame *frames, Batch
     fc.canonical[k] = f.canonical;
     fc.n[k] = f.num_corrs;
     fc.max_n = std::max(fc.max_n, f.num_corrs);
+    fc.cbase[k + 1] = fc.cbase[k] + q->N + (4 * f.num_corrs + kAsmChunk - 1) / kAsmChunk;
     fd.depth[k] = f.depth;
     fd.xf[k] = invert_pose(f.camera_pose);
   }

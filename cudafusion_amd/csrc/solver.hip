@@ -204,7 +204,23 @@ struct AsmArgs {
   float *values, *b, *Minv;
   const int32_t *stop;
   const float *crec;  // per-corr records (corr_prep_kernel), MFMA form only
+  // chunks of a row's memberships (MFMA form): chunk_off[i] .. chunk_off[i+1]
+  // are row i's chunks, chunk_row maps a chunk to its row; rows of several
+  // chunks leave per-chunk partial sums in part
+  const int32_t *chunk_off, *chunk_row;
+  float *part;
 };
+
+// Memberships per assembly chunk: a row with more is split, so that the
+// longest serial chain of 64-membership batches is 4 batches long.
+constexpr int kChunk = kAsmChunk;
+// floats of one chunk's partial sums: the accumulators of lanes 0..31 (rows
+// 0..5 of every column strip live there), NS strips + the FP32 RHS column
+__host__ __device__ constexpr int part_floats(int ns) { return (ns + 1) * 32 * 4; }
+// chunk slots for N rows and n correspondences (4 memberships each at most)
+inline size_t asm_max_chunks(int N, int n) {
+  return (size_t)N + ((size_t)4 * (n > 0 ? n : 0) + kChunk - 1) / kChunk;
+}
 
 // staging layout per entry (floats): fw, r, kslot, pos[4] (as float bits), J[4][6]
 constexpr int kStageData = 3 + 4 + 24;
@@ -566,8 +582,12 @@ __device__ inline floatx4_t mfma_lp(typename LpType<PREC>::vec a, typename LpTyp
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// mode 0: the whole row; mode 1: memberships [pb, pe) of the row, partial
+// sums to chunk slot `slot`; mode 2: the sums of the row's nslot chunk slots
+// from `slot` on, added in slot order, then the rest of the row.
 template <int NS, int PREC>
-__device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_rhs) {
+__device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_rhs, int mode = 0, int pb = 0,
+                                  int pe = 0, int slot = 0, int nslot = 0) {
   if (a.stop && *a.stop) return;
   const int lane = threadIdx.x;
   const int rb = a.row_ptr[i], rlen = a.row_ptr[i + 1] - rb;
@@ -599,7 +619,19 @@ __device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_
     col_c[s] = j % 6;
     col_rhs[s] = j == ncol - 1;
   }
-  const int mb = a.m_off[i], me = a.m_off[i + 1];
+  const int mb = mode == 1 ? pb : a.m_off[i], me = mode == 1 ? pe : (mode == 2 ? mb : a.m_off[i + 1]);
+  if (mode == 2 && lane < 32) {
+    for (int k = 0; k < nslot; ++k) {
+      const float4 *src = reinterpret_cast<const float4 *>(a.part + (size_t)(slot + k) * part_floats(NS));
+#pragma unroll
+      for (int s2 = 0; s2 < NS; ++s2) {
+        const float4 v = src[s2 * 32 + lane];
+        C[s2][0] += v.x; C[s2][1] += v.y; C[s2][2] += v.z; C[s2][3] += v.w;
+      }
+      const float4 v = src[NS * 32 + lane];
+      Crhs[0] += v.x; Crhs[1] += v.y; Crhs[2] += v.z; Crhs[3] += v.w;
+    }
+  }
   for (int base = mb; base < me; base += 64) {
     const int e = base + lane;
     float *st = stage + lane * kStage;
@@ -698,6 +730,15 @@ __device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_
     }
     __syncthreads();
   }
+  if (mode == 1) {  // partial sums of this chunk
+    if (lane < 32) {
+      float4 *dst = reinterpret_cast<float4 *>(a.part + (size_t)slot * part_floats(NS));
+#pragma unroll
+      for (int s2 = 0; s2 < NS; ++s2) dst[s2 * 32 + lane] = make_float4(C[s2][0], C[s2][1], C[s2][2], C[s2][3]);
+      dst[NS * 32 + lane] = make_float4(Crhs[0], Crhs[1], Crhs[2], Crhs[3]);
+    }
+    return;
+  }
   // D[m][j] sits in lane j % 16 + 16 (m / 4), element m % 4: rows 0..5 are in
   // lanes 0..31.
   for (int j = lane; j < rlen * 36; j += 64) acc[j] = 0.f;
@@ -725,11 +766,53 @@ __device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_
   const float bacc = (ra >= 0 && ra < 6) ? s_rhs[ra] : 0.f;
   row_tail(a, i, rb, rlen, acc, stage, bacc);
 }
+// One workgroup per chunk b (grid: an upper bound on the chunk count, the
+// rest leave): a row of one chunk is assembled whole, a chunk of a longer row
+// leaves its partial sums for assemble_rows_tail_kernel.
+template <int NS, int PREC>
+__device__ inline void assemble_chunk(const AsmArgs &a, int N, int b, float *lds, float *s_rhs) {
+  if (b >= a.chunk_off[N]) return;
+  const int i = a.chunk_row[b];
+  const int c0 = a.chunk_off[i], np = a.chunk_off[i + 1] - c0;
+  if (np == 1) {
+    assemble_row_mfma<NS, PREC>(a, i, lds, s_rhs);
+  } else {
+    const int mb = a.m_off[i] + (b - c0) * kChunk;
+    assemble_row_mfma<NS, PREC>(a, i, lds, s_rhs, 1, mb, min(a.m_off[i + 1], mb + kChunk), b);
+  }
+}
+template <int NS, int PREC>
+__device__ inline void assemble_tail(const AsmArgs &a, int i, float *lds, float *s_rhs) {
+  const int c0 = a.chunk_off[i], np = a.chunk_off[i + 1] - c0;
+  if (np > 1) assemble_row_mfma<NS, PREC>(a, i, lds, s_rhs, 2, 0, 0, c0, np);
+}
 template <int NS, int PREC>
 __global__ __launch_bounds__(64) void assemble_rows_mfma_kernel(AsmArgs a) {
   extern __shared__ float lds[];
   __shared__ float s_rhs[6];
-  assemble_row_mfma<NS, PREC>(a, (int)blockIdx.x, lds, s_rhs);
+  assemble_chunk<NS, PREC>(a, a.N, (int)blockIdx.x, lds, s_rhs);
+}
+template <int NS, int PREC>
+__global__ __launch_bounds__(64) void assemble_rows_tail_kernel(AsmArgs a) {
+  extern __shared__ float lds[];
+  __shared__ float s_rhs[6];
+  assemble_tail<NS, PREC>(a, (int)blockIdx.x, lds, s_rhs);
+}
+
+// Chunk tables of the memberships (once per frame): cnt[i] = chunks of row
+// i, chunk_off = exclusive scan (exclusive_scan_kernel), chunk_row[c] = i.
+__device__ inline void chunk_count_body(const int32_t *__restrict__ m_off, int N, int32_t *__restrict__ cnt, int i) {
+  if (i < N) cnt[i] = max(1, (m_off[i + 1] - m_off[i] + kChunk - 1) / kChunk);
+}
+__device__ inline void chunk_fill_body(const int32_t *__restrict__ off, int N, int32_t *__restrict__ row, int i) {
+  if (i < N)
+    for (int c = off[i]; c < off[i + 1]; ++c) row[c] = i;
+}
+__global__ void chunk_count_kernel(const int32_t *__restrict__ m_off, int N, int32_t *__restrict__ cnt) {
+  chunk_count_body(m_off, N, cnt, blockIdx.x * blockDim.x + threadIdx.x);
+}
+__global__ void chunk_fill_kernel(const int32_t *__restrict__ off, int N, int32_t *__restrict__ row) {
+  chunk_fill_body(off, N, row, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ----------------------------------------------------------------- PCG
@@ -1638,8 +1721,26 @@ __global__ __launch_bounds__(256) void corr_prep_batch_kernel(const InstDev *__r
   a.canonical = fr.canonical[k];
   corr_prep_one(a, c, const_cast<float *>(a.crec));
 }
+// Chunks of all instances: workgroup b is chunk slot b - cbase[k] of the
+// instance k with cbase[k] <= b < cbase[k + 1].
 template <int NS, int PREC>
-__global__ __launch_bounds__(64) void assemble_rows_batch_kernel(const InstDev *__restrict__ inst, BatchCorrs fr,
+__global__ __launch_bounds__(64) void assemble_chunks_batch_kernel(const InstDev *__restrict__ inst, BatchCorrs fr,
+                                                                   int count) {
+  extern __shared__ float lds[];
+  __shared__ float s_rhs[6];
+  const int b = blockIdx.x;
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (fr.cbase[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  AsmArgs a = inst[lo].as;
+  a.corrs = fr.corrs[lo];
+  a.canonical = fr.canonical[lo];
+  assemble_chunk<NS, PREC>(a, a.N, b - fr.cbase[lo], lds, s_rhs);
+}
+template <int NS, int PREC>
+__global__ __launch_bounds__(64) void assemble_tail_batch_kernel(const InstDev *__restrict__ inst, BatchCorrs fr,
                                                                  const int32_t *__restrict__ row_inst,
                                                                  const int32_t *__restrict__ row_local) {
   extern __shared__ float lds[];
@@ -1648,7 +1749,19 @@ __global__ __launch_bounds__(64) void assemble_rows_batch_kernel(const InstDev *
   AsmArgs a = inst[k].as;
   a.corrs = fr.corrs[k];
   a.canonical = fr.canonical[k];
-  assemble_row_mfma<NS, PREC>(a, row_local[blockIdx.x], lds, s_rhs);
+  assemble_tail<NS, PREC>(a, row_local[blockIdx.x], lds, s_rhs);
+}
+__global__ void chunk_count_batch_kernel(const InstDev *__restrict__ inst) {
+  const InstDev &d = inst[blockIdx.y];
+  chunk_count_body(d.m_off, d.as.N, d.m_cnt, blockIdx.x * blockDim.x + threadIdx.x);
+}
+__global__ __launch_bounds__(1024) void chunk_scan_batch_kernel(const InstDev *__restrict__ inst) {
+  const InstDev &d = inst[blockIdx.x];
+  exclusive_scan_body(d.m_cnt, const_cast<int32_t *>(d.as.chunk_off), nullptr, d.as.N);
+}
+__global__ void chunk_fill_batch_kernel(const InstDev *__restrict__ inst) {
+  const InstDev &d = inst[blockIdx.y];
+  chunk_fill_body(d.as.chunk_off, d.as.N, const_cast<int32_t *>(d.as.chunk_row), blockIdx.x * blockDim.x + threadIdx.x);
 }
 __global__ void pcg_prep_batch_kernel(const InstDev *__restrict__ inst, int count) {
   const int k = threadIdx.x;
@@ -1812,6 +1925,9 @@ struct nrf_solver {
   int32_t *stop = nullptr;
   size_t c_stop = 0;
   float *crec = nullptr;  // per-corr records of the MFMA assembly
+  int32_t *chunk_off = nullptr, *chunk_row = nullptr;  // membership chunks of the MFMA assembly
+  float *part = nullptr;                                // partial sums of split rows
+  size_t c_chunk_off = 0, c_chunk_row = 0, c_part = 0;
   size_t c_crec = 0;
   float *rres = nullptr, *dref = nullptr;  // refinement: residual and correction
   size_t c_rres = 0, c_dref = 0;
@@ -1954,6 +2070,15 @@ static int solver_build_memberships(nrf_solver *s, const nrf_corr *corrs, int n,
   }
   mem_sort_kernel<<<N, kSortThreads, 0, st>>>(s->m_off, s->m_keys, s->m_keys2);
   NRF_LAUNCH_CHECK();
+  // chunk tables of the MFMA assembly (m_cnt is free again)
+  if ((rc = dev_alloc(s->chunk_off, (size_t)N + 1, s->c_chunk_off))) return rc;
+  if ((rc = dev_alloc(s->chunk_row, asm_max_chunks(N, n), s->c_chunk_row))) return rc;
+  chunk_count_kernel<<<(N + 255) / 256, 256, 0, st>>>(s->m_off, N, s->m_cnt);
+  NRF_LAUNCH_CHECK();
+  exclusive_scan_kernel<<<1, 1024, 0, st>>>(s->m_cnt, s->chunk_off, nullptr, N);
+  NRF_LAUNCH_CHECK();
+  chunk_fill_kernel<<<(N + 255) / 256, 256, 0, st>>>(s->chunk_off, N, s->chunk_row);
+  NRF_LAUNCH_CHECK();
   return NRF_OK;
 }
 
@@ -1987,6 +2112,9 @@ static AsmArgs solver_asm_args(const nrf_solver *s, const nrf_corr *corrs, const
   a.Minv = s->Minv;
   a.stop = s->stop;
   a.crec = nullptr;
+  a.chunk_off = s->chunk_off;
+  a.chunk_row = s->chunk_row;
+  a.part = s->part;
   return a;
 }
 
@@ -2011,14 +2139,24 @@ static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, int n, const fl
   }
   const int prec = s->params.jtj_precision;
   if (!mfma && prec != 0) return NRF_ERR_INVALID_ARGUMENT;  // reduced precision exists in the MFMA form only
+  const int nch = (int)asm_max_chunks(N, n);  // chunk slots: an upper bound, the others leave
+  if (mfma) {
+    int rc;
+    if ((rc = dev_alloc(s->part, (size_t)nch * part_floats(8), s->c_part))) return rc;
+    a.part = s->part;
+  }
 #define NRF_ASM_CASE(NSV)                                                                       \
   case NSV:                                                                                     \
-    if (prec == 1)                                                                              \
-      assemble_rows_mfma_kernel<NSV, 1><<<N, 64, lds, st>>>(a);                                 \
-    else if (prec == 2)                                                                         \
-      assemble_rows_mfma_kernel<NSV, 2><<<N, 64, lds, st>>>(a);                                 \
-    else                                                                                        \
-      assemble_rows_mfma_kernel<NSV, 0><<<N, 64, lds, st>>>(a);                                 \
+    if (prec == 1) {                                                                            \
+      assemble_rows_mfma_kernel<NSV, 1><<<nch, 64, lds, st>>>(a);                               \
+      assemble_rows_tail_kernel<NSV, 1><<<N, 64, lds, st>>>(a);                                 \
+    } else if (prec == 2) {                                                                     \
+      assemble_rows_mfma_kernel<NSV, 2><<<nch, 64, lds, st>>>(a);                               \
+      assemble_rows_tail_kernel<NSV, 2><<<N, 64, lds, st>>>(a);                                 \
+    } else {                                                                                    \
+      assemble_rows_mfma_kernel<NSV, 0><<<nch, 64, lds, st>>>(a);                               \
+      assemble_rows_tail_kernel<NSV, 0><<<N, 64, lds, st>>>(a);                                 \
+    }                                                                                           \
     break;
   switch (mfma ? ns : 0) {
     NRF_ASM_CASE(1)
@@ -2196,7 +2334,7 @@ extern "C" void nrf_solver_destroy(nrf_solver *s) {
                   s->rows, s->e_w, s->values, s->b, s->Minv, s->zpub, s->P0, s->P1, s->recs, s->ctl, s->lcol,
                   s->halo_off, s->halo,
                   s->m_cnt, s->m_off, s->m_cur, s->m_keys, s->m_keys2, s->stop, s->d_res, s->d_tot,
-                  s->crec, s->rres, s->dref, s->rpart, s->rnorm};
+                  s->crec, s->rres, s->dref, s->rpart, s->rnorm, s->chunk_off, s->chunk_row, s->part};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
@@ -2388,6 +2526,10 @@ int solver_reserve(nrf_solver *s, int max_corrs) {
   if ((rc = dev_alloc(s->m_keys, nc * 4, s->c_m_keys))) return rc;
   if ((rc = dev_alloc(s->m_keys2, nc * 4, s->c_m_keys2))) return rc;
   if ((rc = dev_alloc(s->rres, 6 * N, s->c_rres))) return rc;
+  if ((rc = dev_alloc(s->chunk_off, N + 1, s->c_chunk_off))) return rc;
+  const size_t nch = asm_max_chunks((int)N, (int)nc);
+  if ((rc = dev_alloc(s->chunk_row, nch, s->c_chunk_row))) return rc;
+  if ((rc = dev_alloc(s->part, nch * part_floats(8), s->c_part))) return rc;
   return dev_alloc(s->dref, 6 * N, s->c_dref);
 }
 
@@ -2559,6 +2701,12 @@ static int batch_memberships(SolverBatch *b, const BatchCorrs &fr, hipStream_t s
   }
   mem_sort_batch_kernel<<<b->rows, kSortThreads, 0, st>>>(b->d_inst, b->row_inst, b->row_local);
   NRF_LAUNCH_CHECK();
+  chunk_count_batch_kernel<<<gN, 256, 0, st>>>(b->d_inst);
+  NRF_LAUNCH_CHECK();
+  chunk_scan_batch_kernel<<<b->count, 1024, 0, st>>>(b->d_inst);
+  NRF_LAUNCH_CHECK();
+  chunk_fill_batch_kernel<<<gN, 256, 0, st>>>(b->d_inst);
+  NRF_LAUNCH_CHECK();
   gn_begin_batch_kernel<<<1, 64, 0, st>>>(b->d_inst, b->count);
   NRF_LAUNCH_CHECK();
   return NRF_OK;
@@ -2570,14 +2718,19 @@ static int batch_assemble(SolverBatch *b, const BatchCorrs &fr, hipStream_t st) 
     NRF_LAUNCH_CHECK();
   }
   const size_t lds = b->asm_lds;
+  const int nch = fr.cbase[b->count];
 #define NRF_ASMB_CASE(NSV)                                                                                    \
   case NSV:                                                                                                   \
-    if (b->prec == 1)                                                                                         \
-      assemble_rows_batch_kernel<NSV, 1><<<b->rows, 64, lds, st>>>(b->d_inst, fr, b->row_inst, b->row_local); \
-    else if (b->prec == 2)                                                                                    \
-      assemble_rows_batch_kernel<NSV, 2><<<b->rows, 64, lds, st>>>(b->d_inst, fr, b->row_inst, b->row_local); \
-    else                                                                                                      \
-      assemble_rows_batch_kernel<NSV, 0><<<b->rows, 64, lds, st>>>(b->d_inst, fr, b->row_inst, b->row_local); \
+    if (b->prec == 1) {                                                                                       \
+      assemble_chunks_batch_kernel<NSV, 1><<<nch, 64, lds, st>>>(b->d_inst, fr, b->count);                    \
+      assemble_tail_batch_kernel<NSV, 1><<<b->rows, 64, lds, st>>>(b->d_inst, fr, b->row_inst, b->row_local); \
+    } else if (b->prec == 2) {                                                                                \
+      assemble_chunks_batch_kernel<NSV, 2><<<nch, 64, lds, st>>>(b->d_inst, fr, b->count);                    \
+      assemble_tail_batch_kernel<NSV, 2><<<b->rows, 64, lds, st>>>(b->d_inst, fr, b->row_inst, b->row_local); \
+    } else {                                                                                                  \
+      assemble_chunks_batch_kernel<NSV, 0><<<nch, 64, lds, st>>>(b->d_inst, fr, b->count);                    \
+      assemble_tail_batch_kernel<NSV, 0><<<b->rows, 64, lds, st>>>(b->d_inst, fr, b->row_inst, b->row_local); \
+    }                                                                                                         \
     break;
   switch (b->ns) {
     NRF_ASMB_CASE(1)
