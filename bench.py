@@ -117,6 +117,8 @@ def main() -> int:
     ap.add_argument("--pcg-form", type=int, choices=[1, 2], default=2,
                     help="1: A p = A z + beta A p with the product overlapping the (r.r, r.z) all-reduce; 2: also "
                          "p.Ap from z.Az, one all-reduce per iteration (Chronopoulos-Gear)")
+    ap.add_argument("--pcg-layout", default="", metavar="LANES,NODES",
+                    help="PCG lanes per scalar row (1, 2, 4) and target nodes per workgroup (default 4,40)")
     ap.add_argument("--precision", choices=["bf16", "fp16"], default=None,
                     help="precision form: FP16 TSDF + u8 weight, compact kNN, JtJ blocks in BF16 / FP16 on the "
                          "matrix cores, FP64 Cholesky inverses, one FP64-residual refinement step per solve; "
@@ -154,6 +156,9 @@ def main() -> int:
                 pcg_pipelined=0 if args.textbook_pcg else args.pcg_form, jtj_precision=prec, refine_iterations=10 if prec else 0,
                 precond_fp64=bool(prec), storage16=bool(prec))
     h.setup()
+    if args.pcg_layout:
+        lanes, wg_nodes = (int(v) for v in args.pcg_layout.split(","))
+        L.check(h.lib.nrf_solver_set_pcg_layout(h.lib.nrf_pipeline_solver(h.pipe), lanes, wg_nodes), "pcg_layout")
     log(f"[bench] setup {time.perf_counter() - t0:.1f} s, {h.N} nodes, "
         f"{int(h.frames_host[1].valid.sum())} valid pixels in frame 1")
     nf = args.frames
@@ -257,6 +262,7 @@ def main() -> int:
                    "graph": bool(args.graph),
                    "solver_policy": ("reference (ARAP sign, whole-step rejection)" if ref else
                                      "consistent ARAP sign, per-node clamp, no whole-step rejection"),
+                   "pcg_layout": args.pcg_layout or "4,40",
                    "pcg": "textbook" if args.textbook_pcg else
                           {1: "overlapped (A z + beta A p)", 2: "single reduction (Chronopoulos-Gear)"}[args.pcg_form]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -1940,7 +1940,7 @@ struct nrf_solver {
   int cus = 0, max_lds = 0, coop = 0;
   int forced_G = 0;
   int pcg_sub = 4;        // lanes per scalar row of the PCG (1, 2, 4)
-  int pcg_wg_nodes = 40;  // target nodes per PCG workgroup
+  int pcg_wg_nodes = 24;  // target nodes per PCG workgroup (measured best on the full-scale body: 20-24)
   int asm_mfma = 1;  // data term on the matrix cores (rows of up to 21 blocks)
   unsigned gen = 0;  // advanced by every partition rebuild
   // optional profiling events (see solver_set_event_sink)

@@ -309,7 +309,8 @@ const nrf_pcg_result *nrf_solver_device_result(const nrf_solver *s);
 /* Layout of the persistent PCG (no reference counterpart): lanes per scalar
    row (1, 2 or 4) and the target number of nodes per workgroup (capped at
    1024 / (6 lanes_per_row)); rebuilds the partition of the current pattern.
-   The textbook PCG form (pcg_pipelined = 0) needs 4 lanes per row. */
+   Default: 4 lanes per row, 24 nodes per workgroup.  The textbook PCG form
+   (pcg_pipelined = 0) needs 4 lanes per row. */
 int nrf_solver_set_pcg_layout(nrf_solver *s, int lanes_per_row, int wg_nodes);
 
 /* PCG alone on the last assembled system (A, b, block-Jacobi inverse), with
