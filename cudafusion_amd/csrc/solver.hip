@@ -52,28 +52,6 @@ __global__ void fill_i32(int32_t *p, int n, int32_t v) {
 // count per node, exclusive scan, scatter, then each node's bucket is
 // ordered by key (corr * 4 + slot), which fixes the order in which the
 // assembly sums a row's contributions.
-//
-// Appends one entry per lane whose id >= 0 to bucket `id`: the lanes of the
-// wave that share an id take consecutive positions from one atomic issued by
-// a leader lane, so a node seen by many adjacent pixels costs one atomic per
-// wave instead of one per pixel.  Returns the lane's position (-1 if none).
-// Every lane of the wave must call it.
-__device__ inline int wave_claim(int32_t *ctr, int id) {
-  const int lane = threadIdx.x & 63;
-  unsigned long long active = __ballot(id >= 0);
-  int pos = -1;
-  while (active) {
-    const int leader = __ffsll((unsigned long long)active) - 1;
-    const int v = __shfl(id, leader, 64);
-    const unsigned long long grp = __ballot(id == v) & active;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&ctr[v], __popcll(grp));
-    base = __shfl(base, leader, 64);
-    if ((grp >> lane) & 1ull) pos = base + __popcll(grp & ((1ull << lane) - 1ull));
-    active &= ~grp;
-  }
-  return pos;
-}
 
 // The node of slot k of a valid corr, or -1 when the slot takes no part
 // (kernels/solver_kernels.cu:94-96: id in range and weight > 1e-8).
@@ -82,11 +60,46 @@ __device__ inline int member_node(const nrf_corr &cr, int k, int N) {
   return (id >= 0 && id < N && cr.node_ws[k] > 1e-8f) ? id : -1;
 }
 
+// Count and scatter aggregate a 256-lane workgroup's claims in an LDS hash
+// table first (one entry per distinct node, at most 1024: four slots per
+// lane), so a node seen by many of the workgroup's pixels costs one global
+// atomic per workgroup, and the workgroup's global atomics are issued in
+// parallel.  Positions within a bucket depend on the atomics' order; the
+// per-bucket sort that follows fixes the order.
+constexpr int kMemThreads = 256, kMemHash = 4 * kMemThreads;
+struct MemHash {
+  int32_t key[kMemHash], cnt[kMemHash], base[kMemHash];
+};
+__device__ inline void mem_hash_clear(MemHash &hs) {
+  for (int j = threadIdx.x; j < kMemHash; j += kMemThreads) {
+    hs.key[j] = -1;
+    hs.cnt[j] = 0;
+  }
+  __syncthreads();
+}
+// slot of id in the table (inserted if absent); linear probing always ends
+// since the table holds more entries than the workgroup can insert
+__device__ inline int mem_hash_slot(MemHash &hs, int id) {
+  int h = (int)(((unsigned)id * 2654435761u) >> 22) & (kMemHash - 1);
+  for (;;) {
+    const int prev = atomicCAS(&hs.key[h], -1, id);
+    if (prev == -1 || prev == id) return h;
+    h = (h + 1) & (kMemHash - 1);
+  }
+}
 __device__ inline void mem_count_body(const nrf_corr *__restrict__ corrs, int n, int N, int32_t *__restrict__ cnt,
                                       int c) {
+  __shared__ MemHash hs;
+  mem_hash_clear(hs);
   const bool ok = c < n && corrs[c].valid;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) wave_claim(cnt, ok ? member_node(corrs[c], k, N) : -1);
+  for (int k = 0; k < 4; ++k) {
+    const int id = ok ? member_node(corrs[c], k, N) : -1;
+    if (id >= 0) atomicAdd(&hs.cnt[mem_hash_slot(hs, id)], 1);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < kMemHash; j += kMemThreads)
+    if (hs.key[j] >= 0) atomicAdd(&cnt[hs.key[j]], hs.cnt[j]);
 }
 __global__ __launch_bounds__(256) void mem_count_kernel(const nrf_corr *__restrict__ corrs, int n, int N,
                                                         int32_t *__restrict__ cnt) {
@@ -125,12 +138,27 @@ __global__ __launch_bounds__(1024) void exclusive_scan_kernel(const int32_t *__r
 
 __device__ inline void mem_scatter_body(const nrf_corr *__restrict__ corrs, int n, int N,
                                         int32_t *__restrict__ cur, int32_t *__restrict__ keys, int c) {
+  __shared__ MemHash hs;
+  mem_hash_clear(hs);
   const bool ok = c < n && corrs[c].valid;
+  int slot[4], rank[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const int pos = wave_claim(cur, ok ? member_node(corrs[c], k, N) : -1);
-    if (pos >= 0) keys[pos] = c * 4 + k;
+    const int id = ok ? member_node(corrs[c], k, N) : -1;
+    slot[k] = -1;
+    rank[k] = 0;
+    if (id >= 0) {
+      slot[k] = mem_hash_slot(hs, id);
+      rank[k] = atomicAdd(&hs.cnt[slot[k]], 1);
+    }
   }
+  __syncthreads();
+  for (int j = threadIdx.x; j < kMemHash; j += kMemThreads)
+    if (hs.key[j] >= 0) hs.base[j] = atomicAdd(&cur[hs.key[j]], hs.cnt[j]);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (slot[k] >= 0) keys[hs.base[slot[k]] + rank[k]] = c * 4 + k;
 }
 __global__ __launch_bounds__(256) void mem_scatter_kernel(const nrf_corr *__restrict__ corrs, int n, int N,
                                                           int32_t *__restrict__ cur, int32_t *__restrict__ keys) {
@@ -1690,7 +1718,7 @@ __global__ __launch_bounds__(256) void mem_fill_batch_kernel(const InstDev *__re
 }
 __global__ __launch_bounds__(256) void mem_count_batch_kernel(const InstDev *__restrict__ inst, BatchCorrs fr) {
   const int k = blockIdx.y, n = fr.n[k];
-  if ((int)blockIdx.x * 256 >= n) return;  // whole workgroup: wave_claim needs every lane
+  if ((int)blockIdx.x * 256 >= n) return;  // whole workgroup: every lane reaches the barriers
   mem_count_body(fr.corrs[k], n, inst[k].as.N, inst[k].m_cnt, blockIdx.x * 256 + threadIdx.x);
 }
 __global__ __launch_bounds__(1024) void exclusive_scan_batch_kernel(const InstDev *__restrict__ inst) {
