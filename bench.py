@@ -353,7 +353,8 @@ def pcg_to_tolerance(h, rel_tol, reps=20):
 
 def precision_errors(h, f):
     """One more (untimed) frame f, and the same frame from the same state
-    run by the FP64 oracle: node transforms (max rotation / translation
+    run by the FP64 oracle (with the device's PCG + refinement iterations as
+    its PCG iterations per GN step): node transforms (max rotation / translation
     difference), TSDF (max difference on voxels away from a rounding
     boundary of the FP64 projection, in truncation units) and vertices
     (largest distance from a device vertex to the nearest FP64 vertex, and
@@ -374,8 +375,13 @@ def precision_errors(h, f):
     vox1 = h.download_voxels()
     mesh = h.mesh().copy()
     corrs, can = h.frame_corrs_host(f)
+    # the FP64 solve runs as many PCG iterations per GN step as the device's
+    # PCG and refinement together
+    from cudafusion_amd import _lib as L
+    sp64 = L.SolverParams.from_buffer_copy(bytes(h.sp))
+    sp64.pcg_iterations = h.sp.pcg_iterations + h.sp.refine_iterations
     T64, _, _, _, _ = po.solve_gn_f64(corrs, can, h.nodes, T0.view(np.float32).reshape(-1, 8).astype(np.float64),
-                                      h.gn, h.sp)
+                                      h.gn, sp64)
     er, et = transform_errors(T1, T64)
     t64, w64 = vox0["tsdf"].astype(np.float64), vox0["weight"].astype(np.float64)
     depth = h.frames_host[f].depth.reshape(-1)
