@@ -118,7 +118,7 @@ def main() -> int:
                     help="1: A p = A z + beta A p with the product overlapping the (r.r, r.z) all-reduce; 2: also "
                          "p.Ap from z.Az, one all-reduce per iteration (Chronopoulos-Gear)")
     ap.add_argument("--pcg-layout", default="", metavar="LANES,NODES",
-                    help="PCG lanes per scalar row (1, 2, 4) and target nodes per workgroup (default 4,40)")
+                    help="PCG lanes per scalar row (1, 2, 4) and target nodes per workgroup (default 4,24)")
     ap.add_argument("--precision", choices=["bf16", "fp16"], default=None,
                     help="precision form: FP16 TSDF + u8 weight, compact kNN, JtJ blocks in BF16 / FP16 on the "
                          "matrix cores, FP64 Cholesky inverses, one FP64-residual refinement step per solve; "
@@ -262,7 +262,7 @@ def main() -> int:
                    "graph": bool(args.graph),
                    "solver_policy": ("reference (ARAP sign, whole-step rejection)" if ref else
                                      "consistent ARAP sign, per-node clamp, no whole-step rejection"),
-                   "pcg_layout": args.pcg_layout or "4,40",
+                   "pcg_layout": args.pcg_layout or "4,24",
                    "pcg": "textbook" if args.textbook_pcg else
                           {1: "overlapped (A z + beta A p)", 2: "single reduction (Chronopoulos-Gear)"}[args.pcg_form]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
