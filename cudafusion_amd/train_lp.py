@@ -108,7 +108,7 @@ def _h2d_dev(dst_ptr: int, src: torch.Tensor) -> None:
         raise L.NrfError(f"hipMemcpyAsync D2D failed ({rc})")
 
 
-def pcg_solve(lib, solver, sp0, x: torch.Tensor, tol: float, maxit: int, timed: bool = False):
+def pcg_solve(lib, solver, sp0, x: torch.Tensor, tol: float, maxit: int):
     """The solver's PCG (its current preconditioner) on its current system to
     a relative residual; returns (iterations, true relative residual, ms)."""
     sp = L.SolverParams.from_buffer_copy(bytes(sp0))

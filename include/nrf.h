@@ -310,7 +310,9 @@ const nrf_pcg_result *nrf_solver_device_result(const nrf_solver *s);
    row (1, 2 or 4) and the target number of nodes per workgroup (capped at
    1024 / (6 lanes_per_row)); rebuilds the partition of the current pattern.
    Default: 4 lanes per row, 24 nodes per workgroup.  The textbook PCG form
-   (pcg_pipelined = 0) needs 4 lanes per row. */
+   (pcg_pipelined = 0) needs 4 lanes per row.  Synchronous; no solve of this
+   solver may be in flight, and a pipeline captured into a graph is captured
+   again at its next step. */
 int nrf_solver_set_pcg_layout(nrf_solver *s, int lanes_per_row, int wg_nodes);
 
 /* PCG alone on the last assembled system (A, b, block-Jacobi inverse), with

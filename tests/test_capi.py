@@ -76,3 +76,23 @@ def test_struct_layouts_match_the_header(tmp_path):
         assert int(got[cname]) == C.sizeof(py), cname
         for f, _ in py._fields_:
             assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, f"{cname}.{f}"
+
+
+def test_host_checks_without_device():
+    """Entry points that reject their arguments on the host, before any
+    device call; the learned preconditioner's parameter count equals the
+    oracle's definition."""
+    from cudafusion_amd import _lib
+    from oracle import pyoracle as po
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libnrf.so not built")
+    lib = _lib.load()
+    assert lib.nrf_lp_param_count() == po.lp_param_count() == 46025
+    assert not lib.nrf_batch_create(None, 0, None, 0)
+    assert not lib.nrf_batch_create(None, 40, None, 0)
+    assert not lib.nrf_lp_create(2, 100, 1)  # no such variant
+    assert not lib.nrf_lp_create(0, 0, 1)
+    assert lib.nrf_batch_step(None, None, None) == 1
+    assert lib.nrf_lp_adam(None, 1e-3, 1, None) == 1
+    sp = _lib.solver_params(5, 30, 0.0, 10.0, 1.0, 1e-2, 0, 0, 0, 0, 3)  # pcg_pipelined 3 is not a form
+    assert not lib.nrf_solver_create(C.byref(sp), 100)
