@@ -617,7 +617,7 @@ static int batch_tune(nrf_batch *b, const BatchCorrs &fc, const BatchDepth &fd, 
   struct Cand {
     int lanes, nodes;
   };
-  const Cand cands[] = {{4, 24}, {4, 32}, {4, 40}, {2, 48}, {2, 64}, {2, 84}, {1, 96}, {1, 128}, {1, 170}};
+  const Cand cands[] = {{4, 24}, {4, 32}, {2, 48}, {2, 64}, {2, 80}, {1, 96}, {1, 128}, {1, 170}};
   best.pcg_ms = -1.f;
   for (const Cand &c : cands) {
     SolverBatch *sb = nullptr;

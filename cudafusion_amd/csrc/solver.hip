@@ -857,8 +857,12 @@ __global__ void chunk_fill_kernel(const int32_t *__restrict__ off, int N, int32_
 // to, (launch epoch << 16) + point, compared for equality.
 typedef unsigned long long ll_word;
 
-struct alignas(64) SyncRec {
-  ll_word w[2][6];  // per parity: hi / lo halves of up to three partial sums
+// One workgroup's all-reduce record: per parity the hi / lo halves of up to
+// three partial sums, each parity in a 64-byte line of its own.  (Records of
+// different workgroups in shared lines, word-major, measured slower: the
+// writers of one line contend.)
+struct alignas(128) SyncRec {
+  ll_word w[2][8];
 };
 struct PcgCtl {
   unsigned long long epoch;  // advanced once per launch by pcg_prep_kernel
@@ -885,6 +889,26 @@ __device__ inline double dpp_add(double v) {
   const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWMASK, 0xF, false);
   const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, 0xF, false);
   return v + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// A float from another lane of the same quad (DPP quad_perm; 0xB1 reads
+// lane ^ 1, 0x4E lane ^ 2).  Every lane of the wave must take part.
+template <int CTRL>
+__device__ inline float quad_perm_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+// wave_sum_last for values that are equal in each aligned group of SUB
+// lanes (SUB = 1, 2, 4), counted once per group: the quad steps that would
+// add a group's copies are skipped.  Same bits as wave_sum_last over the
+// values kept in the group's first lane and zeros elsewhere.
+template <int SUB>
+__device__ inline double wave_sum_groups(double v) {
+  if constexpr (SUB == 1) v = dpp_add<0xB1, 0xF>(v);  // quad_perm [1,0,3,2]
+  if constexpr (SUB <= 2) v = dpp_add<0x4E, 0xF>(v);  // quad_perm [2,3,0,1]
+  v = dpp_add<0x141, 0xF>(v);  // row_half_mirror
+  v = dpp_add<0x140, 0xF>(v);  // row_mirror
+  v = dpp_add<0x142, 0xA>(v);  // row_bcast:15 into rows 1, 3
+  v = dpp_add<0x143, 0xC>(v);  // row_bcast:31 into rows 2, 3
+  return v;
 }
 __device__ inline double wave_sum_last(double v) {
   v = dpp_add<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
@@ -1029,18 +1053,25 @@ __device__ inline bool allreduce2(const PcgArgs &a, unsigned tag, int slot, doub
 }
 
 // The same all-reduce for three doubles (lanes 0..5 publish the hi / lo
-// words; the sums follow the fixed orders of the two-value form).
-__device__ inline void ar3_publish(const PcgArgs &a, int g, unsigned tag, int slot, double (&v)[3], double *red) {
-  double s[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) s[k] = wave_sum_last(v[k]);
+// words; the sums follow the fixed orders of the two-value form).  The
+// partials are equal in each group of SUB lanes (a row's lanes) and counted
+// once per group; only the first nwa waves hold rows (the others would add
+// zeros and are skipped).
+template <int SUB>
+__device__ inline void ar3_publish(const PcgArgs &a, int g, unsigned tag, int slot, double (&v)[3], double *red,
+                                   int nwa) {
   const int t = threadIdx.x, w = t >> 6;
-  if ((t & 63) == 63)
-    for (int k = 0; k < 3; ++k) red[3 * w + k] = s[k];
+  if (w < nwa) {
+    double s[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s[k] = wave_sum_groups<SUB>(v[k]);
+    if ((t & 63) == 63)
+      for (int k = 0; k < 3; ++k) red[3 * w + k] = s[k];
+  }
   __syncthreads();
   if (t < 6) {
     double acc = 0;
-    for (int kk = 0; kk < kPcgWaves; ++kk) acc += red[3 * kk + (t >> 1)];
+    for (int kk = 0; kk < nwa; ++kk) acc += red[3 * kk + (t >> 1)];
     const ll_word bits = (ll_word)__double_as_longlong(acc);
     ll_put(&a.recs[g].w[slot][t], tag, (t & 1) ? (unsigned)bits : (unsigned)(bits >> 32));
   }
@@ -1270,22 +1301,39 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
   const int h0 = a.halo_off[g], nh = a.halo_off[g + 1] - h0;
   float *Al = lds;
   int32_t *Ll = reinterpret_cast<int32_t *>(Al + (a.lds_a ? (size_t)nblk * 36 : 0));
+  // the halo's values, then one zero entry (index nh) that the register
+  // blocks of short rows point at
   float *ph = reinterpret_cast<float *>(Ll + (a.lds_a ? nblk : 0));
-  float *rl = ph + (size_t)nh * 6;  // own rows' r, then own rows' z (ROWS each)
-  float *zl = rl + ROWS;
+  float *rl = ph + (size_t)(nh + 1) * 6;  // own rows' r (ROWS entries)
   if (a.lds_a) {
     const float4 *src = reinterpret_cast<const float4 *>(a.values + (size_t)blk0 * 36);
     float4 *dst = reinterpret_cast<float4 *>(Al);
     for (int j = t; j < nblk * 9; j += kPcgThreads) dst[j] = src[j];
     for (int j = t; j < nblk; j += kPcgThreads) Ll[j] = a.lcol[blk0 + j];
   }
+  if (t < 6) ph[nh * 6 + t] = 0.f;
+  // the own nodes are a contiguous run of the sorted halo list, from s_o0
+  __shared__ int s_o0;
+  for (int e = t; e < nh; e += kPcgThreads)
+    if (a.halo[h0 + e] == n0) s_o0 = e;
   if (t == 0) {
     s_base = (unsigned)(__hip_atomic_load(&a.ctl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) << 16);
     s_bad = 0;
   }
-  const int q = t / SUB, sub = t % SUB;
-  const bool own = q < S, lead = own && sub == 0;
-  const int li = q / 6, row = q % 6, i = n0 + li, s = 6 * n0 + q;
+  // Lanes (SUB = 2, 4): each wave holds NPW whole nodes (6 SUB lanes each,
+  // the rest of the wave idle), so that a node's six r values are exchanged
+  // inside one wave, through LDS without a barrier.  Node li of the
+  // workgroup sits in wave li / NPW; its row `row` in the SUB lanes from
+  // (li % NPW) 6 SUB + row SUB.  SUB = 1 packs the rows contiguously (170
+  // nodes in 1020 lanes, nodes straddle waves) and exchanges r behind a
+  // barrier.
+  constexpr bool WN = SUB != 1;
+  constexpr int NPW = WN ? 64 / (6 * SUB) : 64;
+  const int ln = WN ? (t & 63) / (6 * SUB) : 0, lr = WN ? (t & 63) % (6 * SUB) : t % SUB;
+  const int q = WN ? ((t >> 6) * NPW + ln) * 6 + lr / SUB : t / SUB;
+  const int li = q / 6, row = q % 6, sub = lr % SUB;
+  const bool own = ln < NPW && q < S, lead = own && sub == 0;
+  const int i = n0 + li, s = 6 * n0 + q;
   float M[6] = {0, 0, 0, 0, 0, 0};
   float bv = 0.f;
   int rb = 0, re = 0;
@@ -1295,30 +1343,55 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
     rb = a.row_ptr[i];
     re = a.row_ptr[i + 1];
   }
-  // z = M^-1 r for this lane's row, published under `tag`; z of the owned rows
-  // is left in zl for the product
+  // The lane's first NR blocks of its row (bk = rb + sub + k SUB) stay in
+  // registers for the whole solve: their six coefficients and their halo
+  // column.  Missing blocks are zero and point at the zero entry, so the
+  // product reads every register block without a branch (adding the exact
+  // zeros leaves the sum's bits unchanged).  Blocks past NR (rows longer
+  // than NR SUB blocks) are read from LDS or global memory.
+  constexpr int NR = SUB == 4 ? 3 : (SUB == 2 ? 5 : 4);
+  float Ar[NR][6];
+  int Lr[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    const int bk = rb + sub + k * SUB;
+    const bool have = own && bk < re;
+    Lr[k] = have ? a.lcol[bk] * 6 : nh * 6;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) Ar[k][c] = have ? a.values[(size_t)bk * 36 + row * 6 + c] : 0.f;
+  }
+  const int bk_rest = rb + sub + NR * SUB;
+  // the node of this lane's first halo entry (e = t), read once
+  const int hn0 = t < nh * 6 ? a.halo[h0 + t / 6] : -1;
+  int zo = 0;  // ph index of this lane's row among the own nodes' entries
+  // z = M^-1 r for this lane's row, published under `tag` and stored in the
+  // own nodes' entries of ph for the product.  A node's six r values come
+  // from lanes of the same wave: LDS store and loads in program order, no
+  // barrier.
   auto precond_publish = [&](float r, unsigned tag) -> float {
-    if (sub == 0) rl[q] = r;
-    __syncthreads();
+    if (lead) rl[q] = r;
+    if constexpr (WN)
+      __builtin_amdgcn_wave_barrier();  // keeps the loads after the store
+    else
+      __syncthreads();
     float z = 0.f;
     if (own) {
       for (int c = 0; c < 6; ++c) z += M[c] * rl[li * 6 + c];
       if (lead) {
         ll_put(a.zpub + s, tag, __float_as_uint(z));
-        zl[q] = z;
+        ph[zo] = z;
       }
     }
     return z;
   };
-  // (A z) for this lane's row: owned rows' z from zl, the halo's from the
-  // words published under `tag`
+  // (A z) for this lane's row: the halo's z from the words published under
+  // `tag` (own nodes' entries are already in ph).  The callers order the
+  // gather after every read of ph by the previous product (a barrier lies
+  // between).
   auto product = [&](unsigned tag) -> float {
-    __syncthreads();
     for (int e = t; e < nh * 6; e += kPcgThreads) {
-      const int nd = a.halo[h0 + e / 6], c = e % 6;
-      if (nd >= n0 && nd < n1) {
-        ph[e] = zl[(nd - n0) * 6 + c];
-      } else {
+      const int nd = e == t ? hn0 : a.halo[h0 + e / 6], c = e % 6;
+      if (nd < n0 || nd >= n1) {
         const ll_word *src = a.zpub + 6 * nd + c;
         ll_word wv;
         unsigned spins = 0;
@@ -1332,40 +1405,58 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
     }
     __syncthreads();
     float y = 0.f;
-    if (own) {
-      for (int bk = rb + sub; bk < re; bk += SUB) {
-        const int lc = a.lds_a ? Ll[bk - blk0] : a.lcol[bk];
-        const float *A = a.lds_a ? (Al + (size_t)(bk - blk0) * 36) : (a.values + (size_t)bk * 36);
-        const float *pj = ph + lc * 6;
-        for (int c = 0; c < 6; ++c) y += A[row * 6 + c] * pj[c];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      const float *pj = ph + Lr[k];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) y += Ar[k][c] * pj[c];
+    }
+    if (own && bk_rest < re) {
+      if (a.lds_a) {
+        for (int bk = bk_rest; bk < re; bk += SUB) {
+          const float *A = Al + (size_t)(bk - blk0) * 36 + row * 6;
+          const float *pj = ph + Ll[bk - blk0] * 6;
+          for (int c = 0; c < 6; ++c) y += A[c] * pj[c];
+        }
+      } else {
+        for (int bk = bk_rest; bk < re; bk += SUB) {
+          const float *A = a.values + (size_t)bk * 36 + row * 6;
+          const float *pj = ph + a.lcol[bk] * 6;
+          for (int c = 0; c < 6; ++c) y += A[c] * pj[c];
+        }
       }
     }
-    if constexpr (SUB >= 2) y += __shfl_xor(y, 1, 64);
-    if constexpr (SUB >= 4) y += __shfl_xor(y, 2, 64);
+    // the row's SUB partial products, combined identically in all its lanes
+    if constexpr (SUB >= 2) y += quad_perm_f<0xB1>(y);  // lane ^ 1
+    if constexpr (SUB >= 4) y += quad_perm_f<0x4E>(y);  // lane ^ 2
     return y;
   };
 
   __syncthreads();
   const unsigned base = s_base;
+  zo = (s_o0 + li) * 6 + row;
   if constexpr (CG1) {
     // One all-reduce per iteration (Chronopoulos and Gear, 1989): with
     // gamma = r.z and delta = z.Az reduced together with r.r, p.Ap follows
     // from delta - beta gamma / alpha_old (equal to p.Ap in exact
     // arithmetic, since z_{k+1}.r_k = 0 and p_k.Ap_k = gamma_k / alpha_k),
-    // so A z no longer waits for a p.Ap reduction.  A p is formed as
-    // A z + beta A p as in the overlapped form.
+    // so A z no longer waits for a p.Ap reduction.  With beta = gamma /
+    // gamma_old the correction is beta^2 (p.Ap)_old, which saves a division.
+    // A p is formed as A z + beta A p as in the overlapped form.
     unsigned pt = 1;
     float x = 0.f, r = bv;
     float z = precond_publish(r, base + pt);
     float az = product(base + pt);
-    double v[3] = {lead ? (double)r * (double)z : 0.0, lead ? (double)z * (double)az : 0.0,
-                   lead ? (double)bv * (double)bv : 0.0};
-    ar3_publish(a, g, base + pt, (int)(pt & 1), v, red);
+    // the partials are held by all SUB lanes of a row (equal copies)
+    const int nwa = WN ? (n1 - n0 + NPW - 1) / NPW : (S * SUB + 63) >> 6;
+    double v[3] = {own ? (double)r * (double)z : 0.0, own ? (double)z * (double)az : 0.0,
+                   own ? (double)bv * (double)bv : 0.0};
+    ar3_publish<SUB>(a, g, base + pt, (int)(pt & 1), v, red, nwa);
     bool aborted = ar3_collect(a, base + pt, (int)(pt & 1), v, red, &s_bad);
     double gamma = v[0], pAp = v[1];
     const double bb = v[2];
     double rr = bb;
-    const double tol = a.rel ? (double)a.tol * sqrt(bb) : (double)a.tol;
+    const double tol = a.rel ? (double)a.tol * sqrt(bb) : (double)a.tol, tol2 = tol * tol;
     int iters = 0, flags = 0;
     float p = z, y = az;
     for (int it = 0; it < a.maxit && !aborted; ++it) {
@@ -1380,21 +1471,21 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
       ++pt;
       z = precond_publish(r, base + pt);
       az = product(base + pt);
-      v[0] = lead ? (double)r * (double)r : 0.0;
-      v[1] = lead ? (double)r * (double)z : 0.0;
-      v[2] = lead ? (double)z * (double)az : 0.0;
-      ar3_publish(a, g, base + pt, (int)(pt & 1), v, red);
+      v[0] = own ? (double)r * (double)r : 0.0;
+      v[1] = own ? (double)r * (double)z : 0.0;
+      v[2] = own ? (double)z * (double)az : 0.0;
+      ar3_publish<SUB>(a, g, base + pt, (int)(pt & 1), v, red, nwa);
       aborted = ar3_collect(a, base + pt, (int)(pt & 1), v, red, &s_bad);
       if (aborted) break;
       rr = v[0];
       iters = it + 1;
-      if (sqrt(rr) < tol) {
+      if (rr < tol2) {
         flags |= 2;
         break;
       }
       const double beta = v[1] / gamma;
       gamma = v[1];
-      pAp = v[2] - beta * gamma / alpha;
+      pAp = v[2] - beta * beta * pAp;  // beta gamma / alpha_old = beta^2 (p.Ap)_old
       const float betaf = (float)beta;
       p = z + betaf * p;
       y = az + betaf * y;
@@ -2007,8 +2098,9 @@ static int solver_set_pattern_host(nrf_solver *s, const nrf_node *h, int N) {
   return solver_build_partition(s);
 }
 
-// PCG partition: contiguous node ranges of at most 1024 / (6 pcg_sub) nodes
-// (pcg_sub lanes per scalar row), each with its halo (the distinct columns of
+// PCG partition: contiguous node ranges of at most 32, 80 or 170 nodes for
+// pcg_sub = 4, 2, 1 lanes per scalar row (whole nodes per wave for 4 and 2),
+// each with its halo (the distinct columns of
 // its rows) and the blocks' columns renumbered into that halo.  The target
 // size (pcg_wg_nodes) trades workgroups for rows per workgroup.
 static int solver_build_partition(nrf_solver *s) {
@@ -2016,7 +2108,9 @@ static int solver_build_partition(nrf_solver *s) {
   const Pattern &P = s->P;
   const int N = P.N;
   int st;
-  const int max_nodes = kPcgThreads / s->pcg_sub / 6;
+  // 2 and 4 lanes per row: whole nodes per wave (64 / (6 pcg_sub)) times the
+  // waves; 1 lane per row: rows packed over all lanes
+  const int max_nodes = s->pcg_sub == 1 ? kPcgThreads / 6 : kPcgWaves * (64 / (6 * s->pcg_sub));
   const int target = std::max(1, std::min(max_nodes, s->pcg_wg_nodes));
   int G = s->forced_G;
   const int min_G = std::max(1, (N + max_nodes - 1) / max_nodes);
@@ -2036,7 +2130,7 @@ static int solver_build_partition(nrf_solver *s) {
     hoff[g] = (int32_t)halo.size();
     halo.insert(halo.end(), h.begin(), h.end());
     const size_t nblk = (size_t)(b1 - b0);
-    need_max = std::max(need_max, nblk * (36 * 4 + 4) + h.size() * 24 + (size_t)kPcgThreads * 8);
+    need_max = std::max(need_max, nblk * (36 * 4 + 4) + (h.size() + 1) * 24 + (size_t)kPcgThreads * 8);
   }
   hoff[G] = (int32_t)halo.size();
   size_t halo_max = 0;
@@ -2044,7 +2138,7 @@ static int solver_build_partition(nrf_solver *s) {
   s->G = G;
   const size_t budget = (size_t)std::max(0, s->max_lds - 1024);
   s->lds_a = need_max <= budget ? 1 : 0;
-  s->pcg_lds = align_up(s->lds_a ? need_max : halo_max * 24 + (size_t)kPcgThreads * 8, 16);
+  s->pcg_lds = align_up(s->lds_a ? need_max : (halo_max + 1) * 24 + (size_t)kPcgThreads * 8, 16);
   if ((st = upload(s->rows, s->c_rows, rows))) return st;
   if ((st = upload(s->lcol, s->c_lcol, lcol))) return st;
   if ((st = upload(s->halo_off, s->c_halo_off, hoff))) return st;
@@ -2071,7 +2165,7 @@ static int solver_build_partition(nrf_solver *s) {
     } else {
       (void)hipGetLastError();
       s->lds_a = 0;  // keep the matrix in global memory
-      s->pcg_lds = align_up(halo_max * 24 + (size_t)kPcgThreads * 8, 16);
+      s->pcg_lds = align_up((halo_max + 1) * 24 + (size_t)kPcgThreads * 8, 16);
     }
   }
   return NRF_OK;
