@@ -39,6 +39,8 @@ struct IntegrateArgs {
   int N;
   const int4 *knn;
   const float4 *knn_w;
+  const uint32_t *knn_rows;  // row occupancy of knn, or null
+  int nsx;                   // runs of 64 voxels per x row
   const int32_t *opt;
   int min_opt;
   float decay, wmax;
@@ -68,7 +70,11 @@ __device__ void integrate_tile(const IntegrateArgs &a, int bx, int by, int bz) {
       if (WARP && a.opt && a.opt[idx] < a.min_opt) continue;
       float3 p = f3(px_w, py_w, a.oz + (z + 0.5f) * a.s);
       if (WARP) {
-        const int4 id4 = a.knn[idx];
+        // a run without any id >= 0 (its word is zero) is read as all -1:
+        // the same unwarped path without the 16-byte read per voxel
+        int4 id4 = make_int4(-1, -1, -1, -1);
+        const int run = __builtin_amdgcn_readfirstlane((z * a.Y + y) * a.nsx + bx);  // one per wave
+        if (!a.knn_rows || a.knn_rows[run] != 0u) id4 = a.knn[idx];
         // voxels unobserved at the snapshot (all ids -1, most of the volume)
         // integrate unwarped: their weights are never read
         float4 w4 = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -158,6 +164,24 @@ __global__ __launch_bounds__(256) void integrate_batch_kernel(const IntBatchDev 
   integrate_tile<true, true, ZC>(a, bx, r % d.tiles_y, r / d.tiles_y);
 }
 
+// Row occupancy of a kNN: four waves per workgroup, one run of 64 voxels per
+// wave.
+__global__ __launch_bounds__(256) void knn_rows_kernel(const int4 *__restrict__ knn, int X, int nsx, int64_t nrows,
+                                                       uint32_t *__restrict__ rows) {
+  const int64_t r = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+  if (r >= nrows) return;  // whole waves
+  const int lane = threadIdx.x & 63;
+  const int x = (int)(r % nsx) * kTileX + lane;
+  const int64_t yz = r / nsx;
+  bool obs = false;
+  if (x < X) {
+    const int4 id = knn[yz * X + x];
+    obs = (id.x & id.y & id.z & id.w) >= 0;
+  }
+  const unsigned long long m = __ballot(obs);
+  if (lane == 0) rows[r] = m != 0ull ? 1u : 0u;
+}
+
 __global__ __launch_bounds__(256) void reset_kernel(nrf_voxel *v, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -217,6 +241,8 @@ int integrate_batch_create(const IntegrateBatchInstance *inst, int count, int z_
     a.N = q.N;
     a.knn = reinterpret_cast<const int4 *>(q.knn);
     a.knn_w = reinterpret_cast<const float4 *>(q.knn_w);
+    a.knn_rows = q.knn_rows;
+    a.nsx = (a.X + kTileX - 1) / kTileX;
     a.opt = nullptr;
     a.min_opt = 0;
     a.decay = q.vp.decay_alpha;
@@ -259,15 +285,25 @@ int integrate_batch_run(IntegrateBatch *b, const BatchDepth &fr, hipStream_t st)
   return NRF_OK;
 }
 
-}  // namespace nrf
+int64_t knn_row_words(const nrf_volume_params *vp) {
+  return (int64_t)((vp->dims[0] + kTileX - 1) / kTileX) * vp->dims[1] * vp->dims[2];
+}
 
-extern "C" int nrf_tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels,
-                                  const float *depth, const nrf_camera *cam,
-                                  const float camera_pose[16], const nrf_node *nodes,
-                                  const nrf_dualquat *transforms, int num_nodes,
-                                  const nrf_dualquat *centered, const int32_t *voxel_knn,
-                                  const float *voxel_knn_w, const int32_t *voxel_opt_counts,
-                                  int min_opt_count, int32_t *updated_count, void *stream) {
+int knn_row_flags(const nrf_volume_params *vp, const int32_t *knn, uint32_t *rows, hipStream_t st) {
+  if (!volume_ok(vp) || !knn || !rows) return NRF_ERR_INVALID_ARGUMENT;
+  const int64_t nrows = knn_row_words(vp);
+  const int nsx = (vp->dims[0] + kTileX - 1) / kTileX;
+  knn_rows_kernel<<<(unsigned)((nrows + 3) / 4), 256, 0, st>>>(reinterpret_cast<const int4 *>(knn), vp->dims[0],
+                                                                 nsx, nrows, rows);
+  NRF_LAUNCH_CHECK();
+  return NRF_OK;
+}
+
+int tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels, const float *depth, const nrf_camera *cam,
+                   const float camera_pose[16], const nrf_node *nodes, const nrf_dualquat *transforms, int num_nodes,
+                   const nrf_dualquat *centered, const int32_t *voxel_knn, const float *voxel_knn_w,
+                   const uint32_t *knn_rows, const int32_t *voxel_opt_counts, int min_opt_count,
+                   int32_t *updated_count, hipStream_t st) {
   if (!volume_ok(vp) || !voxels || !depth || !cam || !camera_pose) return NRF_ERR_INVALID_ARGUMENT;
   if (cam->width <= 0 || cam->height <= 0) return NRF_ERR_INVALID_ARGUMENT;
   const bool warp = nodes != nullptr && num_nodes > 0;
@@ -288,13 +324,14 @@ extern "C" int nrf_tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels
   a.N = num_nodes;
   a.knn = reinterpret_cast<const int4 *>(voxel_knn);
   a.knn_w = reinterpret_cast<const float4 *>(voxel_knn_w);
+  a.knn_rows = knn_rows;
+  a.nsx = (a.X + kTileX - 1) / kTileX;
   a.opt = voxel_opt_counts;
   a.min_opt = min_opt_count;
   a.decay = vp->decay_alpha;
   a.wmax = vp->max_weight;
   a.updated = updated_count;
   dim3 grid((a.X + kTileX - 1) / kTileX, (a.Y + kTileY - 1) / kTileY, (a.Z + kZChunk - 1) / kZChunk);
-  hipStream_t st = as_stream(stream);
   if (!warp)
     integrate_kernel<false, false><<<grid, 256, 0, st>>>(a);
   else if (centered)
@@ -303,6 +340,18 @@ extern "C" int nrf_tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels
     integrate_kernel<true, false><<<grid, 256, 0, st>>>(a);
   NRF_LAUNCH_CHECK();
   return NRF_OK;
+}
+
+}  // namespace nrf
+
+extern "C" int nrf_tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels, const float *depth,
+                                  const nrf_camera *cam, const float camera_pose[16], const nrf_node *nodes,
+                                  const nrf_dualquat *transforms, int num_nodes, const nrf_dualquat *centered,
+                                  const int32_t *voxel_knn, const float *voxel_knn_w,
+                                  const int32_t *voxel_opt_counts, int min_opt_count, int32_t *updated_count,
+                                  void *stream) {
+  return tsdf_integrate(vp, voxels, depth, cam, camera_pose, nodes, transforms, num_nodes, centered, voxel_knn,
+                        voxel_knn_w, nullptr, voxel_opt_counts, min_opt_count, updated_count, as_stream(stream));
 }
 
 extern "C" int nrf_tsdf_reset(const nrf_volume_params *vp, nrf_voxel *voxels, void *stream) {

@@ -114,11 +114,26 @@ struct IntegrateBatchInstance {
   int N;
   const int32_t *knn;
   const float *knn_w;
+  const uint32_t *knn_rows;  // row occupancy of knn (knn_row_flags), or null
   int32_t *updated;
 };
 int integrate_batch_create(const IntegrateBatchInstance *inst, int count, int z_chunk, IntegrateBatch **out);
 void integrate_batch_destroy(IntegrateBatch *b);
 int integrate_batch_run(IntegrateBatch *b, const BatchDepth &fr, hipStream_t st);
+
+// Row occupancy of a voxel kNN: one word per run of 64 voxels along x (the
+// runs the integration's waves cover; run bx of row (y, z) at word
+// (z Y + y) ceil(X / 64) + bx), nonzero when some voxel of the run has an
+// id >= 0.  The integration skips the kNN reads of the other runs, whose
+// voxels all integrate unwarped.  Valid until the kNN changes.
+int64_t knn_row_words(const nrf_volume_params *vp);
+int knn_row_flags(const nrf_volume_params *vp, const int32_t *knn, uint32_t *rows, hipStream_t st);
+// nrf_tsdf_integrate with the kNN's row occupancy (null: none)
+int tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels, const float *depth, const nrf_camera *cam,
+                   const float camera_pose[16], const nrf_node *nodes, const nrf_dualquat *transforms, int num_nodes,
+                   const nrf_dualquat *centered, const int32_t *voxel_knn, const float *voxel_knn_w,
+                   const uint32_t *knn_rows, const int32_t *voxel_opt_counts, int min_opt_count,
+                   int32_t *updated_count, hipStream_t st);
 
 }  // namespace nrf
 

@@ -22,6 +22,7 @@ struct nrf_pipeline {
   nrf_voxel *vox = nullptr;
   int32_t *knn = nullptr;
   float *knn_w = nullptr;
+  uint32_t *knn_rows = nullptr;  // row occupancy of knn (knn_row_flags), set with the kNN
   nrf_node *nodes = nullptr;
   nrf_dualquat *T = nullptr;
   nrf_dualquat *centered = nullptr;
@@ -55,7 +56,7 @@ struct nrf_pipeline {
 };
 
 static void pipeline_free(nrf_pipeline *p) {
-  void *ptrs[] = {p->vox, p->knn, p->knn_w, p->nodes, p->T, p->centered, p->delta_x, p->mc_ws,
+  void *ptrs[] = {p->vox, p->knn, p->knn_w, p->knn_rows, p->nodes, p->T, p->centered, p->delta_x, p->mc_ws,
                   p->verts, p->tris, p->counts, p->updated, p->t16, p->w8, p->k16, p->kw16};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
@@ -80,6 +81,8 @@ extern "C" nrf_pipeline *nrf_pipeline_create(const nrf_pipeline_config *cfg) {
   ok = ok && hipMalloc(&p->vox, V * sizeof(nrf_voxel)) == hipSuccess;
   ok = ok && hipMalloc(&p->knn, V * 4 * sizeof(int32_t)) == hipSuccess;
   ok = ok && hipMalloc(&p->knn_w, V * 4 * sizeof(float)) == hipSuccess;
+  const size_t nrun = (size_t)knn_row_words(&cfg->volume);
+  ok = ok && hipMalloc(&p->knn_rows, nrun * sizeof(uint32_t)) == hipSuccess;
   ok = ok && hipMalloc(&p->nodes, (size_t)cfg->max_nodes * sizeof(nrf_node)) == hipSuccess;
   ok = ok && hipMalloc(&p->T, (size_t)cfg->max_nodes * sizeof(nrf_dualquat)) == hipSuccess;
   ok = ok && hipMalloc(&p->centered, (size_t)cfg->max_nodes * sizeof(nrf_dualquat)) == hipSuccess;
@@ -107,6 +110,7 @@ extern "C" nrf_pipeline *nrf_pipeline_create(const nrf_pipeline_config *cfg) {
   if (ok) ok = nrf_tsdf_reset(&cfg->volume, p->vox, nullptr) == NRF_OK;
   if (ok) ok = hipMemset(p->knn, 0xff, V * 4 * sizeof(int32_t)) == hipSuccess;
   if (ok) ok = hipMemset(p->knn_w, 0, V * 4 * sizeof(float)) == hipSuccess;
+  if (ok) ok = hipMemset(p->knn_rows, 0, nrun * sizeof(uint32_t)) == hipSuccess;  // no id >= 0 yet
   if (ok) ok = hipMemset(p->counts, 0, 4 * sizeof(int32_t)) == hipSuccess;
   if (ok) ok = hipMemset(p->updated, 0, 4 * sizeof(int32_t)) == hipSuccess;
   if (ok) ok = hipDeviceSynchronize() == hipSuccess;
@@ -170,6 +174,8 @@ extern "C" int nrf_pipeline_set_nodes(nrf_pipeline *p, const nrf_node *host_node
   if (rc) return rc;
   rc = nrf_voxel_knn(&p->cfg.volume, p->vox, p->nodes, num_nodes, p->knn, p->knn_w, nullptr);
   if (rc) return rc;
+  rc = knn_row_flags(&p->cfg.volume, p->knn, p->knn_rows, nullptr);
+  if (rc) return rc;
   if (p->cfg.storage16) {  // the FP32 state so far, rounded once into the compact form
     rc = nrf_tsdf16_from_voxels(&p->cfg.volume, p->vox, p->t16, p->w8, nullptr);
     if (rc) return rc;
@@ -206,8 +212,8 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
     rc = nrf_tsdf16_integrate(&p->cfg.volume, p->t16, p->w8, depth, &p->cfg.camera, camera_pose, p->centered, p->N,
                               p->k16, p->kw16, p->updated, stream);
   else
-    rc = nrf_tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, p->nodes, p->T, p->N,
-                            p->centered, p->knn, p->knn_w, nullptr, p->cfg.min_opt_count, p->updated, stream);
+    rc = tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, p->nodes, p->T, p->N,
+                        p->centered, p->knn, p->knn_w, p->knn_rows, nullptr, p->cfg.min_opt_count, p->updated, st);
   if (rc) return rc;
   if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[2], st));
   if (profile) NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 2], st));
@@ -397,6 +403,7 @@ static int batch_build_integrate(const std::vector<nrf_pipeline *> &p, int z_chu
     q.N = p[k]->N;
     q.knn = p[k]->knn;
     q.knn_w = p[k]->knn_w;
+    q.knn_rows = p[k]->knn_rows;
     q.updated = updated ? updated : p[k]->updated;
   }
   return integrate_batch_create(ii.data(), (int)p.size(), z_chunk, out);

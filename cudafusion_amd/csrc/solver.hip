@@ -610,6 +610,14 @@ __device__ inline floatx4_t mfma_lp(typename LpType<PREC>::vec a, typename LpTyp
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// Hand-off between the lanes of a one-wave workgroup through LDS: a
+// compiler barrier only (LDS accesses of one wave execute in program order).
+__device__ inline void wave_sync() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_wave_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // mode 0: the whole row; mode 1: memberships [pb, pe) of the row, partial
 // sums to chunk slot `slot`; mode 2: the sums of the row's nslot chunk slots
 // from `slot` on, added in slot order, then the rest of the row.
@@ -660,23 +668,46 @@ __device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_
       Crhs[0] += v.x; Crhs[1] += v.y; Crhs[2] += v.z; Crhs[3] += v.w;
     }
   }
-  for (int base = mb; base < me; base += 64) {
-    const int e = base + lane;
-    float *st = stage + lane * kStage;
-    float R[kRec];
-    int kslot = 0;
+  // Software pipeline over the batches of 64 memberships: the keys of the
+  // first four batches (a chunk holds at most four) are loaded up front, and
+  // the records of batch k + 1 are requested once batch k is staged, so they
+  // travel while batch k's products run.  The workgroup is one wave: its LDS
+  // accesses are ordered by program order, and the staging hand-offs below
+  // use wave barriers, which (unlike a workgroup barrier's fence) do not wait
+  // for the outstanding record loads.
+  int key4[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int e = mb + 64 * k + lane;
+    key4[k] = e < me ? a.m_keys[e] : 0;
+  }
+  auto key_of = [&](int bi, int e) -> int {
+    if (bi < 4) return bi == 0 ? key4[0] : bi == 1 ? key4[1] : bi == 2 ? key4[2] : key4[3];
+    return a.m_keys[e];
+  };
+  // Every loaded word is used, including the two zero padding words (30 and
+  // 31, staged as the zero entry): a load in flight whose register looked
+  // dead would be waited for as soon as the products reuse that register.
+  float R[kRec];
+  auto fetch = [&](int bi) {
+    const int e = mb + 64 * bi + lane;
     if (e < me) {
-      const int key = a.m_keys[e];
-      kslot = key & 3;
-      const float4 *src4 = reinterpret_cast<const float4 *>(a.crec + (size_t)(key >> 2) * kRec);
+      const float4 *src4 = reinterpret_cast<const float4 *>(a.crec + (size_t)(key_of(bi, e) >> 2) * kRec);
 #pragma unroll
       for (int k = 0; k < kRec / 4; ++k) {
         const float4 v = src4[k];
         R[4 * k] = v.x; R[4 * k + 1] = v.y; R[4 * k + 2] = v.z; R[4 * k + 3] = v.w;
       }
     } else {
+#pragma unroll
       for (int k = 0; k < kRec; ++k) R[k] = 0.f;
     }
+  };
+  if (mb < me) fetch(0);
+  for (int base = mb, bi = 0; base < me; base += 64, ++bi) {
+    const int e = base + lane;
+    float *st = stage + lane * kStage;
+    const int kslot = e < me ? (key_of(bi, e) & 3) : 0;
     const float fw = R[0];
     // f <= 1e-8 skips the membership (kernels/solver_kernels.cu:84); NaN stays
     const bool act = e < me && !(fw <= 1e-8f);
@@ -698,8 +729,9 @@ __device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_
     }
     st[kP] = __uint_as_float(packed);
     for (int q = 0; q < 24; ++q) st[kJ + q] = act ? R[6 + q] : 0.f;
-    st[kZ] = 0.f;
-    __syncthreads();
+    st[kZ] = R[30] + R[31];  // the record's padding: zero
+    if (base + 64 < me) fetch(bi + 1);  // R is staged: the next batch's records
+    wave_sync();
     if constexpr (PREC == 0) {
       // k-steps outer, slabs inner: the slabs' products are independent, and
       // A and the packed positions are read once per k-step.  B[e][j] is one
@@ -756,7 +788,7 @@ __device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_
                                                     0);
       }
     }
-    __syncthreads();
+    wave_sync();
   }
   if (mode == 1) {  // partial sums of this chunk
     if (lane < 32) {
@@ -1005,6 +1037,24 @@ __device__ inline bool ll_spin(unsigned &spins, PcgCtl *ctl, int *bad) {
   return false;
 }
 
+// Polling a word that was not ready at the first load: two loads in flight,
+// the second issued kPollStagger sleep units (64 clocks each) after the
+// first, re-issued alternately, so that a value landing just after one
+// stream's load passed is seen by the other stream's next load.
+constexpr int kPollStagger = 6;
+__device__ inline ll_word ll_wait2(const ll_word *src, unsigned tag, PcgCtl *ctl, int *bad) {
+  ll_word u = ll_load(src);
+  __builtin_amdgcn_s_sleep(kPollStagger);
+  ll_word w = ll_load(src);
+  unsigned spins = 0;
+  for (;;) {
+    if (ll_has(u, tag) || ll_spin(spins, ctl, bad)) return u;
+    u = ll_load(src);
+    if (ll_has(w, tag) || ll_spin(spins, ctl, bad)) return w;
+    w = ll_load(src);
+  }
+}
+
 // All-reduce of two doubles across the G workgroups, in two halves so that
 // work can be placed between them.  ar2_publish: this workgroup's sums go
 // out under `tag` (lanes 0..3 write the hi / lo words).  ar2_collect: lanes
@@ -1082,17 +1132,38 @@ __device__ inline bool ar3_collect(const PcgArgs &a, unsigned tag, int slot, dou
   double g[3] = {0, 0, 0};
   if (t < a.G) {
     const ll_word *w = a.recs[t].w[slot];
-    ll_word x[6];
-    unsigned spins = 0;
-    for (;;) {
-      bool all = true;
+    ll_word x[6], y[6];
+    bool all = true;
 #pragma unroll
-      for (int k = 0; k < 6; ++k) {
-        x[k] = ll_load(w + k);
-        all = all && ll_has(x[k], tag);
+    for (int k = 0; k < 6; ++k) {
+      x[k] = ll_load(w + k);
+      all = all && ll_has(x[k], tag);
+    }
+    if (!all) {  // two streams of polls, as in ll_wait2
+#pragma unroll
+      for (int k = 0; k < 6; ++k) x[k] = ll_load(w + k);
+      __builtin_amdgcn_s_sleep(kPollStagger);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) y[k] = ll_load(w + k);
+      unsigned spins = 0;
+      for (;;) {
+        all = true;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) all = all && ll_has(x[k], tag);
+        if (all || ll_spin(spins, a.ctl, s_bad)) break;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) x[k] = ll_load(w + k);
+        all = true;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) all = all && ll_has(y[k], tag);
+        if (all || ll_spin(spins, a.ctl, s_bad)) {
+#pragma unroll
+          for (int k = 0; k < 6; ++k) x[k] = y[k];
+          break;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) y[k] = ll_load(w + k);
       }
-      if (all) break;
-      if (ll_spin(spins, a.ctl, s_bad)) break;
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k)
@@ -1393,13 +1464,8 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
       const int nd = e == t ? hn0 : a.halo[h0 + e / 6], c = e % 6;
       if (nd < n0 || nd >= n1) {
         const ll_word *src = a.zpub + 6 * nd + c;
-        ll_word wv;
-        unsigned spins = 0;
-        for (;;) {
-          wv = ll_load(src);
-          if (ll_has(wv, tag)) break;
-          if (ll_spin(spins, a.ctl, &s_bad)) break;
-        }
+        ll_word wv = ll_load(src);
+        if (!ll_has(wv, tag)) wv = ll_wait2(src, tag, a.ctl, &s_bad);
         ph[e] = __uint_as_float((unsigned)wv);
       }
     }

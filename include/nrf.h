@@ -370,6 +370,10 @@ void nrf_pipeline_destroy(nrf_pipeline *p);
 nrf_voxel *nrf_pipeline_voxels(nrf_pipeline *p);
 nrf_dualquat *nrf_pipeline_transforms(nrf_pipeline *p);
 nrf_node *nrf_pipeline_nodes(nrf_pipeline *p);
+/* The voxel kNN (computed by nrf_pipeline_set_nodes): read-only for the
+   caller; the pipeline keeps a map of the 64-voxel runs along x that hold an
+   id >= 0 (the warped integration reads the kNN of those runs only), made
+   from the kNN at nrf_pipeline_set_nodes. */
 int32_t *nrf_pipeline_knn(nrf_pipeline *p);
 float *nrf_pipeline_knn_w(nrf_pipeline *p);
 float *nrf_pipeline_mesh(nrf_pipeline *p, int32_t **counts);
