@@ -242,6 +242,8 @@ struct AsmArgs {
 // Memberships per assembly chunk: a row with more is split, so that the
 // longest serial chain of 64-membership batches is 4 batches long.
 constexpr int kChunk = kAsmChunk;
+constexpr int kAsmBatch = 32;     // memberships per batch of the MFMA assembly
+constexpr int kAsmBcStride = 34;  // column stride of its dense B operand (2 mod 32 banks)
 // floats of one chunk's partial sums: the accumulators of lanes 0..31 (rows
 // 0..5 of every column strip live there), NS strips + the FP32 RHS column
 __host__ __device__ constexpr int part_floats(int ns) { return (ns + 1) * 32 * 4; }
@@ -629,11 +631,9 @@ __device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_
   const int rb = a.row_ptr[i], rlen = a.row_ptr[i + 1] - rb;
   float *acc = lds;                                      // [Rmax][36]
   int32_t *cols = reinterpret_cast<int32_t *>(acc + a.Rmax * 36);  // [Rmax]
-  float *stage = reinterpret_cast<float *>(cols + a.Rmax);         // [64][kStage]
+  float *stage = reinterpret_cast<float *>(cols + a.Rmax);         // dense operands, then the tail's scratch
   for (int j = lane; j < rlen; j += 64) cols[j] = a.col_idx[rb + j];
   __syncthreads();
-  // staging layout per membership: a[6], r, packed pos (4 bytes), J[4][6], 0
-  constexpr int kA = 0, kR = 6, kP = 7, kJ = 8, kZ = 32;
   typedef floatx4_t floatx4;
   typedef typename LpType<PREC>::vec lp4;
   floatx4 C[NS];
@@ -643,18 +643,6 @@ __device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_
   const int ncol = 6 * rlen + 1;  // block columns + RHS column
   const int am = lane % 16;       // A row / B, D column within a slab
   const int kq = lane / 16;       // k within a 4-step
-  // this lane's B / D column of every slab: block, entry within the block,
-  // and whether it is the RHS column (no block matches a column >= 6 R)
-  unsigned col_blk[NS];
-  int col_c[NS];
-  bool col_rhs[NS];
-#pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    const int j = 16 * s + am;
-    col_blk[s] = (unsigned)(j / 6);
-    col_c[s] = j % 6;
-    col_rhs[s] = j == ncol - 1;
-  }
   const int mb = mode == 1 ? pb : a.m_off[i], me = mode == 1 ? pe : (mode == 2 ? mb : a.m_off[i + 1]);
   if (mode == 2 && lane < 32) {
     for (int k = 0; k < nslot; ++k) {
@@ -668,29 +656,54 @@ __device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_
       Crhs[0] += v.x; Crhs[1] += v.y; Crhs[2] += v.z; Crhs[3] += v.w;
     }
   }
-  // Software pipeline over the batches of 64 memberships: the keys of the
-  // first four batches (a chunk holds at most four) are loaded up front, and
-  // the records of batch k + 1 are requested once batch k is staged, so they
-  // travel while batch k's products run.  The workgroup is one wave: its LDS
-  // accesses are ordered by program order, and the staging hand-offs below
-  // use wave barriers, which (unlike a workgroup barrier's fence) do not wait
-  // for the outstanding record loads.
-  int key4[4];
+  // Dense operands of the products, per batch of kAsmBatch = 32 memberships:
+  // Ab[m][0..7] = a_m (six entries, then two zeros) and Bc[j][m] = b_m[j],
+  // column j of membership m's dense row (J_l in the columns of block
+  // pos_l, r in column 6 R, zero elsewhere), stored by column with stride
+  // kAsmBcStride so that the 16 x 2 lanes of a half wave reading one
+  // k-step of a slab hit 32 distinct banks.  Lane (m, h) stages membership
+  // m: half h scatters the Jacobians of slots 2h and 2h + 1 after clearing
+  // the two blocks it scattered for the previous batch; half 0 also writes
+  // a_m and r.  The operands are the scalar form's, so are the products.
+  //
+  // Software pipeline over the batches: the keys of the chunk (at most
+  // kChunk / kAsmBatch batches) are loaded up front, and the records of
+  // batch k + 1 are requested once batch k is staged, so they travel while
+  // batch k's products run.  The workgroup is one wave: its LDS accesses
+  // are ordered by program order, and the hand-offs use wave barriers,
+  // which (unlike a workgroup barrier's fence) do not wait for the
+  // outstanding record loads.
+  constexpr int kBat = kAsmBatch, kBS = kAsmBcStride, kKeys = kChunk / kAsmBatch;
+  float *Ab = stage;            // [kBat][8]
+  float *Bc = stage + kBat * 8;  // [16 NS][kBS]
+  const int m = lane & (kBat - 1), h = lane / kBat;
+  if (mb < me) {
+    for (int j = lane; j < 16 * NS * kBS; j += 64) Bc[j] = 0.f;
+    if (lane < kBat) {
+      Ab[lane * 8 + 6] = 0.f;
+      Ab[lane * 8 + 7] = 0.f;
+    }
+  }
+  int keys[kKeys];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int e = mb + 64 * k + lane;
-    key4[k] = e < me ? a.m_keys[e] : 0;
+  for (int k = 0; k < kKeys; ++k) keys[k] = 0;
+  if (mb < me) {  // indices clamped into [mb, me): the loads go out together
+#pragma unroll
+    for (int k = 0; k < kKeys; ++k) keys[k] = a.m_keys[min(mb + kBat * k + m, me - 1)];
   }
   auto key_of = [&](int bi, int e) -> int {
-    if (bi < 4) return bi == 0 ? key4[0] : bi == 1 ? key4[1] : bi == 2 ? key4[2] : key4[3];
-    return a.m_keys[e];
+    if (bi >= kKeys) return a.m_keys[e];
+    int key = keys[0];
+#pragma unroll
+    for (int k = 1; k < kKeys; ++k) key = bi == k ? keys[k] : key;
+    return key;
   };
   // Every loaded word is used, including the two zero padding words (30 and
-  // 31, staged as the zero entry): a load in flight whose register looked
-  // dead would be waited for as soon as the products reuse that register.
+  // 31): a load in flight whose register looked dead would be waited for as
+  // soon as the products reuse that register.
   float R[kRec];
   auto fetch = [&](int bi) {
-    const int e = mb + 64 * bi + lane;
+    const int e = mb + kBat * bi + m;
     if (e < me) {
       const float4 *src4 = reinterpret_cast<const float4 *>(a.crec + (size_t)(key_of(bi, e) >> 2) * kRec);
 #pragma unroll
@@ -703,53 +716,68 @@ __device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_
       for (int k = 0; k < kRec; ++k) R[k] = 0.f;
     }
   };
+  int prev[2] = {-1, -1};  // blocks this lane scattered for the previous batch
+  // the row's columns in scalar registers (uniform), for the position search
+  constexpr int kMaxBlk = (16 * NS - 1) / 6;  // 6 R + 1 <= 16 NS
+  int cr[kMaxBlk];
+#pragma unroll
+  for (int q = 0; q < kMaxBlk; ++q) cr[q] = q < rlen ? a.col_idx[rb + q] : -2;
+  wave_sync();
   if (mb < me) fetch(0);
-  for (int base = mb, bi = 0; base < me; base += 64, ++bi) {
-    const int e = base + lane;
-    float *st = stage + lane * kStage;
+#pragma nounroll
+  for (int base = mb, bi = 0; base < me; base += kBat, ++bi) {
+    const int e = base + m;
     const int kslot = e < me ? (key_of(bi, e) & 3) : 0;
     const float fw = R[0];
     // f <= 1e-8 skips the membership (kernels/solver_kernels.cu:84); NaN stays
     const bool act = e < me && !(fw <= 1e-8f);
-    float Jk[6];
+    // this half's two slots: their blocks in the row (absent: dropped, Q2)
+    int pos[2];
+    float Js[2][6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q)  // the row node's slot, selected without dynamic indexing
-      Jk[q] = kslot == 0 ? R[6 + q] : kslot == 1 ? R[12 + q] : kslot == 2 ? R[18 + q] : R[24 + q];
-    for (int q = 0; q < 6; ++q) st[kA + q] = act ? fw * Jk[q] : 0.f;
-    st[kR] = act ? R[1] : 0.f;
-    unsigned packed = 0xffffffffu;
+    for (int l2 = 0; l2 < 2; ++l2) {
+      const int id = __float_as_int(h ? R[4 + l2] : R[2 + l2]);
 #pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      const int id = __float_as_int(R[2 + l]);
+      for (int c = 0; c < 6; ++c) Js[l2][c] = h ? R[18 + 6 * l2 + c] : R[6 + 6 * l2 + c];
       int p = -1;
-      if (act && id >= 0)
-        for (int q = 0; q < rlen; ++q)  // rows are short; linear scan (absent: dropped, Q2)
-          if (cols[q] == id) { p = q; break; }
-      if (p >= 0) packed = (packed & ~(0xffu << (8 * l))) | ((unsigned)p << (8 * l));
+      if (act && id >= 0) {
+#pragma unroll
+        for (int q = 0; q < kMaxBlk; ++q) p = cr[q] == id ? q : p;  // columns are distinct
+      }
+      pos[l2] = p;
     }
-    st[kP] = __uint_as_float(packed);
-    for (int q = 0; q < 24; ++q) st[kJ + q] = act ? R[6 + q] : 0.f;
-    st[kZ] = R[30] + R[31];  // the record's padding: zero
-    if (base + 64 < me) fetch(bi + 1);  // R is staged: the next batch's records
+#pragma unroll
+    for (int l2 = 0; l2 < 2; ++l2)
+      if (prev[l2] >= 0)
+        for (int c = 0; c < 6; ++c) Bc[(prev[l2] * 6 + c) * kBS + m] = 0.f;
+    wave_sync();  // every clear before any scatter (a lane's new block may be another's old one)
+#pragma unroll
+    for (int l2 = 0; l2 < 2; ++l2) {
+      if (pos[l2] >= 0)
+        for (int c = 0; c < 6; ++c) Bc[(pos[l2] * 6 + c) * kBS + m] = Js[l2][c];
+      prev[l2] = pos[l2];
+    }
+    if (h == 0) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {  // the row node's slot, selected without dynamic indexing
+        const float jk = kslot == 0 ? R[6 + q] : kslot == 1 ? R[12 + q] : kslot == 2 ? R[18 + q] : R[24 + q];
+        Ab[m * 8 + q] = act ? fw * jk : 0.f;
+      }
+      // r; adding the record's two padding words (zero) keeps their
+      // registers live, so no load in flight lands on a reused register
+      Bc[(ncol - 1) * kBS + m] = act ? R[1] + (R[30] + R[31]) : 0.f;
+    }
+    if (base + kBat < me) fetch(bi + 1);  // R is staged: the next batch's records
     wave_sync();
+    const int ar = am < 6 ? am : 6;  // A rows 6..15 read the zero entry
     if constexpr (PREC == 0) {
-      // k-steps outer, slabs inner: the slabs' products are independent, and
-      // A and the packed positions are read once per k-step.  B[e][j] is one
-      // LDS read at the offset of the slot whose block is column j's (r for
-      // the RHS column, the staged zero otherwise): straight-line code.
 #pragma unroll
-      for (int t = 0; t < 16; ++t) {
-        const float *sq = stage + (4 * t + kq) * kStage;
-        const float av = sq[am < 6 ? kA + am : kZ];
-        const unsigned pk = __float_as_uint(sq[kP]);
+      for (int t = 0; t < kBat / 4; ++t) {
+        const float av = Ab[(4 * t + kq) * 8 + ar];
+        const float *bq = Bc + am * kBS + 4 * t + kq;
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-          int off = col_rhs[s] ? kR : kZ;
-#pragma unroll
-          for (int l = 0; l < 4; ++l)
-            off = (((pk >> (8 * l)) & 0xffu) == col_blk[s]) ? kJ + 6 * l + col_c[s] : off;
-          C[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, sq[off], C[s], 0, 0, 0);
-        }
+        for (int s = 0; s < NS; ++s)
+          C[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bq[16 * s * kBS], C[s], 0, 0, 0);
       }
     } else {
       // JtJ blocks in BF16 (PREC 1) or FP16 (PREC 2) with FP32 accumulation:
@@ -757,36 +785,23 @@ __device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_
       // 16 t + 4 kq + q, q = 0..3.  The RHS column stays FP32 (below), so
       // the reduced precision touches the Hessian approximation only.
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const float *sqq[4];
-        unsigned pk[4];
+      for (int t = 0; t < kBat / 16; ++t) {
         lp4 av;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          sqq[q] = stage + (16 * t + 4 * kq + q) * kStage;
-          pk[q] = __float_as_uint(sqq[q][kP]);
-          av[q] = to_lp<PREC>(sqq[q][am < 6 ? kA + am : kZ]);
-        }
+        for (int q = 0; q < 4; ++q) av[q] = to_lp<PREC>(Ab[(16 * t + 4 * kq + q) * 8 + ar]);
+        const float *bq = Bc + am * kBS + 16 * t + 4 * kq;
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
           lp4 bv;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            int off = kZ;
-#pragma unroll
-            for (int l = 0; l < 4; ++l)
-              off = (((pk[q] >> (8 * l)) & 0xffu) == col_blk[s]) ? kJ + 6 * l + col_c[s] : off;
-            bv[q] = to_lp<PREC>(sqq[q][off]);
-          }
+          for (int q = 0; q < 4; ++q) bv[q] = to_lp<PREC>(bq[16 * s * kBS + q]);
           C[s] = mfma_lp<PREC>(av, bv, C[s]);
         }
       }
 #pragma unroll
-      for (int t = 0; t < 16; ++t) {
-        const float *sq = stage + (4 * t + kq) * kStage;
-        Crhs = __builtin_amdgcn_mfma_f32_16x16x4f32(sq[am < 6 ? kA + am : kZ], am == 0 ? sq[kR] : 0.f, Crhs, 0, 0,
-                                                    0);
-      }
+      for (int t = 0; t < kBat / 4; ++t)
+        Crhs = __builtin_amdgcn_mfma_f32_16x16x4f32(Ab[(4 * t + kq) * 8 + ar],
+                                                    am == 0 ? Bc[(ncol - 1) * kBS + 4 * t + kq] : 0.f, Crhs, 0, 0, 0);
     }
     wave_sync();
   }
@@ -2308,8 +2323,14 @@ static AsmArgs solver_asm_args(const nrf_solver *s, const nrf_corr *corrs, const
 
 // LDS bytes of one assembly workgroup and the MFMA column strips of a row
 // of Rmax blocks (block columns + the RHS column, 16 per strip).
-static size_t asm_lds_bytes(int Rmax) { return (size_t)Rmax * 36 * 4 + (size_t)Rmax * 4 + 64 * kStage * 4; }
 static int asm_strips(int Rmax) { return (6 * Rmax + 1 + 15) / 16; }
+// accumulators and columns of a row, then the staging area: the scalar
+// form's [64][kStage] (also the row tail's scratch) or the MFMA form's dense
+// operands, [kAsmBatch][8] + [16 slabs][kAsmBcStride]
+static size_t asm_lds_bytes(int Rmax) {
+  const size_t stage = std::max((size_t)64 * kStage, (size_t)kAsmBatch * 8 + (size_t)16 * asm_strips(Rmax) * kAsmBcStride);
+  return (size_t)Rmax * 36 * 4 + (size_t)Rmax * 4 + stage * 4;
+}
 
 static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, int n, const float *canonical,
                            const nrf_node *nodes, const DQ *T, const DQ *centered, int N, hipStream_t st) {
