@@ -232,6 +232,10 @@ struct AsmArgs {
   float *values, *b, *Minv;
   const int32_t *stop;
   const float *crec;  // per-corr records (corr_prep_kernel), MFMA form only
+  // per-edge-side ARAP records (corr_prep_kernel, kArapRec floats each), or
+  // null: the row tail then computes them
+  float *arec;
+  int n_es;
   // chunks of a row's memberships (MFMA form): chunk_off[i] .. chunk_off[i+1]
   // are row i's chunks, chunk_row maps a chunk to its row; rows of several
   // chunks leave per-chunk partial sums in part
@@ -257,6 +261,7 @@ constexpr int kStageData = 3 + 4 + 24;
 // staging per ARAP edge-side: lam, side, offrel, r[3], Js[6][3], Jo[6][3]
 constexpr int kStageArap = 6 + 18 + 18;
 constexpr int kStage = kStageArap > kStageData ? kStageArap : kStageData;
+constexpr int kArapRec = 44;  // kStageArap padded to whole float4
 
 __device__ inline void arap_jac(float3 q, float sgn, float J[6][3]) {
   // compute_arap_jacobian (kernels/solver_kernels.cu:140-165).  sgn = 1 gives
@@ -354,6 +359,45 @@ __device__ inline float stage_membership(const AsmArgs &a, int key, const int32_
   return fw;
 }
 
+// One ARAP edge-side e (kernels/solver_kernels.cu:202-352): stages side,
+// offrel, r and the two Jacobians in st[1..41] when its weight passes the
+// threshold (:271) and returns the weight (0 otherwise: st[1..] untouched).
+__device__ inline float arap_side(const AsmArgs &a, int e, float *st) {
+  const int ent = a.es_entry[e];
+  const int edge = ent >> 1, side = ent & 1;
+  const int ni = a.e_i[edge], nj = a.e_j[edge];
+  const float nw = fmaxf(a.e_w[edge], 0.05f);
+  const float3 xi = f3(a.nodes[ni].pos), xj = f3(a.nodes[nj].pos);
+  const DQ Ti = a.T[ni], Tj = a.T[nj];
+  const float3 Ti_xj = dq_transform_centered(Ti, xj, xi);
+  const float3 Ti_xi = dq_transform_centered(Ti, xi, xi);
+  const float3 Tj_xj = dq_transform_centered(Tj, xj, xj);
+  const float3 rest = sub(xi, xj);
+  const float3 curr = sub(Ti_xi, Tj_xj);
+  const float dr = sqrtf(rest.x * rest.x + rest.y * rest.y + rest.z * rest.z);
+  const float dc = sqrtf(curr.x * curr.x + curr.y * curr.y + curr.z * curr.z);
+  const float strain = fabsf(dc - dr) / fmaxf(dr, 1e-6f);
+  const float ratio = strain / 0.25f;
+  const float sw = expf(-(ratio * ratio));
+  const float3 r = sub(Ti_xj, Tj_xj);
+  const float rn = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
+  const float hw = huber_w(rn, 0.05f);
+  const float lam = a.lambda_smooth * nw * hw * sw;
+  if (lam <= 1e-8f) return 0.f;
+  float Ji[6][3], Jj[6][3];
+  arap_jac(sub(Ti_xj, xi), a.arap_sign, Ji);
+  arap_jac(sub(Tj_xj, xj), a.arap_sign, Jj);
+  st[1] = __int_as_float(side);
+  st[2] = __int_as_float(a.es_offrel[e]);
+  st[3] = r.x; st[4] = r.y; st[5] = r.z;
+  for (int p = 0; p < 6; ++p)
+    for (int c = 0; c < 3; ++c) {
+      st[6 + p * 3 + c] = side ? Jj[p][c] : Ji[p][c];
+      st[24 + p * 3 + c] = side ? Ji[p][c] : Jj[p][c];
+    }
+  return lam;
+}
+
 // ARAP edge-sides of row i (kernels/solver_kernels.cu:202-352), damping
 // (:167-194), write-out of the row's blocks and RHS, then the block-Jacobi
 // inverse of the diagonal block (:358-399, include/se3_math.cuh:152-202).
@@ -367,45 +411,24 @@ __device__ inline void row_tail(const AsmArgs &a, int i, int rb, int rlen, float
   for (int base = sb; base < se; base += 64) {
     const int e = base + lane;
     float *st = stage + lane * kStage;
-    float lam = 0.f;
-    if (e < se) {
-      const int ent = a.es_entry[e];
-      const int edge = ent >> 1, side = ent & 1;
-      const int ni = a.e_i[edge], nj = a.e_j[edge];
-      const float nw = fmaxf(a.e_w[edge], 0.05f);
-      const float3 xi = f3(a.nodes[ni].pos), xj = f3(a.nodes[nj].pos);
-      const DQ Ti = a.T[ni], Tj = a.T[nj];
-      const float3 Ti_xj = dq_transform_centered(Ti, xj, xi);
-      const float3 Ti_xi = dq_transform_centered(Ti, xi, xi);
-      const float3 Tj_xj = dq_transform_centered(Tj, xj, xj);
-      const float3 rest = sub(xi, xj);
-      const float3 curr = sub(Ti_xi, Tj_xj);
-      const float dr = sqrtf(rest.x * rest.x + rest.y * rest.y + rest.z * rest.z);
-      const float dc = sqrtf(curr.x * curr.x + curr.y * curr.y + curr.z * curr.z);
-      const float strain = fabsf(dc - dr) / fmaxf(dr, 1e-6f);
-      const float ratio = strain / 0.25f;
-      const float sw = expf(-(ratio * ratio));
-      const float3 r = sub(Ti_xj, Tj_xj);
-      const float rn = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
-      const float hw = huber_w(rn, 0.05f);
-      lam = a.lambda_smooth * nw * hw * sw;
-      if (!(lam <= 1e-8f)) {  // kernels/solver_kernels.cu:271
-        float Ji[6][3], Jj[6][3];
-        arap_jac(sub(Ti_xj, xi), a.arap_sign, Ji);
-        arap_jac(sub(Tj_xj, xj), a.arap_sign, Jj);
-        st[1] = __int_as_float(side);
-        st[2] = __int_as_float(a.es_offrel[e]);
-        st[3] = r.x; st[4] = r.y; st[5] = r.z;
-        for (int p = 0; p < 6; ++p)
-          for (int c = 0; c < 3; ++c) {
-            st[6 + p * 3 + c] = side ? Jj[p][c] : Ji[p][c];
-            st[24 + p * 3 + c] = side ? Ji[p][c] : Jj[p][c];
-          }
+    if (e < se && a.arec) {  // the record corr_prep_kernel made
+      const float4 *src = reinterpret_cast<const float4 *>(a.arec + (size_t)e * kArapRec);
+#pragma unroll
+      for (int k = 0; k < kArapRec / 4; ++k) {
+        const float4 v = src[k];
+        if (4 * k < kStageArap) st[4 * k] = v.x;
+        if (4 * k + 1 < kStageArap) st[4 * k + 1] = v.y;
+        if (4 * k + 2 < kStageArap) st[4 * k + 2] = v.z;
+        if (4 * k + 3 < kStageArap) st[4 * k + 3] = v.w;
       }
+    } else {
+      st[0] = e < se ? arap_side(a, e, st) : 0.f;
     }
-    st[0] = (lam <= 1e-8f) ? 0.f : lam;
     __syncthreads();
     const int cnt = min(64, se - base);
+    // the diagonal entry of lane ab accumulates in a register (same order of
+    // additions; the off-diagonal block differs per edge-side)
+    float dacc = lane < 36 ? acc[diag * 36 + ab] : 0.f;
     for (int q = 0; q < cnt; ++q) {
       const float *sq = stage + q * kStage;
       const float lm = sq[0];
@@ -417,7 +440,7 @@ __device__ inline void row_tail(const AsmArgs &a, int i, int rb, int rlen, float
         float vss = 0, vso = 0;
         for (int c = 0; c < 3; ++c) vss += Js[ea * 3 + c] * Js[eb * 3 + c];
         for (int c = 0; c < 3; ++c) vso += Js[ea * 3 + c] * Jo[eb * 3 + c];
-        acc[diag * 36 + ab] += lm * vss;
+        dacc += lm * vss;
         acc[offrel * 36 + ab] += -lm * vso;
       } else if (ra >= 0 && ra < 6) {
         float v = 0;
@@ -425,6 +448,7 @@ __device__ inline void row_tail(const AsmArgs &a, int i, int rb, int rlen, float
         bacc += side ? lm * v : -lm * v;
       }
     }
+    if (lane < 36) acc[diag * 36 + ab] = dacc;
     __syncthreads();
   }
 
@@ -570,9 +594,23 @@ __device__ void corr_prep_one(const AsmArgs &a, int c, float *__restrict__ rec) 
 #pragma unroll
   for (int k = 0; k < kRec / 4; ++k) dst4[k] = make_float4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
 }
+// Threads [0, n): the per-corr records; [n, n + n_es): the ARAP edge-side
+// records (when a.arec is set), so that the row tails read them instead of
+// walking edge -> node -> transform loads themselves.
 __global__ __launch_bounds__(256) void corr_prep_kernel(AsmArgs a, int n, float *__restrict__ rec) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < n) corr_prep_one(a, c, rec);
+  if (c < n) {
+    corr_prep_one(a, c, rec);
+  } else if (a.arec && c - n < a.n_es) {
+    const int e = c - n;
+    float o[kArapRec];
+#pragma unroll
+    for (int k = 0; k < kArapRec; ++k) o[k] = 0.f;
+    o[0] = arap_side(a, e, o);
+    float4 *dst = reinterpret_cast<float4 *>(a.arec + (size_t)e * kArapRec);
+#pragma unroll
+    for (int k = 0; k < kArapRec / 4; ++k) dst[k] = make_float4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+  }
 }
 
 // MFMA form of the data term (NS = column slabs of 16 covering the 6 R
@@ -2125,6 +2163,8 @@ struct nrf_solver {
   int32_t *stop = nullptr;
   size_t c_stop = 0;
   float *crec = nullptr;  // per-corr records of the MFMA assembly
+  float *arec = nullptr;  // per-edge-side ARAP records of the MFMA assembly
+  size_t c_arec = 0;
   int32_t *chunk_off = nullptr, *chunk_row = nullptr;  // membership chunks of the MFMA assembly
   float *part = nullptr;                                // partial sums of split rows
   size_t c_chunk_off = 0, c_chunk_row = 0, c_part = 0;
@@ -2175,6 +2215,9 @@ static int solver_set_pattern_host(nrf_solver *s, const nrf_node *h, int N) {
   if ((st = upload(s->es_off, s->c_es_off, P.es_off))) return st;
   if ((st = upload(s->es_entry, s->c_es_entry, P.es_entry))) return st;
   if ((st = upload(s->es_offrel, s->c_es_offrel, P.es_offrel))) return st;
+  if (!P.es_entry.empty() &&
+      (st = dev_alloc(s->arec, P.es_entry.size() * (size_t)kArapRec, s->c_arec)))
+    return st;
   if ((st = dev_alloc(s->values, (size_t)std::max(P.nb, 1) * 36, s->c_values))) return st;
   return solver_build_partition(s);
 }
@@ -2315,6 +2358,8 @@ static AsmArgs solver_asm_args(const nrf_solver *s, const nrf_corr *corrs, const
   a.Minv = s->Minv;
   a.stop = s->stop;
   a.crec = nullptr;
+  a.arec = nullptr;
+  a.n_es = 0;
   a.chunk_off = s->chunk_off;
   a.chunk_row = s->chunk_row;
   a.part = s->part;
@@ -2343,7 +2388,12 @@ static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, int n, const fl
     int rc;
     if ((rc = dev_alloc(s->crec, (size_t)n * kRec, s->c_crec))) return rc;
     a.crec = s->crec;
-    corr_prep_kernel<<<(n + 255) / 256, 256, 0, st>>>(a, n, s->crec);
+    const int n_es = s->P.es_off.empty() ? 0 : s->P.es_off.back();
+    if (n_es > 0 && s->arec) {  // the row tails read the edge-side records
+      a.arec = s->arec;
+      a.n_es = n_es;
+    }
+    corr_prep_kernel<<<(n + a.n_es + 255) / 256, 256, 0, st>>>(a, n, s->crec);
     NRF_LAUNCH_CHECK();
   }
   const int prec = s->params.jtj_precision;
@@ -2543,7 +2593,7 @@ extern "C" void nrf_solver_destroy(nrf_solver *s) {
                   s->rows, s->e_w, s->values, s->b, s->Minv, s->zpub, s->P0, s->P1, s->recs, s->ctl, s->lcol,
                   s->halo_off, s->halo,
                   s->m_cnt, s->m_off, s->m_cur, s->m_keys, s->m_keys2, s->stop, s->d_res, s->d_tot,
-                  s->crec, s->rres, s->dref, s->rpart, s->rnorm, s->chunk_off, s->chunk_row, s->part};
+                  s->crec, s->arec, s->rres, s->dref, s->rpart, s->rnorm, s->chunk_off, s->chunk_row, s->part};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
