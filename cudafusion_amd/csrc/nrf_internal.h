@@ -53,7 +53,7 @@ bool device_cooperative();
 // depends on nothing but them and the instances' device state.
 constexpr int kMaxBatch = 32;
 // memberships per chunk of the MFMA assembly (rows with more are split)
-constexpr int kAsmChunk = 256;
+constexpr int kAsmChunk = 128;
 struct BatchCorrs {
   const nrf_corr *corrs[kMaxBatch];
   const float *canonical[kMaxBatch];

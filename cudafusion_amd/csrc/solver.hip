@@ -244,7 +244,8 @@ struct AsmArgs {
 };
 
 // Memberships per assembly chunk: a row with more is split, so that the
-// longest serial chain of 64-membership batches is 4 batches long.
+// longest serial chain of 32-membership batches is 4 batches long (128
+// measured faster than 256 and 64 on the full-scale body workload).
 constexpr int kChunk = kAsmChunk;
 constexpr int kAsmBatch = 32;     // memberships per batch of the MFMA assembly
 constexpr int kAsmBcStride = 34;  // column stride of its dense B operand (2 mod 32 banks)
