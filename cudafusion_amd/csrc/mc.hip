@@ -26,20 +26,6 @@ namespace {
 
 constexpr int kMcBlock = 256;
 
-struct McLayout {
-  size_t cases_off, rowcnt_off, rowoff_off, total;
-};
-
-McLayout mc_layout(const nrf_volume_params *vp) {
-  McLayout L;
-  size_t V = (size_t)volume_count(vp);
-  size_t R = (size_t)vp->dims[1] * vp->dims[2];
-  L.cases_off = 0;
-  L.rowcnt_off = align_up(V, 256);
-  L.rowoff_off = L.rowcnt_off + align_up(R * 4, 256);
-  L.total = L.rowoff_off + align_up((R + 1) * 4, 256);
-  return L;
-}
 
 struct VoxelReaderF32 {
   const nrf_voxel *v;
@@ -161,12 +147,13 @@ extern "C" int nrf_mc_extract(const nrf_volume_params *vp, const nrf_voxel *voxe
   int32_t *rowoff = reinterpret_cast<int32_t *>(ws + L.rowoff_off);
   const int R = Y * Z;
   hipStream_t st = as_stream(stream);
-  fill_zero_kernel<<<(R + 255) / 256, 256, 0, st>>>(rowcnt, R);
+  const int nz = (int)L.zero_ints;  // the counts and the scan's status words
+  fill_zero_kernel<<<(nz + 255) / 256, 256, 0, st>>>(rowcnt, nz);
   NRF_LAUNCH_CHECK();
   classify_brick_kernel<<<classify_brick_grid(X, Y, Z), kMcBrickThreads, 0, st>>>(VoxelReaderF32{voxels}, X, Y, Z,
                                                                                    cases, rowcnt);
   NRF_LAUNCH_CHECK();
-  scan_rows_kernel<<<1, 1024, 0, st>>>(rowcnt, rowoff, R, counts);
+  mc_scan_rows(L, rowcnt, rowoff, R, counts, workspace, st);
   NRF_LAUNCH_CHECK();
   if (X >= 2 && Y >= 2 && Z >= 2 && max_triangles > 0) {
     emit_kernel<<<dim3(Y, Z), kMcBlock, 0, st>>>(voxels, X, Y, Z,

@@ -9,6 +9,7 @@
 #include <cstdint>
 
 #include "mc_tables.h"
+#include "nrf_internal.h"
 
 namespace nrf {
 namespace {
@@ -123,6 +124,131 @@ __global__ __launch_bounds__(1024) void scan_rows_kernel(const int32_t *__restri
       counts[0] = 3 * part[1023];
       counts[1] = part[1023];
     }
+  }
+}
+
+// Exclusive scan of n row counts in blocks of kScanBlock rows, one
+// workgroup of kScanThreads lanes per block (four rows per lane, one 16-byte
+// vector): the block scans its rows, publishes its total in status[b] (flag
+// in the high word; the status words start zeroed), and adds the totals of
+// all earlier blocks, which one wave reads.  Workgroups are dispatched in
+// index order and publish before they wait, so every wait ends; the wait is
+// bounded all the same (then counts[1] = -1).  Integer sums: the result does
+// not depend on the order of the additions.  cnt and off 16-byte aligned;
+// counts (optional) receives {3 T, T}.
+constexpr int kScanBlock = 1024, kScanThreads = 256, kScanMaxBlocks = 4096;
+constexpr unsigned kScanSpinLimit = 1u << 24;
+__global__ __launch_bounds__(kScanThreads) void scan_rows_lb_kernel(const int32_t *__restrict__ cnt,
+                                                                    int32_t *__restrict__ off, int n,
+                                                                    int32_t *__restrict__ counts,
+                                                                    unsigned long long *__restrict__ status) {
+  __shared__ int32_t wsum[kScanThreads / 64];
+  __shared__ int32_t s_prefix;
+  __shared__ int s_bad;
+  const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int r0 = b * kScanBlock + 4 * t;
+  int4 v = make_int4(0, 0, 0, 0);
+  if (r0 + 3 < n) {
+    v = *reinterpret_cast<const int4 *>(cnt + r0);
+  } else {
+    if (r0 < n) v.x = cnt[r0];
+    if (r0 + 1 < n) v.y = cnt[r0 + 1];
+    if (r0 + 2 < n) v.z = cnt[r0 + 2];
+  }
+  const int s = v.x + v.y + v.z + v.w;
+  int incl = s;  // inclusive scan of the wave
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += u;
+  }
+  if (lane == 63) wsum[w] = incl;
+  if (t == 0) s_bad = 0;
+  __syncthreads();
+  int wpre = 0, total = 0;
+#pragma unroll
+  for (int k = 0; k < kScanThreads / 64; ++k) {
+    wpre += k < w ? wsum[k] : 0;
+    total += wsum[k];
+  }
+  if (t == 0)
+    __hip_atomic_store(&status[b], (1ull << 32) | (unsigned)total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (w == 0) {
+    int acc = 0;
+    for (int l0 = 0; l0 < b; l0 += 64) {
+      const int l = l0 + lane;
+      if (l < b) {
+        unsigned long long sv = 0;
+        unsigned spins = 0;
+        for (;;) {
+          sv = __hip_atomic_load(&status[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (sv >> 32) break;
+          if (++spins > kScanSpinLimit) {
+            s_bad = 1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        acc += (int)(unsigned)sv;
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) s_prefix = acc;
+  }
+  __syncthreads();
+  int run = s_prefix + wpre + incl - s;
+  int4 o4;
+  o4.x = run; run += v.x;
+  o4.y = run; run += v.y;
+  o4.z = run; run += v.z;
+  o4.w = run;
+  if (r0 + 3 < n) {
+    *reinterpret_cast<int4 *>(off + r0) = o4;
+  } else {
+    if (r0 < n) off[r0] = o4.x;
+    if (r0 + 1 < n) off[r0 + 1] = o4.y;
+    if (r0 + 2 < n) off[r0 + 2] = o4.z;
+  }
+  if (b == (int)gridDim.x - 1 && t == 0) {
+    const int T = s_prefix + total;
+    off[n] = T;
+    if (counts) {
+      counts[0] = 3 * T;
+      counts[1] = s_bad ? -1 : T;
+    }
+  }
+}
+
+// Workspace of marching cubes: the case byte of every cube, the row counts
+// followed by the scan's block status words (zeroed together before the
+// classify pass: zero_ints words from rowcnt_off), the row offsets.
+struct McLayout {
+  size_t cases_off, rowcnt_off, status_off, rowoff_off, total, zero_ints;
+  int nb;
+};
+inline McLayout mc_layout(const nrf_volume_params *vp) {
+  McLayout L;
+  const size_t V = (size_t)volume_count(vp), R = (size_t)vp->dims[1] * vp->dims[2];
+  L.nb = (int)((R + kScanBlock - 1) / kScanBlock);
+  L.cases_off = 0;
+  L.rowcnt_off = align_up(V, 256);
+  L.status_off = L.rowcnt_off + align_up(R * 4, 8);
+  L.rowoff_off = align_up(L.status_off + (size_t)L.nb * 8, 256);
+  L.total = L.rowoff_off + align_up((R + 1) * 4, 256);
+  L.zero_ints = (L.status_off - L.rowcnt_off) / 4 + (size_t)L.nb * 2;
+  return L;
+}
+
+// The scan of the row counts: the multi-workgroup form, or one workgroup
+// beyond kScanMaxBlocks blocks.
+inline void mc_scan_rows(const McLayout &L, const int32_t *rowcnt, int32_t *rowoff, int R, int32_t *counts,
+                         void *workspace, hipStream_t st) {
+  if (L.nb <= kScanMaxBlocks) {
+    auto *status = reinterpret_cast<unsigned long long *>(reinterpret_cast<char *>(workspace) + L.status_off);
+    scan_rows_lb_kernel<<<L.nb, kScanThreads, 0, st>>>(rowcnt, rowoff, R, counts, status);
+  } else {
+    scan_rows_kernel<<<1, 1024, 0, st>>>(rowcnt, rowoff, R, counts);
   }
 }
 

@@ -293,21 +293,22 @@ extern "C" int nrf_mc16_extract(const nrf_volume_params *vp, const uint16_t *tsd
   if (max_triangles < 0 || (max_triangles > 0 && (!vertices || !triangles))) return NRF_ERR_INVALID_ARGUMENT;
   const int X = vp->dims[0], Y = vp->dims[1], Z = vp->dims[2];
   if (Y > 65535 || Z > 65535) return NRF_ERR_TOO_LARGE;
-  const size_t V = (size_t)volume_count(vp), R = (size_t)Y * Z;
-  const size_t rowcnt_off = align_up(V, 256), rowoff_off = rowcnt_off + align_up(R * 4, 256);
-  if (workspace_bytes < rowoff_off + align_up((R + 1) * 4, 256)) return NRF_ERR_WORKSPACE;
+  const size_t R = (size_t)Y * Z;
+  const McLayout L = mc_layout(vp);
+  if (workspace_bytes < L.total) return NRF_ERR_WORKSPACE;
   char *ws = reinterpret_cast<char *>(workspace);
-  uint8_t *cases = reinterpret_cast<uint8_t *>(ws);
-  int32_t *rowcnt = reinterpret_cast<int32_t *>(ws + rowcnt_off);
-  int32_t *rowoff = reinterpret_cast<int32_t *>(ws + rowoff_off);
+  uint8_t *cases = reinterpret_cast<uint8_t *>(ws + L.cases_off);
+  int32_t *rowcnt = reinterpret_cast<int32_t *>(ws + L.rowcnt_off);
+  int32_t *rowoff = reinterpret_cast<int32_t *>(ws + L.rowoff_off);
   hipStream_t st = as_stream(stream);
   const __half *t16 = reinterpret_cast<const __half *>(tsdf16);
-  zero_i32<<<(unsigned)((R + 255) / 256), 256, 0, st>>>(rowcnt, (int)R);
+  const int nz = (int)L.zero_ints;  // the counts and the scan's status words
+  zero_i32<<<(unsigned)((nz + 255) / 256), 256, 0, st>>>(rowcnt, nz);
   NRF_LAUNCH_CHECK();
   classify_brick_kernel<<<classify_brick_grid(X, Y, Z), kMcBrickThreads, 0, st>>>(VoxelReaderF16{t16, weight8}, X,
                                                                                    Y, Z, cases, rowcnt);
   NRF_LAUNCH_CHECK();
-  scan_rows_kernel<<<1, 1024, 0, st>>>(rowcnt, rowoff, (int)R, counts);
+  mc_scan_rows(L, rowcnt, rowoff, (int)R, counts, workspace, st);
   NRF_LAUNCH_CHECK();
   if (X >= 2 && Y >= 2 && Z >= 2 && max_triangles > 0) {
     emit16_kernel<<<dim3(Y, Z), kMcBlock, 0, st>>>(t16, X, Y, Z,
