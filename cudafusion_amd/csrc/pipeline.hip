@@ -458,7 +458,7 @@ extern "C" nrf_batch *nrf_batch_create(nrf_pipeline *const *pipelines, int count
   b->use_graph = use_graph ? 1 : 0;
   nrf_batch_tuning t{};
   t.lanes_per_row = 4;
-  t.wg_nodes = 40;
+  t.wg_nodes = 32;
   t.z_chunk = 4;
   if (tuning) t = *tuning;
   bool ok = true;
@@ -664,7 +664,7 @@ static int batch_tune(nrf_batch *b, const BatchCorrs &fc, const BatchDepth &fd, 
     rc = NRF_ERR_HIP;
   }
   best.integrate_ms = -1.f;
-  const int zcs[] = {1, 2, 4, 8};
+  const int zcs[] = {1, 2, 4, 8, 16};
   for (int zc : zcs) {
     if (rc) break;
     IntegrateBatch *ib = nullptr;

@@ -435,12 +435,12 @@ typedef struct nrf_batch_frame {
 typedef struct nrf_batch_tuning {
   int32_t lanes_per_row;    /* PCG lanes per scalar row: 1, 2 or 4 */
   int32_t wg_nodes;         /* PCG target nodes per workgroup */
-  int32_t z_chunk;          /* integration voxels per lane along z: 1, 2, 4, 8 */
+  int32_t z_chunk;          /* integration voxels per lane along z: 1, 2, 4, 8, 16 */
   int32_t launch_groups;    /* out: persistent PCG launches per iteration */
   float pcg_ms;             /* out (nrf_batch_autotune): PCG stage time, all instances */
   float integrate_ms;       /* out (nrf_batch_autotune): integration time, all instances */
 } nrf_batch_tuning;
-/* tuning NULL: 4 lanes per row, 40 nodes per workgroup, z_chunk 4.
+/* tuning NULL: 4 lanes per row, 32 nodes per workgroup, z_chunk 4.
    use_graph: replay each step as a captured hipGraph (re-captured when the
    frame pointers, counts or poses change). */
 nrf_batch *nrf_batch_create(nrf_pipeline *const *pipelines, int count, const nrf_batch_tuning *tuning,
