@@ -73,6 +73,22 @@ def observed_voxels(h) -> int:
     return int((knn.reshape(-1, 4).max(axis=1) >= 0).sum())
 
 
+def knn_id_bytes(h) -> int:
+    """kNN id bytes the warped integration reads: the 16-byte ids of every
+    64-voxel run along x that holds an observed voxel, plus the run map's
+    4-byte word per run (runs without one integrate unwarped unread)."""
+    import numpy as np
+    X, Y, Z = (int(d) for d in h.w.dims)
+    knn, _ = h.download_knn()
+    obs = (knn.reshape(Z, Y, X, 4).max(axis=3) >= 0)
+    nsx = (X + 63) // 64
+    pad = np.zeros((Z, Y, nsx * 64), bool)
+    pad[:, :, :X] = obs
+    occ = pad.reshape(Z, Y, nsx, 64).any(axis=3)
+    run_len = np.minimum(64, X - 64 * np.arange(nsx))
+    return int((occ * run_len[None, None, :]).sum()) * 16 + Z * Y * nsx * 4
+
+
 def cpu_baseline(h, frames: int, threads: int):
     """The oracle (C++/OpenMP restatement, FP32) on `frames` frames of the
     same workload: GN solve + warped integration + marching cubes."""
@@ -213,9 +229,9 @@ def main() -> int:
     # algorithmic bytes per launch (counted once, minimum traffic of the algorithm)
     n_obs = observed_voxels(h)
     algo = {
-        # kNN ids of every voxel (16 B), kNN weights of the voxels with a snapshot neighbour (16 B), read +
-        # write of updated voxels, depth, centred node table
-        "integrate": V * 16 + n_obs * 16 + upd * 24 + 4 * w.width * w.height + 32 * h.N,
+        # kNN ids of the runs holding an observed voxel (16 B each) and the run map, kNN weights of the
+        # voxels with a snapshot neighbour (16 B), read + write of updated voxels, depth, centred node table
+        "integrate": knn_id_bytes(h) + n_obs * 16 + upd * 24 + 4 * w.width * w.height + 32 * h.N,
         # per iteration: BSR values + column ids, row pointers, p gather + Ap, block-Jacobi
         # blocks, r/z/x/p vector traffic of the updates and the three dot products
         "pcg": w.pcg_iterations * (nb * (36 * 4 + 4) + 4 * (h.N + 1) + 2 * 24 * h.N + h.N * 36 * 4
@@ -492,8 +508,7 @@ def run_batched(args) -> int:
     # integration roofline of the batched launch (algorithmic bytes as in the single-instance line)
     algo = 0
     for h, u in zip(batch, upd):
-        V = int(np.prod(h.w.dims))
-        algo += V * 16 + observed_voxels(h) * 16 + u * 24 + 4 * h.w.width * h.w.height + 32 * h.N
+        algo += knn_id_bytes(h) + observed_voxels(h) * 16 + u * 24 + 4 * h.w.width * h.w.height + 32 * h.N
     int_ms = stages[1] if stages[1] > 0 else tune.integrate_ms  # graph replays have no stage split
     ach = algo / (int_ms * 1e-3) / 1e9
     out = {
