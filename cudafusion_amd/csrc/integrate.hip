@@ -53,6 +53,13 @@ __device__ inline DQ fetch_centered(const IntegrateArgs &a, int id, bool table) 
   return dq_centered(a.transforms[id], f3(n.pos));
 }
 
+// A lane's ZC voxels (one column along z) go through three stages so that
+// each stage's loads are in flight together: (1) warp and projection, one
+// voxel after another (the kNN of a run without an observed voxel is not
+// read: those voxels integrate unwarped, as with all-(-1) ids), leaving the
+// pixel and camera depth of each voxel; (2) the depth pixels; (3) the
+// voxels that pass the tests, then their updates.  Same arithmetic per
+// voxel as one voxel at a time.
 template <bool WARP, bool TABLE, int ZC>
 __device__ void integrate_tile(const IntegrateArgs &a, int bx, int by, int bz) {
   const int x = bx * kTileX + (threadIdx.x & (kTileX - 1));
@@ -62,11 +69,19 @@ __device__ void integrate_tile(const IntegrateArgs &a, int bx, int by, int bz) {
   if (x < a.X && y < a.Y) {
     const float px_w = a.ox + (x + 0.5f) * a.s;
     const float py_w = a.oy + (y + 0.5f) * a.s;
+    const int plane = a.X * a.Y, col = y * a.X + x;
+    int pix[ZC];
+    float czs[ZC];
+#pragma unroll
+    for (int dz = 0; dz < ZC; ++dz) {
+      pix[dz] = -1;
+      czs[dz] = 0.f;
+    }
 #pragma unroll
     for (int dz = 0; dz < ZC; ++dz) {
       const int z = z0 + dz;
       if (z >= a.Z) break;
-      const int idx = z * a.X * a.Y + y * a.X + x;
+      const int idx = z * plane + col;
       if (WARP && a.opt && a.opt[idx] < a.min_opt) continue;
       float3 p = f3(px_w, py_w, a.oz + (z + 0.5f) * a.s);
       if (WARP) {
@@ -75,31 +90,30 @@ __device__ void integrate_tile(const IntegrateArgs &a, int bx, int by, int bz) {
         int4 id4 = make_int4(-1, -1, -1, -1);
         const int run = __builtin_amdgcn_readfirstlane((z * a.Y + y) * a.nsx + bx);  // one per wave
         if (!a.knn_rows || a.knn_rows[run] != 0u) id4 = a.knn[idx];
-        // voxels unobserved at the snapshot (all ids -1, most of the volume)
-        // integrate unwarped: their weights are never read
-        float4 w4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        if ((id4.x & id4.y & id4.z & id4.w) >= 0) w4 = a.knn_w[idx];  // sign bit clear: some id >= 0
-        const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
-        const float ws[4] = {w4.x, w4.y, w4.z, w4.w};
-        DQ b;
-        b.r = make_float4(0.f, 0.f, 0.f, 0.f);
-        b.d = make_float4(0.f, 0.f, 0.f, 0.f);
-        float wsum = 0.f;
+        if ((id4.x & id4.y & id4.z & id4.w) >= 0) {  // sign bit clear: some id >= 0
+          const float4 w4 = a.knn_w[idx];
+          const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
+          const float ws[4] = {w4.x, w4.y, w4.z, w4.w};
+          DQ b;
+          b.r = make_float4(0.f, 0.f, 0.f, 0.f);
+          b.d = make_float4(0.f, 0.f, 0.f, 0.f);
+          float wsum = 0.f;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int id = ids[k];
-          const float w = ws[k];
-          if (id < 0 || id >= a.N || w < 1e-8f) continue;
-          DQ q = fetch_centered(a, id, TABLE);
-          if (qdot(q.r, b.r) < 0) {
-            q.r.x = -q.r.x; q.r.y = -q.r.y; q.r.z = -q.r.z; q.r.w = -q.r.w;
-            q.d.x = -q.d.x; q.d.y = -q.d.y; q.d.z = -q.d.z; q.d.w = -q.d.w;
+          for (int k = 0; k < 4; ++k) {
+            const int id = ids[k];
+            const float w = ws[k];
+            if (id < 0 || id >= a.N || w < 1e-8f) continue;
+            DQ q = fetch_centered(a, id, TABLE);
+            if (qdot(q.r, b.r) < 0) {
+              q.r.x = -q.r.x; q.r.y = -q.r.y; q.r.z = -q.r.z; q.r.w = -q.r.w;
+              q.d.x = -q.d.x; q.d.y = -q.d.y; q.d.z = -q.d.z; q.d.w = -q.d.w;
+            }
+            b.r.x += w * q.r.x; b.r.y += w * q.r.y; b.r.z += w * q.r.z; b.r.w += w * q.r.w;
+            b.d.x += w * q.d.x; b.d.y += w * q.d.y; b.d.z += w * q.d.z; b.d.w += w * q.d.w;
+            wsum += w;
           }
-          b.r.x += w * q.r.x; b.r.y += w * q.r.y; b.r.z += w * q.r.z; b.r.w += w * q.r.w;
-          b.d.x += w * q.d.x; b.d.y += w * q.d.y; b.d.z += w * q.d.z; b.d.w += w * q.d.w;
-          wsum += w;
+          if (wsum >= 1e-8f) p = dq_transform(dq_normalize(b), p);
         }
-        if (wsum >= 1e-8f) p = dq_transform(dq_normalize(b), p);
       }
       const float cxp = a.T.m[0][0] * p.x + a.T.m[0][1] * p.y + a.T.m[0][2] * p.z + a.T.m[0][3];
       const float cyp = a.T.m[1][0] * p.x + a.T.m[1][1] * p.y + a.T.m[1][2] * p.z + a.T.m[1][3];
@@ -112,16 +126,36 @@ __device__ void integrate_tile(const IntegrateArgs &a, int bx, int by, int bz) {
       const int u = (int)rintf(fu);
       const int v = (int)rintf(fv);
       if (u < 0 || u >= a.W || v < 0 || v >= a.H) continue;
-      const float d = a.depth[v * a.W + u];
-      if (d <= 0) continue;
-      const float sdf = d - czp;
-      if (sdf < -a.trunc) continue;
-      const float t = fminf(1.f, sdf / a.trunc);
-      nrf_voxel &vx = a.vox[idx];
-      const float w_old = vx.weight;
-      const float dw = w_old * a.decay;
+      pix[dz] = v * a.W + u;
+      czs[dz] = czp;
+    }
+    float tv[ZC];
+#pragma unroll
+    for (int dz = 0; dz < ZC; ++dz) tv[dz] = pix[dz] >= 0 ? a.depth[pix[dz]] : 0.f;
+#pragma unroll
+    for (int dz = 0; dz < ZC; ++dz) {  // depth -> t, or NaN when the voxel is not updated
+      const float d = tv[dz];
+      const float sdf = d - czs[dz];
+      const bool upd = pix[dz] >= 0 && !(d <= 0) && !(sdf < -a.trunc);
+      tv[dz] = upd ? fminf(1.f, sdf / a.trunc) : __int_as_float(0x7fc00000);
+      pix[dz] = upd ? 1 : 0;
+    }
+    float2 tw[ZC];
+#pragma unroll
+    for (int dz = 0; dz < ZC; ++dz) {
+      tw[dz] = make_float2(0.f, 0.f);
+      if (pix[dz]) {
+        const nrf_voxel &vx = a.vox[(z0 + dz) * plane + col];
+        tw[dz] = make_float2(vx.tsdf, vx.weight);
+      }
+    }
+#pragma unroll
+    for (int dz = 0; dz < ZC; ++dz) {
+      if (!pix[dz]) continue;
+      nrf_voxel &vx = a.vox[(z0 + dz) * plane + col];
+      const float dw = tw[dz].y * a.decay;
       const float nw = dw + 1.f;
-      vx.tsdf = (vx.tsdf * dw + t * 1.f) / nw;
+      vx.tsdf = (tw[dz].x * dw + tv[dz] * 1.f) / nw;
       vx.weight = fminf(nw, a.wmax);
       ++n_upd;
     }
