@@ -213,6 +213,7 @@ __global__ __launch_bounds__(kSortThreads) void mem_sort_kernel(const int32_t *_
 }
 
 // ------------------------------------------------------------ assembly
+struct PcgCtl;
 struct AsmArgs {
   const nrf_corr *corrs;
   const float *canonical;  // NULL: use stored src
@@ -241,7 +242,11 @@ struct AsmArgs {
   // chunks leave per-chunk partial sums in part
   const int32_t *chunk_off, *chunk_row;
   float *part;
+  // when set, the assembly's last launch opens the next PCG launch (advances
+  // its epoch and clears its abort flag), which then needs no launch of its own
+  PcgCtl *pcg_open;
 };
+__device__ void pcg_open(PcgCtl *ctl);
 
 // Memberships per assembly chunk: a row with more is split, so that the
 // longest serial chain of 32-membership batches is 4 batches long (128
@@ -495,6 +500,7 @@ __device__ inline void row_tail(const AsmArgs &a, int i, int rb, int rlen, float
 __global__ __launch_bounds__(64) void assemble_rows_kernel(AsmArgs a) {
   extern __shared__ float lds[];
   const int i = blockIdx.x;
+  if (a.pcg_open && i == 0 && threadIdx.x == 0) pcg_open(a.pcg_open);
   if (a.stop && *a.stop) return;
   const int lane = threadIdx.x;
   const int rb = a.row_ptr[i], rlen = a.row_ptr[i + 1] - rb;
@@ -910,6 +916,7 @@ template <int NS, int PREC>
 __global__ __launch_bounds__(64) void assemble_rows_tail_kernel(AsmArgs a) {
   extern __shared__ float lds[];
   __shared__ float s_rhs[6];
+  if (a.pcg_open && blockIdx.x == 0 && threadIdx.x == 0) pcg_open(a.pcg_open);
   assemble_tail<NS, PREC>(a, (int)blockIdx.x, lds, s_rhs);
 }
 
@@ -1754,11 +1761,13 @@ __global__ void axpy_kernel(float *__restrict__ x, const float *__restrict__ d, 
   if (q < n) x[q] += d[q];
 }
 
+// Opens a PCG launch: a new epoch for its tags, the abort flag cleared.
+__device__ void pcg_open(PcgCtl *ctl) {
+  ctl->epoch += 1;
+  ctl->abort = 0;
+}
 __global__ void pcg_prep_kernel(PcgCtl *ctl) {
-  if (threadIdx.x == 0) {
-    ctl->epoch += 1;
-    ctl->abort = 0;
-  }
+  if (threadIdx.x == 0) pcg_open(ctl);
 }
 
 __device__ inline void gn_begin_body(int32_t *stop, nrf_pcg_result *res, nrf_solver_totals *tot) {
@@ -2361,6 +2370,7 @@ static AsmArgs solver_asm_args(const nrf_solver *s, const nrf_corr *corrs, const
   a.crec = nullptr;
   a.arec = nullptr;
   a.n_es = 0;
+  a.pcg_open = nullptr;
   a.chunk_off = s->chunk_off;
   a.chunk_row = s->chunk_row;
   a.part = s->part;
@@ -2378,9 +2388,13 @@ static size_t asm_lds_bytes(int Rmax) {
   return (size_t)Rmax * 36 * 4 + (size_t)Rmax * 4 + stage * 4;
 }
 
+// open_pcg: the assembly's last launch also opens the PCG launch that follows
+// it (solver_pcg with `opened`).
 static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, int n, const float *canonical,
-                           const nrf_node *nodes, const DQ *T, const DQ *centered, int N, hipStream_t st) {
+                           const nrf_node *nodes, const DQ *T, const DQ *centered, int N, hipStream_t st,
+                           bool open_pcg = false) {
   AsmArgs a = solver_asm_args(s, corrs, canonical, nodes, T, centered, N);
+  a.pcg_open = open_pcg ? s->ctl : nullptr;
   const size_t lds = asm_lds_bytes(s->P.Rmax);
   if (lds > 64 * 1024) return NRF_ERR_TOO_LARGE;
   const int ns = asm_strips(s->P.Rmax);
@@ -2434,10 +2448,12 @@ static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, int n, const fl
   return NRF_OK;
 }
 
-static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxit, bool record, hipStream_t st);
+static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxit, bool record, hipStream_t st,
+                      bool opened = false);
 
-static int solver_pcg(nrf_solver *s, float *x, int N, hipStream_t st) {
-  int rc = pcg_launch(s, s->b, x, N, s->params.pcg_iterations, true, st);
+// opened: the launch was opened by the assembly (solver_assemble's open_pcg)
+static int solver_pcg(nrf_solver *s, float *x, int N, hipStream_t st, bool opened = false) {
+  int rc = pcg_launch(s, s->b, x, N, s->params.pcg_iterations, true, st, opened);
   if (rc || s->params.refine_iterations <= 0) return rc;
   // one iterative-refinement step: r = b - A x accumulated in FP64, A d = r
   // by refine_iterations PCG iterations, x += d
@@ -2493,9 +2509,12 @@ static bool stream_capturing(hipStream_t st) {
   return cs != hipStreamCaptureStatusNone;
 }
 
-static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxit, bool record, hipStream_t st) {
-  pcg_prep_kernel<<<1, 64, 0, st>>>(s->ctl);
-  NRF_LAUNCH_CHECK();
+static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxit, bool record, hipStream_t st,
+                      bool opened) {
+  if (!opened) {
+    pcg_prep_kernel<<<1, 64, 0, st>>>(s->ctl);
+    NRF_LAUNCH_CHECK();
+  }
   if (record) sink_record(s, st);
   PcgArgs a = solver_pcg_args(s, rhs, x, N, maxit);
   void *args[] = {&a};
@@ -2625,9 +2644,9 @@ extern "C" int nrf_solver_solve(nrf_solver *s, const nrf_corr *corrs, int num_co
   gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res, s->d_tot);
   NRF_LAUNCH_CHECK();
   if ((rc = solver_assemble(s, corrs, num_corrs, nullptr, nodes, reinterpret_cast<const DQ *>(transforms), nullptr,
-                            num_nodes, st)))
+                            num_nodes, st, true)))
     return rc;
-  if ((rc = solver_pcg(s, delta_x, num_nodes, st))) return rc;
+  if ((rc = solver_pcg(s, delta_x, num_nodes, st, true))) return rc;
   return solver_fetch_result(s, result, st);
 }
 
@@ -2660,10 +2679,10 @@ extern "C" int nrf_solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const f
   g.res = s->d_res;
   for (int it = 0; it < std::max(1, gn->gn_iterations); ++it) {
     if ((rc = solver_assemble(s, corrs, num_corrs, canonical, nodes, reinterpret_cast<const DQ *>(transforms),
-                              reinterpret_cast<const DQ *>(centered), num_nodes, st)))
+                              reinterpret_cast<const DQ *>(centered), num_nodes, st, true)))
       return rc;
     sink_record(s, st);
-    if ((rc = solver_pcg(s, delta_x, num_nodes, st))) return rc;
+    if ((rc = solver_pcg(s, delta_x, num_nodes, st, true))) return rc;
     sink_record(s, st);
     if (g.validate) {
       gn_validate_kernel<<<1, 1024, 0, st>>>(g);
