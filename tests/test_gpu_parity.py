@@ -501,3 +501,42 @@ def test_true_residual_and_refinement(sphere, orc, solver_inputs):
     print(f"[refine] true relative residual: 10 it {rel10:.2e}, 10 + refine 10 {rel_ref:.2e}, 20 it {rel20:.2e}",
           flush=True)
     assert rel_ref < 0.5 * rel10 and rel_ref < 3 * rel20
+
+
+def test_pipeline_step_odd_volume(gpu, orc):
+    """A pipeline step on a volume whose sizes are not multiples of the
+    kernels' tiles: x = 100 (one full and one partial 64-voxel run of the
+    integration's kNN run map), 71 x 81 rows of cubes (several blocks of the
+    marching-cubes row scan, the last one partial).  The warped integration
+    equals the FP32 oracle's (weights bit-exact, TSDF to 1e-5) on the same
+    transforms and kNN, and the mesh equals the oracle's marching cubes of
+    the resulting volume."""
+    from dataclasses import replace
+
+    import torch
+
+    from cudafusion_amd import scene as S
+    from cudafusion_amd.harness import Harness
+    w = replace(S.sphere_workload(frames=2), dims=(100, 71, 81))
+    h = Harness(workload=w, frames=1, arap_consistent_sign=True, validate=False, pcg_pipelined=2)
+    h.setup()
+    vox0 = h.download_voxels().copy()
+    knn, knn_w = h.download_knn()
+    assert (knn.reshape(-1, 4).max(axis=1) >= 0).sum() > 1000
+    h.step(1)
+    torch.cuda.synchronize()
+    vox1 = h.download_voxels().copy()
+    T = h.download_transforms().copy()
+    ref = vox0.copy()
+    depth = h.frames_host[1].depth.reshape(-1)
+    orc.integrate_f32(h.vp, ref, depth, h.cam, h.pose, h.nodes, T, knn, knn_w)
+    changed = vox1["weight"] != vox0["weight"]
+    print(f"[odd] {int(changed.sum())} voxels updated, {h.triangle_count()} triangles", flush=True)
+    assert changed.sum() > 1000
+    np.testing.assert_array_equal(vox1["weight"], ref["weight"])
+    np.testing.assert_allclose(vox1["tsdf"], ref["tsdf"], atol=1e-5)
+    _, rv, _ = orc.mc_f32(h.vp, vox1, normals=False)
+    v = h.mesh()
+    assert h.triangle_count() == len(rv) // 3 and len(rv) > 300
+    np.testing.assert_allclose(v, rv, atol=1e-6)
+    h.close()
