@@ -308,7 +308,8 @@ const nrf_pcg_result *nrf_solver_device_result(const nrf_solver *s);
 
 /* Layout of the persistent PCG (no reference counterpart): lanes per scalar
    row (1, 2 or 4) and the target number of nodes per workgroup (capped at
-   1024 / (6 lanes_per_row)); rebuilds the partition of the current pattern.
+   32, 80 and 170 for 4, 2 and 1 lanes per row); rebuilds the partition of
+   the current pattern.
    Default: 4 lanes per row, 24 nodes per workgroup.  The textbook PCG form
    (pcg_pipelined = 0) needs 4 lanes per row.  Synchronous; no solve of this
    solver may be in flight, and a pipeline captured into a graph is captured
