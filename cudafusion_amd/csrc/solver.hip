@@ -31,6 +31,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "nrf_internal.h"
@@ -983,6 +984,14 @@ __device__ inline double dpp_add(double v) {
   const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, 0xF, false);
   return v + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
+// A double that is equal in every lane, moved to scalar registers (read
+// from the first active lane).
+__device__ inline double uniform_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)b);
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 // A float from another lane of the same quad (DPP quad_perm; 0xB1 reads
 // lane ^ 1, 0x4E lane ^ 2).  Every lane of the wave must take part.
 template <int CTRL>
@@ -1075,11 +1084,15 @@ struct PcgArgs {
   const int32_t *stop;
 };
 
+// The words live in global memory: the casts give global (not flat)
+// instructions also where the pointer came from memory (batch kernels).
+typedef __attribute__((address_space(1))) ll_word global_ll_word;
 __device__ inline void ll_put(ll_word *p, unsigned tag, unsigned payload) {
-  __hip_atomic_store(p, ((ll_word)tag << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store((global_ll_word *)p, ((ll_word)tag << 32) | payload, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline ll_word ll_load(const ll_word *p) {
-  return __hip_atomic_load(const_cast<ll_word *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load((global_ll_word *)const_cast<ll_word *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline bool ll_has(ll_word w, unsigned tag) { return (unsigned)(w >> 32) == tag; }
 
@@ -1422,7 +1435,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
 // Same iterates as pcg_kernel in exact arithmetic; in FP32 the one extra
 // recurrence (for Ap) keeps the iterates within a few ulps of the textbook
 // ones.  Layout, publication and the lane-per-row split are pcg_kernel's.
-template <int SUB, bool CG1>
+template <int SUB, int FORM>
 __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, int &s_bad, unsigned &s_base) {
   constexpr int ROWS = kPcgThreads / SUB;
   if (a.stop && *a.stop) return;
@@ -1481,7 +1494,7 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
   // product reads every register block without a branch (adding the exact
   // zeros leaves the sum's bits unchanged).  Blocks past NR (rows longer
   // than NR SUB blocks) are read from LDS or global memory.
-  constexpr int NR = SUB == 4 ? 3 : (SUB == 2 ? 5 : 4);
+  constexpr int NR = FORM == 2 ? (SUB == 4 ? 2 : (SUB == 2 ? 3 : 3)) : SUB == 4 ? 3 : (SUB == 2 ? 5 : 4);
   float Ar[NR][6];
   int Lr[NR];
 #pragma unroll
@@ -1500,7 +1513,7 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
   // own nodes' entries of ph for the product.  A node's six r values come
   // from lanes of the same wave: LDS store and loads in program order, no
   // barrier.
-  auto precond_publish = [&](float r, unsigned tag) -> float {
+  auto precond_publish = [&](float r, unsigned tag, ll_word *pub) -> float {
     if (lead) rl[q] = r;
     if constexpr (WN)
       __builtin_amdgcn_wave_barrier();  // keeps the loads after the store
@@ -1510,7 +1523,7 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
     if (own) {
       for (int c = 0; c < 6; ++c) z += M[c] * rl[li * 6 + c];
       if (lead) {
-        ll_put(a.zpub + s, tag, __float_as_uint(z));
+        ll_put(pub + s, tag, __float_as_uint(z));
         ph[zo] = z;
       }
     }
@@ -1520,49 +1533,158 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
   // `tag` (own nodes' entries are already in ph).  The callers order the
   // gather after every read of ph by the previous product (a barrier lies
   // between).
-  auto product = [&](unsigned tag) -> float {
+  // ACC = float: products and sums in FP32.  ACC = double: each product is
+  // exact in FP64 (two FP32 factors), the sums FP64.
+  auto product = [&](unsigned tag, const ll_word *pub, auto acc0) {
+    using ACC = decltype(acc0);
     for (int e = t; e < nh * 6; e += kPcgThreads) {
       const int nd = e == t ? hn0 : a.halo[h0 + e / 6], c = e % 6;
       if (nd < n0 || nd >= n1) {
-        const ll_word *src = a.zpub + 6 * nd + c;
+        const ll_word *src = pub + 6 * nd + c;
         ll_word wv = ll_load(src);
         if (!ll_has(wv, tag)) wv = ll_wait2(src, tag, a.ctl, &s_bad);
         ph[e] = __uint_as_float((unsigned)wv);
       }
     }
     __syncthreads();
-    float y = 0.f;
+    ACC y = 0;
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
       const float *pj = ph + Lr[k];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) y += Ar[k][c] * pj[c];
+      for (int c = 0; c < 6; ++c) {
+        y += (ACC)Ar[k][c] * (ACC)pj[c];
+      }
     }
     if (own && bk_rest < re) {
       if (a.lds_a) {
         for (int bk = bk_rest; bk < re; bk += SUB) {
           const float *A = Al + (size_t)(bk - blk0) * 36 + row * 6;
           const float *pj = ph + Ll[bk - blk0] * 6;
-          for (int c = 0; c < 6; ++c) y += A[c] * pj[c];
+          for (int c = 0; c < 6; ++c) y += (ACC)A[c] * (ACC)pj[c];
         }
       } else {
         for (int bk = bk_rest; bk < re; bk += SUB) {
           const float *A = a.values + (size_t)bk * 36 + row * 6;
           const float *pj = ph + a.lcol[bk] * 6;
-          for (int c = 0; c < 6; ++c) y += A[c] * pj[c];
+          for (int c = 0; c < 6; ++c) y += (ACC)A[c] * (ACC)pj[c];
         }
       }
     }
     // the row's SUB partial products, combined identically in all its lanes
-    if constexpr (SUB >= 2) y += quad_perm_f<0xB1>(y);  // lane ^ 1
-    if constexpr (SUB >= 4) y += quad_perm_f<0x4E>(y);  // lane ^ 2
+    if constexpr (std::is_same<ACC, double>::value) {
+      if constexpr (SUB >= 2) y = dpp_add<0xB1, 0xF>(y);  // lane ^ 1
+      if constexpr (SUB >= 4) y = dpp_add<0x4E, 0xF>(y);  // lane ^ 2
+    } else {
+      if constexpr (SUB >= 2) y += quad_perm_f<0xB1>(y);  // lane ^ 1
+      if constexpr (SUB >= 4) y += quad_perm_f<0x4E>(y);  // lane ^ 2
+    }
     return y;
   };
 
   __syncthreads();
   const unsigned base = s_base;
   zo = (s_o0 + li) * 6 + row;
-  if constexpr (CG1) {
+  // the reduction partials are held by all SUB lanes of a row (equal copies)
+  const int nwa = WN ? (n1 - n0 + NPW - 1) / NPW : (S * SUB + 63) >> 6;
+  if constexpr (FORM == 2) {
+    // Pipelined PCG (Ghysels and Vanroose, 2014, Alg. 4, with the block-Jacobi
+    // M): besides x, r, u = M^-1 r and p, each row carries w = A u, s = A p,
+    // q = M^-1 s and zz = A q as recurrences.  The all-reduce of
+    // (r.r, r.u, w.u) starts an iteration and travels while m = M^-1 w is
+    // published, its halo gathered and n = A m formed, so the iteration's one
+    // halo exchange and one all-reduce overlap.  p.Ap = w.u - beta^2 (p.Ap)_old
+    // as in the single-reduction form; the same iterates as the textbook form
+    // in exact arithmetic.
+    // Rounding: these coupled recurrences stay close to the textbook iterates
+    // only while n is the product of A with exactly the m that enters q, so m
+    // is one FP32 value (published, in the own entries of ph and in q alike),
+    // n = A m is summed in FP64 from exact products, and the coupled vectors
+    // r, u, w, s, q, zz are FP64 (FP32 vectors or an FP32 sum, even per block,
+    // leave x 1e-4..1e-3 from the FP64 solve after 30 iterations and stall the
+    // residual near 1e-3).  x and p feed nothing else and stay FP32.
+    // The published m alternates between two arrays by point parity: a
+    // workgroup publishes its record of a point before it gathers that
+    // point's halo, so a neighbour can already publish the next point's m, but
+    // not the one after (that waits for this workgroup's next record).  After
+    // maxit updates one more all-reduce yields ||r||.
+    unsigned pt = 1;
+    float x = 0.f, p = 0.f;
+    double r = bv;
+    double u = precond_publish(bv, base + pt, a.ppub1);
+    double w = product(base + pt, a.ppub1, 0.0);
+    // every read of ph by this product ends before the first iteration
+    // stores the own nodes' m there (in the loop, the collection's barrier
+    // does this)
+    __syncthreads();
+    double sv = 0.0, qv = 0.0, zv = 0.0;
+    double v[3], gamma_old = 0.0, pAp = 0.0, rr = 0.0, bb = 0.0, tol2 = 0.0;
+    int iters = 0, flags = 0;
+    bool aborted = false;
+    for (int it = 0;; ++it) {
+      ++pt;
+      ll_word *pub = (pt & 1) ? a.ppub1 : a.ppub0;
+      const bool last = it >= a.maxit;
+      float m = 0.f;
+      double n = 0.0;
+      if (!last) m = precond_publish((float)w, base + pt, pub);
+      v[0] = own ? r * r : 0.0;
+      v[1] = own ? r * u : 0.0;
+      v[2] = own ? w * u : 0.0;
+      ar3_publish<SUB>(a, g, base + pt, (int)(pt & 1), v, red, nwa);
+      if (!last) n = product(base + pt, pub, 0.0);
+      aborted = ar3_collect(a, base + pt, (int)(pt & 1), v, red, &s_bad);
+      if (aborted) break;
+      // the sums are equal in all lanes: scalar registers for the loop state
+      rr = uniform_d(v[0]);
+      iters = it;
+      if (it == 0) {
+        bb = rr;
+        const double tol = a.rel ? (double)a.tol * sqrt(bb) : (double)a.tol;
+        tol2 = uniform_d(tol * tol);
+      } else if (rr < tol2) {
+        flags |= 2;
+        break;
+      }
+      if (last) break;
+      const double gamma = uniform_d(v[1]);
+      double beta = 0.0;
+      if (it == 0) {
+        pAp = uniform_d(v[2]);
+      } else {
+        beta = uniform_d(gamma / gamma_old);
+        pAp = uniform_d(v[2] - beta * beta * pAp);
+      }
+      if (a.rel ? !(pAp > 0.0) : fabs(pAp) < 1e-12) {
+        flags |= 1;
+        break;
+      }
+      gamma_old = gamma;
+      const double alpha = uniform_d(gamma / pAp);
+      zv = n + beta * zv;
+      qv = (double)m + beta * qv;
+      sv = w + beta * sv;
+      p = (float)u + (float)beta * p;
+      x += (float)alpha * p;
+      r -= alpha * sv;
+      u -= alpha * qv;
+      w -= alpha * zv;
+    }
+    if (aborted) flags |= 8;
+    if (lead) a.x[s] = x;
+    if (g == 0 && t == 0) {
+      a.res->iterations = iters;
+      a.res->total_iterations += iters;
+      a.res->flags |= flags;
+      a.res->final_residual = sqrt(rr);
+      a.res->rhs_norm = sqrt(bb);
+      a.res->converged += (flags & 2) ? 1 : 0;
+      a.res->breakdowns += (flags & 1) ? 1 : 0;
+      a.res->runs += 1;
+    }
+    return;
+  }
+  if constexpr (FORM == 1) {
     // One all-reduce per iteration (Chronopoulos and Gear, 1989): with
     // gamma = r.z and delta = z.Az reduced together with r.r, p.Ap follows
     // from delta - beta gamma / alpha_old (equal to p.Ap in exact
@@ -1572,10 +1694,8 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
     // A p is formed as A z + beta A p as in the overlapped form.
     unsigned pt = 1;
     float x = 0.f, r = bv;
-    float z = precond_publish(r, base + pt);
-    float az = product(base + pt);
-    // the partials are held by all SUB lanes of a row (equal copies)
-    const int nwa = WN ? (n1 - n0 + NPW - 1) / NPW : (S * SUB + 63) >> 6;
+    float z = precond_publish(r, base + pt, a.zpub);
+    float az = product(base + pt, a.zpub, 0.f);
     double v[3] = {own ? (double)r * (double)z : 0.0, own ? (double)z * (double)az : 0.0,
                    own ? (double)bv * (double)bv : 0.0};
     ar3_publish<SUB>(a, g, base + pt, (int)(pt & 1), v, red, nwa);
@@ -1596,8 +1716,8 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
       x += alf * p;
       r += -alf * y;
       ++pt;
-      z = precond_publish(r, base + pt);
-      az = product(base + pt);
+      z = precond_publish(r, base + pt, a.zpub);
+      az = product(base + pt, a.zpub, 0.f);
       v[0] = own ? (double)r * (double)r : 0.0;
       v[1] = own ? (double)r * (double)z : 0.0;
       v[2] = own ? (double)z * (double)az : 0.0;
@@ -1634,10 +1754,10 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
   // init: x = 0, r = b, z = M^-1 r, p = z, Ap = A z (kernels/solver_kernels.cu:620-632)
   unsigned pt = 1;
   float x = 0.f, r = bv;
-  float z = precond_publish(r, base + pt);
+  float z = precond_publish(r, base + pt, a.zpub);
   double rz = lead ? (double)r * (double)z : 0.0, bb = lead ? (double)bv * (double)bv : 0.0;
   ar2_publish(a, g, base + pt, (int)(pt & 1), rz, bb, red);
-  float p = z, y = product(base + pt);
+  float p = z, y = product(base + pt, a.zpub, 0.f);
   bool aborted = ar2_collect(a, base + pt, (int)(pt & 1), rz, bb, red, &s_bad);
   double rr = bb;
   const double tol = a.rel ? (double)a.tol * sqrt(bb) : (double)a.tol;
@@ -1658,10 +1778,10 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
     ++pt;  // the (r.r, r.z) point
     x += alf * p;
     r += -alf * y;
-    z = precond_publish(r, base + pt);
+    z = precond_publish(r, base + pt, a.zpub);
     double rr_l = lead ? (double)r * (double)r : 0.0, rz_l = lead ? (double)r * (double)z : 0.0;
     ar2_publish(a, g, base + pt, (int)(pt & 1), rr_l, rz_l, red);
-    const float az = product(base + pt);
+    const float az = product(base + pt, a.zpub, 0.f);
     aborted = ar2_collect(a, base + pt, (int)(pt & 1), rr_l, rz_l, red, &s_bad);
     if (aborted) break;
     rr = rr_l;
@@ -1689,13 +1809,13 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
   }
 }
 
-template <int SUB, bool CG1>
+template <int SUB, int FORM>
 __global__ __launch_bounds__(kPcgThreads) void pcg_pipe_kernel(PcgArgs a) {
   extern __shared__ float lds[];
   __shared__ double red[kRedSize];
   __shared__ int s_bad;
   __shared__ unsigned s_base;
-  pcg_pipe_body<SUB, CG1>(a, (int)blockIdx.x, lds, red, s_bad, s_base);
+  pcg_pipe_body<SUB, FORM>(a, (int)blockIdx.x, lds, red, s_bad, s_base);
 }
 
 
@@ -2022,7 +2142,7 @@ __global__ void pcg_prep_batch_kernel(const InstDev *__restrict__ inst, int coun
 // instance wg_inst[b] as its workgroup wg_local[b].  The instances share no
 // data; each synchronises its own workgroups through its own records, so
 // only the workgroups of one instance need to be co-resident.
-template <int SUB, bool CG1>
+template <int SUB, int FORM>
 __global__ __launch_bounds__(kPcgThreads) void pcg_pipe_batch_kernel(const InstDev *__restrict__ inst,
                                                                      const int32_t *__restrict__ wg_inst,
                                                                      const int32_t *__restrict__ wg_local) {
@@ -2030,8 +2150,32 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_pipe_batch_kernel(const InstD
   __shared__ double red[kRedSize];
   __shared__ int s_bad;
   __shared__ unsigned s_base;
-  const PcgArgs a = inst[wg_inst[blockIdx.x]].pcg;
-  pcg_pipe_body<SUB, CG1>(a, wg_local[blockIdx.x], lds, red, s_bad, s_base);
+  // the instance and workgroup indices are uniform: read them into scalar
+  // registers so that the arguments (and every loop bound and branch on
+  // them) are uniform to the compiler, as kernel arguments are
+  const int k = __builtin_amdgcn_readfirstlane(wg_inst[blockIdx.x]);
+  const int g = __builtin_amdgcn_readfirstlane(wg_local[blockIdx.x]);
+  const PcgArgs a = inst[k].pcg;
+  pcg_pipe_body<SUB, FORM>(a, g, lds, red, s_bad, s_base);
+}
+
+// The persistent PCG kernels by lanes per row (1, 2, 4) and form (0
+// overlapped, 1 single reduction, 2 pipelined: pcg_pipelined - 1).
+template <int FORM>
+static const void *pcg_fn_of(int sub, bool batch) {
+  if (batch)
+    return sub == 1   ? reinterpret_cast<const void *>(pcg_pipe_batch_kernel<1, FORM>)
+           : sub == 2 ? reinterpret_cast<const void *>(pcg_pipe_batch_kernel<2, FORM>)
+                      : reinterpret_cast<const void *>(pcg_pipe_batch_kernel<4, FORM>);
+  return sub == 1   ? reinterpret_cast<const void *>(pcg_pipe_kernel<1, FORM>)
+         : sub == 2 ? reinterpret_cast<const void *>(pcg_pipe_kernel<2, FORM>)
+                    : reinterpret_cast<const void *>(pcg_pipe_kernel<4, FORM>);
+}
+static const void *pcg_pipe_fn(int sub, int form) {
+  return form == 2 ? pcg_fn_of<2>(sub, false) : form == 1 ? pcg_fn_of<1>(sub, false) : pcg_fn_of<0>(sub, false);
+}
+static const void *pcg_batch_fn(int sub, int form) {
+  return form == 2 ? pcg_fn_of<2>(sub, true) : form == 1 ? pcg_fn_of<1>(sub, true) : pcg_fn_of<0>(sub, true);
 }
 __global__ __launch_bounds__(1024) void gn_validate_batch_kernel(const InstDev *__restrict__ inst) {
   const GnArgs &g = inst[blockIdx.x].gn;
@@ -2278,22 +2422,12 @@ static int solver_build_partition(nrf_solver *s) {
   if ((st = upload(s->halo_off, s->c_halo_off, hoff))) return st;
   if ((st = upload(s->halo, s->c_halo, halo))) return st;
   if (s->pcg_lds > 65536 && s->pcg_lds > s->lds_attr) {
-    const void *fns[] = {reinterpret_cast<const void *>(pcg_kernel),
-                         reinterpret_cast<const void *>(pcg_pipe_kernel<1, false>),
-                         reinterpret_cast<const void *>(pcg_pipe_kernel<2, false>),
-                         reinterpret_cast<const void *>(pcg_pipe_kernel<4, false>),
-                         reinterpret_cast<const void *>(pcg_pipe_kernel<1, true>),
-                         reinterpret_cast<const void *>(pcg_pipe_kernel<2, true>),
-                         reinterpret_cast<const void *>(pcg_pipe_kernel<4, true>),
-                         reinterpret_cast<const void *>(pcg_pipe_batch_kernel<1, false>),
-                         reinterpret_cast<const void *>(pcg_pipe_batch_kernel<2, false>),
-                         reinterpret_cast<const void *>(pcg_pipe_batch_kernel<4, false>),
-                         reinterpret_cast<const void *>(pcg_pipe_batch_kernel<1, true>),
-                         reinterpret_cast<const void *>(pcg_pipe_batch_kernel<2, true>),
-                         reinterpret_cast<const void *>(pcg_pipe_batch_kernel<4, true>)};
-    bool ok = true;
-    for (const void *fn : fns)
-      ok = ok && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->pcg_lds) == hipSuccess;
+    bool ok = hipFuncSetAttribute(reinterpret_cast<const void *>(pcg_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->pcg_lds) == hipSuccess;
+    for (int form = 0; form < 3; ++form)
+      for (int sub : {1, 2, 4})
+        for (const void *fn : {pcg_pipe_fn(sub, form), pcg_batch_fn(sub, form)})
+          ok = ok && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->pcg_lds) == hipSuccess;
     if (ok) {
       s->lds_attr = s->pcg_lds;
     } else {
@@ -2520,14 +2654,8 @@ static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxi
   void *args[] = {&a};
   const bool pipe = s->params.pcg_pipelined != 0;
   if (!pipe && s->pcg_sub != kPcgSub) return NRF_ERR_INVALID_ARGUMENT;  // textbook form: 4 lanes per row only
-  const bool cg1 = s->params.pcg_pipelined == 2;
-  const void *fn = !pipe                     ? reinterpret_cast<const void *>(pcg_kernel)
-                   : cg1 && s->pcg_sub == 1  ? reinterpret_cast<const void *>(pcg_pipe_kernel<1, true>)
-                   : cg1 && s->pcg_sub == 2  ? reinterpret_cast<const void *>(pcg_pipe_kernel<2, true>)
-                   : cg1                     ? reinterpret_cast<const void *>(pcg_pipe_kernel<4, true>)
-                   : s->pcg_sub == 1         ? reinterpret_cast<const void *>(pcg_pipe_kernel<1, false>)
-                   : s->pcg_sub == 2         ? reinterpret_cast<const void *>(pcg_pipe_kernel<2, false>)
-                                             : reinterpret_cast<const void *>(pcg_pipe_kernel<4, false>);
+  const void *fn = pipe ? pcg_pipe_fn(s->pcg_sub, s->params.pcg_pipelined - 1)
+                        : reinterpret_cast<const void *>(pcg_kernel);
   if (s->coop && !stream_capturing(st)) {
     NRF_HIP_CHECK(hipLaunchCooperativeKernel(fn, dim3(s->G), dim3(kPcgThreads), args, (unsigned)s->pcg_lds, st));
   } else {
@@ -2560,7 +2688,7 @@ static int solver_ensure_pattern(nrf_solver *s, const nrf_node *d_nodes, int N, 
 // fit the 16 point bits of a published word's tag.
 static bool params_ok(const nrf_solver_params *p) {
   return p->pcg_iterations >= 0 && 1 + 2 * (long)p->pcg_iterations <= kPcgMaxPoints && p->jtj_precision >= 0 &&
-         p->jtj_precision <= 2 && p->pcg_pipelined >= 0 && p->pcg_pipelined <= 2;
+         p->jtj_precision <= 2 && p->pcg_pipelined >= 0 && p->pcg_pipelined <= 3;
 }
 
 extern "C" nrf_solver *nrf_solver_create(const nrf_solver_params *params, int max_nodes) {
@@ -2772,7 +2900,7 @@ extern "C" int nrf_solver_get_totals(nrf_solver *s, nrf_solver_totals *out, void
 struct nrf::SolverBatch {
   std::vector<SolverBatchInstance> in;
   int count = 0, sub = 4, prec = 0, ns = 1, gn_iters = 1, any_validate = 0, coop = 0;
-  bool cg1 = false;  // the single-reduction PCG form (pcg_pipelined = 2)
+  int form = 0;  // the PCG form of every instance (pcg_pipelined - 1)
   int max_N = 0, rows = 0, wgs = 0;
   size_t asm_lds = 0, pcg_lds = 0;
   InstDev *d_inst = nullptr;
@@ -2820,16 +2948,6 @@ static int upload_new(T *&p, const std::vector<T> &v) {
   return upload(p, cap, v);
 }
 
-static const void *pcg_batch_fn(int sub, bool cg1) {
-  if (cg1)
-    return sub == 1   ? reinterpret_cast<const void *>(pcg_pipe_batch_kernel<1, true>)
-           : sub == 2 ? reinterpret_cast<const void *>(pcg_pipe_batch_kernel<2, true>)
-                      : reinterpret_cast<const void *>(pcg_pipe_batch_kernel<4, true>);
-  return sub == 1   ? reinterpret_cast<const void *>(pcg_pipe_batch_kernel<1, false>)
-         : sub == 2 ? reinterpret_cast<const void *>(pcg_pipe_batch_kernel<2, false>)
-                    : reinterpret_cast<const void *>(pcg_pipe_batch_kernel<4, false>);
-}
-
 int solver_batch_create(const SolverBatchInstance *inst, int count, int lanes_per_row, int wg_nodes,
                         SolverBatch **out) {
   if (!inst || !out || count < 1 || count > kMaxBatch) return NRF_ERR_INVALID_ARGUMENT;
@@ -2861,12 +2979,12 @@ int solver_batch_create(const SolverBatchInstance *inst, int count, int lanes_pe
     if (!sp.pcg_pipelined || sp.refine_iterations != 0 || !s->asm_mfma || ns > 8 ||
         asm_lds_bytes(s->P.Rmax) > 64 * 1024 ||
         (k > 0 && (sp.jtj_precision != b->prec || std::max(1, q.gn.gn_iterations) != b->gn_iters ||
-                   (sp.pcg_pipelined == 2) != b->cg1))) {
+                   sp.pcg_pipelined - 1 != b->form))) {
       rc = NRF_ERR_INVALID_ARGUMENT;
       break;
     }
     if (k == 0) {
-      b->cg1 = sp.pcg_pipelined == 2;
+      b->form = sp.pcg_pipelined - 1;
       b->prec = sp.jtj_precision;
       b->gn_iters = std::max(1, q.gn.gn_iterations);
       b->coop = s->coop;
@@ -2914,7 +3032,7 @@ int solver_batch_create(const SolverBatchInstance *inst, int count, int lanes_pe
     }
   }
   if (!rc && b->pcg_lds > 65536 &&
-      hipFuncSetAttribute(pcg_batch_fn(b->sub, b->cg1), hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->pcg_lds) !=
+      hipFuncSetAttribute(pcg_batch_fn(b->sub, b->form), hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->pcg_lds) !=
           hipSuccess) {
     (void)hipGetLastError();
     rc = NRF_ERR_TOO_LARGE;
@@ -2922,7 +3040,7 @@ int solver_batch_create(const SolverBatchInstance *inst, int count, int lanes_pe
   if (!rc) {
     // launch groups: consecutive instances whose workgroups are co-resident
     int per_cu = 0;
-    NRF_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pcg_batch_fn(b->sub, b->cg1), kPcgThreads,
+    NRF_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pcg_batch_fn(b->sub, b->form), kPcgThreads,
                                                                b->pcg_lds));
     const int cap = per_cu * device_cu_count();
     int first = 0, used = 0, wg = 0;
@@ -3030,7 +3148,7 @@ static int batch_assemble(SolverBatch *b, const BatchCorrs &fr, hipStream_t st) 
 int solver_batch_pcg(SolverBatch *b, hipStream_t st) {
   pcg_prep_batch_kernel<<<1, 64, 0, st>>>(b->d_inst, b->count);
   NRF_LAUNCH_CHECK();
-  const void *fn = pcg_batch_fn(b->sub, b->cg1);
+  const void *fn = pcg_batch_fn(b->sub, b->form);
   const bool coop = b->coop && !stream_capturing(st);
   for (const auto &g : b->groups) {
     const InstDev *inst = b->d_inst;

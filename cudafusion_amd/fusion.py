@@ -235,7 +235,8 @@ class GaussNewtonSolver:
         relative_tolerance: bool = False
         precond_fp64: bool = False
         arap_consistent_sign: bool = False  # opt-in: ARAP rotation rows with the data term's sign
-        pcg_pipelined: int = 0  # opt-in: 1 A p by recurrence, product overlapping an all-reduce; 2 one all-reduce
+        pcg_pipelined: int = 0  # opt-in: 1 A p by recurrence, product overlapping an all-reduce; 2 one all-reduce;
+        # 3 pipelined: the one all-reduce overlapping the halo exchange and the product
         jtj_precision: int = 0  # 0 FP32, 1 BF16, 2 FP16 JtJ blocks on the matrix cores (RHS stays FP32)
         refine_iterations: int = 0  # > 0: one FP64-residual iterative-refinement step per solve
 

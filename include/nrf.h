@@ -114,8 +114,11 @@ typedef struct nrf_solver_params {
   int32_t pcg_pipelined;       /* 1: A p by the recurrence A z + beta A p, so that the
                                   product overlaps the (r.r, r.z) all-reduce; 2: in
                                   addition p.Ap from z.Az (Chronopoulos-Gear), reduced
-                                  with r.r and r.z: one all-reduce per iteration.  The
-                                  same iterates in exact arithmetic */
+                                  with r.r and r.z: one all-reduce per iteration; 3:
+                                  pipelined (Ghysels-Vanroose): A M^-1 A p and the like
+                                  as recurrences, so that the one all-reduce overlaps
+                                  the halo exchange and the product.  The same iterates
+                                  in exact arithmetic */
   int32_t arap_consistent_sign; /* 1: ARAP rotation Jacobian rows with the sign of
                                    d(omega x q)/d(omega), as the data term and the
                                    exp map use; 0 keeps the reference's negated rows
@@ -420,8 +423,8 @@ long long nrf_pipeline_updated_voxels(nrf_pipeline *p);
    instance.  A batch step of an instance gives the same bits as that
    pipeline's own nrf_pipeline_step with the same PCG layout.
    Requirements: every pipeline has its nodes set, the FP32 storage form,
-   the overlapped PCG (pcg_pipelined = 1), no refinement step, and all share
-   jtj_precision and gn_iterations.  Changing a member's nodes or PCG layout
+   one of the persistent PCG forms (pcg_pipelined 1, 2 or 3, the same for
+   all), no refinement step, and all share jtj_precision and gn_iterations.  Changing a member's nodes or PCG layout
    afterwards invalidates the batch (destroy and create it again). */
 #define NRF_MAX_BATCH 32
 typedef struct nrf_batch nrf_batch;

@@ -94,5 +94,5 @@ def test_host_checks_without_device():
     assert not lib.nrf_lp_create(0, 0, 1)
     assert lib.nrf_batch_step(None, None, None) == 1
     assert lib.nrf_lp_adam(None, 1e-3, 1, None) == 1
-    sp = _lib.solver_params(5, 30, 0.0, 10.0, 1.0, 1e-2, 0, 0, 0, 0, 3)  # pcg_pipelined 3 is not a form
+    sp = _lib.solver_params(5, 30, 0.0, 10.0, 1.0, 1e-2, 0, 0, 0, 0, 4)  # pcg_pipelined 4 is not a form
     assert not lib.nrf_solver_create(C.byref(sp), 100)

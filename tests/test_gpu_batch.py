@@ -28,7 +28,7 @@ def _workloads():
 def _sets(pcg_form=1):
     """Two harness sets over the same workloads (the second reuses the
     first's rendered frames and nodes); pcg_form 1 overlapped, 2 single
-    reduction."""
+    reduction, 3 pipelined."""
     from cudafusion_amd.harness import Harness
     a, b = [], []
     for w, kw in _workloads():
@@ -68,11 +68,12 @@ def _set_layout(hs, t):
                 "set_pcg_layout")
 
 
-def test_batch_matches_unbatched_eager(gpu):
+@pytest.mark.parametrize("pcg_form", [1, 3], ids=["overlapped", "pipelined"])
+def test_batch_matches_unbatched_eager(gpu, pcg_form):
     import torch
 
     from cudafusion_amd.harness import BatchHarness
-    a, b = _sets()
+    a, b = _sets(pcg_form)
     bh = BatchHarness(b)
     t = bh.tuning()
     assert (t.lanes_per_row, t.wg_nodes, t.z_chunk, t.launch_groups) == (4, 32, 4, 1)

@@ -86,11 +86,12 @@ def test_marching_cubes_full_size(body, orc):
     np.testing.assert_allclose(mesh, verts, atol=1e-6)
 
 
-@pytest.mark.parametrize("pcg_form", [1, 2], ids=["overlapped", "single_reduction"])
+@pytest.mark.parametrize("pcg_form", [1, 2, 3], ids=["overlapped", "single_reduction", "pipelined"])
 def test_tracking_sequence_consistent_policy(gpu, orc, pcg_form):
     """Eight frames of the sequence through the pipeline with the benchmark's
     solver policy (consistent ARAP sign, per-node clamp, no whole-step
-    rejection), with the overlapped and the single-reduction PCG forms.
+    rejection), with the overlapped, the single-reduction and the pipelined
+    (Ghysels-Vanroose) PCG forms.
     Per frame: the node transforms equal the FP64 oracle's solve from the
     same starting transforms (1e-4 rad / 1e-5 m), every GN step is
     applied, and the point-to-plane RMS of the correspondences after the solve

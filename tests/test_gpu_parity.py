@@ -269,14 +269,15 @@ def solver_inputs(sphere, orc):
     return dict(wf=wf, dcorr=dcorr, dcan=dcan, hcorr=hcorr, hcan=hcan)
 
 
-@pytest.mark.parametrize("pcg", ["textbook", "pipelined", "single_reduction"])
+@pytest.mark.parametrize("pcg", ["textbook", "pipelined", "single_reduction", "ghysels_vanroose"])
 @pytest.mark.parametrize("form", ["mfma", "scalar"])
 @pytest.mark.parametrize("consistent", [0, 1])
 def test_solver_system_and_pcg(sphere, orc, solver_inputs, consistent, form, pcg, monkeypatch):
     """solve(): one linearisation with the stored src (GaussNewtonSolver::solve),
     in the reference's ARAP sign convention and in the opt-in consistent one,
     with the data term on the matrix cores and in the scalar LDS form, and
-    with the textbook, the overlapped and the single-reduction PCG forms."""
+    with the textbook, the overlapped, the single-reduction and the pipelined
+    (Ghysels-Vanroose) PCG forms."""
     import torch
 
     from cudafusion_amd import _lib as L
@@ -300,7 +301,8 @@ def test_solver_system_and_pcg(sphere, orc, solver_inputs, consistent, form, pcg
     dcorr = torch.from_numpy(corrs.view(np.uint8).copy()).cuda()
     p = GaussNewtonSolver.Params(pcg_iterations=20, pcg_tolerance=0.0, lambda_smooth=w.lambda_smooth,
                                  arap_consistent_sign=bool(consistent),
-                                 pcg_pipelined={"textbook": 0, "pipelined": 1, "single_reduction": 2}[pcg])
+                                 pcg_pipelined={"textbook": 0, "pipelined": 1, "single_reduction": 2,
+                                                "ghysels_vanroose": 3}[pcg])
     monkeypatch.setenv("NRF_ASM_MFMA", "1" if form == "mfma" else "0")  # read when the solver is created
     s = GaussNewtonSolver(p, 8192)
     dx = torch.zeros(6 * N, dtype=torch.float32, device="cuda")
