@@ -1,7 +1,7 @@
 """Parity of one full non-rigid step at full scale (256^3 at 4 mm, 640x480,
 capsule body, 5 GN x 30 PCG) against the FP64 oracle on identical inputs.
 
-Bars (north_star): node transforms within 1e-4 rad / 1e-5 m of the FP64
+Bars: node transforms within 1e-4 rad / 1e-5 m of the FP64
 solver, TSDF within 1e-3 (normalised units) of the FP64 integration,
 marching-cubes case indices and triangle counts bit-exact given the same
 TSDF.  Size-independent properties are checked as well (the integration

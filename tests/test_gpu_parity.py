@@ -1,7 +1,7 @@
 """GPU parity of every kernel of the step against the CPU oracle, on the
 128^3 / 320x240 sphere workload (all calls go through the C ABI).
 
-Tolerances (north_star): TSDF within 1e-3 (in units of the truncation, i.e.
+Tolerances: TSDF within 1e-3 (in units of the truncation, i.e.
 of the stored normalised tsdf) of the FP64 oracle; marching-cubes case
 indices and triangle counts bit-exact given an identical TSDF; node
 transforms within 1e-4 rad / 1e-5 m of the FP64 solver on identical inputs.
