@@ -117,8 +117,11 @@ typedef struct nrf_solver_params {
                                   with r.r and r.z: one all-reduce per iteration; 3:
                                   pipelined (Ghysels-Vanroose): A M^-1 A p and the like
                                   as recurrences, so that the one all-reduce overlaps
-                                  the halo exchange and the product.  The same iterates
-                                  in exact arithmetic */
+                                  the halo exchange and the product (FP64 recurrences).
+                                  The same iterates in exact arithmetic.  Form 3 suits
+                                  fixed iteration counts: its recurrence residual drifts
+                                  from the true one near 1e-6, so a tolerance stop
+                                  should use form 2 */
   int32_t arap_consistent_sign; /* 1: ARAP rotation Jacobian rows with the sign of
                                    d(omega x q)/d(omega), as the data term and the
                                    exp map use; 0 keeps the reference's negated rows
