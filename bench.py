@@ -186,8 +186,11 @@ def main() -> int:
     nf = args.frames
     seq = [1 + (i % nf) for i in range(args.warmup + args.steps)]
     lib = h.lib
+    # live timing of the dominant kernel: two events around one PCG launch per
+    # step (GN iteration step mod G), which cost the step far less than
+    # events around every kernel group (about 3 us of device time each)
     if not args.graph:
-        L.check(lib.nrf_pipeline_profile(h.pipe, max(64, args.steps)), "profile")
+        L.check(lib.nrf_pipeline_profile_pcg(h.pipe, max(64, args.steps)), "profile")
     for f in seq[: args.warmup]:
         h.step(f)
     torch.cuda.synchronize()
@@ -220,14 +223,15 @@ def main() -> int:
         f"frames ending on a rejected step {rejected}")
     kern = None
     if not args.graph:
-        rows = []
+        pcg_launch = []
         for k in range(args.steps):
             out = (C.c_float * 6)()
             L.check(lib.nrf_pipeline_profile_read(h.pipe, k, out), "profile_read")
-            rows.append(list(out))
-        kern = np.mean(np.array(rows), axis=0)
-        log("[bench] per-step kernel ms: membership %.4f assembly %.4f pcg %.4f update %.4f integrate %.4f "
-            "mc %.4f" % tuple(kern))
+            pcg_launch.append(out[2])
+        # [pcg per launch (mean over the timed steps), integrate, mc (last step's stages)]
+        kern = [float(np.mean(pcg_launch)), ms[1], ms[2]]
+        log("[bench] PCG launch %.4f ms (mean of %d, GN iterations sampled in turn); last step integrate %.4f, "
+            "mc %.4f ms" % (kern[0], len(pcg_launch), kern[1], kern[2]))
     w = h.w
     V = int(np.prod(w.dims))
     G = w.gn_iterations
@@ -250,8 +254,8 @@ def main() -> int:
     pcg_launch_iters = it_per_step / G
     algo["pcg"] = algo["pcg"] / w.pcg_iterations * pcg_launch_iters
     if kern is not None:
-        per_launch = {"integrate": kern[4], "pcg": kern[2] / G, "mc": kern[5]}
-        per_step = {"integrate": kern[4], "pcg": kern[2], "mc": kern[5]}
+        per_launch = {"integrate": kern[1], "pcg": kern[0], "mc": kern[2]}
+        per_step = {"integrate": kern[1], "pcg": kern[0] * G, "mc": kern[2]}
     else:
         per_launch = {"integrate": ms[1], "pcg": float("nan"), "mc": ms[2]}
         per_step = dict(per_launch)
@@ -291,8 +295,8 @@ def main() -> int:
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None or dom not in traffic else traffic[dom], "kernel": dom,
                      "launch_ms": round(per_launch[dom], 5), "algorithmic_bytes": int(algo[dom])},
-        "kernel_ms_per_step": None if kern is None else dict(zip(
-            ["membership", "assembly", "pcg", "update", "integrate", "mc"], [round(float(v), 5) for v in kern])),
+        "kernel_ms_per_step": None if kern is None else {"pcg": round(kern[0] * G, 5), "integrate": round(kern[1], 5),
+                                                           "mc": round(kern[2], 5)},
         "roofline_other": {k: round(algo[k] / (per_launch[k] * 1e-3) / 1e9, 1) for k in algo
                            if per_launch[k] == per_launch[k]},
         "work": {"gn_applied_per_step": round(gn_per_step, 3), "pcg_iterations_per_step": round(it_per_step, 2),

@@ -134,6 +134,7 @@ _SIGS = {
     "nrf_pipeline_step": (I, [P, P, P, P, P, I, P]),
     "nrf_pipeline_stage_ms": (I, [P, C.POINTER(F)]),
     "nrf_pipeline_profile": (I, [P, I]),
+    "nrf_pipeline_profile_pcg": (I, [P, I]),
     "nrf_pipeline_profile_read": (I, [P, I, C.POINTER(F)]),
     "nrf_pipeline_updated_voxels": (C.c_longlong, [P]),
     "nrf_batch_create": (P, [C.POINTER(P), I, C.POINTER(BatchTuning), I]),

@@ -43,7 +43,7 @@ int device_cu_count();
 // the membership build, then per GN iteration after the assembly, just
 // before the PCG kernel, after the PCG kernel and after the update, i.e.
 // 1 + 4 * gn_iterations events.
-void solver_set_event_sink(nrf_solver *s, hipEvent_t *events, int count);
+void solver_set_event_sink(nrf_solver *s, hipEvent_t *events, int count, int skip = 0);
 int device_max_lds_per_block();
 bool device_cooperative();
 

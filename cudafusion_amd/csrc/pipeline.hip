@@ -44,7 +44,7 @@ struct nrf_pipeline {
   hipGraph_t graph = nullptr;
   // per-kernel profiling ring (eager mode)
   std::vector<hipEvent_t> prof;
-  int prof_ring = 0, prof_per = 0, prof_G = 0;
+  int prof_ring = 0, prof_per = 0, prof_G = 0, prof_pcg = 0;
   long prof_steps = 0;
   const void *g_key[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int g_n = -1;
@@ -199,8 +199,13 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
   hipEvent_t *pe = nullptr;
   if (profile) {
     pe = &p->prof[(size_t)(p->prof_steps % p->prof_ring) * p->prof_per];
-    NRF_HIP_CHECK(hipEventRecord(pe[0], st));
-    solver_set_event_sink(p->solver, pe + 1, p->prof_per - 3);
+    if (p->prof_pcg) {
+      // the PCG launch of GN iteration (step mod G): sink points 2 + 4 g, 3 + 4 g
+      solver_set_event_sink(p->solver, pe, 2, 2 + 4 * (int)(p->prof_steps % p->prof_G));
+    } else {
+      NRF_HIP_CHECK(hipEventRecord(pe[0], st));
+      solver_set_event_sink(p->solver, pe + 1, p->prof_per - 3);
+    }
   }
   if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[0], st));
   rc = nrf_solver_solve_gn(p->solver, corrs, canonical, num_corrs, p->nodes, p->T, p->centered, p->N, &p->cfg.gn,
@@ -216,7 +221,7 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
                         p->centered, p->knn, p->knn_w, p->knn_rows, nullptr, p->cfg.min_opt_count, p->updated, st);
   if (rc) return rc;
   if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[2], st));
-  if (profile) NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 2], st));
+  if (profile && !p->prof_pcg) NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 2], st));
   if (p->cfg.storage16)
     rc = nrf_mc16_extract(&p->cfg.volume, p->t16, p->w8, p->mc_ws, p->mc_bytes, p->verts, p->tris,
                           p->cfg.max_triangles, p->counts, stream);
@@ -226,7 +231,7 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
   if (rc) return rc;
   if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[3], st));
   if (profile) {
-    NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 1], st));
+    if (!p->prof_pcg) NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 1], st));
     ++p->prof_steps;
   }
   return NRF_OK;
@@ -293,21 +298,29 @@ extern "C" int nrf_pipeline_stage_ms(nrf_pipeline *p, float out_ms[3]) {
   return NRF_OK;
 }
 
-extern "C" int nrf_pipeline_profile(nrf_pipeline *p, int ring_steps) {
+static int pipeline_profile_setup(nrf_pipeline *p, int ring_steps, int pcg_only) {
   if (!p || ring_steps < 0) return NRF_ERR_INVALID_ARGUMENT;
   for (auto &e : p->prof)
     if (e) (void)hipEventDestroy(e);
   p->prof.clear();
   p->prof_ring = 0;
   p->prof_steps = 0;
+  p->prof_pcg = 0;
   if (ring_steps == 0) return NRF_OK;
   const int G = std::max(1, p->cfg.gn.gn_iterations);
   p->prof_G = G;
-  p->prof_per = 1 + (1 + 4 * G) + 2;
+  p->prof_pcg = pcg_only;
+  p->prof_per = pcg_only ? 2 : 1 + (1 + 4 * G) + 2;
   p->prof.assign((size_t)ring_steps * p->prof_per, nullptr);
   for (auto &e : p->prof) NRF_HIP_CHECK(hipEventCreate(&e));
   p->prof_ring = ring_steps;
   return NRF_OK;
+}
+extern "C" int nrf_pipeline_profile(nrf_pipeline *p, int ring_steps) {
+  return pipeline_profile_setup(p, ring_steps, 0);
+}
+extern "C" int nrf_pipeline_profile_pcg(nrf_pipeline *p, int ring_steps) {
+  return pipeline_profile_setup(p, ring_steps, 1);
 }
 
 extern "C" int nrf_pipeline_profile_read(nrf_pipeline *p, int step_back, float out_ms[6]) {
@@ -324,6 +337,7 @@ extern "C" int nrf_pipeline_profile_read(nrf_pipeline *p, int step_back, float o
   };
   for (int k = 0; k < 6; ++k) out_ms[k] = 0;
   int rc;
+  if (p->prof_pcg) return ms(0, 1, out_ms[2]);  // one PCG launch
   if ((rc = ms(0, 1, out_ms[0]))) return rc;  // membership build
   int prev = 1;
   for (int g = 0; g < p->prof_G; ++g) {

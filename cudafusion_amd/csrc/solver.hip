@@ -2339,19 +2339,25 @@ struct nrf_solver {
   unsigned gen = 0;  // advanced by every partition rebuild
   // optional profiling events (see solver_set_event_sink)
   hipEvent_t *sink = nullptr;
-  int sink_n = 0, sink_i = 0;
+  int sink_n = 0, sink_i = 0, sink_skip = 0;
 };
 
 namespace nrf {
 
-void solver_set_event_sink(nrf_solver *s, hipEvent_t *events, int count) {
+void solver_set_event_sink(nrf_solver *s, hipEvent_t *events, int count, int skip) {
   s->sink = events;
   s->sink_n = events ? count : 0;
   s->sink_i = 0;
+  s->sink_skip = skip;
 }
 
+// Sink points of one solve: after the memberships, then per GN iteration
+// after the assembly, before the PCG, after the PCG and after the update.
+// The first `sink_skip` points record nothing.
 static void sink_record(nrf_solver *s, hipStream_t st) {
-  if (s->sink && s->sink_i < s->sink_n) (void)hipEventRecord(s->sink[s->sink_i++], st);
+  if (!s->sink) return;
+  const int j = s->sink_i++ - s->sink_skip;
+  if (j >= 0 && j < s->sink_n) (void)hipEventRecord(s->sink[j], st);
 }
 
 static int solver_build_partition(nrf_solver *s);

@@ -410,6 +410,12 @@ int nrf_pipeline_stage_ms(nrf_pipeline *p, float out_ms[3]);
    time in ms of [membership build, assembly, PCG, twist update, integration,
    marching cubes], assembly / PCG / update summed over the GN iterations. */
 int nrf_pipeline_profile(nrf_pipeline *p, int ring_steps);
+/* No counterpart in the reference; for timing runs that must not be slowed by
+   the events (each costs about 3 us of device time here): with ring_steps > 0
+   every step records only two hipEvents, around the PCG launch of GN
+   iteration (step mod gn_iterations), and nrf_pipeline_profile_read returns
+   that one launch's ms in out_ms[2] (the other entries 0). */
+int nrf_pipeline_profile_pcg(nrf_pipeline *p, int ring_steps);
 int nrf_pipeline_profile_read(nrf_pipeline *p, int step_back, float out_ms[6]);
 /* Voxels written by the warped integrations of all steps so far (syncs). */
 long long nrf_pipeline_updated_voxels(nrf_pipeline *p);

@@ -164,3 +164,45 @@ def test_pipeline_graph_replay_matches_eager(gpu):
         assert ms[0] > 0 and ms[1] == -1
     a.close()
     b.close()
+
+
+def test_pipeline_pcg_profile_sampling(gpu):
+    """nrf_pipeline_profile_pcg: two events per step around the PCG launch of
+    GN iteration (step mod G). Each sampled launch takes positive time, less
+    than the step's solve stage; the average over the GN iterations agrees
+    with the per-kernel ring's PCG time per launch (same steps again), and
+    the step's results do not depend on the profiling mode."""
+    import ctypes as C
+
+    import torch
+
+    from cudafusion_amd import _lib as L
+    from cudafusion_amd import scene as S
+    from cudafusion_amd.harness import Harness
+    w = S.sphere_workload(frames=3)
+    a = Harness(workload=w, frames=3, pcg_pipelined=3)
+    a.setup()
+    b = Harness(workload=w, frames=3, pcg_pipelined=3)
+    b.setup(host=a)
+    G = w.gn_iterations
+    L.check(a.lib.nrf_pipeline_profile_pcg(a.pipe, 16), "profile_pcg")
+    L.check(b.lib.nrf_pipeline_profile(b.pipe, 16), "profile")
+    frames = [1, 2, 3] * G
+    sampled, full = [], []
+    for f in frames:
+        a.step(f)
+        b.step(f)
+        torch.cuda.synchronize()
+        _compare([a], [b], f"profiled step f{f}")
+        out = (C.c_float * 6)()
+        L.check(a.lib.nrf_pipeline_profile_read(a.pipe, 0, out), "profile_read")
+        solve_ms = a.stage_ms()[0]
+        assert 0 < out[2] < solve_ms and out[0] == 0 and out[4] == 0
+        sampled.append(out[2])
+        L.check(b.lib.nrf_pipeline_profile_read(b.pipe, 0, out), "profile_read")
+        full.append(out[2] / G)
+    ms_s, ms_f = float(np.mean(sampled)), float(np.mean(full))
+    print(f"[profile] PCG launch {ms_s:.4f} ms sampled, {ms_f:.4f} ms from the full ring", flush=True)
+    assert abs(ms_s - ms_f) <= 0.25 * ms_f
+    a.close()
+    b.close()
