@@ -138,7 +138,8 @@ def main() -> int:
                          "(Ghysels-Vanroose), the all-reduce overlapping the halo exchange and the product.  "
                          "Default 3 for the fixed-iteration step, 2 for the batched form (PCG to a tolerance)")
     ap.add_argument("--pcg-layout", default="", metavar="LANES,NODES",
-                    help="PCG lanes per scalar row (1, 2, 4) and target nodes per workgroup (default 4,24)")
+                    help="PCG lanes per scalar row (1, 2, 4) and target nodes per workgroup (default 4,16 for the "
+                         "pipelined form, the solver's 4,24 otherwise)")
     ap.add_argument("--precision", choices=["bf16", "fp16"], default=None,
                     help="precision form: FP16 TSDF + u8 weight, compact kNN, JtJ blocks in BF16 / FP16 on the "
                          "matrix cores, FP64 Cholesky inverses, one FP64-residual refinement step per solve; "
@@ -155,6 +156,11 @@ def main() -> int:
     args = ap.parse_args()
     if args.pcg_form is None:
         args.pcg_form = 2 if args.batched else 3
+    # the pipelined form overlaps its waits best with more, smaller workgroups
+    # (sweep on the capsule body: 16 nodes 0.62 ms of PCG per frame, 20: 0.65,
+    # 24: 0.67, 14: 0.67, 32: 0.72); the other forms keep the solver's 24
+    if not args.pcg_layout and args.pcg_form == 3 and not args.textbook_pcg and not args.batched:
+        args.pcg_layout = "4,16"
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs the step on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
