@@ -27,7 +27,7 @@ CamXf invert_pose(const float pose[16]) {
 
 namespace {
 struct DevInfo {
-  int cus = 0, lds = 65536, coop = 0;
+  int cus = 0, lds = 65536;
 };
 DevInfo query() {
   DevInfo d;
@@ -36,7 +36,6 @@ DevInfo query() {
   int v = 0;
   if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess) d.cus = v;
   if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess) d.lds = v;
-  if (hipDeviceGetAttribute(&v, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess) d.coop = v;
   return d;
 }
 const DevInfo &info() {
@@ -51,7 +50,6 @@ int device_max_lds_per_block() {
   // per-kernel opt-in attribute (set by the callers that use it).
   return info().lds > 0 ? info().lds : 65536;
 }
-bool device_cooperative() { return info().coop != 0; }
 
 }  // namespace nrf
 

@@ -20,7 +20,7 @@ namespace {
 // One workgroup per brick of 64 x 4 x 4 cubes: the brick's 65 x 5 x 5 voxels
 // are read once, coalesced along x, into LDS as (tsdf, weight) pairs; wave w
 // then classifies cube row y0 + w for the four z-slices.  Reader maps a voxel
-// index to its (tsdf, weight).
+// index to its (tsdf, weight).  cases may be null (only the row counts).
 constexpr int kMcBx = 64, kMcBy = 4, kMcBz = 4;
 constexpr int kMcVx = kMcBx + 1, kMcVy = kMcBy + 1, kMcVz = kMcBz + 1;
 constexpr int kMcBrickThreads = 64 * kMcBy;
@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kMcBrickThreads) void classify_brick_kernel(Reader 
       }
       if (cube <= 0 || cube >= 255) cube = 0;
     }
-    if (x < X && y < Y && z < Z) cases[((size_t)z * Y + y) * X + x] = (uint8_t)cube;
+    if (cases && x < X && y < Y && z < Z) cases[((size_t)z * Y + y) * X + x] = (uint8_t)cube;
     int nt = kMcNTri[cube];
     for (int off = 32; off > 0; off >>= 1) nt += __shfl_xor(nt, off, 64);
     if (lx == 0 && nt && y < Y && z < Z) atomicAdd(&rowcnt[z * Y + y], nt);
@@ -133,9 +133,13 @@ __global__ __launch_bounds__(1024) void scan_rows_kernel(const int32_t *__restri
 // in the high word; the status words start zeroed), and adds the totals of
 // all earlier blocks, which one wave reads.  Workgroups are dispatched in
 // index order and publish before they wait, so every wait ends; the wait is
-// bounded all the same (then counts[1] = -1).  Integer sums: the result does
-// not depend on the order of the additions.  cnt and off 16-byte aligned;
-// counts (optional) receives {3 T, T}.
+// bounded all the same: a block that gives up raises the flag word
+// status[gridDim.x] and writes counts[1] = -1, and the last block, after
+// writing T, reads the flag and writes -1 if it is raised (sequentially
+// consistent atomics, so whichever store comes last, -1 stays).  Integer
+// sums: the result does not depend on the order of the additions.  cnt and
+// off 16-byte aligned; status holds gridDim.x + 1 zeroed words; counts
+// (optional) receives {3 T, T}.
 constexpr int kScanBlock = 1024, kScanThreads = 256, kScanMaxBlocks = 4096;
 constexpr unsigned kScanSpinLimit = 1u << 24;
 __global__ __launch_bounds__(kScanThreads) void scan_rows_lb_kernel(const int32_t *__restrict__ cnt,
@@ -210,34 +214,44 @@ __global__ __launch_bounds__(kScanThreads) void scan_rows_lb_kernel(const int32_
     if (r0 + 1 < n) off[r0 + 1] = o4.y;
     if (r0 + 2 < n) off[r0 + 2] = o4.z;
   }
+  if (t == 0 && s_bad) {
+    __hip_atomic_store(&status[gridDim.x], 1ull, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+    if (counts) __hip_atomic_store(&counts[1], -1, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+  }
   if (b == (int)gridDim.x - 1 && t == 0) {
     const int T = s_prefix + total;
     off[n] = T;
     if (counts) {
       counts[0] = 3 * T;
-      counts[1] = s_bad ? -1 : T;
+      __hip_atomic_store(&counts[1], s_bad ? -1 : T, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+      if (__hip_atomic_load(&status[gridDim.x], __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT) != 0)
+        __hip_atomic_store(&counts[1], -1, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
 
 // Workspace of marching cubes: the case byte of every cube, the row counts
-// followed by the scan's block status words (zeroed together before the
-// classify pass: zero_ints words from rowcnt_off), the row offsets.
+// followed by the scan's block status words and its timeout flag word
+// (zeroed together before the classify pass: zero_ints words from
+// rowcnt_off), the row offsets.
 struct McLayout {
   size_t cases_off, rowcnt_off, status_off, rowoff_off, total, zero_ints;
   int nb;
 };
-inline McLayout mc_layout(const nrf_volume_params *vp) {
+// case_bytes = 0 leaves out the case array (the workspace-free form).
+inline McLayout mc_layout_for(size_t case_bytes, size_t R) {
   McLayout L;
-  const size_t V = (size_t)volume_count(vp), R = (size_t)vp->dims[1] * vp->dims[2];
   L.nb = (int)((R + kScanBlock - 1) / kScanBlock);
   L.cases_off = 0;
-  L.rowcnt_off = align_up(V, 256);
+  L.rowcnt_off = align_up(case_bytes, 256);
   L.status_off = L.rowcnt_off + align_up(R * 4, 8);
-  L.rowoff_off = align_up(L.status_off + (size_t)L.nb * 8, 256);
+  L.rowoff_off = align_up(L.status_off + (size_t)(L.nb + 1) * 8, 256);
   L.total = L.rowoff_off + align_up((R + 1) * 4, 256);
-  L.zero_ints = (L.status_off - L.rowcnt_off) / 4 + (size_t)L.nb * 2;
+  L.zero_ints = (L.status_off - L.rowcnt_off) / 4 + (size_t)(L.nb + 1) * 2;
   return L;
+}
+inline McLayout mc_layout(const nrf_volume_params *vp) {
+  return mc_layout_for((size_t)volume_count(vp), (size_t)vp->dims[1] * vp->dims[2]);
 }
 
 // The scan of the row counts: the multi-workgroup form, or one workgroup

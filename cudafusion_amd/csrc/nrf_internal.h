@@ -45,7 +45,6 @@ int device_cu_count();
 // 1 + 4 * gn_iterations events.
 void solver_set_event_sink(nrf_solver *s, hipEvent_t *events, int count, int skip = 0);
 int device_max_lds_per_block();
-bool device_cooperative();
 
 // ---------------------------------------------------------------- batches
 // Independent instances (one pipeline each) stepped by common launches.

@@ -2331,7 +2331,7 @@ struct nrf_solver {
   size_t c_res = 0;
   nrf_solver_totals *d_tot = nullptr;
   size_t c_tot = 0;
-  int cus = 0, max_lds = 0, coop = 0;
+  int cus = 0, max_lds = 0;
   int forced_G = 0;
   int pcg_sub = 4;        // lanes per scalar row of the PCG (1, 2, 4)
   int pcg_wg_nodes = 24;  // target nodes per PCG workgroup (measured best on the full-scale body: 20-24)
@@ -2637,18 +2637,9 @@ static PcgArgs solver_pcg_args(const nrf_solver *s, const float *rhs, float *x, 
   return a;
 }
 
-// A persistent kernel captured into a graph takes an ordinary launch: its
-// grid is sized to be co-resident on the device (at most the CU count, or
-// the occupancy bound for batches), and every wait in it is bounded.
-static bool stream_capturing(hipStream_t st) {
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
-    (void)hipGetLastError();
-    return false;
-  }
-  return cs != hipStreamCaptureStatusNone;
-}
-
+// The persistent PCG takes an ordinary launch, eager or captured into a
+// graph: its grid is sized to be co-resident on the device (at most the CU
+// count, or the occupancy bound for batches), and every wait in it is bounded.
 static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxit, bool record, hipStream_t st,
                       bool opened) {
   if (!opened) {
@@ -2662,11 +2653,7 @@ static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxi
   if (!pipe && s->pcg_sub != kPcgSub) return NRF_ERR_INVALID_ARGUMENT;  // textbook form: 4 lanes per row only
   const void *fn = pipe ? pcg_pipe_fn(s->pcg_sub, s->params.pcg_pipelined - 1)
                         : reinterpret_cast<const void *>(pcg_kernel);
-  if (s->coop && !stream_capturing(st)) {
-    NRF_HIP_CHECK(hipLaunchCooperativeKernel(fn, dim3(s->G), dim3(kPcgThreads), args, (unsigned)s->pcg_lds, st));
-  } else {
-    NRF_HIP_CHECK(hipLaunchKernel(fn, dim3(s->G), dim3(kPcgThreads), args, (unsigned)s->pcg_lds, st));
-  }
+  NRF_HIP_CHECK(hipLaunchKernel(fn, dim3(s->G), dim3(kPcgThreads), args, (unsigned)s->pcg_lds, st));
   return NRF_OK;
 }
 
@@ -2693,7 +2680,10 @@ static int solver_ensure_pattern(nrf_solver *s, const nrf_node *d_nodes, int N, 
 // A PCG launch has 1 + 2 * pcg_iterations synchronisation points, which must
 // fit the 16 point bits of a published word's tag.
 static bool params_ok(const nrf_solver_params *p) {
-  return p->pcg_iterations >= 0 && 1 + 2 * (long)p->pcg_iterations <= kPcgMaxPoints && p->jtj_precision >= 0 &&
+  // Both PCG runs of a solve (the main one and the refinement) tag their
+  // synchronisation points in 16 bits: 1 + 2 * iterations must fit.
+  return p->pcg_iterations >= 0 && 1 + 2 * (long)p->pcg_iterations <= kPcgMaxPoints &&
+         p->refine_iterations >= 0 && 1 + 2 * (long)p->refine_iterations <= kPcgMaxPoints && p->jtj_precision >= 0 &&
          p->jtj_precision <= 2 && p->pcg_pipelined >= 0 && p->pcg_pipelined <= 3;
 }
 
@@ -2708,12 +2698,7 @@ extern "C" nrf_solver *nrf_solver_create(const nrf_solver_params *params, int ma
   // sized to be co-resident (at most the CU count at one 1024-lane workgroup
   // per CU, or the occupancy bound for a batch group) and every wait in them
   // is bounded, so a grid that could not become resident ends with the
-  // barrier-timeout flag instead of hanging.  NRF_PCG_COOP=1 uses the
-  // cooperative launch instead, which guarantees co-residency but takes the
-  // device's global wave-sync resource: another process holding it (one that
-  // made a cooperative launch before) makes the launch fail.
-  const char *fc = getenv("NRF_PCG_COOP");
-  s->coop = (fc && fc[0] == '1' && device_cooperative()) ? 1 : 0;
+  // barrier-timeout flag instead of hanging.
   // NRF_ASM_MFMA=0 assembles the data term with the scalar LDS form
   const char *fm = getenv("NRF_ASM_MFMA");
   if (fm && fm[0] == '0') s->asm_mfma = 0;
@@ -2905,7 +2890,7 @@ extern "C" int nrf_solver_get_totals(nrf_solver *s, nrf_solver_totals *out, void
 
 struct nrf::SolverBatch {
   std::vector<SolverBatchInstance> in;
-  int count = 0, sub = 4, prec = 0, ns = 1, gn_iters = 1, any_validate = 0, coop = 0;
+  int count = 0, sub = 4, prec = 0, ns = 1, gn_iters = 1, any_validate = 0;
   int form = 0;  // the PCG form of every instance (pcg_pipelined - 1)
   int max_N = 0, rows = 0, wgs = 0;
   size_t asm_lds = 0, pcg_lds = 0;
@@ -2993,7 +2978,6 @@ int solver_batch_create(const SolverBatchInstance *inst, int count, int lanes_pe
       b->form = sp.pcg_pipelined - 1;
       b->prec = sp.jtj_precision;
       b->gn_iters = std::max(1, q.gn.gn_iterations);
-      b->coop = s->coop;
     }
     b->ns = std::max(b->ns, ns);
     b->asm_lds = std::max(b->asm_lds, asm_lds_bytes(s->P.Rmax));
@@ -3155,17 +3139,12 @@ int solver_batch_pcg(SolverBatch *b, hipStream_t st) {
   pcg_prep_batch_kernel<<<1, 64, 0, st>>>(b->d_inst, b->count);
   NRF_LAUNCH_CHECK();
   const void *fn = pcg_batch_fn(b->sub, b->form);
-  const bool coop = b->coop && !stream_capturing(st);
   for (const auto &g : b->groups) {
     const InstDev *inst = b->d_inst;
     const int32_t *wi = b->wg_inst + g.first, *wl = b->wg_local + g.first;
     void *args[] = {&inst, &wi, &wl};
     const dim3 grid(g.second - g.first);
-    if (coop) {
-      NRF_HIP_CHECK(hipLaunchCooperativeKernel(fn, grid, dim3(kPcgThreads), args, (unsigned)b->pcg_lds, st));
-    } else {
-      NRF_HIP_CHECK(hipLaunchKernel(fn, grid, dim3(kPcgThreads), args, (unsigned)b->pcg_lds, st));
-    }
+    NRF_HIP_CHECK(hipLaunchKernel(fn, grid, dim3(kPcgThreads), args, (unsigned)b->pcg_lds, st));
   }
   return NRF_OK;
 }

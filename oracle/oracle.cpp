@@ -1035,6 +1035,9 @@ double orc_pcg_f64(const int32_t *rp, const int32_t *ci, const double *A, const 
 void orc_spmv_f64(const int32_t *rp, const int32_t *ci, const double *A, const double *x, double *y, int N) {
   spmv(rp, ci, A, x, y, N);
 }
+// Step validation of the caller (include/fusion_app.h:409-491): 1 = accept.
+int orc_validate_f32(const float *dx, int N, const nrf_gn_params *g) { return validate_step(dx, N, *g) ? 1 : 0; }
+int orc_validate_f64(const double *dx, int N, const nrf_gn_params *g) { return validate_step(dx, N, *g) ? 1 : 0; }
 
 int orc_solve_gn_f32(const nrf_corr *corrs, const float *canonical, int n, const nrf_node *nodes, int N,
                      nrf_dualquat *Tq, const nrf_gn_params *g, const nrf_solver_params *sp, float *dx,

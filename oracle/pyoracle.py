@@ -50,6 +50,8 @@ _SIGS = {
     "orc_pcg_f32": (D, [P, P, P, P, P, I, I, F, I, P, P, P, P]),
     "orc_pcg_f64": (D, [P, P, P, P, P, I, I, D, I, P, P, P, P]),
     "orc_spmv_f64": (None, [P, P, P, P, P, I]),
+    "orc_validate_f32": (I, [P, I, P]),
+    "orc_validate_f64": (I, [P, I, P]),
     "orc_solve_gn_f32": (I, [P, P, I, P, I, P, P, P, P, P, P, P]),
     "orc_solve_gn_f64": (I, [P, P, I, P, I, P, P, P, P, P, P, P]),
     "orc_lp_param_count": (I, []),
@@ -189,6 +191,14 @@ def knn_f32(vp, vox, nodes):
     w = np.zeros(V * 4, np.float32)
     load().orc_knn_f32(C.byref(vp), ptr(vox), ptr(nodes), len(nodes), ptr(knn), ptr(w))
     return knn, w
+
+
+def validate(dx, gn, fp64=False) -> bool:
+    """The caller's step validation (fusion_app.h:409-491) on dx [6N]."""
+    lib = load()
+    d = np.ascontiguousarray(dx, dtype=np.float64 if fp64 else np.float32)
+    fn = lib.orc_validate_f64 if fp64 else lib.orc_validate_f32
+    return bool(fn(ptr(d), d.size // 6, C.byref(gn)))
 
 
 def pattern(nodes):
