@@ -95,6 +95,7 @@ _SIGS = {
     "nrf_mc_workspace_size": (C.c_size_t, [C.POINTER(VolumeParams)]),
     "nrf_mc_extract": (I, [C.POINTER(VolumeParams), P, P, C.c_size_t, P, P, P, I, P, P]),
     "nrf_mc_cases": (P, [P]),
+    "nrf_mc_extract_noalloc": (I, [C.POINTER(VolumeParams), P, P, P, P, I, P, P]),
     "nrf_tsdf16_reset": (I, [C.POINTER(VolumeParams), P, P, P]),
     "nrf_tsdf16_from_voxels": (I, [C.POINTER(VolumeParams), P, P, P, P]),
     "nrf_knn_compact": (I, [C.POINTER(VolumeParams), P, P, P, P, P]),

@@ -21,10 +21,23 @@
 #include "nrf_internal.h"
 #include "nrf_math.h"
 
+#include <mutex>
+
 namespace nrf {
 namespace {
 
 constexpr int kMcBlock = 256;
+
+// Scratch of the workspace-free extraction (nrf_mc_extract_noalloc): row
+// counts, the scan's status and flag words and the row offsets of up to
+// kMcNoAllocMaxRows rows (dims[1] * dims[2]), laid out as mc_layout_for(0, R)
+// (row counts at 0).  One copy per device, held until the process ends.
+constexpr size_t kMcNoAllocMaxRows = 512 * 512;
+constexpr size_t cx_align(size_t v, size_t a) { return (v + a - 1) / a * a; }
+constexpr size_t kMcNoAllocBytes =
+    cx_align(cx_align(kMcNoAllocMaxRows * 4, 8) + ((kMcNoAllocMaxRows + kScanBlock - 1) / kScanBlock + 1) * 8, 256) +
+    cx_align((kMcNoAllocMaxRows + 1) * 4, 256);
+__device__ __align__(256) unsigned char scratch[kMcNoAllocBytes];
 
 
 struct VoxelReaderF32 {
@@ -52,6 +65,24 @@ __device__ inline float3 edge_point(int e, const float *val, int x, int y, int z
   return f3(org.x + p.x * s, org.y + p.y * s, org.z + p.z * s);
 }
 
+// Case index of the cube at voxel index idx from its eight corners, by the
+// rule of classify_brick_kernel (a corner with weight <= 0 voids the cube;
+// bit c set when tsdf_c > 0; 0 and 255 emit nothing).
+__device__ inline int cube_case(const nrf_voxel *__restrict__ vox, int idx, int X, int Y) {
+  int cube = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const nrf_voxel &v = vox[idx + kMcCorner[c][0] + kMcCorner[c][1] * X + kMcCorner[c][2] * X * Y];
+    if (v.weight <= 0.f) cube = -1;
+    if (cube >= 0 && v.tsdf > 0.f) cube |= (1 << c);
+  }
+  return (cube <= 0 || cube >= 255) ? 0 : cube;
+}
+
+// kCases: the case bytes come from the classify pass; otherwise (the
+// workspace-free form, which keeps no case array) each cube's case is
+// recomputed from its corners.
+template <bool kCases>
 __global__ __launch_bounds__(kMcBlock) void emit_kernel(const nrf_voxel *__restrict__ vox, int X, int Y, int Z,
                                                         float3 org, float s, const uint8_t *__restrict__ cases,
                                                         const int32_t *__restrict__ rowcnt,
@@ -67,7 +98,8 @@ __global__ __launch_bounds__(kMcBlock) void emit_kernel(const nrf_voxel *__restr
   for (int x0 = 0; x0 < X - 1; x0 += kMcBlock) {
     const int x = x0 + threadIdx.x;
     const int idx = z * X * Y + y * X + x;
-    const int cube = (x < X - 1) ? cases[idx] : 0;
+    int cube = 0;
+    if (x < X - 1) cube = kCases ? (int)cases[idx] : cube_case(vox, idx, X, Y);
     const int nt = kMcNTri[cube];
     int inc = nt;  // inclusive wave scan
     for (int d = 1; d < 64; d <<= 1) {
@@ -156,10 +188,67 @@ extern "C" int nrf_mc_extract(const nrf_volume_params *vp, const nrf_voxel *voxe
   mc_scan_rows(L, rowcnt, rowoff, R, counts, workspace, st);
   NRF_LAUNCH_CHECK();
   if (X >= 2 && Y >= 2 && Z >= 2 && max_triangles > 0) {
-    emit_kernel<<<dim3(Y, Z), kMcBlock, 0, st>>>(voxels, X, Y, Z,
+    emit_kernel<true><<<dim3(Y, Z), kMcBlock, 0, st>>>(voxels, X, Y, Z,
                                                  make_float3(vp->origin[0], vp->origin[1], vp->origin[2]),
                                                  vp->voxel_size, cases, rowcnt, rowoff, max_triangles, vertices,
                                                  normals, triangles);
+    NRF_LAUNCH_CHECK();
+  }
+  return NRF_OK;
+}
+
+// Device address of this device's copy of `scratch`, looked up once per
+// device (the host-side name of the array is not a device address).
+static int mc_scratch_ptr(unsigned char **out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static unsigned char *ptrs[kMaxDevices] = {};
+  int dev = 0;
+  NRF_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return NRF_ERR_TOO_LARGE;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptrs[dev]) {
+    void *p = nullptr;
+    NRF_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ptrs[dev] = reinterpret_cast<unsigned char *>(p);
+  }
+  *out = ptrs[dev];
+  return NRF_OK;
+}
+
+extern "C" int nrf_mc_extract_noalloc(const nrf_volume_params *vp, const nrf_voxel *voxels, float *vertices,
+                                      float *normals, int32_t *triangles, int max_triangles, int32_t *counts,
+                                      void *stream) {
+  if (!volume_ok(vp) || !voxels || !counts) return NRF_ERR_INVALID_ARGUMENT;
+  if (max_triangles < 0 || (max_triangles > 0 && (!vertices || !triangles))) return NRF_ERR_INVALID_ARGUMENT;
+  const int X = vp->dims[0], Y = vp->dims[1], Z = vp->dims[2];
+  const size_t R = (size_t)Y * Z;
+  if (R > kMcNoAllocMaxRows) {
+    set_last_error_msg("nrf_mc_extract_noalloc: dims[1] * dims[2] = " + std::to_string(R) +
+                       " rows exceeds the scratch capacity of " + std::to_string(kMcNoAllocMaxRows) + " rows");
+    return NRF_ERR_TOO_LARGE;
+  }
+  unsigned char *ws = nullptr;
+  int rc = mc_scratch_ptr(&ws);
+  if (rc) return rc;
+  const McLayout L = mc_layout_for(0, R);
+  if (L.total > kMcNoAllocBytes) return NRF_ERR_TOO_LARGE;  // cannot happen for R <= kMcNoAllocMaxRows
+  int32_t *rowcnt = reinterpret_cast<int32_t *>(ws + L.rowcnt_off);
+  int32_t *rowoff = reinterpret_cast<int32_t *>(ws + L.rowoff_off);
+  hipStream_t st = as_stream(stream);
+  const int nz = (int)L.zero_ints;  // the counts and the scan's status words (a kernel, no memset node)
+  fill_zero_kernel<<<(nz + 255) / 256, 256, 0, st>>>(rowcnt, nz);
+  NRF_LAUNCH_CHECK();
+  classify_brick_kernel<<<classify_brick_grid(X, Y, Z), kMcBrickThreads, 0, st>>>(VoxelReaderF32{voxels}, X, Y, Z,
+                                                                                   nullptr, rowcnt);
+  NRF_LAUNCH_CHECK();
+  mc_scan_rows(L, rowcnt, rowoff, (int)R, counts, ws, st);
+  NRF_LAUNCH_CHECK();
+  if (X >= 2 && Y >= 2 && Z >= 2 && max_triangles > 0) {
+    emit_kernel<false><<<dim3(Y, Z), kMcBlock, 0, st>>>(voxels, X, Y, Z,
+                                                        make_float3(vp->origin[0], vp->origin[1], vp->origin[2]),
+                                                        vp->voxel_size, nullptr, rowcnt, rowoff, max_triangles,
+                                                        vertices, normals, triangles);
     NRF_LAUNCH_CHECK();
   }
   return NRF_OK;

@@ -195,6 +195,20 @@ int nrf_mc_extract(const nrf_volume_params *vp, const nrf_voxel *voxels,
 /* Optional: case index per cube (device, dims product bytes; 0 for cubes the
    reference skips), from the workspace of the last nrf_mc_extract call. */
 const uint8_t *nrf_mc_cases(const void *workspace);
+/* No counterpart in the reference.  For callers that cannot allocate device
+   memory (code under HIP graph capture, serving loops, harnesses that forbid
+   hipMalloc): nrf_mc_extract with the same outputs, bit for bit, and no
+   workspace argument.  Its scratch (row counts, scan status words, row
+   offsets; no case array, the emission recomputes each cube's case) is a
+   library-owned device array of 2.1 MB, one per device, held until the
+   process ends: dims[1] * dims[2] must not exceed 262144 rows
+   (NRF_ERR_TOO_LARGE otherwise, with the reason in nrf_last_error).  The
+   call allocates nothing and clears the scratch with a kernel (no memset
+   node).  Every call on a device shares that device's scratch: calls that
+   overlap on one GPU (other streams, other threads) race. */
+int nrf_mc_extract_noalloc(const nrf_volume_params *vp, const nrf_voxel *voxels, float *vertices,
+                           float *normals, int32_t *triangles, int max_triangles, int32_t *counts,
+                           void *stream);
 
 /* ---------------------------------------------- compact (precision) form */
 /* The TSDF stored as FP16 value + uint8 weight (3 B/voxel) and the voxel kNN

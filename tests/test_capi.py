@@ -96,3 +96,16 @@ def test_host_checks_without_device():
     assert lib.nrf_lp_adam(None, 1e-3, 1, None) == 1
     sp = _lib.solver_params(5, 30, 0.0, 10.0, 1.0, 1e-2, 0, 0, 0, 0, 4)  # pcg_pipelined 4 is not a form
     assert not lib.nrf_solver_create(C.byref(sp), 100)
+    # the PCG synchronisation tags hold 1 + 2 * iterations in 16 bits, for the
+    # main PCG and the refinement PCG alike
+    sp = _lib.solver_params(5, 32768, pcg_pipelined=2)
+    assert not lib.nrf_solver_create(C.byref(sp), 100)
+    sp = _lib.solver_params(5, 30, pcg_pipelined=2, refine_iterations=32768)
+    assert not lib.nrf_solver_create(C.byref(sp), 100)
+    # the workspace-free marching cubes refuses a volume beyond its scratch
+    # (dims[1] * dims[2] rows) before any device call
+    vp = _lib.volume_params((8, 600, 600), 0.01, 0.03, (0, 0, 0))
+    counts = (C.c_int32 * 2)()
+    assert lib.nrf_mc_extract_noalloc(C.byref(vp), C.c_void_p(256), None, None, None, 0, C.addressof(counts),
+                                      None) == 2
+    assert b"262144" in lib.nrf_last_error()
