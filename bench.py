@@ -140,10 +140,14 @@ def main() -> int:
     ap.add_argument("--pcg-layout", default="", metavar="LANES,NODES",
                     help="PCG lanes per scalar row (1, 2, 4) and target nodes per workgroup (default 4,16 for the "
                          "pipelined form, the solver's 4,24 otherwise)")
-    ap.add_argument("--precision", choices=["bf16", "fp16"], default=None,
-                    help="precision form: FP16 TSDF + u8 weight, compact kNN, JtJ blocks in BF16 / FP16 on the "
-                         "matrix cores, FP64 Cholesky inverses, one FP64-residual refinement step per solve; "
-                         "reports errors against an FP64 run of one frame")
+    ap.add_argument("--precision", choices=["bf16", "fp16", "bf16x1", "fp16x1"], default=None,
+                    help="precision form: FP16 TSDF + u8 weight, compact kNN, JtJ blocks from split BF16 / FP16 "
+                         "operands on the matrix cores (bf16x1 / fp16x1: one unsplit product), FP64 Cholesky "
+                         "inverses, one FP64-residual refinement step per solve; reports errors against an FP64 "
+                         "run of one frame")
+    ap.add_argument("--refine", type=int, default=0,
+                    help="PCG iterations of the precision form's refinement step (0: the correction solve is the "
+                         "block-Jacobi inverse, x += M^-1 (b - A x), one kernel)")
     ap.add_argument("--reference-policy", action="store_true",
                     help="reference ARAP rotation sign and whole-step rejection (frames lose GN work)")
     ap.add_argument("--batched", type=int, default=0, metavar="COUNT",
@@ -179,9 +183,10 @@ def main() -> int:
 
     t0 = time.perf_counter()
     ref = args.reference_policy
-    prec = {"bf16": 1, "fp16": 2}.get(args.precision, 0)
+    prec = {"bf16": 3, "fp16": 4, "bf16x1": 1, "fp16x1": 2}.get(args.precision, 0)
     h = Harness("body", frames=args.frames, use_graph=args.graph, arap_consistent_sign=not ref, validate=ref,
-                pcg_pipelined=0 if args.textbook_pcg else args.pcg_form, jtj_precision=prec, refine_iterations=10 if prec else 0,
+                pcg_pipelined=0 if args.textbook_pcg else args.pcg_form, jtj_precision=prec,
+                refine_iterations=args.refine if prec else 0, refine_jacobi=bool(prec) and args.refine == 0,
                 precond_fp64=bool(prec), storage16=bool(prec))
     h.setup()
     if args.pcg_layout:
@@ -313,8 +318,12 @@ def main() -> int:
     }
     out["pcg_to_1e-6"] = pcg_to_tolerance(h, 1e-6)
     if prec:
-        out["config"]["workload"] += ", precision form (%s JtJ, FP16 TSDF + u8 weight, FP64 Cholesky and " \
-                                     "refinement)" % args.precision
+        out["config"]["workload"] += ", precision form (%s JtJ, FP16 TSDF + u8 weight, FP64 Cholesky, one " \
+                                     "FP64-residual refinement step: %s)" % (
+                                         {"bf16": "split BF16", "fp16": "split FP16"}.get(args.precision,
+                                                                                          args.precision),
+                                         "x += M^-1 (b - A x)" if args.refine == 0 else
+                                         "%d PCG iterations on b - A x" % args.refine)
         out["dtype"] = args.precision + "/f16/f32"
         out["errors_vs_fp64"] = precision_errors(h, seq[-1] % nf + 1)
     print(json.dumps(out))
@@ -339,6 +348,7 @@ def pcg_to_tolerance(h, rel_tol, reps=20):
     solver = lib.nrf_pipeline_solver(h.pipe)
     sp = L.solver_params(**{f: getattr(h.sp, f) for f, _ in L.SolverParams._fields_})
     sp.pcg_iterations, sp.pcg_tolerance, sp.relative_tolerance, sp.refine_iterations = 5000, rel_tol, 1, 0
+    sp.refine_jacobi = 0
     # a stop on the recurrence's residual needs that residual to follow the
     # true one: the pipelined form's drifts (6.8e-5 true at 1e-6 on this
     # system), so this leg runs the single-reduction form
@@ -391,12 +401,14 @@ def pcg_to_tolerance(h, rel_tol, reps=20):
 
 def precision_errors(h, f):
     """One more (untimed) frame f, and the same frame from the same state
-    run by the FP64 oracle (with the device's PCG + refinement iterations as
-    its PCG iterations per GN step): node transforms (max rotation / translation
-    difference), TSDF (max difference on voxels away from a rounding
-    boundary of the FP64 projection, in truncation units) and vertices
-    (largest distance from a device vertex to the nearest FP64 vertex, and
-    back, in voxels)."""
+    run by the FP64 oracle with the same algorithm (PCG and refinement
+    iterations): node transforms (max rotation / translation difference),
+    TSDF (max difference on voxels away from a rounding boundary of the FP64
+    projection, in truncation units; the FP64 integration from the device's
+    decoded state with the device's transforms and its compact kNN decoded,
+    so that it measures the integration alone) and vertices (largest
+    distance from a device vertex to the nearest FP64 vertex, and back, in
+    voxels)."""
     import numpy as np
     import torch
     from scipy.spatial import cKDTree
@@ -404,28 +416,32 @@ def precision_errors(h, f):
     from cudafusion_amd.harness import transform_errors
     from oracle import pyoracle as po
     po.build()
+    import ctypes as C
+
+    from cudafusion_amd import _lib as L
     T0 = h.download_transforms().copy()
     vox0 = h.download_voxels().copy()
-    knn, knn_w = h.download_knn()
+    V = int(np.prod(h.w.dims))
+    t16, w8, k16, kw16 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    L.check(h.lib.nrf_pipeline_volume16(h.pipe, C.byref(t16), C.byref(w8), C.byref(k16), C.byref(kw16)), "volume16")
+    knn = h._d2h(k16.value, 4 * V, np.uint16).astype(np.int32)
+    knn[knn == 0xFFFF] = -1
+    knn_w = h._d2h(kw16.value, 4 * V, np.float16).astype(np.float32)
     h.step(f)
     torch.cuda.synchronize()
     T1 = h.download_transforms().copy()
     vox1 = h.download_voxels()
     mesh = h.mesh().copy()
     corrs, can = h.frame_corrs_host(f)
-    # the FP64 solve runs as many PCG iterations per GN step as the device's
-    # PCG and refinement together
-    from cudafusion_amd import _lib as L
-    sp64 = L.SolverParams.from_buffer_copy(bytes(h.sp))
-    sp64.pcg_iterations = h.sp.pcg_iterations + h.sp.refine_iterations
     T64, _, _, _, _ = po.solve_gn_f64(corrs, can, h.nodes, T0.view(np.float32).reshape(-1, 8).astype(np.float64),
-                                      h.gn, sp64)
+                                      h.gn, h.sp)
     er, et = transform_errors(T1, T64)
     t64, w64 = vox0["tsdf"].astype(np.float64), vox0["weight"].astype(np.float64)
     depth = h.frames_host[f].depth.reshape(-1)
-    po.integrate_f64(h.vp, t64, w64, depth, h.cam, h.pose, h.nodes, T64, knn, knn_w)
-    margins = po.integrate_margin_f64(h.vp, depth, h.cam, h.pose, h.nodes, T64, knn, knn_w)
-    tsdf_err, amb, wmis = po.tsdf_fp64_errors(vox1, t64, w64, margins, px_eps=2e-2, sdf_eps=2e-3)
+    Td = T1.view(np.float32).reshape(-1, 8).astype(np.float64)
+    po.integrate_f64(h.vp, t64, w64, depth, h.cam, h.pose, h.nodes, Td, knn, knn_w)
+    margins = po.integrate_margin_f64(h.vp, depth, h.cam, h.pose, h.nodes, Td, knn, knn_w)
+    tsdf_err, amb, wmis = po.tsdf_fp64_errors(vox1, t64, w64, margins)
     _, v64, _ = po.mc_f64(h.vp, t64, w64, normals=False)
     s = h.w.voxel_size
     d1, _ = cKDTree(v64).query(mesh.reshape(-1, 3).astype(np.float64))

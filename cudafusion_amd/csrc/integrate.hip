@@ -12,6 +12,8 @@
 // and every lane has ZC independent voxels in flight.  The per-voxel DQ
 // blend reads a per-node table of centred transforms (N x 32 B, resident in
 // L2) instead of recomputing dq_centered four times per voxel.
+#include <hip/hip_fp16.h>
+
 #include <vector>
 
 #include "nrf_internal.h"
@@ -47,6 +49,54 @@ struct IntegrateArgs {
   int32_t *updated;
 };
 
+// Storage of the volume: the reference's 12-byte AoS voxels, or the compact
+// form (FP16 value + uint8 weight, the update rounded once when stored:
+// FP16 round-to-nearest-even, weight to the nearest integer, capped at 255).
+struct VoxAoS {
+  nrf_voxel *v;
+  __device__ float2 load(int idx) const {
+    const nrf_voxel &x = v[idx];
+    return make_float2(x.tsdf, x.weight);
+  }
+  __device__ void store(int idx, float tsdf, float weight) const {
+    nrf_voxel &x = v[idx];
+    x.tsdf = tsdf;
+    x.weight = weight;
+  }
+};
+struct VoxCompact {
+  __half *t;
+  uint8_t *w;
+  __device__ float2 load(int idx) const { return make_float2(__half2float(t[idx]), (float)w[idx]); }
+  __device__ void store(int idx, float tsdf, float weight) const {
+    t[idx] = __float2half_rn(tsdf);
+    w[idx] = (uint8_t)fminf(rintf(weight), 255.f);
+  }
+};
+// Storage of the voxel kNN: int32 ids + FP32 weights, or uint16 ids
+// (0xffff = none) + FP16 weights, decoded exactly.
+struct KnnF32 {
+  const int4 *id;
+  const float4 *w;
+  __device__ int4 ids(int idx) const { return id[idx]; }
+  __device__ float4 ws(int idx) const { return w[idx]; }
+};
+struct Knn16 {
+  const ushort4 *id;
+  const uint2 *w;
+  __device__ int4 ids(int idx) const {
+    const ushort4 k = id[idx];
+    auto d = [](unsigned short v) { return v == 0xffff ? -1 : (int)v; };
+    return make_int4(d(k.x), d(k.y), d(k.z), d(k.w));
+  }
+  __device__ float4 ws(int idx) const {
+    const uint2 u = w[idx];
+    const float2 a = __half22float2(*reinterpret_cast<const __half2 *>(&u.x));
+    const float2 b = __half22float2(*reinterpret_cast<const __half2 *>(&u.y));
+    return make_float4(a.x, a.y, b.x, b.y);
+  }
+};
+
 __device__ inline DQ fetch_centered(const IntegrateArgs &a, int id, bool table) {
   if (table) return a.centered[id];
   const nrf_node &n = a.nodes[id];
@@ -59,9 +109,10 @@ __device__ inline DQ fetch_centered(const IntegrateArgs &a, int id, bool table) 
 // read: those voxels integrate unwarped, as with all-(-1) ids), leaving the
 // pixel and camera depth of each voxel; (2) the depth pixels; (3) the
 // voxels that pass the tests, then their updates.  Same arithmetic per
-// voxel as one voxel at a time.
-template <bool WARP, bool TABLE, int ZC>
-__device__ void integrate_tile(const IntegrateArgs &a, int bx, int by, int bz) {
+// voxel as one voxel at a time.  Vox / Knn: the storage of the volume and of
+// the kNN (the same arithmetic on the decoded values).
+template <bool WARP, bool TABLE, int ZC, class Vox, class Knn>
+__device__ void integrate_tile(const IntegrateArgs &a, const Vox &vs, const Knn &kn, int bx, int by, int bz) {
   const int x = bx * kTileX + (threadIdx.x & (kTileX - 1));
   const int y = by * kTileY + (threadIdx.x / kTileX);
   const int z0 = bz * ZC;
@@ -89,9 +140,9 @@ __device__ void integrate_tile(const IntegrateArgs &a, int bx, int by, int bz) {
         // the same unwarped path without the 16-byte read per voxel
         int4 id4 = make_int4(-1, -1, -1, -1);
         const int run = __builtin_amdgcn_readfirstlane((z * a.Y + y) * a.nsx + bx);  // one per wave
-        if (!a.knn_rows || a.knn_rows[run] != 0u) id4 = a.knn[idx];
+        if (!a.knn_rows || a.knn_rows[run] != 0u) id4 = kn.ids(idx);
         if ((id4.x & id4.y & id4.z & id4.w) >= 0) {  // sign bit clear: some id >= 0
-          const float4 w4 = a.knn_w[idx];
+          const float4 w4 = kn.ws(idx);
           const int ids[4] = {id4.x, id4.y, id4.z, id4.w};
           const float ws[4] = {w4.x, w4.y, w4.z, w4.w};
           DQ b;
@@ -144,19 +195,14 @@ __device__ void integrate_tile(const IntegrateArgs &a, int bx, int by, int bz) {
 #pragma unroll
     for (int dz = 0; dz < ZC; ++dz) {
       tw[dz] = make_float2(0.f, 0.f);
-      if (pix[dz]) {
-        const nrf_voxel &vx = a.vox[(z0 + dz) * plane + col];
-        tw[dz] = make_float2(vx.tsdf, vx.weight);
-      }
+      if (pix[dz]) tw[dz] = vs.load((z0 + dz) * plane + col);
     }
 #pragma unroll
     for (int dz = 0; dz < ZC; ++dz) {
       if (!pix[dz]) continue;
-      nrf_voxel &vx = a.vox[(z0 + dz) * plane + col];
       const float dw = tw[dz].y * a.decay;
       const float nw = dw + 1.f;
-      vx.tsdf = (tw[dz].x * dw + tv[dz] * 1.f) / nw;
-      vx.weight = fminf(nw, a.wmax);
+      vs.store((z0 + dz) * plane + col, (tw[dz].x * dw + tv[dz] * 1.f) / nw, fminf(nw, a.wmax));
       ++n_upd;
     }
   }
@@ -168,7 +214,13 @@ __device__ void integrate_tile(const IntegrateArgs &a, int bx, int by, int bz) {
 }
 template <bool WARP, bool TABLE>
 __global__ __launch_bounds__(256) void integrate_kernel(IntegrateArgs a) {
-  integrate_tile<WARP, TABLE, kZChunk>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+  integrate_tile<WARP, TABLE, kZChunk>(a, VoxAoS{a.vox}, KnnF32{a.knn, a.knn_w}, blockIdx.x, blockIdx.y,
+                                       blockIdx.z);
+}
+// The compact form (warped through the centred table when a.N > 0).
+template <bool WARP>
+__global__ __launch_bounds__(256) void integrate16_kernel(IntegrateArgs a, VoxCompact vs, Knn16 kn) {
+  integrate_tile<WARP, true, kZChunk>(a, vs, kn, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // All instances of a batch in one launch.  Instance k owns the tiles
@@ -195,7 +247,7 @@ __global__ __launch_bounds__(256) void integrate_batch_kernel(const IntBatchDev 
   a.T = fr.xf[lo];
   const int l = t - tile_base[lo];
   const int bx = l % d.tiles_x, r = l / d.tiles_x;
-  integrate_tile<true, true, ZC>(a, bx, r % d.tiles_y, r / d.tiles_y);
+  integrate_tile<true, true, ZC>(a, VoxAoS{a.vox}, KnnF32{a.knn, a.knn_w}, bx, r % d.tiles_y, r / d.tiles_y);
 }
 
 // Row occupancy of a kNN: four waves per workgroup, one run of 64 voxels per
@@ -373,6 +425,49 @@ int tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels, const float *
     integrate_kernel<true, true><<<grid, 256, 0, st>>>(a);
   else
     integrate_kernel<true, false><<<grid, 256, 0, st>>>(a);
+  NRF_LAUNCH_CHECK();
+  return NRF_OK;
+}
+
+int tsdf16_integrate(const nrf_volume_params *vp, uint16_t *tsdf16, uint8_t *weight8, const float *depth,
+                     const nrf_camera *cam, const float camera_pose[16], const nrf_dualquat *centered, int num_nodes,
+                     const uint16_t *knn16, const uint16_t *knn_w16, const uint32_t *knn_rows,
+                     int32_t *updated_count, hipStream_t st) {
+  if (!volume_ok(vp) || !tsdf16 || !weight8 || !depth || !cam || !camera_pose) return NRF_ERR_INVALID_ARGUMENT;
+  if (cam->width <= 0 || cam->height <= 0) return NRF_ERR_INVALID_ARGUMENT;
+  if (num_nodes > 0 && (!centered || !knn16 || !knn_w16)) return NRF_ERR_INVALID_ARGUMENT;
+  if (num_nodes > 65535) return NRF_ERR_TOO_LARGE;
+  if (vp->max_weight > 255.f) return NRF_ERR_TOO_LARGE;
+  IntegrateArgs a;
+  a.vox = nullptr;
+  a.X = vp->dims[0]; a.Y = vp->dims[1]; a.Z = vp->dims[2];
+  a.ox = vp->origin[0]; a.oy = vp->origin[1]; a.oz = vp->origin[2];
+  a.s = vp->voxel_size;
+  a.trunc = vp->truncation;
+  a.depth = depth;
+  a.W = cam->width; a.H = cam->height;
+  a.fx = cam->fx; a.fy = cam->fy; a.cx = cam->cx; a.cy = cam->cy;
+  a.T = invert_pose(camera_pose);
+  a.nodes = nullptr;
+  a.transforms = nullptr;
+  a.centered = reinterpret_cast<const DQ *>(centered);
+  a.N = num_nodes;
+  a.knn = nullptr;
+  a.knn_w = nullptr;
+  a.knn_rows = knn_rows;
+  a.nsx = (a.X + kTileX - 1) / kTileX;
+  a.opt = nullptr;
+  a.min_opt = 0;
+  a.decay = vp->decay_alpha;
+  a.wmax = vp->max_weight;
+  a.updated = updated_count;
+  const VoxCompact vs{reinterpret_cast<__half *>(tsdf16), weight8};
+  const Knn16 kn{reinterpret_cast<const ushort4 *>(knn16), reinterpret_cast<const uint2 *>(knn_w16)};
+  dim3 grid((a.X + kTileX - 1) / kTileX, (a.Y + kTileY - 1) / kTileY, (a.Z + kZChunk - 1) / kZChunk);
+  if (num_nodes > 0)
+    integrate16_kernel<true><<<grid, 256, 0, st>>>(a, vs, kn);
+  else
+    integrate16_kernel<false><<<grid, 256, 0, st>>>(a, vs, kn);
   NRF_LAUNCH_CHECK();
   return NRF_OK;
 }

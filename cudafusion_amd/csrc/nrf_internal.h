@@ -127,6 +127,11 @@ int integrate_batch_run(IntegrateBatch *b, const BatchDepth &fr, hipStream_t st)
 // voxels all integrate unwarped.  Valid until the kNN changes.
 int64_t knn_row_words(const nrf_volume_params *vp);
 int knn_row_flags(const nrf_volume_params *vp, const int32_t *knn, uint32_t *rows, hipStream_t st);
+// nrf_tsdf16_integrate with the kNN's row occupancy (null: none)
+int tsdf16_integrate(const nrf_volume_params *vp, uint16_t *tsdf16, uint8_t *weight8, const float *depth,
+                     const nrf_camera *cam, const float camera_pose[16], const nrf_dualquat *centered, int num_nodes,
+                     const uint16_t *knn16, const uint16_t *knn_w16, const uint32_t *knn_rows,
+                     int32_t *updated_count, hipStream_t st);
 // nrf_tsdf_integrate with the kNN's row occupancy (null: none)
 int tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels, const float *depth, const nrf_camera *cam,
                    const float camera_pose[16], const nrf_node *nodes, const nrf_dualquat *transforms, int num_nodes,

@@ -214,8 +214,8 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
   if (rc) return rc;
   if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[1], st));
   if (p->cfg.storage16)
-    rc = nrf_tsdf16_integrate(&p->cfg.volume, p->t16, p->w8, depth, &p->cfg.camera, camera_pose, p->centered, p->N,
-                              p->k16, p->kw16, p->updated, stream);
+    rc = tsdf16_integrate(&p->cfg.volume, p->t16, p->w8, depth, &p->cfg.camera, camera_pose, p->centered, p->N,
+                          p->k16, p->kw16, p->knn_rows, p->updated, st);
   else
     rc = tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, p->nodes, p->T, p->N,
                         p->centered, p->knn, p->knn_w, p->knn_rows, nullptr, p->cfg.min_opt_count, p->updated, st);

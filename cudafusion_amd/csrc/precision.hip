@@ -17,10 +17,6 @@
 namespace nrf {
 namespace {
 
-constexpr int kTileX = 64;
-constexpr int kTileY = 4;
-constexpr int kZChunk = 4;
-
 __global__ void compact_knn_kernel(const int4 *__restrict__ knn, const float4 *__restrict__ w, int64_t V,
                                    ushort4 *__restrict__ knn16, uint2 *__restrict__ w16) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -40,95 +36,6 @@ __global__ void voxels_to_compact_kernel(const nrf_voxel *__restrict__ v, int64_
   t16[i] = __float2half_rn(v[i].tsdf);
   const float w = fminf(fmaxf(rintf(v[i].weight), 0.f), 255.f);
   w8[i] = (uint8_t)w;
-}
-
-struct Int16Args {
-  __half *t16;
-  uint8_t *w8;
-  int X, Y, Z;
-  float ox, oy, oz, s, trunc;
-  const float *depth;
-  int W, H;
-  float fx, fy, cx, cy;
-  CamXf T;
-  const DQ *centered;
-  int N;
-  const ushort4 *knn;
-  const uint2 *knn_w;
-  float decay, wmax;
-  int32_t *updated;
-};
-
-__global__ __launch_bounds__(256) void integrate16_kernel(Int16Args a) {
-  const int x = blockIdx.x * kTileX + (threadIdx.x & (kTileX - 1));
-  const int y = blockIdx.y * kTileY + (threadIdx.x / kTileX);
-  const int z0 = blockIdx.z * kZChunk;
-  int n_upd = 0;
-  if (x < a.X && y < a.Y) {
-    const float px_w = a.ox + (x + 0.5f) * a.s;
-    const float py_w = a.oy + (y + 0.5f) * a.s;
-#pragma unroll
-    for (int dz = 0; dz < kZChunk; ++dz) {
-      const int z = z0 + dz;
-      if (z >= a.Z) break;
-      const int idx = z * a.X * a.Y + y * a.X + x;
-      float3 p = f3(px_w, py_w, a.oz + (z + 0.5f) * a.s);
-      if (a.N > 0) {
-        const ushort4 k4 = a.knn[idx];
-        const uint2 w2 = a.knn_w[idx];
-        const __half2 wa = *reinterpret_cast<const __half2 *>(&w2.x);
-        const __half2 wb = *reinterpret_cast<const __half2 *>(&w2.y);
-        const float2 fa = __half22float2(wa), fb = __half22float2(wb);
-        const int ids[4] = {k4.x == 0xffff ? -1 : (int)k4.x, k4.y == 0xffff ? -1 : (int)k4.y,
-                            k4.z == 0xffff ? -1 : (int)k4.z, k4.w == 0xffff ? -1 : (int)k4.w};
-        const float ws[4] = {fa.x, fa.y, fb.x, fb.y};
-        DQ b;
-        b.r = make_float4(0.f, 0.f, 0.f, 0.f);
-        b.d = make_float4(0.f, 0.f, 0.f, 0.f);
-        float wsum = 0.f;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int id = ids[k];
-          const float w = ws[k];
-          if (id < 0 || id >= a.N || w < 1e-8f) continue;
-          DQ q = a.centered[id];
-          if (qdot(q.r, b.r) < 0) {
-            q.r.x = -q.r.x; q.r.y = -q.r.y; q.r.z = -q.r.z; q.r.w = -q.r.w;
-            q.d.x = -q.d.x; q.d.y = -q.d.y; q.d.z = -q.d.z; q.d.w = -q.d.w;
-          }
-          b.r.x += w * q.r.x; b.r.y += w * q.r.y; b.r.z += w * q.r.z; b.r.w += w * q.r.w;
-          b.d.x += w * q.d.x; b.d.y += w * q.d.y; b.d.z += w * q.d.z; b.d.w += w * q.d.w;
-          wsum += w;
-        }
-        if (wsum >= 1e-8f) p = dq_transform(dq_normalize(b), p);
-      }
-      const float cxp = a.T.m[0][0] * p.x + a.T.m[0][1] * p.y + a.T.m[0][2] * p.z + a.T.m[0][3];
-      const float cyp = a.T.m[1][0] * p.x + a.T.m[1][1] * p.y + a.T.m[1][2] * p.z + a.T.m[1][3];
-      const float czp = a.T.m[2][0] * p.x + a.T.m[2][1] * p.y + a.T.m[2][2] * p.z + a.T.m[2][3];
-      if (czp <= 0) continue;
-      const float fu = a.fx * cxp / czp + a.cx;
-      const float fv = a.fy * cyp / czp + a.cy;
-      if (!(fu > -1.0e9f && fu < 1.0e9f && fv > -1.0e9f && fv < 1.0e9f)) continue;
-      const int u = (int)rintf(fu);
-      const int v = (int)rintf(fv);
-      if (u < 0 || u >= a.W || v < 0 || v >= a.H) continue;
-      const float d = a.depth[v * a.W + u];
-      if (d <= 0) continue;
-      const float sdf = d - czp;
-      if (sdf < -a.trunc) continue;
-      const float t = fminf(1.f, sdf / a.trunc);
-      const float w_old = (float)a.w8[idx];
-      const float dw = w_old * a.decay;
-      const float nw = dw + 1.f;
-      a.t16[idx] = __float2half_rn((__half2float(a.t16[idx]) * dw + t * 1.f) / nw);
-      a.w8[idx] = (uint8_t)fminf(rintf(fminf(nw, a.wmax)), 255.f);
-      ++n_upd;
-    }
-  }
-  if (a.updated) {
-    for (int off = 32; off > 0; off >>= 1) n_upd += __shfl_xor(n_upd, off, 64);
-    if ((threadIdx.x & 63) == 0 && n_upd) atomicAdd(a.updated, n_upd);
-  }
 }
 
 __global__ void fill16_kernel(__half *t16, uint8_t *w8, int64_t V) {
@@ -258,32 +165,8 @@ extern "C" int nrf_tsdf16_integrate(const nrf_volume_params *vp, uint16_t *tsdf1
                                     const float *depth, const nrf_camera *cam, const float camera_pose[16],
                                     const nrf_dualquat *centered, int num_nodes, const uint16_t *knn16,
                                     const uint16_t *knn_w16, int32_t *updated_count, void *stream) {
-  if (!volume_ok(vp) || !tsdf16 || !weight8 || !depth || !cam || !camera_pose) return NRF_ERR_INVALID_ARGUMENT;
-  if (num_nodes > 0 && (!centered || !knn16 || !knn_w16)) return NRF_ERR_INVALID_ARGUMENT;
-  if (num_nodes > 65535) return NRF_ERR_TOO_LARGE;
-  if (vp->max_weight > 255.f) return NRF_ERR_TOO_LARGE;
-  Int16Args a;
-  a.t16 = reinterpret_cast<__half *>(tsdf16);
-  a.w8 = weight8;
-  a.X = vp->dims[0]; a.Y = vp->dims[1]; a.Z = vp->dims[2];
-  a.ox = vp->origin[0]; a.oy = vp->origin[1]; a.oz = vp->origin[2];
-  a.s = vp->voxel_size;
-  a.trunc = vp->truncation;
-  a.depth = depth;
-  a.W = cam->width; a.H = cam->height;
-  a.fx = cam->fx; a.fy = cam->fy; a.cx = cam->cx; a.cy = cam->cy;
-  a.T = invert_pose(camera_pose);
-  a.centered = reinterpret_cast<const DQ *>(centered);
-  a.N = num_nodes;
-  a.knn = reinterpret_cast<const ushort4 *>(knn16);
-  a.knn_w = reinterpret_cast<const uint2 *>(knn_w16);
-  a.decay = vp->decay_alpha;
-  a.wmax = vp->max_weight;
-  a.updated = updated_count;
-  dim3 grid((a.X + kTileX - 1) / kTileX, (a.Y + kTileY - 1) / kTileY, (a.Z + kZChunk - 1) / kZChunk);
-  integrate16_kernel<<<grid, 256, 0, as_stream(stream)>>>(a);
-  NRF_LAUNCH_CHECK();
-  return NRF_OK;
+  return tsdf16_integrate(vp, tsdf16, weight8, depth, cam, camera_pose, centered, num_nodes, knn16, knn_w16,
+                          nullptr, updated_count, as_stream(stream));
 }
 
 extern "C" int nrf_mc16_extract(const nrf_volume_params *vp, const uint16_t *tsdf16, const uint8_t *weight8,

@@ -630,25 +630,32 @@ __global__ __launch_bounds__(256) void corr_prep_kernel(AsmArgs a, int n, float 
 // as v_mfma_f32_16x16x4f32 products (rows 6..15 of A are zero).  Then the
 // blocks are C[:, 0:6R] and the RHS is -C[:, 6R].  Same products as the
 // scalar form, FP32, summed in the matrix unit's order.
-// Reduced-precision operands of the JtJ products: bf16 (round to nearest
-// even on the upper 16 bits of the FP32 pattern, NaN kept quiet) or fp16
-// (the hardware conversion, round to nearest even).
+// Reduced-precision operands of the JtJ products: bf16 or fp16, both by the
+// hardware conversion (round to nearest even).
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 template <int PREC> struct LpType;
 template <> struct LpType<1> { typedef short elem; typedef bf16x4 vec; };
 template <> struct LpType<2> { typedef _Float16 elem; typedef f16x4 vec; };
 template <> struct LpType<0> { typedef short elem; typedef bf16x4 vec; };  // unused
+// PREC 3 / 4: BF16 / FP16 operands split in two (hi = rounded value, lo =
+// the rounded remainder), three products per pair (lo hi + hi lo + hi hi).
+template <> struct LpType<3> : LpType<1> {};
+template <> struct LpType<4> : LpType<2> {};
+template <int PREC> constexpr int lp_base() { return PREC >= 3 ? PREC - 2 : PREC; }
 template <int PREC>
 __device__ inline typename LpType<PREC>::elem to_lp(float f) {
-  if constexpr (PREC == 2) {
+  if constexpr (PREC == 2)
     return (_Float16)f;
-  } else {
-    unsigned u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (short)((u >> 16) | 0x40u);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (short)(u >> 16);
-  }
+  else
+    return __builtin_bit_cast(short, (__bf16)f);  // v_cvt_pk_bf16_f32: round to nearest even
+}
+template <int PREC>
+__device__ inline float from_lp(typename LpType<PREC>::elem v) {
+  if constexpr (PREC == 2)
+    return (float)v;
+  else
+    return __uint_as_float((unsigned)(unsigned short)v << 16);
 }
 template <int PREC>
 __device__ inline floatx4_t mfma_lp(typename LpType<PREC>::vec a, typename LpType<PREC>::vec b, floatx4_t c) {
@@ -826,22 +833,41 @@ __device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_
           C[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bq[16 * s * kBS], C[s], 0, 0, 0);
       }
     } else {
-      // JtJ blocks in BF16 (PREC 1) or FP16 (PREC 2) with FP32 accumulation:
-      // 16 memberships per product, lane (am, kq) supplying memberships
-      // 16 t + 4 kq + q, q = 0..3.  The RHS column stays FP32 (below), so
-      // the reduced precision touches the Hessian approximation only.
+      // JtJ blocks in BF16 (PREC 1, 3) or FP16 (PREC 2, 4) with FP32
+      // accumulation: 16 memberships per product, lane (am, kq) supplying
+      // memberships 16 t + 4 kq + q, q = 0..3.  The split forms (3, 4) carry
+      // each operand as hi + lo, both in the type, and add lo_a hi_b +
+      // hi_a lo_b before hi_a hi_b: about 16 (BF16) / 22 (FP16) significant
+      // bits per operand, only lo_a lo_b dropped.  The FP16 split scales
+      // both operands by 2^6 (undone exactly on the FP32 sums) so that the
+      // low parts stay normal numbers.  The RHS column stays FP32 (below).
+      constexpr int B = lp_base<PREC>();
+      constexpr bool kSplit = PREC >= 3;
+      constexpr float kSc = PREC == 4 ? 64.f : 1.f;
 #pragma unroll
       for (int t = 0; t < kBat / 16; ++t) {
-        lp4 av;
+        lp4 ah, al;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) av[q] = to_lp<PREC>(Ab[(16 * t + 4 * kq + q) * 8 + ar]);
+        for (int q = 0; q < 4; ++q) {
+          const float v = Ab[(16 * t + 4 * kq + q) * 8 + ar] * kSc;
+          ah[q] = to_lp<B>(v);
+          if constexpr (kSplit) al[q] = to_lp<B>(v - from_lp<B>(ah[q]));
+        }
         const float *bq = Bc + am * kBS + 16 * t + 4 * kq;
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-          lp4 bv;
+          lp4 bh, bl;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) bv[q] = to_lp<PREC>(bq[16 * s * kBS + q]);
-          C[s] = mfma_lp<PREC>(av, bv, C[s]);
+          for (int q = 0; q < 4; ++q) {
+            const float v = bq[16 * s * kBS + q] * kSc;
+            bh[q] = to_lp<B>(v);
+            if constexpr (kSplit) bl[q] = to_lp<B>(v - from_lp<B>(bh[q]));
+          }
+          if constexpr (kSplit) {
+            C[s] = mfma_lp<B>(al, bh, C[s]);
+            C[s] = mfma_lp<B>(ah, bl, C[s]);
+          }
+          C[s] = mfma_lp<B>(ah, bh, C[s]);
         }
       }
 #pragma unroll
@@ -850,6 +876,12 @@ __device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_
                                                     am == 0 ? Bc[(ncol - 1) * kBS + 4 * t + kq] : 0.f, Crhs, 0, 0, 0);
     }
     wave_sync();
+  }
+  if constexpr (PREC == 4) {  // undo the operand scaling (exact: a power of two)
+    if (mode != 2) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) C[s] = C[s] * (1.f / 4096.f);
+    }
   }
   if (mode == 1) {  // partial sums of this chunk
     if (lane < 32) {
@@ -1864,6 +1896,44 @@ __global__ __launch_bounds__(kResThreads) void residual_fp64_kernel(const int32_
     partial[2 * blockIdx.x + 1] = s1;
   }
 }
+// One iterative-refinement step whose correction solve is the block-Jacobi
+// inverse: x_out = x + M^-1 (b - A x), the residual (products and sums) and
+// the correction in FP64.  One workgroup of 256 lanes per kRefNodes nodes:
+// lane 6 k + r forms row r of node k's residual in LDS, then row r of its
+// correction.  x and x_out must not alias (a row reads its neighbours' x).
+// A raised GN stop flag makes it a no-op.
+constexpr int kRefNodes = 42;
+__global__ __launch_bounds__(256) void refine_jacobi_kernel(const int32_t *__restrict__ row_ptr,
+                                                           const int32_t *__restrict__ col_idx,
+                                                           const float *__restrict__ values,
+                                                           const float *__restrict__ b,
+                                                           const float *__restrict__ Minv,
+                                                           const float *__restrict__ x, int N,
+                                                           const int32_t *__restrict__ stop,
+                                                           float *__restrict__ x_out) {
+  __shared__ double rs[kRefNodes * 6];
+  if (stop && *stop) return;
+  const int t = threadIdx.x, k = t / 6, row = t % 6;
+  const int i = blockIdx.x * kRefNodes + k;
+  const bool act = t < kRefNodes * 6 && i < N;
+  if (act) {
+    double acc = 0.0;
+    for (int bk = row_ptr[i]; bk < row_ptr[i + 1]; ++bk) {
+      const float *A = values + (size_t)bk * 36 + row * 6;
+      const float *xj = x + 6 * col_idx[bk];
+      for (int c = 0; c < 6; ++c) acc += (double)A[c] * (double)xj[c];
+    }
+    rs[t] = (double)b[6 * i + row] - acc;
+  }
+  __syncthreads();
+  if (act) {
+    const float *Mi = Minv + (size_t)i * 36 + row * 6;
+    double z = 0.0;
+    for (int c = 0; c < 6; ++c) z += (double)Mi[c] * rs[6 * k + c];
+    x_out[6 * i + row] = (float)((double)x[6 * i + row] + z);
+  }
+}
+
 // Sums the per-workgroup partials in index order: out = {|b - Ax|, |b|}.
 __global__ void residual_norms_kernel(const double *__restrict__ partial, int nblocks, double *__restrict__ out) {
   if (threadIdx.x == 0) {
@@ -2000,11 +2070,8 @@ __global__ __launch_bounds__(1024) void gn_validate_kernel(GnArgs a) { gn_valida
 // scale (upload_scaled_delta_x, include/fusion_app.h:1353-1375), then
 // apply_twist_increment(..., 1.0) (src/warp_field.cu:1121-1141) and the
 // centred-table refresh.
-__device__ void gn_update_body(const GnArgs &a) {
-  if (*a.stop) return;
-  const int t = threadIdx.x;
-  for (int i = blockIdx.x * 256 + t; i < a.N; i += gridDim.x * 256) {
-    const float *d = a.dx + i * 6;
+__device__ inline void gn_update_node(const GnArgs &a, int i, const float d[6]) {
+  {
     float tw[6];
     for (int k = 0; k < 6; ++k) tw[k] = d[k];
     const float rn = sqrtf(tw[0] * tw[0] + tw[1] * tw[1] + tw[2] * tw[2]);
@@ -2031,9 +2098,60 @@ __device__ void gn_update_body(const GnArgs &a) {
     a.T[i] = Tn;
     if (a.centered) a.centered[i] = dq_centered(Tn, f3(a.nodes[i].pos));
   }
+}
+__device__ void gn_update_body(const GnArgs &a) {
+  if (*a.stop) return;
+  const int t = threadIdx.x;
+  for (int i = blockIdx.x * 256 + t; i < a.N; i += gridDim.x * 256) gn_update_node(a, i, a.dx + i * 6);
   if (blockIdx.x == 0 && t == 0) a.res->gn_done += 1;
 }
 __global__ __launch_bounds__(256) void gn_update_kernel(GnArgs a) { gn_update_body(a); }
+
+// The refinement step x = d + M^-1 (b - A d) (refine_jacobi_kernel's
+// arithmetic, residual and correction in FP64) fused with the update when
+// no validation sits between them.  Six lanes per node (lane 6 k + r of a
+// wave: row r of node k, ten nodes per wave) form the node's residual rows
+// and correction, exchanged by lane moves; lane 6 k stores the refined step
+// in dx_out and applies it.  d (the PCG's solution) is only read, so no lane
+// reads a value another lane writes.
+struct RefineArgs {
+  const int32_t *row_ptr, *col_idx;
+  const float *values, *b, *Minv, *d;
+  float *dx_out;
+};
+constexpr int kRuNodesPerWave = 10;
+__global__ __launch_bounds__(256) void gn_refine_update_kernel(GnArgs a, RefineArgs r) {
+  if (*a.stop) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int k = lane / 6, row = lane % 6;
+  const int i = (blockIdx.x * 4 + w) * kRuNodesPerWave + k;
+  const bool act = lane < 6 * kRuNodesPerWave && i < a.N;
+  double res = 0.0;
+  if (act) {
+    double acc = 0.0;
+    for (int bk = r.row_ptr[i]; bk < r.row_ptr[i + 1]; ++bk) {
+      const float *A = r.values + (size_t)bk * 36 + row * 6;
+      const float *xj = r.d + 6 * r.col_idx[bk];
+      for (int c = 0; c < 6; ++c) acc += (double)A[c] * (double)xj[c];
+    }
+    res = (double)r.b[6 * i + row] - acc;
+  }
+  const int base = 6 * k;
+  double z = 0.0;
+  for (int c = 0; c < 6; ++c) {
+    const double rc = __shfl(res, (base + c) & 63, 64);
+    if (act) z += (double)r.Minv[(size_t)i * 36 + row * 6 + c] * rc;
+  }
+  float x = 0.f;
+  if (act) {
+    x = (float)((double)r.d[6 * i + row] + z);
+    r.dx_out[6 * i + row] = x;
+  }
+  float xs[6];
+  for (int c = 0; c < 6; ++c) xs[c] = __shfl(x, (base + c) & 63, 64);
+  if (act && row == 0) gn_update_node(a, i, xs);
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.res->gn_done += 1;
+}
 
 // ------------------------------------------------------------- batches
 // Several independent solves stepped by common launches (nrf_batch): every
@@ -2567,6 +2685,12 @@ static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, int n, const fl
     } else if (prec == 2) {                                                                     \
       assemble_rows_mfma_kernel<NSV, 2><<<nch, 64, lds, st>>>(a);                               \
       assemble_rows_tail_kernel<NSV, 2><<<N, 64, lds, st>>>(a);                                 \
+    } else if (prec == 3) {                                                                     \
+      assemble_rows_mfma_kernel<NSV, 3><<<nch, 64, lds, st>>>(a);                               \
+      assemble_rows_tail_kernel<NSV, 3><<<N, 64, lds, st>>>(a);                                 \
+    } else if (prec == 4) {                                                                     \
+      assemble_rows_mfma_kernel<NSV, 4><<<nch, 64, lds, st>>>(a);                               \
+      assemble_rows_tail_kernel<NSV, 4><<<N, 64, lds, st>>>(a);                                 \
     } else {                                                                                    \
       assemble_rows_mfma_kernel<NSV, 0><<<nch, 64, lds, st>>>(a);                               \
       assemble_rows_tail_kernel<NSV, 0><<<N, 64, lds, st>>>(a);                                 \
@@ -2593,7 +2717,18 @@ static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxi
 
 // opened: the launch was opened by the assembly (solver_assemble's open_pcg)
 static int solver_pcg(nrf_solver *s, float *x, int N, hipStream_t st, bool opened = false) {
-  int rc = pcg_launch(s, s->b, x, N, s->params.pcg_iterations, true, st, opened);
+  int rc;
+  if (s->params.refine_jacobi) {
+    // the PCG solves into the scratch vector; one kernel then writes
+    // x = d + M^-1 (b - A d)
+    if ((rc = dev_alloc(s->dref, (size_t)6 * N, s->c_dref))) return rc;
+    if ((rc = pcg_launch(s, s->b, s->dref, N, s->params.pcg_iterations, true, st, opened))) return rc;
+    refine_jacobi_kernel<<<(N + kRefNodes - 1) / kRefNodes, 256, 0, st>>>(s->row_ptr, s->col_idx, s->values, s->b,
+                                                                           s->Minv, s->dref, N, s->stop, x);
+    NRF_LAUNCH_CHECK();
+    return NRF_OK;
+  }
+  rc = pcg_launch(s, s->b, x, N, s->params.pcg_iterations, true, st, opened);
   if (rc || s->params.refine_iterations <= 0) return rc;
   // one iterative-refinement step: r = b - A x accumulated in FP64, A d = r
   // by refine_iterations PCG iterations, x += d
@@ -2684,7 +2819,8 @@ static bool params_ok(const nrf_solver_params *p) {
   // synchronisation points in 16 bits: 1 + 2 * iterations must fit.
   return p->pcg_iterations >= 0 && 1 + 2 * (long)p->pcg_iterations <= kPcgMaxPoints &&
          p->refine_iterations >= 0 && 1 + 2 * (long)p->refine_iterations <= kPcgMaxPoints && p->jtj_precision >= 0 &&
-         p->jtj_precision <= 2 && p->pcg_pipelined >= 0 && p->pcg_pipelined <= 3;
+         p->jtj_precision <= 4 && p->pcg_pipelined >= 0 && p->pcg_pipelined <= 3 &&
+         (p->refine_jacobi == 0 || (p->refine_jacobi == 1 && p->refine_iterations == 0));
 }
 
 extern "C" nrf_solver *nrf_solver_create(const nrf_solver_params *params, int max_nodes) {
@@ -2801,6 +2937,17 @@ extern "C" int nrf_solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const f
                               reinterpret_cast<const DQ *>(centered), num_nodes, st, true)))
       return rc;
     sink_record(s, st);
+    if (s->params.refine_jacobi && !g.validate) {
+      // the refinement step fused into the update (no validation between)
+      if ((rc = dev_alloc(s->dref, (size_t)6 * num_nodes, s->c_dref))) return rc;
+      if ((rc = pcg_launch(s, s->b, s->dref, num_nodes, s->params.pcg_iterations, true, st, true))) return rc;
+      sink_record(s, st);
+      const RefineArgs ra{s->row_ptr, s->col_idx, s->values, s->b, s->Minv, s->dref, delta_x};
+      gn_refine_update_kernel<<<(num_nodes + 4 * kRuNodesPerWave - 1) / (4 * kRuNodesPerWave), 256, 0, st>>>(g, ra);
+      NRF_LAUNCH_CHECK();
+      sink_record(s, st);
+      continue;
+    }
     if ((rc = solver_pcg(s, delta_x, num_nodes, st, true))) return rc;
     sink_record(s, st);
     if (g.validate) {
@@ -2967,7 +3114,7 @@ int solver_batch_create(const SolverBatchInstance *inst, int count, int lanes_pe
     // precision and GN iteration count, no refinement step
     const nrf_solver_params &sp = s->params;
     const int ns = asm_strips(s->P.Rmax);
-    if (!sp.pcg_pipelined || sp.refine_iterations != 0 || !s->asm_mfma || ns > 8 ||
+    if (!sp.pcg_pipelined || sp.refine_iterations != 0 || sp.refine_jacobi != 0 || !s->asm_mfma || ns > 8 ||
         asm_lds_bytes(s->P.Rmax) > 64 * 1024 ||
         (k > 0 && (sp.jtj_precision != b->prec || std::max(1, q.gn.gn_iterations) != b->gn_iters ||
                    sp.pcg_pipelined - 1 != b->form))) {
@@ -3114,6 +3261,12 @@ static int batch_assemble(SolverBatch *b, const BatchCorrs &fr, hipStream_t st) 
     } else if (b->prec == 2) {                                                                                \
       assemble_chunks_batch_kernel<NSV, 2><<<nch, 64, lds, st>>>(b->d_inst, fr, b->count);                    \
       assemble_tail_batch_kernel<NSV, 2><<<b->rows, 64, lds, st>>>(b->d_inst, fr, b->row_inst, b->row_local); \
+    } else if (b->prec == 3) {                                                                                \
+      assemble_chunks_batch_kernel<NSV, 3><<<nch, 64, lds, st>>>(b->d_inst, fr, b->count);                    \
+      assemble_tail_batch_kernel<NSV, 3><<<b->rows, 64, lds, st>>>(b->d_inst, fr, b->row_inst, b->row_local); \
+    } else if (b->prec == 4) {                                                                                \
+      assemble_chunks_batch_kernel<NSV, 4><<<nch, 64, lds, st>>>(b->d_inst, fr, b->count);                    \
+      assemble_tail_batch_kernel<NSV, 4><<<b->rows, 64, lds, st>>>(b->d_inst, fr, b->row_inst, b->row_local); \
     } else {                                                                                                  \
       assemble_chunks_batch_kernel<NSV, 0><<<nch, 64, lds, st>>>(b->d_inst, fr, b->count);                    \
       assemble_tail_batch_kernel<NSV, 0><<<b->rows, 64, lds, st>>>(b->d_inst, fr, b->row_inst, b->row_local); \

@@ -237,15 +237,16 @@ class GaussNewtonSolver:
         arap_consistent_sign: bool = False  # opt-in: ARAP rotation rows with the data term's sign
         pcg_pipelined: int = 0  # opt-in: 1 A p by recurrence, product overlapping an all-reduce; 2 one all-reduce;
         # 3 pipelined: the one all-reduce overlapping the halo exchange and the product
-        jtj_precision: int = 0  # 0 FP32, 1 BF16, 2 FP16 JtJ blocks on the matrix cores (RHS stays FP32)
+        jtj_precision: int = 0  # 0 FP32, 1 BF16, 2 FP16, 3 / 4 split BF16 / FP16 JtJ blocks (RHS stays FP32)
         refine_iterations: int = 0  # > 0: one FP64-residual iterative-refinement step per solve
+        refine_jacobi: bool = False  # one refinement step with the block-Jacobi inverse as its correction solve
 
         def to_c(self) -> L.SolverParams:
             return L.solver_params(self.gn_iterations, self.pcg_iterations, self.pcg_tolerance, self.lambda_smooth,
                                    self.lambda_rot, self.lambda_damping, int(self.debug),
                                    int(self.relative_tolerance), int(self.precond_fp64),
                                    int(self.arap_consistent_sign), int(self.pcg_pipelined),
-                                   int(self.jtj_precision), int(self.refine_iterations))
+                                   int(self.jtj_precision), int(self.refine_iterations), int(self.refine_jacobi))
 
     def __init__(self, p: "GaussNewtonSolver.Params", max_nodes: int):
         self.lib = L.load()

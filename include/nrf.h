@@ -127,11 +127,18 @@ typedef struct nrf_solver_params {
                                    exp map use; 0 keeps the reference's negated rows
                                    (kernels/solver_kernels.cu:145-153) */
   int32_t jtj_precision;       /* data-term JtJ blocks on the matrix cores in FP32 (0),
-                                  BF16 (1) or FP16 (2), FP32 accumulation; the RHS
-                                  stays FP32 (precision form) */
+                                  BF16 (1) or FP16 (2), or with each operand split
+                                  into two BF16 (3) / FP16 (4) parts and three
+                                  products per pair (about 16 / 22 significant bits);
+                                  FP32 accumulation; the RHS stays FP32 (precision
+                                  form).  The FP16 forms need |weight * J| < 1000 */
   int32_t refine_iterations;   /* > 0: after each solve, one iterative-refinement
                                   step: r = b - A x accumulated in FP64, A d = r by
                                   this many PCG iterations, x += d */
+  int32_t refine_jacobi;       /* 1: one iterative-refinement step after each solve
+                                  whose correction solve is the block-Jacobi inverse:
+                                  x += M^-1 (b - A x), residual and correction in
+                                  FP64 (one kernel); refine_iterations must be 0 */
 } nrf_solver_params;
 
 /* Caller-side GN loop settings (include/fusion_app.h:72-119 names). */
@@ -283,7 +290,8 @@ typedef struct nrf_solver nrf_solver;
 
 /* GaussNewtonSolver(const Params&, int max_nodes) (kernels/solver_kernels.cu:486-496).
    Returns NULL (and set_params NRF_ERR_INVALID_ARGUMENT) for pcg_iterations
-   outside [0, 32767] or jtj_precision outside [0, 2]. */
+   or refine_iterations outside [0, 32767], jtj_precision outside [0, 4] or
+   pcg_pipelined outside [0, 3]. */
 nrf_solver *nrf_solver_create(const nrf_solver_params *params, int max_nodes);
 void nrf_solver_destroy(nrf_solver *s);
 int nrf_solver_set_params(nrf_solver *s, const nrf_solver_params *params);

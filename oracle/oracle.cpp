@@ -788,6 +788,23 @@ int solve_gn(const nrf_corr *corrs, const float *canonical, int n, const nrf_nod
         sp.relative_tolerance, x.data(), &iters, &flags, nullptr);
     tot += iters;
     li = iters;
+    if (sp.refine_iterations > 0 || sp.refine_jacobi) {
+      // the solver's optional iterative-refinement step (no reference
+      // counterpart): r = b - A x, then the correction d from A d = r, by
+      // refine_iterations PCG iterations from d = 0 with the same stopping
+      // rules, or d = M^-1 r (refine_jacobi); x += d
+      std::vector<R> r(6 * N), d(6 * N);
+      spmv(rp.data(), ci.data(), A.data(), x.data(), r.data(), N);
+      for (int k = 0; k < 6 * N; ++k) r[k] = b[k] - r[k];
+      if (sp.refine_jacobi) {
+        precond(M.data(), r.data(), d.data(), N);
+      } else {
+        int it2 = 0, fl2 = 0;
+        pcg(rp.data(), ci.data(), A.data(), r.data(), M.data(), N, sp.refine_iterations, (R)sp.pcg_tolerance,
+            sp.relative_tolerance, d.data(), &it2, &fl2, nullptr);
+      }
+      for (int k = 0; k < 6 * N; ++k) x[k] += d[k];
+    }
     if (g.validate && !validate_step(x.data(), N, g)) break;
     apply_step(x.data(), T, N, g);
     ++done;

@@ -167,3 +167,70 @@ def test_pipeline_precision_form_tracks_fp32(gpu, orc, prec):
     np.testing.assert_array_equal(v16["weight"] > 0, v32["weight"] > 0)
     assert np.percentile(dt, 99.9) < 2e-3
     assert abs(n16 - n32) <= 0.02 * n32
+
+
+@pytest.mark.parametrize("prec", [3, 4], ids=["bf16x3", "fp16x3"])
+def test_precision_form_full_scale_against_fp64(gpu, orc, prec):
+    """The precision form at its full-scale workload (256^3 capsule body at
+    4 mm, 640x480, ~2.4k nodes, 5 GN x 30 PCG in the pipelined form at the
+    benchmark's 16 nodes per workgroup): FP16 TSDF + u8 weights, compact
+    kNN, JtJ blocks from split BF16 / FP16 operands on the matrix cores,
+    FP64 Cholesky inverses and one FP64-residual refinement step per solve
+    (x += M^-1 (b - A x), the benchmark's setting).  Two frames, each against
+    the FP64 oracle from the device's state at the start of the frame, with
+    the same algorithm (same PCG iterations and refinement step):
+      * node transforms within 1e-4 rad / 1e-5 m of the FP64 solve, every
+        GN step applied;
+      * TSDF within 1e-3 (truncation units) of the FP64 integration of the
+        frame (device transforms, the compact kNN decoded), weights equal,
+        away from voxels within 2e-3 px / 1e-4 truncations of a rounding
+        boundary of the FP64 projection;
+      * marching cubes on the FP16 TSDF: the oracle's triangle count and
+        vertices (FP32) on the decoded volume."""
+    import ctypes as C
+
+    import torch
+
+    from cudafusion_amd import _lib as L
+    from cudafusion_amd.harness import Harness, transform_errors
+    h = Harness("body", frames=3, arap_consistent_sign=True, validate=False, pcg_pipelined=3,
+                jtj_precision=prec, refine_jacobi=True, precond_fp64=True, storage16=True)
+    h.setup()
+    lib = h.lib
+    L.check(lib.nrf_solver_set_pcg_layout(lib.nrf_pipeline_solver(h.pipe), 4, 16), "pcg_layout")
+    V = int(np.prod(h.w.dims))
+    t16, w8, k16, kw16 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    L.check(lib.nrf_pipeline_volume16(h.pipe, C.byref(t16), C.byref(w8), C.byref(k16), C.byref(kw16)), "volume16")
+    ids = h._d2h(k16.value, 4 * V, np.uint16).astype(np.int32)
+    ids[ids == 0xFFFF] = -1
+    kw = h._d2h(kw16.value, 4 * V, np.float16).astype(np.float32)
+    for f in (1, 2):
+        T0 = h.download_transforms().copy()
+        vox0 = h.download_voxels().copy()
+        h.step(f)
+        torch.cuda.synchronize()
+        T1 = h.download_transforms().copy()
+        vox1 = h.download_voxels().copy()
+        res = h.solver_result()
+        mesh = h.mesh().copy()
+        corrs, can = h.frame_corrs_host(f)
+        T64, _, _, done, _ = orc.solve_gn_f64(corrs, can, h.nodes, T0.view(np.float32).reshape(-1, 8)
+                                              .astype(np.float64), h.gn, h.sp)
+        er, et = transform_errors(T1, T64)
+        # integration of this frame in FP64 from the decoded state, device transforms
+        Td = T1.view(np.float32).reshape(-1, 8).astype(np.float64)
+        t64, w64 = vox0["tsdf"].astype(np.float64), vox0["weight"].astype(np.float64)
+        depth = h.frames_host[f].depth.reshape(-1)
+        orc.integrate_f64(h.vp, t64, w64, depth, h.cam, h.pose, h.nodes, Td, ids, kw)
+        margins = orc.integrate_margin_f64(h.vp, depth, h.cam, h.pose, h.nodes, Td, ids, kw)
+        terr, amb, wmis = orc.tsdf_fp64_errors(vox1, t64, w64, margins)
+        _, rv, _ = orc.mc_f32(h.vp, vox1, normals=False)
+        print(f"[precision {prec}] frame {f}: gn_done {res.gn_done}/{done}, vs FP64 rot {er:.2e} rad trans "
+              f"{et:.2e} m; TSDF max {terr:.2e} (ambiguous {amb:.2e}, weight mismatches {wmis}); triangles "
+              f"{len(mesh) // 3} vs {len(rv) // 3}", flush=True)
+        assert res.gn_done == done == h.w.gn_iterations and not (res.flags & 8)
+        assert er < 1e-4 and et < 1e-5
+        assert wmis == 0 and amb < 0.05 and terr < 1e-3
+        assert len(mesh) == len(rv) and len(mesh) > 30000
+        np.testing.assert_allclose(mesh, rv, atol=1e-6)
+    h.close()

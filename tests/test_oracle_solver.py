@@ -391,3 +391,53 @@ def test_gn_drives_point_to_plane_residuals_to_zero(orc):
     r0 = np.abs(((pts - tgt) * (nrm @ R.T)).sum(1))
     assert r0.max() > 5e-3
     assert res[0].max() < 1e-3 and res[1].max() < 2e-4 and res[1].mean() < 2e-5
+
+
+def test_refinement_step_completes_the_solve(orc):
+    """The optional refinement step of the oracle's GN loop: after a 2-iteration
+    PCG, r = b - A x and A d = r by as many PCG iterations as unknowns from
+    d = 0 (exact in exact arithmetic), x += d.  The returned step is then numpy's dense
+    solve of the assembled system (a refinement that solved A d = b, or that
+    replaced x by d, would not be), while the 2-iteration step alone is not."""
+    nodes = _graph()
+    corrs = _corrs(nodes)
+    N = len(nodes)
+    T64 = orc.identity_dq64(N)
+    rp, ci, A, b, _ = orc.assemble_f64(corrs, nodes, T64, 10.0, 1e-2)
+    dense = _bsr_to_dense(rp, ci, A, N)
+    x_star = np.linalg.solve(dense, b)
+    gn = L.gn_params(1, 1.0, 0.0, 0.0, 0, 0.0)
+    steps = {}
+    for refine in (0, 6 * N):
+        sp = L.solver_params(1, 2, 0.0, 10.0, 1.0, 1e-2, refine_iterations=refine)
+        _, dx, _, done, _ = orc.solve_gn_f64(corrs, None, nodes, T64, gn, sp)
+        assert done == 1
+        steps[refine] = dx
+    rel = lambda v: np.linalg.norm(v - x_star) / np.linalg.norm(x_star)  # noqa: E731
+    assert rel(steps[6 * N]) < 1e-7  # CG in n steps, FP64 rounding of an ill-conditioned system
+    assert rel(steps[0]) > 1e-3
+
+
+def test_refinement_step_block_jacobi_form(orc):
+    """The refinement step with the block-Jacobi inverse as its correction
+    solve: the returned step is x + M^-1 (b - A x) for the PCG's x and the
+    oracle's own assembled A, b and M, and it lowers the residual of a
+    2-iteration PCG on this system."""
+    nodes = _graph()
+    corrs = _corrs(nodes)
+    N = len(nodes)
+    T64 = orc.identity_dq64(N)
+    rp, ci, A, b, M = orc.assemble_f64(corrs, nodes, T64, 10.0, 1e-2)
+    dense = _bsr_to_dense(rp, ci, A, N)
+    gn = L.gn_params(1, 1.0, 0.0, 0.0, 0, 0.0)
+    out = {}
+    for jac in (0, 1):
+        sp = L.solver_params(1, 2, 0.0, 10.0, 1.0, 1e-2, refine_jacobi=jac)
+        out[jac] = orc.solve_gn_f64(corrs, None, nodes, T64, gn, sp)[1]
+    x = out[0]
+    r = b - dense @ x
+    z = np.concatenate([M[i] @ r[6 * i:6 * i + 6] for i in range(N)])
+    assert np.linalg.norm(z) > 1e-3 * np.linalg.norm(x)  # the correction is not negligible
+    # r = b - A x cancels: its rounding depends on the order of the sums
+    np.testing.assert_allclose(out[1] - x, z, rtol=1e-5, atol=1e-6 * np.abs(z).max())
+    assert np.linalg.norm(b - dense @ out[1]) < np.linalg.norm(r)
