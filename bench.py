@@ -469,8 +469,13 @@ def run_batched(args) -> int:
     nf = max(1, args.batch_frames)
     t0 = time.perf_counter()
     ws = S.batch_workloads(count, frames=nf, seed=0, pcg_iterations=300)
+    # convergence judged by the FP64 true residual (correction rounds after a PCG
+    # that stops short of it), and data-term pairs outside the pattern kept on
+    # the diagonal: with the reference's dropped pairs some instances' systems
+    # are indefinite and no PCG reaches the tolerance on them
     kw = dict(frames=nf, use_graph=args.graph, pcg_tolerance=1e-6, relative_tolerance=True,
-              arap_consistent_sign=True, validate=False, pcg_pipelined=args.pcg_form)
+              arap_consistent_sign=True, validate=False, pcg_pipelined=args.pcg_form, true_residual_rounds=2,
+              psd_dropped_pairs=True)
     solo, batch = [], []
     for w in ws:
         a = Harness(workload=w, **kw)
@@ -488,7 +493,7 @@ def run_batched(args) -> int:
         conv = sum(b.pcg_converged - a.pcg_converged for a, b in zip(t0s, t1s))
         brk = sum(b.pcg_breakdowns - a.pcg_breakdowns for a, b in zip(t0s, t1s))
         return {"pcg_runs": int(runs), "reached_1e-6": int(conv), "stopped_on_breakdown": int(brk),
-                "hit_300_iterations": int(runs - conv - brk)}
+                "not_reached": int(runs - conv - brk)}
 
     # batched: tune on frame 1 (state left as it was), then warm up and time
     bh = BatchHarness(batch, use_graph=args.graph)
@@ -563,7 +568,8 @@ def run_batched(args) -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "%d capsule-body instances, %d-%d nodes (%d total), 128^3-320^3 at 4-8 mm, "
-                               "640x480 / 320x240, 5 GN x PCG to 1e-6 relative residual (max 300)"
+                               "640x480 / 320x240, 5 GN x PCG to a 1e-6 relative true residual (max 300 "
+                               "iterations, up to 2 correction rounds), dropped data-term pairs on the diagonal"
                                % (count, min(nodes), max(nodes), sum(nodes)),
                    "instances": [w.name for w in ws], "graph": bool(args.graph),
                    "solver_policy": "consistent ARAP sign, per-node clamp, no whole-step rejection",

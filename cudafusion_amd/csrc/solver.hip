@@ -229,6 +229,7 @@ struct AsmArgs {
   const float *e_w;
   float lambda_smooth, lambda_damping;
   float arap_sign;  // +1 reference rotation rows, -1 consistent sign
+  int psd_pairs;    // 1: a data-term pair outside the pattern adds f J_k^T J_k to the diagonal block
   int precond_fp64;
   int Rmax;
   float *values, *b, *Minv;
@@ -545,7 +546,10 @@ __global__ __launch_bounds__(64) void assemble_rows_kernel(AsmArgs a) {
 #pragma unroll
         for (int l = 0; l < 4; ++l) {
           const int pos = __float_as_int(sq[3 + l]);
-          if (pos >= 0) acc[pos * 36 + ab] += fa * sq[7 + 6 * l + eb];
+          if (pos >= 0)
+            acc[pos * 36 + ab] += fa * sq[7 + 6 * l + eb];
+          else if (pos == -2 && a.psd_pairs)  // outside the pattern: f J_k^T J_k on the diagonal
+            acc[a.diag_rel[i] * 36 + ab] += fa * Jk[eb];
         }
       } else if (ra >= 0 && ra < 6) {
         bacc += -f * Jk[ra] * sq[1];
@@ -798,6 +802,23 @@ __device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_
         for (int q = 0; q < kMaxBlk; ++q) p = cr[q] == id ? q : p;  // columns are distinct
       }
       pos[l2] = p;
+    }
+    if (a.psd_pairs) {
+      // psd_dropped_pairs: each active slot outside the row's pattern adds
+      // f J_k^T J_k to the diagonal block, i.e. the row node's slot k enters
+      // the dense row (1 + m) times, m = the membership's dropped slots
+      int dropped = 0;
+#pragma unroll
+      for (int l2 = 0; l2 < 2; ++l2) {
+        const int id = __float_as_int(h ? R[4 + l2] : R[2 + l2]);
+        dropped += (act && id >= 0 && pos[l2] < 0) ? 1 : 0;
+      }
+      const int mtot = dropped + __shfl_xor(dropped, 32, 64);
+      if ((kslot >> 1) == h) {
+        const float sc = 1.f + (float)mtot;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) Js[kslot & 1][c] *= sc;
+      }
     }
 #pragma unroll
     for (int l2 = 0; l2 < 2; ++l2)
@@ -1101,6 +1122,19 @@ __device__ inline void wg_sum2_records(double &v0, double &v1, int G, double *re
   v1 = t1;
 }
 
+// State of the true-residual rounds of one PCG run (solver option
+// true_residual_rounds): after the PCG, r = b - A x is recomputed in FP64;
+// while ||r|| >= tol ||b|| a correction PCG solves A d = r to the absolute
+// tolerance tol ||b|| and x += d.  The run's outcome (converged, breakdown,
+// final residual) is the true residual's, recorded by tr_decide_kernel.
+struct TrState {
+  int32_t done;        // the run has its outcome (correction rounds are no-ops)
+  int32_t last_flags;  // stop flags of the run's last PCG
+  double tol_abs;      // tol * ||b|| of the run's system
+  float *r, *d;        // rhs and solution of the correction system (6N)
+  double *partial;     // per-workgroup partial sums of the residual kernel
+};
+
 struct PcgArgs {
   const int32_t *row_ptr, *lcol;
   const float *values, *b, *Minv;
@@ -1114,7 +1148,36 @@ struct PcgArgs {
   int rel;
   nrf_pcg_result *res;
   const int32_t *stop;
+  TrState *tr;  // true-residual rounds (null: off)
+  int round;    // 0: the run's PCG; > 0: a correction round (A d = r, to tr->tol_abs)
 };
+
+// A correction round of a run that already has its outcome does nothing.
+__device__ inline bool pcg_skip(const PcgArgs &a) { return a.tr && a.round > 0 && a.tr->done; }
+// The stopping tolerance on ||r|| (bb = ||rhs||^2).
+__device__ inline double pcg_tol(const PcgArgs &a, double bb) {
+  if (a.tr && a.round > 0) return a.tr->tol_abs;
+  return a.rel ? (double)a.tol * sqrt(bb) : (double)a.tol;
+}
+// The run's record (one lane): with true-residual rounds the outcome is left
+// to tr_decide_kernel.
+__device__ inline void pcg_tally(const PcgArgs &a, int iters, int flags, double rr, double bb) {
+  a.res->total_iterations += iters;
+  if (!a.tr) {
+    a.res->iterations = iters;
+    a.res->flags |= flags;
+    a.res->final_residual = sqrt(rr);
+    a.res->rhs_norm = sqrt(bb);
+    a.res->converged += (flags & 2) ? 1 : 0;
+    a.res->breakdowns += (flags & 1) ? 1 : 0;
+    a.res->runs += 1;
+    return;
+  }
+  a.res->iterations = a.round == 0 ? iters : a.res->iterations + iters;
+  a.res->flags |= flags & 8;
+  if (a.round == 0) a.tr->done = 0;
+  a.tr->last_flags = flags;
+}
 
 // The words live in global memory: the casts give global (not flat)
 // instructions also where the pointer came from memory (batch kernels).
@@ -1313,7 +1376,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
   __shared__ double red[kRedSize];
   __shared__ int s_bad;
   __shared__ unsigned s_base;
-  if (a.stop && *a.stop) return;
+  if ((a.stop && *a.stop) || pcg_skip(a)) return;
   const int g = blockIdx.x, t = threadIdx.x;
   const int n0 = a.rows[g], n1 = a.rows[g + 1];
   const int S = 6 * (n1 - n0);
@@ -1360,7 +1423,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
   double rz = lead ? (double)r * (double)z : 0.0, bb = lead ? (double)bv * (double)bv : 0.0;
   bool aborted = allreduce2(a, base + pt, (int)(pt & 1), rz, bb, red, &s_bad);
   double rr = bb;
-  const double tol = a.rel ? (double)a.tol * sqrt(bb) : (double)a.tol;
+  const double tol = pcg_tol(a, bb);
   int iters = 0, flags = 0;
   float betaf = 0.f;
   for (int it = 0; it < a.maxit && !aborted; ++it) {
@@ -1443,14 +1506,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
   if (aborted) flags |= 8;
   if (lead) a.x[s] = x;
   if (g == 0 && t == 0) {
-    a.res->iterations = iters;
-    a.res->total_iterations += iters;
-    a.res->flags |= flags;
-    a.res->final_residual = sqrt(rr);
-    a.res->rhs_norm = sqrt(bb);
-    a.res->converged += (flags & 2) ? 1 : 0;
-    a.res->breakdowns += (flags & 1) ? 1 : 0;
-    a.res->runs += 1;
+    pcg_tally(a, iters, flags, rr, bb);
   }
 }
 
@@ -1470,7 +1526,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
 template <int SUB, int FORM>
 __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, int &s_bad, unsigned &s_base) {
   constexpr int ROWS = kPcgThreads / SUB;
-  if (a.stop && *a.stop) return;
+  if ((a.stop && *a.stop) || pcg_skip(a)) return;
   const int t = threadIdx.x;
   const int n0 = a.rows[g], n1 = a.rows[g + 1];
   const int S = 6 * (n1 - n0);
@@ -1672,7 +1728,7 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
       iters = it;
       if (it == 0) {
         bb = rr;
-        const double tol = a.rel ? (double)a.tol * sqrt(bb) : (double)a.tol;
+        const double tol = pcg_tol(a, bb);
         tol2 = uniform_d(tol * tol);
       } else if (rr < tol2) {
         flags |= 2;
@@ -1705,14 +1761,7 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
     if (aborted) flags |= 8;
     if (lead) a.x[s] = x;
     if (g == 0 && t == 0) {
-      a.res->iterations = iters;
-      a.res->total_iterations += iters;
-      a.res->flags |= flags;
-      a.res->final_residual = sqrt(rr);
-      a.res->rhs_norm = sqrt(bb);
-      a.res->converged += (flags & 2) ? 1 : 0;
-      a.res->breakdowns += (flags & 1) ? 1 : 0;
-      a.res->runs += 1;
+      pcg_tally(a, iters, flags, rr, bb);
     }
     return;
   }
@@ -1735,7 +1784,7 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
     double gamma = v[0], pAp = v[1];
     const double bb = v[2];
     double rr = bb;
-    const double tol = a.rel ? (double)a.tol * sqrt(bb) : (double)a.tol, tol2 = tol * tol;
+    const double tol = pcg_tol(a, bb), tol2 = tol * tol;
     int iters = 0, flags = 0;
     float p = z, y = az;
     for (int it = 0; it < a.maxit && !aborted; ++it) {
@@ -1772,14 +1821,7 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
     if (aborted) flags |= 8;
     if (lead) a.x[s] = x;
     if (g == 0 && t == 0) {
-      a.res->iterations = iters;
-      a.res->total_iterations += iters;
-      a.res->flags |= flags;
-      a.res->final_residual = sqrt(rr);
-      a.res->rhs_norm = sqrt(bb);
-      a.res->converged += (flags & 2) ? 1 : 0;
-      a.res->breakdowns += (flags & 1) ? 1 : 0;
-      a.res->runs += 1;
+      pcg_tally(a, iters, flags, rr, bb);
     }
     return;
   }
@@ -1792,7 +1834,7 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
   float p = z, y = product(base + pt, a.zpub, 0.f);
   bool aborted = ar2_collect(a, base + pt, (int)(pt & 1), rz, bb, red, &s_bad);
   double rr = bb;
-  const double tol = a.rel ? (double)a.tol * sqrt(bb) : (double)a.tol;
+  const double tol = pcg_tol(a, bb);
   int iters = 0, flags = 0;
   for (int it = 0; it < a.maxit && !aborted; ++it) {
     ++pt;  // the p.Ap point
@@ -1830,14 +1872,7 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
   if (aborted) flags |= 8;
   if (lead) a.x[s] = x;
   if (g == 0 && t == 0) {
-    a.res->iterations = iters;
-    a.res->total_iterations += iters;
-    a.res->flags |= flags;
-    a.res->final_residual = sqrt(rr);
-    a.res->rhs_norm = sqrt(bb);
-    a.res->converged += (flags & 2) ? 1 : 0;
-    a.res->breakdowns += (flags & 1) ? 1 : 0;
-    a.res->runs += 1;
+    pcg_tally(a, iters, flags, rr, bb);
   }
 }
 
@@ -1855,13 +1890,10 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_pipe_kernel(PcgArgs a) {
 // and x are FP32); optionally stores r (rounded to FP32) and per-workgroup
 // partial sums of |r|^2 and |b|^2 for the true relative residual.
 constexpr int kResThreads = 256;
-__global__ __launch_bounds__(kResThreads) void residual_fp64_kernel(const int32_t *__restrict__ row_ptr,
-                                                                    const int32_t *__restrict__ col_idx,
-                                                                    const float *__restrict__ values,
-                                                                    const float *__restrict__ b,
-                                                                    const float *__restrict__ x, int N,
-                                                                    float *__restrict__ r_out,
-                                                                    double *__restrict__ partial) {
+__device__ inline void residual_fp64_body(const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col_idx,
+                                          const float *__restrict__ values, const float *__restrict__ b,
+                                          const float *__restrict__ x, int N, float *__restrict__ r_out,
+                                          double *__restrict__ partial) {
   __shared__ double red[2 * kResThreads / 64];
   const int q = blockIdx.x * kResThreads + threadIdx.x;
   double rr = 0.0, bb = 0.0;
@@ -1896,6 +1928,68 @@ __global__ __launch_bounds__(kResThreads) void residual_fp64_kernel(const int32_
     partial[2 * blockIdx.x + 1] = s1;
   }
 }
+__global__ __launch_bounds__(kResThreads) void residual_fp64_kernel(const int32_t *__restrict__ row_ptr,
+                                                                    const int32_t *__restrict__ col_idx,
+                                                                    const float *__restrict__ values,
+                                                                    const float *__restrict__ b,
+                                                                    const float *__restrict__ x, int N,
+                                                                    float *__restrict__ r_out,
+                                                                    double *__restrict__ partial) {
+  residual_fp64_body(row_ptr, col_idx, values, b, x, N, r_out, partial);
+}
+
+// ---- true-residual rounds (TrState): residual, decision, correction add.
+// A raised GN stop flag, or a run that has its outcome, makes each a no-op.
+__global__ __launch_bounds__(kResThreads) void tr_residual_kernel(const int32_t *__restrict__ row_ptr,
+                                                                  const int32_t *__restrict__ col_idx,
+                                                                  const float *__restrict__ values,
+                                                                  const float *__restrict__ b,
+                                                                  const float *__restrict__ x, int N,
+                                                                  TrState *tr, const int32_t *stop) {
+  if ((stop && *stop) || tr->done) return;
+  residual_fp64_body(row_ptr, col_idx, values, b, x, N, tr->r, tr->partial);
+}
+// Sums the residual kernel's partials in index order and decides: converged
+// when ||b - A x|| < tol ||b|| (or b = 0); the correction rounds then stop.
+// In the final check (or once converged) the run's outcome is recorded:
+// runs + 1, converged or (when the last PCG stopped on its breakdown test)
+// breakdown, and the true residual as final_residual.
+__device__ inline void tr_decide_body(TrState *tr, int nblocks, float tol, int final_check, nrf_pcg_result *res,
+                                      const int32_t *stop) {
+  if ((stop && *stop) || tr->done) return;
+  double rr = 0.0, bb = 0.0;
+  for (int k = 0; k < nblocks; ++k) {
+    rr += tr->partial[2 * k];
+    bb += tr->partial[2 * k + 1];
+  }
+  const double tl = (double)tol * sqrt(bb);
+  const bool conv = sqrt(rr) < tl || bb == 0.0;
+  tr->tol_abs = tl;
+  if (conv || final_check) {
+    tr->done = 1;
+    res->final_residual = sqrt(rr);
+    res->rhs_norm = sqrt(bb);
+    res->runs += 1;
+    if (conv) {
+      res->converged += 1;
+      res->flags |= 2;
+    } else if (tr->last_flags & 1) {
+      res->breakdowns += 1;
+      res->flags |= 1;
+    }
+  }
+}
+__global__ void tr_decide_kernel(TrState *tr, int nblocks, float tol, int final_check, nrf_pcg_result *res,
+                                 const int32_t *stop) {
+  if (threadIdx.x == 0) tr_decide_body(tr, nblocks, tol, final_check, res, stop);
+}
+__global__ __launch_bounds__(256) void tr_axpy_kernel(float *__restrict__ x, int n, const TrState *tr,
+                                                      const int32_t *stop) {
+  if ((stop && *stop) || tr->done) return;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) x[i] += tr->d[i];
+}
+
 // One iterative-refinement step whose correction solve is the block-Jacobi
 // inverse: x_out = x + M^-1 (b - A x), the residual (products and sums) and
 // the correction in FP64.  One workgroup of 256 lanes per kRefNodes nodes:
@@ -2263,7 +2357,8 @@ __global__ void pcg_prep_batch_kernel(const InstDev *__restrict__ inst, int coun
 template <int SUB, int FORM>
 __global__ __launch_bounds__(kPcgThreads) void pcg_pipe_batch_kernel(const InstDev *__restrict__ inst,
                                                                      const int32_t *__restrict__ wg_inst,
-                                                                     const int32_t *__restrict__ wg_local) {
+                                                                     const int32_t *__restrict__ wg_local,
+                                                                     int round) {
   extern __shared__ float lds[];
   __shared__ double red[kRedSize];
   __shared__ int s_bad;
@@ -2273,8 +2368,35 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_pipe_batch_kernel(const InstD
   // them) are uniform to the compiler, as kernel arguments are
   const int k = __builtin_amdgcn_readfirstlane(wg_inst[blockIdx.x]);
   const int g = __builtin_amdgcn_readfirstlane(wg_local[blockIdx.x]);
-  const PcgArgs a = inst[k].pcg;
+  PcgArgs a = inst[k].pcg;
+  if (round > 0 && a.tr) {  // a correction round of the true-residual rounds: A d = r
+    a.round = round;
+    a.b = a.tr->r;
+    a.x = a.tr->d;
+  }
   pcg_pipe_body<SUB, FORM>(a, g, lds, red, s_bad, s_base);
+}
+// True-residual rounds of all instances (TrState of each): residual,
+// decision and correction add, one launch each over the batch.
+__global__ __launch_bounds__(kResThreads) void tr_residual_batch_kernel(const InstDev *__restrict__ inst) {
+  const InstDev &d = inst[blockIdx.y];
+  if (!d.pcg.tr || (d.pcg.stop && *d.pcg.stop) || d.pcg.tr->done) return;
+  if ((int)blockIdx.x * kResThreads >= 6 * d.as.N) return;  // whole workgroup
+  residual_fp64_body(d.as.row_ptr, d.as.col_idx, d.as.values, d.as.b, d.pcg.x, d.as.N, d.pcg.tr->r,
+                     d.pcg.tr->partial);
+}
+__global__ void tr_decide_batch_kernel(const InstDev *__restrict__ inst, int count, int final_check) {
+  const int k = threadIdx.x;
+  if (k >= count || !inst[k].pcg.tr) return;
+  const InstDev &d = inst[k];
+  tr_decide_body(d.pcg.tr, (6 * d.as.N + kResThreads - 1) / kResThreads, d.pcg.tol, final_check, d.pcg.res,
+                 d.pcg.stop);
+}
+__global__ __launch_bounds__(256) void tr_axpy_batch_kernel(const InstDev *__restrict__ inst) {
+  const InstDev &d = inst[blockIdx.y];
+  if (!d.pcg.tr || (d.pcg.stop && *d.pcg.stop) || d.pcg.tr->done) return;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < 6 * d.as.N) d.pcg.x[i] += d.pcg.tr->d[i];
 }
 
 // The persistent PCG kernels by lanes per row (1, 2, 4) and form (0
@@ -2445,6 +2567,8 @@ struct nrf_solver {
   size_t c_rres = 0, c_dref = 0;
   double *rpart = nullptr, *rnorm = nullptr;  // true residual partials and norms
   size_t c_rpart = 0, c_rnorm = 0;
+  TrState *tr = nullptr;  // true-residual rounds (over rres, dref, rpart)
+  size_t c_tr = 0;
   nrf_pcg_result *d_res = nullptr;
   size_t c_res = 0;
   nrf_solver_totals *d_tot = nullptr;
@@ -2618,6 +2742,7 @@ static AsmArgs solver_asm_args(const nrf_solver *s, const nrf_corr *corrs, const
   a.e_w = s->e_w;
   a.lambda_smooth = s->params.lambda_smooth;
   a.arap_sign = s->params.arap_consistent_sign ? -1.0f : 1.0f;
+  a.psd_pairs = s->params.psd_dropped_pairs;
   a.lambda_damping = s->params.lambda_damping;
   a.precond_fp64 = s->params.precond_fp64;
   a.Rmax = s->P.Rmax;
@@ -2713,7 +2838,8 @@ static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, int n, const fl
 }
 
 static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxit, bool record, hipStream_t st,
-                      bool opened = false);
+                      bool opened = false, int round = 0);
+static int solver_tr_rounds(nrf_solver *s, float *x, int N, hipStream_t st);
 
 // opened: the launch was opened by the assembly (solver_assemble's open_pcg)
 static int solver_pcg(nrf_solver *s, float *x, int N, hipStream_t st, bool opened = false) {
@@ -2729,7 +2855,9 @@ static int solver_pcg(nrf_solver *s, float *x, int N, hipStream_t st, bool opene
     return NRF_OK;
   }
   rc = pcg_launch(s, s->b, x, N, s->params.pcg_iterations, true, st, opened);
-  if (rc || s->params.refine_iterations <= 0) return rc;
+  if (rc) return rc;
+  if (s->params.true_residual_rounds > 0) return solver_tr_rounds(s, x, N, st);
+  if (s->params.refine_iterations <= 0) return rc;
   // one iterative-refinement step: r = b - A x accumulated in FP64, A d = r
   // by refine_iterations PCG iterations, x += d
   const int n6 = 6 * N, nblk = (n6 + kResThreads - 1) / kResThreads;
@@ -2769,6 +2897,8 @@ static PcgArgs solver_pcg_args(const nrf_solver *s, const float *rhs, float *x, 
   a.rel = s->params.relative_tolerance;
   a.res = s->d_res;
   a.stop = s->stop;
+  a.tr = s->params.true_residual_rounds > 0 ? s->tr : nullptr;
+  a.round = 0;
   return a;
 }
 
@@ -2776,19 +2906,44 @@ static PcgArgs solver_pcg_args(const nrf_solver *s, const float *rhs, float *x, 
 // graph: its grid is sized to be co-resident on the device (at most the CU
 // count, or the occupancy bound for batches), and every wait in it is bounded.
 static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxit, bool record, hipStream_t st,
-                      bool opened) {
+                      bool opened, int round) {
   if (!opened) {
     pcg_prep_kernel<<<1, 64, 0, st>>>(s->ctl);
     NRF_LAUNCH_CHECK();
   }
   if (record) sink_record(s, st);
   PcgArgs a = solver_pcg_args(s, rhs, x, N, maxit);
+  a.round = round;
   void *args[] = {&a};
   const bool pipe = s->params.pcg_pipelined != 0;
   if (!pipe && s->pcg_sub != kPcgSub) return NRF_ERR_INVALID_ARGUMENT;  // textbook form: 4 lanes per row only
   const void *fn = pipe ? pcg_pipe_fn(s->pcg_sub, s->params.pcg_pipelined - 1)
                         : reinterpret_cast<const void *>(pcg_kernel);
   NRF_HIP_CHECK(hipLaunchKernel(fn, dim3(s->G), dim3(kPcgThreads), args, (unsigned)s->pcg_lds, st));
+  return NRF_OK;
+}
+
+// True-residual rounds after a PCG run that left its solution in x: up to
+// true_residual_rounds times, r = b - A x in FP64 and, while ||r|| >=
+// tol ||b||, a correction PCG (A d = r to the absolute tolerance tol ||b||,
+// the same iteration budget) and x += d; then the final check records the
+// run's outcome from the true residual.  Every kernel after the decision
+// that the run converged is a no-op (device flag), so the launch sequence
+// is fixed (graph-safe).
+static int solver_tr_rounds(nrf_solver *s, float *x, int N, hipStream_t st) {
+  const int n6 = 6 * N, nblk = (n6 + kResThreads - 1) / kResThreads;
+  const int R = s->params.true_residual_rounds;
+  for (int k = 1; k <= R + 1; ++k) {
+    tr_residual_kernel<<<nblk, kResThreads, 0, st>>>(s->row_ptr, s->col_idx, s->values, s->b, x, N, s->tr, s->stop);
+    NRF_LAUNCH_CHECK();
+    tr_decide_kernel<<<1, 64, 0, st>>>(s->tr, nblk, s->params.pcg_tolerance, k == R + 1 ? 1 : 0, s->d_res, s->stop);
+    NRF_LAUNCH_CHECK();
+    if (k == R + 1) break;
+    int rc = pcg_launch(s, s->rres, s->dref, N, s->params.pcg_iterations, false, st, false, k);
+    if (rc) return rc;
+    tr_axpy_kernel<<<(n6 + 255) / 256, 256, 0, st>>>(x, n6, s->tr, s->stop);
+    NRF_LAUNCH_CHECK();
+  }
   return NRF_OK;
 }
 
@@ -2820,7 +2975,11 @@ static bool params_ok(const nrf_solver_params *p) {
   return p->pcg_iterations >= 0 && 1 + 2 * (long)p->pcg_iterations <= kPcgMaxPoints &&
          p->refine_iterations >= 0 && 1 + 2 * (long)p->refine_iterations <= kPcgMaxPoints && p->jtj_precision >= 0 &&
          p->jtj_precision <= 4 && p->pcg_pipelined >= 0 && p->pcg_pipelined <= 3 &&
-         (p->refine_jacobi == 0 || (p->refine_jacobi == 1 && p->refine_iterations == 0));
+         (p->refine_jacobi == 0 || (p->refine_jacobi == 1 && p->refine_iterations == 0)) &&
+         p->true_residual_rounds >= 0 && p->true_residual_rounds <= 8 &&
+         (p->true_residual_rounds == 0 ||
+          (p->relative_tolerance == 1 && p->refine_iterations == 0 && p->refine_jacobi == 0)) &&
+         (p->psd_dropped_pairs == 0 || p->psd_dropped_pairs == 1);
 }
 
 extern "C" nrf_solver *nrf_solver_create(const nrf_solver_params *params, int max_nodes) {
@@ -2845,7 +3004,14 @@ extern "C" nrf_solver *nrf_solver_create(const nrf_solver_params *params, int ma
             dev_alloc(s->zpub, n6, s->c_zpub) == 0 && dev_alloc(s->P0, n6, s->c_P0) == 0 &&
             dev_alloc(s->P1, n6, s->c_P1) == 0 && dev_alloc(s->recs, (size_t)kPcgMaxWg, s->c_recs) == 0 &&
             dev_alloc(s->ctl, 1, s->c_ctl) == 0 && dev_alloc(s->stop, 1, s->c_stop) == 0 &&
-            dev_alloc(s->d_res, 1, s->c_res) == 0 && dev_alloc(s->d_tot, 1, s->c_tot) == 0;
+            dev_alloc(s->d_res, 1, s->c_res) == 0 && dev_alloc(s->d_tot, 1, s->c_tot) == 0 &&
+            dev_alloc(s->rres, n6, s->c_rres) == 0 && dev_alloc(s->dref, n6, s->c_dref) == 0 &&
+            dev_alloc(s->rpart, 2 * ((n6 + kResThreads - 1) / kResThreads), s->c_rpart) == 0 &&
+            dev_alloc(s->tr, 1, s->c_tr) == 0;
+  if (ok) {  // the rounds' buffers are sized for max_nodes: their addresses never change
+    TrState h{1, 0, 0.0, s->rres, s->dref, s->rpart};
+    ok = hipMemcpy(s->tr, &h, sizeof(h), hipMemcpyHostToDevice) == hipSuccess;
+  }
   // published words start with tag 0, which no synchronisation point uses
   if (ok) ok = hipMemset(s->zpub, 0, sizeof(ll_word) * n6) == hipSuccess &&
                hipMemset(s->P0, 0, sizeof(ll_word) * n6) == hipSuccess &&
@@ -2868,7 +3034,8 @@ extern "C" void nrf_solver_destroy(nrf_solver *s) {
                   s->rows, s->e_w, s->values, s->b, s->Minv, s->zpub, s->P0, s->P1, s->recs, s->ctl, s->lcol,
                   s->halo_off, s->halo,
                   s->m_cnt, s->m_off, s->m_cur, s->m_keys, s->m_keys2, s->stop, s->d_res, s->d_tot,
-                  s->crec, s->arec, s->rres, s->dref, s->rpart, s->rnorm, s->chunk_off, s->chunk_row, s->part};
+                  s->crec, s->arec, s->rres, s->dref, s->rpart, s->rnorm, s->chunk_off, s->chunk_row, s->part,
+                  s->tr};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
@@ -3037,7 +3204,7 @@ extern "C" int nrf_solver_get_totals(nrf_solver *s, nrf_solver_totals *out, void
 
 struct nrf::SolverBatch {
   std::vector<SolverBatchInstance> in;
-  int count = 0, sub = 4, prec = 0, ns = 1, gn_iters = 1, any_validate = 0;
+  int count = 0, sub = 4, prec = 0, ns = 1, gn_iters = 1, any_validate = 0, tr_rounds = 0;
   int form = 0;  // the PCG form of every instance (pcg_pipelined - 1)
   int max_N = 0, rows = 0, wgs = 0;
   size_t asm_lds = 0, pcg_lds = 0;
@@ -3117,13 +3284,14 @@ int solver_batch_create(const SolverBatchInstance *inst, int count, int lanes_pe
     if (!sp.pcg_pipelined || sp.refine_iterations != 0 || sp.refine_jacobi != 0 || !s->asm_mfma || ns > 8 ||
         asm_lds_bytes(s->P.Rmax) > 64 * 1024 ||
         (k > 0 && (sp.jtj_precision != b->prec || std::max(1, q.gn.gn_iterations) != b->gn_iters ||
-                   sp.pcg_pipelined - 1 != b->form))) {
+                   sp.pcg_pipelined - 1 != b->form || sp.true_residual_rounds != b->tr_rounds))) {
       rc = NRF_ERR_INVALID_ARGUMENT;
       break;
     }
     if (k == 0) {
       b->form = sp.pcg_pipelined - 1;
       b->prec = sp.jtj_precision;
+      b->tr_rounds = sp.true_residual_rounds;
       b->gn_iters = std::max(1, q.gn.gn_iterations);
     }
     b->ns = std::max(b->ns, ns);
@@ -3288,16 +3456,36 @@ static int batch_assemble(SolverBatch *b, const BatchCorrs &fr, hipStream_t st) 
   return NRF_OK;
 }
 
-int solver_batch_pcg(SolverBatch *b, hipStream_t st) {
+static int batch_pcg_launch(SolverBatch *b, int round, hipStream_t st) {
   pcg_prep_batch_kernel<<<1, 64, 0, st>>>(b->d_inst, b->count);
   NRF_LAUNCH_CHECK();
   const void *fn = pcg_batch_fn(b->sub, b->form);
   for (const auto &g : b->groups) {
     const InstDev *inst = b->d_inst;
     const int32_t *wi = b->wg_inst + g.first, *wl = b->wg_local + g.first;
-    void *args[] = {&inst, &wi, &wl};
+    void *args[] = {&inst, &wi, &wl, &round};
     const dim3 grid(g.second - g.first);
     NRF_HIP_CHECK(hipLaunchKernel(fn, grid, dim3(kPcgThreads), args, (unsigned)b->pcg_lds, st));
+  }
+  return NRF_OK;
+}
+
+// The PCG of every instance, then (true_residual_rounds > 0) the rounds of
+// solver_tr_rounds over all instances at once.
+int solver_batch_pcg(SolverBatch *b, hipStream_t st) {
+  int rc = batch_pcg_launch(b, 0, st);
+  if (rc || b->tr_rounds <= 0) return rc;
+  const int maxblk = (6 * b->max_N + kResThreads - 1) / kResThreads;
+  const dim3 gR(maxblk, b->count), gA((6 * b->max_N + 255) / 256, b->count);
+  for (int k = 1; k <= b->tr_rounds + 1; ++k) {
+    tr_residual_batch_kernel<<<gR, kResThreads, 0, st>>>(b->d_inst);
+    NRF_LAUNCH_CHECK();
+    tr_decide_batch_kernel<<<1, 64, 0, st>>>(b->d_inst, b->count, k == b->tr_rounds + 1 ? 1 : 0);
+    NRF_LAUNCH_CHECK();
+    if (k == b->tr_rounds + 1) break;
+    if ((rc = batch_pcg_launch(b, k, st))) return rc;
+    tr_axpy_batch_kernel<<<gA, 256, 0, st>>>(b->d_inst);
+    NRF_LAUNCH_CHECK();
   }
   return NRF_OK;
 }

@@ -240,13 +240,16 @@ class GaussNewtonSolver:
         jtj_precision: int = 0  # 0 FP32, 1 BF16, 2 FP16, 3 / 4 split BF16 / FP16 JtJ blocks (RHS stays FP32)
         refine_iterations: int = 0  # > 0: one FP64-residual iterative-refinement step per solve
         refine_jacobi: bool = False  # one refinement step with the block-Jacobi inverse as its correction solve
+        true_residual_rounds: int = 0  # > 0: convergence by the FP64 true residual, correction PCG rounds
+        psd_dropped_pairs: bool = False  # opt-in: data-term pairs outside the pattern kept PSD on the diagonal
 
         def to_c(self) -> L.SolverParams:
             return L.solver_params(self.gn_iterations, self.pcg_iterations, self.pcg_tolerance, self.lambda_smooth,
                                    self.lambda_rot, self.lambda_damping, int(self.debug),
                                    int(self.relative_tolerance), int(self.precond_fp64),
                                    int(self.arap_consistent_sign), int(self.pcg_pipelined),
-                                   int(self.jtj_precision), int(self.refine_iterations), int(self.refine_jacobi))
+                                   int(self.jtj_precision), int(self.refine_iterations), int(self.refine_jacobi),
+                                   int(self.true_residual_rounds), int(self.psd_dropped_pairs))
 
     def __init__(self, p: "GaussNewtonSolver.Params", max_nodes: int):
         self.lib = L.load()

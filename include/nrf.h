@@ -139,6 +139,22 @@ typedef struct nrf_solver_params {
                                   whose correction solve is the block-Jacobi inverse:
                                   x += M^-1 (b - A x), residual and correction in
                                   FP64 (one kernel); refine_iterations must be 0 */
+  int32_t true_residual_rounds; /* > 0 (at most 8, needs relative_tolerance = 1 and
+                                  no refinement step): convergence judged by the
+                                  true residual.  After the PCG, r = b - A x in FP64;
+                                  while ||r|| >= tol ||b||, up to this many times, a
+                                  correction PCG solves A d = r to tol ||b|| (same
+                                  iteration budget) and x += d.  The result's
+                                  converged / breakdowns / final_residual are then
+                                  those of the true residual */
+  int32_t psd_dropped_pairs;   /* 1: a data-term pair of skinning slots whose nodes
+                                  are not adjacent in the pattern adds f J_k^T J_k to
+                                  the diagonal block of slot k instead of being
+                                  dropped (kernels/solver_kernels.cu:123-125 drops it,
+                                  which can leave the system indefinite); the system
+                                  is then the full J^T J plus the sum over dropped
+                                  pairs of (J_k - J_l)^T (J_k - J_l): positive
+                                  semidefinite before the damping */
 } nrf_solver_params;
 
 /* Caller-side GN loop settings (include/fusion_app.h:72-119 names). */

@@ -489,7 +489,7 @@ template <typename R> R huber(R r, R delta) {
 template <typename R>
 void assemble(const nrf_corr *corrs, int n, const float *canonical, const nrf_node *nodes, const DQ<R> *T, int N,
               R lam_s, R lam_d, const int32_t *rp, const int32_t *ci, R *A, R *b, R *Minv, int precond_fp64,
-              int consistent_sign) {
+              int consistent_sign, int psd_pairs = 0) {
   const int nb = rp[N];
   std::fill(A, A + (long)nb * 36, (R)0);
   std::fill(b, b + (long)N * 6, (R)0);
@@ -522,7 +522,16 @@ void assemble(const nrf_corr *corrs, int n, const float *canonical, const nrf_no
       for (int kj = 0; kj < 4; ++kj) {
         if (!ok[kj]) continue;
         int bp = find_block(cr.node_ids[ki], cr.node_ids[kj], rp, ci);
-        if (bp < 0) continue;
+        if (bp < 0) {
+          // Q2: the pair is dropped; psd_pairs (solver option, no reference
+          // counterpart) adds f J_ki^T J_ki to the diagonal block instead
+          if (!psd_pairs) continue;
+          bp = find_block(cr.node_ids[ki], cr.node_ids[ki], rp, ci);
+          R *blk = A + (long)bp * 36;
+          for (int a = 0; a < 6; ++a)
+            for (int bb = 0; bb < 6; ++bb) blk[a * 6 + bb] += fw * J[ki][a] * J[ki][bb];
+          continue;
+        }
         R *blk = A + (long)bp * 36;
         for (int a = 0; a < 6; ++a)
           for (int bb = 0; bb < 6; ++bb) blk[a * 6 + bb] += fw * J[ki][a] * J[kj][bb];
@@ -782,7 +791,7 @@ int solve_gn(const nrf_corr *corrs, const float *canonical, int n, const nrf_nod
   int tot = 0, done = 0, li = 0;
   for (int it = 0; it < std::max(1, g.gn_iterations); ++it) {
     assemble(corrs, n, canonical, nodes, T, N, (R)sp.lambda_smooth, (R)sp.lambda_damping, rp.data(), ci.data(),
-             A.data(), b.data(), M.data(), sp.precond_fp64, sp.arap_consistent_sign);
+             A.data(), b.data(), M.data(), sp.precond_fp64, sp.arap_consistent_sign, sp.psd_dropped_pairs);
     int iters = 0, flags = 0;
     pcg(rp.data(), ci.data(), A.data(), b.data(), M.data(), N, sp.pcg_iterations, (R)sp.pcg_tolerance,
         sp.relative_tolerance, x.data(), &iters, &flags, nullptr);
@@ -1015,23 +1024,23 @@ int orc_pattern(const nrf_node *nodes, int N, int32_t *row_ptr, int32_t *col_idx
 
 int orc_assemble_f32(const nrf_corr *corrs, int n, const float *canonical, const nrf_node *nodes,
                      const nrf_dualquat *Tq, int N, float lam_s, float lam_d, int precond_fp64, float *A, float *b,
-                     float *M, int consistent_sign) {
+                     float *M, int consistent_sign, int psd_pairs) {
   std::vector<int32_t> rp, ci;
   pattern(nodes, N, rp, ci);
   auto T = load_T<float>(Tq, N);
   assemble(corrs, n, canonical, nodes, T.data(), N, lam_s, lam_d, rp.data(), ci.data(), A, b, M, precond_fp64,
-           consistent_sign);
+           consistent_sign, psd_pairs);
   return rp[N];
 }
 int orc_assemble_f64(const nrf_corr *corrs, int n, const float *canonical, const nrf_node *nodes,
                      const double *T8, int N, double lam_s, double lam_d, double *A, double *b, double *M,
-                     int consistent_sign) {
+                     int consistent_sign, int psd_pairs) {
   std::vector<int32_t> rp, ci;
   pattern(nodes, N, rp, ci);
   std::vector<DQ<double>> T(N);
   for (int i = 0; i < N; ++i) T[i] = load_dq8(T8 + 8 * i);
   assemble(corrs, n, canonical, nodes, T.data(), N, lam_s, lam_d, rp.data(), ci.data(), A, b, M, 1,
-           consistent_sign);
+           consistent_sign, psd_pairs);
   return rp[N];
 }
 

@@ -45,8 +45,8 @@ _SIGS = {
     "orc_knn_f32": (None, [P, P, P, I, P, P]),
     "orc_knn_f64": (None, [P, P, P, I, P, P]),
     "orc_pattern": (I, [P, I, P, P, I]),
-    "orc_assemble_f32": (I, [P, I, P, P, P, I, F, F, I, P, P, P, I]),
-    "orc_assemble_f64": (I, [P, I, P, P, P, I, D, D, P, P, P, I]),
+    "orc_assemble_f32": (I, [P, I, P, P, P, I, F, F, I, P, P, P, I, I]),
+    "orc_assemble_f64": (I, [P, I, P, P, P, I, D, D, P, P, P, I, I]),
     "orc_pcg_f32": (D, [P, P, P, P, P, I, I, F, I, P, P, P, P]),
     "orc_pcg_f64": (D, [P, P, P, P, P, I, I, D, I, P, P, P, P]),
     "orc_spmv_f64": (None, [P, P, P, P, P, I]),
@@ -211,7 +211,7 @@ def pattern(nodes):
     return rp, ci[:nb]
 
 
-def assemble_f32(corrs, nodes, T, lam_s, lam_d, canonical=None, precond_fp64=0, consistent_sign=0):
+def assemble_f32(corrs, nodes, T, lam_s, lam_d, canonical=None, precond_fp64=0, consistent_sign=0, psd_pairs=0):
     N = len(nodes)
     rp, ci = pattern(nodes)
     nb = len(ci)
@@ -219,11 +219,11 @@ def assemble_f32(corrs, nodes, T, lam_s, lam_d, canonical=None, precond_fp64=0, 
     b = np.zeros(N * 6, np.float32)
     M = np.zeros(N * 36, np.float32)
     load().orc_assemble_f32(ptr(corrs), len(corrs), ptr(canonical), ptr(nodes), ptr(T), N, lam_s, lam_d,
-                            precond_fp64, ptr(A), ptr(b), ptr(M), consistent_sign)
+                            precond_fp64, ptr(A), ptr(b), ptr(M), consistent_sign, psd_pairs)
     return rp, ci, A.reshape(nb, 6, 6), b, M.reshape(N, 6, 6)
 
 
-def assemble_f64(corrs, nodes, T64, lam_s, lam_d, canonical=None, consistent_sign=0):
+def assemble_f64(corrs, nodes, T64, lam_s, lam_d, canonical=None, consistent_sign=0, psd_pairs=0):
     N = len(nodes)
     rp, ci = pattern(nodes)
     nb = len(ci)
@@ -231,7 +231,7 @@ def assemble_f64(corrs, nodes, T64, lam_s, lam_d, canonical=None, consistent_sig
     b = np.zeros(N * 6)
     M = np.zeros(N * 36)
     load().orc_assemble_f64(ptr(corrs), len(corrs), ptr(canonical), ptr(nodes), ptr(T64), N, lam_s, lam_d, ptr(A),
-                            ptr(b), ptr(M), consistent_sign)
+                            ptr(b), ptr(M), consistent_sign, psd_pairs)
     return rp, ci, A.reshape(nb, 6, 6), b, M.reshape(N, 6, 6)
 
 

@@ -91,7 +91,8 @@ def test_tracking_sequence_consistent_policy(gpu, orc, pcg_form):
     """Eight frames of the sequence through the pipeline with the benchmark's
     solver policy (consistent ARAP sign, per-node clamp, no whole-step
     rejection), with the overlapped, the single-reduction and the pipelined
-    (Ghysels-Vanroose) PCG forms.
+    (Ghysels-Vanroose) PCG forms, the pipelined form at the benchmark's PCG
+    layout (4 lanes per row, 16 nodes per workgroup).
     Per frame: the node transforms equal the FP64 oracle's solve from the
     same starting transforms (1e-4 rad / 1e-5 m), every GN step is
     applied, and the point-to-plane RMS of the correspondences after the solve
@@ -99,8 +100,11 @@ def test_tracking_sequence_consistent_policy(gpu, orc, pcg_form):
     import torch
 
     from cudafusion_amd.harness import Harness, transform_errors
+    from cudafusion_amd import _lib as L
     h = Harness("body", frames=9, arap_consistent_sign=True, validate=False, pcg_pipelined=pcg_form)
     h.setup()
+    if pcg_form == 3:
+        L.check(h.lib.nrf_solver_set_pcg_layout(h.lib.nrf_pipeline_solver(h.pipe), 4, 16), "pcg_layout")
     N = h.N
 
     def rms(corrs, can, T):

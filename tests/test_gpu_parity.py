@@ -542,3 +542,76 @@ def test_pipeline_step_odd_volume(gpu, orc):
     assert h.triangle_count() == len(rv) // 3 and len(rv) > 300
     np.testing.assert_allclose(v, rv, atol=1e-6)
     h.close()
+
+
+@pytest.mark.parametrize("pcg", [0, 2, 3], ids=["textbook", "single_reduction", "ghysels_vanroose"])
+def test_true_residual_rounds_reach_tolerance(sphere, orc, solver_inputs, pcg):
+    """true_residual_rounds: after the PCG, r = b - A x in FP64 and, while
+    ||r|| >= 1e-6 ||b||, correction PCG rounds on A d = r.  The solve then
+    reports convergence by the true residual: one run, converged, its
+    final_residual equal to nrf_solver_true_residual of the returned x and
+    below 1e-6 ||b||; and x matches the FP64 oracle's dense solve of the same
+    FP32 system to the accuracy the residual allows."""
+    from cudafusion_amd.fusion import copy_device_to_numpy
+    N = len(sphere["nodes"])
+    s, dcorr, corrs, T, T64, dx, res = _sphere_system(sphere, orc, solver_inputs)
+    wf = solver_inputs["wf"]
+    p = s.params
+    p.pcg_iterations, p.pcg_tolerance, p.relative_tolerance, p.pcg_pipelined = 300, 1e-6, True, pcg
+    p.true_residual_rounds = 2
+    s.set_params(p)
+    res = s.solve(dcorr, len(corrs), wf.device_nodes(), wf.device_transforms(), N, dx)
+    r_gpu, b_gpu = s.true_residual(dx)
+    print(f"[true residual] form {pcg}: runs {res.runs} converged {res.converged} breakdowns {res.breakdowns}, "
+          f"iterations {res.total_iterations}, reported {res.final_residual / res.rhs_norm:.2e}, "
+          f"true {r_gpu / b_gpu:.2e}", flush=True)
+    assert res.runs == 1 and res.converged == 1 and res.breakdowns == 0 and (res.flags & 2)
+    assert r_gpu < 1e-6 * b_gpu
+    assert abs(res.final_residual - r_gpu) <= 1e-9 * b_gpu and abs(res.rhs_norm - b_gpu) <= 1e-12 * b_gpu
+    rp_p, ci_p, va_p, nb, rhs_p, _ = s.system_matrix()
+    rp = copy_device_to_numpy(rp_p, N + 1, np.int32)
+    ci = copy_device_to_numpy(ci_p, nb, np.int32)
+    A = copy_device_to_numpy(va_p, nb * 36, np.float32).reshape(nb, 6, 6).astype(np.float64)
+    b = copy_device_to_numpy(rhs_p, 6 * N, np.float32).astype(np.float64)
+    D = np.zeros((6 * N, 6 * N))
+    for i in range(N):
+        for q in range(rp[i], rp[i + 1]):
+            D[6 * i:6 * i + 6, 6 * ci[q]:6 * ci[q] + 6] = A[q]
+    x_star = np.linalg.solve(D, b)
+    x = dx.cpu().numpy().astype(np.float64)
+    # ||x - x*|| <= ||A^-1|| ||b - A x||, ||A^-1|| = 1 / lambda_min (A SPD)
+    bound = r_gpu / np.linalg.eigvalsh(0.5 * (D + D.T)).min()
+    assert np.linalg.norm(x - x_star) <= 1.01 * bound + 1e-7 * np.linalg.norm(x_star)
+    p.true_residual_rounds = 0
+    s.set_params(p)
+
+
+@pytest.mark.parametrize("prec", [0, 3], ids=["fp32", "bf16x3"])
+@pytest.mark.parametrize("form", ["mfma", "scalar"])
+def test_psd_dropped_pairs_system(sphere, orc, solver_inputs, form, prec, monkeypatch):
+    """psd_dropped_pairs: the assembled system (data-term pairs outside the
+    pattern on the diagonal) equals the FP32 oracle's with the same option,
+    in the matrix-core and the scalar form, and in the split-BF16 form at
+    16 significant bits."""
+    from cudafusion_amd.fusion import copy_device_to_numpy
+    if form == "scalar" and prec:
+        pytest.skip("the reduced-precision products exist in the matrix-core form only")
+    monkeypatch.setenv("NRF_ASM_MFMA", "1" if form == "mfma" else "0")
+    w = sphere["w"]
+    nodes = sphere["nodes"]
+    N = len(nodes)
+    s, dcorr, corrs, T, T64, dx, res = _sphere_system(sphere, orc, solver_inputs, psd_dropped_pairs=True,
+                                                      arap_consistent_sign=True, jtj_precision=prec)
+    rp_p, ci_p, va_p, nb, rhs_p, _ = s.system_matrix()
+    rp, ci, A, b, M = orc.assemble_f32(corrs, nodes, T, w.lambda_smooth, w.lambda_damping, consistent_sign=1,
+                                       psd_pairs=1)
+    _, _, Aq, _, _ = orc.assemble_f32(corrs, nodes, T, w.lambda_smooth, w.lambda_damping, consistent_sign=1)
+    gA = copy_device_to_numpy(va_p, nb * 36, np.float32).reshape(nb, 6, 6)
+    gb = copy_device_to_numpy(rhs_p, 6 * N, np.float32)
+    scale = np.abs(A).max()
+    diff = np.abs(A - Aq).max()
+    print(f"[psd pairs {form} {prec}] largest compensation {diff / scale:.2e} of the largest entry, device vs "
+          f"oracle {np.abs(gA - A).max() / scale:.2e}", flush=True)
+    assert diff > 1e-4 * scale  # the option changes this system
+    np.testing.assert_allclose(gA, A, atol=(2e-6 if prec == 0 else 3e-5) * scale)
+    np.testing.assert_allclose(gb, b, atol=2e-6 * np.abs(b).max())

@@ -441,3 +441,78 @@ def test_refinement_step_block_jacobi_form(orc):
     # r = b - A x cancels: its rounding depends on the order of the sums
     np.testing.assert_allclose(out[1] - x, z, rtol=1e-5, atol=1e-6 * np.abs(z).max())
     assert np.linalg.norm(b - dense @ out[1]) < np.linalg.norm(r)
+
+
+def _data_jacobians(orc, nodes, corrs):
+    """Per valid correspondence: (fw, [(node, J_k)]) with J_k by central
+    differences of n.(warp(src) - dst) about the identity."""
+    N = len(nodes)
+    P = nodes["pos"].astype(np.float64)
+    I8 = np.array([0, 0, 0, 1, 0, 0, 0, 0.0])
+    eps = 1e-6
+    out = []
+    for c in corrs:
+        if not c["valid"]:
+            continue
+        src, dst, n = c["src"].astype(np.float64), c["dst"].astype(np.float64), c["normal"].astype(np.float64)
+        r = float(n @ (src - dst))
+        fw = max(float(c["weight"]), 0.0) * _huber(np.float32(r), 0.2)
+        if fw <= 1e-8:
+            continue
+        Js = []
+        for k in range(4):
+            nid = int(c["node_ids"][k])
+            if nid < 0 or nid >= N or c["node_ws"][k] <= 1e-8:
+                continue
+            g = np.zeros(6)
+            for a in range(6):
+                d = np.zeros(6)
+                d[a] = eps
+                vals = [float(n @ (_xf_c(orc, _twist(orc, sg * d, I8), src, P[nid]) - dst)) for sg in (1, -1)]
+                g[a] = float(c["node_ws"][k]) * (vals[0] - vals[1]) / (2 * eps)
+            Js.append((nid, g))
+        out.append((fw, Js))
+    return out
+
+
+def test_psd_dropped_pairs_keeps_the_data_term_psd(orc):
+    """psd_dropped_pairs (opt-in): a skinning pair (k, l) outside the graph
+    pattern adds f J_k^T J_k to block (k, k) instead of being dropped (Q2).
+    The data term is then the full J^T J plus, for every dropped unordered
+    pair, v v^T with v = e_k J_k - e_l J_l (finite-difference Jacobians),
+    hence positive semidefinite, while the reference's assembly of the same
+    correspondences differs from it by exactly those terms."""
+    nodes = _graph()
+    corrs = _corrs(nodes)
+    N = len(nodes)
+    T64 = orc.identity_dq64(N)
+    rp, ci, Aq, _, _ = orc.assemble_f64(corrs, nodes, T64, 0.0, 0.0)
+    _, _, Ap, _, _ = orc.assemble_f64(corrs, nodes, T64, 0.0, 0.0, psd_pairs=1)
+    Dq, Dp = _bsr_to_dense(rp, ci, Aq, N), _bsr_to_dense(rp, ci, Ap, N)
+    inpat = np.zeros((N, N), bool)
+    for i in range(N):
+        inpat[i, ci[rp[i]:rp[i + 1]]] = True
+    full = np.zeros((6 * N, 6 * N))
+    corr = np.zeros((6 * N, 6 * N))
+    ndrop = 0
+    for fw, Js in _data_jacobians(orc, nodes, corrs):
+        v = np.zeros(6 * N)
+        for ni, g in Js:
+            v[6 * ni:6 * ni + 6] += g
+        full += fw * np.outer(v, v)
+        for a in range(len(Js)):
+            for b2 in range(a + 1, len(Js)):
+                (ni, gi), (nj, gj) = Js[a], Js[b2]
+                if inpat[ni, nj]:
+                    continue
+                ndrop += 1
+                u = np.zeros(6 * N)
+                u[6 * ni:6 * ni + 6] += gi
+                u[6 * nj:6 * nj + 6] -= gj
+                corr += fw * np.outer(u, u)
+    assert ndrop > 10
+    scale = np.abs(full).max()
+    np.testing.assert_allclose(Dp, full + corr, atol=1e-6 * scale)
+    assert np.linalg.eigvalsh(Dp).min() > -1e-9 * scale
+    # the reference's system (Q2) is the full J^T J with the out-of-pattern blocks removed
+    np.testing.assert_allclose(Dq, full * np.kron(inpat, np.ones((6, 6))), atol=1e-6 * scale)
