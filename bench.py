@@ -349,9 +349,12 @@ def pcg_to_tolerance(h, rel_tol, reps=20):
     sp = L.solver_params(**{f: getattr(h.sp, f) for f, _ in L.SolverParams._fields_})
     sp.pcg_iterations, sp.pcg_tolerance, sp.relative_tolerance, sp.refine_iterations = 5000, rel_tol, 1, 0
     sp.refine_jacobi = 0
-    # a stop on the recurrence's residual needs that residual to follow the
-    # true one: the pipelined form's drifts (6.8e-5 true at 1e-6 on this
-    # system), so this leg runs the single-reduction form
+    # convergence judged by the FP64 true residual: after the PCG, correction
+    # rounds on b - A x while ||b - A x|| >= tol ||b||
+    sp.true_residual_rounds = 2
+    # the pipelined form's recurrence residual drifts from the true one near
+    # 1e-6 (6.8e-5 true at 1e-6 on this system), which would cost correction
+    # rounds; this leg runs the single-reduction form
     if sp.pcg_pipelined == 3:
         sp.pcg_pipelined = 2
     L.check(lib.nrf_solver_set_params(solver, C.byref(sp)), "set_params")
@@ -368,7 +371,8 @@ def pcg_to_tolerance(h, rel_tol, reps=20):
     out = (C.c_double * 2)()
     L.check(lib.nrf_solver_true_residual(solver, x.data_ptr(), out, stream.cuda_stream), "true_residual")
     result = {"preconditioner": "block-Jacobi", "pcg": PCG_FORMS.get(int(sp.pcg_pipelined), "textbook"),
-              "ms": round(e0.elapsed_time(e1) / reps, 4),
+              "stop": "FP64 true residual < 1e-6 ||b|| (correction rounds after the PCG)",
+              "ms": round(e0.elapsed_time(e1) / reps, 4), "converged": bool(res.converged),
               "iterations": int(res.iterations), "true_rel_residual": float(out[0] / out[1]), "learned": None}
     # learned block-diagonal preconditioner (trained weights of cudafusion_amd.train_lp): network
     # forward + fold into the block-Jacobi inverses once, then the same PCG
@@ -380,10 +384,13 @@ def pcg_to_tolerance(h, rel_tol, reps=20):
         lp.set_graph(h.nodes)
         d_nodes = lib.nrf_pipeline_nodes(h.pipe)
         f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        lp.forward(solver, d_nodes)  # warm-up (first launches load the code objects); the fold stays single
+        torch.cuda.synchronize()
         f0.record(stream)
         lp.forward(solver, d_nodes)
         lp.fold(solver)
         f1.record(stream)
+        fallbacks = lp.fallbacks()
         L.check(lib.nrf_solver_resolve(solver, x.data_ptr(), C.byref(res), stream.cuda_stream), "resolve")
         e0.record(stream)
         for _ in range(reps):
@@ -393,6 +400,7 @@ def pcg_to_tolerance(h, rel_tol, reps=20):
         L.check(lib.nrf_solver_true_residual(solver, x.data_ptr(), out, stream.cuda_stream), "true_residual")
         result["learned"] = {"preconditioner": "BJ^-1 + L L^T, L block diagonal (3-layer MPNN, BF16 weights)",
                              "ms": round(e0.elapsed_time(e1) / reps, 4), "network_ms": round(f0.elapsed_time(f1), 4),
+                             "converged": bool(res.converged), "fallbacks": int(fallbacks),
                              "iterations": int(res.iterations), "true_rel_residual": float(out[0] / out[1])}
         lp.close()
     L.check(lib.nrf_solver_set_params(solver, C.byref(h.sp)), "set_params")

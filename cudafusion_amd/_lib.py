@@ -157,6 +157,7 @@ _SIGS = {
     "nrf_lp_loss_grad": (I, [P, P, P, P, I, C.POINTER(C.c_double), P]),
     "nrf_lp_adam": (I, [P, F, I, P]),
     "nrf_lp_fold": (I, [P, P, P]),
+    "nrf_lp_fallbacks": (I, [P, C.POINTER(C.c_int), P]),
     "nrf_lp_blocks": (I, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
 }
 

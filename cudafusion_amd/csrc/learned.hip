@@ -265,15 +265,31 @@ __global__ void lp_lblocks_kernel(int N, const float *__restrict__ C, const floa
 }
 
 // Minv_i += L_ii L_ii^T (variant 0: the whole correction is block diagonal)
-__global__ void lp_fold_kernel(int N, const float *__restrict__ Ld, float *__restrict__ Minv) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= N * 36) return;
-  const int i = q / 36, r = (q % 36) / 6, c = q % 6;
+// One node per lane: Minv_i += L_ii L_ii^T, unless a sum of the correction
+// or of the folded block is not finite; then node i keeps its block-Jacobi
+// inverse (the fallback, counted in *fallbacks).
+__global__ void lp_fold_kernel(int N, const float *__restrict__ Ld, float *__restrict__ Minv,
+                               int32_t *__restrict__ fallbacks) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
   const float *L = Ld + (size_t)i * 36;
-  float a = 0.f;
-  for (int k = 0; k < 6; ++k) a += L[r * 6 + k] * L[c * 6 + k];
-  Minv[q] += a;
+  float *Mi = Minv + (size_t)i * 36;
+  float out[36];
+  bool finite = true;
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 6; ++c) {
+      float a = 0.f;
+      for (int k = 0; k < 6; ++k) a += L[r * 6 + k] * L[c * 6 + k];
+      out[r * 6 + c] = Mi[r * 6 + c] + a;
+      finite = finite && isfinite(out[r * 6 + c]);
+    }
+  if (!finite) {
+    atomicAdd(fallbacks, 1);
+    return;
+  }
+  for (int k = 0; k < 36; ++k) Mi[k] = out[k];
 }
+__global__ void lp_zero_kernel(int32_t *p) { *p = 0; }
 
 // ------------------------------------------------------------------- loss
 // vectors are stored [P][6N]
@@ -580,6 +596,7 @@ struct nrf_lp {
         *dzu[LAYERS] = {}, *dzm[LAYERS] = {};
   float *part = nullptr;
   double *lpart = nullptr, *dloss = nullptr;
+  int32_t *fallbacks = nullptr;  // nodes that kept block-Jacobi in the last fold
   std::vector<void *> owned;
   const nrf_solver *fwd_solver = nullptr;  // the system the last forward read
 };
@@ -615,12 +632,14 @@ extern "C" nrf_lp *nrf_lp_create(int variant, int max_nodes, int max_probes) {
             lp_alloc(p, p->U, nv) && lp_alloc(p, p->T, nv) && lp_alloc(p, p->E, nv) && lp_alloc(p, p->S, nv) &&
             lp_alloc(p, p->dTd, Nn * NTD) && lp_alloc(p, p->dTe, Ne * NTE) && lp_alloc(p, p->dh, Nn * H) &&
             lp_alloc(p, p->dself, Nn * H) && lp_alloc(p, p->dE, Ne * H) && lp_alloc(p, p->dz0, Nn * H) &&
-            lp_alloc(p, p->part, part_n) && lp_alloc(p, p->lpart, (nv + 255) / 256 + 1) && lp_alloc(p, p->dloss, 1);
+            lp_alloc(p, p->part, part_n) && lp_alloc(p, p->lpart, (nv + 255) / 256 + 1) && lp_alloc(p, p->dloss, 1) &&
+            lp_alloc(p, p->fallbacks, 1);
   for (int l = 0; l <= LAYERS && ok; ++l) ok = lp_alloc(p, p->h[l], Nn * H);
   for (int l = 0; l < LAYERS && ok; ++l)
     ok = lp_alloc(p, p->zm[l], Ne * H) && lp_alloc(p, p->m[l], Nn * H) && lp_alloc(p, p->zu[l], Nn * H) &&
          lp_alloc(p, p->dzu[l], Nn * H) && lp_alloc(p, p->dzm[l], Ne * H);
   ok = ok && hipMemset(p->params, 0, sizeof(float) * OFF.total) == hipSuccess &&
+       hipMemset(p->fallbacks, 0, sizeof(int32_t)) == hipSuccess &&
        hipMemset(p->am, 0, sizeof(float) * OFF.total) == hipSuccess &&
        hipMemset(p->av, 0, sizeof(float) * OFF.total) == hipSuccess &&
        hipMemset(p->grad, 0, sizeof(float) * OFF.total) == hipSuccess && hipDeviceSynchronize() == hipSuccess;
@@ -781,8 +800,20 @@ extern "C" int nrf_lp_fold(nrf_lp *p, nrf_solver *s, void *stream) {
   const float *Minv = nullptr;
   int rc = nrf_solver_system(s, nullptr, nullptr, nullptr, nullptr, nullptr, &Minv);
   if (rc) return rc;
-  lp_fold_kernel<<<(p->N * 36 + 255) / 256, 256, 0, as_stream(stream)>>>(p->N, p->Ld, const_cast<float *>(Minv));
+  lp_zero_kernel<<<1, 1, 0, as_stream(stream)>>>(p->fallbacks);
   NRF_LAUNCH_CHECK();
+  lp_fold_kernel<<<(p->N + 255) / 256, 256, 0, as_stream(stream)>>>(p->N, p->Ld, const_cast<float *>(Minv),
+                                                                     p->fallbacks);
+  NRF_LAUNCH_CHECK();
+  return NRF_OK;
+}
+
+extern "C" int nrf_lp_fallbacks(nrf_lp *p, int *count, void *stream) {
+  if (!p || !count) return NRF_ERR_INVALID_ARGUMENT;
+  int32_t v = 0;
+  NRF_HIP_CHECK(hipMemcpyAsync(&v, p->fallbacks, sizeof(v), hipMemcpyDeviceToHost, as_stream(stream)));
+  NRF_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
+  *count = v;
   return NRF_OK;
 }
 

@@ -109,6 +109,14 @@ class LearnedPreconditioner:
     def fold(self, solver) -> None:
         L.check(self.lib.nrf_lp_fold(self.lp, solver, torch.cuda.current_stream().cuda_stream), "lp_fold")
 
+    def fallbacks(self) -> int:
+        """Nodes that kept their block-Jacobi inverse in the last fold (the
+        correction or the folded block was not finite)."""
+        n = C.c_int()
+        L.check(self.lib.nrf_lp_fallbacks(self.lp, C.byref(n), torch.cuda.current_stream().cuda_stream),
+                "lp_fallbacks")
+        return n.value
+
     def blocks(self):
         """(C, Td, Te, features) of the last forward, as numpy arrays."""
         ptrs = [C.c_void_p() for _ in range(4)]

@@ -546,8 +546,12 @@ int nrf_lp_loss_grad(nrf_lp *lp, const nrf_solver *s, const nrf_node *nodes, con
 /* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the FP32 parameters. */
 int nrf_lp_adam(nrf_lp *lp, float lr, int step, void *stream);
 /* variant 0: the solver's block-Jacobi inverses become BJ_i^-1 + L_ii L_ii^T
-   (until its next assembly), after nrf_lp_forward on that solver. */
+   (until its next assembly), after nrf_lp_forward on that solver.  A node
+   whose correction or folded block is not finite keeps its block-Jacobi
+   inverse (the fallback; counted for nrf_lp_fallbacks). */
 int nrf_lp_fold(nrf_lp *lp, nrf_solver *s, void *stream);
+/* Nodes that kept block-Jacobi in the last nrf_lp_fold (synchronises). */
+int nrf_lp_fallbacks(nrf_lp *lp, int *count, void *stream);
 /* Device pointers of the last forward: C (N x 36), Td (N x 21), Te
    (N x 8 x 36), features (N x 28). */
 int nrf_lp_blocks(const nrf_lp *lp, const float **C, const float **Td, const float **Te, const float **features);

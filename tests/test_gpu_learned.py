@@ -134,13 +134,15 @@ def test_training_reduces_loss_and_folded_pcg_converges(gpu):
     h.close()
 
 
-def test_held_out_iterations_both_variants(gpu, orc):
+@pytest.fixture(scope="module")
+def held_out(gpu, orc):
     """The trained weights (cudafusion_amd/lp_weights_v0.npz, _v1.npz, made by
-    cudafusion_amd.train_lp) on a held-out capsule-body sequence (seed 1000):
-    PCG iterations to 1e-6 relative residual against block-Jacobi.  Block-
-    diagonal: the solver's PCG with the folded preconditioner (device);
-    graph pattern: the FP64 oracle PCG with M^-1 = BJ^-1 + L L^T, L from the
-    device forward.  Reports the numbers; asserts convergence only."""
+    cudafusion_amd.train_lp) on a held-out capsule-body sequence (seed 1000),
+    frames 1 and 2: PCG iterations to 1e-6 relative residual with block-
+    Jacobi and with each learned variant.  Block diagonal: the solver's PCG
+    with the folded preconditioner (device), judged by the FP64 true
+    residual; graph pattern: the FP64 oracle PCG with M^-1 = BJ^-1 + L L^T,
+    L from the device forward."""
     import os
 
     import torch
@@ -160,13 +162,14 @@ def test_held_out_iterations_both_variants(gpu, orc):
     lib, N = h.lib, h.N
     solver = lib.nrf_pipeline_solver(h.pipe)
     d_nodes = lib.nrf_pipeline_nodes(h.pipe)
+    rows = []
     for f in (1, 2):
         h.step(f)
         torch.cuda.synchronize()
         rp, ci, A, b, Minv = LP.solver_system(lib, solver, N)
         x = torch.zeros(6 * N, dtype=torch.float32, device="cuda")
         it_bj, rel_bj, _ = pcg_solve(lib, solver, h.sp, x, 1e-6, 1000)
-        out = {"N": N, "frame": f, "block_jacobi": it_bj}
+        out = {"N": N, "frame": f, "block_jacobi": it_bj, "rel_block_jacobi": rel_bj}
         for v in (1, 0):  # the block-diagonal fold changes the solver's inverses: last
             lp = LP.LearnedPreconditioner(v, max_nodes=N, max_probes=1)
             lp.load(paths[v])
@@ -177,14 +180,67 @@ def test_held_out_iterations_both_variants(gpu, orc):
                 z = np.zeros((N, 21)), np.zeros((N, 8, 36))
                 it0, _, _ = po.lp_pcg_f64(1, h.nodes, rp, ci, A, b, Cm, Minv, *z, 1000, 1e-6)
                 it1, r1, _ = po.lp_pcg_f64(1, h.nodes, rp, ci, A, b, Cm, Minv, Td, Te, 1000, 1e-6)
-                out.update(fp64_block_jacobi=it0, fp64_graph_pattern=it1)
-                assert r1 < 1e-6
+                out.update(fp64_block_jacobi=it0, fp64_graph_pattern=it1, rel_graph_pattern=r1)
             else:
                 lp.fold(solver)
+                out["fallbacks"] = lp.fallbacks()
                 x.zero_()
                 it_l, rel_l, _ = pcg_solve(lib, solver, h.sp, x, 1e-6, 1000)
-                out.update(block_diagonal=it_l)
-                assert rel_l < 2e-6
+                out.update(block_diagonal=it_l, rel_block_diagonal=rel_l)
             lp.close()
         print(f"[lp held-out] {out}", flush=True)
+        rows.append(out)
+    h.close()
+    return rows
+
+
+def test_held_out_iterations_both_variants(held_out):
+    """Both learned variants converge to 1e-6 on the held-out systems, with
+    no node falling back to block-Jacobi; the iteration counts are printed
+    (the gain test below records that they do not halve)."""
+    for out in held_out:
+        assert out["rel_graph_pattern"] < 1e-6 and out["rel_block_diagonal"] < 2e-6 and out["rel_block_jacobi"] < 2e-6
+        assert out["fallbacks"] == 0
+
+
+@pytest.mark.xfail(strict=True, reason="target not met: the trained one-hop corrections BJ^-1 + L L^T raise the "
+                                       "PCG iterations against block-Jacobi on held-out systems instead of halving "
+                                       "them (the additive PSD form cannot lower the spectrum's large end)")
+def test_learned_preconditioner_halves_iterations(held_out):
+    """The north-star target: at least 2x fewer PCG iterations to 1e-6 than
+    block-Jacobi, for the better of the two variants, on every held-out
+    system.  Kept as a strict expected failure: it documents the shortfall
+    and fails if the target is reached without this record being updated."""
+    for out in held_out:
+        best = min(out["block_diagonal"], out["fp64_graph_pattern"])
+        assert 2 * best <= out["block_jacobi"]
+
+
+def test_fold_falls_back_to_block_jacobi_on_non_finite(gpu):
+    """Network parameters with a NaN in the diagonal head's bias give a
+    non-finite L_ii at every node: the fold keeps every block-Jacobi inverse
+    unchanged and counts N fallbacks (a host-forced condition: no kernel
+    faults or waits)."""
+    import torch
+
+    from cudafusion_amd import learned as LP
+    from oracle import pyoracle as po
+    h = _system()
+    lib, N = h.lib, h.N
+    solver = lib.nrf_pipeline_solver(h.pipe)
+    d_nodes = lib.nrf_pipeline_nodes(h.pipe)
+    lp = LP.LearnedPreconditioner(0, max_nodes=N, max_probes=1, seed=0)
+    lp.set_graph(h.nodes)
+    params = lp.params()
+    o, n = po.lp_offsets()["bo"]
+    params[o] = np.nan
+    lp.set_params(params)
+    M0 = LP.solver_system(lib, solver, N)[4].copy()
+    lp.forward(solver, d_nodes)
+    lp.fold(solver)
+    torch.cuda.synchronize()
+    M1 = LP.solver_system(lib, solver, N)[4]
+    assert lp.fallbacks() == N
+    np.testing.assert_array_equal(M1, M0)
+    lp.close()
     h.close()
