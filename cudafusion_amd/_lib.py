@@ -37,6 +37,16 @@ class Camera(C.Structure):
                 ("width", C.c_int32), ("height", C.c_int32)]
 
 
+class AssocParams(C.Structure):
+    _fields_ = [("dist_threshold", C.c_float), ("angle_threshold", C.c_float), ("view_threshold", C.c_float),
+                ("search_radius_px", C.c_int32)]
+
+
+def assoc_params(dist_threshold=0.05, angle_threshold=0.7, view_threshold=0.2, search_radius_px=3) -> "AssocParams":
+    """Defaults are the reference pipeline's (include/fusion_app.h:79-82)."""
+    return AssocParams(dist_threshold, angle_threshold, view_threshold, int(search_radius_px))
+
+
 class SolverParams(C.Structure):
     _fields_ = [("gn_iterations", C.c_int32), ("pcg_iterations", C.c_int32), ("pcg_tolerance", C.c_float),
                 ("lambda_smooth", C.c_float), ("lambda_rot", C.c_float), ("lambda_damping", C.c_float),
@@ -97,6 +107,10 @@ _SIGS = {
     "nrf_mc_extract": (I, [C.POINTER(VolumeParams), P, P, C.c_size_t, P, P, P, I, P, P]),
     "nrf_mc_cases": (P, [P]),
     "nrf_mc_extract_noalloc": (I, [C.POINTER(VolumeParams), P, P, P, P, I, P, P]),
+    "nrf_depth_normals": (I, [P, C.POINTER(Camera), P, P]),
+    "nrf_pixel_knn": (I, [P, I, P, P, P, P, P]),
+    "nrf_find_correspondences": (I, [P, P, P, P, C.POINTER(Camera), P, P, P, C.POINTER(AssocParams), P, P, P, P]),
+    "nrf_mark_optimized_voxels": (I, [P, P, I, C.POINTER(VolumeParams), I, P, P, P]),
     "nrf_tsdf16_reset": (I, [C.POINTER(VolumeParams), P, P, P]),
     "nrf_tsdf16_from_voxels": (I, [C.POINTER(VolumeParams), P, P, P, P]),
     "nrf_knn_compact": (I, [C.POINTER(VolumeParams), P, P, P, P, P]),

@@ -301,6 +301,59 @@ int nrf_corr_pack(const nrf_volume_params *vp, const int32_t *voxel_knn,
                   const float *dst, const float *normals, const float *weights,
                   const uint8_t *valid, int count, nrf_corr *out, void *stream);
 
+/* ------------------------------------------------ projective association */
+/* The step that produces the solver's correspondences and the integration's
+   optimised-voxel counts from the live surface prediction (per-pixel live
+   vertices, normals and canonical hit voxel, world frame) and the depth
+   frame.  Vectors are 3 floats per pixel, x fastest; camera_pose is the
+   row-major 4x4 camera-to-world pose (inverted on the host as
+   include/fusion_math.h:80-95 does). */
+
+/* = the association fields of the pipeline parameters
+   (include/fusion_app.h:79-82 and their defaults). */
+typedef struct nrf_assoc_params {
+  float dist_threshold;   /* 0.05: point-to-plane distance limit (m) */
+  float angle_threshold;  /* 0.7: minimum |cos| between live and depth normals */
+  float view_threshold;   /* 0.2: minimum |cos| between live normal and view ray */
+  int32_t search_radius_px; /* 3: half-size of the search window around the projection */
+} nrf_assoc_params;
+
+/* Replaces compute_depth_normals_kernel (kernels/tsdf_kernels.cu:180-230):
+   the normal of each pixel from its right and lower neighbours' back
+   projections, pointing to the camera; zero where a depth is missing and in
+   the last row and column. */
+int nrf_depth_normals(const float *depth, const nrf_camera *cam, float *normals, void *stream);
+
+/* Replaces compute_pixel_knn_kernel (kernels/tsdf_kernels.cu:573-604): the
+   4-NN of each pixel's canonical hit voxel (hit_voxel[px] < 0: ids -1,
+   weights 0). */
+int nrf_pixel_knn(const int32_t *hit_voxel, int num_pixels, const int32_t *voxel_knn, const float *voxel_knn_w,
+                  int32_t *pixel_knn, float *pixel_knn_w, void *stream);
+
+/* Replaces find_correspondences_kernel (kernels/tsdf_kernels.cu:386-563):
+   per pixel, the live vertex projected into the depth frame and the best
+   depth pixel of the (2 r + 1)^2 window by point-to-plane distance (ties to
+   the first in row-major window order), with the reference's distance,
+   normal-angle and view-angle tests and confidence weight; corrs[px] gets
+   src = live vertex, dst / normal in the world frame and the pixel's 4-NN.
+   Pixels without one get valid = 0, weight 0, zero vectors and ids -1 (the
+   reference leaves those fields unset).  num_valid (device int) and stats
+   (optional, device, 6 ints: invalid live, out of projection, no depth, none
+   close, angle or view, valid) are added to. */
+int nrf_find_correspondences(const float *live_vertices, const float *live_normals, const float *depth,
+                             const float *depth_normals, const nrf_camera *cam, const float camera_pose[16],
+                             const int32_t *pixel_knn, const float *pixel_knn_w, const nrf_assoc_params *params,
+                             nrf_corr *corrs, int32_t *num_valid, int32_t *stats, void *stream);
+
+/* Replaces mark_optimized_voxels_kernel (kernels/tsdf_kernels.cu:606-651):
+   every voxel of the 3x3x3 neighbourhood of a valid correspondence's hit
+   voxel gets voxel_opt_counts += 1 once per frame (voxel_opt_frame holds the
+   last frame that counted it).  The counts do not depend on the order of the
+   updates. */
+int nrf_mark_optimized_voxels(const nrf_corr *corrs, const int32_t *hit_voxel, int num_pixels,
+                              const nrf_volume_params *vp, int frame_id, int32_t *voxel_opt_counts,
+                              int32_t *voxel_opt_frame, void *stream);
+
 /* ---------------------------------------------------------------- solver */
 typedef struct nrf_solver nrf_solver;
 

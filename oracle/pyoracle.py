@@ -51,6 +51,10 @@ _SIGS = {
     "orc_pcg_f64": (D, [P, P, P, P, P, I, I, D, I, P, P, P, P]),
     "orc_spmv_f64": (None, [P, P, P, P, P, I]),
     "orc_validate_f32": (I, [P, I, P]),
+    "orc_depth_normals_f32": (None, [P, P, P]),
+    "orc_pixel_knn": (None, [P, I, P, P, P, P]),
+    "orc_find_correspondences_f32": (I, [P, P, P, P, P, P, P, P, P, P, P]),
+    "orc_mark_optimized_voxels": (None, [P, P, I, P, I, P, P]),
     "orc_validate_f64": (I, [P, I, P]),
     "orc_solve_gn_f32": (I, [P, P, I, P, I, P, P, P, P, P, P, P]),
     "orc_solve_gn_f64": (I, [P, P, I, P, I, P, P, P, P, P, P, P]),
@@ -199,6 +203,44 @@ def validate(dx, gn, fp64=False) -> bool:
     d = np.ascontiguousarray(dx, dtype=np.float64 if fp64 else np.float32)
     fn = lib.orc_validate_f64 if fp64 else lib.orc_validate_f32
     return bool(fn(ptr(d), d.size // 6, C.byref(gn)))
+
+
+# ---------------------------------------------------- projective association
+
+def depth_normals_f32(depth, cam):
+    n = np.zeros(cam.width * cam.height * 3, np.float32)
+    d = np.ascontiguousarray(depth, np.float32)
+    load().orc_depth_normals_f32(ptr(d), C.byref(cam), ptr(n))
+    return n
+
+
+def pixel_knn(hit, voxel_knn, voxel_knn_w):
+    hit = np.ascontiguousarray(hit, np.int32)
+    ids = np.zeros(len(hit) * 4, np.int32)
+    ws = np.zeros(len(hit) * 4, np.float32)
+    load().orc_pixel_knn(ptr(hit), len(hit), ptr(voxel_knn), ptr(voxel_knn_w), ptr(ids), ptr(ws))
+    return ids, ws
+
+
+def find_correspondences_f32(live_v, live_n, depth, dnorm, cam, pose, pknn, pknn_w, ap):
+    """Returns (corrs, valid count, stats[6])."""
+    from cudafusion_amd import _lib as L
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (live_v, live_n, depth, dnorm, pose)]
+    corrs = np.zeros(cam.width * cam.height, L.CORR)
+    stats = np.zeros(6, np.int32)
+    pk = None if pknn is None else np.ascontiguousarray(pknn, np.int32)
+    pw = None if pknn_w is None else np.ascontiguousarray(pknn_w, np.float32)
+    n = load().orc_find_correspondences_f32(ptr(arrs[0]), ptr(arrs[1]), ptr(arrs[2]), ptr(arrs[3]), C.byref(cam),
+                                            ptr(arrs[4]), ptr(pk), ptr(pw), C.byref(ap), ptr(corrs), ptr(stats))
+    return corrs, n, stats
+
+
+def mark_optimized_voxels(corrs, hit, dims, frame_id, counts, last_frame):
+    """Updates counts / last_frame (int32 arrays of the volume size) in place."""
+    hit = np.ascontiguousarray(hit, np.int32)
+    d = np.ascontiguousarray(dims, np.int32)
+    load().orc_mark_optimized_voxels(ptr(corrs), ptr(hit), len(hit), ptr(d), int(frame_id), ptr(counts),
+                                     ptr(last_frame))
 
 
 def pattern(nodes):
