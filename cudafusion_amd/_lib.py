@@ -111,6 +111,10 @@ _SIGS = {
     "nrf_pixel_knn": (I, [P, I, P, P, P, P, P]),
     "nrf_find_correspondences": (I, [P, P, P, P, C.POINTER(Camera), P, P, P, C.POINTER(AssocParams), P, P, P, P]),
     "nrf_mark_optimized_voxels": (I, [P, P, I, C.POINTER(VolumeParams), I, P, P, P]),
+    "nrf_skin_vertices": (I, [P, I, C.POINTER(VolumeParams), P, I, P, P, P, P]),
+    "nrf_raster_workspace_size": (C.c_size_t, [C.POINTER(Camera)]),
+    "nrf_rasterize_mesh": (I, [P, P, P, I, I, C.POINTER(Camera), P, C.POINTER(VolumeParams), P, C.c_size_t, P, P,
+                               P, P, P]),
     "nrf_tsdf16_reset": (I, [C.POINTER(VolumeParams), P, P, P]),
     "nrf_tsdf16_from_voxels": (I, [C.POINTER(VolumeParams), P, P, P, P]),
     "nrf_knn_compact": (I, [C.POINTER(VolumeParams), P, P, P, P, P]),

@@ -127,6 +127,14 @@ int integrate_batch_run(IntegrateBatch *b, const BatchDepth &fr, hipStream_t st)
 // voxels all integrate unwarped.  Valid until the kNN changes.
 int64_t knn_row_words(const nrf_volume_params *vp);
 int knn_row_flags(const nrf_volume_params *vp, const int32_t *knn, uint32_t *rows, hipStream_t st);
+// live surface prediction (raster.hip): nrf_skin_vertices and
+// nrf_rasterize_mesh with the depth/triangle key buffer (8 B per pixel)
+int skin_vertices(const float *canonical, int n, const nrf_volume_params *vp, const nrf_dualquat *centered,
+                  int num_nodes, const int32_t *voxel_knn, const float *voxel_knn_w, float *out, hipStream_t st);
+int rasterize_mesh(const float *canonical, const float *warped, const int32_t *triangles, int num_vertices,
+                   int num_triangles, const nrf_camera *cam, const float camera_pose[16], const nrf_volume_params *vp,
+                   unsigned long long *ztri, float *live_vertices, float *live_normals, int32_t *hit_voxel,
+                   float *canonical_points, hipStream_t st);
 // nrf_tsdf16_integrate with the kNN's row occupancy (null: none)
 int tsdf16_integrate(const nrf_volume_params *vp, uint16_t *tsdf16, uint8_t *weight8, const float *depth,
                      const nrf_camera *cam, const float camera_pose[16], const nrf_dualquat *centered, int num_nodes,

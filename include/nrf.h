@@ -354,6 +354,32 @@ int nrf_mark_optimized_voxels(const nrf_corr *corrs, const int32_t *hit_voxel, i
                               const nrf_volume_params *vp, int frame_id, int32_t *voxel_opt_counts,
                               int32_t *voxel_opt_frame, void *stream);
 
+/* --------------------------------------------- live surface prediction */
+/* Replaces skin_vertices_from_voxel_knn_kernel (src/skinning_kernels.cu:
+   29-63): each canonical vertex warped through the 4-NN of the voxel that
+   contains it (floor of (p - origin) / voxel_size); unchanged outside the
+   volume or when num_nodes = 0.  centered = nrf_warp_centered_table of the
+   current transforms (the same blend as warp_point). */
+int nrf_skin_vertices(const float *canonical_vertices, int num_vertices, const nrf_volume_params *vp,
+                      const nrf_dualquat *centered, int num_nodes, const int32_t *voxel_knn,
+                      const float *voxel_knn_w, float *warped_vertices, void *stream);
+
+/* Replaces rasterize_triangles_z_kernel + resolve_rasterized_triangles_kernel
+   (src/mesh_raster_kernels.cu:76-200) and the caller's resets
+   (include/fusion_app.h:1226-1230): the warped mesh is rasterised into the
+   camera (pixel centres at +0.5, barycentric test -1e-5, nearest depth then
+   lowest triangle index wins); each covered pixel gets the interpolated live
+   vertex, the face normal and the voxel of the interpolated canonical point
+   (and, when canonical_points != NULL, that point); the others zero vectors
+   and hit voxel -1.  triangles: 3 vertex indices each.  The workspace holds
+   one 64-bit key per pixel (nrf_raster_workspace_size). */
+size_t nrf_raster_workspace_size(const nrf_camera *cam);
+int nrf_rasterize_mesh(const float *canonical_vertices, const float *warped_vertices, const int32_t *triangles,
+                       int num_vertices, int num_triangles, const nrf_camera *cam, const float camera_pose[16],
+                       const nrf_volume_params *vp, void *workspace, size_t workspace_bytes,
+                       float *live_vertices, float *live_normals, int32_t *hit_voxel, float *canonical_points,
+                       void *stream);
+
 /* ---------------------------------------------------------------- solver */
 typedef struct nrf_solver nrf_solver;
 

@@ -15,6 +15,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <vector>
+#include <algorithm>
 
 #include "../include/nrf.h"
 
@@ -218,6 +220,111 @@ void orc_mark_optimized_voxels(const nrf_corr *corrs, const int32_t *hit_voxel, 
         }
       }
     }
+  }
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- live surface prediction
+namespace {
+inline float edge_fn(float ax, float ay, float bx, float by, float px, float py) {
+  return (px - ax) * (by - ay) - (py - ay) * (bx - ax);
+}
+inline int sat_int(float v) { return (int)std::fmin(std::fmax(v, -1073741824.f), 1073741824.f); }
+inline int voxel_of(F3 p, const nrf_volume_params &vp) {
+  const int vx = sat_int(std::floor((p.x - vp.origin[0]) / vp.voxel_size));
+  const int vy = sat_int(std::floor((p.y - vp.origin[1]) / vp.voxel_size));
+  const int vz = sat_int(std::floor((p.z - vp.origin[2]) / vp.voxel_size));
+  if (vx < 0 || vx >= vp.dims[0] || vy < 0 || vy >= vp.dims[1] || vz < 0 || vz >= vp.dims[2]) return -1;
+  return (vz * vp.dims[1] + vy) * vp.dims[0] + vx;
+}
+// project_triangle (src/mesh_raster_kernels.cu:44-72)
+bool project_tri(const float *warped, const int ids[3], const nrf_camera &cam, const Rigid &Tcw, F3 wc[3], F3 cc[3],
+                 float px[3], float py[3], float &area) {
+  for (int k = 0; k < 3; ++k) {
+    wc[k] = ld3(warped + 3 * (long)ids[k]);
+    cc[k] = Tcw.point(wc[k]);
+  }
+  if (cc[0].z <= 0.0f || cc[1].z <= 0.0f || cc[2].z <= 0.0f) return false;
+  for (int k = 0; k < 3; ++k) {
+    px[k] = cam.fx * cc[k].x / cc[k].z + cam.cx;
+    py[k] = cam.fy * cc[k].y / cc[k].z + cam.cy;
+  }
+  area = edge_fn(px[0], py[0], px[1], py[1], px[2], py[2]);
+  return std::fabs(area) >= 1e-6f;
+}
+}  // namespace
+
+extern "C" {
+
+// rasterize_triangles_z_kernel + resolve_rasterized_triangles_kernel
+// (src/mesh_raster_kernels.cu:76-200) with the caller's resets: per pixel
+// the nearest covering triangle (then the lowest index), its interpolated
+// live vertex, face normal, canonical point and that point's voxel.
+void orc_rasterize_f32(const float *canon, const float *warped, const int32_t *tris, int nv, int nt,
+                       const nrf_camera *cam, const float pose[16], const nrf_volume_params *vp, float *live_v,
+                       float *live_n, int32_t *hit, float *canon_pt) {
+  const int W = cam->width, H = cam->height;
+  const Rigid Tcw = inverse_rigid(pose);
+  std::vector<unsigned long long> zt((size_t)W * H, ~0ull);
+  auto ids_of = [&](int t, int ids[3]) {
+    for (int k = 0; k < 3; ++k) ids[k] = tris[3 * (long)t + k];
+    return !(ids[0] < 0 || ids[1] < 0 || ids[2] < 0 || ids[0] >= nv || ids[1] >= nv || ids[2] >= nv);
+  };
+  for (int t = 0; t < nt; ++t) {
+    int ids[3];
+    if (!ids_of(t, ids)) continue;
+    F3 wc[3], cc[3];
+    float px[3], py[3], area = 0;
+    if (!project_tri(warped, ids, *cam, Tcw, wc, cc, px, py, area)) continue;
+    const int min_u = std::max(0, sat_int(std::floor(std::fmin(px[0], std::fmin(px[1], px[2])))));
+    const int max_u = std::min(W - 1, sat_int(std::ceil(std::fmax(px[0], std::fmax(px[1], px[2])))));
+    const int min_v = std::max(0, sat_int(std::floor(std::fmin(py[0], std::fmin(py[1], py[2])))));
+    const int max_v = std::min(H - 1, sat_int(std::ceil(std::fmax(py[0], std::fmax(py[1], py[2])))));
+    if (min_u > max_u || min_v > max_v) continue;
+    const F3 fn = normalize3(cross3(sub(wc[1], wc[0]), sub(wc[2], wc[0])));
+    if (norm3(fn) < 1e-6f) continue;
+    for (int y = min_v; y <= max_v; ++y)
+      for (int x = min_u; x <= max_u; ++x) {
+        const float qx = (float)x + 0.5f, qy = (float)y + 0.5f;
+        const float w0 = edge_fn(px[1], py[1], px[2], py[2], qx, qy) / area;
+        const float w1 = edge_fn(px[2], py[2], px[0], py[0], qx, qy) / area;
+        const float w2 = edge_fn(px[0], py[0], px[1], py[1], qx, qy) / area;
+        if (w0 < -1e-5f || w1 < -1e-5f || w2 < -1e-5f) continue;
+        const float z = w0 * cc[0].z + w1 * cc[1].z + w2 * cc[2].z;
+        if (z <= 0.0f || !std::isfinite(z)) continue;
+        uint32_t zb;
+        std::memcpy(&zb, &z, 4);
+        const unsigned long long key = ((unsigned long long)zb << 32) | (unsigned)t;
+        unsigned long long &slot = zt[(size_t)y * W + x];
+        if (key < slot) slot = key;
+      }
+  }
+  for (int p = 0; p < W * H; ++p) {
+    st3(live_v + 3 * (long)p, f3(0, 0, 0));
+    st3(live_n + 3 * (long)p, f3(0, 0, 0));
+    if (canon_pt) st3(canon_pt + 3 * (long)p, f3(0, 0, 0));
+    hit[p] = -1;
+    if (zt[p] == ~0ull) continue;
+    const int t = (int)(zt[p] & 0xffffffffull);
+    int ids[3];
+    if (!ids_of(t, ids)) continue;
+    F3 wc[3], cc[3];
+    float px[3], py[3], area = 0;
+    if (!project_tri(warped, ids, *cam, Tcw, wc, cc, px, py, area)) continue;
+    const float qx = (float)(p % W) + 0.5f, qy = (float)(p / W) + 0.5f;
+    const float w0 = edge_fn(px[1], py[1], px[2], py[2], qx, qy) / area;
+    const float w1 = edge_fn(px[2], py[2], px[0], py[0], qx, qy) / area;
+    const float w2 = edge_fn(px[0], py[0], px[1], py[1], qx, qy) / area;
+    if (w0 < -1e-5f || w1 < -1e-5f || w2 < -1e-5f) continue;
+    st3(live_v + 3 * (long)p, f3(w0 * wc[0].x + w1 * wc[1].x + w2 * wc[2].x, w0 * wc[0].y + w1 * wc[1].y + w2 * wc[2].y,
+                                w0 * wc[0].z + w1 * wc[1].z + w2 * wc[2].z));
+    st3(live_n + 3 * (long)p, normalize3(cross3(sub(wc[1], wc[0]), sub(wc[2], wc[0]))));
+    const F3 c0 = ld3(canon + 3 * (long)ids[0]), c1 = ld3(canon + 3 * (long)ids[1]), c2 = ld3(canon + 3 * (long)ids[2]);
+    const F3 cp = f3(w0 * c0.x + w1 * c1.x + w2 * c2.x, w0 * c0.y + w1 * c1.y + w2 * c2.y,
+                     w0 * c0.z + w1 * c1.z + w2 * c2.z);
+    if (canon_pt) st3(canon_pt + 3 * (long)p, cp);
+    hit[p] = voxel_of(cp, *vp);
   }
 }
 

@@ -1061,6 +1061,30 @@ double orc_pcg_f64(const int32_t *rp, const int32_t *ci, const double *A, const 
 void orc_spmv_f64(const int32_t *rp, const int32_t *ci, const double *A, const double *x, double *y, int N) {
   spmv(rp, ci, A, x, y, N);
 }
+// skin_vertices_from_voxel_knn_kernel (src/skinning_kernels.cu:29-63): each
+// vertex warped (warp_point) through the 4-NN of the voxel containing it,
+// unchanged outside the volume or without nodes.
+void orc_skin_vertices_f32(const float *canon, int n, const nrf_volume_params *vp, const nrf_node *nodes,
+                           const nrf_dualquat *Tq, int N, const int32_t *knn, const float *knn_w, float *out) {
+  std::vector<DQ<float>> cen;
+  if (N > 0) {
+    auto T = load_T<float>(Tq, N);
+    cen = centered_table(nodes, T.data(), N);
+  }
+  for (int i = 0; i < n; ++i) {
+    V3<float> p = v3f<float>(canon + 3 * (long)i);
+    if (N > 0) {
+      const int vx = (int)std::floor((p.x - vp->origin[0]) / vp->voxel_size);
+      const int vy = (int)std::floor((p.y - vp->origin[1]) / vp->voxel_size);
+      const int vz = (int)std::floor((p.z - vp->origin[2]) / vp->voxel_size);
+      if (vx >= 0 && vx < vp->dims[0] && vy >= 0 && vy < vp->dims[1] && vz >= 0 && vz < vp->dims[2]) {
+        const long v = ((long)vz * vp->dims[1] + vy) * vp->dims[0] + vx;
+        p = warp_point(p, cen.data(), N, knn + 4 * v, knn_w + 4 * v);
+      }
+    }
+    out[3 * i] = p.x; out[3 * i + 1] = p.y; out[3 * i + 2] = p.z;
+  }
+}
 // Step validation of the caller (include/fusion_app.h:409-491): 1 = accept.
 int orc_validate_f32(const float *dx, int N, const nrf_gn_params *g) { return validate_step(dx, N, *g) ? 1 : 0; }
 int orc_validate_f64(const double *dx, int N, const nrf_gn_params *g) { return validate_step(dx, N, *g) ? 1 : 0; }

@@ -55,6 +55,8 @@ _SIGS = {
     "orc_pixel_knn": (None, [P, I, P, P, P, P]),
     "orc_find_correspondences_f32": (I, [P, P, P, P, P, P, P, P, P, P, P]),
     "orc_mark_optimized_voxels": (None, [P, P, I, P, I, P, P]),
+    "orc_skin_vertices_f32": (None, [P, I, P, P, P, I, P, P, P]),
+    "orc_rasterize_f32": (None, [P, P, P, I, I, P, P, P, P, P, P, P]),
     "orc_validate_f64": (I, [P, I, P]),
     "orc_solve_gn_f32": (I, [P, P, I, P, I, P, P, P, P, P, P, P]),
     "orc_solve_gn_f64": (I, [P, P, I, P, I, P, P, P, P, P, P, P]),
@@ -241,6 +243,29 @@ def mark_optimized_voxels(corrs, hit, dims, frame_id, counts, last_frame):
     d = np.ascontiguousarray(dims, np.int32)
     load().orc_mark_optimized_voxels(ptr(corrs), ptr(hit), len(hit), ptr(d), int(frame_id), ptr(counts),
                                      ptr(last_frame))
+
+
+def skin_vertices_f32(vp, canon, nodes, T, knn, knn_w):
+    c = np.ascontiguousarray(canon, np.float32)
+    out = np.zeros_like(c)
+    N = 0 if nodes is None else len(nodes)
+    load().orc_skin_vertices_f32(ptr(c), c.size // 3, C.byref(vp), ptr(nodes), ptr(T), N, ptr(knn), ptr(knn_w),
+                                 ptr(out))
+    return out
+
+
+def rasterize_f32(canon, warped, tris, cam, pose, vp):
+    """Returns (live vertices, live normals, hit voxel, canonical points) per pixel."""
+    c = np.ascontiguousarray(canon, np.float32)
+    w = np.ascontiguousarray(warped, np.float32)
+    t = np.ascontiguousarray(tris, np.int32)
+    P_ = np.ascontiguousarray(pose, np.float32)
+    npx = cam.width * cam.height
+    lv, ln, cp = (np.zeros(npx * 3, np.float32) for _ in range(3))
+    hit = np.zeros(npx, np.int32)
+    load().orc_rasterize_f32(ptr(c), ptr(w), ptr(t), c.size // 3, t.size // 3, C.byref(cam), ptr(P_), C.byref(vp),
+                             ptr(lv), ptr(ln), ptr(hit), ptr(cp))
+    return lv, ln, hit, cp
 
 
 def pattern(nodes):

@@ -33,7 +33,7 @@ def scene(gpu, orc):
     wf = WarpField(w.node_spacing)
     wf.upload_nodes(nodes)
     knn = wf.compute_voxel_knn(vol)
-    return dict(w=w, frames=frames, cam=cam, pose=pose, vol=vol, knn=knn, torch=torch)
+    return dict(w=w, frames=frames, cam=cam, pose=pose, vol=vol, knn=knn, nodes=nodes, torch=torch)
 
 
 def _live_maps(sc, f):
@@ -108,3 +108,62 @@ def test_association_matches_oracle(scene, orc, radius):
         np.testing.assert_array_equal(counts.cpu().numpy(), h_counts)
         np.testing.assert_array_equal(last.cpu().numpy(), h_last)
     assert h_counts.max() == 2
+
+
+def test_skin_and_raster_match_oracle(scene, orc):
+    """The frame-0 mesh skinned through the voxel kNN with perturbed node
+    transforms (nrf_skin_vertices) and rasterised into the camera
+    (nrf_rasterize_mesh), against the oracle: skinned vertices to 1e-6; with
+    the device's skinned vertices as input, the winning triangle of every
+    pixel is the oracle's (hit voxels bit-exact, covered pixels equal) and
+    the live vertices, normals and canonical points agree to 1e-6."""
+    from cudafusion_amd import _lib as L
+    torch = scene["torch"]
+    lib = L.load()
+    cam = scene["cam"]
+    vol = scene["vol"]
+    verts, tris = vol.extract_surface_device()
+    nv, nt = verts.shape[0], tris.shape[0]
+    assert nt > 1000
+    nodes = scene["nodes"]
+    N = len(nodes)
+    rng = np.random.default_rng(4)
+    T = np.zeros(N, L.DUALQUAT)
+    T["real"][:, 3] = 1
+    for i in range(N):  # small random node motions
+        tw = np.concatenate([rng.normal(0, 0.01, 3), rng.normal(0, 0.003, 3)]).astype(np.float32)
+        out = np.zeros(1, L.DUALQUAT)
+        orc.load().orc_twist_update_f32(tw.ctypes.data, T[i:i + 1].copy().ctypes.data, 0.0, 0.0, 1.0,
+                                        out.ctypes.data)
+        T[i] = out[0]
+    d_nodes = _dev(torch, nodes.view(np.uint8))
+    d_T = _dev(torch, T.view(np.uint8))
+    d_cen = torch.empty(N * 32, dtype=torch.uint8, device="cuda")
+    L.check(lib.nrf_warp_centered_table(d_nodes.data_ptr(), d_T.data_ptr(), N, d_cen.data_ptr(), None), "centered")
+    knn, knn_w = scene["knn"]
+    d_warp = torch.empty(nv * 3, dtype=torch.float32, device="cuda")
+    L.check(lib.nrf_skin_vertices(verts.data_ptr(), nv, C.byref(vol.vp), d_cen.data_ptr(), N, knn.data_ptr(),
+                                  knn_w.data_ptr(), d_warp.data_ptr(), None), "skin")
+    pose = scene["pose"]
+    npx = cam.width * cam.height
+    ws = torch.empty(lib.nrf_raster_workspace_size(C.byref(cam)), dtype=torch.uint8, device="cuda")
+    lv, ln, cp = (torch.empty(npx * 3, dtype=torch.float32, device="cuda") for _ in range(3))
+    hit = torch.empty(npx, dtype=torch.int32, device="cuda")
+    L.check(lib.nrf_rasterize_mesh(verts.data_ptr(), d_warp.data_ptr(), tris.data_ptr(), nv, nt, C.byref(cam),
+                                   pose.ctypes.data, C.byref(vol.vp), ws.data_ptr(), ws.numel(), lv.data_ptr(),
+                                   ln.data_ptr(), hit.data_ptr(), cp.data_ptr(), None), "raster")
+    torch.cuda.synchronize()
+    h_verts = verts.cpu().numpy().reshape(-1)
+    ref_w = orc.skin_vertices_f32(vol.vp, h_verts, nodes, T, knn.cpu().numpy(), knn_w.cpu().numpy())
+    g_w = d_warp.cpu().numpy()
+    moved = np.abs(ref_w - h_verts).max()
+    np.testing.assert_allclose(g_w, ref_w, atol=1e-6)
+    rlv, rln, rhit, rcp = orc.rasterize_f32(h_verts, g_w, tris.cpu().numpy().reshape(-1), cam, pose, vol.vp)
+    ghit = hit.cpu().numpy()
+    print(f"[raster] {nt} triangles, skinning moved vertices by up to {moved:.2e} m, covered pixels "
+          f"{int((ghit >= 0).sum())} (oracle {int((rhit >= 0).sum())})", flush=True)
+    assert moved > 1e-4 and (ghit >= 0).sum() > 5000
+    np.testing.assert_array_equal(ghit, rhit)
+    np.testing.assert_allclose(lv.cpu().numpy(), rlv, atol=1e-6)
+    np.testing.assert_allclose(ln.cpu().numpy(), rln, atol=1e-6)
+    np.testing.assert_allclose(cp.cpu().numpy(), rcp, atol=1e-6)

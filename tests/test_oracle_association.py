@@ -157,3 +157,107 @@ def test_mark_optimized_voxels_counts(orc):
     c2 = np.zeros(V, np.int32)
     orc.mark_optimized_voxels(corrs2, np.array([idx(5, 4, 3)], np.int32), dims, 0, c2, np.full(V, -1, np.int32))
     assert c2.sum() == 8
+
+
+# ------------------------------------------------- live surface prediction
+
+def test_raster_fronto_parallel_triangles(orc):
+    """Two fronto-parallel triangles at 1.0 m and 0.8 m overlapping in the
+    image: every pixel centre inside a triangle's projection gets that
+    triangle (the nearer where both cover it), its live vertex is the
+    pixel-centre ray's point on the plane (screen-space and object-space
+    barycentrics agree on a fronto-parallel triangle), the face normal, the
+    canonical point interpolated the same way and that point's voxel."""
+    vp = L.volume_params((64, 64, 64), 0.0213, 0.06, (-0.6413, -0.6387, 0.2071))
+    pose = np.eye(4, dtype=np.float32).reshape(16)
+
+    def tri_at(z, u0, v0, u1, v1, u2, v2):
+        return [_unproject(float(u), float(v), z) for u, v in ((u0, v0), (u1, v1), (u2, v2))]
+
+    A = tri_at(1.0, 4, 4, 30, 6, 8, 26)
+    B = tri_at(0.8, 15, 10, 36, 12, 20, 28)
+    verts = np.array(A + B, np.float32).reshape(-1)
+    canon = verts.copy()
+    tris = np.array([0, 1, 2, 3, 4, 5], np.int32)
+    lv, ln, hit, cp = orc.rasterize_f32(canon, verts, tris, CAM, pose, vp)
+    lv, ln, cp = lv.reshape(H, W, 3), ln.reshape(H, W, 3), cp.reshape(H, W, 3)
+    hit = hit.reshape(H, W)
+
+    def inside(tri, x, y):
+        P = np.array([[CAM.fx * p[0] / p[2] + CAM.cx, CAM.fy * p[1] / p[2] + CAM.cy] for p in tri])
+        def e(a, b, q):
+            return (q[0] - a[0]) * (b[1] - a[1]) - (q[1] - a[1]) * (b[0] - a[0])
+        q = (x + 0.5, y + 0.5)
+        ar = e(P[0], P[1], P[2])
+        w = [e(P[1], P[2], q) / ar, e(P[2], P[0], q) / ar, e(P[0], P[1], q) / ar]
+        return min(w) > 1e-4, min(w) < -1e-4  # clearly inside / clearly outside
+    nA = nB = 0
+    for y in range(H):
+        for x in range(W):
+            inA, outA = inside(A, x, y)
+            inB, outB = inside(B, x, y)
+            if not (inA or outA) or not (inB or outB):
+                continue  # on an edge: left to the -1e-5 test
+            z = 0.8 if inB else (1.0 if inA else None)
+            if z is None:
+                assert hit[y, x] == -1 and (lv[y, x] == 0).all() and (ln[y, x] == 0).all()
+                continue
+            nA += z == 1.0
+            nB += z == 0.8
+            ref = _unproject(x + 0.5, y + 0.5, z)
+            np.testing.assert_allclose(lv[y, x], ref, atol=2e-6)
+            np.testing.assert_allclose(cp[y, x], ref, atol=2e-6)
+            np.testing.assert_allclose(np.abs(ln[y, x]), [0, 0, 1], atol=1e-6)
+            # the voxel of the canonical point, floor((p - origin) / s) in FP32
+            q = cp[y, x].astype(np.float32)
+            vox = np.floor((q - np.array(vp.origin[:], np.float32)) / np.float32(vp.voxel_size)).astype(int)
+            assert hit[y, x] == (vox[2] * 64 + vox[1]) * 64 + vox[0]
+    assert nA > 50 and nB > 50
+
+
+def test_raster_equal_depth_lowest_triangle_wins(orc):
+    vp = L.volume_params((8, 8, 8), 0.1, 0.3, (-0.4, -0.4, 0.6))
+    pose = np.eye(4, dtype=np.float32).reshape(16)
+    t = [_unproject(float(u), float(v), 1.0) for u, v in ((5, 5), (25, 5), (5, 25))]
+    verts = np.array(t + t, np.float32).reshape(-1)
+    canon = verts.copy()
+    canon[9:] += np.float32(0.05)  # the second copy's canonical points differ
+    lv, ln, hit, cp = orc.rasterize_f32(canon, verts, np.arange(6, dtype=np.int32), CAM, pose, vp)
+    cp = cp.reshape(H, W, 3)
+    np.testing.assert_allclose(cp[10, 10], _unproject(10.5, 10.5, 1.0), atol=2e-6)  # triangle 0, not 1
+
+
+def test_skinning_identity_and_rigid_node(orc):
+    """Identity transforms leave every vertex in place; one node moved by a
+    rigid transform moves every vertex whose voxel has that node as its only
+    neighbour by exactly that transform; vertices outside the volume stay."""
+    vp = L.volume_params((10, 10, 10), 0.05, 0.15, (-0.25, -0.25, 0.75))
+    rng = np.random.default_rng(5)
+    canon = rng.uniform(-0.3, 0.3, (200, 3)).astype(np.float32) + np.float32([0, 0, 1.0])
+    nodes = np.zeros(1, L.NODE)
+    nodes["pos"] = [0.0, 0.0, 1.0]
+    nodes["radius"] = 0.1
+    V = 1000
+    knn = np.full(V * 4, -1, np.int32)
+    knn[0::4] = 0
+    knn_w = np.zeros(V * 4, np.float32)
+    knn_w[0::4] = 1.0
+    T = np.zeros(1, L.DUALQUAT)
+    T["real"][0] = [0, 0, 0, 1]
+    np.testing.assert_array_equal(orc.skin_vertices_f32(vp, canon, nodes, T, knn, knn_w), canon)
+    # a rotation of 0.1 rad about z through the node, plus a shift
+    th = 0.1
+    Rz = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1.0]])
+    t = np.array([0.01, -0.02, 0.005])
+    T64 = np.zeros(8)
+    m = np.hstack([Rz, t.reshape(3, 1)])
+    orc.load().orc_mat_to_dq_f64(np.ascontiguousarray(m).ctypes.data, T64.ctypes.data)
+    T["real"][0], T["dual"][0] = T64[:4], T64[4:]
+    out = orc.skin_vertices_f32(vp, canon, nodes, T, knn, knn_w)
+    c = np.array([0, 0, 1.0])
+    inside = np.all((canon >= -0.25 + np.float32([0, 0, 1.0])) & (canon < 0.25 + np.float32([0, 0, 1.0])), axis=1)
+    # T applied about the node position (dq_centered): R (p - c) + c + t
+    ref = (canon.astype(np.float64) - c) @ Rz.T + c + t
+    np.testing.assert_allclose(out[inside], ref[inside], atol=2e-6)
+    np.testing.assert_array_equal(out[~inside], canon[~inside])
+    assert inside.sum() > 20 and (~inside).sum() > 20
