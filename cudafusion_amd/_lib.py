@@ -47,6 +47,26 @@ def assoc_params(dist_threshold=0.05, angle_threshold=0.7, view_threshold=0.2, s
     return AssocParams(dist_threshold, angle_threshold, view_threshold, int(search_radius_px))
 
 
+TRACK_MAX_GN = 16
+
+
+class TrackParams(C.Structure):
+    _fields_ = [("assoc", AssocParams), ("min_valid_corrs", C.c_int32), ("line_search", C.c_int32)]
+
+
+def track_params(assoc=None, min_valid_corrs=2000, line_search=True) -> "TrackParams":
+    """Defaults are the reference pipeline's (include/fusion_app.h:79-83;
+    the line search of :494-641 on)."""
+    return TrackParams(assoc if assoc is not None else assoc_params(), int(min_valid_corrs), int(bool(line_search)))
+
+
+class TrackStats(C.Structure):
+    _fields_ = [("gn_run", C.c_int32), ("gn_applied", C.c_int32), ("stop_reason", C.c_int32),
+                ("last_valid", C.c_int32), ("num_valid", C.c_int32 * TRACK_MAX_GN),
+                ("best_scale", C.c_float * TRACK_MAX_GN), ("base_rmse", C.c_double * TRACK_MAX_GN),
+                ("best_rmse", C.c_double * TRACK_MAX_GN)]
+
+
 class SolverParams(C.Structure):
     _fields_ = [("gn_iterations", C.c_int32), ("pcg_iterations", C.c_int32), ("pcg_tolerance", C.c_float),
                 ("lambda_smooth", C.c_float), ("lambda_rot", C.c_float), ("lambda_damping", C.c_float),
@@ -157,6 +177,10 @@ _SIGS = {
     "nrf_pipeline_profile_pcg": (I, [P, I]),
     "nrf_pipeline_profile_read": (I, [P, I, C.POINTER(F)]),
     "nrf_pipeline_updated_voxels": (C.c_longlong, [P]),
+    "nrf_pipeline_track": (I, [P, P, P, C.POINTER(TrackParams), I, P]),
+    "nrf_pipeline_track_stats": (I, [P, C.POINTER(TrackStats)]),
+    "nrf_pipeline_track_maps": (I, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P),
+                                    C.POINTER(P)]),
     "nrf_batch_create": (P, [C.POINTER(P), I, C.POINTER(BatchTuning), I]),
     "nrf_batch_destroy": (None, [P]),
     "nrf_batch_set_tuning": (I, [P, C.POINTER(BatchTuning)]),

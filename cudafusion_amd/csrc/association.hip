@@ -93,9 +93,16 @@ struct AssocArgs {
   nrf_assoc_params ap;
   nrf_corr *corrs;
   int32_t *num_valid, *stats;
+  // the pixel 4-NN read through the hit voxel (the pixel kNN step fused;
+  // used when pknn is null and hit is set)
+  const int32_t *hit;
+  const int4 *vknn;
+  const float4 *vw;
+  const int32_t *skip;  // optional: nonzero = leave every output as it is
 };
 
 __global__ __launch_bounds__(kTx *kTy) void find_corrs_kernel(AssocArgs a) {
+  if (a.skip && *a.skip) return;
   const int u = blockIdx.x * kTx + (threadIdx.x & (kTx - 1));
   const int v = blockIdx.y * kTy + threadIdx.x / kTx;
   const int W = a.cam.width, H = a.cam.height;
@@ -176,9 +183,22 @@ __global__ __launch_bounds__(kTx *kTy) void find_corrs_kernel(AssocArgs a) {
       st3(c.normal, normalize3(xf_normal(a.Twc, best_n)));
       c.weight = best_weight;
       c.valid = 1;
+      int4 id = make_int4(-1, -1, -1, -1);
+      float4 wk = make_float4(0.f, 0.f, 0.f, 0.f);
+      bool have = false;
       if (a.pknn) {
-        const int4 id = a.pknn[px];
-        const float4 wk = a.pw[px];
+        id = a.pknn[px];
+        wk = a.pw[px];
+        have = true;
+      } else if (a.hit) {
+        const int vidx = a.hit[px];
+        if (vidx >= 0) {
+          id = a.vknn[vidx];
+          wk = a.vw[vidx];
+        }
+        have = true;
+      }
+      if (have) {
         c.node_ids[0] = id.x; c.node_ids[1] = id.y; c.node_ids[2] = id.z; c.node_ids[3] = id.w;
         c.node_ws[0] = wk.x; c.node_ws[1] = wk.y; c.node_ws[2] = wk.z; c.node_ws[3] = wk.w;
       }
@@ -241,13 +261,15 @@ extern "C" int nrf_pixel_knn(const int32_t *hit_voxel, int num_pixels, const int
   return NRF_OK;
 }
 
-extern "C" int nrf_find_correspondences(const float *live_vertices, const float *live_normals, const float *depth,
-                                        const float *depth_normals, const nrf_camera *cam,
-                                        const float camera_pose[16], const int32_t *pixel_knn,
-                                        const float *pixel_knn_w, const nrf_assoc_params *params, nrf_corr *corrs,
-                                        int32_t *num_valid, int32_t *stats, void *stream) {
+namespace nrf {
+int find_correspondences(const float *live_vertices, const float *live_normals, const float *depth,
+                         const float *depth_normals, const nrf_camera *cam, const float camera_pose[16],
+                         const int32_t *pixel_knn, const float *pixel_knn_w, const int32_t *hit_voxel,
+                         const int32_t *voxel_knn, const float *voxel_knn_w, const nrf_assoc_params *params,
+                         nrf_corr *corrs, int32_t *num_valid, int32_t *stats, const int32_t *skip, hipStream_t st) {
   if (!live_vertices || !live_normals || !depth || !depth_normals || !cam || !camera_pose || !params || !corrs ||
-      !num_valid || cam->width <= 0 || cam->height <= 0 || (!pixel_knn != !pixel_knn_w))
+      !num_valid || cam->width <= 0 || cam->height <= 0 || (!pixel_knn != !pixel_knn_w) ||
+      (hit_voxel && (!voxel_knn || !voxel_knn_w)))
     return NRF_ERR_INVALID_ARGUMENT;
   AssocArgs a;
   a.live_v = live_vertices;
@@ -264,10 +286,25 @@ extern "C" int nrf_find_correspondences(const float *live_vertices, const float 
   a.corrs = corrs;
   a.num_valid = num_valid;
   a.stats = stats;
+  a.hit = pixel_knn ? nullptr : hit_voxel;
+  a.vknn = reinterpret_cast<const int4 *>(voxel_knn);
+  a.vw = reinterpret_cast<const float4 *>(voxel_knn_w);
+  a.skip = skip;
   const dim3 grid((cam->width + kTx - 1) / kTx, (cam->height + kTy - 1) / kTy);
-  find_corrs_kernel<<<grid, kTx * kTy, 0, as_stream(stream)>>>(a);
+  find_corrs_kernel<<<grid, kTx * kTy, 0, st>>>(a);
   NRF_LAUNCH_CHECK();
   return NRF_OK;
+}
+}  // namespace nrf
+
+extern "C" int nrf_find_correspondences(const float *live_vertices, const float *live_normals, const float *depth,
+                                        const float *depth_normals, const nrf_camera *cam,
+                                        const float camera_pose[16], const int32_t *pixel_knn,
+                                        const float *pixel_knn_w, const nrf_assoc_params *params, nrf_corr *corrs,
+                                        int32_t *num_valid, int32_t *stats, void *stream) {
+  return find_correspondences(live_vertices, live_normals, depth, depth_normals, cam, camera_pose, pixel_knn,
+                              pixel_knn_w, nullptr, nullptr, nullptr, params, corrs, num_valid, stats, nullptr,
+                              as_stream(stream));
 }
 
 extern "C" int nrf_mark_optimized_voxels(const nrf_corr *corrs, const int32_t *hit_voxel, int num_pixels,

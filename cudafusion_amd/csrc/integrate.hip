@@ -432,7 +432,7 @@ int tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels, const float *
 int tsdf16_integrate(const nrf_volume_params *vp, uint16_t *tsdf16, uint8_t *weight8, const float *depth,
                      const nrf_camera *cam, const float camera_pose[16], const nrf_dualquat *centered, int num_nodes,
                      const uint16_t *knn16, const uint16_t *knn_w16, const uint32_t *knn_rows,
-                     int32_t *updated_count, hipStream_t st) {
+                     int32_t *updated_count, hipStream_t st, const int32_t *voxel_opt_counts, int min_opt_count) {
   if (!volume_ok(vp) || !tsdf16 || !weight8 || !depth || !cam || !camera_pose) return NRF_ERR_INVALID_ARGUMENT;
   if (cam->width <= 0 || cam->height <= 0) return NRF_ERR_INVALID_ARGUMENT;
   if (num_nodes > 0 && (!centered || !knn16 || !knn_w16)) return NRF_ERR_INVALID_ARGUMENT;
@@ -456,8 +456,8 @@ int tsdf16_integrate(const nrf_volume_params *vp, uint16_t *tsdf16, uint8_t *wei
   a.knn_w = nullptr;
   a.knn_rows = knn_rows;
   a.nsx = (a.X + kTileX - 1) / kTileX;
-  a.opt = nullptr;
-  a.min_opt = 0;
+  a.opt = voxel_opt_counts;
+  a.min_opt = min_opt_count;
   a.decay = vp->decay_alpha;
   a.wmax = vp->max_weight;
   a.updated = updated_count;

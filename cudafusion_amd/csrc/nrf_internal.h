@@ -127,19 +127,79 @@ int integrate_batch_run(IntegrateBatch *b, const BatchDepth &fr, hipStream_t st)
 // voxels all integrate unwarped.  Valid until the kNN changes.
 int64_t knn_row_words(const nrf_volume_params *vp);
 int knn_row_flags(const nrf_volume_params *vp, const int32_t *knn, uint32_t *rows, hipStream_t st);
+// nrf_solver_solve_gn; begin = false continues the solve of the previous
+// call (no reset of its stop flag and record: a rejected step stops the
+// later calls too, and the record sums them)
+int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical, int num_corrs,
+                    const nrf_node *nodes, nrf_dualquat *transforms, nrf_dualquat *centered, int num_nodes,
+                    const nrf_gn_params *gn, float *delta_x, nrf_pcg_result *result, hipStream_t st, bool begin);
+// The start of a solve's record (clears the stop flag and the device
+// result, folding the previous solve into the totals).
+int solver_gn_begin(nrf_solver *s, hipStream_t st);
+// The device flag that turns every later kernel of the solve into a no-op
+// (raised by a rejected step; the tracking loop raises it too).  Written
+// only by single-workgroup kernels, so that no multi-workgroup kernel sees it
+// change while it runs.
+int32_t *solver_stop_flag(nrf_solver *s);
+nrf_pcg_result *solver_result_device(nrf_solver *s);
+// One GN linearisation without the update: memberships, assembly of the
+// stored corrs[i].src (no canonical points), PCG with the configured
+// refinement or true-residual rounds (the fused refinement-update form runs
+// unfused), and with gn->validate the step validation (a rejected step
+// raises the stop flag).  delta_x receives the step.
+int solver_gn_linearise(nrf_solver *s, const nrf_corr *corrs, int num_corrs, const nrf_node *nodes,
+                        const nrf_dualquat *transforms, const nrf_dualquat *centered, int num_nodes,
+                        const nrf_gn_params *gn, float *delta_x, hipStream_t st);
 // live surface prediction (raster.hip): nrf_skin_vertices and
-// nrf_rasterize_mesh with the depth/triangle key buffer (8 B per pixel)
+// nrf_rasterize_mesh with the depth/triangle key buffer (8 B per pixel).
+// d_ntri (optional): the triangle count of a triangle-soup mesh on the
+// device (vertices 3 per triangle), capped by num_vertices / 3 resp.
+// num_triangles, so that no host read of the count is needed.
 int skin_vertices(const float *canonical, int n, const nrf_volume_params *vp, const nrf_dualquat *centered,
-                  int num_nodes, const int32_t *voxel_knn, const float *voxel_knn_w, float *out, hipStream_t st);
+                  int num_nodes, const int32_t *voxel_knn, const float *voxel_knn_w, float *out, hipStream_t st,
+                  const int32_t *d_ntri = nullptr, const int32_t *skip = nullptr);
 int rasterize_mesh(const float *canonical, const float *warped, const int32_t *triangles, int num_vertices,
                    int num_triangles, const nrf_camera *cam, const float camera_pose[16], const nrf_volume_params *vp,
                    unsigned long long *ztri, float *live_vertices, float *live_normals, int32_t *hit_voxel,
-                   float *canonical_points, hipStream_t st);
-// nrf_tsdf16_integrate with the kNN's row occupancy (null: none)
+                   float *canonical_points, hipStream_t st, const int32_t *d_ntri = nullptr,
+                   const int32_t *skip = nullptr);
+// nrf_find_correspondences; with pixel_knn NULL and hit_voxel set, the
+// pixel 4-NN is read through the hit voxel (the pixel kNN step fused).
+// skip (optional, device): nonzero leaves every output as it is.
+int find_correspondences(const float *live_vertices, const float *live_normals, const float *depth,
+                         const float *depth_normals, const nrf_camera *cam, const float camera_pose[16],
+                         const int32_t *pixel_knn, const float *pixel_knn_w, const int32_t *hit_voxel,
+                         const int32_t *voxel_knn, const float *voxel_knn_w, const nrf_assoc_params *params,
+                         nrf_corr *corrs, int32_t *num_valid, int32_t *stats, const int32_t *skip, hipStream_t st);
+// tracking-loop control (track.hip): the device record of a tracking frame
+// and its decisions, each gated by the solver's stop flag
+struct TrackDev {
+  nrf_track_stats stats;
+  int32_t counter;  // valid correspondences of the running association
+  float best_scale;
+  double best_rmse;
+};
+int track_begin(TrackDev *t, hipStream_t st);
+int track_count_reset(TrackDev *t, const int32_t *stop, hipStream_t st);
+int track_gate(TrackDev *t, int g, int min_valid, int32_t *stop, hipStream_t st);
+int track_rmse_blocks(int n);
+// judge = false: the base RMSE of iteration g; true: the trial test of scale
+int track_rmse(TrackDev *t, const nrf_corr *corrs, int n, double *part, int g, bool judge, float scale,
+               int min_valid, const int32_t *stop, hipStream_t st);
+int track_ls_begin(TrackDev *t, int g, float update_scale, const nrf_dualquat *T, nrf_dualquat *Tsave, int N,
+                   int32_t *stop, hipStream_t st);
+int track_ls_apply(TrackDev *t, const float *dx, const nrf_dualquat *Tsave, nrf_dualquat *T, nrf_dualquat *centered,
+                   const nrf_node *nodes, int N, float scale, bool use_best, float max_rot, float max_trans,
+                   const int32_t *stop, hipStream_t st);
+int track_ls_commit(TrackDev *t, int g, int32_t *stop, hipStream_t st);
+int track_end(TrackDev *t, const nrf_pcg_result *res, bool line_search, const int32_t *stop, hipStream_t st);
+// nrf_tsdf16_integrate with the kNN's row occupancy (null: none) and the
+// optimised-voxel gate of nrf_tsdf_integrate (warped only)
 int tsdf16_integrate(const nrf_volume_params *vp, uint16_t *tsdf16, uint8_t *weight8, const float *depth,
                      const nrf_camera *cam, const float camera_pose[16], const nrf_dualquat *centered, int num_nodes,
                      const uint16_t *knn16, const uint16_t *knn_w16, const uint32_t *knn_rows,
-                     int32_t *updated_count, hipStream_t st);
+                     int32_t *updated_count, hipStream_t st, const int32_t *voxel_opt_counts = nullptr,
+                     int min_opt_count = 0);
 // nrf_tsdf_integrate with the kNN's row occupancy (null: none)
 int tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels, const float *depth, const nrf_camera *cam,
                    const float camera_pose[16], const nrf_node *nodes, const nrf_dualquat *transforms, int num_nodes,

@@ -246,6 +246,35 @@ __device__ inline bool dq_blend4(const DQ *table, const int ids[4], const float 
   return true;
 }
 
+// One node's step of the caller: the per-node clamp, then the scale
+// (upload_scaled_delta_x, include/fusion_app.h:1353-1375), then
+// apply_twist_kernel with update_scale 1 (src/warp_field.cu:250-291), which
+// clamps the scaled twist again and composes dq(exp(twist)) on the left.
+__host__ __device__ inline DQ scaled_twist_update(const float d[6], float scale, float max_rot, float max_trans,
+                                                  const DQ &T) {
+  float tw[6];
+  for (int k = 0; k < 6; ++k) tw[k] = d[k];
+  const float rn = sqrtf(tw[0] * tw[0] + tw[1] * tw[1] + tw[2] * tw[2]);
+  const float tn = sqrtf(tw[3] * tw[3] + tw[4] * tw[4] + tw[5] * tw[5]);
+  float rsc = 1.0f, tsc = 1.0f;
+  if (max_rot > 0.0f && rn > max_rot) rsc = max_rot / rn;
+  if (max_trans > 0.0f && tn > max_trans) tsc = max_trans / tn;
+  for (int k = 0; k < 3; ++k) tw[k] *= rsc * scale;
+  for (int k = 3; k < 6; ++k) tw[k] *= tsc * scale;
+  for (int k = 0; k < 6; ++k) tw[k] = 1.0f * tw[k];
+  const float rn2 = sqrtf(tw[0] * tw[0] + tw[1] * tw[1] + tw[2] * tw[2]);
+  if (max_rot > 0.0f && rn2 > max_rot) {
+    const float f = max_rot / rn2;
+    tw[0] *= f; tw[1] *= f; tw[2] *= f;
+  }
+  const float tn2 = sqrtf(tw[3] * tw[3] + tw[4] * tw[4] + tw[5] * tw[5]);
+  if (max_trans > 0.0f && tn2 > max_trans) {
+    const float f = max_trans / tn2;
+    tw[3] *= f; tw[4] *= f; tw[5] *= f;
+  }
+  return dq_mul(mat_to_dq(exp_se3(tw)), T);
+}
+
 // Huber weight, kernels/solver_kernels.cu:13-24.
 __host__ __device__ inline float huber_w(float r, float delta) {
   float a = fabsf(r);

@@ -53,11 +53,24 @@ struct nrf_pipeline {
   unsigned g_gen = 0;
   hipStream_t cap = nullptr;  // capture stream (the caller's may be the null stream)
   bool graph_step = false;    // the last step was a graph replay (stage events: whole step only)
+  bool mesh_valid = false;    // verts / tris / counts hold the marching cubes of the current volume
+  // tracking frame buffers (allocated by the first nrf_pipeline_track)
+  struct {
+    float *dnorm = nullptr, *warped = nullptr, *live_v = nullptr, *live_n = nullptr;
+    unsigned long long *ztri = nullptr;
+    int32_t *hit = nullptr, *opt = nullptr, *opt_frame = nullptr;
+    nrf_corr *corrs = nullptr;
+    double *part = nullptr;
+    nrf_dualquat *Tsave = nullptr;
+    TrackDev *dev = nullptr;
+  } tk;
 };
 
 static void pipeline_free(nrf_pipeline *p) {
   void *ptrs[] = {p->vox, p->knn, p->knn_w, p->knn_rows, p->nodes, p->T, p->centered, p->delta_x, p->mc_ws,
-                  p->verts, p->tris, p->counts, p->updated, p->t16, p->w8, p->k16, p->kw16};
+                  p->verts, p->tris, p->counts, p->updated, p->t16, p->w8, p->k16, p->kw16,
+                  p->tk.dnorm, p->tk.warped, p->tk.live_v, p->tk.live_n, p->tk.ztri, p->tk.hit, p->tk.opt,
+                  p->tk.opt_frame, p->tk.corrs, p->tk.part, p->tk.Tsave, p->tk.dev};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (auto &e : p->ev)
@@ -149,6 +162,7 @@ extern "C" int nrf_pipeline_volume16(nrf_pipeline *p, uint16_t **tsdf16, uint8_t
 extern "C" int nrf_pipeline_integrate_rigid(nrf_pipeline *p, const float *depth, const float camera_pose[16],
                                             void *stream) {
   if (!p || !depth || !camera_pose) return NRF_ERR_INVALID_ARGUMENT;
+  p->mesh_valid = false;
   return nrf_tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, nullptr, nullptr, 0,
                             nullptr, nullptr, nullptr, nullptr, 0, nullptr, stream);
 }
@@ -190,6 +204,19 @@ extern "C" int nrf_pipeline_set_nodes(nrf_pipeline *p, const nrf_node *host_node
   return NRF_OK;
 }
 
+// the marching cubes of the current volume into the pipeline's mesh
+static int pipeline_mc(nrf_pipeline *p, hipStream_t st) {
+  int rc;
+  if (p->cfg.storage16)
+    rc = nrf_mc16_extract(&p->cfg.volume, p->t16, p->w8, p->mc_ws, p->mc_bytes, p->verts, p->tris,
+                          p->cfg.max_triangles, p->counts, st);
+  else
+    rc = nrf_mc_extract(&p->cfg.volume, p->vox, p->mc_ws, p->mc_bytes, p->verts, nullptr, p->tris,
+                        p->cfg.max_triangles, p->counts, st);
+  if (rc == NRF_OK) p->mesh_valid = true;
+  return rc;
+}
+
 // `events`: record the stage events (not inside a capture).
 static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float camera_pose[16],
                             const nrf_corr *corrs, const float *canonical, int num_corrs, void *stream,
@@ -222,13 +249,7 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
   if (rc) return rc;
   if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[2], st));
   if (profile && !p->prof_pcg) NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 2], st));
-  if (p->cfg.storage16)
-    rc = nrf_mc16_extract(&p->cfg.volume, p->t16, p->w8, p->mc_ws, p->mc_bytes, p->verts, p->tris,
-                          p->cfg.max_triangles, p->counts, stream);
-  else
-    rc = nrf_mc_extract(&p->cfg.volume, p->vox, p->mc_ws, p->mc_bytes, p->verts, nullptr, p->tris,
-                        p->cfg.max_triangles, p->counts, stream);
-  if (rc) return rc;
+  if ((rc = pipeline_mc(p, st))) return rc;
   if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[3], st));
   if (profile) {
     if (!p->prof_pcg) NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 1], st));
@@ -349,6 +370,152 @@ extern "C" int nrf_pipeline_profile_read(nrf_pipeline *p, int step_back, float o
   }
   if ((rc = ms(prev, p->prof_per - 2, out_ms[4]))) return rc;
   if ((rc = ms(p->prof_per - 2, p->prof_per - 1, out_ms[5]))) return rc;
+  return NRF_OK;
+}
+
+// ----------------------------------------------------------- tracking frame
+
+static int track_alloc(nrf_pipeline *p) {
+  const nrf_camera &cam = p->cfg.camera;
+  const size_t npx = (size_t)cam.width * cam.height;
+  const size_t V = (size_t)volume_count(&p->cfg.volume);
+  const size_t T = (size_t)std::max(1, p->cfg.max_triangles);
+  auto &k = p->tk;
+  NRF_HIP_CHECK(hipMalloc(&k.dnorm, npx * 3 * sizeof(float)));
+  NRF_HIP_CHECK(hipMalloc(&k.warped, T * 9 * sizeof(float)));
+  NRF_HIP_CHECK(hipMalloc(&k.live_v, npx * 3 * sizeof(float)));
+  NRF_HIP_CHECK(hipMalloc(&k.live_n, npx * 3 * sizeof(float)));
+  NRF_HIP_CHECK(hipMalloc(&k.ztri, npx * sizeof(unsigned long long)));
+  NRF_HIP_CHECK(hipMalloc(&k.hit, npx * sizeof(int32_t)));
+  NRF_HIP_CHECK(hipMalloc(&k.corrs, npx * sizeof(nrf_corr)));
+  NRF_HIP_CHECK(hipMalloc(&k.part, (size_t)track_rmse_blocks((int)npx) * 2 * sizeof(double)));
+  NRF_HIP_CHECK(hipMalloc(&k.Tsave, (size_t)p->cfg.max_nodes * sizeof(nrf_dualquat)));
+  NRF_HIP_CHECK(hipMalloc(&k.dev, sizeof(TrackDev)));
+  NRF_HIP_CHECK(hipMalloc(&k.opt, V * sizeof(int32_t)));
+  NRF_HIP_CHECK(hipMalloc(&k.opt_frame, V * sizeof(int32_t)));
+  // d_voxel_opt_counts_ / d_voxel_opt_frame_ start at zero (fusion_app.h:143-146)
+  NRF_HIP_CHECK(hipMemset(k.opt, 0, V * sizeof(int32_t)));
+  NRF_HIP_CHECK(hipMemset(k.opt_frame, 0, V * sizeof(int32_t)));
+  NRF_HIP_CHECK(hipMemset(k.dev, 0, sizeof(TrackDev)));
+  NRF_HIP_CHECK(hipMemset(k.corrs, 0, npx * sizeof(nrf_corr)));
+  NRF_HIP_CHECK(hipMemset(k.hit, 0xff, npx * sizeof(int32_t)));
+  int rc = solver_reserve(p->solver, std::max(p->cfg.max_corrs, (int)npx));
+  if (rc) return rc;
+  NRF_HIP_CHECK(hipDeviceSynchronize());
+  return NRF_OK;
+}
+
+// live surface prediction + association at the current transforms, each
+// kernel gated by the stop flag
+static int track_associate(nrf_pipeline *p, const float *depth, const float camera_pose[16],
+                           const nrf_assoc_params *ap, const int32_t *stop, hipStream_t st) {
+  auto &k = p->tk;
+  const nrf_camera *cam = &p->cfg.camera;
+  const int maxT = std::max(1, p->cfg.max_triangles);
+  const int32_t *ntri = p->counts + 1;
+  int rc;
+  if ((rc = skin_vertices(p->verts, 3 * maxT, &p->cfg.volume, p->centered, p->N, p->knn, p->knn_w, k.warped, st,
+                          ntri, stop)))
+    return rc;
+  if ((rc = rasterize_mesh(p->verts, k.warped, p->tris, 3 * maxT, maxT, cam, camera_pose, &p->cfg.volume, k.ztri,
+                           k.live_v, k.live_n, k.hit, nullptr, st, ntri, stop)))
+    return rc;
+  if ((rc = track_count_reset(k.dev, stop, st))) return rc;
+  return find_correspondences(k.live_v, k.live_n, depth, k.dnorm, cam, camera_pose, nullptr, nullptr, k.hit, p->knn,
+                              p->knn_w, ap, k.corrs, &k.dev->counter, nullptr, stop, st);
+}
+
+extern "C" int nrf_pipeline_track(nrf_pipeline *p, const float *depth, const float camera_pose[16],
+                                  const nrf_track_params *tp, int frame_id, void *stream) {
+  if (!p || !depth || !camera_pose || !tp) return NRF_ERR_INVALID_ARGUMENT;
+  if (p->N <= 0) return NRF_ERR_NOT_READY;
+  const nrf_gn_params &gn = p->cfg.gn;
+  const int G = std::max(1, gn.gn_iterations);
+  if (G > NRF_TRACK_MAX_GN || p->cfg.camera.width <= 0 || p->cfg.camera.height <= 0)
+    return NRF_ERR_INVALID_ARGUMENT;
+  int rc;
+  if (!p->tk.dev && (rc = track_alloc(p))) return rc;
+  hipStream_t st = as_stream(stream);
+  auto &k = p->tk;
+  const nrf_camera *cam = &p->cfg.camera;
+  const int npx = cam->width * cam->height;
+  const bool ls = tp->line_search != 0;
+  p->graph_step = false;
+  if (!p->mesh_valid && (rc = pipeline_mc(p, st))) return rc;
+  if ((rc = nrf_depth_normals(depth, cam, k.dnorm, st))) return rc;
+  if ((rc = solver_gn_begin(p->solver, st))) return rc;
+  int32_t *stop = solver_stop_flag(p->solver);
+  if ((rc = track_begin(k.dev, st))) return rc;
+  NRF_HIP_CHECK(hipEventRecord(p->ev[0], st));
+  nrf_gn_params g1 = gn;
+  g1.gn_iterations = 1;
+  // fusion_app.h:500-511, the products in FP32 as there
+  const float us = gn.update_scale;
+  const float trials[10] = {us, 0.5f * us, 0.25f * us, 0.125f * us, 0.0625f * us,
+                            -0.0625f * us, -0.125f * us, -0.25f * us, -0.5f * us, -us};
+  for (int g = 0; g < G; ++g) {
+    if ((rc = track_associate(p, depth, camera_pose, &tp->assoc, stop, st))) return rc;
+    if ((rc = track_gate(k.dev, g, tp->min_valid_corrs, stop, st))) return rc;
+    if (!ls) {
+      if ((rc = solver_solve_gn(p->solver, k.corrs, nullptr, npx, p->nodes, p->T, p->centered, p->N, &g1,
+                                p->delta_x, nullptr, st, false)))
+        return rc;
+      continue;
+    }
+    if ((rc = track_rmse(k.dev, k.corrs, npx, k.part, g, false, 0.f, 0, stop, st))) return rc;
+    if ((rc = solver_gn_linearise(p->solver, k.corrs, npx, p->nodes, p->T, p->centered, p->N, &gn, p->delta_x, st)))
+      return rc;
+    if ((rc = track_ls_begin(k.dev, g, us, p->T, k.Tsave, p->N, stop, st))) return rc;
+    for (float s : trials) {
+      if ((rc = track_ls_apply(k.dev, p->delta_x, k.Tsave, p->T, p->centered, p->nodes, p->N, s, false,
+                               gn.max_update_rot, gn.max_update_trans, stop, st)))
+        return rc;
+      if ((rc = track_associate(p, depth, camera_pose, &tp->assoc, stop, st))) return rc;
+      if ((rc = track_rmse(k.dev, k.corrs, npx, k.part, g, true, s, tp->min_valid_corrs, stop, st))) return rc;
+    }
+    if ((rc = track_ls_apply(k.dev, p->delta_x, k.Tsave, p->T, p->centered, p->nodes, p->N, 0.f, true,
+                             gn.max_update_rot, gn.max_update_trans, stop, st)))
+      return rc;
+    if ((rc = track_ls_commit(k.dev, g, stop, st))) return rc;
+    // the association after the applied step (:636-639); the next
+    // iteration's first association would repeat it on the same transforms
+    if (g == G - 1 && (rc = track_associate(p, depth, camera_pose, &tp->assoc, stop, st))) return rc;
+  }
+  if ((rc = track_end(k.dev, solver_result_device(p->solver), ls, stop, st))) return rc;
+  NRF_HIP_CHECK(hipEventRecord(p->ev[1], st));
+  if ((rc = nrf_mark_optimized_voxels(k.corrs, k.hit, npx, &p->cfg.volume, frame_id, k.opt, k.opt_frame, st)))
+    return rc;
+  if (p->cfg.storage16)
+    rc = tsdf16_integrate(&p->cfg.volume, p->t16, p->w8, depth, cam, camera_pose, p->centered, p->N, p->k16, p->kw16,
+                          p->knn_rows, p->updated, st, k.opt, p->cfg.min_opt_count);
+  else
+    rc = tsdf_integrate(&p->cfg.volume, p->vox, depth, cam, camera_pose, p->nodes, p->T, p->N, p->centered, p->knn,
+                        p->knn_w, p->knn_rows, k.opt, p->cfg.min_opt_count, p->updated, st);
+  if (rc) return rc;
+  NRF_HIP_CHECK(hipEventRecord(p->ev[2], st));
+  if ((rc = pipeline_mc(p, st))) return rc;
+  NRF_HIP_CHECK(hipEventRecord(p->ev[3], st));
+  return NRF_OK;
+}
+
+extern "C" int nrf_pipeline_track_stats(nrf_pipeline *p, nrf_track_stats *out) {
+  if (!p || !out) return NRF_ERR_INVALID_ARGUMENT;
+  if (!p->tk.dev) return NRF_ERR_NOT_READY;
+  NRF_HIP_CHECK(hipDeviceSynchronize());
+  NRF_HIP_CHECK(hipMemcpy(out, &p->tk.dev->stats, sizeof(nrf_track_stats), hipMemcpyDeviceToHost));
+  return NRF_OK;
+}
+
+extern "C" int nrf_pipeline_track_maps(nrf_pipeline *p, float **live_vertices, float **live_normals,
+                                       int32_t **hit_voxel, nrf_corr **corrs, int32_t **voxel_opt_counts,
+                                       float **depth_normals) {
+  if (!p) return NRF_ERR_INVALID_ARGUMENT;
+  if (live_vertices) *live_vertices = p->tk.live_v;
+  if (live_normals) *live_normals = p->tk.live_n;
+  if (hit_voxel) *hit_voxel = p->tk.hit;
+  if (corrs) *corrs = p->tk.corrs;
+  if (voxel_opt_counts) *voxel_opt_counts = p->tk.opt;
+  if (depth_normals) *depth_normals = p->tk.dnorm;
   return NRF_OK;
 }
 

@@ -11,6 +11,8 @@
 // interpolated live vertex, face normal and canonical point.  One lane per
 // triangle in the first pass (the full-scale mesh has ~180k triangles of a
 // few pixels each); FP32 in the reference's operation order.
+#include <algorithm>
+
 #include "nrf_internal.h"
 #include "nrf_math.h"
 
@@ -47,6 +49,7 @@ struct RasterArgs {
   const float *canon, *warped;
   const int32_t *tris;
   int nv, nt;
+  const int32_t *d_nt;  // optional: the triangle count on the device (at most nt; vertices 3 per triangle)
   nrf_camera cam;
   CamXf Tcw;
   float3 org;
@@ -55,6 +58,7 @@ struct RasterArgs {
   unsigned long long *ztri;
   float *live_v, *live_n, *canon_pt;
   int32_t *hit;
+  const int32_t *skip;  // optional: nonzero = leave every output as it is
 };
 
 // project_triangle (src/mesh_raster_kernels.cu:44-72)
@@ -74,39 +78,48 @@ __device__ inline bool project_tri(const RasterArgs &a, const int ids[3], float3
   return fabsf(area) >= 1e-6f;
 }
 
+__device__ inline int tri_count(const RasterArgs &a) { return a.d_nt ? min(max(*a.d_nt, 0), a.nt) : a.nt; }
+__device__ inline int vert_count(const RasterArgs &a) { return a.d_nt ? 3 * tri_count(a) : a.nv; }
 __device__ inline bool tri_ids(const RasterArgs &a, int t, int ids[3]) {
+  const int nv = vert_count(a);
   for (int k = 0; k < 3; ++k) ids[k] = a.tris[3 * (size_t)t + k];
-  return !(ids[0] < 0 || ids[1] < 0 || ids[2] < 0 || ids[0] >= a.nv || ids[1] >= a.nv || ids[2] >= a.nv);
+  return !(ids[0] < 0 || ids[1] < 0 || ids[2] < 0 || ids[0] >= nv || ids[1] >= nv || ids[2] >= nv);
 }
 
 // skin_vertices_from_voxel_knn_kernel (src/skinning_kernels.cu:29-63): the
 // vertex warped through the 4-NN of the voxel containing it (the centred
 // node table gives the same blend as warp_point), unchanged outside the
 // volume or without nodes.
+// Grid-stride over the vertices, so that a launch sized for a capacity
+// serves a count held on the device.
 __global__ __launch_bounds__(256) void skin_kernel(const float *__restrict__ canon, int n, float3 org, float s,
                                                    int X, int Y, int Z, const DQ *__restrict__ centered, int N,
                                                    const int4 *__restrict__ knn, const float4 *__restrict__ knn_w,
-                                                   float *__restrict__ out) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  float3 p = ld3(canon + 3 * (size_t)i);
-  if (N > 0 && centered) {
-    const int v = voxel_of(p, org, s, X, Y, Z);
-    if (v >= 0) {
-      const int4 i4 = knn[v];
-      const float4 w4 = knn_w[v];
-      int ids[4] = {i4.x, i4.y, i4.z, i4.w};
-      const float ws[4] = {w4.x, w4.y, w4.z, w4.w};
-      for (int k = 0; k < 4; ++k)
-        if (ids[k] >= N) ids[k] = -1;
-      DQ b;
-      if (dq_blend4(centered, ids, ws, b)) p = dq_transform(b, p);
+                                                   float *__restrict__ out, const int32_t *__restrict__ d_ntri,
+                                                   const int32_t *__restrict__ skip) {
+  if (skip && *skip) return;
+  const int cnt = d_ntri ? 3 * min(max(*d_ntri, 0), n / 3) : n;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < cnt; i += gridDim.x * 256) {
+    float3 p = ld3(canon + 3 * (size_t)i);
+    if (N > 0 && centered) {
+      const int v = voxel_of(p, org, s, X, Y, Z);
+      if (v >= 0) {
+        const int4 i4 = knn[v];
+        const float4 w4 = knn_w[v];
+        int ids[4] = {i4.x, i4.y, i4.z, i4.w};
+        const float ws[4] = {w4.x, w4.y, w4.z, w4.w};
+        for (int k = 0; k < 4; ++k)
+          if (ids[k] >= N) ids[k] = -1;
+        DQ b;
+        if (dq_blend4(centered, ids, ws, b)) p = dq_transform(b, p);
+      }
     }
+    st3(out + 3 * (size_t)i, p);
   }
-  st3(out + 3 * (size_t)i, p);
 }
 
 __global__ __launch_bounds__(256) void raster_reset_kernel(RasterArgs a) {
+  if (a.skip && *a.skip) return;
   const int px = blockIdx.x * 256 + threadIdx.x;
   if (px >= a.cam.width * a.cam.height) return;
   a.ztri[px] = ~0ull;
@@ -117,9 +130,7 @@ __global__ __launch_bounds__(256) void raster_reset_kernel(RasterArgs a) {
 }
 
 // rasterize_triangles_z_kernel (src/mesh_raster_kernels.cu:76-137)
-__global__ __launch_bounds__(256) void raster_z_kernel(RasterArgs a) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= a.nt) return;
+__device__ inline void raster_z_tri(const RasterArgs &a, int t) {
   int ids[3];
   if (!tri_ids(a, t, ids)) return;
   float3 wc[3], cc[3];
@@ -146,10 +157,17 @@ __global__ __launch_bounds__(256) void raster_z_kernel(RasterArgs a) {
       atomicMin(&a.ztri[(size_t)y * a.cam.width + x], key);
     }
 }
+// grid-stride over the triangles (see skin_kernel)
+__global__ __launch_bounds__(256) void raster_z_kernel(RasterArgs a) {
+  if (a.skip && *a.skip) return;
+  const int nt = tri_count(a);
+  for (int t = blockIdx.x * 256 + threadIdx.x; t < nt; t += gridDim.x * 256) raster_z_tri(a, t);
+}
 
 // resolve_rasterized_triangles_kernel (src/mesh_raster_kernels.cu:139-200),
 // and the interpolated canonical point itself when canon_pt is set.
 __global__ __launch_bounds__(256) void raster_resolve_kernel(RasterArgs a) {
+  if (a.skip && *a.skip) return;
   const int px = blockIdx.x * 256 + threadIdx.x;
   if (px >= a.cam.width * a.cam.height) return;
   const unsigned long long key = a.ztri[px];
@@ -179,19 +197,24 @@ __global__ __launch_bounds__(256) void raster_resolve_kernel(RasterArgs a) {
   a.hit[px] = voxel_of(cp, a.org, a.s, a.X, a.Y, a.Z);
 }
 
+// blocks of a grid-stride launch over n items: one item per lane up to
+// 4096 workgroups (16 per CU)
+inline int grid_for(int n) { return std::max(1, std::min((n + 255) / 256, 4096)); }
+
 }  // namespace
 
 int skin_vertices(const float *canonical, int n, const nrf_volume_params *vp, const nrf_dualquat *centered,
-                  int num_nodes, const int32_t *voxel_knn, const float *voxel_knn_w, float *out, hipStream_t st) {
+                  int num_nodes, const int32_t *voxel_knn, const float *voxel_knn_w, float *out, hipStream_t st,
+                  const int32_t *d_ntri, const int32_t *skip) {
   if (!volume_ok(vp) || n < 0 || num_nodes < 0) return NRF_ERR_INVALID_ARGUMENT;
   if (n == 0) return NRF_OK;
   if (!canonical || !out || (num_nodes > 0 && (!centered || !voxel_knn || !voxel_knn_w)))
     return NRF_ERR_INVALID_ARGUMENT;
-  skin_kernel<<<(n + 255) / 256, 256, 0, st>>>(canonical, n, make_float3(vp->origin[0], vp->origin[1], vp->origin[2]),
+  skin_kernel<<<grid_for(n), 256, 0, st>>>(canonical, n, make_float3(vp->origin[0], vp->origin[1], vp->origin[2]),
                                                vp->voxel_size, vp->dims[0], vp->dims[1], vp->dims[2],
                                                reinterpret_cast<const DQ *>(centered), num_nodes,
                                                reinterpret_cast<const int4 *>(voxel_knn),
-                                               reinterpret_cast<const float4 *>(voxel_knn_w), out);
+                                               reinterpret_cast<const float4 *>(voxel_knn_w), out, d_ntri, skip);
   NRF_LAUNCH_CHECK();
   return NRF_OK;
 }
@@ -199,7 +222,7 @@ int skin_vertices(const float *canonical, int n, const nrf_volume_params *vp, co
 int rasterize_mesh(const float *canonical, const float *warped, const int32_t *triangles, int num_vertices,
                    int num_triangles, const nrf_camera *cam, const float camera_pose[16], const nrf_volume_params *vp,
                    unsigned long long *ztri, float *live_vertices, float *live_normals, int32_t *hit_voxel,
-                   float *canonical_points, hipStream_t st) {
+                   float *canonical_points, hipStream_t st, const int32_t *d_ntri, const int32_t *skip) {
   if (!cam || !camera_pose || !volume_ok(vp) || !ztri || !live_vertices || !live_normals || !hit_voxel ||
       num_vertices < 0 || num_triangles < 0 || cam->width <= 0 || cam->height <= 0)
     return NRF_ERR_INVALID_ARGUMENT;
@@ -210,6 +233,7 @@ int rasterize_mesh(const float *canonical, const float *warped, const int32_t *t
   a.tris = triangles;
   a.nv = num_vertices;
   a.nt = num_triangles;
+  a.d_nt = d_ntri;
   a.cam = *cam;
   a.Tcw = invert_pose(camera_pose);
   a.org = make_float3(vp->origin[0], vp->origin[1], vp->origin[2]);
@@ -220,11 +244,12 @@ int rasterize_mesh(const float *canonical, const float *warped, const int32_t *t
   a.live_n = live_normals;
   a.canon_pt = canonical_points;
   a.hit = hit_voxel;
+  a.skip = skip;
   const int npx = cam->width * cam->height;
   raster_reset_kernel<<<(npx + 255) / 256, 256, 0, st>>>(a);
   NRF_LAUNCH_CHECK();
   if (num_triangles == 0 || num_vertices == 0) return NRF_OK;
-  raster_z_kernel<<<(num_triangles + 255) / 256, 256, 0, st>>>(a);
+  raster_z_kernel<<<grid_for(num_triangles), 256, 0, st>>>(a);
   NRF_LAUNCH_CHECK();
   raster_resolve_kernel<<<(npx + 255) / 256, 256, 0, st>>>(a);
   NRF_LAUNCH_CHECK();

@@ -2165,33 +2165,9 @@ __global__ __launch_bounds__(1024) void gn_validate_kernel(GnArgs a) { gn_valida
 // apply_twist_increment(..., 1.0) (src/warp_field.cu:1121-1141) and the
 // centred-table refresh.
 __device__ inline void gn_update_node(const GnArgs &a, int i, const float d[6]) {
-  {
-    float tw[6];
-    for (int k = 0; k < 6; ++k) tw[k] = d[k];
-    const float rn = sqrtf(tw[0] * tw[0] + tw[1] * tw[1] + tw[2] * tw[2]);
-    const float tn = sqrtf(tw[3] * tw[3] + tw[4] * tw[4] + tw[5] * tw[5]);
-    float rsc = 1.0f, tsc = 1.0f;
-    if (a.max_rot > 0.0f && rn > a.max_rot) rsc = a.max_rot / rn;
-    if (a.max_trans > 0.0f && tn > a.max_trans) tsc = a.max_trans / tn;
-    for (int k = 0; k < 3; ++k) tw[k] *= rsc * a.scale;
-    for (int k = 3; k < 6; ++k) tw[k] *= tsc * a.scale;
-    // apply_twist_kernel with update_scale = 1
-    for (int k = 0; k < 6; ++k) tw[k] = 1.0f * tw[k];
-    const float rn2 = sqrtf(tw[0] * tw[0] + tw[1] * tw[1] + tw[2] * tw[2]);
-    if (a.max_rot > 0.0f && rn2 > a.max_rot) {
-      const float f = a.max_rot / rn2;
-      tw[0] *= f; tw[1] *= f; tw[2] *= f;
-    }
-    const float tn2 = sqrtf(tw[3] * tw[3] + tw[4] * tw[4] + tw[5] * tw[5]);
-    if (a.max_trans > 0.0f && tn2 > a.max_trans) {
-      const float f = a.max_trans / tn2;
-      tw[3] *= f; tw[4] *= f; tw[5] *= f;
-    }
-    const DQ dT = mat_to_dq(exp_se3(tw));
-    const DQ Tn = dq_mul(dT, a.T[i]);
-    a.T[i] = Tn;
-    if (a.centered) a.centered[i] = dq_centered(Tn, f3(a.nodes[i].pos));
-  }
+  const DQ Tn = scaled_twist_update(d, a.scale, a.max_rot, a.max_trans, a.T[i]);
+  a.T[i] = Tn;
+  if (a.centered) a.centered[i] = dq_centered(Tn, f3(a.nodes[i].pos));
 }
 __device__ void gn_update_body(const GnArgs &a) {
   if (*a.stop) return;
@@ -3072,19 +3048,20 @@ extern "C" int nrf_solver_solve(nrf_solver *s, const nrf_corr *corrs, int num_co
   return solver_fetch_result(s, result, st);
 }
 
-extern "C" int nrf_solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical, int num_corrs,
-                                   const nrf_node *nodes, nrf_dualquat *transforms, nrf_dualquat *centered,
-                                   int num_nodes, const nrf_gn_params *gn, float *delta_x,
-                                   nrf_pcg_result *result, void *stream) {
+namespace nrf {
+int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical, int num_corrs,
+                    const nrf_node *nodes, nrf_dualquat *transforms, nrf_dualquat *centered, int num_nodes,
+                    const nrf_gn_params *gn, float *delta_x, nrf_pcg_result *result, hipStream_t st, bool begin) {
   if (!s || !gn || num_corrs < 0 || num_nodes < 0 || num_nodes > s->max_nodes) return NRF_ERR_INVALID_ARGUMENT;
   if (num_nodes == 0) return NRF_OK;
   if (!nodes || !transforms || !centered || !delta_x || (num_corrs > 0 && !corrs)) return NRF_ERR_INVALID_ARGUMENT;
   if (s->P.N != num_nodes) return NRF_ERR_NOT_READY;
-  hipStream_t st = as_stream(stream);
   int rc;
   if ((rc = solver_build_memberships(s, corrs, num_corrs, num_nodes, st))) return rc;
-  gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res, s->d_tot);
-  NRF_LAUNCH_CHECK();
+  if (begin) {
+    gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res, s->d_tot);
+    NRF_LAUNCH_CHECK();
+  }
   sink_record(s, st);
   GnArgs g;
   g.dx = delta_x;
@@ -3126,6 +3103,58 @@ extern "C" int nrf_solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const f
     sink_record(s, st);
   }
   return solver_fetch_result(s, result, st);
+}
+
+int solver_gn_begin(nrf_solver *s, hipStream_t st) {
+  if (!s) return NRF_ERR_INVALID_ARGUMENT;
+  gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res, s->d_tot);
+  NRF_LAUNCH_CHECK();
+  return NRF_OK;
+}
+int32_t *solver_stop_flag(nrf_solver *s) { return s ? s->stop : nullptr; }
+nrf_pcg_result *solver_result_device(nrf_solver *s) { return s ? s->d_res : nullptr; }
+
+int solver_gn_linearise(nrf_solver *s, const nrf_corr *corrs, int num_corrs, const nrf_node *nodes,
+                        const nrf_dualquat *transforms, const nrf_dualquat *centered, int num_nodes,
+                        const nrf_gn_params *gn, float *delta_x, hipStream_t st) {
+  if (!s || !gn || num_corrs < 0 || num_nodes <= 0 || num_nodes > s->max_nodes) return NRF_ERR_INVALID_ARGUMENT;
+  if (!nodes || !transforms || !centered || !delta_x || (num_corrs > 0 && !corrs)) return NRF_ERR_INVALID_ARGUMENT;
+  if (s->P.N != num_nodes) return NRF_ERR_NOT_READY;
+  int rc;
+  if ((rc = solver_build_memberships(s, corrs, num_corrs, num_nodes, st))) return rc;
+  if ((rc = solver_assemble(s, corrs, num_corrs, nullptr, nodes, reinterpret_cast<const DQ *>(transforms),
+                            reinterpret_cast<const DQ *>(centered), num_nodes, st, true)))
+    return rc;
+  sink_record(s, st);
+  if ((rc = solver_pcg(s, delta_x, num_nodes, st, true))) return rc;
+  sink_record(s, st);
+  if (gn->validate) {
+    GnArgs g;
+    g.dx = delta_x;
+    g.T = nullptr;
+    g.centered = nullptr;
+    g.nodes = nodes;
+    g.N = num_nodes;
+    g.scale = gn->update_scale;
+    g.max_rot = gn->max_update_rot;
+    g.max_trans = gn->max_update_trans;
+    g.max_dx_mean = gn->max_dx_mean;
+    g.validate = 1;
+    g.stop = s->stop;
+    g.res = s->d_res;
+    gn_validate_kernel<<<1, 1024, 0, st>>>(g);
+    NRF_LAUNCH_CHECK();
+  }
+  return NRF_OK;
+}
+}  // namespace nrf
+
+extern "C" int nrf_solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical, int num_corrs,
+                                   const nrf_node *nodes, nrf_dualquat *transforms, nrf_dualquat *centered,
+                                   int num_nodes, const nrf_gn_params *gn, float *delta_x,
+                                   nrf_pcg_result *result, void *stream) {
+  return solver_solve_gn(s, corrs, canonical, num_corrs, nodes, transforms, centered, num_nodes, gn, delta_x, result,
+                         as_stream(stream), true);
 }
 
 extern "C" int nrf_solver_system(const nrf_solver *s, const int32_t **row_ptr, const int32_t **col_idx,

@@ -148,6 +148,33 @@ class Harness:
                                            self.corrs[f].data_ptr(), self.canonical[f].data_ptr(),
                                            self.w.width * self.w.height, stream), "pipeline_step")
 
+    def track(self, f: int, params: L.TrackParams, frame_id: int | None = None) -> None:
+        """Frame f with the device association (nrf_pipeline_track): the
+        prediction of the current mesh, projective correspondences against
+        frame f's depth, the GN loop, marking, warped integration, MC."""
+        stream = torch.cuda.current_stream().cuda_stream
+        L.check(self.lib.nrf_pipeline_track(self.pipe, self.depth[f].data_ptr(), self.pose.ctypes.data,
+                                            C.byref(params), f if frame_id is None else frame_id, stream),
+                "pipeline_track")
+
+    def track_stats(self) -> L.TrackStats:
+        out = L.TrackStats()
+        L.check(self.lib.nrf_pipeline_track_stats(self.pipe, C.byref(out)), "track_stats")
+        return out
+
+    def track_maps(self) -> dict:
+        """Host copies of the last tracking frame's device maps."""
+        ptrs = [C.c_void_p() for _ in range(6)]
+        L.check(self.lib.nrf_pipeline_track_maps(self.pipe, *[C.byref(q) for q in ptrs]), "track_maps")
+        npx = self.w.width * self.w.height
+        V = int(np.prod(self.w.dims))
+        return dict(live_v=self._d2h(ptrs[0].value, 3 * npx, np.float32),
+                    live_n=self._d2h(ptrs[1].value, 3 * npx, np.float32),
+                    hit=self._d2h(ptrs[2].value, npx, np.int32),
+                    corrs=self._d2h(ptrs[3].value, npx, L.CORR),
+                    opt=self._d2h(ptrs[4].value, V, np.int32),
+                    dnorm=self._d2h(ptrs[5].value, 3 * npx, np.float32))
+
     def solver_result(self) -> L.PcgResult:
         """The device-side PCG / GN record of the last step (syncs)."""
         solver = self.lib.nrf_pipeline_solver(self.pipe)

@@ -537,6 +537,61 @@ int nrf_pipeline_profile_read(nrf_pipeline *p, int step_back, float out_ms[6]);
 /* Voxels written by the warped integrations of all steps so far (syncs). */
 long long nrf_pipeline_updated_voxels(nrf_pipeline *p);
 
+/* ------------------------------------------------------- tracking frame */
+#define NRF_TRACK_MAX_GN 16
+
+/* The tracking fields of the pipeline parameters (include/fusion_app.h:
+   79-83 and the line search of :494-641). */
+typedef struct nrf_track_params {
+  nrf_assoc_params assoc;
+  int32_t min_valid_corrs; /* 2000: an association with fewer valid correspondences ends the GN loop */
+  int32_t line_search;     /* 1: the reference's ten-scale search per GN iteration; 0: the step is
+                              applied at gn.update_scale (the pipeline step's policy) */
+} nrf_track_params;
+
+/* What the last nrf_pipeline_track did (device-side record). */
+typedef struct nrf_track_stats {
+  int32_t gn_run;      /* GN iterations whose association ran */
+  int32_t gn_applied;  /* GN iterations whose step was applied */
+  int32_t stop_reason; /* 0 all gn_iterations ran, 1 too few correspondences, 2 step rejected by the
+                          validation, 3 no trial scale lowered the RMSE, 4 base RMSE not finite or
+                          update_scale <= 0 */
+  int32_t last_valid;  /* valid correspondences of the association that marked the voxels */
+  int32_t num_valid[NRF_TRACK_MAX_GN]; /* of GN iteration g's association */
+  float best_scale[NRF_TRACK_MAX_GN];  /* line search: the scale applied at iteration g (0: none) */
+  double base_rmse[NRF_TRACK_MAX_GN];  /* line search: correspondence RMSE before the step */
+  double best_rmse[NRF_TRACK_MAX_GN];  /* line search: RMSE at the chosen scale */
+} nrf_track_stats;
+
+/* One non-rigid frame with its own association, the caller contract of
+   DynamicFusionPipeline::process_frame (include/fusion_app.h:253-669) for
+   frames after the first, without rigid tracking, SIFT and the bounding
+   box: depth normals; per GN iteration (gn.gn_iterations, at most
+   NRF_TRACK_MAX_GN) the live surface prediction of the canonical mesh
+   (nrf_skin_vertices + nrf_rasterize_mesh of the last marching cubes output,
+   extracted first when the volume changed since), the association
+   (nrf_find_correspondences, all W x H slots passed to the solver as
+   fusion_app.h:387 does), the min_valid_corrs test, one linearisation of
+   the stored src and the step (validated when gn.validate; with
+   line_search the ten trial scales, each with its own prediction and
+   association, and the lowest RMSE kept); then nrf_mark_optimized_voxels
+   with frame_id (the reference's frame_count_; the per-voxel frame record
+   starts at 0), the warped integration gated by min_opt_count and marching
+   cubes.  Stream-ordered: every decision is taken on the device, and no
+   host synchronisation happens after the first call (which allocates the
+   tracking buffers: association maps, W x H correspondence slots, 8 B per
+   voxel of optimised-voxel counts).  depth: device, W x H. */
+int nrf_pipeline_track(nrf_pipeline *p, const float *depth, const float camera_pose[16],
+                       const nrf_track_params *params, int frame_id, void *stream);
+/* The record of the last nrf_pipeline_track (synchronises the device). */
+int nrf_pipeline_track_stats(nrf_pipeline *p, nrf_track_stats *out);
+/* Device maps of the last nrf_pipeline_track, owned by the pipeline (NULL
+   before the first): live vertices / normals (3 floats per pixel), canonical
+   hit voxel per pixel, the W x H correspondence slots, the per-voxel
+   optimised counts and the depth normals. */
+int nrf_pipeline_track_maps(nrf_pipeline *p, float **live_vertices, float **live_normals, int32_t **hit_voxel,
+                            nrf_corr **corrs, int32_t **voxel_opt_counts, float **depth_normals);
+
 /* ---------------------------------------------------------------- batch
    Build-only form (no counterpart in the reference): up to NRF_MAX_BATCH
    independent pipelines (their own volume, camera, nodes and solver

@@ -14,6 +14,7 @@
 // camera-to-world matrix, inverted as include/fusion_math.h:80-95 does.
 #include <cmath>
 #include <cstdint>
+#include <limits>
 #include <cstring>
 #include <vector>
 #include <algorithm>
@@ -221,6 +222,23 @@ void orc_mark_optimized_voxels(const nrf_corr *corrs, const int32_t *hit_voxel, 
       }
     }
   }
+}
+
+// correspondence_rmse (include/fusion_app.h:1331-1351): over the valid slots
+// with weight > 0, r = n . (src - dst), the difference in FP32 and the dot
+// in FP64, summed in slot order in FP64; infinity without weight.
+double orc_corr_rmse(const nrf_corr *corrs, int n) {
+  double sum = 0.0, wsum = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const nrf_corr &c = corrs[i];
+    if (!c.valid || c.weight <= 0.0f) continue;
+    const float dx = c.src[0] - c.dst[0], dy = c.src[1] - c.dst[1], dz = c.src[2] - c.dst[2];
+    const double r = (double)c.normal[0] * dx + (double)c.normal[1] * dy + (double)c.normal[2] * dz;
+    const double w = std::max(0.0f, c.weight);
+    sum += w * r * r;
+    wsum += w;
+  }
+  return wsum > 0.0 ? std::sqrt(sum / wsum) : std::numeric_limits<double>::infinity();
 }
 
 }  // extern "C"

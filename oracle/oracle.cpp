@@ -1085,6 +1085,19 @@ void orc_skin_vertices_f32(const float *canon, int n, const nrf_volume_params *v
     out[3 * i] = p.x; out[3 * i + 1] = p.y; out[3 * i + 2] = p.z;
   }
 }
+// upload_scaled_delta_x(scale) (include/fusion_app.h:1353-1375), then
+// apply_twist_increment(trial, max_rot, max_trans, 1) (src/warp_field.cu:
+// 1121-1141) on every node, in place.
+void orc_apply_step_f32(const float *dx, nrf_dualquat *Tq, int N, const nrf_gn_params *g, float scale) {
+  nrf_gn_params gs = *g;
+  gs.update_scale = scale;
+  auto T = load_T<float>(Tq, N);
+  apply_step(dx, T.data(), N, gs);
+  for (int i = 0; i < N; ++i) {
+    Tq[i].real[0] = T[i].r.x; Tq[i].real[1] = T[i].r.y; Tq[i].real[2] = T[i].r.z; Tq[i].real[3] = T[i].r.w;
+    Tq[i].dual[0] = T[i].d.x; Tq[i].dual[1] = T[i].d.y; Tq[i].dual[2] = T[i].d.z; Tq[i].dual[3] = T[i].d.w;
+  }
+}
 // Step validation of the caller (include/fusion_app.h:409-491): 1 = accept.
 int orc_validate_f32(const float *dx, int N, const nrf_gn_params *g) { return validate_step(dx, N, *g) ? 1 : 0; }
 int orc_validate_f64(const double *dx, int N, const nrf_gn_params *g) { return validate_step(dx, N, *g) ? 1 : 0; }

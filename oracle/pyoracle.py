@@ -58,6 +58,8 @@ _SIGS = {
     "orc_skin_vertices_f32": (None, [P, I, P, P, P, I, P, P, P]),
     "orc_rasterize_f32": (None, [P, P, P, I, I, P, P, P, P, P, P, P]),
     "orc_validate_f64": (I, [P, I, P]),
+    "orc_apply_step_f32": (None, [P, P, I, P, F]),
+    "orc_corr_rmse": (D, [P, I]),
     "orc_solve_gn_f32": (I, [P, P, I, P, I, P, P, P, P, P, P, P]),
     "orc_solve_gn_f64": (I, [P, P, I, P, I, P, P, P, P, P, P, P]),
     "orc_lp_param_count": (I, []),
@@ -351,6 +353,105 @@ def solve_gn_f64(corrs, canonical, nodes, T64, gn, sp):
     load().orc_solve_gn_f64(ptr(corrs), ptr(canonical), len(corrs), ptr(nodes), N, ptr(T64), C.byref(gn),
                             C.byref(sp), ptr(dx), C.byref(tot), C.byref(done), C.byref(last))
     return T64, dx, tot.value, done.value, last.value
+
+
+def apply_step_f32(dx, T, gn, scale):
+    """upload_scaled_delta_x(scale) + apply_twist_increment on every node
+    (fusion_app.h:1353-1375, warp_field.cu:1121-1141); returns new T."""
+    T = np.array(T, copy=True)
+    d = np.ascontiguousarray(dx, np.float32)
+    load().orc_apply_step_f32(ptr(d), ptr(T), len(T), C.byref(gn), float(scale))
+    return T
+
+
+def corr_rmse(corrs) -> float:
+    """correspondence_rmse (fusion_app.h:1331-1351)."""
+    return float(load().orc_corr_rmse(ptr(corrs), len(corrs)))
+
+
+def track_frame_f32(vp, cam, pose, depth, verts, nodes, T, knn, knn_w, gn, sp, tp, frame_id, opt, opt_frame,
+                    vox=None, min_opt=0):
+    """One tracking frame of DynamicFusionPipeline::process_frame
+    (include/fusion_app.h:253-669, frames after the first, no rigid
+    tracking / SIFT / bounding box), composed of the oracle's stages in the
+    reference's order: per GN iteration the live surface prediction of the
+    canonical mesh `verts` (a triangle soup, 3 vertices per triangle)
+    (:259-263, 1216-1320), the association (:288-291, 1791-1845), the
+    min_valid_corrs break (:342-343), then either the step at update_scale
+    (tp.line_search 0) or the base RMSE, one linearisation, the validation
+    (:409-491) and the ten-scale search (:494-641); then
+    mark_optimized_voxels (:1322-1329) and, with vox, the warped integration
+    gated by min_opt (:655-666).  opt / opt_frame are updated in place.
+    Returns (T, stats dict, corrs, hit) where corrs / hit are the slots the
+    marking read."""
+    from cudafusion_amd import _lib as Lb
+    npx = cam.width * cam.height
+    depth = np.ascontiguousarray(depth, np.float32)
+    dn = depth_normals_f32(depth, cam)
+    verts = np.ascontiguousarray(verts, np.float32).reshape(-1)
+    tris = np.arange(verts.size // 3, dtype=np.int32)
+    G = max(1, gn.gn_iterations)
+    ls = bool(tp.line_search)
+    st = dict(num_valid=[], best_scale=[], base_rmse=[], best_rmse=[], stop_reason=0, gn_applied=0)
+    g1 = Lb.gn_params(1, gn.update_scale, gn.max_update_rot, gn.max_update_trans, gn.validate, gn.max_dx_mean)
+
+    def associate(Tc):
+        warped = skin_vertices_f32(vp, verts, nodes, Tc, knn, knn_w)
+        lv, ln, hit, _ = rasterize_f32(verts, warped, tris, cam, pose, vp)
+        pk, pw = pixel_knn(hit, knn, knn_w)
+        c, nv, _ = find_correspondences_f32(lv, ln, depth, dn, cam, pose, pk, pw, tp.assoc)
+        return c, hit, nv
+
+    us = np.float32(gn.update_scale)
+    trials = [np.float32(f) * us for f in (1.0, 0.5, 0.25, 0.125, 0.0625, -0.0625, -0.125, -0.25, -0.5)]
+    trials[0] = us
+    trials.append(-us)
+    corrs = hit = None
+    for g in range(G):
+        corrs, hit, nv = associate(T)
+        st["num_valid"].append(nv)
+        if nv < tp.min_valid_corrs:
+            st["stop_reason"] = 1
+            break
+        if not ls:
+            T, _, _, done, _ = solve_gn_f32(corrs, None, nodes, T, g1, sp)
+            if not done:
+                st["stop_reason"] = 2
+                break
+            st["gn_applied"] += 1
+            continue
+        base = corr_rmse(corrs)
+        st["base_rmse"].append(base)
+        _, dx, _, _, _ = solve_gn_f32(corrs, None, nodes, T, g1, sp)
+        if gn.validate and not validate(dx, gn):
+            st["stop_reason"] = 2
+            break
+        if not gn.update_scale > 0 or not np.isfinite(base):
+            st["stop_reason"] = 4
+            break
+        Tsave = np.array(T, copy=True)
+        best, best_scale = base, np.float32(0)
+        for s in trials:
+            Tt = apply_step_f32(dx, Tsave, gn, s)
+            corrs, hit, n2 = associate(Tt)
+            r = corr_rmse(corrs)
+            if (n2 >= tp.min_valid_corrs and n2 >= int(np.float32(0.75) * np.float32(nv)) and np.isfinite(r)
+                    and r < best):
+                best, best_scale = r, s
+        st["best_scale"].append(float(best_scale))
+        st["best_rmse"].append(best)
+        T = Tsave
+        if best_scale == 0:
+            st["stop_reason"] = 3
+            break
+        T = apply_step_f32(dx, Tsave, gn, best_scale)
+        st["gn_applied"] += 1
+        corrs, hit, _ = associate(T)
+    mark_optimized_voxels(corrs, hit, vp.dims[:], frame_id, opt, opt_frame)
+    if vox is not None:
+        integrate_f32(vp, vox, depth, cam, pose, nodes, T, knn, knn_w, opt, min_opt)
+    st["last_valid"] = int((corrs["valid"] != 0).sum())
+    return T, st, corrs, hit
 
 
 # ------------------------------------------------- learned preconditioner

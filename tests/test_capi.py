@@ -59,7 +59,8 @@ def test_struct_layouts_match_the_header(tmp_path):
     pairs = {"nrf_volume_params": L.VolumeParams, "nrf_camera": L.Camera, "nrf_solver_params": L.SolverParams,
              "nrf_gn_params": L.GNParams, "nrf_pcg_result": L.PcgResult, "nrf_solver_totals": L.SolverTotals,
              "nrf_pipeline_config": L.PipelineConfig, "nrf_batch_frame": L.BatchFrame,
-             "nrf_assoc_params": L.AssocParams,
+             "nrf_assoc_params": L.AssocParams, "nrf_track_params": L.TrackParams,
+             "nrf_track_stats": L.TrackStats,
              "nrf_batch_tuning": L.BatchTuning}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "nrf.h"', "int main(void) {"]
     for cname, py in pairs.items():
