@@ -91,30 +91,158 @@ def knn_id_bytes(h) -> int:
     return int((occ * run_len[None, None, :]).sum()) * 16 + Z * Y * nsx * 4
 
 
-def cpu_baseline(h, frames: int, threads: int):
-    """The oracle (C++/OpenMP restatement, FP32) on `frames` frames of the
-    same workload: GN solve + warped integration + marching cubes."""
-    import numpy as np
-
-    from oracle import pyoracle as po
-    po.build()
-    po.threads(threads)
-    vox = h.download_voxels().copy()
+def _cpu_frames(h, po, vox, T, frames, threads):
+    """Per-frame seconds of the oracle's FP32 step (solve, fusion, mc) on
+    frames 1..frames, the first one a warm-up."""
     knn, knn_w = h.download_knn()
-    T = np.zeros(h.N, h.download_transforms().dtype)
-    T["real"][:, 3] = 1
-    pose = h.pose
-    times = []
+    po.threads(threads)
+    rec = []
     for f in range(1, frames + 1):
         corrs, can = h.frame_corrs_host(f)
         depth = h.frames_host[f].depth.reshape(-1)
         t0 = time.perf_counter()
         T, _, _, _, _ = po.solve_gn_f32(corrs, can, h.nodes, T, h.gn, h.sp)
-        po.integrate_f32(h.vp, vox, depth, h.cam, pose, h.nodes, T, knn, knn_w)
+        t1 = time.perf_counter()
+        po.integrate_f32(h.vp, vox, depth, h.cam, h.pose, h.nodes, T, knn, knn_w)
+        t2 = time.perf_counter()
         po.mc_f32(h.vp, vox, normals=False)
-        times.append(time.perf_counter() - t0)
-        log(f"[cpu] frame {f}: {times[-1]:.2f} s")
-    return times
+        t3 = time.perf_counter()
+        rec.append((t3 - t0, t1 - t0, t2 - t1, t3 - t2))
+        log(f"[cpu {threads} threads] frame {f}{' (warm-up)' if f == 1 else ''}: {t3 - t0:.2f} s "
+            f"(solve {t1 - t0:.2f}, fusion {t2 - t1:.2f}, mc {t3 - t2:.2f})")
+    return T, rec[1:]
+
+
+def _cpu_host():
+    """The host CPU as lscpu names it (model, sockets, cores per socket)."""
+    import subprocess
+    info = {"hardware_concurrency": os.cpu_count()}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(h, frames: int, threads: int, frames_1t: int):
+    """The oracle (the C++/OpenMP FP32 restatement of the step) on the same
+    workload, on this host's cores: `threads` OpenMP threads over frames
+    1..frames and one thread over frames 1..frames_1t, each run's first frame
+    a warm-up; median and p99 of the per-frame times and per-stage medians
+    under the reference profiler's names (include/fusion_app.h:700-739:
+    tracking.solve, fusion, mc)."""
+    import numpy as np
+
+    from oracle import pyoracle as po
+    po.build()
+    vox0 = h.download_voxels().copy()
+    T0 = np.zeros(h.N, h.download_transforms().dtype)
+    T0["real"][:, 3] = 1
+    omp_max = po.threads(0)
+
+    def summary(rec, nthreads):
+        t = np.array([r[0] for r in rec])
+        return {"threads": nthreads, "frames": len(rec), "median_s": round(float(np.median(t)), 4),
+                "p99_s": round(float(np.percentile(t, 99)), 4),
+                "frames_per_s": round(1.0 / float(np.median(t)), 4),
+                "stages_median_s": {"solve": round(float(np.median([r[1] for r in rec])), 4),
+                                    "fusion": round(float(np.median([r[2] for r in rec])), 4),
+                                    "mc": round(float(np.median([r[3] for r in rec])), 4)}}
+
+    _, rec = _cpu_frames(h, po, vox0.copy(), T0.copy(), frames, threads)
+    many = summary(rec, threads)
+    _, rec1 = _cpu_frames(h, po, vox0.copy(), T0.copy(), frames_1t, 1)
+    one = summary(rec1, 1)
+    po.threads(omp_max)
+    return many, one, dict(_cpu_host(), omp_get_max_threads=omp_max)
+
+
+def equal_residual_leg(host, frames: int):
+    """The full step with every PCG run judged by the FP64 true residual
+    (below 1e-6 ||b||, correction rounds after the PCG, at most 2000
+    iterations) in the plain FP32 block-Jacobi form and in the precision form
+    (split-BF16 JtJ, FP16 TSDF + u8 weight, FP64 Cholesky inverses), each on
+    a fresh pipeline of the same workload: frames/s over `frames` frames after
+    two warm-up frames, PCG work per frame and the runs that reached the
+    residual."""
+    import torch
+
+    from cudafusion_amd.harness import Harness
+    forms = {"plain_fp32": dict(jtj_precision=0, storage16=False, precond_fp64=False),
+             "precision_bf16": dict(jtj_precision=3, storage16=True, precond_fp64=True)}
+    out = {}
+    for name, kw in forms.items():
+        h = Harness("body", frames=host.frames, arap_consistent_sign=True, validate=False, pcg_pipelined=2,
+                    relative_tolerance=True, pcg_tolerance=1e-6, true_residual_rounds=2, **kw)
+        h.sp.pcg_iterations = 2000
+        h.setup(host=host)
+        for f in (1, 2):
+            h.step(f)
+        torch.cuda.synchronize()
+        t0s = h.solver_totals()
+        t0 = time.perf_counter()
+        for f in range(3, 3 + frames):
+            h.step(f)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        t1s = h.solver_totals()
+        runs = t1s.pcg_runs - t0s.pcg_runs
+        out[name] = {"frames_per_s": round(frames / el, 2), "ms_per_frame": round(el * 1e3 / frames, 3),
+                     "pcg_iterations_per_frame": round((t1s.pcg_iterations - t0s.pcg_iterations) / frames, 1),
+                     "pcg_runs": int(runs), "reached_1e-6": int(t1s.pcg_converged - t0s.pcg_converged)}
+        log(f"[equal residual] {name}: {out[name]}")
+        h.close()
+    out["speedup"] = round(out["precision_bf16"]["frames_per_s"] / out["plain_fp32"]["frames_per_s"], 3)
+    out["target"] = ">= 5x (full step, equal 1e-6 relative PCG residual)"
+    return out
+
+
+def tracking_leg(host, frames: int, line_search: bool, reference_policy: bool):
+    """Frames 1..frames tracked with the device association
+    (nrf_pipeline_track) on a fresh pipeline of the same workload: frames/s
+    over the frames after the first, the first frame's stage times, and the
+    loop's record per frame."""
+    import numpy as np
+    import torch
+
+    from cudafusion_amd import _lib as L
+    from cudafusion_amd.harness import Harness
+    h = Harness("body", frames=host.frames, arap_consistent_sign=not reference_policy, validate=reference_policy,
+                pcg_pipelined=3)
+    h.setup(host=host)
+    L.check(h.lib.nrf_solver_set_pcg_layout(h.lib.nrf_pipeline_solver(h.pipe), 4, 16), "pcg_layout")
+    tp = L.track_params(min_valid_corrs=2000, line_search=line_search)
+    h.track(1, tp)
+    torch.cuda.synchronize()
+    st = h.track_stats()
+    first = h.stage_ms()
+    recs = []
+    t0 = time.perf_counter()
+    for f in range(2, frames + 1):
+        h.track(f, tp)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    st_last = h.track_stats()
+    n = frames - 1
+    out = {"frames_per_s": round(n / el, 2), "ms_per_frame": round(el * 1e3 / n, 3), "frames": n,
+           "line_search": bool(line_search),
+           "policy": "reference (ARAP sign, step validation)" if reference_policy else
+                     "consistent ARAP sign, per-node clamp",
+           "first_frame_ms": {"tracking": round(first[0], 3), "fusion": round(first[1], 3), "mc": round(first[2], 3)},
+           "first_frame": {"gn_run": st.gn_run, "gn_applied": st.gn_applied, "stop_reason": st.stop_reason,
+                           "num_valid": list(st.num_valid[: st.gn_run]),
+                           "best_scale": [round(v, 4) for v in st.best_scale[: st.gn_run]],
+                           "rmse_m": [round(v, 6) for v in st.base_rmse[: st.gn_run]]},
+           "last_frame": {"gn_run": st_last.gn_run, "gn_applied": st_last.gn_applied,
+                          "stop_reason": st_last.stop_reason, "num_valid": list(st_last.num_valid[: st_last.gn_run]),
+                          "valid_pixels": int(h.frames_host[frames].valid.sum())}}
+    log(f"[track ls={int(line_search)} ref={int(reference_policy)}] {out}")
+    h.close()
+    return out
 
 
 def main() -> int:
@@ -123,8 +251,16 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--frames", type=int, default=40)
-    ap.add_argument("--cpu-frames", type=int, default=2)
+    ap.add_argument("--cpu-frames", type=int, default=6,
+                    help="frames of the CPU baseline on all threads (the first a warm-up)")
+    ap.add_argument("--cpu-frames-1t", type=int, default=3,
+                    help="frames of the one-thread CPU baseline (the first a warm-up)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--track-frames", type=int, default=12,
+                    help="frames of the tracking legs (device association; 0 skips them)")
+    ap.add_argument("--equal-residual-frames", type=int, default=10,
+                    help="frames of the equal-residual comparison (full step, PCG to a 1e-6 true residual, "
+                         "plain FP32 against the precision form; 0 skips it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -277,11 +413,15 @@ def main() -> int:
     traffic = measured_traffic()
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        times = cpu_baseline(h, args.cpu_frames, threads)
-        cpu = {"value": round(len(times) / sum(times), 4), "unit": "frames/s", "cores": threads, "kind": "port",
-               "sample": f"{len(times)} frames of the same workload (frames 1..{len(times)}), oracle FP32 "
-                         f"solve_gn + warped integrate + marching cubes, OpenMP {threads} threads"}
+        from oracle import pyoracle as po
+        po.build()
+        threads = args.cpu_threads or po.threads(0)
+        many, one, host_cpu = cpu_baseline(h, args.cpu_frames, threads, args.cpu_frames_1t)
+        cpu = {"value": many["frames_per_s"], "unit": "frames/s", "cores": threads, "kind": "port",
+               "sample": f"frames 2..{args.cpu_frames} of the same workload after frame 1 as a warm-up, oracle "
+                         f"FP32 solve_gn + warped integrate + marching cubes, OpenMP {threads} threads; value = "
+                         f"1 / median frame time",
+               "all_threads": many, "one_thread": one, "host": host_cpu}
     out = {
         "metric": METRIC,
         "value": round(fps, 3),
@@ -317,6 +457,13 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     out["pcg_to_1e-6"] = pcg_to_tolerance(h, 1e-6)
+    if args.equal_residual_frames > 0 and not prec:
+        out["equal_residual"] = equal_residual_leg(h, args.equal_residual_frames)
+    if args.track_frames > 1 and not prec:
+        # the device association (live prediction + projective correspondences every GN iteration):
+        # the reference's policy with its line search, and this step's policy without one
+        out["tracking"] = {"reference_line_search": tracking_leg(h, args.track_frames, True, True),
+                           "step_policy": tracking_leg(h, args.track_frames, False, False)}
     if prec:
         out["config"]["workload"] += ", precision form (%s JtJ, FP16 TSDF + u8 weight, FP64 Cholesky, one " \
                                      "FP64-residual refinement step: %s)" % (

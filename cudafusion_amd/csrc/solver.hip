@@ -1992,11 +1992,40 @@ __global__ __launch_bounds__(256) void tr_axpy_kernel(float *__restrict__ x, int
 
 // One iterative-refinement step whose correction solve is the block-Jacobi
 // inverse: x_out = x + M^-1 (b - A x), the residual (products and sums) and
-// the correction in FP64.  One workgroup of 256 lanes per kRefNodes nodes:
-// lane 6 k + r forms row r of node k's residual in LDS, then row r of its
-// correction.  x and x_out must not alias (a row reads its neighbours' x).
-// A raised GN stop flag makes it a no-op.
-constexpr int kRefNodes = 42;
+// the correction in FP64.  One wave per node (4 nodes per workgroup): lane
+// 6 s + r takes row r of the row's blocks s, s + kRefSlots, ..., the six row
+// lanes sum the slots in slot order and form the correction by lane moves
+// (the order of gn_refine_update_kernel, so both give the same bits).  x and
+// x_out must not alias (a row reads its neighbours' x).  A raised GN stop
+// flag makes it a no-op.
+constexpr int kRefSlots = 10;
+__device__ inline double refine_row_residual(const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col_idx,
+                                             const float *__restrict__ values, const float *__restrict__ b,
+                                             const float *__restrict__ x, int i, int lane) {
+  const int slot = lane / 6, row = lane % 6;
+  double acc = 0.0;
+  if (lane < 6 * kRefSlots) {
+    const int b1 = row_ptr[i + 1];
+    for (int bk = row_ptr[i] + slot; bk < b1; bk += kRefSlots) {
+      const float *A = values + (size_t)bk * 36 + row * 6;
+      const float *xj = x + 6 * col_idx[bk];
+      for (int c = 0; c < 6; ++c) acc += (double)A[c] * (double)xj[c];
+    }
+  }
+  double sum = 0.0;
+  for (int s = 0; s < kRefSlots; ++s) sum += __shfl(acc, 6 * s + row, 64);
+  return (double)b[6 * i + row] - sum;
+}
+// x + M^-1 res for lanes 0-5 (row = lane), from the six row residuals
+__device__ inline float refine_row_step(const float *__restrict__ Minv, const float *__restrict__ x, double res, int i,
+                                        int lane) {
+  double z = 0.0;
+  for (int c = 0; c < 6; ++c) {
+    const double rc = __shfl(res, c, 64);
+    if (lane < 6) z += (double)Minv[(size_t)i * 36 + lane * 6 + c] * rc;
+  }
+  return lane < 6 ? (float)((double)x[6 * i + lane] + z) : 0.f;
+}
 __global__ __launch_bounds__(256) void refine_jacobi_kernel(const int32_t *__restrict__ row_ptr,
                                                            const int32_t *__restrict__ col_idx,
                                                            const float *__restrict__ values,
@@ -2005,27 +2034,13 @@ __global__ __launch_bounds__(256) void refine_jacobi_kernel(const int32_t *__res
                                                            const float *__restrict__ x, int N,
                                                            const int32_t *__restrict__ stop,
                                                            float *__restrict__ x_out) {
-  __shared__ double rs[kRefNodes * 6];
   if (stop && *stop) return;
-  const int t = threadIdx.x, k = t / 6, row = t % 6;
-  const int i = blockIdx.x * kRefNodes + k;
-  const bool act = t < kRefNodes * 6 && i < N;
-  if (act) {
-    double acc = 0.0;
-    for (int bk = row_ptr[i]; bk < row_ptr[i + 1]; ++bk) {
-      const float *A = values + (size_t)bk * 36 + row * 6;
-      const float *xj = x + 6 * col_idx[bk];
-      for (int c = 0; c < 6; ++c) acc += (double)A[c] * (double)xj[c];
-    }
-    rs[t] = (double)b[6 * i + row] - acc;
-  }
-  __syncthreads();
-  if (act) {
-    const float *Mi = Minv + (size_t)i * 36 + row * 6;
-    double z = 0.0;
-    for (int c = 0; c < 6; ++c) z += (double)Mi[c] * rs[6 * k + c];
-    x_out[6 * i + row] = (float)((double)x[6 * i + row] + z);
-  }
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= N) return;  // the whole wave
+  const double res = refine_row_residual(row_ptr, col_idx, values, b, x, i, lane);
+  const float xo = refine_row_step(Minv, x, res, i, lane);
+  if (lane < 6) x_out[6 * i + lane] = xo;
 }
 
 // Sums the per-workgroup partials in index order: out = {|b - Ax|, |b|}.
@@ -2177,49 +2192,30 @@ __device__ void gn_update_body(const GnArgs &a) {
 }
 __global__ __launch_bounds__(256) void gn_update_kernel(GnArgs a) { gn_update_body(a); }
 
-// The refinement step x = d + M^-1 (b - A d) (refine_jacobi_kernel's
-// arithmetic, residual and correction in FP64) fused with the update when
-// no validation sits between them.  Six lanes per node (lane 6 k + r of a
-// wave: row r of node k, ten nodes per wave) form the node's residual rows
-// and correction, exchanged by lane moves; lane 6 k stores the refined step
-// in dx_out and applies it.  d (the PCG's solution) is only read, so no lane
-// reads a value another lane writes.
+// The refinement step x = d + M^-1 (b - A d) (residual and correction in
+// FP64, as refine_jacobi_kernel) fused with the update when no validation
+// sits between them.  One wave per node (refine_row_residual: all of a
+// row's blocks in one round for rows of up to 10 blocks, so the loads of a
+// node are in flight together); lane 0 applies the refined step, which
+// lanes 0-5 store in dx_out.
+// d (the PCG's solution) is only read, so no lane reads a value another
+// lane writes.
 struct RefineArgs {
   const int32_t *row_ptr, *col_idx;
   const float *values, *b, *Minv, *d;
   float *dx_out;
 };
-constexpr int kRuNodesPerWave = 10;
 __global__ __launch_bounds__(256) void gn_refine_update_kernel(GnArgs a, RefineArgs r) {
   if (*a.stop) return;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int k = lane / 6, row = lane % 6;
-  const int i = (blockIdx.x * 4 + w) * kRuNodesPerWave + k;
-  const bool act = lane < 6 * kRuNodesPerWave && i < a.N;
-  double res = 0.0;
-  if (act) {
-    double acc = 0.0;
-    for (int bk = r.row_ptr[i]; bk < r.row_ptr[i + 1]; ++bk) {
-      const float *A = r.values + (size_t)bk * 36 + row * 6;
-      const float *xj = r.d + 6 * r.col_idx[bk];
-      for (int c = 0; c < 6; ++c) acc += (double)A[c] * (double)xj[c];
-    }
-    res = (double)r.b[6 * i + row] - acc;
-  }
-  const int base = 6 * k;
-  double z = 0.0;
-  for (int c = 0; c < 6; ++c) {
-    const double rc = __shfl(res, (base + c) & 63, 64);
-    if (act) z += (double)r.Minv[(size_t)i * 36 + row * 6 + c] * rc;
-  }
-  float x = 0.f;
-  if (act) {
-    x = (float)((double)r.d[6 * i + row] + z);
-    r.dx_out[6 * i + row] = x;
-  }
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= a.N) return;  // the whole wave
+  const double res = refine_row_residual(r.row_ptr, r.col_idx, r.values, r.b, r.d, i, lane);
+  const float x = refine_row_step(r.Minv, r.d, res, i, lane);
+  if (lane < 6) r.dx_out[6 * i + lane] = x;
   float xs[6];
-  for (int c = 0; c < 6; ++c) xs[c] = __shfl(x, (base + c) & 63, 64);
-  if (act && row == 0) gn_update_node(a, i, xs);
+  for (int c = 0; c < 6; ++c) xs[c] = __shfl(x, c, 64);
+  if (lane == 0) gn_update_node(a, i, xs);
   if (blockIdx.x == 0 && threadIdx.x == 0) a.res->gn_done += 1;
 }
 
@@ -2825,7 +2821,7 @@ static int solver_pcg(nrf_solver *s, float *x, int N, hipStream_t st, bool opene
     // x = d + M^-1 (b - A d)
     if ((rc = dev_alloc(s->dref, (size_t)6 * N, s->c_dref))) return rc;
     if ((rc = pcg_launch(s, s->b, s->dref, N, s->params.pcg_iterations, true, st, opened))) return rc;
-    refine_jacobi_kernel<<<(N + kRefNodes - 1) / kRefNodes, 256, 0, st>>>(s->row_ptr, s->col_idx, s->values, s->b,
+    refine_jacobi_kernel<<<(N + 3) / 4, 256, 0, st>>>(s->row_ptr, s->col_idx, s->values, s->b,
                                                                            s->Minv, s->dref, N, s->stop, x);
     NRF_LAUNCH_CHECK();
     return NRF_OK;
@@ -3087,7 +3083,7 @@ int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical
       if ((rc = pcg_launch(s, s->b, s->dref, num_nodes, s->params.pcg_iterations, true, st, true))) return rc;
       sink_record(s, st);
       const RefineArgs ra{s->row_ptr, s->col_idx, s->values, s->b, s->Minv, s->dref, delta_x};
-      gn_refine_update_kernel<<<(num_nodes + 4 * kRuNodesPerWave - 1) / (4 * kRuNodesPerWave), 256, 0, st>>>(g, ra);
+      gn_refine_update_kernel<<<(num_nodes + 3) / 4, 256, 0, st>>>(g, ra);
       NRF_LAUNCH_CHECK();
       sink_record(s, st);
       continue;
