@@ -177,6 +177,7 @@ _SIGS = {
     "nrf_pipeline_profile_pcg": (I, [P, I]),
     "nrf_pipeline_profile_read": (I, [P, I, C.POINTER(F)]),
     "nrf_pipeline_updated_voxels": (C.c_longlong, [P]),
+    "nrf_raycast": (I, [C.POINTER(VolumeParams), P, C.POINTER(Camera), P, P, P, I, P, P, P, P, P, P]),
     "nrf_pipeline_track": (I, [P, P, P, C.POINTER(TrackParams), I, P]),
     "nrf_pipeline_track_stats": (I, [P, C.POINTER(TrackStats)]),
     "nrf_pipeline_track_maps": (I, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P),

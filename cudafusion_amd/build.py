@@ -21,7 +21,8 @@ OBJDIR = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libnrf.so")
 
 HIP_SOURCES = ["common.hip", "integrate.hip", "mc.hip", "warp.hip", "solver.hip", "pipeline.hip", "precision.hip",
-               "learned.hip", "association.hip", "raster.hip", "track.hip"]
+               "learned.hip", "association.hip", "raster.hip", "track.hip",
+               "raycast.hip"]
 CXX_SOURCES = ["dropin.cpp"]
 
 HIPCC_FLAGS = [

@@ -73,6 +73,24 @@ void TSDFVolume::integrate(const DeviceArray<float> &depth, const DeviceArray<fl
   NRF_DEVICE_CHECK(hipDeviceSynchronize());
 }
 
+void TSDFVolume::raycast(DeviceArray<float3> &vertices, DeviceArray<float3> &normals, const CameraIntrinsics &cam,
+                         const Mat4 &camera_pose, const DeformNode *nodes, const DualQuat *transforms, int num_nodes,
+                         const int *voxel_knn, const float *voxel_knn_w, int *out_canonical_vidx) {
+  const size_t npx = (size_t)cam.width * cam.height;
+  vertices.allocate(npx);
+  normals.allocate(npx);
+  nrf_volume_params v = vp();
+  nrf_camera c = to_cam(cam);
+  float pose[16];
+  pose16(camera_pose, pose);
+  NRF_STATUS_CHECK(nrf_raycast(&v, reinterpret_cast<const nrf_voxel *>(d_voxels_.data), &c, pose,
+                               reinterpret_cast<const nrf_node *>(nodes),
+                               reinterpret_cast<const nrf_dualquat *>(transforms), num_nodes, voxel_knn, voxel_knn_w,
+                               reinterpret_cast<float *>(vertices.data), reinterpret_cast<float *>(normals.data),
+                               out_canonical_vidx, nullptr));
+  NRF_DEVICE_CHECK(hipDeviceSynchronize());
+}
+
 void TSDFVolume::extract_surface(std::vector<float3> &vertices, std::vector<float3> &normals,
                                  std::vector<int3> &triangles) const {
   vertices.clear();

@@ -246,6 +246,28 @@ __device__ inline bool dq_blend4(const DQ *table, const int ids[4], const float 
   return true;
 }
 
+// warp_normal (include/se3_math.cuh:512-558): blends the real parts of the
+// *uncentred* node transforms (running-sign fix on the real part only),
+// normalises, rotates n and renormalises it.  There is no weight-sum test:
+// with no valid neighbour the blend normalises to the identity.
+__host__ __device__ inline float3 dq_warp_normal(const DQ *T, int N, const int ids[4], const float ws[4], float3 n) {
+  DQ b;
+  b.r = make_float4(0.f, 0.f, 0.f, 0.f);
+  b.d = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int k = 0; k < 4; ++k) {
+    const int id = ids[k];
+    const float w = ws[k];
+    if (id < 0 || id >= N || w < 1e-8f) continue;
+    float4 q = T[id].r;
+    if (qdot(q, b.r) < 0) q = make_float4(-q.x, -q.y, -q.z, -q.w);
+    b.r.x += w * q.x; b.r.y += w * q.y; b.r.z += w * q.z; b.r.w += w * q.w;
+  }
+  b = dq_normalize(b);
+  const float4 r = qmul(qmul(b.r, make_float4(n.x, n.y, n.z, 0.f)), qconj(b.r));
+  const float m = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
+  return m > 1e-9f ? f3(r.x / m, r.y / m, r.z / m) : f3(r.x, r.y, r.z);
+}
+
 // One node's step of the caller: the per-node clamp, then the scale
 // (upload_scaled_delta_x, include/fusion_app.h:1353-1375), then
 // apply_twist_kernel with update_scale 1 (src/warp_field.cu:250-291), which

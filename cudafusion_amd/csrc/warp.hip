@@ -149,10 +149,7 @@ __global__ void warp_points_kernel(const float *__restrict__ pts, const int4 *__
   out[3 * i + 2] = p.z;
 }
 
-// warp_normal (include/se3_math.cuh:512-558): blends the real parts of the
-// *uncentred* node transforms (running-sign fix on the real part only),
-// normalises, rotates n and renormalises it.  There is no weight-sum test:
-// with no valid neighbour the blend normalises to the identity.
+// warp_normal (include/se3_math.cuh:512-558), dq_warp_normal per point.
 __global__ void warp_normals_kernel(const float *__restrict__ nin, const int4 *__restrict__ ids4,
                                     const float4 *__restrict__ ws4, int count, const DQ *__restrict__ T, int N,
                                     float *__restrict__ out) {
@@ -162,22 +159,7 @@ __global__ void warp_normals_kernel(const float *__restrict__ nin, const int4 *_
   const float4 w4 = ws4[i];
   const int ids[4] = {i4.x, i4.y, i4.z, i4.w};
   const float ws[4] = {w4.x, w4.y, w4.z, w4.w};
-  DQ b;
-  b.r = make_float4(0.f, 0.f, 0.f, 0.f);
-  b.d = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int k = 0; k < 4; ++k) {
-    const int id = ids[k];
-    const float w = ws[k];
-    if (id < 0 || id >= N || w < 1e-8f) continue;
-    float4 q = T[id].r;
-    if (qdot(q, b.r) < 0) q = make_float4(-q.x, -q.y, -q.z, -q.w);
-    b.r.x += w * q.x; b.r.y += w * q.y; b.r.z += w * q.z; b.r.w += w * q.w;
-  }
-  b = dq_normalize(b);
-  const float4 r = qmul(qmul(b.r, make_float4(nin[3 * i], nin[3 * i + 1], nin[3 * i + 2], 0.f)), qconj(b.r));
-  const float n = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
-  float3 o = f3(r.x, r.y, r.z);
-  if (n > 1e-9f) o = f3(r.x / n, r.y / n, r.z / n);
+  const float3 o = dq_warp_normal(T, N, ids, ws, f3(nin[3 * i], nin[3 * i + 1], nin[3 * i + 2]));
   out[3 * i] = o.x;
   out[3 * i + 1] = o.y;
   out[3 * i + 2] = o.z;

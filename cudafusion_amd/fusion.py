@@ -100,6 +100,21 @@ class TSDFVolume:
             _ptr(nodes), _ptr(transforms), int(num_nodes), _ptr(centered), _ptr(voxel_knn), _ptr(voxel_knn_w),
             _ptr(voxel_opt_counts), int(min_opt_count), _ptr(updated_count), _stream()), "nrf_tsdf_integrate")
 
+    def raycast(self, cam, camera_pose, nodes=None, transforms=None, num_nodes=0, voxel_knn=None, voxel_knn_w=None,
+                out_canonical_vidx: bool = False):
+        """TSDFVolume::raycast (include/tsdf_volume.h:38-44): device tensors of
+        live vertices and normals (W*H x 3) and, when asked, the canonical
+        voxel index per pixel."""
+        npx = cam.width * cam.height
+        v = torch.empty(npx * 3, dtype=torch.float32, device=_dev())
+        n = torch.empty(npx * 3, dtype=torch.float32, device=_dev())
+        idx = torch.empty(npx, dtype=torch.int32, device=_dev()) if out_canonical_vidx else None
+        pose = pose_array(camera_pose)
+        L.check(self.lib.nrf_raycast(C.byref(self.vp), _ptr(self._vox), C.byref(cam), pose.ctypes.data, _ptr(nodes),
+                                     _ptr(transforms), int(num_nodes), _ptr(voxel_knn), _ptr(voxel_knn_w), _ptr(v),
+                                     _ptr(n), _ptr(idx), _stream()), "nrf_raycast")
+        return (v.view(-1, 3), n.view(-1, 3)) + ((idx,) if out_canonical_vidx else ())
+
     def _mc(self, normals: bool, max_triangles: int | None):
         if self._mc_ws is None:
             size = self.lib.nrf_mc_workspace_size(C.byref(self.vp))

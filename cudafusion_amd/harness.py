@@ -239,6 +239,33 @@ class Harness:
         T = min(T, self.cfg.max_triangles)
         return self._d2h(v, T * 9, np.float32).reshape(-1, 3)
 
+    def save_mesh_ply(self, path: str, warped: bool = False) -> int:
+        """The pipeline's current canonical mesh as the reference's ASCII PLY
+        (save_mesh_ply, include/fusion_app.h:921-943) with per-face normals;
+        warped=True writes it deformed into the live frame
+        (save_warped_mesh_ply, :901-919), skinned on the device through the
+        voxel kNN as the live surface prediction does (the reference skins on
+        the host through each vertex's own 4 nearest nodes) with the
+        canonical normals kept, as there.  Returns the triangle count."""
+        from . import formats
+        v = self.mesh()
+        n = formats.mesh_normals(v)
+        if warped and self.N > 0 and len(v):
+            dev = self.dev
+            cen = torch.empty(self.N * 32, dtype=torch.uint8, device=dev)
+            L.check(self.lib.nrf_warp_centered_table(self.lib.nrf_pipeline_nodes(self.pipe),
+                                                     self.lib.nrf_pipeline_transforms(self.pipe), self.N,
+                                                     cen.data_ptr(), None), "centered")
+            dv = _t(v.reshape(-1), dev)
+            out = torch.empty_like(dv)
+            L.check(self.lib.nrf_skin_vertices(dv.data_ptr(), len(v), C.byref(self.vp), cen.data_ptr(), self.N,
+                                               self.lib.nrf_pipeline_knn(self.pipe),
+                                               self.lib.nrf_pipeline_knn_w(self.pipe), out.data_ptr(), None), "skin")
+            torch.cuda.synchronize()
+            v = out.cpu().numpy().reshape(-1, 3)
+        formats.save_mesh_ply(path, v, n, np.arange(len(v), dtype=np.int32).reshape(-1, 3))
+        return len(v) // 3
+
     def frame_corrs_host(self, f: int):
         npx = self.w.width * self.w.height
         corrs = self.corrs[f].cpu().numpy().view(L.CORR)

@@ -199,6 +199,23 @@ int nrf_tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels,
                        const float *voxel_knn_w, const int32_t *voxel_opt_counts,
                        int min_opt_count, int32_t *updated_count, void *stream);
 
+/* Replaces TSDFVolume::raycast (src/tsdf_volume.cu:65-92) and raycast_kernel
+   (kernels/tsdf_kernels.cu:236-361, with sample_tsdf_trilinear :14-64): per
+   pixel (integer coordinates) the ray from the camera centre marched from
+   0.05 to 3.5 m in 0.8-voxel steps (0.5 once |tsdf| < 0.5), the first
+   crossing from tsdf in (0, 1) to tsdf <= 0 interpolated linearly, the
+   normal from central differences of trilinear samples one voxel apart.
+   Trilinear samples read voxel k at origin + k s (no half-voxel offset) and
+   fail when any of the 8 voxels has weight <= 0.  Warped (warp_point /
+   warp_normal of the crossing's base-voxel 4-NN) when nodes != NULL and
+   num_nodes > 0.  Outputs (device, 3 floats per pixel; canonical_vidx one
+   int, optional): zero vectors and -1 where no surface is found.
+   camera_pose: camera-to-world, row-major. */
+int nrf_raycast(const nrf_volume_params *vp, const nrf_voxel *voxels, const nrf_camera *cam,
+                const float camera_pose[16], const nrf_node *nodes, const nrf_dualquat *transforms, int num_nodes,
+                const int32_t *voxel_knn, const float *voxel_knn_w, float *vertices, float *normals,
+                int32_t *canonical_vidx, void *stream);
+
 /* Fills voxels with the reference's initial state (tsdf=1, w=0, colour 128). */
 int nrf_tsdf_reset(const nrf_volume_params *vp, nrf_voxel *voxels, void *stream);
 

@@ -1,7 +1,5 @@
 // tsdf_volume.h - TSDFVolume with the reference's interface
 // (include/tsdf_volume.h:10-81), backed by the gfx950 kernels of libnrf.
-// raycast (a visualisation path, not part of the per-frame step) is not
-// provided.
 #pragma once
 #include <string>
 
@@ -28,6 +26,13 @@ class TSDFVolume {
                  int num_nodes = 0, const int *voxel_knn = nullptr, const float *voxel_knn_w = nullptr,
                  const int *voxel_opt_counts = nullptr, int min_opt_count = 0);
 
+  // kernels/tsdf_kernels.cu:236-361 via nrf_raycast (src/tsdf_volume.cu:
+  // 65-92): warped when nodes is set and num_nodes > 0.
+  void raycast(DeviceArray<float3> &vertices, DeviceArray<float3> &normals, const CameraIntrinsics &cam,
+               const Mat4 &camera_pose, const DeformNode *nodes = nullptr, const DualQuat *transforms = nullptr,
+               int num_nodes = 0, const int *voxel_knn = nullptr, const float *voxel_knn_w = nullptr,
+               int *out_canonical_vidx = nullptr);
+
   // Triangle soup: 3 vertices and 3 copies of the face normal per triangle.
   void extract_surface(std::vector<float3> &vertices, std::vector<float3> &normals,
                        std::vector<int3> &triangles) const;
@@ -44,7 +49,7 @@ class TSDFVolume {
     int z = idx / (params_.dims.x * params_.dims.y);
     return make_int3(x, y, z);
   }
-  // no +0.5 here (tsdf_volume.h:162-166 of the reference)
+  // no +0.5 here (include/tsdf_volume.h:72-76 of the reference)
   __host__ float3 voxel_to_world(int3 xyz) const {
     return make_float3(params_.origin.x + xyz.x * params_.voxel_size, params_.origin.y + xyz.y * params_.voxel_size,
                        params_.origin.z + xyz.z * params_.voxel_size);

@@ -1085,6 +1085,105 @@ void orc_skin_vertices_f32(const float *canon, int n, const nrf_volume_params *v
     out[3 * i] = p.x; out[3 * i + 1] = p.y; out[3 * i + 2] = p.z;
   }
 }
+// raycast_kernel (kernels/tsdf_kernels.cu:236-361) with
+// sample_tsdf_trilinear (:14-64), FP32 in the reference's order: the ray of
+// integer pixel (u, v) marched from t = 0.05 to 3.5 m (0.8 voxel steps, 0.5
+// once |tsdf| < 0.5), the first + to - crossing (previous sample in (0, 1))
+// interpolated linearly, the normal from six trilinear samples one voxel
+// apart, warped through the crossing's base-voxel 4-NN when N > 0.
+static bool ray_sample(const nrf_voxel *vox, const nrf_volume_params &vp, float px, float py, float pz, float &tsdf,
+                       int &base) {
+  const float gx = (px - vp.origin[0]) / vp.voxel_size, gy = (py - vp.origin[1]) / vp.voxel_size,
+              gz = (pz - vp.origin[2]) / vp.voxel_size;
+  auto fl = [](float v) { return (int)std::fmin(std::fmax(std::floor(v), -1073741824.f), 1073741824.f); };
+  const int x0 = fl(gx), y0 = fl(gy), z0 = fl(gz);
+  const int X = vp.dims[0], Y = vp.dims[1], Z = vp.dims[2];
+  if (x0 < 0 || x0 >= X - 1 || y0 < 0 || y0 >= Y - 1 || z0 < 0 || z0 >= Z - 1) return false;
+  const float fx = gx - (float)x0, fy = gy - (float)y0, fz = gz - (float)z0;
+  float acc = 0.0f;
+  for (int dz = 0; dz <= 1; ++dz) {
+    const float wz = dz ? fz : (1.0f - fz);
+    for (int dy = 0; dy <= 1; ++dy) {
+      const float wy = dy ? fy : (1.0f - fy);
+      for (int dx = 0; dx <= 1; ++dx) {
+        const float wx = dx ? fx : (1.0f - fx);
+        const nrf_voxel &v = vox[((long)(z0 + dz) * Y + (y0 + dy)) * X + (x0 + dx)];
+        if (v.weight <= 0.0f) return false;
+        acc += wx * wy * wz * v.tsdf;
+      }
+    }
+  }
+  tsdf = acc;
+  base = (z0 * Y + y0) * X + x0;
+  return true;
+}
+
+void orc_raycast_f32(const nrf_volume_params *vp, const nrf_voxel *vox, const nrf_camera *cam, const float pose[16],
+                     const nrf_node *nodes, const nrf_dualquat *Tq, int N, const int32_t *knn, const float *knn_w,
+                     float *verts, float *normals, int32_t *vidx_out) {
+  std::vector<DQ<float>> T, cen;
+  if (N > 0) {
+    T = load_T<float>(Tq, N);
+    cen = centered_table(nodes, T.data(), N);
+  }
+  const float s = vp->voxel_size;
+  auto nrm = [](float x, float y, float z) {
+    const float n = std::sqrt(x * x + y * y + z * z);
+    return n > 1e-9f ? V3<float>{x / n, y / n, z / n} : V3<float>{x, y, z};
+  };
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int v = 0; v < cam->height; ++v)
+    for (int u = 0; u < cam->width; ++u) {
+      const long px = (long)v * cam->width + u;
+      V3<float> ov{0, 0, 0}, on{0, 0, 0};
+      int oi = -1;
+      const float dcx = 1.0f * ((float)u - cam->cx) / cam->fx, dcy = 1.0f * ((float)v - cam->cy) / cam->fy,
+                  dcz = 1.0f;
+      const float ox = pose[0] * 0.f + pose[1] * 0.f + pose[2] * 0.f + pose[3];
+      const float oy = pose[4] * 0.f + pose[5] * 0.f + pose[6] * 0.f + pose[7];
+      const float oz = pose[8] * 0.f + pose[9] * 0.f + pose[10] * 0.f + pose[11];
+      const V3<float> d = nrm(pose[0] * dcx + pose[1] * dcy + pose[2] * dcz, pose[4] * dcx + pose[5] * dcy + pose[6] * dcz,
+                              pose[8] * dcx + pose[9] * dcy + pose[10] * dcz);
+      float t = 0.05f, step = s * 0.8f, prev = 0.f, t_prev = t;
+      bool have_prev = false;
+      for (; t < 3.5f; t += step) {
+        float cur = 1.0f;
+        int vi = -1;
+        if (!ray_sample(vox, *vp, ox + t * d.x, oy + t * d.y, oz + t * d.z, cur, vi)) {
+          have_prev = false;
+          continue;
+        }
+        if (have_prev && prev > 0 && cur <= 0 && prev < 1.0f) {
+          const float ts = t - (t - t_prev) * cur / (cur - prev);
+          const float sx = ox + ts * d.x, sy = oy + ts * d.y, sz = oz + ts * d.z;
+          float g[6];
+          int un = -1;
+          const bool ok = ray_sample(vox, *vp, sx + s, sy, sz, g[0], un) && ray_sample(vox, *vp, sx - s, sy, sz, g[1], un) &&
+                          ray_sample(vox, *vp, sx, sy + s, sz, g[2], un) && ray_sample(vox, *vp, sx, sy - s, sz, g[3], un) &&
+                          ray_sample(vox, *vp, sx, sy, sz + s, g[4], un) && ray_sample(vox, *vp, sx, sy, sz - s, g[5], un);
+          if (!ok) break;
+          V3<float> n = nrm(g[0] - g[1], g[2] - g[3], g[4] - g[5]);
+          oi = vi;
+          V3<float> p{sx, sy, sz};
+          if (N > 0 && nodes) {
+            p = warp_point(p, cen.data(), N, knn + 4 * (long)vi, knn_w + 4 * (long)vi);
+            n = warp_normal(n, T.data(), N, knn + 4 * (long)vi, knn_w + 4 * (long)vi);
+          }
+          ov = p;
+          on = n;
+          break;
+        }
+        prev = cur;
+        have_prev = true;
+        t_prev = t;
+        step = std::fabs(cur) < 0.5f ? s * 0.5f : s * 0.8f;
+      }
+      verts[3 * px] = ov.x; verts[3 * px + 1] = ov.y; verts[3 * px + 2] = ov.z;
+      normals[3 * px] = on.x; normals[3 * px + 1] = on.y; normals[3 * px + 2] = on.z;
+      if (vidx_out) vidx_out[px] = oi;
+    }
+}
+
 // upload_scaled_delta_x(scale) (include/fusion_app.h:1353-1375), then
 // apply_twist_increment(trial, max_rot, max_trans, 1) (src/warp_field.cu:
 // 1121-1141) on every node, in place.

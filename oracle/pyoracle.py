@@ -60,6 +60,7 @@ _SIGS = {
     "orc_validate_f64": (I, [P, I, P]),
     "orc_apply_step_f32": (None, [P, P, I, P, F]),
     "orc_corr_rmse": (D, [P, I]),
+    "orc_raycast_f32": (None, [P, P, P, P, P, P, I, P, P, P, P, P]),
     "orc_solve_gn_f32": (I, [P, P, I, P, I, P, P, P, P, P, P, P]),
     "orc_solve_gn_f64": (I, [P, P, I, P, I, P, P, P, P, P, P, P]),
     "orc_lp_param_count": (I, []),
@@ -268,6 +269,20 @@ def rasterize_f32(canon, warped, tris, cam, pose, vp):
     load().orc_rasterize_f32(ptr(c), ptr(w), ptr(t), c.size // 3, t.size // 3, C.byref(cam), ptr(P_), C.byref(vp),
                              ptr(lv), ptr(ln), ptr(hit), ptr(cp))
     return lv, ln, hit, cp
+
+
+def raycast_f32(vp, vox, cam, pose, nodes=None, T=None, knn=None, knn_w=None):
+    """TSDFVolume::raycast (src/tsdf_volume.cu:65-92): (vertices, normals,
+    canonical voxel index) per pixel."""
+    npx = cam.width * cam.height
+    v = np.zeros(npx * 3, np.float32)
+    n = np.zeros(npx * 3, np.float32)
+    idx = np.zeros(npx, np.int32)
+    P_ = np.ascontiguousarray(pose, np.float32)
+    N = 0 if nodes is None else len(nodes)
+    load().orc_raycast_f32(C.byref(vp), ptr(vox), C.byref(cam), ptr(P_), ptr(nodes), ptr(T), N, ptr(knn), ptr(knn_w),
+                           ptr(v), ptr(n), ptr(idx))
+    return v, n, idx
 
 
 def pattern(nodes):

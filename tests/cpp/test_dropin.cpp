@@ -101,6 +101,22 @@ static void tsdf_integration() {
     near += x.weight > 0 && std::fabs(x.tsdf) < 1.f;
   }
   CHECK(obs > 50000 && near > 5000, "tsdf_integration: %ld observed voxels, %ld inside the truncation band", obs, near);
+  // raycast of the integrated sphere (the valid-ray check of
+  // tests/test_tsdf_integration.cu:600-610): rays hit the sphere's surface
+  DeviceArray<float3> rv, rn;
+  vol.raycast(rv, rn, cam, pose);
+  std::vector<float3> hv;
+  rv.download(hv);
+  long hits = 0, on_sphere = 0, disk = 0;
+  for (int i = 0; i < 640 * 480; ++i) {
+    disk += depth[i] > 0;
+    const float3 q = hv[i];
+    if (q.x == 0.f && q.y == 0.f && q.z == 0.f) continue;
+    ++hits;
+    on_sphere += std::fabs(std::sqrt(q.x * q.x + q.y * q.y + q.z * q.z) - r) < 2 * p.voxel_size;
+  }
+  CHECK(hits > disk / 2 && on_sphere == hits, "raycast: %ld of %ld sphere pixels hit, %ld within 2 voxels of r",
+        hits, disk, on_sphere);
 }
 
 static std::vector<DeformNode> ring_nodes(int n) {

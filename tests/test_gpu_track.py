@@ -140,3 +140,28 @@ def test_track_sequence_runs_and_keeps_tracking(gpu):
         runs[s16] = rec
         h.close()
     assert runs[False][0][:3] == runs[True][0][:3]
+
+
+def test_mesh_export_after_tracking(gpu, tmp_path):
+    """The PLY export of the pipeline's mesh (canonical and warped into the
+    live frame) after a tracked frame: the file holds every triangle, the
+    canonical vertices equal the pipeline's mesh, and the warped ones are the
+    device skinning of them (moved by the frame's non-rigid motion)."""
+    from cudafusion_amd import _lib as L
+    from cudafusion_amd import formats as F
+    from cudafusion_amd.harness import Harness
+    h = Harness("sphere", frames=1, validate=True)
+    h.setup()
+    h.track(1, L.track_params(min_valid_corrs=2000, line_search=1))
+    nt = h.triangle_count()
+    assert h.save_mesh_ply(str(tmp_path / "c.ply")) == nt
+    assert h.save_mesh_ply(str(tmp_path / "w.ply"), warped=True) == nt
+    cv, cn, ct = F.read_ply(str(tmp_path / "c.ply"))
+    wv, wn, wt = F.read_ply(str(tmp_path / "w.ply"))
+    assert len(ct) == nt and len(cv) == 3 * nt
+    np.testing.assert_allclose(cv, h.mesh(), rtol=1e-5, atol=1e-6)
+    np.testing.assert_array_equal(cn, wn)
+    disp = np.linalg.norm(wv - cv, axis=1)
+    print(f"[export] {nt} triangles, warped displacement mean {disp.mean():.2e} max {disp.max():.2e} m", flush=True)
+    assert 1e-5 < disp.max() < 0.05
+    h.close()
