@@ -67,6 +67,20 @@ class TrackStats(C.Structure):
                 ("best_rmse", C.c_double * TRACK_MAX_GN)]
 
 
+class PruneParams(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("remove_disconnected", C.c_int32), ("min_observed_voxels", C.c_int32),
+                ("min_surface_voxels", C.c_int32), ("min_component_size", C.c_int32),
+                ("support_radius_factor", C.c_float), ("surface_tsdf_abs", C.c_float), ("empty_tsdf", C.c_float)]
+
+
+def prune_params(enabled=True, remove_disconnected=True, min_observed_voxels=8, min_surface_voxels=2,
+                 min_component_size=4, support_radius_factor=1.0, surface_tsdf_abs=0.45,
+                 empty_tsdf=0.75) -> "PruneParams":
+    """Defaults are WarpField::PruneParams' (include/warp_field.h:44-53)."""
+    return PruneParams(int(enabled), int(remove_disconnected), min_observed_voxels, min_surface_voxels,
+                       min_component_size, support_radius_factor, surface_tsdf_abs, empty_tsdf)
+
+
 class SolverParams(C.Structure):
     _fields_ = [("gn_iterations", C.c_int32), ("pcg_iterations", C.c_int32), ("pcg_tolerance", C.c_float),
                 ("lambda_smooth", C.c_float), ("lambda_rot", C.c_float), ("lambda_damping", C.c_float),
@@ -178,6 +192,10 @@ _SIGS = {
     "nrf_pipeline_profile_read": (I, [P, I, C.POINTER(F)]),
     "nrf_pipeline_updated_voxels": (C.c_longlong, [P]),
     "nrf_raycast": (I, [C.POINTER(VolumeParams), P, C.POINTER(Camera), P, P, P, I, P, P, P, P, P, P]),
+    "nrf_graph_add_nodes": (I, [P, P, C.POINTER(I), I, P, P, I, C.c_float, C.c_float, C.POINTER(I)]),
+    "nrf_graph_add_nodes_geodesic": (I, [P, P, C.POINTER(I), I, P, P, I, P, I, C.c_float, C.c_float, C.POINTER(I),
+                                         P]),
+    "nrf_graph_prune": (I, [C.POINTER(VolumeParams), P, P, P, C.POINTER(I), C.POINTER(PruneParams), C.POINTER(I), P]),
     "nrf_pipeline_track": (I, [P, P, P, C.POINTER(TrackParams), I, P]),
     "nrf_pipeline_track_stats": (I, [P, C.POINTER(TrackStats)]),
     "nrf_pipeline_track_maps": (I, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P),

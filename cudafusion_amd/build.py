@@ -22,7 +22,7 @@ LIB = os.path.join(PKG, "libnrf.so")
 
 HIP_SOURCES = ["common.hip", "integrate.hip", "mc.hip", "warp.hip", "solver.hip", "pipeline.hip", "precision.hip",
                "learned.hip", "association.hip", "raster.hip", "track.hip",
-               "raycast.hip"]
+               "raycast.hip", "graph.hip"]
 CXX_SOURCES = ["dropin.cpp"]
 
 HIPCC_FLAGS = [

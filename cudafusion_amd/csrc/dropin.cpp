@@ -150,6 +150,91 @@ WarpField::WarpField(float node_radius, int max_nodes)
   h_transforms_.reserve(max_nodes);
 }
 
+// the host mirror after a graph change, uploaded as the reference does
+// (nodes always; transforms and the previous transforms)
+void WarpField::sync_graph(int) {
+  NRF_DEVICE_CHECK(hipMemcpy(d_nodes_.data, h_nodes_.data(), num_nodes_ * sizeof(DeformNode), hipMemcpyHostToDevice));
+  NRF_DEVICE_CHECK(hipMemcpy(d_transforms_.data, h_transforms_.data(), num_nodes_ * sizeof(DualQuat),
+                             hipMemcpyHostToDevice));
+  NRF_DEVICE_CHECK(hipMemcpy(d_transforms_prev_.data, h_transforms_.data(), num_nodes_ * sizeof(DualQuat),
+                             hipMemcpyHostToDevice));
+}
+
+int WarpField::add_nodes_from_surface(const std::vector<float3> &surface_vertices, float min_dist,
+                                      const std::vector<float3> *surface_normals) {
+  const int old = num_nodes_;
+  h_nodes_.resize(max_nodes_);
+  h_transforms_.resize(max_nodes_);
+  int n = num_nodes_, added = 0;
+  const float *nrm = surface_normals && surface_normals->size() >= surface_vertices.size()
+                         ? reinterpret_cast<const float *>(surface_normals->data())
+                         : nullptr;
+  NRF_STATUS_CHECK(nrf_graph_add_nodes(reinterpret_cast<nrf_node *>(h_nodes_.data()),
+                                       reinterpret_cast<nrf_dualquat *>(h_transforms_.data()), &n, max_nodes_,
+                                       reinterpret_cast<const float *>(surface_vertices.data()), nrm,
+                                       (int)surface_vertices.size(), min_dist, node_radius_, &added));
+  num_nodes_ = n;
+  h_nodes_.resize(n);
+  h_transforms_.resize(n);
+  if (added > 0) sync_graph(old);
+  return added;
+}
+
+int WarpField::add_nodes_from_mesh_geodesic(const std::vector<float3> &surface_vertices,
+                                            const std::vector<int3> &surface_triangles, float min_dist,
+                                            const std::vector<float3> *surface_normals) {
+  const int old = num_nodes_;
+  h_nodes_.resize(max_nodes_);
+  h_transforms_.resize(max_nodes_);
+  int n = num_nodes_, added = 0;
+  const float *nrm = surface_normals && surface_normals->size() >= surface_vertices.size()
+                         ? reinterpret_cast<const float *>(surface_normals->data())
+                         : nullptr;
+  NRF_STATUS_CHECK(nrf_graph_add_nodes_geodesic(
+      reinterpret_cast<nrf_node *>(h_nodes_.data()), reinterpret_cast<nrf_dualquat *>(h_transforms_.data()), &n,
+      max_nodes_, reinterpret_cast<const float *>(surface_vertices.data()), nrm, (int)surface_vertices.size(),
+      reinterpret_cast<const int32_t *>(surface_triangles.data()), (int)surface_triangles.size(), min_dist,
+      node_radius_, &added, nullptr));
+  num_nodes_ = n;
+  h_nodes_.resize(n);
+  h_transforms_.resize(n);
+  if (added > 0) sync_graph(old);
+  return added;
+}
+
+int WarpField::prune_nodes(const TSDFVolume &volume, const PruneParams &params) {
+  if (!params.enabled || num_nodes_ == 0) return 0;
+  const auto &p = volume.params();
+  nrf_volume_params v;
+  v.dims[0] = p.dims.x; v.dims[1] = p.dims.y; v.dims[2] = p.dims.z;
+  v.voxel_size = p.voxel_size;
+  v.truncation = p.truncation;
+  v.origin[0] = p.origin.x; v.origin[1] = p.origin.y; v.origin[2] = p.origin.z;
+  v.decay_alpha = p.decay_alpha;
+  v.max_weight = p.max_weight;
+  nrf_prune_params pp;
+  pp.enabled = params.enabled ? 1 : 0;
+  pp.remove_disconnected = params.remove_disconnected ? 1 : 0;
+  pp.min_observed_voxels = params.min_observed_voxels;
+  pp.min_surface_voxels = params.min_surface_voxels;
+  pp.min_component_size = params.min_component_size;
+  pp.support_radius_factor = params.support_radius_factor;
+  pp.surface_tsdf_abs = params.surface_tsdf_abs;
+  pp.empty_tsdf = params.empty_tsdf;
+  const int old = num_nodes_;
+  int n = num_nodes_, removed = 0;
+  h_transforms_.resize(num_nodes_);
+  NRF_STATUS_CHECK(nrf_graph_prune(&v, reinterpret_cast<const nrf_voxel *>(volume.device_data()),
+                                   reinterpret_cast<nrf_node *>(h_nodes_.data()),
+                                   reinterpret_cast<nrf_dualquat *>(h_transforms_.data()), &n, &pp, &removed,
+                                   nullptr));
+  num_nodes_ = n;
+  h_nodes_.resize(n);
+  h_transforms_.resize(n);
+  if (removed > 0) sync_graph(old);
+  return removed;
+}
+
 void WarpField::compute_voxel_knn(const TSDFVolume &volume, DeviceArray<int> &d_voxel_knn,
                                   DeviceArray<float> &d_voxel_knn_w) {
   if (num_nodes_ == 0) return;

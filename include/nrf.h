@@ -318,6 +318,65 @@ int nrf_corr_pack(const nrf_volume_params *vp, const int32_t *voxel_knn,
                   const float *dst, const float *normals, const float *weights,
                   const uint8_t *valid, int count, nrf_corr *out, void *stream);
 
+/* ------------------------------------------------------ graph maintenance */
+/* The graph-update frames of the reference.  nodes / transforms are HOST
+   arrays of capacity max_nodes (the reference keeps this host mirror and
+   uploads it afterwards); *num_nodes is updated.  A pipeline picks up a
+   changed graph with nrf_pipeline_set_nodes. */
+
+/* Replaces WarpField::add_nodes_from_surface (src/warp_field.cu:709-841):
+   greedy, in vertex order, a vertex becomes a node unless an existing node
+   lies closer than min_dist; its graph neighbours are the (at most 8)
+   nearest nodes within max(2.5 min_dist, node_radius) whose normals agree
+   (dot >= 0.7), weight 1; its transform the 1/(d + 1e-6)-weighted blend of
+   those that existed before the call (identity without any).  normals may
+   be NULL. */
+int nrf_graph_add_nodes(nrf_node *nodes, nrf_dualquat *transforms, int *num_nodes, int max_nodes,
+                        const float *vertices, const float *normals, int num_vertices, float min_dist,
+                        float node_radius, int *added);
+
+/* Replaces WarpField::add_nodes_from_mesh_geodesic (src/warp_field.cu:
+   843-954, with geodesic_neighbors_for_nodes_gpu :596-694 and kernels K25-
+   K27 :488-590): nrf_graph_add_nodes, then, when nodes were added, every
+   node's edges rebuilt from geodesic distances over the mesh (vertices
+   welded on a 0.05 min_dist lattice, triangle edges weighted by length,
+   each node at its nearest graph vertex): the (at most 8) nearest other
+   nodes within max(3.5 min_dist, 1.5 node_radius) whose normals agree,
+   weight exp(-d^2 / 2 r^2); new nodes' transforms blended from their
+   pre-existing geodesic neighbours (weights 1 / (sqrt d + 1e-6)).  The
+   distances are the converged bounded shortest paths (the reference stops
+   its relaxation after 96 rounds).  triangles: 3 indices into vertices
+   each (the marching-cubes soup: {3t, 3t+1, 3t+2}).  NRF_ERR_TOO_LARGE when
+   a node's neighbourhood exceeds the device table (12288 graph vertices
+   within the geodesic radius). */
+int nrf_graph_add_nodes_geodesic(nrf_node *nodes, nrf_dualquat *transforms, int *num_nodes, int max_nodes,
+                                 const float *vertices, const float *normals, int num_vertices,
+                                 const int32_t *triangles, int num_triangles, float min_dist, float node_radius,
+                                 int *added, void *stream);
+
+/* = WarpField::PruneParams (include/warp_field.h:44-53 and defaults). */
+typedef struct nrf_prune_params {
+  int32_t enabled;               /* 1 */
+  int32_t remove_disconnected;   /* 1 */
+  int32_t min_observed_voxels;   /* 8 */
+  int32_t min_surface_voxels;    /* 2 */
+  int32_t min_component_size;    /* 4 */
+  float support_radius_factor;   /* 1.0 */
+  float surface_tsdf_abs;        /* 0.45 */
+  float empty_tsdf;              /* 0.75 */
+} nrf_prune_params;
+
+/* Replaces WarpField::prune_nodes (src/warp_field.cu:986-1104) and
+   score_node_tsdf_support_kernel (:407-472): a node is dropped when the
+   observed voxels of its ball (radius max(radius * factor, voxel)) are
+   confidently empty without surface support, then (remove_disconnected)
+   every graph component smaller than min_component_size; the survivors are
+   compacted in order with their neighbour lists remapped.  voxels: device;
+   nodes / transforms: host. */
+int nrf_graph_prune(const nrf_volume_params *vp, const nrf_voxel *voxels, nrf_node *nodes,
+                    nrf_dualquat *transforms, int *num_nodes, const nrf_prune_params *params, int *removed,
+                    void *stream);
+
 /* ------------------------------------------------ projective association */
 /* The step that produces the solver's correspondences and the integration's
    optimised-voxel counts from the live surface prediction (per-pixel live

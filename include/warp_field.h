@@ -1,9 +1,7 @@
 // warp_field.h - WarpField with the reference's interface
-// (include/warp_field.h:12-126) for the per-frame step: node / transform
-// storage, voxel 4-NN snapshot, twist update, save / restore.  Node sampling
-// and graph maintenance (add_nodes_from_surface, prune_nodes) run on graph
-// updates, not per frame, and are not part of this library: nodes are
-// supplied with upload_nodes.
+// (include/warp_field.h:12-126): node / transform storage, graph
+// maintenance (node sampling, geodesic edges, pruning), voxel 4-NN
+// snapshot, twist update, save / restore.
 #pragma once
 #include <vector>
 
@@ -17,6 +15,27 @@ class WarpField {
 
   WarpField(float node_radius, int max_nodes = MAX_NODES);
   ~WarpField() = default;
+
+  // src/warp_field.cu:709-841 via nrf_graph_add_nodes
+  int add_nodes_from_surface(const std::vector<float3> &surface_vertices, float min_dist_between_nodes,
+                             const std::vector<float3> *surface_normals = nullptr);
+  // src/warp_field.cu:843-954 via nrf_graph_add_nodes_geodesic
+  int add_nodes_from_mesh_geodesic(const std::vector<float3> &surface_vertices,
+                                   const std::vector<int3> &surface_triangles, float min_dist_between_nodes,
+                                   const std::vector<float3> *surface_normals = nullptr);
+
+  struct PruneParams {
+    bool enabled = true;
+    bool remove_disconnected = true;
+    int min_observed_voxels = 8;
+    int min_surface_voxels = 2;
+    int min_component_size = 4;
+    float support_radius_factor = 1.0f;
+    float surface_tsdf_abs = 0.45f;
+    float empty_tsdf = 0.75f;
+  };
+  // src/warp_field.cu:986-1104 via nrf_graph_prune
+  int prune_nodes(const TSDFVolume &volume, const PruneParams &params);
 
   int num_nodes() const { return num_nodes_; }
 
@@ -42,6 +61,7 @@ class WarpField {
   void upload_transforms(const std::vector<DualQuat> &transforms);
 
  private:
+  void sync_graph(int old_count);
   float node_radius_;
   int num_nodes_;
   int max_nodes_;

@@ -61,6 +61,9 @@ _SIGS = {
     "orc_apply_step_f32": (None, [P, P, I, P, F]),
     "orc_corr_rmse": (D, [P, I]),
     "orc_raycast_f32": (None, [P, P, P, P, P, P, I, P, P, P, P, P]),
+    "orc_graph_add_nodes": (I, [P, P, P, I, P, P, I, F, F]),
+    "orc_graph_add_nodes_geodesic": (I, [P, P, P, I, P, P, I, P, I, F, F, P, P]),
+    "orc_graph_prune": (I, [P, P, P, P, P, P, P]),
     "orc_solve_gn_f32": (I, [P, P, I, P, I, P, P, P, P, P, P, P]),
     "orc_solve_gn_f64": (I, [P, P, I, P, I, P, P, P, P, P, P, P]),
     "orc_lp_param_count": (I, []),
@@ -283,6 +286,35 @@ def raycast_f32(vp, vox, cam, pose, nodes=None, T=None, knn=None, knn_w=None):
     load().orc_raycast_f32(C.byref(vp), ptr(vox), C.byref(cam), ptr(P_), ptr(nodes), ptr(T), N, ptr(knn), ptr(knn_w),
                            ptr(v), ptr(n), ptr(idx))
     return v, n, idx
+
+
+def graph_add_nodes(nodes, T, num_nodes, verts, normals, min_dist, radius, tris=None):
+    """add_nodes_from_surface (tris None) or add_nodes_from_mesh_geodesic on
+    host arrays of capacity len(nodes) (updated in place); returns (added,
+    num_nodes, geodesic ids, geodesic distances) (the last two None without
+    tris)."""
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1)
+    nrm = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1)
+    n = C.c_int(num_nodes)
+    if tris is None:
+        a = load().orc_graph_add_nodes(ptr(nodes), ptr(T), C.byref(n), len(nodes), ptr(v), ptr(nrm), v.size // 3,
+                                       min_dist, radius)
+        return a, n.value, None, None
+    t = np.ascontiguousarray(tris, np.int32).reshape(-1)
+    gi = np.zeros(len(nodes) * 8, np.int32)
+    gd = np.zeros(len(nodes) * 8, np.float32)
+    a = load().orc_graph_add_nodes_geodesic(ptr(nodes), ptr(T), C.byref(n), len(nodes), ptr(v), ptr(nrm),
+                                            v.size // 3, ptr(t), t.size // 3, min_dist, radius, ptr(gi), ptr(gd))
+    return a, n.value, gi.reshape(-1, 8)[: n.value], gd.reshape(-1, 8)[: n.value]
+
+
+def graph_prune(vp, vox, nodes, T, num_nodes, pp):
+    """prune_nodes (warp_field.cu:986-1104) in place; returns (removed,
+    num_nodes, keep flags before compaction)."""
+    n = C.c_int(num_nodes)
+    keep = np.zeros(max(num_nodes, 1), np.uint8)
+    r = load().orc_graph_prune(C.byref(vp), ptr(vox), ptr(nodes), ptr(T), C.byref(n), C.byref(pp), ptr(keep))
+    return r, n.value, keep[:num_nodes]
 
 
 def pattern(nodes):

@@ -117,6 +117,21 @@ static void tsdf_integration() {
   }
   CHECK(hits > disk / 2 && on_sphere == hits, "raycast: %ld of %ld sphere pixels hit, %ld within 2 voxels of r",
         hits, disk, on_sphere);
+  // graph maintenance on the extracted surface (initialize_node_graph,
+  // include/fusion_app.h:994-1003): geodesic node graph, then pruning
+  std::vector<float3> sv, sn;
+  std::vector<int3> st;
+  vol.extract_surface(sv, sn, st);
+  WarpField wf(0.04f);
+  const int added = wf.add_nodes_from_mesh_geodesic(sv, st, 0.025f, &sn);
+  std::vector<DeformNode> gn = wf.download_nodes();
+  long edges = 0;
+  for (const DeformNode &x : gn) edges += x.num_neighbors;
+  CHECK(added > 50 && wf.num_nodes() == added && edges > 3L * added,
+        "graph: %d nodes sampled on the surface, %ld geodesic edges", added, edges);
+  const int removed = wf.prune_nodes(vol, WarpField::PruneParams());
+  CHECK(removed >= 0 && wf.num_nodes() == added - removed && (int)wf.download_transforms().size() == wf.num_nodes(),
+        "graph: pruning removed %d of %d nodes", removed, added);
 }
 
 static std::vector<DeformNode> ring_nodes(int n) {
