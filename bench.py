@@ -437,6 +437,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "capsule-body 256^3 @4mm, 640x480, %d nodes, 5 GN x 30 PCG" % h.N,
                    "graph": bool(args.graph),
+                   "graph_captures": int(lib.nrf_pipeline_graph_captures(h.pipe)) if args.graph else None,
                    "solver_policy": ("reference (ARAP sign, whole-step rejection)" if ref else
                                      "consistent ARAP sign, per-node clamp, no whole-step rejection"),
                    "pcg_layout": args.pcg_layout or "4,24",

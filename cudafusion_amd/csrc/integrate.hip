@@ -35,6 +35,7 @@ struct IntegrateArgs {
   int W, H;
   float fx, fy, cx, cy;
   CamXf T;
+  const CamXf *Tdev;  // optional: the world-to-camera transform read from device memory instead of T
   const nrf_node *nodes;
   const DQ *transforms;
   const DQ *centered;
@@ -117,6 +118,7 @@ __device__ void integrate_tile(const IntegrateArgs &a, const Vox &vs, const Knn 
   const int y = by * kTileY + (threadIdx.x / kTileX);
   const int z0 = bz * ZC;
   int n_upd = 0;
+  const CamXf T = a.Tdev ? *a.Tdev : a.T;
   if (x < a.X && y < a.Y) {
     const float px_w = a.ox + (x + 0.5f) * a.s;
     const float py_w = a.oy + (y + 0.5f) * a.s;
@@ -166,9 +168,9 @@ __device__ void integrate_tile(const IntegrateArgs &a, const Vox &vs, const Knn 
           if (wsum >= 1e-8f) p = dq_transform(dq_normalize(b), p);
         }
       }
-      const float cxp = a.T.m[0][0] * p.x + a.T.m[0][1] * p.y + a.T.m[0][2] * p.z + a.T.m[0][3];
-      const float cyp = a.T.m[1][0] * p.x + a.T.m[1][1] * p.y + a.T.m[1][2] * p.z + a.T.m[1][3];
-      const float czp = a.T.m[2][0] * p.x + a.T.m[2][1] * p.y + a.T.m[2][2] * p.z + a.T.m[2][3];
+      const float cxp = T.m[0][0] * p.x + T.m[0][1] * p.y + T.m[0][2] * p.z + T.m[0][3];
+      const float cyp = T.m[1][0] * p.x + T.m[1][1] * p.y + T.m[1][2] * p.z + T.m[1][3];
+      const float czp = T.m[2][0] * p.x + T.m[2][1] * p.y + T.m[2][2] * p.z + T.m[2][3];
       if (czp <= 0) continue;
       const float fu = a.fx * cxp / czp + a.cx;
       const float fv = a.fy * cyp / czp + a.cy;
@@ -245,6 +247,7 @@ __global__ __launch_bounds__(256) void integrate_batch_kernel(const IntBatchDev 
   IntegrateArgs a = d.a;
   a.depth = fr.depth[lo];
   a.T = fr.xf[lo];
+  a.Tdev = nullptr;
   const int l = t - tile_base[lo];
   const int bx = l % d.tiles_x, r = l / d.tiles_x;
   integrate_tile<true, true, ZC>(a, VoxAoS{a.vox}, KnnF32{a.knn, a.knn_w}, bx, r % d.tiles_y, r / d.tiles_y);
@@ -386,11 +389,21 @@ int knn_row_flags(const nrf_volume_params *vp, const int32_t *knn, uint32_t *row
   return NRF_OK;
 }
 
+__global__ void set_pose_kernel(CamXf *dev, CamXf T) {
+  if (threadIdx.x == 0) *dev = T;
+}
+int set_device_pose(CamXf *dev, const float camera_pose[16], hipStream_t st) {
+  if (!dev || !camera_pose) return NRF_ERR_INVALID_ARGUMENT;
+  set_pose_kernel<<<1, 64, 0, st>>>(dev, invert_pose(camera_pose));
+  NRF_LAUNCH_CHECK();
+  return NRF_OK;
+}
+
 int tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels, const float *depth, const nrf_camera *cam,
                    const float camera_pose[16], const nrf_node *nodes, const nrf_dualquat *transforms, int num_nodes,
                    const nrf_dualquat *centered, const int32_t *voxel_knn, const float *voxel_knn_w,
                    const uint32_t *knn_rows, const int32_t *voxel_opt_counts, int min_opt_count,
-                   int32_t *updated_count, hipStream_t st) {
+                   int32_t *updated_count, hipStream_t st, const void *dev_pose) {
   if (!volume_ok(vp) || !voxels || !depth || !cam || !camera_pose) return NRF_ERR_INVALID_ARGUMENT;
   if (cam->width <= 0 || cam->height <= 0) return NRF_ERR_INVALID_ARGUMENT;
   const bool warp = nodes != nullptr && num_nodes > 0;
@@ -405,6 +418,7 @@ int tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels, const float *
   a.W = cam->width; a.H = cam->height;
   a.fx = cam->fx; a.fy = cam->fy; a.cx = cam->cx; a.cy = cam->cy;
   a.T = invert_pose(camera_pose);
+  a.Tdev = static_cast<const CamXf *>(dev_pose);
   a.nodes = nodes;
   a.transforms = reinterpret_cast<const DQ *>(transforms);
   a.centered = reinterpret_cast<const DQ *>(centered);
@@ -432,7 +446,8 @@ int tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels, const float *
 int tsdf16_integrate(const nrf_volume_params *vp, uint16_t *tsdf16, uint8_t *weight8, const float *depth,
                      const nrf_camera *cam, const float camera_pose[16], const nrf_dualquat *centered, int num_nodes,
                      const uint16_t *knn16, const uint16_t *knn_w16, const uint32_t *knn_rows,
-                     int32_t *updated_count, hipStream_t st, const int32_t *voxel_opt_counts, int min_opt_count) {
+                     int32_t *updated_count, hipStream_t st, const int32_t *voxel_opt_counts, int min_opt_count,
+                     const void *dev_pose) {
   if (!volume_ok(vp) || !tsdf16 || !weight8 || !depth || !cam || !camera_pose) return NRF_ERR_INVALID_ARGUMENT;
   if (cam->width <= 0 || cam->height <= 0) return NRF_ERR_INVALID_ARGUMENT;
   if (num_nodes > 0 && (!centered || !knn16 || !knn_w16)) return NRF_ERR_INVALID_ARGUMENT;
@@ -448,6 +463,7 @@ int tsdf16_integrate(const nrf_volume_params *vp, uint16_t *tsdf16, uint8_t *wei
   a.W = cam->width; a.H = cam->height;
   a.fx = cam->fx; a.fy = cam->fy; a.cx = cam->cx; a.cy = cam->cy;
   a.T = invert_pose(camera_pose);
+  a.Tdev = static_cast<const CamXf *>(dev_pose);
   a.nodes = nullptr;
   a.transforms = nullptr;
   a.centered = reinterpret_cast<const DQ *>(centered);

@@ -199,13 +199,18 @@ int tsdf16_integrate(const nrf_volume_params *vp, uint16_t *tsdf16, uint8_t *wei
                      const nrf_camera *cam, const float camera_pose[16], const nrf_dualquat *centered, int num_nodes,
                      const uint16_t *knn16, const uint16_t *knn_w16, const uint32_t *knn_rows,
                      int32_t *updated_count, hipStream_t st, const int32_t *voxel_opt_counts = nullptr,
-                     int min_opt_count = 0);
+                     int min_opt_count = 0, const void *dev_pose = nullptr);
+// dev_pose (tsdf_integrate / tsdf16_integrate): a device CamXf (the
+// world-to-camera transform, as invert_pose gives it) that the kernel reads
+// instead of camera_pose's inverse, so a captured launch can take each
+// replay's pose from memory (set_device_pose writes it stream-ordered).
+int set_device_pose(CamXf *dev, const float camera_pose[16], hipStream_t st);
 // nrf_tsdf_integrate with the kNN's row occupancy (null: none)
 int tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels, const float *depth, const nrf_camera *cam,
                    const float camera_pose[16], const nrf_node *nodes, const nrf_dualquat *transforms, int num_nodes,
                    const nrf_dualquat *centered, const int32_t *voxel_knn, const float *voxel_knn_w,
                    const uint32_t *knn_rows, const int32_t *voxel_opt_counts, int min_opt_count,
-                   int32_t *updated_count, hipStream_t st);
+                   int32_t *updated_count, hipStream_t st, const void *dev_pose = nullptr);
 
 }  // namespace nrf
 

@@ -6,6 +6,7 @@
 // integration -> marching cubes (the extract_surface_device form).  Nothing
 // synchronises the host inside a step, so a step can be captured into a
 // hipGraph and replayed.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -46,11 +47,17 @@ struct nrf_pipeline {
   std::vector<hipEvent_t> prof;
   int prof_ring = 0, prof_per = 0, prof_G = 0, prof_pcg = 0;
   long prof_steps = 0;
-  const void *g_key[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int g_n = -1;
-  float g_pose[16];
-  void *g_stream = nullptr;
+  bool g_has_can = false;
   unsigned g_gen = 0;
+  long long g_captures = 0;
+  // frame inputs of the captured step: each step copies its depth,
+  // correspondences and canonical points here and writes its pose to g_pose,
+  // so one capture serves every frame
+  float *g_depth = nullptr;
+  nrf_corr *g_corrs = nullptr;
+  float *g_canon = nullptr;
+  CamXf *g_pose = nullptr;
   hipStream_t cap = nullptr;  // capture stream (the caller's may be the null stream)
   bool graph_step = false;    // the last step was a graph replay (stage events: whole step only)
   bool mesh_valid = false;    // verts / tris / counts hold the marching cubes of the current volume
@@ -70,7 +77,8 @@ static void pipeline_free(nrf_pipeline *p) {
   void *ptrs[] = {p->vox, p->knn, p->knn_w, p->knn_rows, p->nodes, p->T, p->centered, p->delta_x, p->mc_ws,
                   p->verts, p->tris, p->counts, p->updated, p->t16, p->w8, p->k16, p->kw16,
                   p->tk.dnorm, p->tk.warped, p->tk.live_v, p->tk.live_n, p->tk.ztri, p->tk.hit, p->tk.opt,
-                  p->tk.opt_frame, p->tk.corrs, p->tk.part, p->tk.Tsave, p->tk.dev};
+                  p->tk.opt_frame, p->tk.corrs, p->tk.part, p->tk.Tsave, p->tk.dev,
+                  p->g_depth, p->g_corrs, p->g_canon, p->g_pose};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (auto &e : p->ev)
@@ -217,10 +225,11 @@ static int pipeline_mc(nrf_pipeline *p, hipStream_t st) {
   return rc;
 }
 
-// `events`: record the stage events (not inside a capture).
+// `events`: record the stage events (not inside a capture).  dev_pose: the
+// integration reads its camera transform there (a captured step).
 static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float camera_pose[16],
                             const nrf_corr *corrs, const float *canonical, int num_corrs, void *stream,
-                            bool profile, bool events = true) {
+                            bool profile, bool events = true, const CamXf *dev_pose = nullptr) {
   hipStream_t st = as_stream(stream);
   int rc;
   hipEvent_t *pe = nullptr;
@@ -242,10 +251,11 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
   if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[1], st));
   if (p->cfg.storage16)
     rc = tsdf16_integrate(&p->cfg.volume, p->t16, p->w8, depth, &p->cfg.camera, camera_pose, p->centered, p->N,
-                          p->k16, p->kw16, p->knn_rows, p->updated, st);
+                          p->k16, p->kw16, p->knn_rows, p->updated, st, nullptr, 0, dev_pose);
   else
     rc = tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, p->nodes, p->T, p->N,
-                        p->centered, p->knn, p->knn_w, p->knn_rows, nullptr, p->cfg.min_opt_count, p->updated, st);
+                        p->centered, p->knn, p->knn_w, p->knn_rows, nullptr, p->cfg.min_opt_count, p->updated, st,
+                        dev_pose);
   if (rc) return rc;
   if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[2], st));
   if (profile && !p->prof_pcg) NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 2], st));
@@ -267,10 +277,29 @@ extern "C" int nrf_pipeline_step(nrf_pipeline *p, const float *depth, const floa
     p->graph_step = false;
     return pipeline_enqueue(p, depth, camera_pose, corrs, canonical, num_corrs, stream, p->prof_ring > 0);
   }
-  const void *key[5] = {depth, corrs, canonical, stream, nullptr};
-  bool same = p->exec && p->g_n == num_corrs && memcmp(p->g_pose, camera_pose, sizeof(p->g_pose)) == 0 &&
-              p->g_gen == solver_state_generation(p->solver);
-  for (int k = 0; k < 4 && same; ++k) same = key[k] == p->g_key[k];
+  // the frame into the captured step's input buffers (stream-ordered, ahead
+  // of the replay on the same stream)
+  hipStream_t cs = as_stream(stream);
+  const nrf_camera &cam = p->cfg.camera;
+  const size_t npx = (size_t)cam.width * cam.height;
+  if (!p->g_depth) {
+    NRF_HIP_CHECK(hipMalloc(&p->g_depth, std::max<size_t>(npx, 1) * sizeof(float)));
+    NRF_HIP_CHECK(hipMalloc(&p->g_corrs, (size_t)std::max(1, p->cfg.max_corrs) * sizeof(nrf_corr)));
+    NRF_HIP_CHECK(hipMalloc(&p->g_canon, (size_t)std::max(1, p->cfg.max_corrs) * 3 * sizeof(float)));
+    NRF_HIP_CHECK(hipMalloc(&p->g_pose, sizeof(CamXf)));
+  }
+  if (depth != p->g_depth)
+    NRF_HIP_CHECK(hipMemcpyAsync(p->g_depth, depth, npx * sizeof(float), hipMemcpyDeviceToDevice, cs));
+  if (num_corrs > 0 && corrs != p->g_corrs)
+    NRF_HIP_CHECK(hipMemcpyAsync(p->g_corrs, corrs, (size_t)num_corrs * sizeof(nrf_corr), hipMemcpyDeviceToDevice, cs));
+  if (num_corrs > 0 && canonical && canonical != p->g_canon)
+    NRF_HIP_CHECK(
+        hipMemcpyAsync(p->g_canon, canonical, (size_t)num_corrs * 3 * sizeof(float), hipMemcpyDeviceToDevice, cs));
+  int rc0 = set_device_pose(p->g_pose, camera_pose, cs);
+  if (rc0) return rc0;
+  const bool has_can = canonical != nullptr;
+  const bool same = p->exec && p->g_n == num_corrs && p->g_has_can == has_can &&
+                    p->g_gen == solver_state_generation(p->solver);
   if (!same) {
     if (p->exec) {
       (void)hipGraphExecDestroy(p->exec);
@@ -282,7 +311,8 @@ extern "C" int nrf_pipeline_step(nrf_pipeline *p, const float *depth, const floa
     }
     hipStream_t st = p->cap;
     NRF_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = pipeline_enqueue(p, depth, camera_pose, corrs, canonical, num_corrs, st, false, false);
+    int rc = pipeline_enqueue(p, p->g_depth, camera_pose, p->g_corrs, has_can ? p->g_canon : nullptr, num_corrs, st,
+                              false, false, p->g_pose);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(st, &g);
     if (rc) {
@@ -295,10 +325,10 @@ extern "C" int nrf_pipeline_step(nrf_pipeline *p, const float *depth, const floa
     }
     p->graph = g;
     NRF_HIP_CHECK(hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
-    for (int k = 0; k < 4; ++k) p->g_key[k] = key[k];
     p->g_n = num_corrs;
-    memcpy(p->g_pose, camera_pose, sizeof(p->g_pose));
+    p->g_has_can = has_can;
     p->g_gen = solver_state_generation(p->solver);
+    ++p->g_captures;
   }
   hipStream_t st = as_stream(stream);
   NRF_HIP_CHECK(hipEventRecord(p->ev[0], st));
@@ -307,6 +337,8 @@ extern "C" int nrf_pipeline_step(nrf_pipeline *p, const float *depth, const floa
   p->graph_step = true;
   return NRF_OK;
 }
+
+extern "C" long long nrf_pipeline_graph_captures(nrf_pipeline *p) { return p ? p->g_captures : -1; }
 
 extern "C" int nrf_pipeline_stage_ms(nrf_pipeline *p, float out_ms[3]) {
   if (!p || !out_ms) return NRF_ERR_INVALID_ARGUMENT;

@@ -557,7 +557,7 @@ typedef struct nrf_pipeline_config {
   int32_t max_corrs;
   int32_t max_triangles;
   int32_t min_opt_count;   /* integrate_min_optimized_count */
-  int32_t use_graph;       /* 1: replay the step as a captured hipGraph */
+  int32_t use_graph;       /* 1: replay the step as a captured hipGraph (nrf_pipeline_graph_captures) */
   int32_t storage16;       /* 1: precision form of the per-frame step: FP16 TSDF + u8
                               weight and u16 / FP16 voxel kNN (nrf_tsdf16_*, nrf_mc16_*);
                               frame 0 and the kNN snapshot use the FP32 volume, which
@@ -596,6 +596,13 @@ int nrf_pipeline_step(nrf_pipeline *p, const float *depth,
    the stream has been synchronised): [solve, integrate, mc].  After a graph
    replay (use_graph) out_ms[0] is the whole step and the others are -1. */
 int nrf_pipeline_stage_ms(nrf_pipeline *p, float out_ms[3]);
+/* With use_graph: the number of times the step was captured.  A step copies
+   its depth, correspondences and canonical points into pipeline-owned
+   buffers and writes its pose to device memory ahead of the replay (on the
+   caller's stream), so one capture serves every frame; it is captured again
+   only when num_corrs, the presence of canonical points or the solver's
+   buffers / pattern change. */
+long long nrf_pipeline_graph_captures(nrf_pipeline *p);
 /* Per-kernel profiling: with ring_steps > 0, every step records hipEvents
    around each kernel group on the step's stream (eager launches, no graph)
    into a ring of ring_steps steps; 0 disables.  nrf_pipeline_profile_read

@@ -143,9 +143,11 @@ def test_batch_rejects_mismatched_instances(gpu):
 
 
 def test_pipeline_graph_replay_matches_eager(gpu):
-    """One pipeline with use_graph: captured on its own stream, replayed on
-    the caller's; repeated frames replay the graph, new inputs re-capture it.
-    Every replay equals the eager step bit for bit."""
+    """One pipeline with use_graph: captured once on its own stream and
+    replayed on the caller's for every frame, whose inputs (other depth,
+    correspondence and canonical buffers, a camera pose that moves each step)
+    are copied into the captured step's buffers ahead of the replay.  Every
+    replay equals the eager step bit for bit."""
     import torch
 
     from cudafusion_amd import scene as S
@@ -155,13 +157,17 @@ def test_pipeline_graph_replay_matches_eager(gpu):
     a.setup()
     b = Harness(workload=w, frames=3, pcg_pipelined=True, use_graph=True)
     b.setup(host=a)
-    for i, f in enumerate((1, 1, 2, 2, 3)):
+    for i, f in enumerate((1, 1, 2, 2, 3, 1)):
+        pose = np.eye(4, dtype=np.float32)
+        pose[0, 3] = 0.0005 * i  # the camera moves 0.5 mm per step
+        a.pose = b.pose = pose.reshape(16)
         a.step(f)
         b.step(f)
         torch.cuda.synchronize()
         _compare([a], [b], f"pipeline graph step {i} f{f}")
         ms = b.stage_ms()
         assert ms[0] > 0 and ms[1] == -1
+        assert b.lib.nrf_pipeline_graph_captures(b.pipe) == 1
     a.close()
     b.close()
 
