@@ -449,7 +449,10 @@ __device__ inline void row_tail(const AsmArgs &a, int i, int rb, int rlen, float
         for (int c = 0; c < 3; ++c) vss += Js[ea * 3 + c] * Js[eb * 3 + c];
         for (int c = 0; c < 3; ++c) vso += Js[ea * 3 + c] * Jo[eb * 3 + c];
         dacc += lm * vss;
-        acc[offrel * 36 + ab] += -lm * vso;
+        // an edge whose block is absent from the pattern (a pattern kept over
+        // a graph change, Q4) drops its off-diagonal term only, as the
+        // reference's find_block < 0 test does (kernels/solver_kernels.cu:286-330)
+        if (offrel >= 0) acc[offrel * 36 + ab] += -lm * vso;
       } else if (ra >= 0 && ra < 6) {
         float v = 0;
         for (int c = 0; c < 3; ++c) v += Js[ra * 3 + c] * sq[3 + c];
@@ -2409,6 +2412,7 @@ struct Pattern {
 // build_sparsity_pattern (kernels/solver_kernels.cu:498-541) plus the ARAP
 // edge lists of each block row, in the order a sequential pass over nodes
 // and their neighbour slots visits them (solver_kernels.cu:225-233).
+static void build_edges(const nrf_node *h, int N, Pattern &P);
 static void build_pattern(const nrf_node *h, int N, Pattern &P) {
   std::vector<std::vector<int32_t>> rc(N);
   for (int i = 0; i < N; ++i) rc[i].push_back(i);
@@ -2438,9 +2442,18 @@ static void build_pattern(const nrf_node *h, int N, Pattern &P) {
   }
   P.row_ptr[N] = (int32_t)P.col_idx.size();
   P.nb = (int)P.col_idx.size();
+  build_edges(h, N, P);
+}
+
+// The ARAP edge-sides of the nodes' current edges against the pattern P
+// (rows' column lists unchanged): an edge's block offset is -1 when the
+// pattern lacks it.
+static void build_edges(const nrf_node *h, int N, Pattern &P) {
+  auto nnb = [](const nrf_node &n) { return std::max(0, std::min((int)n.num_neighbors, (int)NRF_K_GRAPH)); };
   auto rel = [&](int row, int col) {
     auto b = P.col_idx.begin() + P.row_ptr[row], e = P.col_idx.begin() + P.row_ptr[row + 1];
-    return (int32_t)(std::lower_bound(b, e, col) - b);
+    auto it = std::lower_bound(b, e, col);
+    return (int32_t)(it != e && *it == col ? it - b : -1);
   };
   P.e_i.clear(); P.e_j.clear(); P.e_w.clear();
   std::vector<std::vector<std::pair<int32_t, int32_t>>> es(N);
@@ -2575,6 +2588,22 @@ static void sink_record(nrf_solver *s, hipStream_t st) {
 }
 
 static int solver_build_partition(nrf_solver *s);
+
+static int solver_upload_edges(nrf_solver *s) {
+  const Pattern &P = s->P;
+  int st;
+  if ((st = upload(s->e_i, s->c_e_i, P.e_i))) return st;
+  if ((st = upload(s->e_j, s->c_e_j, P.e_j))) return st;
+  if ((st = upload(s->e_w, s->c_e_w, P.e_w))) return st;
+  if ((st = upload(s->es_off, s->c_es_off, P.es_off))) return st;
+  if ((st = upload(s->es_entry, s->c_es_entry, P.es_entry))) return st;
+  if ((st = upload(s->es_offrel, s->c_es_offrel, P.es_offrel))) return st;
+  if (!P.es_entry.empty() &&
+      (st = dev_alloc(s->arec, P.es_entry.size() * (size_t)kArapRec, s->c_arec)))
+    return st;
+  ++s->gen;
+  return NRF_OK;
+}
 
 static int solver_set_pattern_host(nrf_solver *s, const nrf_node *h, int N) {
   build_pattern(h, N, s->P);
@@ -2927,14 +2956,28 @@ static int solver_fetch_result(nrf_solver *s, nrf_pcg_result *out, hipStream_t s
   return NRF_OK;
 }
 
+static int solver_upload_edges(nrf_solver *s);
 static int solver_ensure_pattern(nrf_solver *s, const nrf_node *d_nodes, int N, hipStream_t st) {
-  if (N == s->P.N) return NRF_OK;
-  // Q4 of the reference: rebuilt only when the node count changes
-  // (kernels/solver_kernels.cu:554-561).
+  // Q4 of the reference: the pattern is rebuilt only when the node count
+  // changes (kernels/solver_kernels.cu:554-561); the ARAP term still reads
+  // the current nodes' edges (kernels/solver_kernels.cu:202-233), so a graph
+  // change at the same count refreshes the edge tables against the kept
+  // pattern (blocks it lacks are dropped).
   std::vector<nrf_node> h(N);
   NRF_HIP_CHECK(hipMemcpyAsync(h.data(), d_nodes, sizeof(nrf_node) * N, hipMemcpyDeviceToHost, st));
   NRF_HIP_CHECK(hipStreamSynchronize(st));
-  return solver_set_pattern_host(s, h.data(), N);
+  if (N != s->P.N) return solver_set_pattern_host(s, h.data(), N);
+  Pattern Q = s->P;
+  build_edges(h.data(), N, Q);
+  if (Q.e_i == s->P.e_i && Q.e_j == s->P.e_j && Q.e_w == s->P.e_w && Q.es_offrel == s->P.es_offrel) return NRF_OK;
+  s->P.e_i.swap(Q.e_i);
+  s->P.e_j.swap(Q.e_j);
+  s->P.e_w.swap(Q.e_w);
+  s->P.E = Q.E;
+  s->P.es_off.swap(Q.es_off);
+  s->P.es_entry.swap(Q.es_entry);
+  s->P.es_offrel.swap(Q.es_offrel);
+  return solver_upload_edges(s);
 }
 
 }  // namespace nrf

@@ -1032,6 +1032,14 @@ int orc_assemble_f32(const nrf_corr *corrs, int n, const float *canonical, const
            consistent_sign, psd_pairs);
   return rp[N];
 }
+// assemble_system with a given pattern (rp, ci), as after a graph change
+// that kept the node count (Q4: the pattern is not rebuilt)
+void orc_assemble_pattern_f32(const nrf_corr *corrs, int n, const float *canonical, const nrf_node *nodes,
+                              const nrf_dualquat *Tq, int N, float lam_s, float lam_d, const int32_t *rp,
+                              const int32_t *ci, float *A, float *b, float *M, int consistent_sign) {
+  auto T = load_T<float>(Tq, N);
+  assemble(corrs, n, canonical, nodes, T.data(), N, lam_s, lam_d, rp, ci, A, b, M, 0, consistent_sign, 0);
+}
 int orc_assemble_f64(const nrf_corr *corrs, int n, const float *canonical, const nrf_node *nodes,
                      const double *T8, int N, double lam_s, double lam_d, double *A, double *b, double *M,
                      int consistent_sign, int psd_pairs) {

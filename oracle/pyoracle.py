@@ -59,6 +59,7 @@ _SIGS = {
     "orc_rasterize_f32": (None, [P, P, P, I, I, P, P, P, P, P, P, P]),
     "orc_validate_f64": (I, [P, I, P]),
     "orc_apply_step_f32": (None, [P, P, I, P, F]),
+    "orc_assemble_pattern_f32": (None, [P, I, P, P, P, I, F, F, P, P, P, P, P, I]),
     "orc_corr_rmse": (D, [P, I]),
     "orc_raycast_f32": (None, [P, P, P, P, P, P, I, P, P, P, P, P]),
     "orc_graph_add_nodes": (I, [P, P, P, I, P, P, I, F, F]),
@@ -337,6 +338,20 @@ def assemble_f32(corrs, nodes, T, lam_s, lam_d, canonical=None, precond_fp64=0, 
     load().orc_assemble_f32(ptr(corrs), len(corrs), ptr(canonical), ptr(nodes), ptr(T), N, lam_s, lam_d,
                             precond_fp64, ptr(A), ptr(b), ptr(M), consistent_sign, psd_pairs)
     return rp, ci, A.reshape(nb, 6, 6), b, M.reshape(N, 6, 6)
+
+
+def assemble_pattern_f32(corrs, nodes, T, lam_s, lam_d, rp, ci, canonical=None, consistent_sign=0):
+    """The FP32 system on a given pattern (rp, ci) instead of the nodes' own."""
+    N = len(nodes)
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    nb = len(ci)
+    A = np.zeros(nb * 36, np.float32)
+    b = np.zeros(N * 6, np.float32)
+    M = np.zeros(N * 36, np.float32)
+    load().orc_assemble_pattern_f32(ptr(corrs), len(corrs), ptr(canonical), ptr(nodes), ptr(T), N, lam_s, lam_d,
+                                    ptr(rp), ptr(ci), ptr(A), ptr(b), ptr(M), consistent_sign)
+    return A.reshape(nb, 6, 6), b, M.reshape(N, 6, 6)
 
 
 def assemble_f64(corrs, nodes, T64, lam_s, lam_d, canonical=None, consistent_sign=0, psd_pairs=0):
