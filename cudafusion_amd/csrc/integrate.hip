@@ -48,7 +48,9 @@ struct IntegrateArgs {
   int min_opt;
   float decay, wmax;
   int32_t *updated;
+  uint8_t *occ;  // marching-cubes brick occupancy: set for every brick holding an updated voxel, or null
 };
+static_assert(kTileX == kMcBrickX && kTileY == kMcBrickY, "a tile's waves lie in one column of bricks");
 
 // Storage of the volume: the reference's 12-byte AoS voxels, or the compact
 // form (FP16 value + uint8 weight, the update rounded once when stored:
@@ -118,6 +120,7 @@ __device__ void integrate_tile(const IntegrateArgs &a, const Vox &vs, const Knn 
   const int y = by * kTileY + (threadIdx.x / kTileX);
   const int z0 = bz * ZC;
   int n_upd = 0;
+  unsigned occ_bits = 0;  // bit k: a voxel of brick z0 / 4 + k was updated
   const CamXf T = a.Tdev ? *a.Tdev : a.T;
   if (x < a.X && y < a.Y) {
     const float px_w = a.ox + (x + 0.5f) * a.s;
@@ -206,6 +209,15 @@ __device__ void integrate_tile(const IntegrateArgs &a, const Vox &vs, const Knn 
       const float nw = dw + 1.f;
       vs.store((z0 + dz) * plane + col, (tw[dz].x * dw + tv[dz] * 1.f) / nw, fminf(nw, a.wmax));
       ++n_upd;
+      occ_bits |= 1u << (((z0 + dz) / kMcBrickZ) - z0 / kMcBrickZ);
+    }
+  }
+  if (a.occ) {
+    // the wave's 64 voxels along x at one y share their bricks
+    for (int off = 32; off > 0; off >>= 1) occ_bits |= __shfl_xor(occ_bits, off, 64);
+    if ((threadIdx.x & 63) == 0 && occ_bits && y < a.Y) {
+      for (int k = 0; occ_bits >> k; ++k)
+        if ((occ_bits >> k) & 1u) a.occ[mc_brick_of(bx * kTileX, y, (z0 / kMcBrickZ + k) * kMcBrickZ, a.X, a.Y)] = 1;
     }
   }
   if (a.updated) {
@@ -337,6 +349,7 @@ int integrate_batch_create(const IntegrateBatchInstance *inst, int count, int z_
     a.decay = q.vp.decay_alpha;
     a.wmax = q.vp.max_weight;
     a.updated = q.updated;
+    a.occ = q.occ;
     h[k].tiles_x = (a.X + kTileX - 1) / kTileX;
     h[k].tiles_y = (a.Y + kTileY - 1) / kTileY;
     base[k] = (int32_t)tiles;
@@ -403,7 +416,7 @@ int tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels, const float *
                    const float camera_pose[16], const nrf_node *nodes, const nrf_dualquat *transforms, int num_nodes,
                    const nrf_dualquat *centered, const int32_t *voxel_knn, const float *voxel_knn_w,
                    const uint32_t *knn_rows, const int32_t *voxel_opt_counts, int min_opt_count,
-                   int32_t *updated_count, hipStream_t st, const void *dev_pose) {
+                   int32_t *updated_count, hipStream_t st, const void *dev_pose, uint8_t *occ) {
   if (!volume_ok(vp) || !voxels || !depth || !cam || !camera_pose) return NRF_ERR_INVALID_ARGUMENT;
   if (cam->width <= 0 || cam->height <= 0) return NRF_ERR_INVALID_ARGUMENT;
   const bool warp = nodes != nullptr && num_nodes > 0;
@@ -432,6 +445,7 @@ int tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels, const float *
   a.decay = vp->decay_alpha;
   a.wmax = vp->max_weight;
   a.updated = updated_count;
+  a.occ = occ;
   dim3 grid((a.X + kTileX - 1) / kTileX, (a.Y + kTileY - 1) / kTileY, (a.Z + kZChunk - 1) / kZChunk);
   if (!warp)
     integrate_kernel<false, false><<<grid, 256, 0, st>>>(a);
@@ -447,7 +461,7 @@ int tsdf16_integrate(const nrf_volume_params *vp, uint16_t *tsdf16, uint8_t *wei
                      const nrf_camera *cam, const float camera_pose[16], const nrf_dualquat *centered, int num_nodes,
                      const uint16_t *knn16, const uint16_t *knn_w16, const uint32_t *knn_rows,
                      int32_t *updated_count, hipStream_t st, const int32_t *voxel_opt_counts, int min_opt_count,
-                     const void *dev_pose) {
+                     const void *dev_pose, uint8_t *occ) {
   if (!volume_ok(vp) || !tsdf16 || !weight8 || !depth || !cam || !camera_pose) return NRF_ERR_INVALID_ARGUMENT;
   if (cam->width <= 0 || cam->height <= 0) return NRF_ERR_INVALID_ARGUMENT;
   if (num_nodes > 0 && (!centered || !knn16 || !knn_w16)) return NRF_ERR_INVALID_ARGUMENT;
@@ -477,6 +491,7 @@ int tsdf16_integrate(const nrf_volume_params *vp, uint16_t *tsdf16, uint8_t *wei
   a.decay = vp->decay_alpha;
   a.wmax = vp->max_weight;
   a.updated = updated_count;
+  a.occ = occ;
   const VoxCompact vs{reinterpret_cast<__half *>(tsdf16), weight8};
   const Knn16 kn{reinterpret_cast<const ushort4 *>(knn16), reinterpret_cast<const uint2 *>(knn_w16)};
   dim3 grid((a.X + kTileX - 1) / kTileX, (a.Y + kTileY - 1) / kTileY, (a.Z + kZChunk - 1) / kZChunk);

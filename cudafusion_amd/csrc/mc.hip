@@ -88,7 +88,7 @@ __global__ __launch_bounds__(kMcBlock) void emit_kernel(const nrf_voxel *__restr
                                                         const int32_t *__restrict__ rowcnt,
                                                         const int32_t *__restrict__ rowoff, int max_tris,
                                                         float *__restrict__ verts, float *__restrict__ normals,
-                                                        int32_t *__restrict__ tris) {
+                                                        int32_t *__restrict__ tris, const uint8_t *__restrict__ occ) {
   const int y = blockIdx.x, z = blockIdx.y;
   const int row = z * Y + y;
   if (rowcnt[row] == 0) return;
@@ -99,7 +99,8 @@ __global__ __launch_bounds__(kMcBlock) void emit_kernel(const nrf_voxel *__restr
     const int x = x0 + threadIdx.x;
     const int idx = z * X * Y + y * X + x;
     int cube = 0;
-    if (x < X - 1) cube = kCases ? (int)cases[idx] : cube_case(vox, idx, X, Y);
+    if (x < X - 1 && (!occ || occ[mc_brick_of(x, y, z, X, Y)]))
+      cube = kCases ? (int)cases[idx] : cube_case(vox, idx, X, Y);
     const int nt = kMcNTri[cube];
     int inc = nt;  // inclusive wave scan
     for (int d = 1; d < 64; d <<= 1) {
@@ -164,9 +165,9 @@ extern "C" const uint8_t *nrf_mc_cases(const void *workspace) {
   return reinterpret_cast<const uint8_t *>(workspace);
 }
 
-extern "C" int nrf_mc_extract(const nrf_volume_params *vp, const nrf_voxel *voxels, void *workspace,
-                              size_t workspace_bytes, float *vertices, float *normals, int32_t *triangles,
-                              int max_triangles, int32_t *counts, void *stream) {
+int nrf::mc_extract(const nrf_volume_params *vp, const nrf_voxel *voxels, const uint8_t *occ, void *workspace,
+                    size_t workspace_bytes, float *vertices, float *normals, int32_t *triangles, int max_triangles,
+                    int32_t *counts, hipStream_t st) {
   if (!volume_ok(vp) || !voxels || !workspace || !counts) return NRF_ERR_INVALID_ARGUMENT;
   if (max_triangles < 0 || (max_triangles > 0 && (!vertices || !triangles))) return NRF_ERR_INVALID_ARGUMENT;
   const int X = vp->dims[0], Y = vp->dims[1], Z = vp->dims[2];
@@ -178,12 +179,11 @@ extern "C" int nrf_mc_extract(const nrf_volume_params *vp, const nrf_voxel *voxe
   int32_t *rowcnt = reinterpret_cast<int32_t *>(ws + L.rowcnt_off);
   int32_t *rowoff = reinterpret_cast<int32_t *>(ws + L.rowoff_off);
   const int R = Y * Z;
-  hipStream_t st = as_stream(stream);
   const int nz = (int)L.zero_ints;  // the counts and the scan's status words
   fill_zero_kernel<<<(nz + 255) / 256, 256, 0, st>>>(rowcnt, nz);
   NRF_LAUNCH_CHECK();
   classify_brick_kernel<<<classify_brick_grid(X, Y, Z), kMcBrickThreads, 0, st>>>(VoxelReaderF32{voxels}, X, Y, Z,
-                                                                                   cases, rowcnt);
+                                                                                   cases, rowcnt, occ);
   NRF_LAUNCH_CHECK();
   mc_scan_rows(L, rowcnt, rowoff, R, counts, workspace, st);
   NRF_LAUNCH_CHECK();
@@ -191,10 +191,21 @@ extern "C" int nrf_mc_extract(const nrf_volume_params *vp, const nrf_voxel *voxe
     emit_kernel<true><<<dim3(Y, Z), kMcBlock, 0, st>>>(voxels, X, Y, Z,
                                                  make_float3(vp->origin[0], vp->origin[1], vp->origin[2]),
                                                  vp->voxel_size, cases, rowcnt, rowoff, max_triangles, vertices,
-                                                 normals, triangles);
+                                                 normals, triangles, occ);
     NRF_LAUNCH_CHECK();
   }
   return NRF_OK;
+}
+
+extern "C" int nrf_mc_extract(const nrf_volume_params *vp, const nrf_voxel *voxels, void *workspace,
+                              size_t workspace_bytes, float *vertices, float *normals, int32_t *triangles,
+                              int max_triangles, int32_t *counts, void *stream) {
+  return mc_extract(vp, voxels, nullptr, workspace, workspace_bytes, vertices, normals, triangles, max_triangles,
+                    counts, as_stream(stream));
+}
+
+size_t nrf::mc_occupancy_bytes(const nrf_volume_params *vp) {
+  return mc_brick_count(vp->dims[0], vp->dims[1], vp->dims[2]);
 }
 
 // Device address of this device's copy of `scratch`, looked up once per
@@ -240,7 +251,7 @@ extern "C" int nrf_mc_extract_noalloc(const nrf_volume_params *vp, const nrf_vox
   fill_zero_kernel<<<(nz + 255) / 256, 256, 0, st>>>(rowcnt, nz);
   NRF_LAUNCH_CHECK();
   classify_brick_kernel<<<classify_brick_grid(X, Y, Z), kMcBrickThreads, 0, st>>>(VoxelReaderF32{voxels}, X, Y, Z,
-                                                                                   nullptr, rowcnt);
+                                                                                   nullptr, rowcnt, nullptr);
   NRF_LAUNCH_CHECK();
   mc_scan_rows(L, rowcnt, rowoff, (int)R, counts, ws, st);
   NRF_LAUNCH_CHECK();
@@ -248,7 +259,7 @@ extern "C" int nrf_mc_extract_noalloc(const nrf_volume_params *vp, const nrf_vox
     emit_kernel<false><<<dim3(Y, Z), kMcBlock, 0, st>>>(voxels, X, Y, Z,
                                                         make_float3(vp->origin[0], vp->origin[1], vp->origin[2]),
                                                         vp->voxel_size, nullptr, rowcnt, rowoff, max_triangles,
-                                                        vertices, normals, triangles);
+                                                        vertices, normals, triangles, nullptr);
     NRF_LAUNCH_CHECK();
   }
   return NRF_OK;

@@ -21,15 +21,25 @@ namespace {
 // are read once, coalesced along x, into LDS as (tsdf, weight) pairs; wave w
 // then classifies cube row y0 + w for the four z-slices.  Reader maps a voxel
 // index to its (tsdf, weight).  cases may be null (only the row counts).
-constexpr int kMcBx = 64, kMcBy = 4, kMcBz = 4;
+//
+// Brick occupancy (occ, optional): one byte per brick, brick (bx, by, bz) at
+// (bz * nby + by) * nbx + bx, nonzero when some voxel of the brick's own
+// 64 x 4 x 4 region may have weight > 0.  Corner 0 of every cube of a brick
+// lies in that region, so a zero byte means that every cube of the brick has
+// a zero-weight corner: case 0, no triangle.  Such a brick reads no voxel and
+// writes no case byte (readers of the case array take occ into account).
+constexpr int kMcBx = kMcBrickX, kMcBy = kMcBrickY, kMcBz = kMcBrickZ;
 constexpr int kMcVx = kMcBx + 1, kMcVy = kMcBy + 1, kMcVz = kMcBz + 1;
 constexpr int kMcBrickThreads = 64 * kMcBy;
+static_assert(kMcBx == 64, "one wave per x-row of a brick");
 
 template <class Reader>
 __global__ __launch_bounds__(kMcBrickThreads) void classify_brick_kernel(Reader rd, int X, int Y, int Z,
                                                                          uint8_t *__restrict__ cases,
-                                                                         int32_t *__restrict__ rowcnt) {
+                                                                         int32_t *__restrict__ rowcnt,
+                                                                         const uint8_t *__restrict__ occ) {
   __shared__ float2 sv[kMcVz][kMcVy][kMcVx];
+  if (occ && occ[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] == 0) return;
   const int x0 = blockIdx.x * kMcBx, y0 = blockIdx.y * kMcBy, z0 = blockIdx.z * kMcBz;
   const int t = threadIdx.x;
   for (int j = t; j < kMcVx * kMcVy * kMcVz; j += kMcBrickThreads) {
@@ -63,7 +73,7 @@ __global__ __launch_bounds__(kMcBrickThreads) void classify_brick_kernel(Reader 
 }
 
 inline dim3 classify_brick_grid(int X, int Y, int Z) {
-  return dim3((X + kMcBx - 1) / kMcBx, (Y + kMcBy - 1) / kMcBy, (Z + kMcBz - 1) / kMcBz);
+  return dim3(mc_bricks_x(X), mc_bricks_y(Y), (Z + kMcBz - 1) / kMcBz);
 }
 
 // Exclusive scan of n row counts by one workgroup of 1024 lanes.  Lane t

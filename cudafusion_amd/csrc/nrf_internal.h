@@ -50,6 +50,19 @@ int device_max_lds_per_block();
 // Independent instances (one pipeline each) stepped by common launches.
 // Per-frame inputs travel as kernel arguments, so a captured batch step
 // depends on nothing but them and the instances' device state.
+// Marching-cubes bricks: 64 x 4 x 4 voxels (cubes by their corner 0),
+// numbered x fastest.  The integration marks the bricks it updates in an
+// occupancy map of one byte per brick, which marching cubes reads.
+constexpr int kMcBrickX = 64, kMcBrickY = 4, kMcBrickZ = 4;
+__host__ __device__ inline int mc_bricks_x(int X) { return (X + kMcBrickX - 1) / kMcBrickX; }
+__host__ __device__ inline int mc_bricks_y(int Y) { return (Y + kMcBrickY - 1) / kMcBrickY; }
+__host__ __device__ inline int mc_brick_of(int x, int y, int z, int X, int Y) {
+  return ((z / kMcBrickZ) * mc_bricks_y(Y) + y / kMcBrickY) * mc_bricks_x(X) + x / kMcBrickX;
+}
+inline size_t mc_brick_count(int X, int Y, int Z) {
+  return (size_t)mc_bricks_x(X) * mc_bricks_y(Y) * ((Z + kMcBrickZ - 1) / kMcBrickZ);
+}
+
 constexpr int kMaxBatch = 32;
 // memberships per chunk of the MFMA assembly (rows with more are split)
 constexpr int kAsmChunk = 128;
@@ -115,6 +128,7 @@ struct IntegrateBatchInstance {
   const float *knn_w;
   const uint32_t *knn_rows;  // row occupancy of knn (knn_row_flags), or null
   int32_t *updated;
+  uint8_t *occ;  // marching-cubes brick occupancy to mark (mc_occupancy_bytes), or null
 };
 int integrate_batch_create(const IntegrateBatchInstance *inst, int count, int z_chunk, IntegrateBatch **out);
 void integrate_batch_destroy(IntegrateBatch *b);
@@ -199,18 +213,31 @@ int tsdf16_integrate(const nrf_volume_params *vp, uint16_t *tsdf16, uint8_t *wei
                      const nrf_camera *cam, const float camera_pose[16], const nrf_dualquat *centered, int num_nodes,
                      const uint16_t *knn16, const uint16_t *knn_w16, const uint32_t *knn_rows,
                      int32_t *updated_count, hipStream_t st, const int32_t *voxel_opt_counts = nullptr,
-                     int min_opt_count = 0, const void *dev_pose = nullptr);
+                     int min_opt_count = 0, const void *dev_pose = nullptr, uint8_t *occ = nullptr);
 // dev_pose (tsdf_integrate / tsdf16_integrate): a device CamXf (the
 // world-to-camera transform, as invert_pose gives it) that the kernel reads
 // instead of camera_pose's inverse, so a captured launch can take each
 // replay's pose from memory (set_device_pose writes it stream-ordered).
 int set_device_pose(CamXf *dev, const float camera_pose[16], hipStream_t st);
+// Marching cubes (nrf_mc_extract / nrf_mc16_extract) with an optional brick
+// occupancy map: one byte per 64 x 4 x 4 voxel brick (mc_occupancy_bytes),
+// nonzero when a voxel of the brick may have weight > 0; bricks with a zero
+// byte are skipped (none of their cubes can emit).  The map is kept by the
+// integration calls that take it (they set the byte of every brick holding
+// an updated voxel and never clear one).
+size_t mc_occupancy_bytes(const nrf_volume_params *vp);
+int mc_extract(const nrf_volume_params *vp, const nrf_voxel *voxels, const uint8_t *occ, void *workspace,
+               size_t workspace_bytes, float *vertices, float *normals, int32_t *triangles, int max_triangles,
+               int32_t *counts, hipStream_t st);
+int mc16_extract(const nrf_volume_params *vp, const uint16_t *tsdf16, const uint8_t *weight8, const uint8_t *occ,
+                 void *workspace, size_t workspace_bytes, float *vertices, int32_t *triangles, int max_triangles,
+                 int32_t *counts, hipStream_t st);
 // nrf_tsdf_integrate with the kNN's row occupancy (null: none)
 int tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels, const float *depth, const nrf_camera *cam,
                    const float camera_pose[16], const nrf_node *nodes, const nrf_dualquat *transforms, int num_nodes,
                    const nrf_dualquat *centered, const int32_t *voxel_knn, const float *voxel_knn_w,
                    const uint32_t *knn_rows, const int32_t *voxel_opt_counts, int min_opt_count,
-                   int32_t *updated_count, hipStream_t st, const void *dev_pose = nullptr);
+                   int32_t *updated_count, hipStream_t st, const void *dev_pose = nullptr, uint8_t *occ = nullptr);
 
 }  // namespace nrf
 

@@ -31,6 +31,11 @@ struct nrf_pipeline {
   float *delta_x = nullptr;
   void *mc_ws = nullptr;
   size_t mc_bytes = 0;
+  // marching-cubes brick occupancy: set by the pipeline's integrations for
+  // every brick holding an updated voxel (a superset of the bricks holding a
+  // voxel with weight > 0, since only an update makes a weight positive);
+  // marching cubes skips the other bricks
+  uint8_t *occ = nullptr;
   float *verts = nullptr;
   int32_t *tris = nullptr;
   int32_t *counts = nullptr;
@@ -74,7 +79,7 @@ struct nrf_pipeline {
 };
 
 static void pipeline_free(nrf_pipeline *p) {
-  void *ptrs[] = {p->vox, p->knn, p->knn_w, p->knn_rows, p->nodes, p->T, p->centered, p->delta_x, p->mc_ws,
+  void *ptrs[] = {p->vox, p->knn, p->knn_w, p->knn_rows, p->nodes, p->T, p->centered, p->delta_x, p->mc_ws, p->occ,
                   p->verts, p->tris, p->counts, p->updated, p->t16, p->w8, p->k16, p->kw16,
                   p->tk.dnorm, p->tk.warped, p->tk.live_v, p->tk.live_n, p->tk.ztri, p->tk.hit, p->tk.opt,
                   p->tk.opt_frame, p->tk.corrs, p->tk.part, p->tk.Tsave, p->tk.dev,
@@ -110,6 +115,8 @@ extern "C" nrf_pipeline *nrf_pipeline_create(const nrf_pipeline_config *cfg) {
   ok = ok && hipMalloc(&p->delta_x, (size_t)cfg->max_nodes * 6 * sizeof(float)) == hipSuccess;
   p->mc_bytes = nrf_mc_workspace_size(&cfg->volume);
   ok = ok && hipMalloc(&p->mc_ws, p->mc_bytes) == hipSuccess;
+  const size_t nocc = mc_occupancy_bytes(&cfg->volume);
+  ok = ok && hipMalloc(&p->occ, nocc) == hipSuccess;
   const size_t T = (size_t)std::max(1, cfg->max_triangles);
   ok = ok && hipMalloc(&p->verts, T * 9 * sizeof(float)) == hipSuccess;
   ok = ok && hipMalloc(&p->tris, T * 3 * sizeof(int32_t)) == hipSuccess;
@@ -132,6 +139,7 @@ extern "C" nrf_pipeline *nrf_pipeline_create(const nrf_pipeline_config *cfg) {
   if (ok) ok = hipMemset(p->knn, 0xff, V * 4 * sizeof(int32_t)) == hipSuccess;
   if (ok) ok = hipMemset(p->knn_w, 0, V * 4 * sizeof(float)) == hipSuccess;
   if (ok) ok = hipMemset(p->knn_rows, 0, nrun * sizeof(uint32_t)) == hipSuccess;  // no id >= 0 yet
+  if (ok) ok = hipMemset(p->occ, 0, nocc) == hipSuccess;  // every weight is 0 after the reset
   if (ok) ok = hipMemset(p->counts, 0, 4 * sizeof(int32_t)) == hipSuccess;
   if (ok) ok = hipMemset(p->updated, 0, 4 * sizeof(int32_t)) == hipSuccess;
   if (ok) ok = hipDeviceSynchronize() == hipSuccess;
@@ -171,8 +179,8 @@ extern "C" int nrf_pipeline_integrate_rigid(nrf_pipeline *p, const float *depth,
                                             void *stream) {
   if (!p || !depth || !camera_pose) return NRF_ERR_INVALID_ARGUMENT;
   p->mesh_valid = false;
-  return nrf_tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, nullptr, nullptr, 0,
-                            nullptr, nullptr, nullptr, nullptr, 0, nullptr, stream);
+  return tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, nullptr, nullptr, 0, nullptr,
+                        nullptr, nullptr, nullptr, nullptr, 0, nullptr, as_stream(stream), nullptr, p->occ);
 }
 
 extern "C" int nrf_pipeline_set_nodes(nrf_pipeline *p, const nrf_node *host_nodes, int num_nodes) {
@@ -216,11 +224,11 @@ extern "C" int nrf_pipeline_set_nodes(nrf_pipeline *p, const nrf_node *host_node
 static int pipeline_mc(nrf_pipeline *p, hipStream_t st) {
   int rc;
   if (p->cfg.storage16)
-    rc = nrf_mc16_extract(&p->cfg.volume, p->t16, p->w8, p->mc_ws, p->mc_bytes, p->verts, p->tris,
-                          p->cfg.max_triangles, p->counts, st);
+    rc = mc16_extract(&p->cfg.volume, p->t16, p->w8, p->occ, p->mc_ws, p->mc_bytes, p->verts, p->tris,
+                      p->cfg.max_triangles, p->counts, st);
   else
-    rc = nrf_mc_extract(&p->cfg.volume, p->vox, p->mc_ws, p->mc_bytes, p->verts, nullptr, p->tris,
-                        p->cfg.max_triangles, p->counts, st);
+    rc = mc_extract(&p->cfg.volume, p->vox, p->occ, p->mc_ws, p->mc_bytes, p->verts, nullptr, p->tris,
+                    p->cfg.max_triangles, p->counts, st);
   if (rc == NRF_OK) p->mesh_valid = true;
   return rc;
 }
@@ -251,11 +259,11 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
   if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[1], st));
   if (p->cfg.storage16)
     rc = tsdf16_integrate(&p->cfg.volume, p->t16, p->w8, depth, &p->cfg.camera, camera_pose, p->centered, p->N,
-                          p->k16, p->kw16, p->knn_rows, p->updated, st, nullptr, 0, dev_pose);
+                          p->k16, p->kw16, p->knn_rows, p->updated, st, nullptr, 0, dev_pose, p->occ);
   else
     rc = tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, p->nodes, p->T, p->N,
                         p->centered, p->knn, p->knn_w, p->knn_rows, nullptr, p->cfg.min_opt_count, p->updated, st,
-                        dev_pose);
+                        dev_pose, p->occ);
   if (rc) return rc;
   if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[2], st));
   if (profile && !p->prof_pcg) NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 2], st));
@@ -519,10 +527,10 @@ extern "C" int nrf_pipeline_track(nrf_pipeline *p, const float *depth, const flo
     return rc;
   if (p->cfg.storage16)
     rc = tsdf16_integrate(&p->cfg.volume, p->t16, p->w8, depth, cam, camera_pose, p->centered, p->N, p->k16, p->kw16,
-                          p->knn_rows, p->updated, st, k.opt, p->cfg.min_opt_count);
+                          p->knn_rows, p->updated, st, k.opt, p->cfg.min_opt_count, nullptr, p->occ);
   else
     rc = tsdf_integrate(&p->cfg.volume, p->vox, depth, cam, camera_pose, p->nodes, p->T, p->N, p->centered, p->knn,
-                        p->knn_w, p->knn_rows, k.opt, p->cfg.min_opt_count, p->updated, st);
+                        p->knn_w, p->knn_rows, k.opt, p->cfg.min_opt_count, p->updated, st, nullptr, p->occ);
   if (rc) return rc;
   NRF_HIP_CHECK(hipEventRecord(p->ev[2], st));
   if ((rc = pipeline_mc(p, st))) return rc;
@@ -618,6 +626,7 @@ static int batch_build_integrate(const std::vector<nrf_pipeline *> &p, int z_chu
     q.knn_w = p[k]->knn_w;
     q.knn_rows = p[k]->knn_rows;
     q.updated = updated ? updated : p[k]->updated;
+    q.occ = vox ? nullptr : p[k]->occ;  // the tuner's scratch copies mark nothing
   }
   return integrate_batch_create(ii.data(), (int)p.size(), z_chunk, out);
 }
@@ -729,8 +738,8 @@ static int batch_enqueue(nrf_batch *b, const BatchCorrs &fc, const BatchDepth &f
   if ((rc = integrate_batch_run(b->ib, fd, st))) return rc;
   if (events) NRF_HIP_CHECK(hipEventRecord(b->ev[2], st));
   for (nrf_pipeline *p : b->p) {
-    rc = nrf_mc_extract(&p->cfg.volume, p->vox, p->mc_ws, p->mc_bytes, p->verts, nullptr, p->tris,
-                        p->cfg.max_triangles, p->counts, st);
+    rc = mc_extract(&p->cfg.volume, p->vox, p->occ, p->mc_ws, p->mc_bytes, p->verts, nullptr, p->tris,
+                    p->cfg.max_triangles, p->counts, st);
     if (rc) return rc;
   }
   if (events) NRF_HIP_CHECK(hipEventRecord(b->ev[3], st));

@@ -78,7 +78,8 @@ __global__ __launch_bounds__(kMcBlock) void emit16_kernel(const __half *__restri
                                                           float3 org, float s, const uint8_t *__restrict__ cases,
                                                           const int32_t *__restrict__ rowcnt,
                                                           const int32_t *__restrict__ rowoff, int max_tris,
-                                                          float *__restrict__ verts, int32_t *__restrict__ tris) {
+                                                          float *__restrict__ verts, int32_t *__restrict__ tris,
+                                                          const uint8_t *__restrict__ occ) {
   const int y = blockIdx.x, z = blockIdx.y;
   const int row = z * Y + y;
   if (rowcnt[row] == 0) return;
@@ -88,7 +89,7 @@ __global__ __launch_bounds__(kMcBlock) void emit16_kernel(const __half *__restri
   for (int x0 = 0; x0 < X - 1; x0 += kMcBlock) {
     const int x = x0 + threadIdx.x;
     const int idx = z * X * Y + y * X + x;
-    const int cube = (x < X - 1) ? cases[idx] : 0;
+    const int cube = (x < X - 1 && (!occ || occ[mc_brick_of(x, y, z, X, Y)])) ? cases[idx] : 0;
     const int nt = kMcNTri[cube];
     int inc = nt;
     for (int d = 1; d < 64; d <<= 1) {
@@ -169,9 +170,9 @@ extern "C" int nrf_tsdf16_integrate(const nrf_volume_params *vp, uint16_t *tsdf1
                           nullptr, updated_count, as_stream(stream));
 }
 
-extern "C" int nrf_mc16_extract(const nrf_volume_params *vp, const uint16_t *tsdf16, const uint8_t *weight8,
-                                void *workspace, size_t workspace_bytes, float *vertices, int32_t *triangles,
-                                int max_triangles, int32_t *counts, void *stream) {
+int nrf::mc16_extract(const nrf_volume_params *vp, const uint16_t *tsdf16, const uint8_t *weight8,
+                      const uint8_t *occ, void *workspace, size_t workspace_bytes, float *vertices,
+                      int32_t *triangles, int max_triangles, int32_t *counts, hipStream_t st) {
   if (!volume_ok(vp) || !tsdf16 || !weight8 || !workspace || !counts) return NRF_ERR_INVALID_ARGUMENT;
   if (max_triangles < 0 || (max_triangles > 0 && (!vertices || !triangles))) return NRF_ERR_INVALID_ARGUMENT;
   const int X = vp->dims[0], Y = vp->dims[1], Z = vp->dims[2];
@@ -183,13 +184,12 @@ extern "C" int nrf_mc16_extract(const nrf_volume_params *vp, const uint16_t *tsd
   uint8_t *cases = reinterpret_cast<uint8_t *>(ws + L.cases_off);
   int32_t *rowcnt = reinterpret_cast<int32_t *>(ws + L.rowcnt_off);
   int32_t *rowoff = reinterpret_cast<int32_t *>(ws + L.rowoff_off);
-  hipStream_t st = as_stream(stream);
   const __half *t16 = reinterpret_cast<const __half *>(tsdf16);
   const int nz = (int)L.zero_ints;  // the counts and the scan's status words
   zero_i32<<<(unsigned)((nz + 255) / 256), 256, 0, st>>>(rowcnt, nz);
   NRF_LAUNCH_CHECK();
   classify_brick_kernel<<<classify_brick_grid(X, Y, Z), kMcBrickThreads, 0, st>>>(VoxelReaderF16{t16, weight8}, X,
-                                                                                   Y, Z, cases, rowcnt);
+                                                                                   Y, Z, cases, rowcnt, occ);
   NRF_LAUNCH_CHECK();
   mc_scan_rows(L, rowcnt, rowoff, (int)R, counts, workspace, st);
   NRF_LAUNCH_CHECK();
@@ -197,8 +197,15 @@ extern "C" int nrf_mc16_extract(const nrf_volume_params *vp, const uint16_t *tsd
     emit16_kernel<<<dim3(Y, Z), kMcBlock, 0, st>>>(t16, X, Y, Z,
                                                    make_float3(vp->origin[0], vp->origin[1], vp->origin[2]),
                                                    vp->voxel_size, cases, rowcnt, rowoff, max_triangles, vertices,
-                                                   triangles);
+                                                   triangles, occ);
     NRF_LAUNCH_CHECK();
   }
   return NRF_OK;
+}
+
+extern "C" int nrf_mc16_extract(const nrf_volume_params *vp, const uint16_t *tsdf16, const uint8_t *weight8,
+                                void *workspace, size_t workspace_bytes, float *vertices, int32_t *triangles,
+                                int max_triangles, int32_t *counts, void *stream) {
+  return mc16_extract(vp, tsdf16, weight8, nullptr, workspace, workspace_bytes, vertices, triangles, max_triangles,
+                      counts, as_stream(stream));
 }

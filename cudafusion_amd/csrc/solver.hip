@@ -8,24 +8,33 @@
 //
 //  * Assembly is atomic-free and deterministic.  Per frame, the
 //    (correspondence, neighbour-slot) pairs are bucketed by node and each
-//    bucket sorted by correspondence index ("memberships").  One wavefront
-//    then owns one block row i: it stages 64 memberships at a time in LDS
-//    (residual, Huber weight, the four 6-vector Jacobian rows, the row
-//    positions of the four neighbours) and 36 lanes accumulate the 6x6
-//    entries of every block of the row in LDS (one lane per entry, so there
-//    are no conflicts and the summation order is fixed), six lanes the RHS.
-//    The ARAP edge contributions of the row follow, then damping, then the
-//    block-Jacobi Cholesky inverse of the diagonal block.  One kernel per GN
-//    iteration replaces the reference's data / smooth / damping /
-//    preconditioner kernels and their float atomics.
-//  * PCG is one persistent kernel: each workgroup owns a contiguous range of
-//    block rows and keeps their matrix blocks in LDS for all iterations;
-//    dot products are FP64, reduced per workgroup and combined in a fixed
-//    order after a grid barrier, so every workgroup holds bit-identical
-//    alpha / beta and takes the same branch.  The update p = z + beta p is
-//    recomputed on the fly by the readers of p (double-buffered), which
-//    leaves two grid barriers per iteration.  The barrier wait is bounded:
-//    on timeout the kernel raises a flag and every workgroup exits.
+//    bucket sorted by key corr * 4 + slot ("memberships").  Per GN
+//    iteration a prep launch writes one 128-byte record per valid
+//    correspondence (warped source, residual, Huber weight, the four slot
+//    Jacobians) and one record per ARAP edge-side.  One wavefront then owns
+//    a chunk of at most 128 memberships of a block row and forms its 6 x 6
+//    blocks and RHS as sums of outer products on the matrix cores
+//    (v_mfma_f32_16x16x4f32, or BF16 / FP16 products in the precision
+//    form), with the membership's dense row staged in LDS.  Rows longer than
+//    one chunk leave partial sums that a tail launch adds in chunk order,
+//    together with the ARAP edge-sides, damping and the block-Jacobi
+//    Cholesky inverse of the diagonal block.  These launches replace the
+//    reference's data / smooth / damping / preconditioner kernels and their
+//    float atomics; every sum has a fixed order.
+//  * PCG is one persistent kernel (ordinary launch, grid sized to be
+//    co-resident): each workgroup owns a contiguous range of nodes and keeps
+//    their matrix blocks in registers and LDS for all iterations.  Values
+//    that cross workgroups (halo entries of the direction vector, per-
+//    workgroup FP64 partial dot products) travel as tagged 64-bit words,
+//    tag = launch epoch << 16 | synchronisation point, written and polled
+//    with agent-scope relaxed atomics, so no grid barrier, fence or cache
+//    maintenance is needed and every workgroup combines the partial sums in
+//    the same order (bit-identical alpha / beta, the same branch).  Four
+//    forms share the layout: textbook, overlapped (A p by recurrence),
+//    single-reduction (Chronopoulos-Gear) and pipelined (Ghysels-Vanroose,
+//    FP64 recurrences).  Every wait is bounded: on timeout the kernel
+//    raises a flag, every workgroup exits, and the call reports
+//    NRF_ERR_BARRIER_TIMEOUT.
 #include <atomic>
 #include <algorithm>
 #include <climits>
