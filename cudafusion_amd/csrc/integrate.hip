@@ -49,6 +49,8 @@ struct IntegrateArgs {
   float decay, wmax;
   int32_t *updated;
   uint8_t *occ;  // marching-cubes brick occupancy: set for every brick holding an updated voxel, or null
+  const float *dmax;  // depth block maxima (depth_block_max_kernel) for the wave cull, or null
+  int dmw;            // blocks per image row
 };
 static_assert(kTileX == kMcBrickX && kTileY == kMcBrickY, "a tile's waves lie in one column of bricks");
 
@@ -100,6 +102,92 @@ struct Knn16 {
   }
 };
 
+// Depth block maxima for the wave cull: one float per kDepthBlock x
+// kDepthBlock pixels, the largest depth the integration can use there (d > 0,
+// or NaN, which passes the integration's tests and counts as +inf), -1 when
+// the block holds none.
+constexpr int kDepthBlock = 8;
+// One lane per pixel row of a block (8 pixels), the block's 8 rows on 8
+// adjacent lanes, reduced by lane moves.
+__global__ __launch_bounds__(256) void depth_block_max_kernel(const float *__restrict__ depth, int W, int H, int bw,
+                                                              int bh, float *__restrict__ dmax) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = t / kDepthBlock, r = t % kDepthBlock;
+  float m = -1.f;
+  if (b < bw * bh) {
+    const int u0 = (b % bw) * kDepthBlock, v = (b / bw) * kDepthBlock + r;
+    if (v < H) {
+      const float *row = depth + (size_t)v * W;
+      float d[kDepthBlock];
+#pragma unroll
+      for (int du = 0; du < kDepthBlock; ++du) d[du] = (u0 + du < W) ? row[u0 + du] : 0.f;
+#pragma unroll
+      for (int du = 0; du < kDepthBlock; ++du)
+        if (!(d[du] <= 0.f)) m = fmaxf(m, d[du] != d[du] ? INFINITY : d[du]);
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < kDepthBlock; off <<= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  if (b < bw * bh && r == 0) dmax[b] = m;
+}
+
+// True when no voxel of the wave's set (the 64 voxels along x of row y in
+// z slices z0 .. z0 + ZC - 1) can be updated: all of them integrate unwarped
+// (no kNN id in their runs), the rectangle of their centres lies in front of
+// the camera, and every pixel a centre can round to (the image bounds of the
+// rectangle's corners, which bound every centre's projection, widened by
+// 2 px) holds no usable depth or one nearer than the rectangle by more than
+// the truncation (plus 0.1 mm, far above the rounding of the per-voxel
+// arithmetic).  Every centre then fails the image, d > 0 or sdf >= -trunc
+// test.  Wave-uniform (every lane takes the reduced values).
+template <bool WARP, int ZC>
+__device__ bool wave_culled(const IntegrateArgs &a, const CamXf &T, int bx, int y, int z0) {
+  const int lane = threadIdx.x & 63;
+  if (y >= a.Y) return true;
+  if (WARP) {
+    if (!a.knn_rows) return false;
+    const bool obs = lane < ZC && z0 + lane < a.Z && a.knn_rows[((z0 + lane) * a.Y + y) * a.nsx + bx] != 0u;
+    if (__ballot(obs) != 0ull) return false;
+  }
+  const int xa = bx * kTileX, xb = min(a.X, xa + kTileX) - 1, zb = min(a.Z, z0 + ZC) - 1;
+  float czmin = INFINITY, umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
+  if (lane < 4) {
+    const float3 p = f3(a.ox + (((lane & 1) ? xb : xa) + 0.5f) * a.s, a.oy + (y + 0.5f) * a.s,
+                        a.oz + (((lane & 2) ? zb : z0) + 0.5f) * a.s);
+    const float cxp = T.m[0][0] * p.x + T.m[0][1] * p.y + T.m[0][2] * p.z + T.m[0][3];
+    const float cyp = T.m[1][0] * p.x + T.m[1][1] * p.y + T.m[1][2] * p.z + T.m[1][3];
+    const float czp = T.m[2][0] * p.x + T.m[2][1] * p.y + T.m[2][2] * p.z + T.m[2][3];
+    czmin = czp;
+    umin = umax = a.fx * cxp / czp + a.cx;
+    vmin = vmax = a.fy * cyp / czp + a.cy;
+  }
+#pragma unroll
+  for (int off = 1; off < 4; off <<= 1) {  // lanes 0..3, then broadcast
+    czmin = fminf(czmin, __shfl_xor(czmin, off, 64));
+    umin = fminf(umin, __shfl_xor(umin, off, 64));
+    umax = fmaxf(umax, __shfl_xor(umax, off, 64));
+    vmin = fminf(vmin, __shfl_xor(vmin, off, 64));
+    vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
+  }
+  czmin = __shfl(czmin, 0, 64);
+  umin = __shfl(umin, 0, 64);
+  umax = __shfl(umax, 0, 64);
+  vmin = __shfl(vmin, 0, 64);
+  vmax = __shfl(vmax, 0, 64);
+  // near the camera plane, or projections far outside any image: no cull
+  if (!(czmin > 1e-3f) || !(umin > -1e6f && umax < 1e6f && vmin > -1e6f && vmax < 1e6f)) return false;
+  const int u0 = max(0, (int)floorf(umin) - 2), u1 = min(a.W - 1, (int)ceilf(umax) + 2);
+  const int v0 = max(0, (int)floorf(vmin) - 2), v1 = min(a.H - 1, (int)ceilf(vmax) + 2);
+  if (u0 > u1 || v0 > v1) return true;  // every centre projects outside the image
+  const int bu0 = u0 / kDepthBlock, bv0 = v0 / kDepthBlock;
+  const int nbu = u1 / kDepthBlock - bu0 + 1, nb = nbu * (v1 / kDepthBlock - bv0 + 1);
+  float m = -1.f;
+  for (int j = lane; j < nb; j += 64) m = fmaxf(m, a.dmax[(bv0 + j / nbu) * a.dmw + bu0 + j % nbu]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  return m < 0.f || m < czmin - a.trunc - 1e-4f;
+}
+
 __device__ inline DQ fetch_centered(const IntegrateArgs &a, int id, bool table) {
   if (table) return a.centered[id];
   const nrf_node &n = a.nodes[id];
@@ -122,6 +210,7 @@ __device__ void integrate_tile(const IntegrateArgs &a, const Vox &vs, const Knn 
   int n_upd = 0;
   unsigned occ_bits = 0;  // bit k: a voxel of brick z0 / 4 + k was updated
   const CamXf T = a.Tdev ? *a.Tdev : a.T;
+  if (a.dmax && wave_culled<WARP, ZC>(a, T, bx, y, z0)) return;
   if (x < a.X && y < a.Y) {
     const float px_w = a.ox + (x + 0.5f) * a.s;
     const float py_w = a.oy + (y + 0.5f) * a.s;
@@ -350,6 +439,8 @@ int integrate_batch_create(const IntegrateBatchInstance *inst, int count, int z_
     a.wmax = q.vp.max_weight;
     a.updated = q.updated;
     a.occ = q.occ;
+    a.dmax = nullptr;  // the wave cull needs each frame's block maxima: not in the batched form
+    a.dmw = 0;
     h[k].tiles_x = (a.X + kTileX - 1) / kTileX;
     h[k].tiles_y = (a.Y + kTileY - 1) / kTileY;
     base[k] = (int32_t)tiles;
@@ -388,6 +479,11 @@ int integrate_batch_run(IntegrateBatch *b, const BatchDepth &fr, hipStream_t st)
   return NRF_OK;
 }
 
+size_t depth_block_count(const nrf_camera *cam) {
+  if (cam->width <= 0 || cam->height <= 0) return 0;
+  return (size_t)((cam->width + kDepthBlock - 1) / kDepthBlock) * ((cam->height + kDepthBlock - 1) / kDepthBlock);
+}
+
 int64_t knn_row_words(const nrf_volume_params *vp) {
   return (int64_t)((vp->dims[0] + kTileX - 1) / kTileX) * vp->dims[1] * vp->dims[2];
 }
@@ -416,7 +512,8 @@ int tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels, const float *
                    const float camera_pose[16], const nrf_node *nodes, const nrf_dualquat *transforms, int num_nodes,
                    const nrf_dualquat *centered, const int32_t *voxel_knn, const float *voxel_knn_w,
                    const uint32_t *knn_rows, const int32_t *voxel_opt_counts, int min_opt_count,
-                   int32_t *updated_count, hipStream_t st, const void *dev_pose, uint8_t *occ) {
+                   int32_t *updated_count, hipStream_t st, const void *dev_pose, uint8_t *occ,
+                   float *depth_blocks) {
   if (!volume_ok(vp) || !voxels || !depth || !cam || !camera_pose) return NRF_ERR_INVALID_ARGUMENT;
   if (cam->width <= 0 || cam->height <= 0) return NRF_ERR_INVALID_ARGUMENT;
   const bool warp = nodes != nullptr && num_nodes > 0;
@@ -446,6 +543,14 @@ int tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels, const float *
   a.wmax = vp->max_weight;
   a.updated = updated_count;
   a.occ = occ;
+  a.dmax = depth_blocks;
+  a.dmw = (a.W + kDepthBlock - 1) / kDepthBlock;
+  if (depth_blocks) {
+    const int bh = (a.H + kDepthBlock - 1) / kDepthBlock;
+    depth_block_max_kernel<<<(a.dmw * bh * kDepthBlock + 255) / 256, 256, 0, st>>>(depth, a.W, a.H, a.dmw, bh,
+                                                                                   depth_blocks);
+    NRF_LAUNCH_CHECK();
+  }
   dim3 grid((a.X + kTileX - 1) / kTileX, (a.Y + kTileY - 1) / kTileY, (a.Z + kZChunk - 1) / kZChunk);
   if (!warp)
     integrate_kernel<false, false><<<grid, 256, 0, st>>>(a);
@@ -461,7 +566,7 @@ int tsdf16_integrate(const nrf_volume_params *vp, uint16_t *tsdf16, uint8_t *wei
                      const nrf_camera *cam, const float camera_pose[16], const nrf_dualquat *centered, int num_nodes,
                      const uint16_t *knn16, const uint16_t *knn_w16, const uint32_t *knn_rows,
                      int32_t *updated_count, hipStream_t st, const int32_t *voxel_opt_counts, int min_opt_count,
-                     const void *dev_pose, uint8_t *occ) {
+                     const void *dev_pose, uint8_t *occ, float *depth_blocks) {
   if (!volume_ok(vp) || !tsdf16 || !weight8 || !depth || !cam || !camera_pose) return NRF_ERR_INVALID_ARGUMENT;
   if (cam->width <= 0 || cam->height <= 0) return NRF_ERR_INVALID_ARGUMENT;
   if (num_nodes > 0 && (!centered || !knn16 || !knn_w16)) return NRF_ERR_INVALID_ARGUMENT;
@@ -492,6 +597,14 @@ int tsdf16_integrate(const nrf_volume_params *vp, uint16_t *tsdf16, uint8_t *wei
   a.wmax = vp->max_weight;
   a.updated = updated_count;
   a.occ = occ;
+  a.dmax = depth_blocks;
+  a.dmw = (a.W + kDepthBlock - 1) / kDepthBlock;
+  if (depth_blocks) {
+    const int bh = (a.H + kDepthBlock - 1) / kDepthBlock;
+    depth_block_max_kernel<<<(a.dmw * bh * kDepthBlock + 255) / 256, 256, 0, st>>>(depth, a.W, a.H, a.dmw, bh,
+                                                                                   depth_blocks);
+    NRF_LAUNCH_CHECK();
+  }
   const VoxCompact vs{reinterpret_cast<__half *>(tsdf16), weight8};
   const Knn16 kn{reinterpret_cast<const ushort4 *>(knn16), reinterpret_cast<const uint2 *>(knn_w16)};
   dim3 grid((a.X + kTileX - 1) / kTileX, (a.Y + kTileY - 1) / kTileY, (a.Z + kZChunk - 1) / kZChunk);

@@ -213,7 +213,8 @@ int tsdf16_integrate(const nrf_volume_params *vp, uint16_t *tsdf16, uint8_t *wei
                      const nrf_camera *cam, const float camera_pose[16], const nrf_dualquat *centered, int num_nodes,
                      const uint16_t *knn16, const uint16_t *knn_w16, const uint32_t *knn_rows,
                      int32_t *updated_count, hipStream_t st, const int32_t *voxel_opt_counts = nullptr,
-                     int min_opt_count = 0, const void *dev_pose = nullptr, uint8_t *occ = nullptr);
+                     int min_opt_count = 0, const void *dev_pose = nullptr, uint8_t *occ = nullptr,
+                     float *depth_blocks = nullptr);
 // dev_pose (tsdf_integrate / tsdf16_integrate): a device CamXf (the
 // world-to-camera transform, as invert_pose gives it) that the kernel reads
 // instead of camera_pose's inverse, so a captured launch can take each
@@ -232,12 +233,19 @@ int mc_extract(const nrf_volume_params *vp, const nrf_voxel *voxels, const uint8
 int mc16_extract(const nrf_volume_params *vp, const uint16_t *tsdf16, const uint8_t *weight8, const uint8_t *occ,
                  void *workspace, size_t workspace_bytes, float *vertices, int32_t *triangles, int max_triangles,
                  int32_t *counts, hipStream_t st);
+// depth_blocks (tsdf_integrate / tsdf16_integrate): scratch of
+// depth_block_count floats; when given, the call first writes the depth's
+// block maxima there and every wave of the integration whose voxels all
+// integrate unwarped and cannot reach a usable depth within the truncation
+// skips its loads (the same result).
+size_t depth_block_count(const nrf_camera *cam);
 // nrf_tsdf_integrate with the kNN's row occupancy (null: none)
 int tsdf_integrate(const nrf_volume_params *vp, nrf_voxel *voxels, const float *depth, const nrf_camera *cam,
                    const float camera_pose[16], const nrf_node *nodes, const nrf_dualquat *transforms, int num_nodes,
                    const nrf_dualquat *centered, const int32_t *voxel_knn, const float *voxel_knn_w,
                    const uint32_t *knn_rows, const int32_t *voxel_opt_counts, int min_opt_count,
-                   int32_t *updated_count, hipStream_t st, const void *dev_pose = nullptr, uint8_t *occ = nullptr);
+                   int32_t *updated_count, hipStream_t st, const void *dev_pose = nullptr, uint8_t *occ = nullptr,
+                   float *depth_blocks = nullptr);
 
 }  // namespace nrf
 

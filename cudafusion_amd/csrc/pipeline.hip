@@ -36,6 +36,7 @@ struct nrf_pipeline {
   // voxel with weight > 0, since only an update makes a weight positive);
   // marching cubes skips the other bricks
   uint8_t *occ = nullptr;
+  float *dblk = nullptr;  // depth block maxima of the frame being integrated (integration's wave cull)
   float *verts = nullptr;
   int32_t *tris = nullptr;
   int32_t *counts = nullptr;
@@ -79,7 +80,7 @@ struct nrf_pipeline {
 };
 
 static void pipeline_free(nrf_pipeline *p) {
-  void *ptrs[] = {p->vox, p->knn, p->knn_w, p->knn_rows, p->nodes, p->T, p->centered, p->delta_x, p->mc_ws, p->occ,
+  void *ptrs[] = {p->vox, p->knn, p->knn_w, p->knn_rows, p->nodes, p->T, p->centered, p->delta_x, p->mc_ws, p->occ, p->dblk,
                   p->verts, p->tris, p->counts, p->updated, p->t16, p->w8, p->k16, p->kw16,
                   p->tk.dnorm, p->tk.warped, p->tk.live_v, p->tk.live_n, p->tk.ztri, p->tk.hit, p->tk.opt,
                   p->tk.opt_frame, p->tk.corrs, p->tk.part, p->tk.Tsave, p->tk.dev,
@@ -117,6 +118,7 @@ extern "C" nrf_pipeline *nrf_pipeline_create(const nrf_pipeline_config *cfg) {
   ok = ok && hipMalloc(&p->mc_ws, p->mc_bytes) == hipSuccess;
   const size_t nocc = mc_occupancy_bytes(&cfg->volume);
   ok = ok && hipMalloc(&p->occ, nocc) == hipSuccess;
+  ok = ok && hipMalloc(&p->dblk, std::max<size_t>(1, depth_block_count(&cfg->camera)) * sizeof(float)) == hipSuccess;
   const size_t T = (size_t)std::max(1, cfg->max_triangles);
   ok = ok && hipMalloc(&p->verts, T * 9 * sizeof(float)) == hipSuccess;
   ok = ok && hipMalloc(&p->tris, T * 3 * sizeof(int32_t)) == hipSuccess;
@@ -180,7 +182,7 @@ extern "C" int nrf_pipeline_integrate_rigid(nrf_pipeline *p, const float *depth,
   if (!p || !depth || !camera_pose) return NRF_ERR_INVALID_ARGUMENT;
   p->mesh_valid = false;
   return tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, nullptr, nullptr, 0, nullptr,
-                        nullptr, nullptr, nullptr, nullptr, 0, nullptr, as_stream(stream), nullptr, p->occ);
+                        nullptr, nullptr, nullptr, nullptr, 0, nullptr, as_stream(stream), nullptr, p->occ, p->dblk);
 }
 
 extern "C" int nrf_pipeline_set_nodes(nrf_pipeline *p, const nrf_node *host_nodes, int num_nodes) {
@@ -259,11 +261,11 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
   if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[1], st));
   if (p->cfg.storage16)
     rc = tsdf16_integrate(&p->cfg.volume, p->t16, p->w8, depth, &p->cfg.camera, camera_pose, p->centered, p->N,
-                          p->k16, p->kw16, p->knn_rows, p->updated, st, nullptr, 0, dev_pose, p->occ);
+                          p->k16, p->kw16, p->knn_rows, p->updated, st, nullptr, 0, dev_pose, p->occ, p->dblk);
   else
     rc = tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, p->nodes, p->T, p->N,
                         p->centered, p->knn, p->knn_w, p->knn_rows, nullptr, p->cfg.min_opt_count, p->updated, st,
-                        dev_pose, p->occ);
+                        dev_pose, p->occ, p->dblk);
   if (rc) return rc;
   if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[2], st));
   if (profile && !p->prof_pcg) NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 2], st));
@@ -527,10 +529,10 @@ extern "C" int nrf_pipeline_track(nrf_pipeline *p, const float *depth, const flo
     return rc;
   if (p->cfg.storage16)
     rc = tsdf16_integrate(&p->cfg.volume, p->t16, p->w8, depth, cam, camera_pose, p->centered, p->N, p->k16, p->kw16,
-                          p->knn_rows, p->updated, st, k.opt, p->cfg.min_opt_count, nullptr, p->occ);
+                          p->knn_rows, p->updated, st, k.opt, p->cfg.min_opt_count, nullptr, p->occ, p->dblk);
   else
     rc = tsdf_integrate(&p->cfg.volume, p->vox, depth, cam, camera_pose, p->nodes, p->T, p->N, p->centered, p->knn,
-                        p->knn_w, p->knn_rows, k.opt, p->cfg.min_opt_count, p->updated, st, nullptr, p->occ);
+                        p->knn_w, p->knn_rows, k.opt, p->cfg.min_opt_count, p->updated, st, nullptr, p->occ, p->dblk);
   if (rc) return rc;
   NRF_HIP_CHECK(hipEventRecord(p->ev[2], st));
   if ((rc = pipeline_mc(p, st))) return rc;
