@@ -87,7 +87,7 @@ class SolverParams(C.Structure):
                 ("debug", C.c_int32), ("relative_tolerance", C.c_int32), ("precond_fp64", C.c_int32),
                 ("pcg_pipelined", C.c_int32), ("arap_consistent_sign", C.c_int32), ("jtj_precision", C.c_int32),
                 ("refine_iterations", C.c_int32), ("refine_jacobi", C.c_int32),
-                ("true_residual_rounds", C.c_int32), ("psd_dropped_pairs", C.c_int32)]
+                ("true_residual_rounds", C.c_int32), ("psd_dropped_pairs", C.c_int32), ("plain_form", C.c_int32)]
 
 
 class GNParams(C.Structure):
@@ -277,11 +277,11 @@ def camera(fx, fy, cx, cy, width, height) -> Camera:
 def solver_params(gn_iterations=3, pcg_iterations=10, pcg_tolerance=1e-4, lambda_smooth=1.0, lambda_rot=1.0,
                   lambda_damping=1e-2, debug=0, relative_tolerance=0, precond_fp64=0,
                   arap_consistent_sign=0, pcg_pipelined=0, jtj_precision=0, refine_iterations=0,
-                  refine_jacobi=0, true_residual_rounds=0, psd_dropped_pairs=0) -> SolverParams:
+                  refine_jacobi=0, true_residual_rounds=0, psd_dropped_pairs=0, plain_form=0) -> SolverParams:
     return SolverParams(gn_iterations, pcg_iterations, pcg_tolerance, lambda_smooth, lambda_rot, lambda_damping,
                         debug, relative_tolerance, precond_fp64, pcg_pipelined, arap_consistent_sign, jtj_precision,
                         refine_iterations, refine_jacobi, true_residual_rounds,
-                        psd_dropped_pairs)
+                        psd_dropped_pairs, plain_form)
 
 
 def gn_params(gn_iterations=3, update_scale=0.25, max_update_rot=0.05, max_update_trans=0.03, validate=1,

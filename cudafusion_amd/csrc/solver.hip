@@ -376,14 +376,12 @@ __device__ inline float stage_membership(const AsmArgs &a, int key, const int32_
   return fw;
 }
 
-// One ARAP edge-side e (kernels/solver_kernels.cu:202-352): stages side,
-// offrel, r and the two Jacobians in st[1..41] when its weight passes the
-// threshold (:271) and returns the weight (0 otherwise: st[1..] untouched).
-__device__ inline float arap_side(const AsmArgs &a, int e, float *st) {
-  const int ent = a.es_entry[e];
-  const int edge = ent >> 1, side = ent & 1;
-  const int ni = a.e_i[edge], nj = a.e_j[edge];
-  const float nw = fmaxf(a.e_w[edge], 0.05f);
+// ARAP term of the edge (ni, nj) with graph weight nw (kernels/solver_kernels.cu:
+// 236-290): residual r = T_i x_j - T_j x_j, the Jacobians of both ends, and
+// the weight lambda * nw * Huber * strain, or 0 when it fails the
+// threshold of :271 (Ji, Jj untouched then).
+__device__ inline float arap_edge(const AsmArgs &a, int ni, int nj, float nw, float3 &r, float Ji[6][3],
+                                  float Jj[6][3]) {
   const float3 xi = f3(a.nodes[ni].pos), xj = f3(a.nodes[nj].pos);
   const DQ Ti = a.T[ni], Tj = a.T[nj];
   const float3 Ti_xj = dq_transform_centered(Ti, xj, xi);
@@ -396,14 +394,26 @@ __device__ inline float arap_side(const AsmArgs &a, int e, float *st) {
   const float strain = fabsf(dc - dr) / fmaxf(dr, 1e-6f);
   const float ratio = strain / 0.25f;
   const float sw = expf(-(ratio * ratio));
-  const float3 r = sub(Ti_xj, Tj_xj);
+  r = sub(Ti_xj, Tj_xj);
   const float rn = sqrtf(r.x * r.x + r.y * r.y + r.z * r.z);
   const float hw = huber_w(rn, 0.05f);
   const float lam = a.lambda_smooth * nw * hw * sw;
   if (lam <= 1e-8f) return 0.f;
-  float Ji[6][3], Jj[6][3];
   arap_jac(sub(Ti_xj, xi), a.arap_sign, Ji);
   arap_jac(sub(Tj_xj, xj), a.arap_sign, Jj);
+  return lam;
+}
+
+// One ARAP edge-side e: stages side, offrel, r and the two Jacobians in
+// st[1..41] when its weight passes the threshold and returns the weight
+// (0 otherwise: st[1..] untouched).
+__device__ inline float arap_side(const AsmArgs &a, int e, float *st) {
+  const int ent = a.es_entry[e];
+  const int edge = ent >> 1, side = ent & 1;
+  float3 r;
+  float Ji[6][3], Jj[6][3];
+  const float lam = arap_edge(a, a.e_i[edge], a.e_j[edge], fmaxf(a.e_w[edge], 0.05f), r, Ji, Jj);
+  if (lam == 0.f) return 0.f;
   st[1] = __int_as_float(side);
   st[2] = __int_as_float(a.es_offrel[e]);
   st[3] = r.x; st[4] = r.y; st[5] = r.z;
@@ -2231,6 +2241,287 @@ __global__ __launch_bounds__(256) void gn_refine_update_kernel(GnArgs a, RefineA
   if (blockIdx.x == 0 && threadIdx.x == 0) a.res->gn_done += 1;
 }
 
+// ------------------------------------------------------ plain form
+// The reference's GPU structure of the solve (kernels/solver_kernels.cu:
+// 543-741), kept as the baseline that the optimised forms are measured
+// against (solver option plain_form): one kernel per operation, float
+// atomics into the BSR blocks (one lane per correspondence, one per node for
+// the ARAP edges j > i), and a host loop around the PCG that reads each dot
+// product back to the host, as thrust::inner_product does.  The dot
+// products are summed in FP64 (the configuration's FP32 step with FP64 PCG
+// dots); everything else is FP32.  Not graph-capturable (host reads).
+constexpr int kPlainThreads = 256;
+
+// find_block (kernels/solver_kernels.cu:28-44): binary search of column j
+// in the sorted row i.
+__device__ inline int plain_find_block(const int32_t *row_ptr, const int32_t *col_idx, int i, int j) {
+  int lo = row_ptr[i], hi = row_ptr[i + 1] - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int c = col_idx[mid];
+    if (c == j) return mid;
+    if (c < j) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  return -1;
+}
+
+__global__ __launch_bounds__(kPlainThreads) void plain_clear_kernel(float *__restrict__ values, int nv,
+                                                                   float *__restrict__ b, int n6) {
+  for (int k = blockIdx.x * kPlainThreads + threadIdx.x; k < nv; k += gridDim.x * kPlainThreads) values[k] = 0.f;
+  for (int k = blockIdx.x * kPlainThreads + threadIdx.x; k < n6; k += gridDim.x * kPlainThreads) b[k] = 0.f;
+}
+
+// assemble_data_term_kernel (kernels/solver_kernels.cu:51-138): the source
+// re-warped from the canonical point when one is given (the multi-GN entry).
+__global__ __launch_bounds__(kPlainThreads) void plain_data_kernel(AsmArgs a, int n) {
+  const int c = blockIdx.x * kPlainThreads + threadIdx.x;
+  if (c >= n) return;
+  const nrf_corr &cr = a.corrs[c];
+  if (!cr.valid) return;
+  const float3 dst = f3(cr.dst), nrm = f3(cr.normal);
+  const int ids[4] = {cr.node_ids[0], cr.node_ids[1], cr.node_ids[2], cr.node_ids[3]};
+  const float ws[4] = {cr.node_ws[0], cr.node_ws[1], cr.node_ws[2], cr.node_ws[3]};
+  float3 src;
+  if (a.canonical) {
+    src = f3(a.canonical[3 * c], a.canonical[3 * c + 1], a.canonical[3 * c + 2]);
+    DQ bl;
+    if (dq_blend4(a.centered, ids, ws, bl)) src = dq_transform(bl, src);
+  } else {
+    src = f3(cr.src);
+  }
+  const float r = dot(nrm, sub(src, dst));
+  const float fw = fmaxf(cr.weight, 0.0f) * huber_w(r, 0.2f);
+  if (fw <= 1e-8f) return;
+  float J[4][6];
+  bool ok[4];
+  for (int k = 0; k < 4; ++k) {
+    const int id = ids[k];
+    ok[k] = id >= 0 && id < a.N && ws[k] > 1e-8f;
+    if (!ok[k]) continue;
+    const float3 cp = cross(sub(src, f3(a.nodes[id].pos)), nrm);
+    const float w = ws[k];
+    J[k][0] = w * cp.x; J[k][1] = w * cp.y; J[k][2] = w * cp.z;
+    J[k][3] = w * nrm.x; J[k][4] = w * nrm.y; J[k][5] = w * nrm.z;
+    for (int q = 0; q < 6; ++q) atomicAdd(&a.b[id * 6 + q], -fw * J[k][q] * r);
+  }
+  for (int ki = 0; ki < 4; ++ki) {
+    if (!ok[ki]) continue;
+    for (int kj = 0; kj < 4; ++kj) {
+      if (!ok[kj]) continue;
+      const int blk = plain_find_block(a.row_ptr, a.col_idx, ids[ki], ids[kj]);
+      if (blk < 0) continue;  // outside the pattern: dropped (Q2)
+      float *B = a.values + (size_t)blk * 36;
+      for (int p = 0; p < 6; ++p)
+        for (int q = 0; q < 6; ++q) atomicAdd(&B[p * 6 + q], fw * J[ki][p] * J[kj][q]);
+    }
+  }
+}
+
+// assemble_smooth_term_kernel (kernels/solver_kernels.cu:202-352): one lane
+// per node i, its edges to graph neighbours j > i.
+__global__ __launch_bounds__(kPlainThreads) void plain_arap_kernel(AsmArgs a) {
+  const int ni = blockIdx.x * kPlainThreads + threadIdx.x;
+  if (ni >= a.N) return;
+  const nrf_node &nd = a.nodes[ni];
+  const int nn = max(0, min((int)nd.num_neighbors, (int)NRF_K_GRAPH));
+  for (int k = 0; k < nn; ++k) {
+    const int nj = nd.neighbors[k];
+    if (nj < 0 || nj >= a.N || nj <= ni) continue;
+    float3 r;
+    float Ji[6][3], Jj[6][3];
+    const float lam = arap_edge(a, ni, nj, fmaxf(nd.neighbor_w[k], 0.05f), r, Ji, Jj);
+    if (lam == 0.f) continue;
+    const int bi[4] = {plain_find_block(a.row_ptr, a.col_idx, ni, ni), plain_find_block(a.row_ptr, a.col_idx, nj, nj),
+                       plain_find_block(a.row_ptr, a.col_idx, ni, nj), plain_find_block(a.row_ptr, a.col_idx, nj, ni)};
+    for (int t = 0; t < 4; ++t) {
+      if (bi[t] < 0) continue;
+      const float(*L)[3] = (t == 0 || t == 2) ? Ji : Jj;
+      const float(*R)[3] = (t == 0 || t == 3) ? Ji : Jj;
+      const float sg = t < 2 ? lam : -lam;
+      float *B = a.values + (size_t)bi[t] * 36;
+      for (int p = 0; p < 6; ++p)
+        for (int q = 0; q < 6; ++q) {
+          float v = 0.f;
+          for (int c = 0; c < 3; ++c) v += L[p][c] * R[q][c];
+          atomicAdd(&B[p * 6 + q], sg * v);
+        }
+    }
+    const float rv[3] = {r.x, r.y, r.z};
+    for (int p = 0; p < 6; ++p) {
+      float vi = 0.f, vj = 0.f;
+      for (int c = 0; c < 3; ++c) {
+        vi += Ji[p][c] * rv[c];
+        vj += Jj[p][c] * rv[c];
+      }
+      atomicAdd(&a.b[ni * 6 + p], -lam * vi);
+      atomicAdd(&a.b[nj * 6 + p], lam * vj);
+    }
+  }
+}
+
+// add_diagonal_damping_kernel (:167-194) and compute_preconditioner_kernel
+// (:358-399, FP64 Cholesky with precond_fp64), one lane per node.
+__global__ __launch_bounds__(kPlainThreads) void plain_damping_kernel(AsmArgs a) {
+  const int i = blockIdx.x * kPlainThreads + threadIdx.x;
+  if (i >= a.N) return;
+  const int blk = plain_find_block(a.row_ptr, a.col_idx, i, i);
+  if (blk < 0) return;
+  float *D = a.values + (size_t)blk * 36;
+  for (int k = 0; k < 6; ++k) {
+    const float cur = D[k * 7];
+    D[k * 7] = cur + a.lambda_damping * fmaxf(fabsf(cur), k < 3 ? 1e-3f : 1e-2f);
+  }
+}
+__global__ __launch_bounds__(kPlainThreads) void plain_precond_kernel(AsmArgs a) {
+  const int i = blockIdx.x * kPlainThreads + threadIdx.x;
+  if (i >= a.N) return;
+  const int blk = plain_find_block(a.row_ptr, a.col_idx, i, i);
+  float D[36];
+  for (int k = 0; k < 36; ++k) D[k] = blk >= 0 ? a.values[(size_t)blk * 36 + k] : 0.f;
+  for (int col = 0; col < 6; ++col) {
+    float outc[6];
+    bool ok = true;
+    if (a.precond_fp64) {
+      double d[36], x[6];
+      for (int k = 0; k < 36; ++k) d[k] = D[k];
+      for (int k = 0; k < 6; ++k) d[k * 7] += (double)1e-4f;
+      chol_inv6<double>(d, x, col, ok);
+      for (int k = 0; k < 6; ++k) outc[k] = (float)x[k];
+      if (!ok)
+        for (int k = 0; k < 6; ++k) outc[k] = (k == col) ? (float)(1.0 / (d[k * 7] + (double)1e-6f)) : 0.f;
+    } else {
+      float d[36];
+      for (int k = 0; k < 36; ++k) d[k] = D[k];
+      for (int k = 0; k < 6; ++k) d[k * 7] += 1e-4f;
+      chol_inv6<float>(d, outc, col, ok);
+      if (!ok)
+        for (int k = 0; k < 6; ++k) outc[k] = (k == col) ? 1.f / (d[k * 7] + 1e-6f) : 0.f;
+    }
+    for (int k = 0; k < 6; ++k) a.Minv[(size_t)i * 36 + k * 6 + col] = outc[k];
+  }
+}
+
+// bsr_spmv_kernel (:406-432): one lane per scalar row.
+__global__ __launch_bounds__(kPlainThreads) void plain_spmv_kernel(const int32_t *__restrict__ row_ptr,
+                                                                  const int32_t *__restrict__ col_idx,
+                                                                  const float *__restrict__ values,
+                                                                  const float *__restrict__ x, float *__restrict__ y,
+                                                                  int N) {
+  const int g = blockIdx.x * kPlainThreads + threadIdx.x;
+  if (g >= 6 * N) return;
+  const int i = g / 6, row = g % 6;
+  float sum = 0.f;
+  for (int b = row_ptr[i]; b < row_ptr[i + 1]; ++b) {
+    const float *B = values + (size_t)b * 36 + row * 6;
+    const float *xj = x + 6 * col_idx[b];
+    for (int c = 0; c < 6; ++c) sum += B[c] * xj[c];
+  }
+  y[g] = sum;
+}
+// apply_precond_kernel (:438-455): z_i = M_i^-1 r_i, one lane per node.
+__global__ __launch_bounds__(kPlainThreads) void plain_precond_apply_kernel(const float *__restrict__ Minv,
+                                                                           const float *__restrict__ r,
+                                                                           float *__restrict__ z, int N) {
+  const int i = blockIdx.x * kPlainThreads + threadIdx.x;
+  if (i >= N) return;
+  for (int p = 0; p < 6; ++p) {
+    float v = 0.f;
+    for (int q = 0; q < 6; ++q) v += Minv[(size_t)i * 36 + p * 6 + q] * r[6 * i + q];
+    z[6 * i + p] = v;
+  }
+}
+// y += alpha x (axpy_kernel) and y = x + alpha y (xpay_kernel).
+__global__ __launch_bounds__(kPlainThreads) void plain_axpy_kernel(float alpha, const float *__restrict__ x,
+                                                                  float *__restrict__ y, int n) {
+  const int k = blockIdx.x * kPlainThreads + threadIdx.x;
+  if (k < n) y[k] += alpha * x[k];
+}
+__global__ __launch_bounds__(kPlainThreads) void plain_xpay_kernel(float alpha, const float *__restrict__ x,
+                                                                  float *__restrict__ y, int n) {
+  const int k = blockIdx.x * kPlainThreads + threadIdx.x;
+  if (k < n) y[k] = x[k] + alpha * y[k];
+}
+// FP64 dot products: per-block partial sums (tree in LDS), then one block
+// adds them in a fixed order.
+__global__ __launch_bounds__(kPlainThreads) void plain_dot_kernel(const float *__restrict__ x,
+                                                                 const float *__restrict__ y, int n,
+                                                                 double *__restrict__ partial) {
+  __shared__ double sh[kPlainThreads];
+  const int k = blockIdx.x * kPlainThreads + threadIdx.x;
+  sh[threadIdx.x] = k < n ? (double)x[k] * (double)y[k] : 0.0;
+  __syncthreads();
+  for (int w = kPlainThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+__global__ __launch_bounds__(kPlainThreads) void plain_sum_kernel(const double *__restrict__ partial, int nblk,
+                                                                 double *__restrict__ out) {
+  __shared__ double sh[kPlainThreads];
+  double v = 0.0;
+  for (int k = threadIdx.x; k < nblk; k += kPlainThreads) v += partial[k];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kPlainThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+// r = b - A x in FP64 (row sums of the FP32 blocks), stored in FP32, with
+// per-block partial sums of r.r and b.b (partial[2 blk], partial[2 blk + 1]).
+__global__ __launch_bounds__(kPlainThreads) void plain_residual_kernel(const int32_t *__restrict__ row_ptr,
+                                                                      const int32_t *__restrict__ col_idx,
+                                                                      const float *__restrict__ values,
+                                                                      const float *__restrict__ b,
+                                                                      const float *__restrict__ x, int N,
+                                                                      float *__restrict__ r_out,
+                                                                      double *__restrict__ partial) {
+  __shared__ double s0[kPlainThreads], s1[kPlainThreads];
+  const int g = blockIdx.x * kPlainThreads + threadIdx.x;
+  double rr = 0.0, bb = 0.0;
+  if (g < 6 * N) {
+    const int i = g / 6, row = g % 6;
+    double acc = b[g];
+    for (int blk = row_ptr[i]; blk < row_ptr[i + 1]; ++blk) {
+      const float *B = values + (size_t)blk * 36 + row * 6;
+      const float *xj = x + 6 * col_idx[blk];
+      for (int c = 0; c < 6; ++c) acc -= (double)B[c] * (double)xj[c];
+    }
+    r_out[g] = (float)acc;
+    rr = acc * acc;
+    bb = (double)b[g] * (double)b[g];
+  }
+  s0[threadIdx.x] = rr;
+  s1[threadIdx.x] = bb;
+  __syncthreads();
+  for (int w = kPlainThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = s0[0];
+    partial[2 * blockIdx.x + 1] = s1[0];
+  }
+}
+// The run's record (pcg_tally / tr_decide semantics) written by one lane.
+__global__ void plain_tally_kernel(nrf_pcg_result *res, int iters, int flags, double rr, double bb) {
+  if (threadIdx.x != 0) return;
+  res->total_iterations += iters;
+  res->iterations = iters;
+  res->flags |= flags;
+  res->final_residual = sqrt(rr);
+  res->rhs_norm = sqrt(bb);
+  res->converged += (flags & 2) ? 1 : 0;
+  res->breakdowns += (flags & 1) ? 1 : 0;
+  res->runs += 1;
+}
+
 // ------------------------------------------------------------- batches
 // Several independent solves stepped by common launches (nrf_batch): every
 // launch covers all instances of the batch.  An instance's device state is
@@ -2563,6 +2854,10 @@ struct nrf_solver {
   size_t c_rpart = 0, c_rnorm = 0;
   TrState *tr = nullptr;  // true-residual rounds (over rres, dref, rpart)
   size_t c_tr = 0;
+  // plain form: PCG vectors, FP64 dot partials, the host-read result
+  float *pl_r = nullptr, *pl_z = nullptr, *pl_p = nullptr, *pl_Ap = nullptr;
+  double *pl_part = nullptr, *pl_out = nullptr, *pl_host = nullptr;
+  size_t c_pl_r = 0, c_pl_z = 0, c_pl_p = 0, c_pl_Ap = 0, c_pl_part = 0, c_pl_out = 0;
   nrf_pcg_result *d_res = nullptr;
   size_t c_res = 0;
   nrf_solver_totals *d_tot = nullptr;
@@ -2957,6 +3252,155 @@ static int solver_tr_rounds(nrf_solver *s, float *x, int N, hipStream_t st) {
   return NRF_OK;
 }
 
+// ---- plain form (the reference's structure; see plain_data_kernel)
+static int plain_buffers(nrf_solver *s, int N) {
+  const size_t n6 = (size_t)6 * N;
+  const size_t nblk = (n6 + kPlainThreads - 1) / kPlainThreads;
+  int rc;
+  if ((rc = dev_alloc(s->pl_r, n6, s->c_pl_r)) || (rc = dev_alloc(s->pl_z, n6, s->c_pl_z)) ||
+      (rc = dev_alloc(s->pl_p, n6, s->c_pl_p)) || (rc = dev_alloc(s->pl_Ap, n6, s->c_pl_Ap)) ||
+      (rc = dev_alloc(s->pl_part, 2 * nblk, s->c_pl_part)) || (rc = dev_alloc(s->pl_out, 2, s->c_pl_out)))
+    return rc;
+  if (!s->pl_host) NRF_HIP_CHECK(hipHostMalloc((void **)&s->pl_host, 2 * sizeof(double), hipHostMallocDefault));
+  return NRF_OK;
+}
+
+// The data term, ARAP term, damping and block-Jacobi inverses as the
+// reference launches them (kernels/solver_kernels.cu:543-614), atomics
+// into the cleared system.
+static int plain_assemble(nrf_solver *s, const nrf_corr *corrs, int n, const float *canonical,
+                          const nrf_node *nodes, const DQ *T, const DQ *centered, int N, hipStream_t st) {
+  const AsmArgs a = solver_asm_args(s, corrs, canonical, nodes, T, centered, N);
+  const int nv = s->P.nb * 36, gn = (N + kPlainThreads - 1) / kPlainThreads;
+  plain_clear_kernel<<<std::max(1, std::min(2048, (nv + kPlainThreads - 1) / kPlainThreads)), kPlainThreads, 0, st>>>(
+      s->values, nv, s->b, 6 * N);
+  NRF_LAUNCH_CHECK();
+  if (n > 0) {
+    plain_data_kernel<<<(n + kPlainThreads - 1) / kPlainThreads, kPlainThreads, 0, st>>>(a, n);
+    NRF_LAUNCH_CHECK();
+  }
+  plain_arap_kernel<<<gn, kPlainThreads, 0, st>>>(a);
+  NRF_LAUNCH_CHECK();
+  plain_damping_kernel<<<gn, kPlainThreads, 0, st>>>(a);
+  NRF_LAUNCH_CHECK();
+  plain_precond_kernel<<<gn, kPlainThreads, 0, st>>>(a);
+  NRF_LAUNCH_CHECK();
+  return NRF_OK;
+}
+
+// x . y in FP64, read back to the host (thrust::inner_product's round trip).
+static int plain_dot(nrf_solver *s, const float *x, const float *y, int n, hipStream_t st, double &out) {
+  const int nblk = (n + kPlainThreads - 1) / kPlainThreads;
+  plain_dot_kernel<<<nblk, kPlainThreads, 0, st>>>(x, y, n, s->pl_part);
+  NRF_LAUNCH_CHECK();
+  plain_sum_kernel<<<1, kPlainThreads, 0, st>>>(s->pl_part, nblk, s->pl_out);
+  NRF_LAUNCH_CHECK();
+  NRF_HIP_CHECK(hipMemcpyAsync(s->pl_host, s->pl_out, sizeof(double), hipMemcpyDeviceToHost, st));
+  NRF_HIP_CHECK(hipStreamSynchronize(st));
+  out = s->pl_host[0];
+  return NRF_OK;
+}
+
+// pcg_solve (kernels/solver_kernels.cu:616-687) from x0 = 0 to ||r|| < tol:
+// SpMV, dot, two axpys, dot, preconditioner, dot, xpay per iteration, each
+// a launch, each dot read by the host.  iters / flags (bit 0 breakdown, bit 1
+// converged) / rr (the recurrence's r.r at the end) are returned.
+static int plain_pcg(nrf_solver *s, const float *rhs, float *x, int N, int maxit, double tol, hipStream_t st,
+                     int &iters, int &flags, double &rr) {
+  const int n = 6 * N, gv = (n + kPlainThreads - 1) / kPlainThreads, gn = (N + kPlainThreads - 1) / kPlainThreads;
+  int rc;
+  NRF_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(float) * n, st));
+  NRF_HIP_CHECK(hipMemcpyAsync(s->pl_r, rhs, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+  plain_precond_apply_kernel<<<gn, kPlainThreads, 0, st>>>(s->Minv, s->pl_r, s->pl_z, N);
+  NRF_LAUNCH_CHECK();
+  NRF_HIP_CHECK(hipMemcpyAsync(s->pl_p, s->pl_z, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+  double rz;
+  if ((rc = plain_dot(s, s->pl_r, s->pl_z, n, st, rz))) return rc;
+  rr = -1.0;
+  iters = 0;
+  flags = 0;
+  const bool rel = s->params.relative_tolerance != 0;
+  for (int it = 0; it < maxit; ++it) {
+    plain_spmv_kernel<<<gv, kPlainThreads, 0, st>>>(s->row_ptr, s->col_idx, s->values, s->pl_p, s->pl_Ap, N);
+    NRF_LAUNCH_CHECK();
+    double pAp;
+    if ((rc = plain_dot(s, s->pl_p, s->pl_Ap, n, st, pAp))) return rc;
+    if (rel ? !(pAp > 0.0) : fabs(pAp) < 1e-12) {  // the breakdown stop of the chosen tolerance form
+      flags |= 1;
+      break;
+    }
+    const double alpha = rz / pAp;
+    plain_axpy_kernel<<<gv, kPlainThreads, 0, st>>>((float)alpha, s->pl_p, x, n);
+    NRF_LAUNCH_CHECK();
+    plain_axpy_kernel<<<gv, kPlainThreads, 0, st>>>((float)-alpha, s->pl_Ap, s->pl_r, n);
+    NRF_LAUNCH_CHECK();
+    if ((rc = plain_dot(s, s->pl_r, s->pl_r, n, st, rr))) return rc;
+    iters = it + 1;
+    if (sqrt(rr) < tol) {
+      flags |= 2;
+      break;
+    }
+    plain_precond_apply_kernel<<<gn, kPlainThreads, 0, st>>>(s->Minv, s->pl_r, s->pl_z, N);
+    NRF_LAUNCH_CHECK();
+    double rz_new;
+    if ((rc = plain_dot(s, s->pl_r, s->pl_z, n, st, rz_new))) return rc;
+    plain_xpay_kernel<<<gv, kPlainThreads, 0, st>>>((float)(rz_new / rz), s->pl_z, s->pl_p, n);
+    NRF_LAUNCH_CHECK();
+    rz = rz_new;
+  }
+  if (rr < 0.0) return plain_dot(s, s->pl_r, s->pl_r, n, st, rr);  // stopped before its first r.r
+  return NRF_OK;
+}
+
+// One solve of the plain form: the PCG on the assembled system and, with
+// true_residual_rounds, correction rounds judged by the FP64 true residual
+// (as solver_tr_rounds); the outcome goes to the solver's result record.
+static int plain_solve(nrf_solver *s, float *x, int N, hipStream_t st) {
+  int rc;
+  if ((rc = plain_buffers(s, N))) return rc;
+  const int n = 6 * N, gv = (n + kPlainThreads - 1) / kPlainThreads;
+  double bb;
+  if ((rc = plain_dot(s, s->b, s->b, n, st, bb))) return rc;
+  const double tol = s->params.relative_tolerance ? (double)s->params.pcg_tolerance * sqrt(bb)
+                                                  : (double)s->params.pcg_tolerance;
+  int iters, flags;
+  double rr;
+  if ((rc = plain_pcg(s, s->b, x, N, s->params.pcg_iterations, tol, st, iters, flags, rr))) return rc;
+  const int R = s->params.true_residual_rounds;
+  if (R > 0) {
+    std::vector<double> part(2 * (size_t)gv);
+    for (int k = 0;; ++k) {
+      plain_residual_kernel<<<gv, kPlainThreads, 0, st>>>(s->row_ptr, s->col_idx, s->values, s->b, x, N, s->rres,
+                                                          s->pl_part);
+      NRF_LAUNCH_CHECK();
+      NRF_HIP_CHECK(hipMemcpyAsync(part.data(), s->pl_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+      NRF_HIP_CHECK(hipStreamSynchronize(st));
+      double tr = 0.0;
+      for (int q = 0; q < gv; ++q) tr += part[2 * q];
+      rr = tr;
+      const bool conv = sqrt(tr) < tol || bb == 0.0;
+      flags = conv ? ((flags & ~1) | 2) : (flags & ~2);
+      if (conv || k == R) break;
+      int it2, fl2;
+      double rr2;
+      if ((rc = plain_pcg(s, s->rres, s->dref, N, s->params.pcg_iterations, tol, st, it2, fl2, rr2))) return rc;
+      iters += it2;
+      flags = (flags & ~1) | (fl2 & 1);
+      plain_axpy_kernel<<<gv, kPlainThreads, 0, st>>>(1.0f, s->dref, x, n);
+      NRF_LAUNCH_CHECK();
+    }
+  }
+  plain_tally_kernel<<<1, 64, 0, st>>>(s->d_res, iters, flags, rr, bb);
+  NRF_LAUNCH_CHECK();
+  return NRF_OK;
+}
+
+static int plain_check_stream(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  NRF_HIP_CHECK(hipStreamIsCapturing(st, &cs));
+  return cs == hipStreamCaptureStatusNone ? NRF_OK : NRF_ERR_INVALID_ARGUMENT;  // host reads inside
+}
+
 static int solver_fetch_result(nrf_solver *s, nrf_pcg_result *out, hipStream_t st) {
   if (!out) return NRF_OK;
   NRF_HIP_CHECK(hipMemcpyAsync(out, s->d_res, sizeof(nrf_pcg_result), hipMemcpyDeviceToHost, st));
@@ -3003,7 +3447,10 @@ static bool params_ok(const nrf_solver_params *p) {
          p->true_residual_rounds >= 0 && p->true_residual_rounds <= 8 &&
          (p->true_residual_rounds == 0 ||
           (p->relative_tolerance == 1 && p->refine_iterations == 0 && p->refine_jacobi == 0)) &&
-         (p->psd_dropped_pairs == 0 || p->psd_dropped_pairs == 1);
+         (p->psd_dropped_pairs == 0 || p->psd_dropped_pairs == 1) &&
+         (p->plain_form == 0 ||
+          (p->plain_form == 1 && p->jtj_precision == 0 && p->refine_iterations == 0 && p->refine_jacobi == 0 &&
+           p->psd_dropped_pairs == 0));
 }
 
 extern "C" nrf_solver *nrf_solver_create(const nrf_solver_params *params, int max_nodes) {
@@ -3059,9 +3506,10 @@ extern "C" void nrf_solver_destroy(nrf_solver *s) {
                   s->halo_off, s->halo,
                   s->m_cnt, s->m_off, s->m_cur, s->m_keys, s->m_keys2, s->stop, s->d_res, s->d_tot,
                   s->crec, s->arec, s->rres, s->dref, s->rpart, s->rnorm, s->chunk_off, s->chunk_row, s->part,
-                  s->tr};
+                  s->tr, s->pl_r, s->pl_z, s->pl_p, s->pl_Ap, s->pl_part, s->pl_out};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  if (s->pl_host) (void)hipHostFree(s->pl_host);
   delete s;
 }
 
@@ -3086,6 +3534,16 @@ extern "C" int nrf_solver_solve(nrf_solver *s, const nrf_corr *corrs, int num_co
   hipStream_t st = as_stream(stream);
   int rc;
   if ((rc = solver_ensure_pattern(s, nodes, num_nodes, st))) return rc;
+  if (s->params.plain_form) {
+    if ((rc = plain_check_stream(st))) return rc;
+    gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res, s->d_tot);
+    NRF_LAUNCH_CHECK();
+    if ((rc = plain_assemble(s, corrs, num_corrs, nullptr, nodes, reinterpret_cast<const DQ *>(transforms), nullptr,
+                             num_nodes, st)))
+      return rc;
+    if ((rc = plain_solve(s, delta_x, num_nodes, st))) return rc;
+    return solver_fetch_result(s, result, st);
+  }
   if ((rc = solver_build_memberships(s, corrs, num_corrs, num_nodes, st))) return rc;
   gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res, s->d_tot);
   NRF_LAUNCH_CHECK();
@@ -3105,7 +3563,9 @@ int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical
   if (!nodes || !transforms || !centered || !delta_x || (num_corrs > 0 && !corrs)) return NRF_ERR_INVALID_ARGUMENT;
   if (s->P.N != num_nodes) return NRF_ERR_NOT_READY;
   int rc;
-  if ((rc = solver_build_memberships(s, corrs, num_corrs, num_nodes, st))) return rc;
+  const bool plain = s->params.plain_form != 0;
+  if (plain && (rc = plain_check_stream(st))) return rc;
+  if (!plain && (rc = solver_build_memberships(s, corrs, num_corrs, num_nodes, st))) return rc;
   if (begin) {
     gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res, s->d_tot);
     NRF_LAUNCH_CHECK();
@@ -3125,6 +3585,28 @@ int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical
   g.stop = s->stop;
   g.res = s->d_res;
   for (int it = 0; it < std::max(1, gn->gn_iterations); ++it) {
+    if (plain) {  // the reference's structure: atomics, host-driven PCG
+      if ((rc = plain_assemble(s, corrs, num_corrs, canonical, nodes, reinterpret_cast<const DQ *>(transforms),
+                               reinterpret_cast<const DQ *>(centered), num_nodes, st)))
+        return rc;
+      sink_record(s, st);
+      if ((rc = plain_solve(s, delta_x, num_nodes, st))) return rc;
+      sink_record(s, st);
+      if (g.validate) {
+        gn_validate_kernel<<<1, 1024, 0, st>>>(g);
+        NRF_LAUNCH_CHECK();
+      }
+      gn_update_kernel<<<(num_nodes + 255) / 256, 256, 0, st>>>(g);
+      NRF_LAUNCH_CHECK();
+      sink_record(s, st);
+      if (g.validate) {  // the caller's loop ends on a rejected step (host read, as the reference's)
+        int32_t stop = 0;
+        NRF_HIP_CHECK(hipMemcpyAsync(&stop, s->stop, sizeof(stop), hipMemcpyDeviceToHost, st));
+        NRF_HIP_CHECK(hipStreamSynchronize(st));
+        if (stop) break;
+      }
+      continue;
+    }
     if ((rc = solver_assemble(s, corrs, num_corrs, canonical, nodes, reinterpret_cast<const DQ *>(transforms),
                               reinterpret_cast<const DQ *>(centered), num_nodes, st, true)))
       return rc;
@@ -3358,7 +3840,8 @@ int solver_batch_create(const SolverBatchInstance *inst, int count, int lanes_pe
     // precision and GN iteration count, no refinement step
     const nrf_solver_params &sp = s->params;
     const int ns = asm_strips(s->P.Rmax);
-    if (!sp.pcg_pipelined || sp.refine_iterations != 0 || sp.refine_jacobi != 0 || !s->asm_mfma || ns > 8 ||
+    if (!sp.pcg_pipelined || sp.plain_form || sp.refine_iterations != 0 || sp.refine_jacobi != 0 || !s->asm_mfma ||
+        ns > 8 ||
         asm_lds_bytes(s->P.Rmax) > 64 * 1024 ||
         (k > 0 && (sp.jtj_precision != b->prec || std::max(1, q.gn.gn_iterations) != b->gn_iters ||
                    sp.pcg_pipelined - 1 != b->form || sp.true_residual_rounds != b->tr_rounds))) {

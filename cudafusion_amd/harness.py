@@ -65,7 +65,7 @@ class Harness:
                  arap_consistent_sign: bool = False, validate: bool = True, pcg_pipelined: int = 0,
                  jtj_precision: int = 0, refine_iterations: int = 0, storage16: bool = False,
                  refine_jacobi: bool = False, true_residual_rounds: int = 0, psd_dropped_pairs: bool = False,
-                 workload: S.Workload | None = None):
+                 plain_form: bool = False, workload: S.Workload | None = None):
         self.kind = kind
         self.scene = S.scene_for(workload) if workload is not None else S.make_scene(kind, frames, seed)
         self.w = self.scene.w
@@ -77,7 +77,7 @@ class Harness:
                                   w.lambda_damping, 0, int(relative_tolerance), int(precond_fp64),
                                   int(arap_consistent_sign), int(pcg_pipelined), int(jtj_precision),
                                   int(refine_iterations), int(refine_jacobi), int(true_residual_rounds),
-                                  int(psd_dropped_pairs))
+                                  int(psd_dropped_pairs), int(plain_form))
         self.gn = L.gn_params(w.gn_iterations, w.update_scale, w.max_update_rot, w.max_update_trans,
                               int(validate), w.max_dx_mean)
         self.use_graph = use_graph

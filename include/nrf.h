@@ -155,6 +155,15 @@ typedef struct nrf_solver_params {
                                   is then the full J^T J plus the sum over dropped
                                   pairs of (J_k - J_l)^T (J_k - J_l): positive
                                   semidefinite before the damping */
+  int32_t plain_form;          /* 1: the reference's own GPU structure of the solve
+                                  (kernels/solver_kernels.cu:543-741): float atomics
+                                  into the BSR blocks, one kernel per operation and a
+                                  host loop around the PCG that reads every dot
+                                  product back (FP64 sums); the baseline the other
+                                  forms are measured against.  pcg_pipelined is
+                                  ignored; needs jtj_precision, refine_* and
+                                  psd_dropped_pairs 0; not graph-capturable
+                                  (NRF_ERR_INVALID_ARGUMENT under capture) */
 } nrf_solver_params;
 
 /* Caller-side GN loop settings (include/fusion_app.h:72-119 names). */

@@ -104,6 +104,11 @@ def test_host_checks_without_device():
     assert not lib.nrf_solver_create(C.byref(sp), 100)
     sp = _lib.solver_params(5, 30, pcg_pipelined=2, refine_iterations=32768)
     assert not lib.nrf_solver_create(C.byref(sp), 100)
+    # the plain form (the reference's structure) exists in FP32 without
+    # refinement or kept pairs only
+    for kw in (dict(jtj_precision=1), dict(refine_jacobi=1), dict(psd_dropped_pairs=1), dict(plain_form=2)):
+        sp = _lib.solver_params(5, 30, **dict(dict(plain_form=1), **kw))
+        assert not lib.nrf_solver_create(C.byref(sp), 100), kw
     # the workspace-free marching cubes refuses a volume beyond its scratch
     # (dims[1] * dims[2] rows) before any device call
     vp = _lib.volume_params((8, 600, 600), 0.01, 0.03, (0, 0, 0))
