@@ -763,19 +763,20 @@ __device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_
       Ab[lane * 8 + 7] = 0.f;
     }
   }
-  int keys[kKeys];
-#pragma unroll
-  for (int k = 0; k < kKeys; ++k) keys[k] = 0;
+  // four named registers (an array here ended up in scratch memory, whose
+  // loads wait for every record load in flight)
+  static_assert(kKeys == 4, "the chunk's keys live in four registers");
+  int k0 = 0, k1 = 0, k2 = 0, k3 = 0;
   if (mb < me) {  // indices clamped into [mb, me): the loads go out together
-#pragma unroll
-    for (int k = 0; k < kKeys; ++k) keys[k] = a.m_keys[min(mb + kBat * k + m, me - 1)];
+    k0 = a.m_keys[min(mb + m, me - 1)];
+    k1 = a.m_keys[min(mb + kBat + m, me - 1)];
+    k2 = a.m_keys[min(mb + 2 * kBat + m, me - 1)];
+    k3 = a.m_keys[min(mb + 3 * kBat + m, me - 1)];
   }
-  auto key_of = [&](int bi, int e) -> int {
-    if (bi >= kKeys) return a.m_keys[e];
-    int key = keys[0];
-#pragma unroll
-    for (int k = 1; k < kKeys; ++k) key = bi == k ? keys[k] : key;
-    return key;
+  const int32_t *m_keys = a.m_keys;
+  auto key_of = [k0, k1, k2, k3, m_keys](int bi, int e) -> int {
+    if (bi >= kKeys) return m_keys[e];
+    return bi == 0 ? k0 : bi == 1 ? k1 : bi == 2 ? k2 : k3;
   };
   // Every loaded word is used, including the two zero padding words (30 and
   // 31): a load in flight whose register looked dead would be waited for as
@@ -838,8 +839,12 @@ __device__ void assemble_row_mfma(const AsmArgs &a, int i, float *lds, float *s_
       const int mtot = dropped + __shfl_xor(dropped, 32, 64);
       if ((kslot >> 1) == h) {
         const float sc = 1.f + (float)mtot;
+        const bool s1 = (kslot & 1) != 0;  // static indices: Js stays in registers
 #pragma unroll
-        for (int c = 0; c < 6; ++c) Js[kslot & 1][c] *= sc;
+        for (int c = 0; c < 6; ++c) {
+          Js[0][c] = s1 ? Js[0][c] : Js[0][c] * sc;
+          Js[1][c] = s1 ? Js[1][c] * sc : Js[1][c];
+        }
       }
     }
 #pragma unroll
@@ -2232,12 +2237,35 @@ __global__ __launch_bounds__(256) void gn_refine_update_kernel(GnArgs a, RefineA
   const int lane = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= a.N) return;  // the whole wave
+  // the loads that do not depend on the residual go out first, so that they
+  // travel with the row's block loads: lane q < 36 holds M^-1 entry
+  // (q / 6, q % 6), lanes 0-5 the PCG's solution, lane 0 the transform and
+  // the node position of the update
+  const float mv = lane < 36 ? r.Minv[(size_t)i * 36 + lane] : 0.f;
+  const float dl = lane < 6 ? r.d[6 * i + lane] : 0.f;
+  DQ T0{};
+  float3 xp = f3(0.f, 0.f, 0.f);
+  if (lane == 0) {
+    T0 = a.T[i];
+    xp = f3(a.nodes[i].pos);
+  }
   const double res = refine_row_residual(r.row_ptr, r.col_idx, r.values, r.b, r.d, i, lane);
-  const float x = refine_row_step(r.Minv, r.d, res, i, lane);
+  // x + M^-1 res for lanes 0-5 (refine_row_step's products, in its order)
+  double z = 0.0;
+  for (int c = 0; c < 6; ++c) {
+    const double rc = __shfl(res, c, 64);
+    const double m = (double)__shfl(mv, (lane * 6 + c) & 63, 64);
+    if (lane < 6) z += m * rc;
+  }
+  const float x = lane < 6 ? (float)((double)dl + z) : 0.f;
   if (lane < 6) r.dx_out[6 * i + lane] = x;
   float xs[6];
   for (int c = 0; c < 6; ++c) xs[c] = __shfl(x, c, 64);
-  if (lane == 0) gn_update_node(a, i, xs);
+  if (lane == 0) {
+    const DQ Tn = scaled_twist_update(xs, a.scale, a.max_rot, a.max_trans, T0);
+    a.T[i] = Tn;
+    if (a.centered) a.centered[i] = dq_centered(Tn, xp);
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) a.res->gn_done += 1;
 }
 
