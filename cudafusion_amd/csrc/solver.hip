@@ -1177,6 +1177,9 @@ struct PcgArgs {
   const int32_t *stop;
   TrState *tr;  // true-residual rounds (null: off)
   int round;    // 0: the run's PCG; > 0: a correction round (A d = r, to tr->tol_abs)
+  // 1: the pipelined form ends with one refinement step, x += M^-1 (b - A x)
+  // with the residual in FP64 (x's halo through zpub); 2 and 4 lanes per row
+  int refine;
 };
 
 // A correction round of a run that already has its outcome does nothing.
@@ -1786,6 +1789,28 @@ __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, 
       w -= alpha * zv;
     }
     if (aborted) flags |= 8;
+    if constexpr (WN) {
+      if (a.refine && !aborted) {
+        // One refinement step inside the launch (the precision form's):
+        // x's halo travels as published words like m's; r = b - A x sums
+        // exact FP64 products of the FP32 matrix and x; then x += M^-1 r with
+        // the node's six residuals from the lanes of its rows (one wave).
+        // Every read of ph by the last product ended at the collection's
+        // barrier, so the own entries can take x now.
+        ++pt;
+        if (lead) {
+          ll_put(a.zpub + s, base + pt, __float_as_uint(x));
+          ph[zo] = x;
+        }
+        const double res = (double)bv - product(base + pt, a.zpub, 0.0);
+        const int l0 = (t & 63) - lr;  // the lane of row 0 of this lane's node
+        double z = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) z += (double)M[c] * __shfl(res, l0 + c * SUB, 64);
+        x = (float)((double)x + z);
+        if (s_bad) flags |= 8;
+      }
+    }
     if (lead) a.x[s] = x;
     if (g == 0 && t == 0) {
       pcg_tally(a, iters, flags, rr, bb);
@@ -3171,7 +3196,7 @@ static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, int n, const fl
 }
 
 static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxit, bool record, hipStream_t st,
-                      bool opened = false, int round = 0);
+                      bool opened = false, int round = 0, bool refine = false);
 static int solver_tr_rounds(nrf_solver *s, float *x, int N, hipStream_t st);
 
 // opened: the launch was opened by the assembly (solver_assemble's open_pcg)
@@ -3231,6 +3256,7 @@ static PcgArgs solver_pcg_args(const nrf_solver *s, const float *rhs, float *x, 
   a.res = s->d_res;
   a.stop = s->stop;
   a.tr = s->params.true_residual_rounds > 0 ? s->tr : nullptr;
+  a.refine = 0;
   a.round = 0;
   return a;
 }
@@ -3239,7 +3265,7 @@ static PcgArgs solver_pcg_args(const nrf_solver *s, const float *rhs, float *x, 
 // graph: its grid is sized to be co-resident on the device (at most the CU
 // count, or the occupancy bound for batches), and every wait in it is bounded.
 static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxit, bool record, hipStream_t st,
-                      bool opened, int round) {
+                      bool opened, int round, bool refine) {
   if (!opened) {
     pcg_prep_kernel<<<1, 64, 0, st>>>(s->ctl);
     NRF_LAUNCH_CHECK();
@@ -3247,6 +3273,7 @@ static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxi
   if (record) sink_record(s, st);
   PcgArgs a = solver_pcg_args(s, rhs, x, N, maxit);
   a.round = round;
+  a.refine = refine ? 1 : 0;
   void *args[] = {&a};
   const bool pipe = s->params.pcg_pipelined != 0;
   if (!pipe && s->pcg_sub != kPcgSub) return NRF_ERR_INVALID_ARGUMENT;  // textbook form: 4 lanes per row only
@@ -3639,6 +3666,21 @@ int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical
                               reinterpret_cast<const DQ *>(centered), num_nodes, st, true)))
       return rc;
     sink_record(s, st);
+    if (s->params.refine_jacobi && s->params.pcg_pipelined == 3 && s->pcg_sub != 1 &&
+        1 + 2 * (long)s->params.pcg_iterations < kPcgMaxPoints) {
+      // the refinement step at the end of the pipelined PCG launch
+      if ((rc = pcg_launch(s, s->b, delta_x, num_nodes, s->params.pcg_iterations, true, st, true, 0, true)))
+        return rc;
+      sink_record(s, st);
+      if (g.validate) {
+        gn_validate_kernel<<<1, 1024, 0, st>>>(g);
+        NRF_LAUNCH_CHECK();
+      }
+      gn_update_kernel<<<(num_nodes + 255) / 256, 256, 0, st>>>(g);
+      NRF_LAUNCH_CHECK();
+      sink_record(s, st);
+      continue;
+    }
     if (s->params.refine_jacobi && !g.validate) {
       // the refinement step fused into the update (no validation between)
       if ((rc = dev_alloc(s->dref, (size_t)6 * num_nodes, s->c_dref))) return rc;
