@@ -116,34 +116,56 @@ __global__ __launch_bounds__(256) void mem_count_kernel(const nrf_corr *__restri
   mem_count_body(corrs, n, N, cnt, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-__device__ inline void exclusive_scan_body(const int32_t *__restrict__ cnt, int32_t *__restrict__ off,
-                                           int32_t *__restrict__ cur, int n) {
-  __shared__ int32_t part[1024];
+// The memberships' scan of one frame: off / cur from the per-node counts,
+// and in the same pass the assembly's chunk offsets (max(1, ceil(count /
+// chunk)) chunks per row); the counts are cleared behind the reads, so the
+// next frame's count pass starts from zero without a fill launch.
+constexpr int kAsmChunkMem = kAsmChunk;
+__device__ inline void member_scan_body(int32_t *__restrict__ cnt, int32_t *__restrict__ off,
+                                        int32_t *__restrict__ cur, int32_t *__restrict__ chunk_off, int n) {
+  __shared__ int32_t part[1024], partc[1024];
   const int t = threadIdx.x;
   const int per = (n + 1023) / 1024;
   const int b = t * per, e = min(n, b + per);
-  int s = 0;
-  for (int i = b; i < e; ++i) s += cnt[i];
+  int s = 0, sc = 0;
+  for (int i = b; i < e; ++i) {
+    const int c = cnt[i];
+    s += c;
+    sc += max(1, (c + kAsmChunkMem - 1) / kAsmChunkMem);
+  }
   part[t] = s;
+  partc[t] = sc;
   __syncthreads();
   for (int d = 1; d < 1024; d <<= 1) {
-    int v = (t >= d) ? part[t - d] : 0;
+    const int v = (t >= d) ? part[t - d] : 0, vc = (t >= d) ? partc[t - d] : 0;
     __syncthreads();
     part[t] += v;
+    partc[t] += vc;
     __syncthreads();
   }
-  int run = part[t] - s;
+  int run = part[t] - s, runc = partc[t] - sc;
   for (int i = b; i < e; ++i) {
+    const int c = cnt[i];
     off[i] = run;
-    if (cur) cur[i] = run;
-    run += cnt[i];
+    cur[i] = run;
+    chunk_off[i] = runc;
+    run += c;
+    runc += max(1, (c + kAsmChunkMem - 1) / kAsmChunkMem);
+    cnt[i] = 0;
   }
-  if (t == 1023) off[n] = part[1023];
+  if (t == 1023) {
+    off[n] = part[1023];
+    chunk_off[n] = partc[1023];
+  }
 }
-__global__ __launch_bounds__(1024) void exclusive_scan_kernel(const int32_t *__restrict__ cnt,
-                                                              int32_t *__restrict__ off,
-                                                              int32_t *__restrict__ cur, int n) {
-  exclusive_scan_body(cnt, off, cur, n);
+__device__ inline void gn_begin_body(int32_t *stop, nrf_pcg_result *res, nrf_solver_totals *tot);
+// stop / res / tot (optional): the solve's GN begin (gn_begin_body) by one lane.
+__global__ __launch_bounds__(1024) void member_scan_kernel(int32_t *__restrict__ cnt, int32_t *__restrict__ off,
+                                                           int32_t *__restrict__ cur, int32_t *__restrict__ chunk_off,
+                                                           int n, int32_t *stop, nrf_pcg_result *res,
+                                                           nrf_solver_totals *tot) {
+  if (stop && threadIdx.x == 0) gn_begin_body(stop, res, tot);
+  member_scan_body(cnt, off, cur, chunk_off, n);
 }
 
 __device__ inline void mem_scatter_body(const nrf_corr *__restrict__ corrs, int n, int N,
@@ -180,10 +202,14 @@ __global__ __launch_bounds__(256) void mem_scatter_kernel(const nrf_corr *__rest
 // fall back to ranking every key against the whole bucket.
 constexpr int kSortLds = 2048;
 constexpr int kSortThreads = 256;
+// chunk_off (optional): the row's chunks get chunk_row[c] = i.
 __device__ void mem_sort_body(const int32_t *__restrict__ off, const int32_t *__restrict__ keys_in,
-                              int32_t *__restrict__ keys_out, int i) {
+                              int32_t *__restrict__ keys_out, int i, const int32_t *__restrict__ chunk_off = nullptr,
+                              int32_t *__restrict__ chunk_row = nullptr) {
   __shared__ int32_t sk[kSortLds];
   const int t = threadIdx.x;
+  if (chunk_off)
+    for (int c = chunk_off[i] + t; c < chunk_off[i + 1]; c += kSortThreads) chunk_row[c] = i;
   const int b = off[i], n = off[i + 1] - b;
   if (n <= 0) return;  // uniform in the workgroup
   if (n > kSortLds) {
@@ -218,8 +244,10 @@ __device__ void mem_sort_body(const int32_t *__restrict__ off, const int32_t *__
 }
 __global__ __launch_bounds__(kSortThreads) void mem_sort_kernel(const int32_t *__restrict__ off,
                                                                 const int32_t *__restrict__ keys_in,
-                                                                int32_t *__restrict__ keys_out) {
-  mem_sort_body(off, keys_in, keys_out, (int)blockIdx.x);
+                                                                int32_t *__restrict__ keys_out,
+                                                                const int32_t *__restrict__ chunk_off,
+                                                                int32_t *__restrict__ chunk_row) {
+  mem_sort_body(off, keys_in, keys_out, (int)blockIdx.x, chunk_off, chunk_row);
 }
 
 // ------------------------------------------------------------ assembly
@@ -1001,21 +1029,6 @@ __global__ __launch_bounds__(64) void assemble_rows_tail_kernel(AsmArgs a) {
   assemble_tail<NS, PREC>(a, (int)blockIdx.x, lds, s_rhs);
 }
 
-// Chunk tables of the memberships (once per frame): cnt[i] = chunks of row
-// i, chunk_off = exclusive scan (exclusive_scan_kernel), chunk_row[c] = i.
-__device__ inline void chunk_count_body(const int32_t *__restrict__ m_off, int N, int32_t *__restrict__ cnt, int i) {
-  if (i < N) cnt[i] = max(1, (m_off[i + 1] - m_off[i] + kChunk - 1) / kChunk);
-}
-__device__ inline void chunk_fill_body(const int32_t *__restrict__ off, int N, int32_t *__restrict__ row, int i) {
-  if (i < N)
-    for (int c = off[i]; c < off[i + 1]; ++c) row[c] = i;
-}
-__global__ void chunk_count_kernel(const int32_t *__restrict__ m_off, int N, int32_t *__restrict__ cnt) {
-  chunk_count_body(m_off, N, cnt, blockIdx.x * blockDim.x + threadIdx.x);
-}
-__global__ void chunk_fill_kernel(const int32_t *__restrict__ off, int N, int32_t *__restrict__ row) {
-  chunk_fill_body(off, N, row, blockIdx.x * blockDim.x + threadIdx.x);
-}
 
 // ----------------------------------------------------------------- PCG
 // Cross-workgroup traffic of the persistent PCG uses a flag-in-data
@@ -2601,9 +2614,9 @@ __global__ __launch_bounds__(256) void mem_count_batch_kernel(const InstDev *__r
   if ((int)blockIdx.x * 256 >= n) return;  // whole workgroup: every lane reaches the barriers
   mem_count_body(fr.corrs[k], n, inst[k].as.N, inst[k].m_cnt, blockIdx.x * 256 + threadIdx.x);
 }
-__global__ __launch_bounds__(1024) void exclusive_scan_batch_kernel(const InstDev *__restrict__ inst) {
+__global__ __launch_bounds__(1024) void member_scan_batch_kernel(const InstDev *__restrict__ inst) {
   const InstDev &d = inst[blockIdx.x];
-  exclusive_scan_body(d.m_cnt, d.m_off, d.m_cur, d.as.N);
+  member_scan_body(d.m_cnt, d.m_off, d.m_cur, const_cast<int32_t *>(d.as.chunk_off), d.as.N);
 }
 __global__ __launch_bounds__(256) void mem_scatter_batch_kernel(const InstDev *__restrict__ inst, BatchCorrs fr) {
   const int k = blockIdx.y, n = fr.n[k];
@@ -2614,7 +2627,8 @@ __global__ __launch_bounds__(kSortThreads) void mem_sort_batch_kernel(const Inst
                                                                       const int32_t *__restrict__ row_inst,
                                                                       const int32_t *__restrict__ row_local) {
   const InstDev &d = inst[row_inst[blockIdx.x]];
-  mem_sort_body(d.m_off, d.m_keys, const_cast<int32_t *>(d.as.m_keys), row_local[blockIdx.x]);
+  mem_sort_body(d.m_off, d.m_keys, const_cast<int32_t *>(d.as.m_keys), row_local[blockIdx.x], d.as.chunk_off,
+                const_cast<int32_t *>(d.as.chunk_row));
 }
 __global__ void gn_begin_batch_kernel(const InstDev *__restrict__ inst, int count) {
   const int k = threadIdx.x;
@@ -2658,18 +2672,6 @@ __global__ __launch_bounds__(64) void assemble_tail_batch_kernel(const InstDev *
   a.corrs = fr.corrs[k];
   a.canonical = fr.canonical[k];
   assemble_tail<NS, PREC>(a, row_local[blockIdx.x], lds, s_rhs);
-}
-__global__ void chunk_count_batch_kernel(const InstDev *__restrict__ inst) {
-  const InstDev &d = inst[blockIdx.y];
-  chunk_count_body(d.m_off, d.as.N, d.m_cnt, blockIdx.x * blockDim.x + threadIdx.x);
-}
-__global__ __launch_bounds__(1024) void chunk_scan_batch_kernel(const InstDev *__restrict__ inst) {
-  const InstDev &d = inst[blockIdx.x];
-  exclusive_scan_body(d.m_cnt, const_cast<int32_t *>(d.as.chunk_off), nullptr, d.as.N);
-}
-__global__ void chunk_fill_batch_kernel(const InstDev *__restrict__ inst) {
-  const InstDev &d = inst[blockIdx.y];
-  chunk_fill_body(d.as.chunk_off, d.as.N, const_cast<int32_t *>(d.as.chunk_row), blockIdx.x * blockDim.x + threadIdx.x);
 }
 __global__ void pcg_prep_batch_kernel(const InstDev *__restrict__ inst, int count) {
   const int k = threadIdx.x;
@@ -2891,6 +2893,8 @@ struct nrf_solver {
   // memberships
   int32_t *m_cnt = nullptr, *m_off = nullptr, *m_cur = nullptr, *m_keys = nullptr, *m_keys2 = nullptr;
   size_t c_m_cnt = 0, c_m_off = 0, c_m_cur = 0, c_m_keys = 0, c_m_keys2 = 0;
+  bool m_cnt_zero = false;  // m_cnt holds zeros for its first m_cnt_n entries (cleared by the last scan)
+  int m_cnt_n = 0;
   // GN state
   int32_t *stop = nullptr;
   size_t c_stop = 0;
@@ -3045,35 +3049,42 @@ static int solver_build_partition(nrf_solver *s) {
   return NRF_OK;
 }
 
-static int solver_build_memberships(nrf_solver *s, const nrf_corr *corrs, int n, int N, hipStream_t st) {
+// begin: the scan kernel also opens the solve (gn_begin_kernel's work).
+static int solver_build_memberships(nrf_solver *s, const nrf_corr *corrs, int n, int N, hipStream_t st,
+                                    bool begin = false) {
   int rc;
+  const size_t cap0 = s->c_m_cnt;
   if ((rc = dev_alloc(s->m_cnt, (size_t)N, s->c_m_cnt))) return rc;
+  if (s->c_m_cnt != cap0) s->m_cnt_zero = false;  // reallocated: not cleared yet
   if ((rc = dev_alloc(s->m_off, (size_t)N + 1, s->c_m_off))) return rc;
   if ((rc = dev_alloc(s->m_cur, (size_t)N, s->c_m_cur))) return rc;
   if ((rc = dev_alloc(s->m_keys, (size_t)n * 4, s->c_m_keys))) return rc;
   if ((rc = dev_alloc(s->m_keys2, (size_t)n * 4, s->c_m_keys2))) return rc;
-  fill_i32<<<(N + 255) / 256, 256, 0, st>>>(s->m_cnt, N, 0);
-  NRF_LAUNCH_CHECK();
+  if ((rc = dev_alloc(s->chunk_off, (size_t)N + 1, s->c_chunk_off))) return rc;
+  if ((rc = dev_alloc(s->chunk_row, asm_max_chunks(N, n), s->c_chunk_row))) return rc;
+  // the counts are zero from the previous frame's scan, which clears them
+  // behind its reads (every count of the buffer is read and cleared, since
+  // the scan covers [0, N) and N never exceeds the allocation)
+  if (!s->m_cnt_zero || N > s->m_cnt_n) {
+    fill_i32<<<((int)s->c_m_cnt + 255) / 256, 256, 0, st>>>(s->m_cnt, (int)s->c_m_cnt, 0);
+    NRF_LAUNCH_CHECK();
+  }
+  s->m_cnt_zero = true;
+  s->m_cnt_n = N;
   if (n > 0) {
     mem_count_kernel<<<(n + 255) / 256, 256, 0, st>>>(corrs, n, N, s->m_cnt);
     NRF_LAUNCH_CHECK();
   }
-  exclusive_scan_kernel<<<1, 1024, 0, st>>>(s->m_cnt, s->m_off, s->m_cur, N);
+  // offsets, cursors and the MFMA assembly's chunk offsets in one pass
+  member_scan_kernel<<<1, 1024, 0, st>>>(s->m_cnt, s->m_off, s->m_cur, s->chunk_off, N, begin ? s->stop : nullptr,
+                                         s->d_res, s->d_tot);
   NRF_LAUNCH_CHECK();
   if (n > 0) {
     mem_scatter_kernel<<<(n + 255) / 256, 256, 0, st>>>(corrs, n, N, s->m_cur, s->m_keys);
     NRF_LAUNCH_CHECK();
   }
-  mem_sort_kernel<<<N, kSortThreads, 0, st>>>(s->m_off, s->m_keys, s->m_keys2);
-  NRF_LAUNCH_CHECK();
-  // chunk tables of the MFMA assembly (m_cnt is free again)
-  if ((rc = dev_alloc(s->chunk_off, (size_t)N + 1, s->c_chunk_off))) return rc;
-  if ((rc = dev_alloc(s->chunk_row, asm_max_chunks(N, n), s->c_chunk_row))) return rc;
-  chunk_count_kernel<<<(N + 255) / 256, 256, 0, st>>>(s->m_off, N, s->m_cnt);
-  NRF_LAUNCH_CHECK();
-  exclusive_scan_kernel<<<1, 1024, 0, st>>>(s->m_cnt, s->chunk_off, nullptr, N);
-  NRF_LAUNCH_CHECK();
-  chunk_fill_kernel<<<(N + 255) / 256, 256, 0, st>>>(s->chunk_off, N, s->chunk_row);
+  // each row's bucket ordered by key; the row's chunk entries filled alongside
+  mem_sort_kernel<<<N, kSortThreads, 0, st>>>(s->m_off, s->m_keys, s->m_keys2, s->chunk_off, s->chunk_row);
   NRF_LAUNCH_CHECK();
   return NRF_OK;
 }
@@ -3599,9 +3610,7 @@ extern "C" int nrf_solver_solve(nrf_solver *s, const nrf_corr *corrs, int num_co
     if ((rc = plain_solve(s, delta_x, num_nodes, st))) return rc;
     return solver_fetch_result(s, result, st);
   }
-  if ((rc = solver_build_memberships(s, corrs, num_corrs, num_nodes, st))) return rc;
-  gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res, s->d_tot);
-  NRF_LAUNCH_CHECK();
+  if ((rc = solver_build_memberships(s, corrs, num_corrs, num_nodes, st, true))) return rc;
   if ((rc = solver_assemble(s, corrs, num_corrs, nullptr, nodes, reinterpret_cast<const DQ *>(transforms), nullptr,
                             num_nodes, st, true)))
     return rc;
@@ -3620,8 +3629,8 @@ int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical
   int rc;
   const bool plain = s->params.plain_form != 0;
   if (plain && (rc = plain_check_stream(st))) return rc;
-  if (!plain && (rc = solver_build_memberships(s, corrs, num_corrs, num_nodes, st))) return rc;
-  if (begin) {
+  if (!plain && (rc = solver_build_memberships(s, corrs, num_corrs, num_nodes, st, begin))) return rc;
+  if (begin && plain) {
     gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res, s->d_tot);
     NRF_LAUNCH_CHECK();
   }
@@ -4025,19 +4034,15 @@ static int batch_memberships(SolverBatch *b, const BatchCorrs &fr, hipStream_t s
     mem_count_batch_kernel<<<gC, 256, 0, st>>>(b->d_inst, fr);
     NRF_LAUNCH_CHECK();
   }
-  exclusive_scan_batch_kernel<<<b->count, 1024, 0, st>>>(b->d_inst);
+  // offsets, cursors and chunk offsets in one pass (the counts are cleared
+  // behind the reads, as in the single-instance build)
+  member_scan_batch_kernel<<<b->count, 1024, 0, st>>>(b->d_inst);
   NRF_LAUNCH_CHECK();
   if (fr.max_n > 0) {
     mem_scatter_batch_kernel<<<gC, 256, 0, st>>>(b->d_inst, fr);
     NRF_LAUNCH_CHECK();
   }
   mem_sort_batch_kernel<<<b->rows, kSortThreads, 0, st>>>(b->d_inst, b->row_inst, b->row_local);
-  NRF_LAUNCH_CHECK();
-  chunk_count_batch_kernel<<<gN, 256, 0, st>>>(b->d_inst);
-  NRF_LAUNCH_CHECK();
-  chunk_scan_batch_kernel<<<b->count, 1024, 0, st>>>(b->d_inst);
-  NRF_LAUNCH_CHECK();
-  chunk_fill_batch_kernel<<<gN, 256, 0, st>>>(b->d_inst);
   NRF_LAUNCH_CHECK();
   gn_begin_batch_kernel<<<1, 64, 0, st>>>(b->d_inst, b->count);
   NRF_LAUNCH_CHECK();
