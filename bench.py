@@ -273,6 +273,9 @@ def main() -> int:
                          "plain FP32 against the precision form; 0 skips it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="run each step's integration and marching cubes on the caller's stream instead of "
+                         "overlapping them with the next step's memberships and first assembly")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="before the timed steps, run one elementwise copy of MIB MiB (x * 1 -> y): a kernel of "
                          "known HBM traffic that calibrates hardware byte counters")
@@ -338,6 +341,9 @@ def main() -> int:
     if args.pcg_layout:
         lanes, wg_nodes = (int(v) for v in args.pcg_layout.split(","))
         L.check(h.lib.nrf_solver_set_pcg_layout(h.lib.nrf_pipeline_solver(h.pipe), lanes, wg_nodes), "pcg_layout")
+    overlap = not args.no_overlap and not args.graph
+    if overlap:  # the frame's fusion overlaps the next frame's memberships and first assembly
+        L.check(h.lib.nrf_pipeline_set_overlap(h.pipe, 1), "overlap")
     log(f"[bench] setup {time.perf_counter() - t0:.1f} s, {h.N} nodes, "
         f"{int(h.frames_host[1].valid.sum())} valid pixels in frame 1")
     nf = args.frames
@@ -447,6 +453,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "capsule-body 256^3 @4mm, 640x480, %d nodes, 5 GN x 30 PCG" % h.N,
                    "graph": bool(args.graph),
+                   "overlapped_fusion": overlap,
                    "graph_captures": int(lib.nrf_pipeline_graph_captures(h.pipe)) if args.graph else None,
                    "solver_policy": ("reference (ARAP sign, whole-step rejection)" if ref else
                                      "consistent ARAP sign, per-node clamp, no whole-step rejection"),

@@ -192,6 +192,8 @@ _SIGS = {
     "nrf_pipeline_profile_read": (I, [P, I, C.POINTER(F)]),
     "nrf_pipeline_updated_voxels": (C.c_longlong, [P]),
     "nrf_pipeline_graph_captures": (C.c_longlong, [P]),
+    "nrf_pipeline_set_overlap": (C.c_int, [P, C.c_int]),
+    "nrf_pipeline_join": (C.c_int, [P, P]),
     "nrf_raycast": (I, [C.POINTER(VolumeParams), P, C.POINTER(Camera), P, P, P, I, P, P, P, P, P, P]),
     "nrf_graph_add_nodes": (I, [P, P, C.POINTER(I), I, P, P, I, C.c_float, C.c_float, C.POINTER(I)]),
     "nrf_graph_add_nodes_geodesic": (I, [P, P, C.POINTER(I), I, P, P, I, P, I, C.c_float, C.c_float, C.POINTER(I),

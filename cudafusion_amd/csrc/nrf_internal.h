@@ -44,6 +44,11 @@ int device_cu_count();
 // before the PCG kernel, after the PCG kernel and after the update, i.e.
 // 1 + 4 * gn_iterations events.
 void solver_set_event_sink(nrf_solver *s, hipEvent_t *events, int count, int skip = 0);
+// The next nrf_solver_solve_gn makes its stream wait for `ev` before its
+// first validation or twist update, the first of its kernels that writes the
+// node transforms or their centred table (null: no wait).  One solve
+// consumes it.
+void solver_set_pcg_wait(nrf_solver *s, hipEvent_t ev);
 int device_max_lds_per_block();
 
 // ---------------------------------------------------------------- batches

@@ -65,6 +65,13 @@ struct nrf_pipeline {
   float *g_canon = nullptr;
   CamXf *g_pose = nullptr;
   hipStream_t cap = nullptr;  // capture stream (the caller's may be the null stream)
+  // overlapped fusion (nrf_pipeline_set_overlap): a step's integration and
+  // marching cubes run on `side`, after its solve and alongside the next
+  // step's memberships, first assembly and first PCG; ev_fused marks their
+  // end
+  bool overlap = false, side_pending = false;
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fused = nullptr;
   bool graph_step = false;    // the last step was a graph replay (stage events: whole step only)
   bool mesh_valid = false;    // verts / tris / counts hold the marching cubes of the current volume
   // tracking frame buffers (allocated by the first nrf_pipeline_track)
@@ -94,8 +101,24 @@ static void pipeline_free(nrf_pipeline *p) {
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->cap) (void)hipStreamDestroy(p->cap);
+  if (p->side) {
+    (void)hipStreamSynchronize(p->side);
+    (void)hipStreamDestroy(p->side);
+  }
+  if (p->ev_fused) (void)hipEventDestroy(p->ev_fused);
   if (p->solver) nrf_solver_destroy(p->solver);
   delete p;
+}
+
+// Every entry point other than the step itself first waits for an
+// overlapped step's integration and marching cubes (host-synchronous; the
+// step's callers that want to stay stream-ordered use nrf_pipeline_join).
+static int pipeline_drain(nrf_pipeline *p) {
+  if (p->side_pending) {
+    NRF_HIP_CHECK(hipStreamSynchronize(p->side));
+    p->side_pending = false;
+  }
+  return NRF_OK;
 }
 
 extern "C" nrf_pipeline *nrf_pipeline_create(const nrf_pipeline_config *cfg) {
@@ -180,6 +203,7 @@ extern "C" int nrf_pipeline_volume16(nrf_pipeline *p, uint16_t **tsdf16, uint8_t
 extern "C" int nrf_pipeline_integrate_rigid(nrf_pipeline *p, const float *depth, const float camera_pose[16],
                                             void *stream) {
   if (!p || !depth || !camera_pose) return NRF_ERR_INVALID_ARGUMENT;
+  if (int rc0 = pipeline_drain(p)) return rc0;
   p->mesh_valid = false;
   return tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, nullptr, nullptr, 0, nullptr,
                         nullptr, nullptr, nullptr, nullptr, 0, nullptr, as_stream(stream), nullptr, p->occ, p->dblk);
@@ -188,6 +212,7 @@ extern "C" int nrf_pipeline_integrate_rigid(nrf_pipeline *p, const float *depth,
 extern "C" int nrf_pipeline_set_nodes(nrf_pipeline *p, const nrf_node *host_nodes, int num_nodes) {
   if (!p || num_nodes < 0 || num_nodes > p->cfg.max_nodes || (num_nodes > 0 && !host_nodes))
     return NRF_ERR_INVALID_ARGUMENT;
+  if (int rc0 = pipeline_drain(p)) return rc0;
   p->N = num_nodes;
   if (num_nodes == 0) return NRF_OK;
   NRF_HIP_CHECK(hipMemcpy(p->nodes, host_nodes, sizeof(nrf_node) * num_nodes, hipMemcpyHostToDevice));
@@ -243,6 +268,12 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
   hipStream_t st = as_stream(stream);
   int rc;
   hipEvent_t *pe = nullptr;
+  // overlapped fusion: eager steps with stage events only (not captured, not
+  // under the per-kernel event ring, whose integration and marching-cubes
+  // events sit on the caller's stream)
+  const bool ovl = p->overlap && events && !dev_pose && !(profile && !p->prof_pcg);
+  if (!ovl && (rc = pipeline_drain(p))) return rc;
+  if (ovl && p->side_pending) solver_set_pcg_wait(p->solver, p->ev_fused);
   if (profile) {
     pe = &p->prof[(size_t)(p->prof_steps % p->prof_ring) * p->prof_per];
     if (p->prof_pcg) {
@@ -257,8 +288,15 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
   rc = nrf_solver_solve_gn(p->solver, corrs, canonical, num_corrs, p->nodes, p->T, p->centered, p->N, &p->cfg.gn,
                            p->delta_x, nullptr, stream);
   if (profile) solver_set_event_sink(p->solver, nullptr, 0);
+  solver_set_pcg_wait(p->solver, nullptr);  // consumed by the solve (or dropped with an error)
   if (rc) return rc;
   if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[1], st));
+  if (ovl) {
+    // the fusion of this frame on the side stream, after the solve; the next
+    // step's solve waits for it before its first twist update
+    NRF_HIP_CHECK(hipStreamWaitEvent(p->side, p->ev[1], 0));
+    st = p->side;
+  }
   if (p->cfg.storage16)
     rc = tsdf16_integrate(&p->cfg.volume, p->t16, p->w8, depth, &p->cfg.camera, camera_pose, p->centered, p->N,
                           p->k16, p->kw16, p->knn_rows, p->updated, st, nullptr, 0, dev_pose, p->occ, p->dblk);
@@ -271,6 +309,10 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
   if (profile && !p->prof_pcg) NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 2], st));
   if ((rc = pipeline_mc(p, st))) return rc;
   if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[3], st));
+  if (ovl) {
+    NRF_HIP_CHECK(hipEventRecord(p->ev_fused, st));
+    p->side_pending = true;
+  }
   if (profile) {
     if (!p->prof_pcg) NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 1], st));
     ++p->prof_steps;
@@ -287,6 +329,7 @@ extern "C" int nrf_pipeline_step(nrf_pipeline *p, const float *depth, const floa
     p->graph_step = false;
     return pipeline_enqueue(p, depth, camera_pose, corrs, canonical, num_corrs, stream, p->prof_ring > 0);
   }
+  if (int rc0 = pipeline_drain(p)) return rc0;
   // the frame into the captured step's input buffers (stream-ordered, ahead
   // of the replay on the same stream)
   hipStream_t cs = as_stream(stream);
@@ -349,6 +392,23 @@ extern "C" int nrf_pipeline_step(nrf_pipeline *p, const float *depth, const floa
 }
 
 extern "C" long long nrf_pipeline_graph_captures(nrf_pipeline *p) { return p ? p->g_captures : -1; }
+
+extern "C" int nrf_pipeline_set_overlap(nrf_pipeline *p, int enable) {
+  if (!p || (enable != 0 && enable != 1)) return NRF_ERR_INVALID_ARGUMENT;
+  if (int rc = pipeline_drain(p)) return rc;
+  if (enable && !p->side) {
+    NRF_HIP_CHECK(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+    NRF_HIP_CHECK(hipEventCreateWithFlags(&p->ev_fused, hipEventDisableTiming));
+  }
+  p->overlap = enable != 0;
+  return NRF_OK;
+}
+
+extern "C" int nrf_pipeline_join(nrf_pipeline *p, void *stream) {
+  if (!p) return NRF_ERR_INVALID_ARGUMENT;
+  if (p->side_pending) NRF_HIP_CHECK(hipStreamWaitEvent(as_stream(stream), p->ev_fused, 0));
+  return NRF_OK;
+}
 
 extern "C" int nrf_pipeline_stage_ms(nrf_pipeline *p, float out_ms[3]) {
   if (!p || !out_ms) return NRF_ERR_INVALID_ARGUMENT;
@@ -476,6 +536,7 @@ extern "C" int nrf_pipeline_track(nrf_pipeline *p, const float *depth, const flo
   if (G > NRF_TRACK_MAX_GN || p->cfg.camera.width <= 0 || p->cfg.camera.height <= 0)
     return NRF_ERR_INVALID_ARGUMENT;
   int rc;
+  if ((rc = pipeline_drain(p))) return rc;  // the mesh of an overlapped step feeds the prediction
   if (!p->tk.dev && (rc = track_alloc(p))) return rc;
   hipStream_t st = as_stream(stream);
   auto &k = p->tk;
@@ -563,6 +624,7 @@ extern "C" int nrf_pipeline_track_maps(nrf_pipeline *p, float **live_vertices, f
 
 extern "C" long long nrf_pipeline_updated_voxels(nrf_pipeline *p) {
   if (!p) return -1;
+  if (pipeline_drain(p)) return -1;
   int32_t v = 0;
   if (hipMemcpy(&v, p->updated, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess) return -1;
   return v;
@@ -672,8 +734,8 @@ extern "C" nrf_batch *nrf_batch_create(nrf_pipeline *const *pipelines, int count
                                        int use_graph) {
   if (!pipelines || count < 1 || count > kMaxBatch) return nullptr;
   for (int k = 0; k < count; ++k) {
-    const nrf_pipeline *q = pipelines[k];
-    if (!q || q->N <= 0 || q->cfg.storage16) return nullptr;
+    nrf_pipeline *q = pipelines[k];
+    if (!q || q->N <= 0 || q->cfg.storage16 || q->overlap) return nullptr;  // batch members step unoverlapped
     for (int j = 0; j < k; ++j)
       if (pipelines[j] == q) return nullptr;
   }

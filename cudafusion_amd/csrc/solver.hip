@@ -2928,9 +2928,12 @@ struct nrf_solver {
   // optional profiling events (see solver_set_event_sink)
   hipEvent_t *sink = nullptr;
   int sink_n = 0, sink_i = 0, sink_skip = 0;
+  hipEvent_t pcg_wait = nullptr;  // see solver_set_pcg_wait
 };
 
 namespace nrf {
+
+void solver_set_pcg_wait(nrf_solver *s, hipEvent_t ev) { s->pcg_wait = ev; }
 
 void solver_set_event_sink(nrf_solver *s, hipEvent_t *events, int count, int skip) {
   s->sink = events;
@@ -3648,6 +3651,15 @@ int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical
   g.validate = gn->validate;
   g.stop = s->stop;
   g.res = s->d_res;
+  // the caller's pending work that reads the centred table (solver_set_pcg_wait)
+  // ends before the first validation or twist update writes anything it reads
+  auto wait_before_update = [&]() -> int {
+    if (s->pcg_wait) {
+      NRF_HIP_CHECK(hipStreamWaitEvent(st, s->pcg_wait, 0));
+      s->pcg_wait = nullptr;
+    }
+    return NRF_OK;
+  };
   for (int it = 0; it < std::max(1, gn->gn_iterations); ++it) {
     if (plain) {  // the reference's structure: atomics, host-driven PCG
       if ((rc = plain_assemble(s, corrs, num_corrs, canonical, nodes, reinterpret_cast<const DQ *>(transforms),
@@ -3656,6 +3668,7 @@ int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical
       sink_record(s, st);
       if ((rc = plain_solve(s, delta_x, num_nodes, st))) return rc;
       sink_record(s, st);
+      if ((rc = wait_before_update())) return rc;
       if (g.validate) {
         gn_validate_kernel<<<1, 1024, 0, st>>>(g);
         NRF_LAUNCH_CHECK();
@@ -3681,6 +3694,7 @@ int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical
       if ((rc = pcg_launch(s, s->b, delta_x, num_nodes, s->params.pcg_iterations, true, st, true, 0, true)))
         return rc;
       sink_record(s, st);
+      if ((rc = wait_before_update())) return rc;
       if (g.validate) {
         gn_validate_kernel<<<1, 1024, 0, st>>>(g);
         NRF_LAUNCH_CHECK();
@@ -3696,6 +3710,7 @@ int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical
       if ((rc = pcg_launch(s, s->b, s->dref, num_nodes, s->params.pcg_iterations, true, st, true))) return rc;
       sink_record(s, st);
       const RefineArgs ra{s->row_ptr, s->col_idx, s->values, s->b, s->Minv, s->dref, delta_x};
+      if ((rc = wait_before_update())) return rc;
       gn_refine_update_kernel<<<(num_nodes + 3) / 4, 256, 0, st>>>(g, ra);
       NRF_LAUNCH_CHECK();
       sink_record(s, st);
@@ -3703,6 +3718,7 @@ int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical
     }
     if ((rc = solver_pcg(s, delta_x, num_nodes, st, true))) return rc;
     sink_record(s, st);
+    if ((rc = wait_before_update())) return rc;
     if (g.validate) {
       gn_validate_kernel<<<1, 1024, 0, st>>>(g);
       NRF_LAUNCH_CHECK();

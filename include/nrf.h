@@ -612,6 +612,22 @@ int nrf_pipeline_stage_ms(nrf_pipeline *p, float out_ms[3]);
    only when num_corrs, the presence of canonical points or the solver's
    buffers / pattern change. */
 long long nrf_pipeline_graph_captures(nrf_pipeline *p);
+/* No counterpart in the reference (which runs one frame after the other on
+   the legacy stream).  enable = 1: eager steps run their warped integration
+   and marching cubes on a stream of the pipeline's own, after the step's
+   solve, overlapping the next step's membership build, first assembly and
+   first PCG; the next step's solve waits for them before its first twist
+   update (the first write of the node transforms they read), so every frame
+   computes what it computes unoverlapped (tested bit for bit).  The
+   persistent PCG then shares the device with them: its workgroups may start
+   late, which every wait in it tolerates (they are bounded, not timed).  A step's volume and mesh are then complete only after
+   nrf_pipeline_join on the reader's stream, a device synchronisation, or any
+   other pipeline call (which waits for them).  Graph-replayed and per-kernel
+   profiled steps do not overlap.  Synchronises the host when switching. */
+int nrf_pipeline_set_overlap(nrf_pipeline *p, int enable);
+/* Makes `stream` wait for the integration and marching cubes of the last
+   overlapped step (a no-op otherwise). */
+int nrf_pipeline_join(nrf_pipeline *p, void *stream);
 /* Per-kernel profiling: with ring_steps > 0, every step records hipEvents
    around each kernel group on the step's stream (eager launches, no graph)
    into a ring of ring_steps steps; 0 disables.  nrf_pipeline_profile_read
