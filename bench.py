@@ -271,6 +271,9 @@ def main() -> int:
     ap.add_argument("--equal-residual-frames", type=int, default=10,
                     help="frames of the equal-residual comparison (full step, PCG to a 1e-6 true residual, "
                          "plain FP32 against the precision form; 0 skips it)")
+    ap.add_argument("--pcg-sample-every", type=int, default=3, metavar="K",
+                    help="time the PCG launch (HIP events around one launch, GN iterations in turn) on every K-th "
+                         "timed step")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true")
     ap.add_argument("--no-overlap", action="store_true",
@@ -352,10 +355,11 @@ def main() -> int:
     # live timing of the dominant kernel: two events around one PCG launch per
     # step (GN iteration step mod G), which cost the step far less than
     # events around every kernel group (about 3 us of device time each)
-    if not args.graph:
-        L.check(lib.nrf_pipeline_profile_pcg(h.pipe, max(64, args.steps)), "profile")
+    stream_ptr = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     for f in seq[: args.warmup]:
         h.step(f)
+    if overlap:  # the last step's fusion is enqueued by the next step or by a join
+        L.check(lib.nrf_pipeline_join(h.pipe, stream_ptr), "join")
     torch.cuda.synchronize()
     if args.copy_probe > 0:
         n = args.copy_probe * (1 << 20) // 4
@@ -366,11 +370,17 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"[bench] copy probe: {n * 4} bytes read, {n * 4} bytes written")
         del xs, ys
+    if not args.graph:  # the sampled steps' PCG events of the timed steps
+        L.check(lib.nrf_pipeline_profile_pcg(h.pipe, max(64, args.steps)), "profile")
+        L.check(lib.nrf_pipeline_profile_every(h.pipe, args.pcg_sample_every), "profile_every")
     upd0 = lib.nrf_pipeline_updated_voxels(h.pipe)
     tot0 = h.solver_totals()
     t0 = time.perf_counter()
     for f in seq[args.warmup:]:
         h.step(f)
+    if overlap:
+        L.check(lib.nrf_pipeline_join(h.pipe, stream_ptr), "join")
+    t_enq = time.perf_counter() - t0  # host time to enqueue the steps
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     upd = (lib.nrf_pipeline_updated_voxels(h.pipe) - upd0) / args.steps
@@ -387,7 +397,7 @@ def main() -> int:
     kern = None
     if not args.graph:
         pcg_launch = []
-        for k in range(args.steps):
+        for k in range((args.steps + args.pcg_sample_every - 1) // args.pcg_sample_every):
             out = (C.c_float * 6)()
             L.check(lib.nrf_pipeline_profile_read(h.pipe, k, out), "profile_read")
             pcg_launch.append(out[2])
@@ -446,6 +456,7 @@ def main() -> int:
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(elapsed * 1e3 / args.steps, 4),
+        "host_enqueue_ms_per_step": round(t_enq * 1e3 / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

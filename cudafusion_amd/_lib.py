@@ -189,6 +189,7 @@ _SIGS = {
     "nrf_pipeline_stage_ms": (I, [P, C.POINTER(F)]),
     "nrf_pipeline_profile": (I, [P, I]),
     "nrf_pipeline_profile_pcg": (I, [P, I]),
+    "nrf_pipeline_profile_every": (I, [P, I]),
     "nrf_pipeline_profile_read": (I, [P, I, C.POINTER(F)]),
     "nrf_pipeline_updated_voxels": (C.c_longlong, [P]),
     "nrf_pipeline_graph_captures": (C.c_longlong, [P]),

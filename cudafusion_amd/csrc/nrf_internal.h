@@ -49,6 +49,17 @@ void solver_set_event_sink(nrf_solver *s, hipEvent_t *events, int count, int ski
 // node transforms or their centred table (null: no wait).  One solve
 // consumes it.
 void solver_set_pcg_wait(nrf_solver *s, hipEvent_t ev);
+// The next nrf_solver_solve_gn calls fn(ctx, stream) once, on the host,
+// right before it enqueues its first PCG launch: work the callback orders
+// after an event it records on `stream` follows the solve's memberships and
+// first assembly.  A non-zero return ends the solve with that status.  One
+// solve consumes it; it is dropped when the solve ends before a PCG launch.
+typedef int (*solver_hook_fn)(void *ctx, hipStream_t stream);
+void solver_set_pcg_hook(nrf_solver *s, solver_hook_fn fn, void *ctx);
+// The next nrf_solver_solve_gn copies the node transforms and their centred
+// table, as they are when it starts, into T / centered (num_nodes entries
+// each), in its first kernel.  One solve consumes it.
+void solver_set_snapshot(nrf_solver *s, nrf_dualquat *T, nrf_dualquat *centered);
 int device_max_lds_per_block();
 
 // ---------------------------------------------------------------- batches

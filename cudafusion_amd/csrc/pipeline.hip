@@ -51,8 +51,8 @@ struct nrf_pipeline {
   hipGraph_t graph = nullptr;
   // per-kernel profiling ring (eager mode)
   std::vector<hipEvent_t> prof;
-  int prof_ring = 0, prof_per = 0, prof_G = 0, prof_pcg = 0;
-  long prof_steps = 0;
+  int prof_ring = 0, prof_per = 0, prof_G = 0, prof_pcg = 0, prof_every = 1;
+  long prof_steps = 0, prof_calls = 0;  // sampled steps, all steps since the set-up
   int g_n = -1;
   bool g_has_can = false;
   unsigned g_gen = 0;
@@ -65,13 +65,32 @@ struct nrf_pipeline {
   float *g_canon = nullptr;
   CamXf *g_pose = nullptr;
   hipStream_t cap = nullptr;  // capture stream (the caller's may be the null stream)
-  // overlapped fusion (nrf_pipeline_set_overlap): a step's integration and
-  // marching cubes run on `side`, after its solve and alongside the next
-  // step's memberships, first assembly and first PCG; ev_fused marks their
-  // end
-  bool overlap = false, side_pending = false;
+  // overlapped fusion (nrf_pipeline_set_overlap).  Step F's integration and
+  // marching cubes are deferred (fz_pending) and enqueued on `side` at step
+  // F+1's first PCG launch (pipeline_fuse_hook), so that they run alongside
+  // that step's PCG:
+  //  - step F+1's first kernel copies the node transforms as step F left
+  //    them into snapshot slot (F+1) & 1 (solver_set_snapshot), which F's
+  //    fusion reads while F+1 updates the live tables;
+  //  - at step F's first PCG launch `side` copies F's depth into depth slot
+  //    F & 1, after the previous fusion; ev_copy marks the copy, and F's
+  //    stream waits for it before the step returns (the caller may then
+  //    overwrite its depth buffer, and every later write of a slot on the
+  //    step's stream follows the fusion that read it);
+  //  - a fusion that no step takes (a join, a drain) is flushed after ev[1]
+  //    (the end of its step) with its snapshot copied on `side`; the next
+  //    step then waits for it (fz_flushed) before its snapshot.
+  // ev_fs / ev_fused mark the last enqueued fusion's start and end.
+  bool overlap = false, side_pending = false, fz_pending = false, fz_events = false, fz_flushed = false;
+  bool fz_copied = false;  // the step being enqueued made its hook's copy
   hipStream_t side = nullptr;
-  hipEvent_t ev_fused = nullptr;
+  hipEvent_t ev_fused = nullptr, ev_hook = nullptr, ev_copy = nullptr, ev_fs = nullptr;
+  float *fz_depth = nullptr;          // two depth slots
+  nrf_dualquat *T_snap = nullptr, *c_snap = nullptr;  // two snapshot slots of max_nodes entries
+  const float *fz_src = nullptr;      // the depth of the step being enqueued
+  long fz_steps = 0;                  // overlapped steps so far (slot parity)
+  int fz_slot = 0, fz_snap = 0;       // the deferred fusion's depth and snapshot slots
+  float fz_pose[16] = {};
   bool graph_step = false;    // the last step was a graph replay (stage events: whole step only)
   bool mesh_valid = false;    // verts / tris / counts hold the marching cubes of the current volume
   // tracking frame buffers (allocated by the first nrf_pipeline_track)
@@ -91,7 +110,7 @@ static void pipeline_free(nrf_pipeline *p) {
                   p->verts, p->tris, p->counts, p->updated, p->t16, p->w8, p->k16, p->kw16,
                   p->tk.dnorm, p->tk.warped, p->tk.live_v, p->tk.live_n, p->tk.ztri, p->tk.hit, p->tk.opt,
                   p->tk.opt_frame, p->tk.corrs, p->tk.part, p->tk.Tsave, p->tk.dev,
-                  p->g_depth, p->g_corrs, p->g_canon, p->g_pose};
+                  p->g_depth, p->g_corrs, p->g_canon, p->g_pose, p->fz_depth, p->T_snap, p->c_snap};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (auto &e : p->ev)
@@ -105,20 +124,83 @@ static void pipeline_free(nrf_pipeline *p) {
     (void)hipStreamSynchronize(p->side);
     (void)hipStreamDestroy(p->side);
   }
-  if (p->ev_fused) (void)hipEventDestroy(p->ev_fused);
+  for (hipEvent_t e : {p->ev_fused, p->ev_hook, p->ev_copy, p->ev_fs})
+    if (e) (void)hipEventDestroy(e);
   if (p->solver) nrf_solver_destroy(p->solver);
   delete p;
 }
 
-// Every entry point other than the step itself first waits for an
-// overlapped step's integration and marching cubes (host-synchronous; the
-// step's callers that want to stay stream-ordered use nrf_pipeline_join).
+static int pipeline_fusion(nrf_pipeline *p, const float *depth, const float camera_pose[16], hipStream_t st,
+                           bool events, const CamXf *dev_pose, hipEvent_t pe_int = nullptr,
+                           hipEvent_t pe_mc = nullptr, const nrf_dualquat *T = nullptr,
+                           const nrf_dualquat *centered = nullptr);
+
+// Enqueues the deferred fusion on `side`, reading its depth and snapshot slots.
+static int pipeline_fuse(nrf_pipeline *p) {
+  if (!p->fz_pending) return NRF_OK;
+  p->fz_pending = false;
+  p->side_pending = true;  // (also on an error: what was enqueued is drained)
+  const size_t npx = (size_t)p->cfg.camera.width * p->cfg.camera.height;
+  const size_t nn = (size_t)std::max(1, p->cfg.max_nodes);
+  NRF_HIP_CHECK(hipEventRecord(p->ev_fs, p->side));
+  if (int rc = pipeline_fusion(p, p->fz_depth + (size_t)p->fz_slot * npx, p->fz_pose, p->side, true, nullptr,
+                               nullptr, nullptr, p->T_snap + (size_t)p->fz_snap * nn,
+                               p->c_snap + (size_t)p->fz_snap * nn))
+    return rc;
+  NRF_HIP_CHECK(hipEventRecord(p->ev_fused, p->side));
+  p->fz_events = true;
+  return NRF_OK;
+}
+
+// The side stream's part at step F's first PCG launch, after `after` (an
+// event on F's stream): the deferred fusion of step F-1 (its snapshot was
+// taken by F's first kernel), then F's depth into its slot.
+static int pipeline_side_step(nrf_pipeline *p, hipEvent_t after) {
+  NRF_HIP_CHECK(hipStreamWaitEvent(p->side, after, 0));
+  p->side_pending = true;
+  if (int rc = pipeline_fuse(p)) return rc;
+  const size_t npx = (size_t)p->cfg.camera.width * p->cfg.camera.height;
+  NRF_HIP_CHECK(hipMemcpyAsync(p->fz_depth + (size_t)(p->fz_steps & 1) * npx, p->fz_src, npx * sizeof(float),
+                               hipMemcpyDeviceToDevice, p->side));
+  NRF_HIP_CHECK(hipEventRecord(p->ev_copy, p->side));
+  p->fz_copied = true;
+  return NRF_OK;
+}
+
+// The solver's callback before a step's first PCG launch.
+static int pipeline_fuse_hook(void *ctx, hipStream_t st) {
+  nrf_pipeline *p = static_cast<nrf_pipeline *>(ctx);
+  NRF_HIP_CHECK(hipEventRecord(p->ev_hook, st));
+  return pipeline_side_step(p, p->ev_hook);
+}
+
+// A deferred fusion that no step takes: after `after` (the end of its
+// step), the snapshot into the slot the next step's first kernel would
+// write, then the fusion.
+static int pipeline_flush(nrf_pipeline *p, hipEvent_t after) {
+  if (!p->fz_pending) return NRF_OK;
+  NRF_HIP_CHECK(hipStreamWaitEvent(p->side, after, 0));
+  p->side_pending = true;
+  const size_t nn = (size_t)std::max(1, p->cfg.max_nodes), bytes = sizeof(nrf_dualquat) * (size_t)p->N;
+  p->fz_snap = (int)(p->fz_steps & 1);
+  NRF_HIP_CHECK(hipMemcpyAsync(p->T_snap + (size_t)p->fz_snap * nn, p->T, bytes, hipMemcpyDeviceToDevice, p->side));
+  NRF_HIP_CHECK(
+      hipMemcpyAsync(p->c_snap + (size_t)p->fz_snap * nn, p->centered, bytes, hipMemcpyDeviceToDevice, p->side));
+  p->fz_flushed = true;
+  return pipeline_fuse(p);
+}
+
+// Every entry point other than the step itself first enqueues a deferred
+// fusion and waits for it (host-synchronous; the step's callers that want to
+// stay stream-ordered use nrf_pipeline_join).
 static int pipeline_drain(nrf_pipeline *p) {
+  int rc = pipeline_flush(p, p->ev[1]);
   if (p->side_pending) {
     NRF_HIP_CHECK(hipStreamSynchronize(p->side));
     p->side_pending = false;
+    p->fz_flushed = false;
   }
-  return NRF_OK;
+  return rc;
 }
 
 extern "C" nrf_pipeline *nrf_pipeline_create(const nrf_pipeline_config *cfg) {
@@ -179,13 +261,14 @@ extern "C" void nrf_pipeline_destroy(nrf_pipeline *p) {
   if (p) pipeline_free(p);
 }
 
-extern "C" nrf_voxel *nrf_pipeline_voxels(nrf_pipeline *p) { return p ? p->vox : nullptr; }
+// The volume and mesh getters wait for an overlapped step's fusion.
+extern "C" nrf_voxel *nrf_pipeline_voxels(nrf_pipeline *p) { return p && !pipeline_drain(p) ? p->vox : nullptr; }
 extern "C" nrf_dualquat *nrf_pipeline_transforms(nrf_pipeline *p) { return p ? p->T : nullptr; }
 extern "C" nrf_node *nrf_pipeline_nodes(nrf_pipeline *p) { return p ? p->nodes : nullptr; }
 extern "C" int32_t *nrf_pipeline_knn(nrf_pipeline *p) { return p ? p->knn : nullptr; }
 extern "C" float *nrf_pipeline_knn_w(nrf_pipeline *p) { return p ? p->knn_w : nullptr; }
 extern "C" float *nrf_pipeline_mesh(nrf_pipeline *p, int32_t **counts) {
-  if (!p) return nullptr;
+  if (!p || pipeline_drain(p)) return nullptr;
   if (counts) *counts = p->counts;
   return p->verts;
 }
@@ -193,6 +276,7 @@ extern "C" nrf_solver *nrf_pipeline_solver(nrf_pipeline *p) { return p ? p->solv
 extern "C" int nrf_pipeline_volume16(nrf_pipeline *p, uint16_t **tsdf16, uint8_t **weight8, uint16_t **knn16,
                                      uint16_t **knn_w16) {
   if (!p) return NRF_ERR_INVALID_ARGUMENT;
+  if (int rc = pipeline_drain(p)) return rc;
   if (tsdf16) *tsdf16 = p->t16;
   if (weight8) *weight8 = p->w8;
   if (knn16) *knn16 = p->k16;
@@ -260,6 +344,31 @@ static int pipeline_mc(nrf_pipeline *p, hipStream_t st) {
   return rc;
 }
 
+// The warped integration and marching cubes of a step on `st` (stage events
+// ev[2] / ev[3] when `events`; per-kernel profiling events pe_int / pe_mc),
+// with the node transforms T / centered (null: the live tables).
+static int pipeline_fusion(nrf_pipeline *p, const float *depth, const float camera_pose[16], hipStream_t st,
+                           bool events, const CamXf *dev_pose, hipEvent_t pe_int, hipEvent_t pe_mc,
+                           const nrf_dualquat *T, const nrf_dualquat *centered) {
+  if (!T) T = p->T;
+  const nrf_dualquat *C = centered ? centered : p->centered;
+  int rc;
+  if (p->cfg.storage16)
+    rc = tsdf16_integrate(&p->cfg.volume, p->t16, p->w8, depth, &p->cfg.camera, camera_pose, C, p->N, p->k16,
+                          p->kw16, p->knn_rows, p->updated, st, nullptr, 0, dev_pose, p->occ, p->dblk);
+  else
+    rc = tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, p->nodes, T, p->N, C, p->knn,
+                        p->knn_w, p->knn_rows, nullptr, p->cfg.min_opt_count, p->updated, st, dev_pose, p->occ,
+                        p->dblk);
+  if (rc) return rc;
+  if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[2], st));
+  if (pe_int) NRF_HIP_CHECK(hipEventRecord(pe_int, st));
+  if ((rc = pipeline_mc(p, st))) return rc;
+  if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[3], st));
+  if (pe_mc) NRF_HIP_CHECK(hipEventRecord(pe_mc, st));
+  return NRF_OK;
+}
+
 // `events`: record the stage events (not inside a capture).  dev_pose: the
 // integration reads its camera transform there (a captured step).
 static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float camera_pose[16],
@@ -273,7 +382,21 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
   // events sit on the caller's stream)
   const bool ovl = p->overlap && events && !dev_pose && !(profile && !p->prof_pcg);
   if (!ovl && (rc = pipeline_drain(p))) return rc;
-  if (ovl && p->side_pending) solver_set_pcg_wait(p->solver, p->ev_fused);
+  if (ovl) {
+    // a flushed fusion read the snapshot slot this step writes
+    if (p->fz_flushed) NRF_HIP_CHECK(hipStreamWaitEvent(st, p->ev_fused, 0));
+    p->fz_flushed = false;
+    // the deferred fusion's snapshot in this solve's first kernel; at its
+    // first PCG launch that fusion and this step's depth copy
+    if (p->fz_pending) {
+      const size_t nn = (size_t)std::max(1, p->cfg.max_nodes);
+      p->fz_snap = (int)(p->fz_steps & 1);
+      solver_set_snapshot(p->solver, p->T_snap + (size_t)p->fz_snap * nn, p->c_snap + (size_t)p->fz_snap * nn);
+    }
+    p->fz_src = depth;
+    p->fz_copied = false;
+    solver_set_pcg_hook(p->solver, pipeline_fuse_hook, p);
+  }
   if (profile) {
     pe = &p->prof[(size_t)(p->prof_steps % p->prof_ring) * p->prof_per];
     if (p->prof_pcg) {
@@ -284,39 +407,37 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
       solver_set_event_sink(p->solver, pe + 1, p->prof_per - 3);
     }
   }
-  if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[0], st));
+  if (events && !ovl) NRF_HIP_CHECK(hipEventRecord(p->ev[0], st));  // (overlapped: no solve stage event)
   rc = nrf_solver_solve_gn(p->solver, corrs, canonical, num_corrs, p->nodes, p->T, p->centered, p->N, &p->cfg.gn,
                            p->delta_x, nullptr, stream);
   if (profile) solver_set_event_sink(p->solver, nullptr, 0);
-  solver_set_pcg_wait(p->solver, nullptr);  // consumed by the solve (or dropped with an error)
+  // all consumed by the solve (or dropped with an error)
+  solver_set_pcg_wait(p->solver, nullptr);
+  solver_set_pcg_hook(p->solver, nullptr, nullptr);
+  solver_set_snapshot(p->solver, nullptr, nullptr);
   if (rc) return rc;
   if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[1], st));
   if (ovl) {
-    // the fusion of this frame on the side stream, after the solve; the next
-    // step's solve waits for it before its first twist update
-    NRF_HIP_CHECK(hipStreamWaitEvent(p->side, p->ev[1], 0));
-    st = p->side;
+    // a solve without a PCG launch (which updates no transform, so its
+    // snapshot holds what the deferred fusion needs) leaves the hook's part
+    // to be done here
+    if (!p->fz_copied && (rc = pipeline_side_step(p, p->ev[1]))) return rc;
+    // this step's fusion is deferred; the caller's stream waits for the copy
+    // of its depth
+    NRF_HIP_CHECK(hipStreamWaitEvent(st, p->ev_copy, 0));
+    std::memcpy(p->fz_pose, camera_pose, sizeof(p->fz_pose));
+    p->fz_slot = (int)(p->fz_steps & 1);
+    ++p->fz_steps;
+    p->fz_pending = true;
+    if (profile) ++p->prof_steps;
+    return NRF_OK;
   }
-  if (p->cfg.storage16)
-    rc = tsdf16_integrate(&p->cfg.volume, p->t16, p->w8, depth, &p->cfg.camera, camera_pose, p->centered, p->N,
-                          p->k16, p->kw16, p->knn_rows, p->updated, st, nullptr, 0, dev_pose, p->occ, p->dblk);
-  else
-    rc = tsdf_integrate(&p->cfg.volume, p->vox, depth, &p->cfg.camera, camera_pose, p->nodes, p->T, p->N,
-                        p->centered, p->knn, p->knn_w, p->knn_rows, nullptr, p->cfg.min_opt_count, p->updated, st,
-                        dev_pose, p->occ, p->dblk);
-  if (rc) return rc;
-  if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[2], st));
-  if (profile && !p->prof_pcg) NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 2], st));
-  if ((rc = pipeline_mc(p, st))) return rc;
-  if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[3], st));
-  if (ovl) {
-    NRF_HIP_CHECK(hipEventRecord(p->ev_fused, st));
-    p->side_pending = true;
-  }
-  if (profile) {
-    if (!p->prof_pcg) NRF_HIP_CHECK(hipEventRecord(pe[p->prof_per - 1], st));
-    ++p->prof_steps;
-  }
+  p->fz_events = false;
+  const bool ring = profile && !p->prof_pcg;
+  if ((rc = pipeline_fusion(p, depth, camera_pose, st, events, dev_pose, ring ? pe[p->prof_per - 2] : nullptr,
+                            ring ? pe[p->prof_per - 1] : nullptr)))
+    return rc;
+  if (profile) ++p->prof_steps;
   return NRF_OK;
 }
 
@@ -327,7 +448,8 @@ extern "C" int nrf_pipeline_step(nrf_pipeline *p, const float *depth, const floa
   if (p->N <= 0) return NRF_ERR_NOT_READY;
   if (!p->cfg.use_graph || p->prof_ring > 0) {
     p->graph_step = false;
-    return pipeline_enqueue(p, depth, camera_pose, corrs, canonical, num_corrs, stream, p->prof_ring > 0);
+    const bool sampled = p->prof_ring > 0 && p->prof_calls++ % p->prof_every == 0;
+    return pipeline_enqueue(p, depth, camera_pose, corrs, canonical, num_corrs, stream, sampled);
   }
   if (int rc0 = pipeline_drain(p)) return rc0;
   // the frame into the captured step's input buffers (stream-ordered, ahead
@@ -398,7 +520,14 @@ extern "C" int nrf_pipeline_set_overlap(nrf_pipeline *p, int enable) {
   if (int rc = pipeline_drain(p)) return rc;
   if (enable && !p->side) {
     NRF_HIP_CHECK(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
-    NRF_HIP_CHECK(hipEventCreateWithFlags(&p->ev_fused, hipEventDisableTiming));
+    for (hipEvent_t *e : {&p->ev_fused, &p->ev_hook, &p->ev_copy})
+      NRF_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    NRF_HIP_CHECK(hipEventCreate(&p->ev_fs));
+    const size_t npx = (size_t)p->cfg.camera.width * p->cfg.camera.height;
+    const size_t nn = (size_t)std::max(1, p->cfg.max_nodes);
+    NRF_HIP_CHECK(hipMalloc(&p->fz_depth, 2 * std::max<size_t>(npx, 1) * sizeof(float)));
+    NRF_HIP_CHECK(hipMalloc(&p->T_snap, 2 * nn * sizeof(nrf_dualquat)));
+    NRF_HIP_CHECK(hipMalloc(&p->c_snap, 2 * nn * sizeof(nrf_dualquat)));
   }
   p->overlap = enable != 0;
   return NRF_OK;
@@ -406,6 +535,7 @@ extern "C" int nrf_pipeline_set_overlap(nrf_pipeline *p, int enable) {
 
 extern "C" int nrf_pipeline_join(nrf_pipeline *p, void *stream) {
   if (!p) return NRF_ERR_INVALID_ARGUMENT;
+  if (int rc = pipeline_flush(p, p->ev[1])) return rc;
   if (p->side_pending) NRF_HIP_CHECK(hipStreamWaitEvent(as_stream(stream), p->ev_fused, 0));
   return NRF_OK;
 }
@@ -417,7 +547,15 @@ extern "C" int nrf_pipeline_stage_ms(nrf_pipeline *p, float out_ms[3]) {
     out_ms[1] = out_ms[2] = -1.f;
     return NRF_OK;
   }
-  for (int k = 0; k < 3; ++k) NRF_HIP_CHECK(hipEventElapsedTime(&out_ms[k], p->ev[k], p->ev[k + 1]));
+  if (int rc = pipeline_drain(p)) return rc;
+  // an overlapped step: no solve stage, its fusion from its start on the
+  // side stream
+  if (p->fz_events)
+    out_ms[0] = -1.f;
+  else
+    NRF_HIP_CHECK(hipEventElapsedTime(&out_ms[0], p->ev[0], p->ev[1]));
+  NRF_HIP_CHECK(hipEventElapsedTime(&out_ms[1], p->fz_events ? p->ev_fs : p->ev[1], p->ev[2]));
+  NRF_HIP_CHECK(hipEventElapsedTime(&out_ms[2], p->ev[2], p->ev[3]));
   return NRF_OK;
 }
 
@@ -428,6 +566,8 @@ static int pipeline_profile_setup(nrf_pipeline *p, int ring_steps, int pcg_only)
   p->prof.clear();
   p->prof_ring = 0;
   p->prof_steps = 0;
+  p->prof_calls = 0;
+  p->prof_every = 1;
   p->prof_pcg = 0;
   if (ring_steps == 0) return NRF_OK;
   const int G = std::max(1, p->cfg.gn.gn_iterations);
@@ -444,6 +584,13 @@ extern "C" int nrf_pipeline_profile(nrf_pipeline *p, int ring_steps) {
 }
 extern "C" int nrf_pipeline_profile_pcg(nrf_pipeline *p, int ring_steps) {
   return pipeline_profile_setup(p, ring_steps, 1);
+}
+
+extern "C" int nrf_pipeline_profile_every(nrf_pipeline *p, int k) {
+  if (!p || k < 1) return NRF_ERR_INVALID_ARGUMENT;
+  p->prof_every = k;
+  p->prof_calls = 0;
+  return NRF_OK;
 }
 
 extern "C" int nrf_pipeline_profile_read(nrf_pipeline *p, int step_back, float out_ms[6]) {

@@ -111,9 +111,21 @@ __device__ inline void mem_count_body(const nrf_corr *__restrict__ corrs, int n,
   for (int j = threadIdx.x; j < kMemHash; j += kMemThreads)
     if (hs.key[j] >= 0) atomicAdd(&cnt[hs.key[j]], hs.cnt[j]);
 }
+// A copy of the node transforms and their centred table taken by a solve's
+// first kernel (solver_set_snapshot); dst_T null: none.
+struct SnapArgs {
+  const DQ *T = nullptr, *C = nullptr;
+  DQ *dst_T = nullptr, *dst_C = nullptr;
+  int N = 0;
+};
 __global__ __launch_bounds__(256) void mem_count_kernel(const nrf_corr *__restrict__ corrs, int n, int N,
-                                                        int32_t *__restrict__ cnt) {
-  mem_count_body(corrs, n, N, cnt, blockIdx.x * blockDim.x + threadIdx.x);
+                                                        int32_t *__restrict__ cnt, SnapArgs sn) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sn.dst_T && c < sn.N) {
+    sn.dst_T[c] = sn.T[c];
+    sn.dst_C[c] = sn.C[c];
+  }
+  mem_count_body(corrs, n, N, cnt, c);
 }
 
 // The memberships' scan of one frame: off / cur from the per-node counts,
@@ -2929,11 +2941,22 @@ struct nrf_solver {
   hipEvent_t *sink = nullptr;
   int sink_n = 0, sink_i = 0, sink_skip = 0;
   hipEvent_t pcg_wait = nullptr;  // see solver_set_pcg_wait
+  solver_hook_fn hook = nullptr;   // see solver_set_pcg_hook
+  void *hook_ctx = nullptr;
+  nrf_dualquat *snap_T = nullptr, *snap_C = nullptr;  // see solver_set_snapshot
 };
 
 namespace nrf {
 
 void solver_set_pcg_wait(nrf_solver *s, hipEvent_t ev) { s->pcg_wait = ev; }
+void solver_set_pcg_hook(nrf_solver *s, solver_hook_fn fn, void *ctx) {
+  s->hook = fn;
+  s->hook_ctx = ctx;
+}
+void solver_set_snapshot(nrf_solver *s, nrf_dualquat *T, nrf_dualquat *centered) {
+  s->snap_T = T;
+  s->snap_C = centered;
+}
 
 void solver_set_event_sink(nrf_solver *s, hipEvent_t *events, int count, int skip) {
   s->sink = events;
@@ -3054,7 +3077,7 @@ static int solver_build_partition(nrf_solver *s) {
 
 // begin: the scan kernel also opens the solve (gn_begin_kernel's work).
 static int solver_build_memberships(nrf_solver *s, const nrf_corr *corrs, int n, int N, hipStream_t st,
-                                    bool begin = false) {
+                                    bool begin = false, SnapArgs sn = SnapArgs()) {
   int rc;
   const size_t cap0 = s->c_m_cnt;
   if ((rc = dev_alloc(s->m_cnt, (size_t)N, s->c_m_cnt))) return rc;
@@ -3074,8 +3097,8 @@ static int solver_build_memberships(nrf_solver *s, const nrf_corr *corrs, int n,
   }
   s->m_cnt_zero = true;
   s->m_cnt_n = N;
-  if (n > 0) {
-    mem_count_kernel<<<(n + 255) / 256, 256, 0, st>>>(corrs, n, N, s->m_cnt);
+  if (n > 0 || sn.dst_T) {
+    mem_count_kernel<<<(std::max(n, sn.dst_T ? sn.N : 0) + 255) / 256, 256, 0, st>>>(corrs, n, N, s->m_cnt, sn);
     NRF_LAUNCH_CHECK();
   }
   // offsets, cursors and the MFMA assembly's chunk offsets in one pass
@@ -3632,7 +3655,22 @@ int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical
   int rc;
   const bool plain = s->params.plain_form != 0;
   if (plain && (rc = plain_check_stream(st))) return rc;
-  if (!plain && (rc = solver_build_memberships(s, corrs, num_corrs, num_nodes, st, begin))) return rc;
+  // the snapshot of the transforms (solver_set_snapshot) in the first kernel
+  SnapArgs sn;
+  if (s->snap_T) {
+    sn.T = reinterpret_cast<const DQ *>(transforms);
+    sn.C = reinterpret_cast<const DQ *>(centered);
+    sn.dst_T = reinterpret_cast<DQ *>(s->snap_T);
+    sn.dst_C = reinterpret_cast<DQ *>(s->snap_C);
+    sn.N = num_nodes;
+    s->snap_T = s->snap_C = nullptr;
+    if (plain) {
+      const size_t bytes = sizeof(DQ) * (size_t)num_nodes;
+      NRF_HIP_CHECK(hipMemcpyAsync(sn.dst_T, sn.T, bytes, hipMemcpyDeviceToDevice, st));
+      NRF_HIP_CHECK(hipMemcpyAsync(sn.dst_C, sn.C, bytes, hipMemcpyDeviceToDevice, st));
+    }
+  }
+  if (!plain && (rc = solver_build_memberships(s, corrs, num_corrs, num_nodes, st, begin, sn))) return rc;
   if (begin && plain) {
     gn_begin_kernel<<<1, 64, 0, st>>>(s->stop, s->d_res, s->d_tot);
     NRF_LAUNCH_CHECK();
@@ -3660,12 +3698,20 @@ int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical
     }
     return NRF_OK;
   };
+  // the caller's callback before the first PCG launch (solver_set_pcg_hook)
+  auto hook_before_pcg = [&]() -> int {
+    if (!s->hook) return NRF_OK;
+    const solver_hook_fn fn = s->hook;
+    s->hook = nullptr;
+    return fn(s->hook_ctx, st);
+  };
   for (int it = 0; it < std::max(1, gn->gn_iterations); ++it) {
     if (plain) {  // the reference's structure: atomics, host-driven PCG
       if ((rc = plain_assemble(s, corrs, num_corrs, canonical, nodes, reinterpret_cast<const DQ *>(transforms),
                                reinterpret_cast<const DQ *>(centered), num_nodes, st)))
         return rc;
       sink_record(s, st);
+      if ((rc = hook_before_pcg())) return rc;
       if ((rc = plain_solve(s, delta_x, num_nodes, st))) return rc;
       sink_record(s, st);
       if ((rc = wait_before_update())) return rc;
@@ -3688,6 +3734,7 @@ int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical
                               reinterpret_cast<const DQ *>(centered), num_nodes, st, true)))
       return rc;
     sink_record(s, st);
+    if ((rc = hook_before_pcg())) return rc;
     if (s->params.refine_jacobi && s->params.pcg_pipelined == 3 && s->pcg_sub != 1 &&
         1 + 2 * (long)s->params.pcg_iterations < kPcgMaxPoints) {
       // the refinement step at the end of the pipelined PCG launch

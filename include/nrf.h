@@ -613,20 +613,25 @@ int nrf_pipeline_stage_ms(nrf_pipeline *p, float out_ms[3]);
    buffers / pattern change. */
 long long nrf_pipeline_graph_captures(nrf_pipeline *p);
 /* No counterpart in the reference (which runs one frame after the other on
-   the legacy stream).  enable = 1: eager steps run their warped integration
-   and marching cubes on a stream of the pipeline's own, after the step's
-   solve, overlapping the next step's membership build, first assembly and
-   first PCG; the next step's solve waits for them before its first twist
-   update (the first write of the node transforms they read), so every frame
-   computes what it computes unoverlapped (tested bit for bit).  The
-   persistent PCG then shares the device with them: its workgroups may start
-   late, which every wait in it tolerates (they are bounded, not timed).  A step's volume and mesh are then complete only after
-   nrf_pipeline_join on the reader's stream, a device synchronisation, or any
-   other pipeline call (which waits for them).  Graph-replayed and per-kernel
-   profiled steps do not overlap.  Synchronises the host when switching. */
+   the legacy stream).  enable = 1: an eager step's warped integration and
+   marching cubes are deferred and run on a stream of the pipeline's own
+   alongside the next step's first PCG (enqueued when that step reaches it),
+   from a copy of the step's depth (made on that stream; the caller's stream
+   waits for the copy, so the depth buffer may be overwritten once the step
+   returns) and a snapshot of the node transforms as the step's solve left
+   them.  The next step's first twist update waits only for the snapshot, so
+   every frame computes what it computes unoverlapped (tested bit for bit).
+   The persistent PCG then shares the device with them: its workgroups may
+   start late, which every wait in it tolerates (they are bounded, not
+   timed).  A step's volume and mesh are complete only after
+   nrf_pipeline_join on the reader's stream or any other pipeline call,
+   including nrf_pipeline_voxels / _mesh / _volume16 (which enqueue a
+   deferred fusion and wait for it); a device synchronisation alone does not
+   run a deferred fusion.  Graph-replayed and per-kernel profiled steps do
+   not overlap.  Synchronises the host when switching. */
 int nrf_pipeline_set_overlap(nrf_pipeline *p, int enable);
-/* Makes `stream` wait for the integration and marching cubes of the last
-   overlapped step (a no-op otherwise). */
+/* Enqueues a deferred fusion of the last overlapped step and makes `stream`
+   wait for it (a no-op otherwise). */
 int nrf_pipeline_join(nrf_pipeline *p, void *stream);
 /* Per-kernel profiling: with ring_steps > 0, every step records hipEvents
    around each kernel group on the step's stream (eager launches, no graph)
@@ -641,6 +646,11 @@ int nrf_pipeline_profile(nrf_pipeline *p, int ring_steps);
    iteration (step mod gn_iterations), and nrf_pipeline_profile_read returns
    that one launch's ms in out_ms[2] (the other entries 0). */
 int nrf_pipeline_profile_pcg(nrf_pipeline *p, int ring_steps);
+/* No counterpart in the reference: the profiling events of either form are
+   recorded on every k-th step only (k >= 1; 1 after each profile set-up);
+   the ring then holds the sampled steps, and step_back of
+   nrf_pipeline_profile_read counts sampled steps.  Restarts the count. */
+int nrf_pipeline_profile_every(nrf_pipeline *p, int k);
 int nrf_pipeline_profile_read(nrf_pipeline *p, int step_back, float out_ms[6]);
 /* Voxels written by the warped integrations of all steps so far (syncs). */
 long long nrf_pipeline_updated_voxels(nrf_pipeline *p);
