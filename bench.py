@@ -278,7 +278,7 @@ def main() -> int:
     ap.add_argument("--graph", action="store_true")
     ap.add_argument("--no-overlap", action="store_true",
                     help="run each step's integration and marching cubes on the caller's stream instead of "
-                         "overlapping them with the next step's memberships and first assembly")
+                         "beside the next step's first PCG launches")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="before the timed steps, run one elementwise copy of MIB MiB (x * 1 -> y): a kernel of "
                          "known HBM traffic that calibrates hardware byte counters")
@@ -345,16 +345,16 @@ def main() -> int:
         lanes, wg_nodes = (int(v) for v in args.pcg_layout.split(","))
         L.check(h.lib.nrf_solver_set_pcg_layout(h.lib.nrf_pipeline_solver(h.pipe), lanes, wg_nodes), "pcg_layout")
     overlap = not args.no_overlap and not args.graph
-    if overlap:  # the frame's fusion overlaps the next frame's memberships and first assembly
+    if overlap:  # each frame's fusion runs beside the next frame's first PCG launches
         L.check(h.lib.nrf_pipeline_set_overlap(h.pipe, 1), "overlap")
     log(f"[bench] setup {time.perf_counter() - t0:.1f} s, {h.N} nodes, "
         f"{int(h.frames_host[1].valid.sum())} valid pixels in frame 1")
     nf = args.frames
     seq = [1 + (i % nf) for i in range(args.warmup + args.steps)]
     lib = h.lib
-    # live timing of the dominant kernel: two events around one PCG launch per
-    # step (GN iteration step mod G), which cost the step far less than
-    # events around every kernel group (about 3 us of device time each)
+    # live timing of the dominant kernel: two events around one PCG launch on
+    # every K-th timed step (GN iterations in turn), which cost the step far
+    # less than events around every kernel group (a few us of device time each)
     stream_ptr = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     for f in seq[: args.warmup]:
         h.step(f)
