@@ -213,8 +213,8 @@ int track_begin(TrackDev *t, hipStream_t st);
 int track_count_reset(TrackDev *t, const int32_t *stop, hipStream_t st);
 int track_gate(TrackDev *t, int g, int min_valid, int32_t *stop, hipStream_t st);
 int track_rmse_blocks(int n);
-// judge = false: the base RMSE of iteration g; true: the trial test of scale
-int track_rmse(TrackDev *t, const nrf_corr *corrs, int n, double *part, int g, bool judge, float scale,
+// trial = false: the base RMSE of iteration g; true: the trial test of scale
+int track_rmse(TrackDev *t, const nrf_corr *corrs, int n, double *part, int g, bool trial, float scale,
                int min_valid, const int32_t *stop, hipStream_t st);
 int track_ls_begin(TrackDev *t, int g, float update_scale, const nrf_dualquat *T, nrf_dualquat *Tsave, int N,
                    int32_t *stop, hipStream_t st);

@@ -178,7 +178,7 @@ __global__ __launch_bounds__(256) void ls_apply_kernel(TrackDev *t, const float 
 
 // The trial test (:572-579): enough valid correspondences (at least
 // min_valid and 75 % of the iteration's), a finite RMSE below the best.
-__global__ __launch_bounds__(kRmseThreads) void ls_judge_kernel(TrackDev *t, const double *part, int nblk, int g,
+__global__ __launch_bounds__(kRmseThreads) void ls_trial_kernel(TrackDev *t, const double *part, int nblk, int g,
                                                                 float scale, int min_valid, const int32_t *stop) {
   if (stopped_uniform(stop)) return;
   const double r = rmse_from_parts(part, nblk);
@@ -229,13 +229,13 @@ int track_gate(TrackDev *t, int g, int min_valid, int32_t *stop, hipStream_t st)
   return NRF_OK;
 }
 int track_rmse_blocks(int n) { return std::max(1, (n + kRmseThreads - 1) / kRmseThreads); }
-int track_rmse(TrackDev *t, const nrf_corr *corrs, int n, double *part, int g, bool judge, float scale,
+int track_rmse(TrackDev *t, const nrf_corr *corrs, int n, double *part, int g, bool trial, float scale,
                int min_valid, const int32_t *stop, hipStream_t st) {
   const int nblk = track_rmse_blocks(n);
   rmse_partial_kernel<<<nblk, kRmseThreads, 0, st>>>(corrs, n, part, stop);
   NRF_LAUNCH_CHECK();
-  if (judge)
-    ls_judge_kernel<<<1, kRmseThreads, 0, st>>>(t, part, nblk, g, scale, min_valid, stop);
+  if (trial)
+    ls_trial_kernel<<<1, kRmseThreads, 0, st>>>(t, part, nblk, g, scale, min_valid, stop);
   else
     ls_base_kernel<<<1, kRmseThreads, 0, st>>>(t, part, nblk, g, stop);
   NRF_LAUNCH_CHECK();

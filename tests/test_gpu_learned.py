@@ -207,7 +207,7 @@ def test_held_out_iterations_both_variants(held_out):
                                        "PCG iterations against block-Jacobi on held-out systems instead of halving "
                                        "them (the additive PSD form cannot lower the spectrum's large end)")
 def test_learned_preconditioner_halves_iterations(held_out):
-    """The north-star target: at least 2x fewer PCG iterations to 1e-6 than
+    """The learned preconditioner's target: at least 2x fewer PCG iterations to 1e-6 than
     block-Jacobi, for the better of the two variants, on every held-out
     system.  Kept as a strict expected failure: it documents the shortfall
     and fails if the target is reached without this record being updated."""
