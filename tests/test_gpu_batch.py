@@ -210,6 +210,19 @@ def test_pipeline_pcg_profile_sampling(gpu):
     ms_s, ms_f = float(np.mean(sampled)), float(np.mean(full))
     print(f"[profile] PCG launch {ms_s:.4f} ms sampled, {ms_f:.4f} ms from the full ring", flush=True)
     assert ms_s > 0 and ms_f > 0
+    # every third step only: 7 steps give the sampled steps 0, 3, 6, whose
+    # records are the ring's last three (a fourth does not exist)
+    L.check(a.lib.nrf_pipeline_profile_pcg(a.pipe, 16), "profile_pcg")
+    L.check(a.lib.nrf_pipeline_profile_every(a.pipe, 3), "profile_every")
+    assert a.lib.nrf_pipeline_profile_every(a.pipe, 0) != 0
+    for f in [1, 2, 3, 1, 2, 3, 1]:
+        a.step(f)
+    torch.cuda.synchronize()
+    out = (C.c_float * 6)()
+    for k in range(3):
+        L.check(a.lib.nrf_pipeline_profile_read(a.pipe, k, out), "profile_read")
+        assert 0 < out[2] and out[0] == 0
+    assert a.lib.nrf_pipeline_profile_read(a.pipe, 3, out) != 0
     a.close()
     b.close()
 
