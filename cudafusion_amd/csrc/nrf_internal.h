@@ -49,13 +49,18 @@ void solver_set_event_sink(nrf_solver *s, hipEvent_t *events, int count, int ski
 // node transforms or their centred table (null: no wait).  One solve
 // consumes it.
 void solver_set_pcg_wait(nrf_solver *s, hipEvent_t ev);
-// The next nrf_solver_solve_gn calls fn(ctx, stream) once, on the host,
-// right before it enqueues its first PCG launch: work the callback orders
-// after an event it records on `stream` follows the solve's memberships and
-// first assembly.  A non-zero return ends the solve with that status.  One
-// solve consumes it; it is dropped when the solve ends before a PCG launch.
+// The next nrf_solver_solve_gn records `ev` on its stream right before its
+// first PCG launch (after the memberships and the first assembly) and, once
+// that launch is enqueued, calls fn(ctx, stream) on the host, which may
+// order work of its own after `ev`.  A non-zero return ends the solve with
+// that status.  One solve consumes it; it is dropped when the solve ends
+// before a PCG launch.
 typedef int (*solver_hook_fn)(void *ctx, hipStream_t stream);
-void solver_set_pcg_hook(nrf_solver *s, solver_hook_fn fn, void *ctx);
+void solver_set_pcg_hook(nrf_solver *s, solver_hook_fn fn, void *ctx, hipEvent_t ev);
+// Enqueues on `stream` a one-wave kernel that ends once every workgroup of
+// the solver's next persistent PCG launch has started (or after a bounded
+// wait), so that work behind it cannot take the device first.
+int solver_pcg_start_wait(nrf_solver *s, hipStream_t stream);
 // The next nrf_solver_solve_gn copies the node transforms and their centred
 // table, as they are when it starts, into T / centered (num_nodes entries
 // each), in its first kernel.  One solve consumes it.

@@ -155,9 +155,13 @@ static int pipeline_fuse(nrf_pipeline *p) {
 // The side stream's part at step F's first PCG launch, after `after` (an
 // event on F's stream): the deferred fusion of step F-1 (its snapshot was
 // taken by F's first kernel), then F's depth into its slot.
-static int pipeline_side_step(nrf_pipeline *p, hipEvent_t after) {
+static int pipeline_side_step(nrf_pipeline *p, hipEvent_t after, bool pcg_next = false) {
   NRF_HIP_CHECK(hipStreamWaitEvent(p->side, after, 0));
   p->side_pending = true;
+  // the fusion's workgroups come after the PCG launch's (which needs whole
+  // halves of CUs free to start)
+  if (pcg_next && p->fz_pending)
+    if (int rc = solver_pcg_start_wait(p->solver, p->side)) return rc;
   if (int rc = pipeline_fuse(p)) return rc;
   const size_t npx = (size_t)p->cfg.camera.width * p->cfg.camera.height;
   NRF_HIP_CHECK(hipMemcpyAsync(p->fz_depth + (size_t)(p->fz_steps & 1) * npx, p->fz_src, npx * sizeof(float),
@@ -167,11 +171,13 @@ static int pipeline_side_step(nrf_pipeline *p, hipEvent_t after) {
   return NRF_OK;
 }
 
-// The solver's callback before a step's first PCG launch.
+// The solver's callback once a step's first PCG launch is enqueued (ev_hook,
+// recorded before that launch, orders the side stream's work after the
+// step's memberships and first assembly).
 static int pipeline_fuse_hook(void *ctx, hipStream_t st) {
   nrf_pipeline *p = static_cast<nrf_pipeline *>(ctx);
-  NRF_HIP_CHECK(hipEventRecord(p->ev_hook, st));
-  return pipeline_side_step(p, p->ev_hook);
+  (void)st;
+  return pipeline_side_step(p, p->ev_hook, true);
 }
 
 // A deferred fusion that no step takes: after `after` (the end of its
@@ -395,7 +401,7 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
     }
     p->fz_src = depth;
     p->fz_copied = false;
-    solver_set_pcg_hook(p->solver, pipeline_fuse_hook, p);
+    solver_set_pcg_hook(p->solver, pipeline_fuse_hook, p, p->ev_hook);
   }
   if (profile) {
     pe = &p->prof[(size_t)(p->prof_steps % p->prof_ring) * p->prof_per];
@@ -413,7 +419,7 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
   if (profile) solver_set_event_sink(p->solver, nullptr, 0);
   // all consumed by the solve (or dropped with an error)
   solver_set_pcg_wait(p->solver, nullptr);
-  solver_set_pcg_hook(p->solver, nullptr, nullptr);
+  solver_set_pcg_hook(p->solver, nullptr, nullptr, nullptr);
   solver_set_snapshot(p->solver, nullptr, nullptr);
   if (rc) return rc;
   if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[1], st));

@@ -1066,7 +1066,7 @@ struct alignas(128) SyncRec {
 struct PcgCtl {
   unsigned long long epoch;  // advanced once per launch by pcg_prep_kernel
   unsigned abort;
-  unsigned pad;
+  unsigned started;  // workgroups of the launch that have started (cleared with the epoch's advance)
 };
 constexpr int kPcgSub = 4;                        // lanes per scalar row
 constexpr int kPcgRows = kPcgThreads / kPcgSub;   // scalar rows per workgroup
@@ -1428,6 +1428,7 @@ __device__ inline bool ar3_collect(const PcgArgs &a, unsigned tag, int slot, dou
 // bits, so the owner's p and its readers' copies are identical.
 __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
   extern __shared__ float lds[];
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(&a.ctl->started, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __shared__ double red[kRedSize];
   __shared__ int s_bad;
   __shared__ unsigned s_base;
@@ -1581,6 +1582,8 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
 template <int SUB, int FORM>
 __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, int &s_bad, unsigned &s_base) {
   constexpr int ROWS = kPcgThreads / SUB;
+  // this workgroup is resident (pcg_started_wait_kernel)
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(&a.ctl->started, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if ((a.stop && *a.stop) || pcg_skip(a)) return;
   const int t = threadIdx.x;
   const int n0 = a.rows[g], n1 = a.rows[g + 1];
@@ -2141,6 +2144,19 @@ __global__ void axpy_kernel(float *__restrict__ x, const float *__restrict__ d, 
 __device__ void pcg_open(PcgCtl *ctl) {
   ctl->epoch += 1;
   ctl->abort = 0;
+  ctl->started = 0;
+}
+// Work on another stream that should not take the device before a PCG
+// launch's workgroups are resident waits behind this one-wave kernel: it
+// polls the launch's start count (bounded: after about 20 ms it lets the
+// stream go on regardless).
+__global__ void pcg_started_wait_kernel(const PcgCtl *ctl, unsigned G) {
+  if (threadIdx.x != 0) return;
+  unsigned *c = const_cast<unsigned *>(&ctl->started);
+  for (unsigned spins = 0; spins < (1u << 14); ++spins) {
+    if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= G) return;
+    __builtin_amdgcn_s_sleep(8);
+  }
 }
 __global__ void pcg_prep_kernel(PcgCtl *ctl) {
   if (threadIdx.x == 0) pcg_open(ctl);
@@ -2943,15 +2959,22 @@ struct nrf_solver {
   hipEvent_t pcg_wait = nullptr;  // see solver_set_pcg_wait
   solver_hook_fn hook = nullptr;   // see solver_set_pcg_hook
   void *hook_ctx = nullptr;
+  hipEvent_t hook_ev = nullptr;
   nrf_dualquat *snap_T = nullptr, *snap_C = nullptr;  // see solver_set_snapshot
 };
 
 namespace nrf {
 
 void solver_set_pcg_wait(nrf_solver *s, hipEvent_t ev) { s->pcg_wait = ev; }
-void solver_set_pcg_hook(nrf_solver *s, solver_hook_fn fn, void *ctx) {
+int solver_pcg_start_wait(nrf_solver *s, hipStream_t st) {
+  pcg_started_wait_kernel<<<1, 64, 0, st>>>(s->ctl, (unsigned)s->G);
+  NRF_LAUNCH_CHECK();
+  return NRF_OK;
+}
+void solver_set_pcg_hook(nrf_solver *s, solver_hook_fn fn, void *ctx, hipEvent_t ev) {
   s->hook = fn;
   s->hook_ctx = ctx;
+  s->hook_ev = ev;
 }
 void solver_set_snapshot(nrf_solver *s, nrf_dualquat *T, nrf_dualquat *centered) {
   s->snap_T = T;
@@ -3698,8 +3721,13 @@ int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical
     }
     return NRF_OK;
   };
-  // the caller's callback before the first PCG launch (solver_set_pcg_hook)
-  auto hook_before_pcg = [&]() -> int {
+  // the caller's event before the first PCG launch and its callback after
+  // that launch is enqueued (solver_set_pcg_hook)
+  auto hook_mark = [&]() -> int {
+    if (s->hook && s->hook_ev) NRF_HIP_CHECK(hipEventRecord(s->hook_ev, st));
+    return NRF_OK;
+  };
+  auto hook_run = [&]() -> int {
     if (!s->hook) return NRF_OK;
     const solver_hook_fn fn = s->hook;
     s->hook = nullptr;
@@ -3711,9 +3739,10 @@ int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical
                                reinterpret_cast<const DQ *>(centered), num_nodes, st)))
         return rc;
       sink_record(s, st);
-      if ((rc = hook_before_pcg())) return rc;
+      if ((rc = hook_mark())) return rc;
       if ((rc = plain_solve(s, delta_x, num_nodes, st))) return rc;
       sink_record(s, st);
+      if ((rc = hook_run())) return rc;
       if ((rc = wait_before_update())) return rc;
       if (g.validate) {
         gn_validate_kernel<<<1, 1024, 0, st>>>(g);
@@ -3734,13 +3763,14 @@ int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical
                               reinterpret_cast<const DQ *>(centered), num_nodes, st, true)))
       return rc;
     sink_record(s, st);
-    if ((rc = hook_before_pcg())) return rc;
+    if ((rc = hook_mark())) return rc;
     if (s->params.refine_jacobi && s->params.pcg_pipelined == 3 && s->pcg_sub != 1 &&
         1 + 2 * (long)s->params.pcg_iterations < kPcgMaxPoints) {
       // the refinement step at the end of the pipelined PCG launch
       if ((rc = pcg_launch(s, s->b, delta_x, num_nodes, s->params.pcg_iterations, true, st, true, 0, true)))
         return rc;
       sink_record(s, st);
+      if ((rc = hook_run())) return rc;
       if ((rc = wait_before_update())) return rc;
       if (g.validate) {
         gn_validate_kernel<<<1, 1024, 0, st>>>(g);
@@ -3756,6 +3786,7 @@ int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical
       if ((rc = dev_alloc(s->dref, (size_t)6 * num_nodes, s->c_dref))) return rc;
       if ((rc = pcg_launch(s, s->b, s->dref, num_nodes, s->params.pcg_iterations, true, st, true))) return rc;
       sink_record(s, st);
+      if ((rc = hook_run())) return rc;
       const RefineArgs ra{s->row_ptr, s->col_idx, s->values, s->b, s->Minv, s->dref, delta_x};
       if ((rc = wait_before_update())) return rc;
       gn_refine_update_kernel<<<(num_nodes + 3) / 4, 256, 0, st>>>(g, ra);
@@ -3765,6 +3796,7 @@ int solver_solve_gn(nrf_solver *s, const nrf_corr *corrs, const float *canonical
     }
     if ((rc = solver_pcg(s, delta_x, num_nodes, st, true))) return rc;
     sink_record(s, st);
+    if ((rc = hook_run())) return rc;
     if ((rc = wait_before_update())) return rc;
     if (g.validate) {
       gn_validate_kernel<<<1, 1024, 0, st>>>(g);
