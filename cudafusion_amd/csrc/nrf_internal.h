@@ -57,10 +57,14 @@ void solver_set_pcg_wait(nrf_solver *s, hipEvent_t ev);
 // before a PCG launch.
 typedef int (*solver_hook_fn)(void *ctx, hipStream_t stream);
 void solver_set_pcg_hook(nrf_solver *s, solver_hook_fn fn, void *ctx, hipEvent_t ev);
-// Enqueues on `stream` a one-wave kernel that ends once every workgroup of
-// the solver's next persistent PCG launch has started (or after a bounded
-// wait), so that work behind it cannot take the device first.
-int solver_pcg_start_wait(nrf_solver *s, hipStream_t stream);
+// The solver's next PCG launch carries `ticket` (non-zero): once all its
+// workgroups have started, it records the ticket.  solver_pcg_start_wait
+// enqueues on `stream` a one-wave kernel that ends when that ticket is
+// recorded (or after a bounded wait): work behind it starts after the stream
+// work that preceded the launch, and cannot take the device before the
+// launch's workgroups.
+void solver_set_pcg_ticket(nrf_solver *s, unsigned ticket);
+int solver_pcg_start_wait(nrf_solver *s, unsigned ticket, hipStream_t stream);
 // The next nrf_solver_solve_gn copies the node transforms and their centred
 // table, as they are when it starts, into T / centered (num_nodes entries
 // each), in its first kernel.  One solve consumes it.

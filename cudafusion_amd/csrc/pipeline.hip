@@ -72,9 +72,12 @@ struct nrf_pipeline {
   //  - step F+1's first kernel copies the node transforms as step F left
   //    them into snapshot slot (F+1) & 1 (solver_set_snapshot), which F's
   //    fusion reads while F+1 updates the live tables;
-  //  - at step F's first PCG launch `side` copies F's depth into depth slot
-  //    F & 1, after the previous fusion; ev_copy marks the copy, and F's
-  //    stream waits for it before the step returns (the caller may then
+  //  - `side` starts its part of step F once all workgroups of F's first
+  //    PCG launch have started (that launch carries the ticket fz_ticket;
+  //    solver_pcg_start_wait), so after F's work before that launch and
+  //    without taking the device first; it then runs step F-1's fusion and
+  //    copies F's depth into depth slot F & 1; ev_copy marks the copy, and
+  //    F's stream waits for it before the step returns (the caller may then
   //    overwrite its depth buffer, and every later write of a slot on the
   //    step's stream follows the fusion that read it);
   //  - a fusion that no step takes (a join, a drain) is flushed after ev[1]
@@ -84,12 +87,13 @@ struct nrf_pipeline {
   bool overlap = false, side_pending = false, fz_pending = false, fz_events = false, fz_flushed = false;
   bool fz_copied = false;  // the step being enqueued made its hook's copy
   hipStream_t side = nullptr;
-  hipEvent_t ev_fused = nullptr, ev_hook = nullptr, ev_copy = nullptr, ev_fs = nullptr;
+  hipEvent_t ev_fused = nullptr, ev_copy = nullptr, ev_fs = nullptr;
   float *fz_depth = nullptr;          // two depth slots
   nrf_dualquat *T_snap = nullptr, *c_snap = nullptr;  // two snapshot slots of max_nodes entries
   const float *fz_src = nullptr;      // the depth of the step being enqueued
   long fz_steps = 0;                  // overlapped steps so far (slot parity)
   int fz_slot = 0, fz_snap = 0;       // the deferred fusion's depth and snapshot slots
+  unsigned fz_ticket = 0;             // the ticket of the step's first PCG launch
   float fz_pose[16] = {};
   bool graph_step = false;    // the last step was a graph replay (stage events: whole step only)
   bool mesh_valid = false;    // verts / tris / counts hold the marching cubes of the current volume
@@ -124,7 +128,7 @@ static void pipeline_free(nrf_pipeline *p) {
     (void)hipStreamSynchronize(p->side);
     (void)hipStreamDestroy(p->side);
   }
-  for (hipEvent_t e : {p->ev_fused, p->ev_hook, p->ev_copy, p->ev_fs})
+  for (hipEvent_t e : {p->ev_fused, p->ev_copy, p->ev_fs})
     if (e) (void)hipEventDestroy(e);
   if (p->solver) nrf_solver_destroy(p->solver);
   delete p;
@@ -155,13 +159,14 @@ static int pipeline_fuse(nrf_pipeline *p) {
 // The side stream's part at step F's first PCG launch, after `after` (an
 // event on F's stream): the deferred fusion of step F-1 (its snapshot was
 // taken by F's first kernel), then F's depth into its slot.
-static int pipeline_side_step(nrf_pipeline *p, hipEvent_t after, bool pcg_next = false) {
-  NRF_HIP_CHECK(hipStreamWaitEvent(p->side, after, 0));
+static int pipeline_side_step(nrf_pipeline *p, hipEvent_t after, unsigned ticket = 0) {
+  if (after) NRF_HIP_CHECK(hipStreamWaitEvent(p->side, after, 0));
   p->side_pending = true;
-  // the fusion's workgroups come after the PCG launch's (which needs whole
-  // halves of CUs free to start)
-  if (pcg_next && p->fz_pending)
-    if (int rc = solver_pcg_start_wait(p->solver, p->side)) return rc;
+  // behind the step's first PCG launch: its workgroups start first (they
+  // need whole halves of CUs), and the step's work before it (the
+  // transforms' snapshot, the caller's depth) has ended
+  if (ticket)
+    if (int rc = solver_pcg_start_wait(p->solver, ticket, p->side)) return rc;
   if (int rc = pipeline_fuse(p)) return rc;
   const size_t npx = (size_t)p->cfg.camera.width * p->cfg.camera.height;
   NRF_HIP_CHECK(hipMemcpyAsync(p->fz_depth + (size_t)(p->fz_steps & 1) * npx, p->fz_src, npx * sizeof(float),
@@ -171,13 +176,12 @@ static int pipeline_side_step(nrf_pipeline *p, hipEvent_t after, bool pcg_next =
   return NRF_OK;
 }
 
-// The solver's callback once a step's first PCG launch is enqueued (ev_hook,
-// recorded before that launch, orders the side stream's work after the
-// step's memberships and first assembly).
+// The solver's callback once a step's first PCG launch, which carries the
+// step's ticket, is enqueued.
 static int pipeline_fuse_hook(void *ctx, hipStream_t st) {
   nrf_pipeline *p = static_cast<nrf_pipeline *>(ctx);
   (void)st;
-  return pipeline_side_step(p, p->ev_hook, true);
+  return pipeline_side_step(p, nullptr, p->fz_ticket);
 }
 
 // A deferred fusion that no step takes: after `after` (the end of its
@@ -401,7 +405,9 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
     }
     p->fz_src = depth;
     p->fz_copied = false;
-    solver_set_pcg_hook(p->solver, pipeline_fuse_hook, p, p->ev_hook);
+    if (++p->fz_ticket == 0) p->fz_ticket = 1;
+    solver_set_pcg_ticket(p->solver, p->fz_ticket);
+    solver_set_pcg_hook(p->solver, pipeline_fuse_hook, p, nullptr);
   }
   if (profile) {
     pe = &p->prof[(size_t)(p->prof_steps % p->prof_ring) * p->prof_per];
@@ -420,6 +426,7 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
   // all consumed by the solve (or dropped with an error)
   solver_set_pcg_wait(p->solver, nullptr);
   solver_set_pcg_hook(p->solver, nullptr, nullptr, nullptr);
+  solver_set_pcg_ticket(p->solver, 0);
   solver_set_snapshot(p->solver, nullptr, nullptr);
   if (rc) return rc;
   if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[1], st));
@@ -526,7 +533,7 @@ extern "C" int nrf_pipeline_set_overlap(nrf_pipeline *p, int enable) {
   if (int rc = pipeline_drain(p)) return rc;
   if (enable && !p->side) {
     NRF_HIP_CHECK(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
-    for (hipEvent_t *e : {&p->ev_fused, &p->ev_hook, &p->ev_copy})
+    for (hipEvent_t *e : {&p->ev_fused, &p->ev_copy})
       NRF_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
     NRF_HIP_CHECK(hipEventCreate(&p->ev_fs));
     const size_t npx = (size_t)p->cfg.camera.width * p->cfg.camera.height;

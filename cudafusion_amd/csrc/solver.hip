@@ -1067,6 +1067,8 @@ struct PcgCtl {
   unsigned long long epoch;  // advanced once per launch by pcg_prep_kernel
   unsigned abort;
   unsigned started;  // workgroups of the launch that have started (cleared with the epoch's advance)
+  unsigned ticket;   // the last launch whose workgroups all started, by its caller's ticket
+  unsigned pad;
 };
 constexpr int kPcgSub = 4;                        // lanes per scalar row
 constexpr int kPcgRows = kPcgThreads / kPcgSub;   // scalar rows per workgroup
@@ -1205,6 +1207,9 @@ struct PcgArgs {
   // 1: the pipelined form ends with one refinement step, x += M^-1 (b - A x)
   // with the residual in FP64 (x's halo through zpub); 2 and 4 lanes per row
   int refine;
+  // non-zero: the last workgroup to start records it in ctl->ticket
+  // (pcg_started_wait_kernel)
+  unsigned ticket = 0;
 };
 
 // A correction round of a run that already has its outcome does nothing.
@@ -1428,7 +1433,11 @@ __device__ inline bool ar3_collect(const PcgArgs &a, unsigned tag, int slot, dou
 // bits, so the owner's p and its readers' copies are identical.
 __global__ __launch_bounds__(kPcgThreads) void pcg_kernel(PcgArgs a) {
   extern __shared__ float lds[];
-  if (threadIdx.x == 0) __hip_atomic_fetch_add(&a.ctl->started, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) {  // resident (pcg_started_wait_kernel)
+    const unsigned n = __hip_atomic_fetch_add(&a.ctl->started, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+    if (a.ticket && n == (unsigned)a.G)
+      __hip_atomic_store(&a.ctl->ticket, a.ticket, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   __shared__ double red[kRedSize];
   __shared__ int s_bad;
   __shared__ unsigned s_base;
@@ -1583,7 +1592,11 @@ template <int SUB, int FORM>
 __device__ void pcg_pipe_body(const PcgArgs &a, int g, float *lds, double *red, int &s_bad, unsigned &s_base) {
   constexpr int ROWS = kPcgThreads / SUB;
   // this workgroup is resident (pcg_started_wait_kernel)
-  if (threadIdx.x == 0) __hip_atomic_fetch_add(&a.ctl->started, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) {
+    const unsigned n = __hip_atomic_fetch_add(&a.ctl->started, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+    if (a.ticket && n == (unsigned)a.G)
+      __hip_atomic_store(&a.ctl->ticket, a.ticket, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   if ((a.stop && *a.stop) || pcg_skip(a)) return;
   const int t = threadIdx.x;
   const int n0 = a.rows[g], n1 = a.rows[g + 1];
@@ -2148,13 +2161,14 @@ __device__ void pcg_open(PcgCtl *ctl) {
 }
 // Work on another stream that should not take the device before a PCG
 // launch's workgroups are resident waits behind this one-wave kernel: it
-// polls the launch's start count (bounded: after about 20 ms it lets the
-// stream go on regardless).
-__global__ void pcg_started_wait_kernel(const PcgCtl *ctl, unsigned G) {
+// polls until the launch that carries `ticket` has all its workgroups
+// started (the stream work before that launch has then ended), bounded:
+// after about 20 ms it lets the stream go on regardless.
+__global__ void pcg_started_wait_kernel(const PcgCtl *ctl, unsigned ticket) {
   if (threadIdx.x != 0) return;
-  unsigned *c = const_cast<unsigned *>(&ctl->started);
+  unsigned *c = const_cast<unsigned *>(&ctl->ticket);
   for (unsigned spins = 0; spins < (1u << 14); ++spins) {
-    if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= G) return;
+    if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == ticket) return;
     __builtin_amdgcn_s_sleep(8);
   }
 }
@@ -2960,17 +2974,19 @@ struct nrf_solver {
   solver_hook_fn hook = nullptr;   // see solver_set_pcg_hook
   void *hook_ctx = nullptr;
   hipEvent_t hook_ev = nullptr;
+  unsigned ticket = 0;  // carried by the next PCG launch (solver_set_pcg_ticket)
   nrf_dualquat *snap_T = nullptr, *snap_C = nullptr;  // see solver_set_snapshot
 };
 
 namespace nrf {
 
 void solver_set_pcg_wait(nrf_solver *s, hipEvent_t ev) { s->pcg_wait = ev; }
-int solver_pcg_start_wait(nrf_solver *s, hipStream_t st) {
-  pcg_started_wait_kernel<<<1, 64, 0, st>>>(s->ctl, (unsigned)s->G);
+int solver_pcg_start_wait(nrf_solver *s, unsigned ticket, hipStream_t st) {
+  pcg_started_wait_kernel<<<1, 64, 0, st>>>(s->ctl, ticket);
   NRF_LAUNCH_CHECK();
   return NRF_OK;
 }
+void solver_set_pcg_ticket(nrf_solver *s, unsigned ticket) { s->ticket = ticket; }
 void solver_set_pcg_hook(nrf_solver *s, solver_hook_fn fn, void *ctx, hipEvent_t ev) {
   s->hook = fn;
   s->hook_ctx = ctx;
@@ -3334,6 +3350,8 @@ static int pcg_launch(nrf_solver *s, const float *rhs, float *x, int N, int maxi
   PcgArgs a = solver_pcg_args(s, rhs, x, N, maxit);
   a.round = round;
   a.refine = refine ? 1 : 0;
+  a.ticket = s->ticket;
+  s->ticket = 0;
   void *args[] = {&a};
   const bool pipe = s->params.pcg_pipelined != 0;
   if (!pipe && s->pcg_sub != kPcgSub) return NRF_ERR_INVALID_ARGUMENT;  // textbook form: 4 lanes per row only

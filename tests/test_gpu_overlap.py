@@ -98,7 +98,8 @@ def test_overlapped_steps_equal_sequential(gpu, storage16):
     b.close()
 
 
-def test_overlapped_steps_with_one_depth_buffer(gpu):
+@pytest.mark.parametrize("storage16", [False, True], ids=["fp32", "precision"])
+def test_overlapped_steps_with_one_depth_buffer(gpu, storage16):
     """A caller that uploads every frame into one depth buffer, right after
     the previous step call returns and on the same stream, still gets each
     frame fused from its own depth (the deferred fusion reads the step's
@@ -110,6 +111,8 @@ def test_overlapped_steps_with_one_depth_buffer(gpu):
     from cudafusion_amd import _lib as L
     from cudafusion_amd.harness import Harness
     kw = dict(frames=6, arap_consistent_sign=True, validate=False, pcg_pipelined=3)
+    if storage16:
+        kw.update(jtj_precision=3, storage16=True, precond_fp64=True, refine_jacobi=True)
     a = Harness("body", **kw)
     a.setup()
     b = Harness("body", **kw)
