@@ -80,9 +80,10 @@ struct nrf_pipeline {
   //    F's stream waits for it before the step returns (the caller may then
   //    overwrite its depth buffer, and every later write of a slot on the
   //    step's stream follows the fusion that read it);
-  //  - a fusion that no step takes (a join, a drain) is flushed after ev[1]
-  //    (the end of its step) with its snapshot copied on `side`; the next
-  //    step then waits for it (fz_flushed) before its snapshot.
+  //  - a fusion that no step takes (a join, a drain) is flushed after an
+  //    event recorded then on its step's stream, with its snapshot copied on
+  //    `side`; the next step then waits for it (fz_flushed) before its
+  //    snapshot.
   // ev_fs / ev_fused mark the last enqueued fusion's start and end.
   bool overlap = false, side_pending = false, fz_pending = false, fz_events = false, fz_flushed = false;
   bool fz_copied = false;  // the step being enqueued made its hook's copy
@@ -94,6 +95,7 @@ struct nrf_pipeline {
   long fz_steps = 0;                  // overlapped steps so far (slot parity)
   int fz_slot = 0, fz_snap = 0;       // the deferred fusion's depth and snapshot slots
   unsigned fz_ticket = 0;             // the ticket of the step's first PCG launch
+  hipStream_t fz_stream = nullptr;    // the deferred step's stream (pipeline_flush)
   float fz_pose[16] = {};
   bool graph_step = false;    // the last step was a graph replay (stage events: whole step only)
   bool mesh_valid = false;    // verts / tris / counts hold the marching cubes of the current volume
@@ -187,9 +189,11 @@ static int pipeline_fuse_hook(void *ctx, hipStream_t st) {
 // A deferred fusion that no step takes: after `after` (the end of its
 // step), the snapshot into the slot the next step's first kernel would
 // write, then the fusion.
-static int pipeline_flush(nrf_pipeline *p, hipEvent_t after) {
+static int pipeline_flush(nrf_pipeline *p) {
   if (!p->fz_pending) return NRF_OK;
-  NRF_HIP_CHECK(hipStreamWaitEvent(p->side, after, 0));
+  // the end of the deferred step's work: an event recorded on its stream now
+  NRF_HIP_CHECK(hipEventRecord(p->ev[1], p->fz_stream));
+  NRF_HIP_CHECK(hipStreamWaitEvent(p->side, p->ev[1], 0));
   p->side_pending = true;
   const size_t nn = (size_t)std::max(1, p->cfg.max_nodes), bytes = sizeof(nrf_dualquat) * (size_t)p->N;
   p->fz_snap = (int)(p->fz_steps & 1);
@@ -204,7 +208,7 @@ static int pipeline_flush(nrf_pipeline *p, hipEvent_t after) {
 // fusion and waits for it (host-synchronous; the step's callers that want to
 // stay stream-ordered use nrf_pipeline_join).
 static int pipeline_drain(nrf_pipeline *p) {
-  int rc = pipeline_flush(p, p->ev[1]);
+  int rc = pipeline_flush(p);
   if (p->side_pending) {
     NRF_HIP_CHECK(hipStreamSynchronize(p->side));
     p->side_pending = false;
@@ -429,12 +433,16 @@ static int pipeline_enqueue(nrf_pipeline *p, const float *depth, const float cam
   solver_set_pcg_ticket(p->solver, 0);
   solver_set_snapshot(p->solver, nullptr, nullptr);
   if (rc) return rc;
-  if (events) NRF_HIP_CHECK(hipEventRecord(p->ev[1], st));
+  if (events && !ovl) NRF_HIP_CHECK(hipEventRecord(p->ev[1], st));
   if (ovl) {
     // a solve without a PCG launch (which updates no transform, so its
     // snapshot holds what the deferred fusion needs) leaves the hook's part
     // to be done here
-    if (!p->fz_copied && (rc = pipeline_side_step(p, p->ev[1]))) return rc;
+    if (!p->fz_copied) {
+      NRF_HIP_CHECK(hipEventRecord(p->ev[1], st));
+      if ((rc = pipeline_side_step(p, p->ev[1]))) return rc;
+    }
+    p->fz_stream = st;  // a flush orders the fusion after this step's work there
     // this step's fusion is deferred; the caller's stream waits for the copy
     // of its depth
     NRF_HIP_CHECK(hipStreamWaitEvent(st, p->ev_copy, 0));
@@ -548,7 +556,7 @@ extern "C" int nrf_pipeline_set_overlap(nrf_pipeline *p, int enable) {
 
 extern "C" int nrf_pipeline_join(nrf_pipeline *p, void *stream) {
   if (!p) return NRF_ERR_INVALID_ARGUMENT;
-  if (int rc = pipeline_flush(p, p->ev[1])) return rc;
+  if (int rc = pipeline_flush(p)) return rc;
   if (p->side_pending) NRF_HIP_CHECK(hipStreamWaitEvent(as_stream(stream), p->ev_fused, 0));
   return NRF_OK;
 }

@@ -627,8 +627,10 @@ long long nrf_pipeline_graph_captures(nrf_pipeline *p);
    nrf_pipeline_join on the reader's stream or any other pipeline call,
    including nrf_pipeline_voxels / _mesh / _volume16 (which enqueue a
    deferred fusion and wait for it); a device synchronisation alone does not
-   run a deferred fusion.  Graph-replayed and per-kernel profiled steps do
-   not overlap.  Synchronises the host when switching. */
+   run a deferred fusion, and the step's stream must outlive it (a join or
+   another pipeline call enqueues it behind that stream's work).
+   Graph-replayed and per-kernel profiled steps do not overlap.
+   Synchronises the host when switching. */
 int nrf_pipeline_set_overlap(nrf_pipeline *p, int enable);
 /* Enqueues a deferred fusion of the last overlapped step and makes `stream`
    wait for it (a no-op otherwise). */
