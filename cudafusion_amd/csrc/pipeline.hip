@@ -158,9 +158,10 @@ static int pipeline_fuse(nrf_pipeline *p) {
   return NRF_OK;
 }
 
-// The side stream's part at step F's first PCG launch, after `after` (an
-// event on F's stream): the deferred fusion of step F-1 (its snapshot was
-// taken by F's first kernel), then F's depth into its slot.
+// The side stream's part of step F: behind F's first PCG launch (`ticket`)
+// or, for a solve without one, after `after` (an event on F's stream): the
+// deferred fusion of step F-1 (its snapshot was taken by F's first kernel),
+// then F's depth into its slot.
 static int pipeline_side_step(nrf_pipeline *p, hipEvent_t after, unsigned ticket = 0) {
   if (after) NRF_HIP_CHECK(hipStreamWaitEvent(p->side, after, 0));
   p->side_pending = true;
@@ -186,8 +187,8 @@ static int pipeline_fuse_hook(void *ctx, hipStream_t st) {
   return pipeline_side_step(p, nullptr, p->fz_ticket);
 }
 
-// A deferred fusion that no step takes: after `after` (the end of its
-// step), the snapshot into the slot the next step's first kernel would
+// A deferred fusion that no step takes: after the deferred step's work on
+// its stream, the snapshot into the slot the next step's first kernel would
 // write, then the fusion.
 static int pipeline_flush(nrf_pipeline *p) {
   if (!p->fz_pending) return NRF_OK;
