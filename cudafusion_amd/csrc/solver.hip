@@ -118,14 +118,18 @@ struct SnapArgs {
   DQ *dst_T = nullptr, *dst_C = nullptr;
   int N = 0;
 };
+// vmask (optional): 1 per valid slot, 0 per other slot (the record
+// launches of the frame's GN iterations read it instead of the slots).
 __global__ __launch_bounds__(256) void mem_count_kernel(const nrf_corr *__restrict__ corrs, int n, int N,
-                                                        int32_t *__restrict__ cnt, SnapArgs sn) {
+                                                        int32_t *__restrict__ cnt, SnapArgs sn,
+                                                        uint8_t *__restrict__ vmask) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (sn.dst_T && c < sn.N) {
     sn.dst_T[c] = sn.T[c];
     sn.dst_C[c] = sn.C[c];
   }
   mem_count_body(corrs, n, N, cnt, c);
+  if (vmask && c < n) vmask[c] = corrs[c].valid ? 1 : 0;
 }
 
 // The memberships' scan of one frame: off / cur from the per-node counts,
@@ -296,6 +300,9 @@ struct AsmArgs {
   // when set, the assembly's last launch opens the next PCG launch (advances
   // its epoch and clears its abort flag), which then needs no launch of its own
   PcgCtl *pcg_open;
+  // per correspondence slot, 1 if valid (made by the frame's membership
+  // count): the record launch reads a slot's body only when it is set
+  const uint8_t *vmask = nullptr;
 };
 __device__ void pcg_open(PcgCtl *ctl);
 
@@ -629,6 +636,7 @@ __global__ __launch_bounds__(64) void assemble_rows_kernel(AsmArgs a) {
 constexpr int kRec = 32;
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 __device__ void corr_prep_one(const AsmArgs &a, int c, float *__restrict__ rec) {
+  if (a.vmask && !a.vmask[c]) return;
   const nrf_corr &cr = a.corrs[c];
   if (!cr.valid) return;
   const float3 dst = f3(cr.dst), nrm = f3(cr.normal);
@@ -2976,6 +2984,8 @@ struct nrf_solver {
   hipEvent_t hook_ev = nullptr;
   unsigned ticket = 0;  // carried by the next PCG launch (solver_set_pcg_ticket)
   nrf_dualquat *snap_T = nullptr, *snap_C = nullptr;  // see solver_set_snapshot
+  uint8_t *vmask = nullptr;  // valid slots of the last membership build (AsmArgs::vmask)
+  size_t c_vmask = 0;
 };
 
 namespace nrf {
@@ -3136,8 +3146,10 @@ static int solver_build_memberships(nrf_solver *s, const nrf_corr *corrs, int n,
   }
   s->m_cnt_zero = true;
   s->m_cnt_n = N;
+  if ((rc = dev_alloc(s->vmask, (size_t)std::max(n, 1), s->c_vmask))) return rc;
   if (n > 0 || sn.dst_T) {
-    mem_count_kernel<<<(std::max(n, sn.dst_T ? sn.N : 0) + 255) / 256, 256, 0, st>>>(corrs, n, N, s->m_cnt, sn);
+    mem_count_kernel<<<(std::max(n, sn.dst_T ? sn.N : 0) + 255) / 256, 256, 0, st>>>(corrs, n, N, s->m_cnt, sn,
+                                                                                    s->vmask);
     NRF_LAUNCH_CHECK();
   }
   // offsets, cursors and the MFMA assembly's chunk offsets in one pass
@@ -3225,6 +3237,7 @@ static int solver_assemble(nrf_solver *s, const nrf_corr *corrs, int n, const fl
       a.arec = s->arec;
       a.n_es = n_es;
     }
+    a.vmask = s->vmask;  // made by this solve's membership build
     corr_prep_kernel<<<(n + a.n_es + 255) / 256, 256, 0, st>>>(a, n, s->crec);
     NRF_LAUNCH_CHECK();
   }
@@ -3639,7 +3652,7 @@ extern "C" void nrf_solver_destroy(nrf_solver *s) {
                   s->halo_off, s->halo,
                   s->m_cnt, s->m_off, s->m_cur, s->m_keys, s->m_keys2, s->stop, s->d_res, s->d_tot,
                   s->crec, s->arec, s->rres, s->dref, s->rpart, s->rnorm, s->chunk_off, s->chunk_row, s->part,
-                  s->tr, s->pl_r, s->pl_z, s->pl_p, s->pl_Ap, s->pl_part, s->pl_out};
+                  s->tr, s->pl_r, s->pl_z, s->pl_p, s->pl_Ap, s->pl_part, s->pl_out, s->vmask};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (s->pl_host) (void)hipHostFree(s->pl_host);
@@ -3988,6 +4001,7 @@ int solver_reserve(nrf_solver *s, int max_corrs) {
   if ((rc = dev_alloc(s->m_cur, N, s->c_m_cur))) return rc;
   if ((rc = dev_alloc(s->m_keys, nc * 4, s->c_m_keys))) return rc;
   if ((rc = dev_alloc(s->m_keys2, nc * 4, s->c_m_keys2))) return rc;
+  if ((rc = dev_alloc(s->vmask, nc, s->c_vmask))) return rc;
   if ((rc = dev_alloc(s->rres, 6 * N, s->c_rres))) return rc;
   if ((rc = dev_alloc(s->chunk_off, N + 1, s->c_chunk_off))) return rc;
   const size_t nch = asm_max_chunks((int)N, (int)nc);
