@@ -184,11 +184,14 @@ __global__ __launch_bounds__(1024) void member_scan_kernel(int32_t *__restrict__
   member_scan_body(cnt, off, cur, chunk_off, n);
 }
 
+// vmask (optional): the count kernel's valid mask, read instead of the slots'
+// valid bytes so that invalid slots are not fetched.
 __device__ inline void mem_scatter_body(const nrf_corr *__restrict__ corrs, int n, int N,
-                                        int32_t *__restrict__ cur, int32_t *__restrict__ keys, int c) {
+                                        int32_t *__restrict__ cur, int32_t *__restrict__ keys, int c,
+                                        const uint8_t *__restrict__ vmask = nullptr) {
   __shared__ MemHash hs;
   mem_hash_clear(hs);
-  const bool ok = c < n && corrs[c].valid;
+  const bool ok = c < n && (vmask ? vmask[c] != 0 : corrs[c].valid != 0);
   int slot[4], rank[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -209,8 +212,9 @@ __device__ inline void mem_scatter_body(const nrf_corr *__restrict__ corrs, int 
     if (slot[k] >= 0) keys[hs.base[slot[k]] + rank[k]] = c * 4 + k;
 }
 __global__ __launch_bounds__(256) void mem_scatter_kernel(const nrf_corr *__restrict__ corrs, int n, int N,
-                                                          int32_t *__restrict__ cur, int32_t *__restrict__ keys) {
-  mem_scatter_body(corrs, n, N, cur, keys, blockIdx.x * blockDim.x + threadIdx.x);
+                                                          int32_t *__restrict__ cur, int32_t *__restrict__ keys,
+                                                          const uint8_t *__restrict__ vmask) {
+  mem_scatter_body(corrs, n, N, cur, keys, blockIdx.x * blockDim.x + threadIdx.x, vmask);
 }
 
 // Orders each node's bucket by key.  Buckets of up to kSortLds keys are
@@ -3157,7 +3161,7 @@ static int solver_build_memberships(nrf_solver *s, const nrf_corr *corrs, int n,
                                          s->d_res, s->d_tot);
   NRF_LAUNCH_CHECK();
   if (n > 0) {
-    mem_scatter_kernel<<<(n + 255) / 256, 256, 0, st>>>(corrs, n, N, s->m_cur, s->m_keys);
+    mem_scatter_kernel<<<(n + 255) / 256, 256, 0, st>>>(corrs, n, N, s->m_cur, s->m_keys, s->vmask);
     NRF_LAUNCH_CHECK();
   }
   // each row's bucket ordered by key; the row's chunk entries filled alongside
